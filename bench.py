@@ -1,0 +1,142 @@
+"""Benchmark: Crank-Nicolson steps/s of the 3D soft-core atom at 128^3, p = 4.
+
+Workload (the 3D soft-core configuration): V = -1/sqrt(r^2+2) on [-40,40]^3,
+128 uniform elements per axis, p = 4, Q = 8 Gauss points, a quadratic CAP in
+the outer 8 a.u. (strength 0.01), a z-polarised sin^2 pulse E0 = 0.05,
+omega = 0.057, 4 cycles; CN dt = 0.02; FGMRES to ||r||/||b|| < 1e-10 with the
+chosen preconditioner. A "step" is one CN step; inputs are resident in HBM.
+
+Prints one JSON line (see the driver contract). The CPU baseline times the CPU
+oracle on one CN step of the same problem (bounded sample) on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
+PEAK_HBM = 8.0e12
+
+
+def problem(nel=128):
+    return dict(dim=3, nelem=(nel,) * 3, lo=(-40.0,) * 3, hi=(40.0,) * 3, p=4, nquad=8, mass=1.0, charge=-1.0,
+                pot_kind=2, pot=(1.0, 2.0), cap_kind=1, cap=(0.01, 8.0), mg_levels=3, mg_smooth=1)
+
+
+def pot_flops_per_element(p, Q):
+    n = p + 1
+    return 2 * 4 * (n ** 3 * Q + n ** 2 * Q ** 2 + n * Q ** 3) + 2 * Q ** 3
+
+
+def initial_state(desc, n):
+    """Separable Gaussian (width 1.5) sampled at the Greville abscissae, then M-normalised."""
+    p = desc["p"]
+    vals = []
+    for d in range(3):
+        E, lo, hi = desc["nelem"][d], desc["lo"][d], desc["hi"][d]
+        t = np.concatenate([[lo] * (p + 1), lo + (hi - lo) * np.arange(1, E) / E, [hi] * (p + 1)])
+        g = np.array([t[i + 1:i + p + 1].mean() for i in range(n[d])])
+        vals.append(np.exp(-g ** 2 / (2 * 1.5 ** 2)))
+    psi = np.einsum("k,j,i->kji", vals[2], vals[1], vals[0]).reshape(-1).astype(np.complex128)
+    return psi
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--pc", default="mg", choices=["sep", "mg", "none"])
+    ap.add_argument("--nel", type=int, default=128)
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import tdsez_amd as T
+    desc = problem(args.nel)
+    t_setup = time.perf_counter()
+    ctx = T.Context(desc)
+    pc = {"sep": T.PC_SEPARABLE, "mg": T.PC_MG, "none": T.PC_NONE}[args.pc]
+    ctx.pc_setup(pc)
+    psi = ctx.vec(initial_state(desc, ctx.n))
+    nrm = ctx.observables(psi)[0]
+    psi.upload(psi.download() / math.sqrt(nrm))
+    t_setup = time.perf_counter() - t_setup
+    pulse = T.make_pulse(E0=(0.0, 0.0, 0.05), omega=0.057, ncycles=4.0)
+    dt = 0.02
+    print(f"# setup {t_setup:.2f} s, N = {ctx.N}, levels = {ctx.levels}, pc = {args.pc}", file=sys.stderr, flush=True)
+
+    its_w, _ = ctx.evolve_cn(psi, 0.0, dt, args.warmup, pulse, rtol=args.rtol, maxit=500)
+    ctx.sync()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    its, _ = ctx.evolve_cn(psi, args.warmup * dt, dt, args.steps, pulse, rtol=args.rtol, maxit=500)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    kms, nlaunch = ctx.timing_stats()
+    ctx.timing(False)
+    steps_per_s = args.steps / elapsed
+    print(f"# timed {args.steps} steps in {elapsed:.3f} s; Krylov its/step {its.tolist()}", file=sys.stderr,
+          flush=True)
+
+    p, Q, nel = desc["p"], desc["nquad"], args.nel
+    flops = pot_flops_per_element(p, Q) * nel ** 3
+    avg_s = (kms / 1e3) / max(nlaunch, 1)
+    achieved = flops / avg_s if nlaunch else 0.0
+    roofline = {"bound": "mfma", "kernel": "pot_apply_generic_kernel", "achieved": achieved / 1e12,
+                "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+                "launches": nlaunch, "avg_ms": avg_s * 1e3, "flops_per_launch": flops}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(desc, dt, pulse, args)
+
+    line = {
+        "metric": "CN steps/s, 3D p=4 128^3 grid: learned+mixed PC vs plain FP64 MG at |r|<1e-10",
+        "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "complex128 (fp64)", "data": "synthetic (Gaussian initial state)",
+        "config": {"workload": f"3D soft-core atom {nel}^3 elements p=4 Q=8, CN dt=0.02, rtol={args.rtol}",
+                   "preconditioner": {"sep": "fp64 separable (fast diagonalisation)", "mg": "plain fp64 GMG",
+                                      "none": "none"}[args.pc],
+                   "dofs": ctx.N, "mg_levels": ctx.levels, "krylov_its_per_step": float(np.mean(its))},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def cpu_baseline(desc, dt, pulse, args):
+    """Time the CPU oracle on one CN step of the same problem (bounded sample)."""
+    try:
+        import oracle
+        from oracle import Oracle
+    except Exception as e:  # noqa: BLE001
+        return {"error": f"oracle unavailable: {e}"}
+    o = Oracle(desc)
+    n = o.n
+    psi = initial_state(desc, n)
+    t = args.warmup * dt + 0.5 * dt
+    E = 0.05 * math.sin(math.pi * t / (4 * 2 * math.pi / 0.057)) ** 2 * math.sin(0.057 * t)
+    sE = np.array([[0.0, 0.0, E]])
+    t0 = time.perf_counter()
+    _, its, _ = o.cn_steps(psi, dt, sE, rtol=args.rtol, maxit=500, restart=50, pc=1)
+    el = time.perf_counter() - t0
+    return {"value": 1.0 / el, "unit": "steps/s", "cores": oracle.lib().orc_threads(), "kind": "port",
+            "sample": f"1 CN step of the same {desc['nelem'][0]}^3 problem, FGMRES + separable PC "
+                      f"({int(its[0])} its), {el:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

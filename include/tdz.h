@@ -1,0 +1,135 @@
+/*
+ * tdz.h — C ABI of the MI355X-native TDSE-Z propagation hot path.
+ *
+ * The reference solves every Crank-Nicolson step through PETSc: TSSetIJacobian /
+ * TSSetIFunction callbacks build J = i a M - H - s E(t) D (src/propagator.cpp:103-105,
+ * src/monitor.cpp:150-162, :367-382), KSPSolve runs (F)GMRES with a right
+ * preconditioner (src/propagator.cpp:131-197) and PCMG applies the geometric
+ * B-spline multigrid (src/multigrid.cpp:205-327). With PETSc present, the
+ * plug-in points are a MATSHELL MATOP_MULT and a PCSHELL apply; the entry points
+ * below have exactly those shapes, on device-resident interleaved complex128
+ * vectors (re, im pairs, x fastest: index i + n0 (j + n1 k), as
+ * src/tensorops.cpp:67-70).
+ *
+ * Conventions (mirroring PetscErrorCode): every function returns 0 on success and
+ * a nonzero code on failure; tdz_last_error() describes the last failure of the
+ * calling thread. A solve that does not converge is an error
+ * (src/propagator.cpp:671-672). The creator destroys: the context owns all
+ * device memory it allocates, the caller owns host buffers. One host thread and
+ * one HIP stream per context.
+ */
+#ifndef TDZ_H
+#define TDZ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tdz_ctx tdz_ctx;
+
+/* Potential kinds: s V(r) enters H (s = -charge, include/tdsez_parser.hpp:85-105). */
+enum { TDZ_POT_NONE = 0, TDZ_POT_HARMONIC = 1 /* 0.5 pot[0] r^2 */, TDZ_POT_SOFTCORE = 2 /* -pot[0]/sqrt(r^2+pot[1]) */ };
+/* CAP kinds: CAP = -i sum_d W_d(x_d) (src/assembly/cap.cpp:1-21). */
+enum { TDZ_CAP_NONE = 0, TDZ_CAP_QUADRATIC = 1 /* cap[0] (cap[1]-s)^2 within cap[1] of a wall */,
+       TDZ_CAP_MANOLOPOULOS = 2 /* kmin = cap[0] (src/assembly/assembly.cpp:170-208) */ };
+/* Preconditioners. */
+enum { TDZ_PC_NONE = 0, TDZ_PC_SEPARABLE = 1 /* fast-diagonalisation inverse of alpha M + g (K + V_sep) */,
+       TDZ_PC_MG = 2 /* geometric B-spline multigrid V-cycle */ };
+/* Single operator terms for tdz_term_apply. */
+enum { TDZ_TERM_M = 0, TDZ_TERM_K = 1, TDZ_TERM_V = 2, TDZ_TERM_CAP = 3, TDZ_TERM_DX = 4, TDZ_TERM_DY = 5,
+       TDZ_TERM_DZ = 6 };
+
+/* Grid and physics of one run: the subset of the reference input deck that the
+ * propagation path reads (include/tdsez_parser.hpp:270-542). */
+typedef struct {
+    int dim;             /* 1..3 */
+    int nelem[3];        /* uniform elements per axis (reference: Nelements, one value) */
+    double lo[3], hi[3]; /* domain per axis */
+    int p;               /* SplineDegree, continuity p-1 (src/core.cpp:1899) */
+    int nquad;           /* NQuadratures, Gauss-Legendre (default 8) */
+    int dirichlet;       /* 1: wall DOFs at both ends of every active axis */
+    double mass;         /* constant Mass */
+    double charge;       /* Charge (-1 electron) */
+    int pot_kind;
+    double pot[4];
+    int cap_kind;
+    double cap[4];
+    int mg_levels;       /* MGLevels: 0 automatic, else the number of levels */
+    int mg_smooth;       /* MGSmooth: smoothing sweeps per level (default 1) */
+    int restart;         /* Krylov restart; 0 = reference rule (30, 50 above 1e5 rows) */
+} tdz_problem;
+
+typedef struct {
+    int its;          /* Krylov iterations of the last solve */
+    int restarts;     /* restart cycles */
+    double res;       /* final relative residual ||b - A x|| / ||b|| (estimate of right-preconditioned GMRES) */
+    double bnorm;     /* ||b|| */
+    int pc_rebuilds;  /* preconditioner refreshes (TDSEZPCRefresh) */
+} tdz_ksp_stats;
+
+/* sin^2-envelope pulse E_a(t) = E0[a] sin^2(pi t / T) sin(omega t + cep), 0 <= t <= T,
+ * T = ncycles 2 pi / omega; zero outside. */
+typedef struct {
+    double E0[3];
+    double omega;
+    double ncycles;
+    double cep;
+} tdz_pulse;
+
+/* Lifetime. Replaces the IGA/assembly set-up that feeds the propagator
+ * (src/core.cpp:1786-2081, src/tensorops.cpp:16-81, src/assembly/cap.cpp:40-144). */
+int tdz_create(const tdz_problem *prob, tdz_ctx **ctx);
+int tdz_destroy(tdz_ctx *ctx);
+const char *tdz_last_error(void);
+long tdz_size(const tdz_ctx *ctx);
+int tdz_dims(const tdz_ctx *ctx, int n[3]);
+int tdz_levels(const tdz_ctx *ctx);
+
+/* Device vectors of N complex128 owned by the context (VecCreate/VecDestroy). */
+int tdz_vec_alloc(tdz_ctx *ctx, double **dvec);
+int tdz_vec_free(tdz_ctx *ctx, double *dvec);
+int tdz_vec_upload(tdz_ctx *ctx, double *dvec, const double *host);
+int tdz_vec_download(tdz_ctx *ctx, const double *dvec, double *host);
+int tdz_sync(tdz_ctx *ctx);
+
+/* A = alpha M + g (H + sum_a sE_a D_a), H = K + s V + CAP: what the IJacobian
+ * callback would record into a shell context (src/monitor.cpp:367-382 with
+ * alpha = i a, g = -1 there; CN uses alpha = 1, g = i dt/2). */
+int tdz_set_shift(tdz_ctx *ctx, double alpha_re, double alpha_im, double g_re, double g_im, const double sE[3]);
+/* MATSHELL MATOP_MULT: y = A x (replaces MatMult(J, x, y) inside KSPSolve). */
+int tdz_op_apply(tdz_ctx *ctx, const double *x, double *y);
+/* y = T x for one term of H, M or D (TDSEZTensorOps::Apply, src/tensorops.cpp:155-188). */
+int tdz_term_apply(tdz_ctx *ctx, int term, const double *x, double *y);
+/* PCSHELL set-up and apply (replaces PCSetUp/PCApply of PCMG, src/multigrid.cpp:205-327). */
+int tdz_pc_setup(tdz_ctx *ctx, int kind);
+int tdz_pc_apply(tdz_ctx *ctx, const double *r, double *z);
+/* KSPSolve: FGMRES, right preconditioning, x holds the initial guess
+ * (src/propagator.cpp:131-182). Not converging within maxit is an error. */
+int tdz_ksp_solve(tdz_ctx *ctx, const double *b, double *x, double rtol, int maxit, tdz_ksp_stats *stats);
+/* One Crank-Nicolson step psi <- (M + i dt/2 H_E)^-1 (M - i dt/2 H_E) psi with
+ * H_E at the midpoint field sE_mid (TSTHETA theta = 1/2 in src/propagator.cpp:94-101). */
+int tdz_step_cn(tdz_ctx *ctx, double *psi, double dt, const double sE_mid[3], double rtol, int maxit,
+                tdz_ksp_stats *stats);
+/* nsteps CN steps from t0 under a pulse (TSSolve of TDSEZPropagator::Evolve,
+ * src/propagator.cpp:78-251), with the preconditioner refresh rule of
+ * TDSEZPCRefresh (include/tdsez_propagator.hpp:17-31). obs (optional, host,
+ * 7 doubles per step) receives [t, norm, Re<H>, Im<H>, <x>, <y>, <z>] after each
+ * step; its (optional, host) the Krylov iterations per step. */
+int tdz_evolve_cn(tdz_ctx *ctx, double *psi, double t0, double dt, int nsteps, const tdz_pulse *pulse,
+                  double rtol, int maxit, double *obs, int *its);
+/* Observables [norm psi^H M psi, Re psi^H H psi, Im psi^H H psi, <x>, <y>, <z>]
+ * (MonitorProducts, src/manager.cpp:584-672; src/monitor.cpp:1145-1194). */
+int tdz_observables(tdz_ctx *ctx, const double *psi, double out[6]);
+/* Field sE_a(t) = s E_a(t) of a pulse (TDSEZParser::Coupling, include/tdsez_parser.hpp:190-194). */
+int tdz_pulse_coupling(const tdz_ctx *ctx, const tdz_pulse *pulse, double t, double sE[3]);
+
+/* Measurement hooks (no reference counterpart; used by bench.py): HIP events
+ * recorded on the context's stream around every launch of the fine-level
+ * potential kernel while enabled; stats returns their summed duration and count. */
+int tdz_timing(tdz_ctx *ctx, int enable);
+int tdz_timing_stats(tdz_ctx *ctx, double *total_ms, long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDZ_H */

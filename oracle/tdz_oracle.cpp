@@ -1,0 +1,1091 @@
+// =============================================================================
+//  CPU oracle of the TDSE-Z time-propagation hot path — TEST INFRASTRUCTURE.
+//
+//  This file is the checker, never the product. Only the tests, the smoke
+//  check of __graft_entry__.py and the cpu_baseline leg of bench.py load it.
+//  The product path (tdsez_amd/) does not link, load or call anything here.
+//
+//  It restates, in plain C++ loops, the reference algorithm of dahbiz/tdsez:
+//    * 1D open uniform knots, continuity p-1        (src/core.cpp:1899)
+//    * Gauss-Legendre rule per element              (src/core.cpp:1793)
+//    * 1D Gram bands M, X, G, A=G-G^T, S            (src/tensorops.cpp:27-61)
+//    * separable CAP bands W_d                      (src/assembly/cap.cpp:40-88,
+//                                                    src/assembly/assembly.cpp:170-208)
+//    * Kronecker three-pass term apply              (src/tensorops.cpp:97-188)
+//    * per-element quadrature evaluation of psi and its transpose for the
+//      potential, cV = s V w                        (src/quadobs.cpp:181-238,
+//                                                    src/assembly/sumfact.cpp:490-502)
+//    * H = K + sV + CAP, coupling s E_a D_a         (include/tdsez_parser.hpp:92-105,190-194)
+//    * right-preconditioned FGMRES, restart, rtol   (src/propagator.cpp:131-197)
+//    * Crank-Nicolson step (M + i dt/2 H_E) y = (M - i dt/2 H_E) x
+//                                                   (src/monitor.cpp:150-162,367-382)
+//    * Pade(2,2) stage pair                         (src/propagator.cpp:536-676)
+//    * knot coarsening and Boehm prolongation       (src/multigrid.cpp:25-93)
+//    * a brute-force dense assembler per quadrature point (tiny grids), the
+//      analogue of TDSEZFormHam                     (src/assembly/assembly.cpp:703-864)
+//
+//  Parity status of each function is recorded by the tests that pin it
+//  (tests/test_oracle_*.py).
+// =============================================================================
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using cplx = std::complex<double>;
+using Knots = std::vector<double>;
+
+extern "C" {
+typedef struct {
+    int dim;             // 1..3 active axes (x, y, z)
+    int nelem[3];        // elements per axis
+    double lo[3], hi[3]; // domain per axis
+    int p;               // spline degree
+    int nquad;           // Gauss points per element and axis
+    int dirichlet;       // 1: first/last function of every active axis is a wall DOF
+    double mass;         // constant carrier mass
+    double charge;       // carrier charge q; the carrier sign is s = -q
+    int pot_kind;        // 0 none, 1 harmonic 0.5 k r^2, 2 soft-core -Z/sqrt(r^2+a)
+    double pot[4];
+    int cap_kind;        // 0 none, 1 quadratic eta*(w-s)^2, 2 Manolopoulos(kmin)
+    double cap[4];
+} orc_params;
+}
+
+namespace {
+
+// ---------------------------------------------------------------- quadrature
+// Gauss-Legendre on [-1, 1] by Newton iteration on P_Q (three-term recurrence).
+void gauss_legendre(int Q, double *x, double *w)
+{
+    for (int i = 0; i < Q; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (Q + 0.5)), dp = 1;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1, p1 = 0;
+            for (int k = 1; k <= Q; ++k) {
+                const double p2 = p1;
+                p1 = p0;
+                p0 = ((2.0 * k - 1) * z * p1 - (k - 1.0) * p2) / k;
+            }
+            dp = Q * (z * p0 - p1) / (z * z - 1);
+            const double dz = p0 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        double p0 = 1, p1 = 0;
+        for (int k = 1; k <= Q; ++k) {
+            const double p2 = p1;
+            p1 = p0;
+            p0 = ((2.0 * k - 1) * z * p1 - (k - 1.0) * p2) / k;
+        }
+        dp = Q * (z * p0 - p1) / (z * z - 1);
+        x[Q - 1 - i] = z;                       // ascending order
+        w[Q - 1 - i] = 2.0 / ((1 - z * z) * dp * dp);
+    }
+}
+
+// ----------------------------------------------------------------- B-splines
+// Cox-de Boor recursion, directly from the definition (0/0 := 0).
+double bspline(const Knots &t, int i, int p, double x, int span)
+{
+    if (p == 0) return i == span ? 1.0 : 0.0;
+    double v = 0;
+    const double d1 = t[i + p] - t[i], d2 = t[i + p + 1] - t[i + 1];
+    if (d1 > 0) v += (x - t[i]) / d1 * bspline(t, i, p - 1, x, span);
+    if (d2 > 0) v += (t[i + p + 1] - x) / d2 * bspline(t, i + 1, p - 1, x, span);
+    return v;
+}
+double bspline_d1(const Knots &t, int i, int p, double x, int span)
+{
+    if (p == 0) return 0.0;
+    double v = 0;
+    const double d1 = t[i + p] - t[i], d2 = t[i + p + 1] - t[i + 1];
+    if (d1 > 0) v += p / d1 * bspline(t, i, p - 1, x, span);
+    if (d2 > 0) v -= p / d2 * bspline(t, i + 1, p - 1, x, span);
+    return v;
+}
+
+// Manolopoulos profile, as Manolopoulos_CAP_profile (src/assembly/assembly.cpp:170-208).
+double manolopoulos(double xq, double xmax, double kmin)
+{
+    const double c = 2.62206, c3 = c * c * c;
+    const double a = 1.0 - 16.0 / c3, b = (1.0 - 17.0 / c3) / (c * c), delta = 0.2;
+    const double Emin = 0.5 * kmin * kmin;
+    const double width = c / (2.0 * delta * kmin);
+    const double rcap = xmax - width;
+    const double ax = std::fabs(xq);
+    if (ax <= rcap) return 0.0;
+    double r = c * ((ax - rcap) / (xmax - rcap));
+    if (ax >= xmax) r = c * (1.0 - 2.220446049250313e-16);
+    return Emin * (a * r - b * r * r * r + 4.0 / ((c - r) * (c - r)) - 4.0 / ((c + r) * (c + r)));
+}
+
+struct Axis {
+    int E = 1, p = 0, Q = 1, n = 1;
+    double lo = 0, hi = 0;
+    bool active = false, wall = false;
+    Knots t;
+    std::vector<double> X, W;   // [E*Q] points, weights (weight * detJac)
+    std::vector<double> N0, N1; // [(e*(p+1)+a)*Q+q]
+    std::vector<double> M, Xb, G, A, S, Wc, Vs;  // bands n x (2p+1); Vs: separable part of s V
+    std::vector<double> capq;  // CAP profile W_d at the points [E*Q]
+    std::vector<char> isWall;  // [n]
+};
+
+struct Problem {
+    orc_params prm;
+    int dim;
+    Axis ax[3];
+    size_t N = 1;
+    double s = 1;  // carrier sign
+
+    double pot(double x, double y, double z) const
+    {
+        switch (prm.pot_kind) {
+        case 1: return 0.5 * prm.pot[0] * (x * x + y * y + z * z);
+        case 2: return -prm.pot[0] / std::sqrt(x * x + y * y + z * z + prm.pot[1]);
+        default: return 0.0;
+        }
+    }
+    double cap_profile(const Axis &A, double u) const
+    {
+        if (prm.cap_kind == 0) return 0.0;
+        const double sdist = std::min(u - A.lo, A.hi - u);
+        if (prm.cap_kind == 1) {
+            const double width = prm.cap[1];
+            if (!(sdist < width)) return 0.0;
+            const double d = width - sdist;
+            return prm.cap[0] * d * d;
+        }
+        // Manolopoulos layer of width c/(2 delta kmin), scaled by 1/m
+        // (src/assembly/cap.cpp:31-36, src/assembly/assembly.cpp:1892-1907)
+        const double kmin = prm.cap[0];
+        const double width = 2.62206 / (2.0 * 0.2 * kmin);
+        if (!(sdist < width)) return 0.0;
+        return manolopoulos(std::min(width - sdist, width * (1.0 - 1e-10)), width, kmin) / prm.mass;
+    }
+};
+
+void setup_axis(Problem &P, int d)
+{
+    Axis &A = P.ax[d];
+    const orc_params &r = P.prm;
+    if (d >= P.dim) {  // degenerate axis: one constant function, one point of weight 1
+        A.E = 1; A.p = 0; A.Q = 1; A.n = 1; A.active = false;
+        A.X = {0.0}; A.W = {1.0}; A.N0 = {1.0}; A.N1 = {0.0};
+        A.M = {1.0}; A.Xb = {0.0}; A.G = {0.0}; A.A = {0.0}; A.S = {0.0}; A.Wc = {0.0}; A.Vs = {0.0};
+        A.capq = {0.0}; A.isWall = {0};
+        return;
+    }
+    A.active = true;
+    A.E = r.nelem[d]; A.p = r.p; A.Q = r.nquad; A.lo = r.lo[d]; A.hi = r.hi[d];
+    A.n = A.E + A.p;
+    A.wall = r.dirichlet != 0;
+    const int p = A.p, E = A.E, Q = A.Q, nen = p + 1, bw = 2 * p + 1;
+    // open uniform knots, continuity p-1 (IGAAxisInitUniform(..., p-1))
+    A.t.assign(E + 2 * p + 1, 0.0);
+    for (int i = 0; i <= p; ++i) { A.t[i] = A.lo; A.t[E + p + i] = A.hi; }
+    for (int i = 1; i < E; ++i) A.t[p + i] = A.lo + (A.hi - A.lo) * i / E;
+    std::vector<double> gx(Q), gw(Q);
+    gauss_legendre(Q, gx.data(), gw.data());
+    A.X.resize((size_t)E * Q); A.W.resize((size_t)E * Q);
+    A.N0.assign((size_t)E * nen * Q, 0.0); A.N1 = A.N0;
+    A.capq.assign((size_t)E * Q, 0.0);
+    for (int e = 0; e < E; ++e) {
+        const double u0 = A.t[p + e], u1 = A.t[p + e + 1], J = 0.5 * (u1 - u0);
+        for (int q = 0; q < Q; ++q) {
+            const double x = 0.5 * (u0 + u1) + J * gx[q];
+            A.X[e * Q + q] = x;
+            A.W[e * Q + q] = gw[q] * J;
+            A.capq[e * Q + q] = P.cap_profile(A, x);
+            for (int a = 0; a < nen; ++a) {
+                A.N0[((size_t)e * nen + a) * Q + q] = bspline(A.t, e + a, p, x, p + e);
+                A.N1[((size_t)e * nen + a) * Q + q] = bspline_d1(A.t, e + a, p, x, p + e);
+            }
+        }
+    }
+    A.isWall.assign(A.n, 0);
+    if (A.wall) { A.isWall[0] = 1; A.isWall[A.n - 1] = 1; }
+    // Gram bands (src/tensorops.cpp:40-57), entry (i, j) at i*bw + j-i+p
+    const size_t sz = (size_t)A.n * bw;
+    A.M.assign(sz, 0); A.Xb.assign(sz, 0); A.G.assign(sz, 0); A.S.assign(sz, 0); A.Wc.assign(sz, 0);
+    A.Vs.assign(sz, 0);
+    // separable part of s V per axis: the harmonic potential is a sum of 1D terms
+    const double vsc = r.pot_kind == 1 ? P.s * 0.5 * r.pot[0] : 0.0;
+    for (int e = 0; e < E; ++e)
+        for (int q = 0; q < Q; ++q) {
+            const double w = A.W[e * Q + q], x = A.X[e * Q + q], cq = A.capq[e * Q + q];
+            for (int a = 0; a < nen; ++a)
+                for (int b = 0; b < nen; ++b) {
+                    const size_t k = (size_t)(e + a) * bw + (b - a + p);
+                    const double Na = A.N0[((size_t)e * nen + a) * Q + q], Nb = A.N0[((size_t)e * nen + b) * Q + q];
+                    const double Da = A.N1[((size_t)e * nen + a) * Q + q], Db = A.N1[((size_t)e * nen + b) * Q + q];
+                    A.M[k] += w * Na * Nb;
+                    A.Xb[k] += w * x * Na * Nb;
+                    A.G[k] += w * Na * Db;
+                    A.S[k] += w * Da * Db;
+                    A.Wc[k] += w * cq * Na * Nb;
+                    A.Vs[k] += w * vsc * x * x * Na * Nb;
+                }
+        }
+    A.A.assign(sz, 0);
+    for (int i = 0; i < A.n; ++i)
+        for (int j = std::max(0, i - p); j <= std::min(A.n - 1, i + p); ++j)
+            A.A[(size_t)i * bw + (j - i + p)] = A.G[(size_t)i * bw + (j - i + p)] - A.G[(size_t)j * bw + (i - j + p)];
+}
+
+inline size_t idx3(const Problem &P, int i, int j, int k) { return (size_t)i + P.ax[0].n * ((size_t)j + (size_t)P.ax[1].n * k); }
+
+bool is_wall(const Problem &P, size_t I)
+{
+    const int n0 = P.ax[0].n, n1 = P.ax[1].n;
+    const int i = (int)(I % n0), j = (int)((I / n0) % n1), k = (int)(I / ((size_t)n0 * n1));
+    return P.ax[0].isWall[i] || P.ax[1].isWall[j] || P.ax[2].isWall[k];
+}
+
+// ------------------------------------------------------ Kronecker term apply
+// y (+)= c (B_z (x) B_y (x) B_x) x, three 1D band passes x, y, z (src/tensorops.cpp:97-153).
+void kron_term(const Problem &P, const std::vector<double> *B[3], cplx c, bool acc, const cplx *x, cplx *y)
+{
+    const int n0 = P.ax[0].n, n1 = P.ax[1].n, n2 = P.ax[2].n;
+    const size_t N = P.N;
+    std::vector<cplx> t1(N), t2(N);
+    {   // x pass
+        const int p = P.ax[0].p, bw = 2 * p + 1;
+        const double *a = B[0]->data();
+#pragma omp parallel for schedule(static)
+        for (long r = 0; r < (long)n1 * n2; ++r)
+            for (int i = 0; i < n0; ++i) {
+                cplx s = 0;
+                for (int J = std::max(0, i - p); J <= std::min(n0 - 1, i + p); ++J) s += a[(size_t)i * bw + (J - i + p)] * x[(size_t)r * n0 + J];
+                t1[(size_t)r * n0 + i] = s;
+            }
+    }
+    {   // y pass
+        const int p = P.ax[1].p, bw = 2 * p + 1;
+        const double *a = B[1]->data();
+#pragma omp parallel for schedule(static)
+        for (long kj = 0; kj < (long)n2 * n1; ++kj) {
+            const int k = (int)(kj / n1), j = (int)(kj % n1);
+            cplx *tr = t2.data() + ((size_t)k * n1 + j) * n0;
+            for (int i = 0; i < n0; ++i) tr[i] = 0;
+            for (int J = std::max(0, j - p); J <= std::min(n1 - 1, j + p); ++J) {
+                const double aj = a[(size_t)j * bw + (J - j + p)];
+                const cplx *src = t1.data() + ((size_t)k * n1 + J) * n0;
+                for (int i = 0; i < n0; ++i) tr[i] += aj * src[i];
+            }
+        }
+    }
+    {   // z pass
+        const int p = P.ax[2].p, bw = 2 * p + 1;
+        const double *a = B[2]->data();
+        const size_t plane = (size_t)n0 * n1;
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < n2; ++k) {
+            cplx *yr = y + (size_t)k * plane;
+            if (!acc) for (size_t m = 0; m < plane; ++m) yr[m] = 0;
+            for (int K = std::max(0, k - p); K <= std::min(n2 - 1, k + p); ++K) {
+                const cplx ak = c * a[(size_t)k * bw + (K - k + p)];
+                const cplx *src = t2.data() + (size_t)K * plane;
+                for (size_t m = 0; m < plane; ++m) yr[m] += ak * src[m];
+            }
+        }
+    }
+}
+
+// --------------------------------------------- per-element potential apply
+// y (+)= c * V x with V_ij = int s V B_i B_j: per element, psi at the points by
+// three 1D contractions (src/quadobs.cpp:181-238), times cV = s V w
+// (src/quadobs.cpp:139), then the transposed contractions scattered into y.
+// Elements are coloured by e_z mod (p+1) so concurrent slabs never share rows.
+void pot_apply(const Problem &P, cplx c, bool acc, const cplx *x, cplx *y)
+{
+    const Axis &X = P.ax[0], &Y = P.ax[1], &Z = P.ax[2];
+    const int n1 = X.p + 1, n2 = Y.p + 1, n3 = Z.p + 1, Q1 = X.Q, Q2 = Y.Q, Q3 = Z.Q;
+    const int g0 = X.n, g1 = Y.n;
+    if (!acc) std::fill(y, y + P.N, cplx(0));
+    if (P.prm.pot_kind == 0) return;
+    const int ncol = n3;
+    for (int col = 0; col < ncol; ++col) {
+#pragma omp parallel
+        {
+            std::vector<cplx> t1((size_t)n3 * n2 * Q1), t2((size_t)n3 * Q2 * Q1), F((size_t)Q3 * Q2 * Q1);
+#pragma omp for schedule(dynamic)
+            for (int e3 = col; e3 < Z.E; e3 += ncol)
+                for (int e2 = 0; e2 < Y.E; ++e2)
+                    for (int e1 = 0; e1 < X.E; ++e1) {
+                        const double *Nx = X.N0.data() + (size_t)e1 * n1 * Q1;
+                        const double *Ny = Y.N0.data() + (size_t)e2 * n2 * Q2;
+                        const double *Nz = Z.N0.data() + (size_t)e3 * n3 * Q3;
+                        // stage x: t1[a3][a2][q1]
+                        for (int a3 = 0; a3 < n3; ++a3)
+                            for (int a2 = 0; a2 < n2; ++a2) {
+                                const cplx *cr = x + (e1 + (size_t)g0 * ((e2 + a2) + (size_t)g1 * (e3 + a3)));
+                                cplx *tr = t1.data() + ((size_t)a3 * n2 + a2) * Q1;
+                                for (int q1 = 0; q1 < Q1; ++q1) tr[q1] = 0;
+                                for (int a1 = 0; a1 < n1; ++a1) {
+                                    const cplx ca = cr[a1];
+                                    for (int q1 = 0; q1 < Q1; ++q1) tr[q1] += Nx[a1 * Q1 + q1] * ca;
+                                }
+                            }
+                        // stage y: t2[a3][q2][q1]
+                        for (int a3 = 0; a3 < n3; ++a3)
+                            for (int q2 = 0; q2 < Q2; ++q2) {
+                                cplx *tr = t2.data() + ((size_t)a3 * Q2 + q2) * Q1;
+                                for (int q1 = 0; q1 < Q1; ++q1) tr[q1] = 0;
+                                for (int a2 = 0; a2 < n2; ++a2) {
+                                    const double ny = Ny[a2 * Q2 + q2];
+                                    const cplx *src = t1.data() + ((size_t)a3 * n2 + a2) * Q1;
+                                    for (int q1 = 0; q1 < Q1; ++q1) tr[q1] += ny * src[q1];
+                                }
+                            }
+                        // stage z: F[q3][q2][q1], times c * s V w
+                        const size_t plane = (size_t)Q2 * Q1;
+                        for (int q3 = 0; q3 < Q3; ++q3) {
+                            cplx *dst = F.data() + (size_t)q3 * plane;
+                            for (size_t m = 0; m < plane; ++m) dst[m] = 0;
+                            for (int a3 = 0; a3 < n3; ++a3) {
+                                const double nz = Nz[a3 * Q3 + q3];
+                                const cplx *src = t2.data() + (size_t)a3 * plane;
+                                for (size_t m = 0; m < plane; ++m) dst[m] += nz * src[m];
+                            }
+                            for (int q2 = 0; q2 < Q2; ++q2)
+                                for (int q1 = 0; q1 < Q1; ++q1) {
+                                    const double xx = X.X[e1 * Q1 + q1], yy = Y.X[e2 * Q2 + q2], zz = Z.X[e3 * Q3 + q3];
+                                    const double w = X.W[e1 * Q1 + q1] * Y.W[e2 * Q2 + q2] * Z.W[e3 * Q3 + q3];
+                                    dst[(size_t)q2 * Q1 + q1] *= c * (P.s * P.pot(xx, yy, zz) * w);
+                                }
+                        }
+                        // transposed stages
+                        for (int a3 = 0; a3 < n3; ++a3) {
+                            cplx *tr = t2.data() + (size_t)a3 * plane;
+                            for (size_t m = 0; m < plane; ++m) tr[m] = 0;
+                            for (int q3 = 0; q3 < Q3; ++q3) {
+                                const double nz = Nz[a3 * Q3 + q3];
+                                const cplx *src = F.data() + (size_t)q3 * plane;
+                                for (size_t m = 0; m < plane; ++m) tr[m] += nz * src[m];
+                            }
+                        }
+                        for (int a3 = 0; a3 < n3; ++a3)
+                            for (int a2 = 0; a2 < n2; ++a2) {
+                                cplx *tr = t1.data() + ((size_t)a3 * n2 + a2) * Q1;
+                                for (int q1 = 0; q1 < Q1; ++q1) tr[q1] = 0;
+                                for (int q2 = 0; q2 < Q2; ++q2) {
+                                    const double ny = Ny[a2 * Q2 + q2];
+                                    const cplx *src = t2.data() + ((size_t)a3 * Q2 + q2) * Q1;
+                                    for (int q1 = 0; q1 < Q1; ++q1) tr[q1] += ny * src[q1];
+                                }
+                            }
+                        for (int a3 = 0; a3 < n3; ++a3)
+                            for (int a2 = 0; a2 < n2; ++a2) {
+                                cplx *yr = y + (e1 + (size_t)g0 * ((e2 + a2) + (size_t)g1 * (e3 + a3)));
+                                const cplx *tr = t1.data() + ((size_t)a3 * n2 + a2) * Q1;
+                                for (int a1 = 0; a1 < n1; ++a1) {
+                                    cplx s = 0;
+                                    for (int q1 = 0; q1 < Q1; ++q1) s += Nx[a1 * Q1 + q1] * tr[q1];
+                                    yr[a1] += s;
+                                }
+                            }
+                    }
+        }
+    }
+}
+
+enum Term { tM = 0, tK = 1, tV = 2, tCAP = 3, tDx = 4, tDy = 5, tDz = 6 };
+
+// y = term x without the wall treatment (the caller applies it)
+void term_apply(const Problem &P, int term, cplx c, bool acc, const cplx *x, cplx *y)
+{
+    const std::vector<double> *Mb[3] = {&P.ax[0].M, &P.ax[1].M, &P.ax[2].M};
+    switch (term) {
+    case tM: kron_term(P, Mb, c, acc, x, y); break;
+    case tK: {  // K = 1/(2m) sum_a S_a (x) M (src/tensorops.cpp:177-183)
+        bool first = !acc;
+        for (int a = 0; a < P.dim; ++a) {
+            const std::vector<double> *U[3] = {Mb[0], Mb[1], Mb[2]};
+            U[a] = &P.ax[a].S;
+            kron_term(P, U, c * (0.5 / P.prm.mass), !first, x, y);
+            first = false;
+        }
+        if (first) std::fill(y, y + P.N, cplx(0));
+        break;
+    }
+    case tV: pot_apply(P, c, acc, x, y); break;
+    case tCAP: {  // CAP = -i sum_d W_d (x) prod M_e (src/assembly/cap.cpp:126-131)
+        bool first = !acc;
+        for (int a = 0; a < P.dim; ++a) {
+            const std::vector<double> *U[3] = {Mb[0], Mb[1], Mb[2]};
+            U[a] = &P.ax[a].Wc;
+            kron_term(P, U, c * cplx(0, -1), !first, x, y);
+            first = false;
+        }
+        if (first) std::fill(y, y + P.N, cplx(0));
+        break;
+    }
+    case tDx: case tDy: case tDz: {
+        const int a = term - tDx;
+        const std::vector<double> *U[3] = {Mb[0], Mb[1], Mb[2]};
+        U[a] = &P.ax[a].Xb;
+        kron_term(P, U, c, acc, x, y);
+        break;
+    }
+    }
+}
+
+// masked copy: wall entries zeroed
+std::vector<cplx> masked(const Problem &P, const cplx *x)
+{
+    std::vector<cplx> v(x, x + P.N);
+    if (P.prm.dirichlet)
+        for (size_t I = 0; I < P.N; ++I) if (is_wall(P, I)) v[I] = 0;
+    return v;
+}
+
+// A = alpha M + g (K + sV + CAP + sum_a sE_a D_a); wall DOFs are identity rows
+// decoupled from the rest (src/assembly/assembly.cpp:601-640 keeps them at zero).
+void op_apply(const Problem &P, cplx alpha, cplx g, const double sE[3], const cplx *xin, cplx *y)
+{
+    std::vector<cplx> xm = masked(P, xin);
+    const cplx *x = xm.data();
+    term_apply(P, tM, alpha, false, x, y);
+    term_apply(P, tK, g, true, x, y);
+    if (P.prm.pot_kind) term_apply(P, tV, g, true, x, y);
+    if (P.prm.cap_kind) term_apply(P, tCAP, g, true, x, y);
+    for (int a = 0; a < P.dim; ++a)
+        if (sE[a] != 0.0) term_apply(P, tDx + a, g * sE[a], true, x, y);
+    if (P.prm.dirichlet)
+        for (size_t I = 0; I < P.N; ++I) if (is_wall(P, I)) y[I] = xin[I];
+}
+
+// ---------------------------------------------------------------- vector ops
+cplx dotc(const cplx *a, const cplx *b, size_t n)
+{
+    double re = 0, im = 0;
+#pragma omp parallel for reduction(+ : re, im) schedule(static)
+    for (long i = 0; i < (long)n; ++i) {
+        const cplx v = std::conj(a[i]) * b[i];
+        re += v.real(); im += v.imag();
+    }
+    return {re, im};
+}
+double nrm2(const cplx *a, size_t n) { return std::sqrt(dotc(a, a, n).real()); }
+
+// ------------------------------------------------- separable preconditioner
+// Exact inverse of the real separable part alpha M + g (K + V_sep) by fast
+// diagonalisation (V_sep: the part of s V that is a sum of 1D terms, the whole
+// harmonic potential, zero otherwise):
+// per axis T v = lambda M v (Cholesky + cyclic Jacobi), then
+// (alpha M + g K)^-1 = (V(x)V(x)V) diag(1/(alpha + g sum lambda)) (V(x)V(x)V)^T.
+struct FD {
+    int n[3];
+    std::vector<double> V[3], lam[3];   // V: n x n row-major (V[i][k] = k-th eigvec at i)
+    std::vector<char> wall[3];
+};
+
+void jacobi_eig(int n, std::vector<double> &A, std::vector<double> &Q, std::vector<double> &lam)
+{
+    Q.assign((size_t)n * n, 0);
+    for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) { const double v = A[(size_t)i * n + j] * A[(size_t)i * n + j]; tot += v; if (i != j) off += v; }
+        if (off <= 1e-30 * tot) break;
+        for (int pp = 0; pp < n - 1; ++pp)
+            for (int qq = pp + 1; qq < n; ++qq) {
+                const double apq = A[(size_t)pp * n + qq];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double app = A[(size_t)pp * n + pp], aqq = A[(size_t)qq * n + qq];
+                const double theta = (aqq - app) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = A[(size_t)k * n + pp], akq = A[(size_t)k * n + qq];
+                    A[(size_t)k * n + pp] = c * akp - s * akq;
+                    A[(size_t)k * n + qq] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A[(size_t)pp * n + k], aqk = A[(size_t)qq * n + k];
+                    A[(size_t)pp * n + k] = c * apk - s * aqk;
+                    A[(size_t)qq * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double qkp = Q[(size_t)k * n + pp], qkq = Q[(size_t)k * n + qq];
+                    Q[(size_t)k * n + pp] = c * qkp - s * qkq;
+                    Q[(size_t)k * n + qq] = s * qkp + c * qkq;
+                }
+            }
+    }
+    lam.resize(n);
+    for (int i = 0; i < n; ++i) lam[i] = A[(size_t)i * n + i];
+}
+
+void fd_setup(const Problem &P, FD &F)
+{
+    for (int d = 0; d < 3; ++d) {
+        const Axis &A = P.ax[d];
+        const int n = A.n, p = A.p, bw = 2 * p + 1;
+        F.n[d] = n;
+        F.wall[d] = A.isWall;
+        // active index list
+        std::vector<int> act;
+        for (int i = 0; i < n; ++i) if (!A.isWall[i]) act.push_back(i);
+        const int m = (int)act.size();
+        std::vector<double> Md((size_t)m * m, 0), Td((size_t)m * m, 0);
+        for (int a = 0; a < m; ++a)
+            for (int b = 0; b < m; ++b) {
+                const int i = act[a], j = act[b];
+                if (std::abs(i - j) > p) continue;
+                Md[(size_t)a * m + b] = A.M[(size_t)i * bw + (j - i + p)];
+                Td[(size_t)a * m + b] = A.active ? 0.5 / P.prm.mass * A.S[(size_t)i * bw + (j - i + p)] + A.Vs[(size_t)i * bw + (j - i + p)] : 0.0;
+            }
+        // Cholesky M = L L^T
+        std::vector<double> L((size_t)m * m, 0);
+        for (int j = 0; j < m; ++j) {
+            double s = Md[(size_t)j * m + j];
+            for (int k = 0; k < j; ++k) s -= L[(size_t)j * m + k] * L[(size_t)j * m + k];
+            L[(size_t)j * m + j] = std::sqrt(s);
+            for (int i = j + 1; i < m; ++i) {
+                double t = Md[(size_t)i * m + j];
+                for (int k = 0; k < j; ++k) t -= L[(size_t)i * m + k] * L[(size_t)j * m + k];
+                L[(size_t)i * m + j] = t / L[(size_t)j * m + j];
+            }
+        }
+        // C = L^-1 T L^-T
+        std::vector<double> Y((size_t)m * m);  // Y = L^-1 T (forward substitution per column)
+        for (int c = 0; c < m; ++c)
+            for (int i = 0; i < m; ++i) {
+                double s = Td[(size_t)i * m + c];
+                for (int k = 0; k < i; ++k) s -= L[(size_t)i * m + k] * Y[(size_t)k * m + c];
+                Y[(size_t)i * m + c] = s / L[(size_t)i * m + i];
+            }
+        std::vector<double> C((size_t)m * m);  // C = Y L^-T: solve L C^T = Y^T
+        for (int r = 0; r < m; ++r)
+            for (int i = 0; i < m; ++i) {
+                double s = Y[(size_t)r * m + i];
+                for (int k = 0; k < i; ++k) s -= L[(size_t)i * m + k] * C[(size_t)r * m + k];
+                C[(size_t)r * m + i] = s / L[(size_t)i * m + i];
+            }
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < i; ++j) { const double v = 0.5 * (C[(size_t)i * m + j] + C[(size_t)j * m + i]); C[(size_t)i * m + j] = C[(size_t)j * m + i] = v; }
+        std::vector<double> Qm, lam;
+        jacobi_eig(m, C, Qm, lam);
+        // V = L^-T Q (back substitution)
+        std::vector<double> Vm((size_t)m * m);
+        for (int c = 0; c < m; ++c)
+            for (int i = m - 1; i >= 0; --i) {
+                double s = Qm[(size_t)i * m + c];
+                for (int k = i + 1; k < m; ++k) s -= L[(size_t)k * m + i] * Vm[(size_t)k * m + c];
+                Vm[(size_t)i * m + c] = s / L[(size_t)i * m + i];
+            }
+        F.V[d].assign((size_t)n * n, 0);
+        F.lam[d].assign(n, 0);
+        for (int a = 0; a < m; ++a)
+            for (int b = 0; b < m; ++b) F.V[d][(size_t)act[a] * n + act[b]] = Vm[(size_t)a * m + b];
+        for (int b = 0; b < m; ++b) F.lam[d][act[b]] = lam[b];
+    }
+}
+
+// y[.., i, ..] = sum_k T[i][k] x[.., k, ..] (trans: T[k][i]) along axis d
+void dense_axis(const Problem &P, int d, const std::vector<double> &T, bool trans, const cplx *x, cplx *y)
+{
+    const int n0 = P.ax[0].n, n1 = P.ax[1].n, n2 = P.ax[2].n, n = P.ax[d].n;
+    const size_t stride = d == 0 ? 1 : (d == 1 ? (size_t)n0 : (size_t)n0 * n1);
+    const size_t lines = P.N / n;
+#pragma omp parallel for schedule(static)
+    for (long l = 0; l < (long)lines; ++l) {
+        size_t base;
+        if (d == 0) base = (size_t)l * n0;
+        else if (d == 1) base = (size_t)(l % n0) + (size_t)(l / n0) * n0 * n1;
+        else base = (size_t)l;
+        (void)n2;
+        std::vector<cplx> in(n);
+        for (int k = 0; k < n; ++k) in[k] = x[base + k * stride];
+        for (int i = 0; i < n; ++i) {
+            cplx s = 0;
+            for (int k = 0; k < n; ++k) s += (trans ? T[(size_t)k * n + i] : T[(size_t)i * n + k]) * in[k];
+            y[base + i * stride] = s;
+        }
+    }
+}
+
+void fd_apply(const Problem &P, const FD &F, cplx alpha, cplx g, const cplx *r, cplx *z)
+{
+    std::vector<cplx> a(r, r + P.N), b(P.N);
+    dense_axis(P, 0, F.V[0], true, a.data(), b.data());
+    dense_axis(P, 1, F.V[1], true, b.data(), a.data());
+    dense_axis(P, 2, F.V[2], true, a.data(), b.data());
+    const int n0 = P.ax[0].n, n1 = P.ax[1].n;
+    for (size_t I = 0; I < P.N; ++I) {
+        const int i = (int)(I % n0), j = (int)((I / n0) % n1), k = (int)(I / ((size_t)n0 * n1));
+        b[I] /= alpha + g * (F.lam[0][i] + F.lam[1][j] + F.lam[2][k]);
+    }
+    dense_axis(P, 0, F.V[0], false, b.data(), a.data());
+    dense_axis(P, 1, F.V[1], false, a.data(), b.data());
+    dense_axis(P, 2, F.V[2], false, b.data(), z);
+    if (P.prm.dirichlet)
+        for (size_t I = 0; I < P.N; ++I) if (is_wall(P, I)) z[I] = r[I];
+}
+
+// --------------------------------------------------------------------- FGMRES
+// Right-preconditioned flexible GMRES(m), modified Gram-Schmidt, complex Givens;
+// stops on the (true, since right-preconditioned) residual estimate
+// |r| <= rtol |b| (src/propagator.cpp:131-182).
+struct Solver {
+    const Problem *P;
+    FD fd;
+    bool fd_ready = false;
+};
+
+int fgmres(const Problem &P, Solver &S, cplx alpha, cplx g, const double sE[3], const cplx *b, cplx *x,
+           double rtol, int maxit, int m, int pc, int *its_out, double *res_out)
+{
+    const size_t N = P.N;
+    std::vector<std::vector<cplx>> V(m + 1, std::vector<cplx>(N)), Z(m, std::vector<cplx>(N));
+    std::vector<cplx> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), w(N);
+    const double bnorm = nrm2(b, N);
+    int its = 0;
+    double res = 0;
+    if (bnorm == 0) { std::fill(x, x + N, cplx(0)); *its_out = 0; *res_out = 0; return 0; }
+    for (;;) {
+        op_apply(P, alpha, g, sE, x, w.data());
+        for (size_t i = 0; i < N; ++i) V[0][i] = b[i] - w[i];
+        double beta = nrm2(V[0].data(), N);
+        res = beta / bnorm;
+        if (res <= rtol || its >= maxit) break;
+        for (size_t i = 0; i < N; ++i) V[0][i] /= beta;
+        std::fill(gv.begin(), gv.end(), cplx(0));
+        gv[0] = beta;
+        int j = 0;
+        for (; j < m && its < maxit; ++j) {
+            if (pc == 1) fd_apply(P, S.fd, alpha, g, V[j].data(), Z[j].data());
+            else Z[j] = V[j];
+            op_apply(P, alpha, g, sE, Z[j].data(), w.data());
+            for (int i = 0; i <= j; ++i) {
+                const cplx h = dotc(V[i].data(), w.data(), N);
+                H[(size_t)i * m + j] = h;
+                for (size_t k = 0; k < N; ++k) w[k] -= h * V[i][k];
+            }
+            const double hn = nrm2(w.data(), N);
+            H[(size_t)(j + 1) * m + j] = hn;
+            if (hn > 0) for (size_t k = 0; k < N; ++k) V[j + 1][k] = w[k] / hn;
+            for (int i = 0; i < j; ++i) {
+                const cplx a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+                H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
+                H[(size_t)(i + 1) * m + j] = -std::conj(sn[i]) * a + cs[i] * bb;
+            }
+            const cplx h1 = H[(size_t)j * m + j], h2 = H[(size_t)(j + 1) * m + j];
+            const double rho = std::sqrt(std::norm(h1) + std::norm(h2));
+            if (std::abs(h1) == 0) { cs[j] = 0; sn[j] = 1; }
+            else { cs[j] = std::abs(h1) / rho; sn[j] = (h1 / std::abs(h1)) * std::conj(h2) / rho; }
+            H[(size_t)j * m + j] = cs[j] * h1 + sn[j] * h2;
+            H[(size_t)(j + 1) * m + j] = 0;
+            gv[j + 1] = -std::conj(sn[j]) * gv[j];
+            gv[j] = cs[j] * gv[j];
+            ++its;
+            res = std::abs(gv[j + 1]) / bnorm;
+            if (res <= rtol) { ++j; break; }
+        }
+        // back substitution and update x += Z y
+        std::vector<cplx> yv(j);
+        for (int i = j - 1; i >= 0; --i) {
+            cplx s = gv[i];
+            for (int k = i + 1; k < j; ++k) s -= H[(size_t)i * m + k] * yv[k];
+            yv[i] = s / H[(size_t)i * m + i];
+        }
+        for (int i = 0; i < j; ++i)
+            for (size_t k = 0; k < N; ++k) x[k] += yv[i] * Z[i][k];
+        if (res <= rtol || its >= maxit) {
+            // true residual of the final iterate
+            op_apply(P, alpha, g, sE, x, w.data());
+            for (size_t i = 0; i < N; ++i) w[i] = b[i] - w[i];
+            res = nrm2(w.data(), N) / bnorm;
+            if (res <= rtol * 10 || its >= maxit) break;
+        }
+    }
+    *its_out = its;
+    *res_out = res;
+    return res <= rtol * 10 ? 0 : 1;
+}
+
+// ------------------------------------------------------------------ multigrid
+bool coarsen_knots(const Knots &U, Knots &C)
+{
+    std::vector<std::pair<double, int>> bp;
+    for (double u : U) {
+        if (!bp.empty() && u == bp.back().first) ++bp.back().second;
+        else bp.push_back({u, 1});
+    }
+    C.clear();
+    bool dropped = false;
+    for (size_t i = 0; i < bp.size(); ++i) {
+        const bool interior = i > 0 && i + 1 < bp.size();
+        if (interior && bp[i].second == 1 && i % 2 == 1) { dropped = true; continue; }
+        C.insert(C.end(), bp[i].second, bp[i].first);
+    }
+    return dropped;
+}
+
+// Boehm knot insertion, one knot at a time, dense rows (src/multigrid.cpp:52-81)
+std::vector<std::vector<double>> prolongation(const Knots &C, const Knots &F, int p)
+{
+    Knots U = C;
+    const int nc = (int)C.size() - p - 1;
+    std::vector<std::vector<double>> Pm(nc, std::vector<double>(nc, 0.0));
+    for (int j = 0; j < nc; ++j) Pm[j][j] = 1.0;
+    Knots add;
+    std::set_difference(F.begin(), F.end(), C.begin(), C.end(), std::back_inserter(add));
+    for (double xi : add) {
+        const int n = (int)U.size() - p - 1;
+        int k = (int)(std::upper_bound(U.begin(), U.end(), xi) - U.begin()) - 1;
+        k = std::min(std::max(k, p), n - 1);
+        std::vector<std::vector<double>> Q(n + 1, std::vector<double>(nc, 0.0));
+        for (int i = 0; i <= k - p; ++i) Q[i] = Pm[i];
+        for (int i = k + 1; i <= n; ++i) Q[i] = Pm[i - 1];
+        for (int i = k - p + 1; i <= k; ++i) {
+            const double a = (xi - U[i]) / (U[i + p] - U[i]);
+            for (int c = 0; c < nc; ++c) Q[i][c] = a * Pm[i][c] + (1 - a) * Pm[i - 1][c];
+        }
+        Pm.swap(Q);
+        U.insert(std::upper_bound(U.begin(), U.end(), xi), xi);
+    }
+    for (auto &r : Pm) for (auto &v : r) if (std::fabs(v) < 1e-14) v = 0;
+    return Pm;
+}
+
+std::vector<double> greville(const Knots &U, int p)
+{
+    const int n = (int)U.size() - p - 1;
+    std::vector<double> g(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 1; j <= p; ++j) g[i] += U[i + j];
+        g[i] /= std::max(p, 1);
+    }
+    return g;
+}
+
+}  // namespace
+
+// ============================================================== C interface
+extern "C" {
+
+void *orc_create(const orc_params *prm)
+{
+    if (!prm || prm->dim < 1 || prm->dim > 3 || prm->p < 1 || prm->nquad < 1) return nullptr;
+    auto *P = new Problem();
+    P->prm = *prm;
+    P->dim = prm->dim;
+    P->s = -prm->charge;
+    for (int d = 0; d < 3; ++d) setup_axis(*P, d);
+    P->N = (size_t)P->ax[0].n * P->ax[1].n * P->ax[2].n;
+    return P;
+}
+void orc_destroy(void *h) { delete static_cast<Problem *>(h); }
+int orc_n(void *h, int d) { return static_cast<Problem *>(h)->ax[d].n; }
+long orc_size(void *h) { return (long)static_cast<Problem *>(h)->N; }
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_gauss(int Q, double *x, double *w) { gauss_legendre(Q, x, w); }
+
+void orc_knots(void *h, int d, double *out)
+{
+    const Axis &A = static_cast<Problem *>(h)->ax[d];
+    std::copy(A.t.begin(), A.t.end(), out);
+}
+
+// pts/wts [E*Q]; N0/N1 [E*(p+1)*Q]; cap [E*Q]
+void orc_axis_tables(void *h, int d, double *pts, double *wts, double *N0, double *N1, double *cap)
+{
+    const Axis &A = static_cast<Problem *>(h)->ax[d];
+    if (pts) std::copy(A.X.begin(), A.X.end(), pts);
+    if (wts) std::copy(A.W.begin(), A.W.end(), wts);
+    if (N0) std::copy(A.N0.begin(), A.N0.end(), N0);
+    if (N1) std::copy(A.N1.begin(), A.N1.end(), N1);
+    if (cap) std::copy(A.capq.begin(), A.capq.end(), cap);
+}
+
+// which: 0 M, 1 X, 2 G, 3 A = G - G^T, 4 S, 5 W (CAP profile band)
+void orc_band(void *h, int d, int which, double *out)
+{
+    const Axis &A = static_cast<Problem *>(h)->ax[d];
+    const std::vector<double> *b[6] = {&A.M, &A.Xb, &A.G, &A.A, &A.S, &A.Wc};
+    std::copy(b[which]->begin(), b[which]->end(), out);
+}
+
+// y = term x (complex interleaved); term: 0 M, 1 K, 2 sV, 3 CAP, 4-6 D_x..D_z
+void orc_apply_term(void *h, int term, const double *x, double *y)
+{
+    const Problem &P = *static_cast<Problem *>(h);
+    term_apply(P, term, 1.0, false, reinterpret_cast<const cplx *>(x), reinterpret_cast<cplx *>(y));
+}
+
+void orc_apply(void *h, double ar, double ai, double gr, double gi, const double *sE, const double *x, double *y)
+{
+    const Problem &P = *static_cast<Problem *>(h);
+    op_apply(P, cplx(ar, ai), cplx(gr, gi), sE, reinterpret_cast<const cplx *>(x), reinterpret_cast<cplx *>(y));
+}
+
+// Brute force: dense N x N complex matrix of one term, accumulated per element
+// and per 3D quadrature point from the point values of the basis
+// (the analogue of TDSEZFormHam, src/assembly/assembly.cpp:703-864).
+int orc_dense(void *h, int term, double *out)
+{
+    const Problem &P = *static_cast<Problem *>(h);
+    const size_t N = P.N;
+    if (N > 6000) return 1;
+    cplx *A = reinterpret_cast<cplx *>(out);
+    std::fill(A, A + N * N, cplx(0));
+    const Axis &X = P.ax[0], &Y = P.ax[1], &Z = P.ax[2];
+    const int n1 = X.p + 1, n2 = Y.p + 1, n3 = Z.p + 1;
+    for (int e3 = 0; e3 < Z.E; ++e3)
+        for (int e2 = 0; e2 < Y.E; ++e2)
+            for (int e1 = 0; e1 < X.E; ++e1)
+                for (int q3 = 0; q3 < Z.Q; ++q3)
+                    for (int q2 = 0; q2 < Y.Q; ++q2)
+                        for (int q1 = 0; q1 < X.Q; ++q1) {
+                            const double r[3] = {X.X[e1 * X.Q + q1], Y.X[e2 * Y.Q + q2], Z.X[e3 * Z.Q + q3]};
+                            const double w = X.W[e1 * X.Q + q1] * Y.W[e2 * Y.Q + q2] * Z.W[e3 * Z.Q + q3];
+                            const int e[3] = {e1, e2, e3}, q[3] = {q1, q2, q3};
+                            double cap = 0;
+                            for (int d = 0; d < P.dim; ++d) cap += P.ax[d].capq[e[d] * P.ax[d].Q + q[d]];
+                            const int na = n1 * n2 * n3;
+                            std::vector<double> N0(na), D[3];
+                            for (int d = 0; d < 3; ++d) D[d].resize(na);
+                            for (int a3 = 0; a3 < n3; ++a3)
+                                for (int a2 = 0; a2 < n2; ++a2)
+                                    for (int a1 = 0; a1 < n1; ++a1) {
+                                        const int aa[3] = {a1, a2, a3};
+                                        double v[3], dv[3];
+                                        for (int d = 0; d < 3; ++d) {
+                                            const Axis &A = P.ax[d];
+                                            v[d] = A.N0[((size_t)e[d] * (A.p + 1) + aa[d]) * A.Q + q[d]];
+                                            dv[d] = A.N1[((size_t)e[d] * (A.p + 1) + aa[d]) * A.Q + q[d]];
+                                        }
+                                        const int k = a1 + n1 * (a2 + n2 * a3);
+                                        N0[k] = v[0] * v[1] * v[2];
+                                        D[0][k] = dv[0] * v[1] * v[2];
+                                        D[1][k] = v[0] * dv[1] * v[2];
+                                        D[2][k] = v[0] * v[1] * dv[2];
+                                    }
+                            for (int ka = 0; ka < na; ++ka)
+                                for (int kb = 0; kb < na; ++kb) {
+                                    const int a1 = ka % n1, a2 = (ka / n1) % n2, a3 = ka / (n1 * n2);
+                                    const int b1 = kb % n1, b2 = (kb / n1) % n2, b3 = kb / (n1 * n2);
+                                    const size_t I = idx3(P, e1 + a1, e2 + a2, e3 + a3), J = idx3(P, e1 + b1, e2 + b2, e3 + b3);
+                                    cplx v = 0;
+                                    switch (term) {
+                                    case tM: v = w * N0[ka] * N0[kb]; break;
+                                    case tK: {
+                                        double s = 0;
+                                        for (int d = 0; d < P.dim; ++d) s += D[d][ka] * D[d][kb];
+                                        v = 0.5 / P.prm.mass * w * s;
+                                        break;
+                                    }
+                                    case tV: v = P.s * P.pot(r[0], r[1], r[2]) * w * N0[ka] * N0[kb]; break;
+                                    case tCAP: v = cplx(0, -1) * cap * w * N0[ka] * N0[kb]; break;
+                                    default: v = r[term - tDx] * w * N0[ka] * N0[kb]; break;
+                                    }
+                                    A[I * N + J] += v;
+                                }
+                        }
+    return 0;
+}
+
+// Solve (alpha M + g H_E) x = b, x holds the initial guess; pc: 0 none, 1 separable.
+int orc_solve(void *h, double ar, double ai, double gr, double gi, const double *sE, const double *b, double *x,
+              double rtol, int maxit, int restart, int pc, int *its, double *res)
+{
+    const Problem &P = *static_cast<Problem *>(h);
+    Solver S;
+    S.P = &P;
+    if (pc == 1) fd_setup(P, S.fd);
+    return fgmres(P, S, cplx(ar, ai), cplx(gr, gi), sE, reinterpret_cast<const cplx *>(b), reinterpret_cast<cplx *>(x),
+                  rtol, maxit, restart, pc, its, res);
+}
+
+// Separable preconditioner alone: z = (alpha M + g K)^-1 r (walls: identity)
+void orc_fd_apply(void *h, double ar, double ai, double gr, double gi, const double *r, double *z)
+{
+    const Problem &P = *static_cast<Problem *>(h);
+    FD F;
+    fd_setup(P, F);
+    fd_apply(P, F, cplx(ar, ai), cplx(gr, gi), reinterpret_cast<const cplx *>(r), reinterpret_cast<cplx *>(z));
+}
+
+// Crank-Nicolson steps: (M + i dt/2 H_E(t_mid)) psi_{n+1} = (M - i dt/2 H_E(t_mid)) psi_n,
+// sE_mid [nsteps*3] holds s E_a at the midpoints. its/res per step may be NULL.
+int orc_cn_steps(void *h, double *psi, int nsteps, double dt, const double *sE_mid, double rtol, int maxit, int restart,
+                 int pc, int *its, double *res)
+{
+    const Problem &P = *static_cast<Problem *>(h);
+    Solver S;
+    S.P = &P;
+    if (pc == 1) fd_setup(P, S.fd);
+    cplx *x = reinterpret_cast<cplx *>(psi);
+    std::vector<cplx> b(P.N);
+    const cplx g(0, 0.5 * dt);
+    for (int s = 0; s < nsteps; ++s) {
+        const double *sE = sE_mid + 3 * s;
+        op_apply(P, 1.0, -g, sE, x, b.data());
+        int it = 0;
+        double r = 0;
+        const int rc = fgmres(P, S, 1.0, g, sE, b.data(), x, rtol, maxit, restart, pc, &it, &r);
+        if (its) its[s] = it;
+        if (res) res[s] = r;
+        if (rc) return 1 + s;
+    }
+    return 0;
+}
+
+// Pade(2,2) step psi <- R22(-i h M^-1 H_E) psi: two stages
+// (M + g_k H_E) y = (M + conj(g_k) H_E) x, g_{1,2} = h(-+sqrt3 + 3i)/12
+// (src/propagator.cpp:564-676).
+int orc_pade22_step(void *h, double *psi, double hstep, const double *sE, double rtol, int maxit, int restart, int pc,
+                    int *its)
+{
+    const Problem &P = *static_cast<Problem *>(h);
+    Solver S;
+    S.P = &P;
+    if (pc == 1) fd_setup(P, S.fd);
+    const double s3 = std::sqrt(3.0);
+    const cplx gk[2] = {hstep * cplx(-s3, 3.0) / 12.0, hstep * cplx(s3, 3.0) / 12.0};
+    cplx *x = reinterpret_cast<cplx *>(psi);
+    std::vector<cplx> b(P.N);
+    for (int k = 0; k < 2; ++k) {
+        op_apply(P, 1.0, std::conj(gk[k]), sE, x, b.data());
+        int it = 0;
+        double r = 0;
+        if (fgmres(P, S, 1.0, gk[k], sE, b.data(), x, rtol, maxit, restart, pc, &it, &r)) return 1 + k;
+        if (its) its[k] = it;
+    }
+    return 0;
+}
+
+// Observables: out = [norm psi^H M psi, Re <H>, Im <H>, <x>, <y>, <z>]
+void orc_observables(void *h, const double *psi_in, double *out)
+{
+    const Problem &P = *static_cast<Problem *>(h);
+    const cplx *psi = reinterpret_cast<const cplx *>(psi_in);
+    std::vector<cplx> v(P.N);
+    const double zero[3] = {0, 0, 0};
+    op_apply(P, 1.0, 0.0, zero, psi, v.data());
+    out[0] = dotc(psi, v.data(), P.N).real();
+    op_apply(P, 0.0, 1.0, zero, psi, v.data());
+    const cplx e = dotc(psi, v.data(), P.N);
+    out[1] = e.real(); out[2] = e.imag();
+    std::vector<cplx> xm = masked(P, psi);
+    for (int a = 0; a < 3; ++a) {
+        if (a >= P.dim) { out[3 + a] = 0; continue; }
+        term_apply(P, tDx + a, 1.0, false, xm.data(), v.data());
+        out[3 + a] = dotc(xm.data(), v.data(), P.N).real();
+    }
+}
+
+// L2 projection of the Gaussian packet (src/core.cpp:1188-1262):
+// b_i = int B_i psi0, M c = b (CG to 1e-13), then c /= sqrt(c^H M c).
+// centre/width/momentum per axis (3 each).
+int orc_project_gaussian(void *h, const double *centre, const double *width, const double *momentum, double *out)
+{
+    const Problem &P = *static_cast<Problem *>(h);
+    const size_t N = P.N;
+    std::vector<cplx> b(N, 0.0);
+    const Axis &X = P.ax[0], &Y = P.ax[1], &Z = P.ax[2];
+    // separable Gaussian: b = b_x (x) b_y (x) b_z
+    std::vector<cplx> bd[3];
+    for (int d = 0; d < 3; ++d) {
+        const Axis &A = P.ax[d];
+        bd[d].assign(A.n, 0.0);
+        if (!A.active) { bd[d][0] = 1.0; continue; }
+        for (int e = 0; e < A.E; ++e)
+            for (int q = 0; q < A.Q; ++q) {
+                const double x = A.X[e * A.Q + q], w = A.W[e * A.Q + q];
+                const double s = width[d], u = x - centre[d];
+                const cplx v = std::pow(2.0 * M_PI * s * s, -0.25) * std::exp(cplx(-u * u / (4.0 * s * s), momentum[d] * x));
+                for (int a = 0; a <= A.p; ++a) bd[d][e + a] += w * A.N0[((size_t)e * (A.p + 1) + a) * A.Q + q] * v;
+            }
+    }
+    for (int k = 0; k < Z.n; ++k)
+        for (int j = 0; j < Y.n; ++j)
+            for (int i = 0; i < X.n; ++i) b[idx3(P, i, j, k)] = bd[0][i] * bd[1][j] * bd[2][k];
+    if (P.prm.dirichlet)
+        for (size_t I = 0; I < N; ++I) if (is_wall(P, I)) b[I] = 0;
+    // CG on M (walls identity), Jacobi preconditioner
+    std::vector<cplx> x(N, 0.0), r = b, z(N), pv(N), q(N);
+    std::vector<double> dg(N);
+    {
+        std::vector<cplx> e(N, 0.0), col(N);
+        const int n0 = X.n, n1 = Y.n;
+        for (size_t I = 0; I < N; ++I) {
+            const int i = (int)(I % n0), j = (int)((I / n0) % n1), k = (int)(I / ((size_t)n0 * n1));
+            const int p0 = X.p, p1 = Y.p, p2 = Z.p;
+            dg[I] = X.M[(size_t)i * (2 * p0 + 1) + p0] * Y.M[(size_t)j * (2 * p1 + 1) + p1] * Z.M[(size_t)k * (2 * p2 + 1) + p2];
+            if (P.prm.dirichlet && is_wall(P, I)) dg[I] = 1;
+        }
+    }
+    auto Mapply = [&](const cplx *in, cplx *o) {
+        const double zero[3] = {0, 0, 0};
+        op_apply(P, 1.0, 0.0, zero, in, o);
+    };
+    const double bn = nrm2(b.data(), N);
+    for (size_t i = 0; i < N; ++i) z[i] = r[i] / dg[i];
+    pv = z;
+    cplx rz = dotc(r.data(), z.data(), N);
+    int it = 0;
+    for (; it < 10000; ++it) {
+        if (nrm2(r.data(), N) <= 1e-13 * bn) break;
+        Mapply(pv.data(), q.data());
+        const cplx al = rz / dotc(pv.data(), q.data(), N);
+        for (size_t i = 0; i < N; ++i) { x[i] += al * pv[i]; r[i] -= al * q[i]; }
+        for (size_t i = 0; i < N; ++i) z[i] = r[i] / dg[i];
+        const cplx rz2 = dotc(r.data(), z.data(), N);
+        const cplx be = rz2 / rz;
+        rz = rz2;
+        for (size_t i = 0; i < N; ++i) pv[i] = z[i] + be * pv[i];
+    }
+    Mapply(x.data(), q.data());
+    const double nrm = std::sqrt(dotc(x.data(), q.data(), N).real());
+    cplx *o = reinterpret_cast<cplx *>(out);
+    for (size_t i = 0; i < N; ++i) o[i] = x[i] / nrm;
+    return it;
+}
+
+// ---- multigrid pieces on plain knot vectors
+// returns the coarse length, 0 if nothing could be dropped
+int orc_coarsen_knots(const double *U, int m, double *C)
+{
+    Knots u(U, U + m), c;
+    if (!coarsen_knots(u, c)) return 0;
+    std::copy(c.begin(), c.end(), C);
+    return (int)c.size();
+}
+// dense nf x nc prolongation by Boehm insertion
+int orc_prolongation(const double *C, int mc, const double *F, int mf, int p, double *out)
+{
+    Knots c(C, C + mc), f(F, F + mf);
+    auto Pm = prolongation(c, f, p);
+    const int nf = mf - p - 1, nc = mc - p - 1;
+    if ((int)Pm.size() != nf) return 1;
+    for (int i = 0; i < nf; ++i)
+        for (int j = 0; j < nc; ++j) out[(size_t)i * nc + j] = Pm[i][j];
+    return 0;
+}
+void orc_greville(const double *U, int m, int p, double *out)
+{
+    Knots u(U, U + m);
+    auto g = greville(u, p);
+    std::copy(g.begin(), g.end(), out);
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+"""MI355X-native TDSE-Z propagation hot path (Python view of the C ABI).
+
+The product is ``libtdz.so`` (HIP kernels + C++ host code) with the C ABI of
+``include/tdz.h``. This module is a thin ctypes binding used by the tests and
+the benchmark; it raises if the library is missing instead of falling back to
+any CPU path.
+
+Problem descriptions are dicts with the fields of ``tdz_problem``
+(include/tdz.h): dim, nelem, lo, hi, p, nquad, dirichlet, mass, charge,
+pot_kind, pot, cap_kind, cap, mg_levels, mg_smooth, restart.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtdz.so")
+
+PC_NONE, PC_SEPARABLE, PC_MG = 0, 1, 2
+TERM_M, TERM_K, TERM_V, TERM_CAP, TERM_DX, TERM_DY, TERM_DZ = range(7)
+POT_NONE, POT_HARMONIC, POT_SOFTCORE = 0, 1, 2
+CAP_NONE, CAP_QUADRATIC, CAP_MANOLOPOULOS = 0, 1, 2
+
+# every symbol include/tdz.h declares
+EXPORTS = [
+    "tdz_create", "tdz_destroy", "tdz_last_error", "tdz_size", "tdz_dims", "tdz_levels",
+    "tdz_vec_alloc", "tdz_vec_free", "tdz_vec_upload", "tdz_vec_download", "tdz_sync",
+    "tdz_set_shift", "tdz_op_apply", "tdz_term_apply", "tdz_pc_setup", "tdz_pc_apply",
+    "tdz_ksp_solve", "tdz_step_cn", "tdz_evolve_cn", "tdz_observables", "tdz_pulse_coupling",
+    "tdz_timing", "tdz_timing_stats",
+]
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("nelem", C.c_int * 3), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+        ("p", C.c_int), ("nquad", C.c_int), ("dirichlet", C.c_int),
+        ("mass", C.c_double), ("charge", C.c_double),
+        ("pot_kind", C.c_int), ("pot", C.c_double * 4),
+        ("cap_kind", C.c_int), ("cap", C.c_double * 4),
+        ("mg_levels", C.c_int), ("mg_smooth", C.c_int), ("restart", C.c_int),
+    ]
+
+
+class KspStats(C.Structure):
+    _fields_ = [("its", C.c_int), ("restarts", C.c_int), ("res", C.c_double), ("bnorm", C.c_double),
+                ("pc_rebuilds", C.c_int)]
+
+
+class Pulse(C.Structure):
+    _fields_ = [("E0", C.c_double * 3), ("omega", C.c_double), ("ncycles", C.c_double), ("cep", C.c_double)]
+
+
+def make_problem(desc):
+    p = Problem()
+    p.dim = desc["dim"]
+    for d in range(3):
+        p.nelem[d] = desc["nelem"][d] if d < desc["dim"] else 1
+        p.lo[d] = desc["lo"][d] if d < desc["dim"] else 0.0
+        p.hi[d] = desc["hi"][d] if d < desc["dim"] else 0.0
+    p.p = desc["p"]
+    p.nquad = desc["nquad"]
+    p.dirichlet = desc.get("dirichlet", 0)
+    p.mass = desc.get("mass", 1.0)
+    p.charge = desc.get("charge", -1.0)
+    p.pot_kind = desc.get("pot_kind", 0)
+    for i, v in enumerate(desc.get("pot", ())):
+        p.pot[i] = v
+    p.cap_kind = desc.get("cap_kind", 0)
+    for i, v in enumerate(desc.get("cap", ())):
+        p.cap[i] = v
+    p.mg_levels = desc.get("mg_levels", 0)
+    p.mg_smooth = desc.get("mg_smooth", 1)
+    p.restart = desc.get("restart", 0)
+    return p
+
+
+def make_pulse(E0=(0, 0, 0), omega=0.0, ncycles=0.0, cep=0.0):
+    pl = Pulse()
+    for a in range(3):
+        pl.E0[a] = E0[a]
+    pl.omega, pl.ncycles, pl.cep = omega, ncycles, cep
+    return pl
+
+
+_lib = None
+
+
+def lib():
+    """Load libtdz.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP extension missing: {LIB_PATH} (run python tdsez_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip, d, i = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_double, C.c_int
+        L.tdz_create.argtypes = [C.POINTER(Problem), C.POINTER(vp)]
+        L.tdz_destroy.argtypes = [vp]
+        L.tdz_last_error.restype = C.c_char_p
+        L.tdz_size.argtypes = [vp]
+        L.tdz_size.restype = C.c_long
+        L.tdz_dims.argtypes = [vp, ip]
+        L.tdz_levels.argtypes = [vp]
+        L.tdz_vec_alloc.argtypes = [vp, C.POINTER(vp)]
+        L.tdz_vec_free.argtypes = [vp, vp]
+        L.tdz_vec_upload.argtypes = [vp, vp, dp]
+        L.tdz_vec_download.argtypes = [vp, vp, dp]
+        L.tdz_sync.argtypes = [vp]
+        L.tdz_set_shift.argtypes = [vp, d, d, d, d, dp]
+        L.tdz_op_apply.argtypes = [vp, vp, vp]
+        L.tdz_term_apply.argtypes = [vp, i, vp, vp]
+        L.tdz_pc_setup.argtypes = [vp, i]
+        L.tdz_pc_apply.argtypes = [vp, vp, vp]
+        L.tdz_ksp_solve.argtypes = [vp, vp, vp, d, i, C.POINTER(KspStats)]
+        L.tdz_step_cn.argtypes = [vp, vp, d, dp, d, i, C.POINTER(KspStats)]
+        L.tdz_evolve_cn.argtypes = [vp, vp, d, d, i, C.POINTER(Pulse), d, i, dp, ip]
+        L.tdz_observables.argtypes = [vp, vp, dp]
+        L.tdz_pulse_coupling.argtypes = [vp, C.POINTER(Pulse), d, dp]
+        L.tdz_timing.argtypes = [vp, i]
+        L.tdz_timing_stats.argtypes = [vp, dp, C.POINTER(C.c_long)]
+        _lib = L
+    return _lib
+
+
+class TdzError(RuntimeError):
+    pass
+
+
+def _chk(rc):
+    if rc:
+        raise TdzError(lib().tdz_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DeviceVec:
+    """N complex128 values in HBM, owned by a Context."""
+
+    def __init__(self, ctx, host=None):
+        self.ctx = ctx
+        ptr = C.c_void_p()
+        _chk(lib().tdz_vec_alloc(ctx.h, C.byref(ptr)))
+        self.ptr = ptr
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host):
+        h = np.ascontiguousarray(host, dtype=np.complex128).reshape(-1)
+        assert h.size == self.ctx.N
+        _chk(lib().tdz_vec_upload(self.ctx.h, self.ptr, _dp(h.view(np.float64))))
+
+    def download(self):
+        out = np.zeros(self.ctx.N, dtype=np.complex128)
+        _chk(lib().tdz_vec_download(self.ctx.h, self.ptr, _dp(out.view(np.float64))))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ctx.h:
+            _chk(lib().tdz_vec_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """One discretised problem on cuda:0 behind the C ABI."""
+
+    def __init__(self, desc):
+        self.desc = dict(desc)
+        self._prob = make_problem(desc)
+        h = C.c_void_p()
+        _chk(lib().tdz_create(C.byref(self._prob), C.byref(h)))
+        self.h = h
+        self.N = int(lib().tdz_size(self.h))
+        n = (C.c_int * 3)()
+        _chk(lib().tdz_dims(self.h, n))
+        self.n = tuple(n)
+        self.levels = int(lib().tdz_levels(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tdz_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def vec(self, host=None):
+        return DeviceVec(self, host)
+
+    def set_shift(self, alpha, g, sE=(0, 0, 0)):
+        s = np.ascontiguousarray(sE, dtype=np.float64)
+        _chk(lib().tdz_set_shift(self.h, complex(alpha).real, complex(alpha).imag, complex(g).real,
+                                 complex(g).imag, _dp(s)))
+
+    def apply(self, x, y):
+        _chk(lib().tdz_op_apply(self.h, x.ptr, y.ptr))
+        _chk(lib().tdz_sync(self.h))
+
+    def term(self, t, x, y):
+        _chk(lib().tdz_term_apply(self.h, t, x.ptr, y.ptr))
+
+    def pc_setup(self, kind):
+        _chk(lib().tdz_pc_setup(self.h, kind))
+
+    def pc_apply(self, r, z):
+        _chk(lib().tdz_pc_apply(self.h, r.ptr, z.ptr))
+
+    def solve(self, b, x, rtol=1e-12, maxit=1000):
+        st = KspStats()
+        _chk(lib().tdz_ksp_solve(self.h, b.ptr, x.ptr, rtol, maxit, C.byref(st)))
+        return st
+
+    def step_cn(self, psi, dt, sE_mid=(0, 0, 0), rtol=1e-12, maxit=1000):
+        st = KspStats()
+        s = np.ascontiguousarray(sE_mid, dtype=np.float64)
+        _chk(lib().tdz_step_cn(self.h, psi.ptr, dt, _dp(s), rtol, maxit, C.byref(st)))
+        return st
+
+    def evolve_cn(self, psi, t0, dt, nsteps, pulse=None, rtol=1e-12, maxit=1000, observe=False):
+        its = np.zeros(nsteps, dtype=np.int32)
+        obs = np.zeros((nsteps, 7)) if observe else None
+        _chk(lib().tdz_evolve_cn(self.h, psi.ptr, t0, dt, nsteps, C.byref(pulse) if pulse is not None else None,
+                                 rtol, maxit, _dp(obs) if observe else None,
+                                 its.ctypes.data_as(C.POINTER(C.c_int))))
+        return its, obs
+
+    def observables(self, psi):
+        out = np.zeros(6)
+        _chk(lib().tdz_observables(self.h, psi.ptr, _dp(out)))
+        return out
+
+    def coupling(self, pulse, t):
+        out = np.zeros(3)
+        _chk(lib().tdz_pulse_coupling(self.h, C.byref(pulse), t, _dp(out)))
+        return out
+
+    def sync(self):
+        _chk(lib().tdz_sync(self.h))
+
+    def timing(self, enable):
+        _chk(lib().tdz_timing(self.h, 1 if enable else 0))
+
+    def timing_stats(self):
+        ms, n = C.c_double(0), C.c_long(0)
+        _chk(lib().tdz_timing_stats(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,913 @@
+// Host side of the MI355X-native propagation path: the context behind the C ABI
+// of include/tdz.h, the matrix-free operator apply, the preconditioners, FGMRES
+// and the Crank-Nicolson driver.
+//
+// Reference behaviour restated here (C++ over HIP kernels, no PETSc):
+//   TSTHETA theta=1/2 CN step, midpoint field     src/propagator.cpp:94-101, src/monitor.cpp:150-162, :367-382
+//   FGMRES, restart 30 / 50 above 1e5 rows, right PC, nonzero guess
+//                                                 src/propagator.cpp:131-182
+//   PC refresh when its > 2*base + 5              include/tdsez_propagator.hpp:17-31, src/propagator.cpp:22-33
+//   GMG hierarchy, Boehm P, R = P^T, Galerkin     src/multigrid.cpp:205-327
+//   H = K + s V + CAP, field terms s E_a D_a      include/tdsez_parser.hpp:92-105, :190-194; src/manager.cpp:1011-1027
+//   monitors: norm, <H>, dipoles                  src/manager.cpp:584-672, src/monitor.cpp:1145-1194
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tdz.h"
+#include "tdz_internal.hpp"
+
+using cplx = std::complex<double>;
+using tdz::dcomplex;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) throw Error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + #x); \
+    } while (0)
+
+inline dcomplex dc(cplx z) { return {z.real(), z.imag()}; }
+
+template <typename T>
+T *dalloc(size_t n)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T *>(p);
+}
+
+// Sparse 1D matrix in CSR on device.
+struct CSR {
+    int rows = 0;
+    int *rp = nullptr, *col = nullptr;
+    double *val = nullptr;
+};
+
+struct Level {
+    tdz::Axis1D ax[3];
+    int n[3] = {1, 1, 1};
+    long N = 1;
+    // device tables
+    double *dB0[3] = {}, *dX[3] = {}, *dW[3] = {};
+    int *dOff[3] = {};
+    double *dM[3] = {};
+    dcomplex *dB[3] = {};
+    dcomplex *dBmass[3] = {};   // bands of M alone (M on x, zero on y and z)
+    unsigned char *dWall[3] = {};
+    // host bands of this level (Galerkin on coarse levels)
+    std::vector<double> M[3], S[3], Xb[3], Wc[3], Vs[3];
+    // separable preconditioner
+    double *dV[3] = {}, *dLam[3] = {};
+    bool fd_ready = false;
+    // transfer to the next coarser level: P (this x coarse) and R = P^T
+    CSR P[3], R[3];
+    // multigrid work vectors (coarse levels: r, z; all: t)
+    dcomplex *r = nullptr, *z = nullptr, *t = nullptr;
+    // dim 3 with element e starting at function e on every axis (tiled kernel applies)
+    bool simple = false;
+};
+
+}  // namespace
+
+struct tdz_ctx {
+    tdz_problem prob{};
+    tdz::PhysParams ph;
+    hipStream_t st = nullptr;
+    std::vector<std::unique_ptr<Level>> lv;
+    cplx alpha = 1.0, g = 0.0;
+    double sE[3] = {0, 0, 0};
+    dcomplex *tmp[4] = {};
+    dcomplex *w1 = nullptr, *w2 = nullptr, *rhs = nullptr;
+    int restart = 30;
+    dcomplex *Vb = nullptr, *Zb = nullptr;
+    dcomplex *dpart = nullptr, *dh = nullptr, *dh2 = nullptr;
+    double *dpartr = nullptr, *dnorm = nullptr;
+    int pc_kind = TDZ_PC_NONE;
+    bool pc_ready = false;
+    std::vector<void *> allocs;     // everything to free
+    std::vector<double *> user_vecs;
+    int refresh_base = -1, rebuilds = 0;
+    bool use_tiled = true;   // tiled 3D potential kernel where it applies
+    // kernel timing (measurement hook)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
+};
+
+namespace {
+
+template <typename T>
+T *track(tdz_ctx *c, T *p)
+{
+    c->allocs.push_back(p);
+    return p;
+}
+
+template <typename T>
+T *upload(tdz_ctx *c, const std::vector<T> &h)
+{
+    T *d = track(c, dalloc<T>(h.size()));
+    if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+// B = P^T A P for a band A (fine level l) and sparse P (fine x coarse), returned as a coarse band.
+std::vector<double> galerkin_band(const std::vector<double> &A, int nf, int p,
+                                  const std::vector<std::vector<std::pair<int, double>>> &P, int nc)
+{
+    const int bw = 2 * p + 1;
+    // AP (nf x nc) dense, then P^T (AP)
+    std::vector<double> AP((size_t)nf * nc, 0.0);
+    for (int i = 0; i < nf; ++i)
+        for (int j = std::max(0, i - p); j <= std::min(nf - 1, i + p); ++j) {
+            const double a = A[(size_t)i * bw + (j - i + p)];
+            if (a == 0.0) continue;
+            for (auto &e : P[j]) AP[(size_t)i * nc + e.first] += a * e.second;
+        }
+    std::vector<double> C((size_t)nc * bw, 0.0);
+    for (int i = 0; i < nf; ++i)
+        for (auto &e : P[i])
+            for (int J = std::max(0, e.first - p); J <= std::min(nc - 1, e.first + p); ++J)
+                C[(size_t)e.first * bw + (J - e.first + p)] += e.second * AP[(size_t)i * nc + J];
+    return C;
+}
+
+CSR make_csr(tdz_ctx *c, const std::vector<std::vector<std::pair<int, double>>> &rows)
+{
+    std::vector<int> rp(1, 0), col;
+    std::vector<double> val;
+    for (auto &r : rows) {
+        for (auto &e : r) { col.push_back(e.first); val.push_back(e.second); }
+        rp.push_back((int)col.size());
+    }
+    CSR s;
+    s.rows = (int)rows.size();
+    s.rp = upload(c, rp);
+    s.col = upload(c, col);
+    s.val = upload(c, val);
+    return s;
+}
+
+void upload_level_tables(tdz_ctx *c, Level &L)
+{
+    for (int d = 0; d < 3; ++d) {
+        const tdz::Axis1D &A = L.ax[d];
+        L.dB0[d] = upload(c, A.B0);
+        L.dX[d] = upload(c, A.X);
+        L.dW[d] = upload(c, A.W);
+        L.dOff[d] = upload(c, A.off);
+        L.dM[d] = upload(c, L.M[d]);
+        std::vector<unsigned char> w(A.isWall.begin(), A.isWall.end());
+        L.dWall[d] = upload(c, w);
+        L.dB[d] = track(c, dalloc<dcomplex>(L.M[d].size()));
+        std::vector<dcomplex> bm(L.M[d].size(), dcomplex{0.0, 0.0});
+        if (d == 0)
+            for (size_t k = 0; k < bm.size(); ++k) bm[k] = {L.M[d][k], 0.0};
+        L.dBmass[d] = upload(c, bm);
+    }
+}
+
+// Complex bands of alpha M + g (K + CAP + sE.D) split over the three axes:
+// B_x = alpha M_x + g T_x, B_y = g T_y, B_z = g T_z, T_d = S_d/(2m) - i W_d + sE_d X_d.
+void update_bands(tdz_ctx *c, Level &L)
+{
+    const double inv2m = 0.5 / c->ph.mass;
+    for (int d = 0; d < 3; ++d) {
+        const size_t sz = L.M[d].size();
+        std::vector<dcomplex> B(sz);
+        const bool active = d < c->prob.dim;
+        for (size_t k = 0; k < sz; ++k) {
+            cplx T = 0.0;
+            if (active) T = inv2m * L.S[d][k] + cplx(0, -1) * L.Wc[d][k] + c->sE[d] * L.Xb[d][k];
+            cplx b = c->g * T;
+            if (d == 0) b += c->alpha * L.M[d][k];
+            B[k] = dc(b);
+        }
+        HIPCHK(hipMemcpyAsync(L.dB[d], B.data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
+tdz::AxisDev axis_dev(const Level &L, int d)
+{
+    const tdz::Axis1D &A = L.ax[d];
+    return {A.E, A.p, A.Q, A.n, L.dB0[d], L.dX[d], L.dW[d], L.dOff[d]};
+}
+
+tdz::SepBands sep_bands(const Level &L)
+{
+    tdz::SepBands sb;
+    for (int d = 0; d < 3; ++d) {
+        sb.n[d] = L.n[d];
+        sb.p[d] = L.ax[d].p;
+        sb.M[d] = L.dM[d];
+        sb.B[d] = L.dB[d];
+    }
+    return sb;
+}
+
+// y += g (s V) x on level L: the tiled 3D kernel where it applies, else the generic one.
+void pot_apply(tdz_ctx *c, Level &L, cplx g, const dcomplex *x, dcomplex *y)
+{
+    tdz::AxisDev ax[3] = {axis_dev(L, 0), axis_dev(L, 1), axis_dev(L, 2)};
+    tdz::PotParams pp{c->ph.pot_kind, c->ph.s, c->ph.pot[0], c->ph.pot[1]};
+    const bool timed = c->timing && &L == c->lv[0].get();
+    if (timed) {
+        if (c->ev_used == c->ev.size()) {
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            c->ev.push_back({a, b});
+        }
+        HIPCHK(hipEventRecord(c->ev[c->ev_used].first, c->st));
+    }
+    if (!(L.simple && c->use_tiled && tdz::launch_pot_apply_3d(c->st, ax, pp, g.real(), g.imag(), x, y)))
+        tdz::launch_pot_apply_generic(c->st, ax, pp, g.real(), g.imag(), x, y);
+    if (timed) HIPCHK(hipEventRecord(c->ev[c->ev_used++].second, c->st));
+}
+
+// y = A_l x with the current shift (separable bands + matrix-free potential).
+void level_apply(tdz_ctx *c, Level &L, const dcomplex *x, dcomplex *y)
+{
+    tdz::launch_sep_apply(c->st, sep_bands(L), x, y, c->tmp);
+    if (c->ph.pot_kind && c->g != 0.0) pot_apply(c, L, c->g, x, y);
+    if (c->prob.dirichlet) {
+        const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+        tdz::launch_wall_fix(c->st, L.n, w, x, y);
+    }
+}
+
+void setup_fd(tdz_ctx *c, Level &L)
+{
+    if (L.fd_ready) return;
+    const double inv2m = 0.5 / c->ph.mass;
+    for (int d = 0; d < 3; ++d) {
+        std::vector<double> V, lam;
+        const tdz::Axis1D &A = L.ax[d];
+        if (d < c->prob.dim) {
+            std::vector<double> T(L.S[d].size());
+            for (size_t k = 0; k < T.size(); ++k) T[k] = inv2m * L.S[d][k] + L.Vs[d][k];
+            tdz::Axis1D Ab = A;
+            Ab.M = L.M[d];
+            tdz::gen_eig_bands(Ab, T, V, lam);
+        } else {
+            V = {1.0};
+            lam = {0.0};
+        }
+        L.dV[d] = upload(c, V);
+        L.dLam[d] = upload(c, lam);
+    }
+    L.fd_ready = true;
+}
+
+// z = (alpha M + g (K + V_sep))^-1 r by fast diagonalisation (walls: identity);
+// V_sep is the part of s V that is a sum of 1D terms. s1, s2: scratch.
+void fd_apply(tdz_ctx *c, Level &L, const dcomplex *r, dcomplex *z, dcomplex *s1, dcomplex *s2)
+{
+    tdz::launch_dense_axis(c->st, 0, L.n, L.dV[0], true, r, s1);
+    tdz::launch_dense_axis(c->st, 1, L.n, L.dV[1], true, s1, s2);
+    tdz::launch_dense_axis(c->st, 2, L.n, L.dV[2], true, s2, s1);
+    const double *lam[3] = {L.dLam[0], L.dLam[1], L.dLam[2]};
+    const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+    tdz::launch_diag_scale(c->st, L.n, lam, w, dc(c->alpha), dc(c->g), s1, s2);
+    tdz::launch_dense_axis(c->st, 0, L.n, L.dV[0], false, s2, s1);
+    tdz::launch_dense_axis(c->st, 1, L.n, L.dV[1], false, s1, s2);
+    tdz::launch_dense_axis(c->st, 2, L.n, L.dV[2], false, s2, z);
+    if (c->prob.dirichlet) tdz::launch_wall_fix(c->st, L.n, w, r, z);
+}
+
+// z = restriction-to-coarse / prolongation along three axes (CSR passes)
+void transfer(tdz_ctx *c, const CSR ops[3], const int nin[3], const int nout[3], const dcomplex *in, dcomplex *out,
+              dcomplex *s1, dcomplex *s2, bool accumulate)
+{
+    int a[3] = {nin[0], nin[1], nin[2]};
+    tdz::launch_csr_axis(c->st, 0, a, nout[0], ops[0].rp, ops[0].col, ops[0].val, in, s1, false);
+    a[0] = nout[0];
+    tdz::launch_csr_axis(c->st, 1, a, nout[1], ops[1].rp, ops[1].col, ops[1].val, s1, s2, false);
+    a[1] = nout[1];
+    tdz::launch_csr_axis(c->st, 2, a, nout[2], ops[2].rp, ops[2].col, ops[2].val, s2, out, accumulate);
+}
+
+// Richardson sweep z += S^-1 (r - A z) on level L (resid in L.t)
+void smooth(tdz_ctx *c, Level &L, const dcomplex *r, dcomplex *z, dcomplex *s1, dcomplex *s2, dcomplex *s3)
+{
+    level_apply(c, L, z, L.t);
+    tdz::launch_axpby(c->st, {1, 0}, r, {-1, 0}, L.t, L.N);  // t = r - A z
+    fd_apply(c, L, L.t, s3, s1, s2);
+    tdz::launch_axpby(c->st, {1, 0}, s3, {1, 0}, z, L.N);   // z += S^-1 t
+}
+
+// V-cycle z = B_l r on level l (z is overwritten).
+void vcycle(tdz_ctx *c, int l, const dcomplex *r, dcomplex *z)
+{
+    Level &L = *c->lv[l];
+    const int nu = std::max(1, c->prob.mg_smooth);
+    // scratch: w1, w2 (fine-level sized) for the transforms; tmp[3] holds a smoother
+    // correction (level_apply, the only other user of tmp, has finished by then)
+    dcomplex *s1 = c->w1, *s2 = c->w2;
+    fd_apply(c, L, r, z, s1, s2);
+    const bool coarsest = l + 1 == (int)c->lv.size();
+    const int pre = coarsest ? nu + 1 : nu;
+    for (int s = 1; s < pre; ++s) smooth(c, L, r, z, s1, s2, c->tmp[3]);
+    if (coarsest) return;
+    Level &Cl = *c->lv[l + 1];
+    level_apply(c, L, z, L.t);
+    tdz::launch_axpby(c->st, {1, 0}, r, {-1, 0}, L.t, L.N);      // t = r - A z
+    transfer(c, L.R, L.n, Cl.n, L.t, Cl.r, s1, s2, false);         // r_c = P^T t
+    vcycle(c, l + 1, Cl.r, Cl.z);
+    transfer(c, L.P, Cl.n, L.n, Cl.z, z, s1, s2, true);            // z += P z_c
+    for (int s = 0; s < nu; ++s) smooth(c, L, r, z, s1, s2, c->tmp[3]);
+}
+
+void pc_apply_dev(tdz_ctx *c, const dcomplex *r, dcomplex *z)
+{
+    Level &L = *c->lv[0];
+    switch (c->pc_kind) {
+    case TDZ_PC_SEPARABLE: fd_apply(c, L, r, z, c->w1, c->w2); break;
+    case TDZ_PC_MG: vcycle(c, 0, r, z); break;
+    default: HIPCHK(hipMemcpyAsync(z, r, L.N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st)); break;
+    }
+}
+
+double dev_norm2(tdz_ctx *c, const dcomplex *x, long N)
+{
+    tdz::launch_norm2(c->st, x, N, c->dpartr, c->dnorm);
+    double h = 0;
+    HIPCHK(hipMemcpyAsync(&h, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return h;
+}
+
+// FGMRES(m), right preconditioning, classical Gram-Schmidt with one
+// re-orthogonalisation (two fused passes), complex Givens rotations. Stops when
+// the residual estimate |g_{j+1}| <= rtol ||b|| (true residual for right PC).
+// With r0_in_v0 the caller has already stored the initial residual b - A x in V_0.
+int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, tdz_ksp_stats *st,
+           bool r0_in_v0 = false)
+{
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    const int m = c->restart;
+    auto Vj = [&](int j) { return c->Vb + (long)j * N; };
+    auto Zj = [&](int j) { return c->pc_kind == TDZ_PC_NONE ? Vj(j) : c->Zb + (long)j * N; };
+    std::vector<cplx> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hv(m + 1), hv2(m + 1);
+    const double bnorm = std::sqrt(dev_norm2(c, b, N));
+    int its = 0, restarts = 0;
+    double res = 0;
+    if (bnorm == 0.0) {
+        HIPCHK(hipMemsetAsync(x, 0, N * sizeof(dcomplex), c->st));
+        if (st) *st = {0, 0, 0.0, 0.0, c->rebuilds};
+        return 0;
+    }
+    bool converged = false;
+    for (;;) {
+        if (!r0_in_v0) {
+            level_apply(c, L, x, Vj(0));
+            tdz::launch_axpby(c->st, {1, 0}, b, {-1, 0}, Vj(0), N);   // V0 = b - A x
+        }
+        r0_in_v0 = false;
+        const double beta = std::sqrt(dev_norm2(c, Vj(0), N));
+        res = beta / bnorm;
+        if (res <= rtol) { converged = true; break; }
+        if (its >= maxit) break;
+        ++restarts;
+        tdz::launch_axpby(c->st, {1.0 / beta, 0}, Vj(0), {0, 0}, Vj(0), N);
+        std::fill(gv.begin(), gv.end(), cplx(0));
+        gv[0] = beta;
+        int j = 0;
+        for (; j < m && its < maxit; ++j) {
+            if (c->pc_kind != TDZ_PC_NONE) pc_apply_dev(c, Vj(j), Zj(j));
+            dcomplex *w = Vj(j + 1);
+            level_apply(c, L, Zj(j), w);
+            // CGS2: two passes of h = V^H w, w -= V h
+            tdz::launch_multi_dot(c->st, c->Vb, N, j + 1, w, N, c->dpart, c->dh);
+            tdz::launch_multi_axpy_norm(c->st, c->Vb, N, j + 1, c->dh, w, N, c->dpartr, c->dnorm);
+            tdz::launch_multi_dot(c->st, c->Vb, N, j + 1, w, N, c->dpart, c->dh2);
+            tdz::launch_multi_axpy_norm(c->st, c->Vb, N, j + 1, c->dh2, w, N, c->dpartr, c->dnorm);
+            std::vector<dcomplex> h1(j + 1), h2(j + 1);
+            double wn2 = 0;
+            HIPCHK(hipMemcpyAsync(h1.data(), c->dh, (j + 1) * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(h2.data(), c->dh2, (j + 1) * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(&wn2, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = cplx(h1[i].re + h2[i].re, h1[i].im + h2[i].im);
+            const double hn = std::sqrt(wn2);
+            H[(size_t)(j + 1) * m + j] = hn;
+            if (hn > 0) tdz::launch_scale_invsqrt(c->st, w, c->dnorm, w, N);
+            for (int i = 0; i < j; ++i) {
+                const cplx a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+                H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
+                H[(size_t)(i + 1) * m + j] = -std::conj(sn[i]) * a + cs[i] * bb;
+            }
+            const cplx h1j = H[(size_t)j * m + j], h2j = H[(size_t)(j + 1) * m + j];
+            const double rho = std::sqrt(std::norm(h1j) + std::norm(h2j));
+            if (std::abs(h1j) == 0.0) { cs[j] = 0.0; sn[j] = 1.0; }
+            else { cs[j] = std::abs(h1j) / rho; sn[j] = (h1j / std::abs(h1j)) * std::conj(h2j) / rho; }
+            H[(size_t)j * m + j] = cs[j] * h1j + sn[j] * h2j;
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            gv[j + 1] = -std::conj(sn[j]) * gv[j];
+            gv[j] = cs[j] * gv[j];
+            ++its;
+            res = std::abs(gv[j + 1]) / bnorm;
+            if (res <= rtol || hn == 0.0) { ++j; converged = true; break; }
+        }
+        std::vector<cplx> yv(j);
+        for (int i = j - 1; i >= 0; --i) {
+            cplx s = gv[i];
+            for (int k = i + 1; k < j; ++k) s -= H[(size_t)i * m + k] * yv[k];
+            yv[i] = s / H[(size_t)i * m + i];
+        }
+        std::vector<dcomplex> yd(j);
+        for (int i = 0; i < j; ++i) yd[i] = dc(yv[i]);
+        HIPCHK(hipMemcpyAsync(c->dh, yd.data(), j * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+        tdz::launch_multi_axpy(c->st, c->pc_kind == TDZ_PC_NONE ? c->Vb : c->Zb, N, j, c->dh, x, N);
+        if (converged || its >= maxit) break;
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (st) *st = {its, restarts, res, bnorm, c->rebuilds};
+    if (!converged) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "KSP did not converge: %d iterations, relative residual %.3e", its, res);
+        throw Error(msg);
+    }
+    return its;
+}
+
+void set_shift(tdz_ctx *c, cplx alpha, cplx g, const double sE[3])
+{
+    c->alpha = alpha;
+    c->g = g;
+    for (int a = 0; a < 3; ++a) c->sE[a] = sE ? sE[a] : 0.0;
+    for (auto &L : c->lv) update_bands(c, *L);
+}
+
+void pc_setup(tdz_ctx *c, int kind)
+{
+    if (kind < TDZ_PC_NONE || kind > TDZ_PC_MG) throw Error("unknown preconditioner kind");
+    if (kind == TDZ_PC_MG && c->lv.size() < 2) throw Error("multigrid needs at least two levels");
+    c->pc_kind = kind;
+    if (kind == TDZ_PC_SEPARABLE) setup_fd(c, *c->lv[0]);
+    if (kind == TDZ_PC_MG)
+        for (auto &L : c->lv) setup_fd(c, *L);
+    c->pc_ready = true;
+}
+
+double pulse_field(const tdz_pulse &pl, int a, double t)
+{
+    if (pl.omega <= 0 || pl.ncycles <= 0) return 0.0;
+    const double T = pl.ncycles * 2.0 * M_PI / pl.omega;
+    if (t < 0 || t > T) return 0.0;
+    const double s = std::sin(M_PI * t / T);
+    return pl.E0[a] * s * s * std::sin(pl.omega * t + pl.cep);
+}
+
+void build(tdz_ctx *c, const tdz_problem &pr)
+{
+    if (pr.dim < 1 || pr.dim > 3) throw Error("dim must be 1, 2 or 3");
+    if (pr.p < 1 || pr.p > 9) throw Error("spline degree must be in [1, 9]");
+    if (pr.nquad < 1 || pr.nquad > 16) throw Error("nquad must be in [1, 16]");
+    if (pr.mass <= 0) throw Error("mass must be positive");
+    for (int d = 0; d < pr.dim; ++d) {
+        if (pr.nelem[d] < 1) throw Error("nelem must be positive");
+        if (!(pr.hi[d] > pr.lo[d])) throw Error("empty domain");
+    }
+    c->prob = pr;
+    c->ph.mass = pr.mass;
+    c->ph.s = -pr.charge;
+    c->ph.pot_kind = pr.pot_kind;
+    c->ph.cap_kind = pr.cap_kind;
+    for (int i = 0; i < 4; ++i) { c->ph.pot[i] = pr.pot[i]; c->ph.cap[i] = pr.cap[i]; }
+    HIPCHK(hipStreamCreate(&c->st));
+
+    // ---- level hierarchy (src/multigrid.cpp:239-268): coarsen while every axis keeps >= p+1 spans
+    std::vector<std::array<std::vector<double>, 3>> knots;
+    std::array<std::vector<double>, 3> k0;
+    for (int d = 0; d < pr.dim; ++d) k0[d] = tdz::uniform_knots(pr.nelem[d], pr.lo[d], pr.hi[d], pr.p);
+    knots.push_back(k0);
+    const int maxLevels = pr.mg_levels > 0 ? pr.mg_levels : 20;
+    while ((int)knots.size() < maxLevels) {
+        auto C = knots.back();
+        bool any = false;
+        long dofs = 1;
+        for (int d = 0; d < pr.dim; ++d) {
+            std::vector<double> Cd;
+            if (tdz::coarsen_knots(knots.back()[d], Cd) && tdz::knot_spans(Cd) >= pr.p + 1) {
+                C[d] = Cd;
+                any = true;
+            }
+            dofs *= (long)C[d].size() - pr.p - 1;
+        }
+        if (!any) break;
+        knots.push_back(C);
+        if (pr.mg_levels == 0 && dofs < 2000) break;
+    }
+    const int nl = (int)knots.size();
+    for (int l = 0; l < nl; ++l) {
+        auto L = std::make_unique<Level>();
+        for (int d = 0; d < 3; ++d) {
+            if (d < pr.dim) tdz::build_axis(L->ax[d], knots[l][d], pr.p, pr.nquad, pr.dirichlet != 0, c->ph);
+            else tdz::build_degenerate_axis(L->ax[d]);
+            L->n[d] = L->ax[d].n;
+            L->M[d] = L->ax[d].M; L->S[d] = L->ax[d].S; L->Xb[d] = L->ax[d].Xb; L->Wc[d] = L->ax[d].Wc;
+            L->Vs[d] = L->ax[d].Vs;
+        }
+        L->N = (long)L->n[0] * L->n[1] * L->n[2];
+        L->simple = pr.dim == 3;
+        for (int d = 0; d < 3; ++d)
+            for (int e = 0; e < L->ax[d].E; ++e) L->simple = L->simple && L->ax[d].off[e] == e;
+        c->lv.push_back(std::move(L));
+    }
+    // ---- transfers and Galerkin bands (P^T A P per axis; exact for separable terms)
+    for (int l = 0; l + 1 < nl; ++l) {
+        Level &F = *c->lv[l], &Cl = *c->lv[l + 1];
+        for (int d = 0; d < 3; ++d) {
+            std::vector<std::vector<std::pair<int, double>>> P;
+            if (d < pr.dim) {
+                P = tdz::boehm_prolongation(knots[l + 1][d], knots[l][d], pr.p);
+                // exactness: partition of unity and linear precision at Greville points (src/multigrid.cpp:283-292)
+                const auto gc = tdz::greville(knots[l + 1][d], pr.p), gf = tdz::greville(knots[l][d], pr.p);
+                double err = 0;
+                for (size_t i = 0; i < P.size(); ++i) {
+                    double s1 = 0, sx = 0;
+                    for (auto &e : P[i]) { s1 += e.second; sx += e.second * gc[e.first]; }
+                    err = std::max({err, std::fabs(s1 - 1), std::fabs(sx - gf[i]) / (1 + std::fabs(gf[i]))});
+                }
+                if (P.size() != gf.size() || err > 1e-10) throw Error("knot-insertion prolongation is not exact");
+                // walls: the coarse wall function maps only to the fine wall function
+                if (pr.dirichlet) {
+                    for (size_t i = 0; i < P.size(); ++i) {
+                        auto &r = P[i];
+                        r.erase(std::remove_if(r.begin(), r.end(), [&](auto &e) {
+                                    return Cl.ax[d].isWall[e.first] || F.ax[d].isWall[i];
+                                }), r.end());
+                    }
+                }
+            } else {
+                P = {{{0, 1.0}}};
+            }
+            const int nf = F.n[d], nc = Cl.n[d];
+            if (d < pr.dim) {
+                Cl.M[d] = galerkin_band(F.M[d], nf, pr.p, P, nc);
+                Cl.S[d] = galerkin_band(F.S[d], nf, pr.p, P, nc);
+                Cl.Xb[d] = galerkin_band(F.Xb[d], nf, pr.p, P, nc);
+                Cl.Wc[d] = galerkin_band(F.Wc[d], nf, pr.p, P, nc);
+                Cl.Vs[d] = galerkin_band(F.Vs[d], nf, pr.p, P, nc);
+                if (pr.dirichlet)
+                    for (int i = 0; i < nc; ++i)
+                        if (Cl.ax[d].isWall[i]) {
+                            const int bw = 2 * pr.p + 1;
+                            for (int k = 0; k < bw; ++k) {
+                                Cl.M[d][(size_t)i * bw + k] = 0; Cl.S[d][(size_t)i * bw + k] = 0;
+                                Cl.Xb[d][(size_t)i * bw + k] = 0; Cl.Wc[d][(size_t)i * bw + k] = 0;
+                                Cl.Vs[d][(size_t)i * bw + k] = 0;
+                            }
+                        }
+            }
+            std::vector<std::vector<std::pair<int, double>>> R(nc);
+            for (int i = 0; i < nf; ++i)
+                for (auto &e : P[i]) R[e.first].push_back({i, e.second});
+            F.P[d] = make_csr(c, P);
+            F.R[d] = make_csr(c, R);
+        }
+    }
+    for (auto &L : c->lv) upload_level_tables(c, *L);
+    const long N = c->lv[0]->N;
+    for (int i = 0; i < 4; ++i) c->tmp[i] = track(c, dalloc<dcomplex>(N));
+    c->w1 = track(c, dalloc<dcomplex>(N));
+    c->w2 = track(c, dalloc<dcomplex>(N));
+    c->rhs = track(c, dalloc<dcomplex>(N));
+    for (int l = 0; l < nl; ++l) {
+        Level &L = *c->lv[l];
+        L.t = track(c, dalloc<dcomplex>(L.N));
+        if (l > 0) {
+            L.r = track(c, dalloc<dcomplex>(L.N));
+            L.z = track(c, dalloc<dcomplex>(L.N));
+        }
+    }
+    c->restart = pr.restart > 0 ? pr.restart : (N > 100000 ? 50 : 30);
+    if (c->restart > 60) throw Error("restart must be <= 60");
+    c->Vb = track(c, dalloc<dcomplex>((size_t)(c->restart + 1) * N));
+    c->Zb = track(c, dalloc<dcomplex>((size_t)c->restart * N));
+    c->dpart = track(c, dalloc<dcomplex>((size_t)(c->restart + 1) * tdz::kRedBlocks));
+    c->dh = track(c, dalloc<dcomplex>(c->restart + 1));
+    c->dh2 = track(c, dalloc<dcomplex>(c->restart + 1));
+    c->dpartr = track(c, dalloc<double>(tdz::kRedBlocks));
+    c->dnorm = track(c, dalloc<double>(1));
+    const double zero[3] = {0, 0, 0};
+    set_shift(c, 1.0, 0.0, zero);
+}
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+const char *tdz_last_error(void) { return g_err.c_str(); }
+
+int tdz_create(const tdz_problem *prob, tdz_ctx **ctx)
+{
+    if (!prob || !ctx) { g_err = "null argument"; return 1; }
+    auto *c = new tdz_ctx();
+    const int rc = guarded([&] { build(c, *prob); });
+    if (rc) {
+        tdz_destroy(c);
+        *ctx = nullptr;
+        return rc;
+    }
+    *ctx = c;
+    return 0;
+}
+
+int tdz_destroy(tdz_ctx *c)
+{
+    if (!c) return 0;
+    if (c->st) hipStreamSynchronize(c->st);
+    for (auto &e : c->ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    for (void *p : c->allocs) hipFree(p);
+    for (double *p : c->user_vecs) hipFree(p);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+    return 0;
+}
+
+long tdz_size(const tdz_ctx *c) { return c ? c->lv[0]->N : 0; }
+int tdz_dims(const tdz_ctx *c, int n[3])
+{
+    if (!c) return 1;
+    for (int d = 0; d < 3; ++d) n[d] = c->lv[0]->n[d];
+    return 0;
+}
+int tdz_levels(const tdz_ctx *c) { return c ? (int)c->lv.size() : 0; }
+
+int tdz_vec_alloc(tdz_ctx *c, double **v)
+{
+    return guarded([&] {
+        dcomplex *p = dalloc<dcomplex>(c->lv[0]->N);
+        HIPCHK(hipMemset(p, 0, c->lv[0]->N * sizeof(dcomplex)));
+        c->user_vecs.push_back(reinterpret_cast<double *>(p));
+        *v = reinterpret_cast<double *>(p);
+    });
+}
+int tdz_vec_free(tdz_ctx *c, double *v)
+{
+    return guarded([&] {
+        auto it = std::find(c->user_vecs.begin(), c->user_vecs.end(), v);
+        if (it == c->user_vecs.end()) throw Error("vector not owned by this context");
+        c->user_vecs.erase(it);
+        HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipFree(v));
+    });
+}
+int tdz_vec_upload(tdz_ctx *c, double *d, const double *h)
+{
+    return guarded([&] {
+        HIPCHK(hipMemcpyAsync(d, h, c->lv[0]->N * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+int tdz_vec_download(tdz_ctx *c, const double *d, double *h)
+{
+    return guarded([&] {
+        HIPCHK(hipMemcpyAsync(h, d, c->lv[0]->N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+int tdz_sync(tdz_ctx *c)
+{
+    return guarded([&] { HIPCHK(hipStreamSynchronize(c->st)); });
+}
+
+int tdz_set_shift(tdz_ctx *c, double ar, double ai, double gr, double gi, const double sE[3])
+{
+    return guarded([&] { set_shift(c, cplx(ar, ai), cplx(gr, gi), sE); });
+}
+
+int tdz_op_apply(tdz_ctx *c, const double *x, double *y)
+{
+    return guarded([&] {
+        level_apply(c, *c->lv[0], reinterpret_cast<const dcomplex *>(x), reinterpret_cast<dcomplex *>(y));
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int tdz_term_apply(tdz_ctx *c, int term, const double *x, double *y)
+{
+    return guarded([&] {
+        if (term < TDZ_TERM_M || term > TDZ_TERM_DZ) throw Error("unknown term");
+        const cplx a0 = c->alpha, g0 = c->g;
+        double s0[3] = {c->sE[0], c->sE[1], c->sE[2]};
+        tdz::PhysParams keep = c->ph;
+        Level &L = *c->lv[0];
+        const dcomplex *xd = reinterpret_cast<const dcomplex *>(x);
+        dcomplex *yd = reinterpret_cast<dcomplex *>(y);
+        // one term = a shift with only that term switched on
+        const double inv2m = 0.5 / c->ph.mass;
+        std::vector<std::vector<dcomplex>> B(3);
+        for (int d = 0; d < 3; ++d) {
+            const size_t sz = L.M[d].size();
+            B[d].assign(sz, {0, 0});
+            const bool act = d < c->prob.dim;
+            for (size_t k = 0; k < sz; ++k) {
+                cplx b = 0;
+                if (term == TDZ_TERM_M && d == 0) b = L.M[d][k];
+                if (term == TDZ_TERM_K && act) b = inv2m * L.S[d][k];
+                if (term == TDZ_TERM_CAP && act) b = cplx(0, -1) * L.Wc[d][k];
+                if (term >= TDZ_TERM_DX && d == term - TDZ_TERM_DX && act) b = L.Xb[d][k];
+                B[d][k] = dc(b);
+            }
+            HIPCHK(hipMemcpyAsync(L.dB[d], B[d].data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+        }
+        if (term == TDZ_TERM_V) {
+            HIPCHK(hipMemsetAsync(yd, 0, L.N * sizeof(dcomplex), c->st));
+            if (c->ph.pot_kind) pot_apply(c, L, 1.0, xd, yd);
+        } else {
+            tdz::launch_sep_apply(c->st, sep_bands(L), xd, yd, c->tmp);
+        }
+        if (c->prob.dirichlet) {
+            const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+            tdz::launch_wall_fix(c->st, L.n, w, xd, yd);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->ph = keep;
+        set_shift(c, a0, g0, s0);
+    });
+}
+
+int tdz_pc_setup(tdz_ctx *c, int kind)
+{
+    return guarded([&] { pc_setup(c, kind); });
+}
+
+int tdz_pc_apply(tdz_ctx *c, const double *r, double *z)
+{
+    return guarded([&] {
+        if (!c->pc_ready) throw Error("preconditioner not set up");
+        pc_apply_dev(c, reinterpret_cast<const dcomplex *>(r), reinterpret_cast<dcomplex *>(z));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int tdz_ksp_solve(tdz_ctx *c, const double *b, double *x, double rtol, int maxit, tdz_ksp_stats *stats)
+{
+    return guarded([&] {
+        if (!c->pc_ready) pc_setup(c, c->pc_kind);
+        fgmres(c, reinterpret_cast<const dcomplex *>(b), reinterpret_cast<dcomplex *>(x), rtol, maxit, stats);
+    });
+}
+
+int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], double rtol, int maxit,
+                tdz_ksp_stats *stats)
+{
+    return guarded([&] {
+        if (!c->pc_ready) pc_setup(c, c->pc_kind);
+        dcomplex *x = reinterpret_cast<dcomplex *>(psi);
+        Level &L = *c->lv[0];
+        const cplx g(0.0, 0.5 * dt);
+        set_shift(c, 1.0, -g, sE_mid);        // rhs = (M - i dt/2 H_E) psi
+        level_apply(c, L, x, c->rhs);
+        // (M + i dt/2 H_E) psi = 2 M psi - rhs, so the initial residual of the guess psi
+        // is r0 = 2 (rhs - M psi): one mass apply instead of a full operator apply
+        // (the same identity as the stage right-hand sides of src/propagator.cpp:589-624)
+        tdz::SepBands mb = sep_bands(L);
+        for (int d = 0; d < 3; ++d) mb.B[d] = L.dBmass[d];
+        tdz::launch_sep_apply(c->st, mb, x, c->w1, c->tmp);
+        if (c->prob.dirichlet) {
+            const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+            tdz::launch_wall_fix(c->st, L.n, w, x, c->w1);
+        }
+        HIPCHK(hipMemcpyAsync(c->Vb, c->rhs, L.N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+        tdz::launch_axpby(c->st, {-2.0, 0}, c->w1, {2.0, 0}, c->Vb, L.N);
+        set_shift(c, 1.0, g, sE_mid);         // solve (M + i dt/2 H_E) psi' = rhs, guess psi
+        fgmres(c, c->rhs, x, rtol, maxit, stats, true);
+    });
+}
+
+int tdz_pulse_coupling(const tdz_ctx *c, const tdz_pulse *pulse, double t, double sE[3])
+{
+    if (!c || !pulse || !sE) { g_err = "null argument"; return 1; }
+    for (int a = 0; a < 3; ++a) sE[a] = a < c->prob.dim ? c->ph.s * pulse_field(*pulse, a, t) : 0.0;
+    return 0;
+}
+
+int tdz_observables(tdz_ctx *c, const double *psi, double out[6])
+{
+    return guarded([&] {
+        Level &L = *c->lv[0];
+        const long N = L.N;
+        std::vector<dcomplex> hp(N), hv(N);
+        HIPCHK(hipMemcpyAsync(hp.data(), psi, N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        auto dot = [&](int term_or_h) {
+            // term_or_h < 0: full H = K + sV + CAP
+            if (term_or_h >= 0) {
+                if (tdz_term_apply(c, term_or_h, psi, reinterpret_cast<double *>(c->w1))) throw Error(g_err);
+            } else {
+                const cplx a0 = c->alpha, g0 = c->g;
+                double s0[3] = {c->sE[0], c->sE[1], c->sE[2]};
+                const double zero[3] = {0, 0, 0};
+                set_shift(c, 0.0, 1.0, zero);
+                level_apply(c, L, reinterpret_cast<const dcomplex *>(psi), c->w1);
+                set_shift(c, a0, g0, s0);
+            }
+            HIPCHK(hipMemcpyAsync(hv.data(), c->w1, N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            cplx s = 0;
+            for (long i = 0; i < N; ++i) s += std::conj(cplx(hp[i].re, hp[i].im)) * cplx(hv[i].re, hv[i].im);
+            return s;
+        };
+        out[0] = dot(TDZ_TERM_M).real();
+        const cplx e = dot(-1);
+        out[1] = e.real();
+        out[2] = e.imag();
+        for (int a = 0; a < 3; ++a) out[3 + a] = a < c->prob.dim ? dot(TDZ_TERM_DX + a).real() : 0.0;
+    });
+}
+
+int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, const tdz_pulse *pulse, double rtol,
+                  int maxit, double *obs, int *its)
+{
+    return guarded([&] {
+        if (!c->pc_ready) pc_setup(c, c->pc_kind);
+        for (int s = 0; s < nsteps; ++s) {
+            const double tm = t0 + (s + 0.5) * dt;
+            double sE[3] = {0, 0, 0};
+            if (pulse) tdz_pulse_coupling(c, pulse, tm, sE);
+            tdz_ksp_stats st{};
+            if (tdz_step_cn(c, psi, dt, sE, rtol, maxit, &st)) throw Error(g_err);
+            // refresh rule (include/tdsez_propagator.hpp:17-31): the preconditioner is rebuilt
+            // from the current operator when a solve needs > 2 base + 5 iterations
+            if (c->refresh_base < 0) c->refresh_base = st.its;
+            else if (st.its > 2 * c->refresh_base + 5) {
+                c->refresh_base = -1;
+                ++c->rebuilds;
+                for (auto &L : c->lv) L->fd_ready = false;
+                pc_setup(c, c->pc_kind);
+            }
+            if (its) its[s] = st.its;
+            if (obs) {
+                double o[6];
+                if (tdz_observables(c, psi, o)) throw Error(g_err);
+                obs[7 * s] = t0 + (s + 1) * dt;
+                for (int k = 0; k < 6; ++k) obs[7 * s + 1 + k] = o[k];
+            }
+        }
+    });
+}
+
+int tdz_timing(tdz_ctx *c, int enable)
+{
+    return guarded([&] {
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->timing = enable != 0;
+        c->ev_used = 0;
+    });
+}
+
+int tdz_timing_stats(tdz_ctx *c, double *total_ms, long *launches)
+{
+    return guarded([&] {
+        HIPCHK(hipStreamSynchronize(c->st));
+        double t = 0;
+        for (size_t i = 0; i < c->ev_used; ++i) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[i].first, c->ev[i].second));
+            t += ms;
+        }
+        *total_ms = t;
+        *launches = (long)c->ev_used;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// Internal interfaces of the MI355X-native TDSE-Z propagation path: host set-up
+// (basis, bands, hierarchy) and the launchers of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace tdz {
+
+// ------------------------------------------------------------ host set-up
+// One axis of one multigrid level: knots, Gauss points, basis tables and the
+// 1D Gram bands (entry (i, j) at i*(2p+1) + j-i+p, as src/tensorops.cpp:27-61).
+struct Axis1D {
+    int E = 1, p = 0, Q = 1, n = 1;
+    bool active = false, wall = false;
+    double lo = 0, hi = 0;
+    std::vector<double> t;               // knots
+    std::vector<double> X, W, cap;       // [E*Q] points, weights, CAP profile
+    std::vector<int> off;                // [E] first function of each element
+    std::vector<double> B0, B1;          // [(e*(p+1)+a)*Q+q] values, first derivatives
+    std::vector<double> M, S, Xb, Wc;    // bands: mass, stiffness, position, CAP profile
+    std::vector<double> Vs;              // band of the separable part of s V
+    std::vector<char> isWall;            // [n]
+};
+
+struct PhysParams {
+    double mass = 1, s = 1;              // carrier mass and sign s = -charge
+    int pot_kind = 0;
+    double pot[4] = {0, 0, 0, 0};
+    int cap_kind = 0;
+    double cap[4] = {0, 0, 0, 0};
+};
+
+// Gauss-Legendre nodes/weights on [-1, 1] (Golub-Welsch).
+void gauss_legendre(int Q, double *x, double *w);
+// Build one axis from a knot vector (degree p, Q points per non-empty span).
+void build_axis(Axis1D &A, const std::vector<double> &knots, int p, int Q, bool wall, const PhysParams &ph);
+// Degenerate axis of a lower-dimensional problem: one function equal to 1.
+void build_degenerate_axis(Axis1D &A);
+// Open uniform knot vector, continuity p-1.
+std::vector<double> uniform_knots(int E, double lo, double hi, int p);
+// Drop every other simple interior breakpoint (src/multigrid.cpp:25-40); false if none dropped.
+bool coarsen_knots(const std::vector<double> &U, std::vector<double> &C);
+int knot_spans(const std::vector<double> &U);
+// Boehm knot-insertion prolongation rows (coarse index, weight), fine x coarse.
+std::vector<std::vector<std::pair<int, double>>> boehm_prolongation(const std::vector<double> &C,
+                                                                    const std::vector<double> &F, int p);
+std::vector<double> greville(const std::vector<double> &U, int p);
+// Symmetric eigen-decomposition (Householder tridiagonalisation + implicit QL):
+// A (n x n row-major) is replaced by the eigenvectors as columns.
+void sym_eig(int n, std::vector<double> &A, std::vector<double> &lam);
+// Generalised T v = lambda M v with M SPD (bands of axis A, restricted to non-wall
+// functions): V (n x n, columns = M-orthonormal eigenvectors, zero at walls), lam.
+void gen_eig_bands(const Axis1D &A, const std::vector<double> &Tband, std::vector<double> &V,
+                   std::vector<double> &lam);
+
+// ------------------------------------------------------------ device side
+struct alignas(16) dcomplex { double re, im; };
+
+// Device copy of the per-axis tables the kernels read.
+struct AxisDev {
+    int E, p, Q, n;
+    const double *B0;   // [E][p+1][Q]
+    const double *X;    // [E*Q]
+    const double *W;    // [E*Q]
+    const int *off;     // [E]
+};
+
+struct PotParams {
+    int kind;
+    double s;           // carrier sign
+    double a0, a1;      // harmonic: k; soft-core: Z, a
+};
+
+// y += g * (s V) x over all elements (generic sum-factorised kernel).
+void launch_pot_apply_generic(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
+                              const dcomplex *x, dcomplex *y);
+// Same for dim 3 with simple interior knots (tiled z-marching kernel); returns
+// false when no instantiation covers (p, Q).
+bool launch_pot_apply_3d(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
+                         const dcomplex *x, dcomplex *y);
+// Separable Kronecker apply y = Mz My Bx x + Mz By Mx x + Bz My Mx x, bands on device:
+// M_d real [n_d*(2p+1)], B_d complex [n_d*(2p+1)]; tmp = 4 scratch vectors.
+struct SepBands {
+    int n[3], p[3];
+    const double *M[3];
+    const dcomplex *B[3];
+};
+void launch_sep_apply(hipStream_t st, const SepBands &sb, const dcomplex *x, dcomplex *y, dcomplex *tmp[4]);
+// y[I] = x[I] where the wall mask flags I (1D flags per axis on device).
+void launch_wall_fix(hipStream_t st, const int n[3], const unsigned char *wall[3], const dcomplex *x, dcomplex *y);
+
+// Deterministic two-stage reductions. partial must hold kMaxBlocks*(nv) entries.
+constexpr int kRedBlocks = 512;
+// h[i] = <V_i, w>, i < nv (V_i = V + i*ld), result complex on device out[nv]
+void launch_multi_dot(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *w, long N,
+                      dcomplex *partial, dcomplex *out);
+// w -= sum_i h_i V_i (h on device), and out_norm2 = ||w||^2 (on device, deterministic)
+void launch_multi_axpy_norm(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *h, dcomplex *w,
+                            long N, double *partial, double *out_norm2);
+// x += sum_i c_i Z_i (c on device)
+void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N);
+// y = a x + b y (scalars on host)
+void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dcomplex *y, long N);
+// y = x * s where s = 1/sqrt(*norm2) read on device
+void launch_scale_invsqrt(hipStream_t st, const dcomplex *x, const double *norm2, dcomplex *y, long N);
+// out = ||x||^2 on device
+void launch_norm2(hipStream_t st, const dcomplex *x, long N, double *partial, double *out);
+
+// Dense axis transform y = T x along axis d (T real n x n, row-major; trans: T^T).
+void launch_dense_axis(hipStream_t st, int d, const int n[3], const double *T, bool trans, const dcomplex *x,
+                       dcomplex *y);
+// y[i,j,k] = x[i,j,k] / (alpha + g (lx[i] + ly[j] + lz[k])), walls: y = x
+void launch_diag_scale(hipStream_t st, const int n[3], const double *lam[3], const unsigned char *wall[3],
+                       dcomplex alpha, dcomplex g, const dcomplex *x, dcomplex *y);
+
+// Sparse 1D transfer along axis d: out (axis d of size m) = A in (axis d of size nin),
+// A in CSR (rowptr[m+1], col, val); other axes unchanged. accumulate: out += A in.
+void launch_csr_axis(hipStream_t st, int d, const int nin[3], int m, const int *rowptr, const int *col,
+                     const double *val, const dcomplex *in, dcomplex *out, bool accumulate);
+
+}  // namespace tdz

@@ -1,0 +1,781 @@
+// HIP kernels of the TDSE-Z propagation path for gfx950 (CDNA4, wave64).
+//
+//   pot_apply_generic : matrix-free potential apply y += g (s V) x, one element
+//                       per workgroup, sum factorisation in LDS: the forward
+//                       evaluation of src/quadobs.cpp:181-238 at the Gauss points,
+//                       times g s V(r) w (src/quadobs.cpp:139), and its transpose.
+//   sep_*             : separable Kronecker terms by three 1D band passes
+//                       (src/tensorops.cpp:97-153), complex bands.
+//   multi_dot, ...    : Krylov vector kernels with deterministic two-stage
+//                       reductions (fixed block count, fixed summation order).
+//   dense_axis, diag  : fast-diagonalisation solve of the separable part.
+//   csr_axis          : Boehm prolongation / restriction along one axis.
+#include <hip/hip_runtime.h>
+
+#include "tdz_internal.hpp"
+
+namespace tdz {
+
+namespace {
+
+__device__ __forceinline__ double pot_value(const PotParams &pp, double r2)
+{
+    if (pp.kind == 1) return 0.5 * pp.a0 * r2;
+    if (pp.kind == 2) return -pp.a0 / sqrt(r2 + pp.a1);
+    return 0.0;
+}
+
+// same values through the FP64 reciprocal square root (no division)
+__device__ __forceinline__ double pot_value_rsq(const PotParams &pp, double r2)
+{
+    if (pp.kind == 1) return 0.5 * pp.a0 * r2;
+    if (pp.kind == 2) return -pp.a0 * rsqrt(r2 + pp.a1);
+    return 0.0;
+}
+
+// ------------------------------------------------------------------ potential
+// One element per workgroup. LDS layout (doubles): tables Bx|By|Bz, then the
+// complex stages c[n3][n2][n1], t1[n3][n2][Q1], t2[n3][Q2][Q1], F[Q3][Q2][Q1].
+__global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, AxisDev a1, AxisDev a2, PotParams pp,
+                                                                double gr, double gi, const dcomplex *__restrict__ x,
+                                                                dcomplex *__restrict__ y)
+{
+    extern __shared__ double smem[];
+    const int n1 = a0.p + 1, n2 = a1.p + 1, n3 = a2.p + 1;
+    const int Q1 = a0.Q, Q2 = a1.Q, Q3 = a2.Q;
+    const long eid = blockIdx.x;
+    const int e1 = (int)(eid % a0.E);
+    const int e2 = (int)((eid / a0.E) % a1.E);
+    const int e3 = (int)(eid / ((long)a0.E * a1.E));
+    const int o1 = a0.off[e1], o2 = a1.off[e2], o3 = a2.off[e3];
+    const long g0 = a0.n, g1 = a1.n;
+
+    double *Bx = smem, *By = Bx + n1 * Q1, *Bz = By + n2 * Q2;
+    double *Xq = Bz + n3 * Q3, *Yq = Xq + Q1, *Zq = Yq + Q2;  // point coordinates
+    double *Wx = Zq + Q3, *Wy = Wx + Q1, *Wz = Wy + Q2;        // weights
+    size_t off = (size_t)(n1 * Q1 + n2 * Q2 + n3 * Q3 + 2 * (Q1 + Q2 + Q3));
+    off = (off + 1) & ~(size_t)1;  // 16-byte alignment for the complex arrays
+    dcomplex *c = reinterpret_cast<dcomplex *>(smem + off);
+    dcomplex *t1 = c + n1 * n2 * n3;
+    dcomplex *t2 = t1 + n3 * n2 * Q1;
+    dcomplex *F = t2 + n3 * Q2 * Q1;
+
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < n1 * Q1; i += nt) Bx[i] = a0.B0[(size_t)e1 * n1 * Q1 + i];
+    for (int i = tid; i < n2 * Q2; i += nt) By[i] = a1.B0[(size_t)e2 * n2 * Q2 + i];
+    for (int i = tid; i < n3 * Q3; i += nt) Bz[i] = a2.B0[(size_t)e3 * n3 * Q3 + i];
+    for (int i = tid; i < Q1; i += nt) { Xq[i] = a0.X[(size_t)e1 * Q1 + i]; Wx[i] = a0.W[(size_t)e1 * Q1 + i]; }
+    for (int i = tid; i < Q2; i += nt) { Yq[i] = a1.X[(size_t)e2 * Q2 + i]; Wy[i] = a1.W[(size_t)e2 * Q2 + i]; }
+    for (int i = tid; i < Q3; i += nt) { Zq[i] = a2.X[(size_t)e3 * Q3 + i]; Wz[i] = a2.W[(size_t)e3 * Q3 + i]; }
+    for (int i = tid; i < n1 * n2 * n3; i += nt) {
+        const int b1 = i % n1, b2 = (i / n1) % n2, b3 = i / (n1 * n2);
+        c[i] = x[(o1 + b1) + g0 * ((o2 + b2) + g1 * (long)(o3 + b3))];
+    }
+    __syncthreads();
+    // stage x: t1[(b3 n2 + b2) Q1 + q1]
+    for (int i = tid; i < n3 * n2 * Q1; i += nt) {
+        const int q1 = i % Q1, r = i / Q1;
+        double sr = 0, si = 0;
+        for (int b1 = 0; b1 < n1; ++b1) {
+            const double bv = Bx[b1 * Q1 + q1];
+            const dcomplex v = c[r * n1 + b1];
+            sr += bv * v.re; si += bv * v.im;
+        }
+        t1[i] = {sr, si};
+    }
+    __syncthreads();
+    // stage y: t2[(b3 Q2 + q2) Q1 + q1]
+    for (int i = tid; i < n3 * Q2 * Q1; i += nt) {
+        const int q1 = i % Q1, q2 = (i / Q1) % Q2, b3 = i / (Q1 * Q2);
+        double sr = 0, si = 0;
+        for (int b2 = 0; b2 < n2; ++b2) {
+            const double bv = By[b2 * Q2 + q2];
+            const dcomplex v = t1[(b3 * n2 + b2) * Q1 + q1];
+            sr += bv * v.re; si += bv * v.im;
+        }
+        t2[i] = {sr, si};
+    }
+    __syncthreads();
+    // stage z and the point coefficient g s V(r) w
+    for (int i = tid; i < Q3 * Q2 * Q1; i += nt) {
+        const int q1 = i % Q1, q2 = (i / Q1) % Q2, q3 = i / (Q1 * Q2);
+        double sr = 0, si = 0;
+        for (int b3 = 0; b3 < n3; ++b3) {
+            const double bv = Bz[b3 * Q3 + q3];
+            const dcomplex v = t2[(b3 * Q2 + q2) * Q1 + q1];
+            sr += bv * v.re; si += bv * v.im;
+        }
+        const double r2 = Xq[q1] * Xq[q1] + Yq[q2] * Yq[q2] + Zq[q3] * Zq[q3];
+        const double cv = pp.s * pot_value(pp, r2) * (Wx[q1] * Wy[q2] * Wz[q3]);
+        const double cr = gr * cv, ci = gi * cv;
+        F[i] = {cr * sr - ci * si, cr * si + ci * sr};
+    }
+    __syncthreads();
+    // transposed z: t2[(b3 Q2 + q2) Q1 + q1] = sum_q3 Bz[b3,q3] F[(q3 Q2 + q2) Q1 + q1]
+    for (int i = tid; i < n3 * Q2 * Q1; i += nt) {
+        const int q21 = i % (Q2 * Q1), b3 = i / (Q2 * Q1);
+        double sr = 0, si = 0;
+        for (int q3 = 0; q3 < Q3; ++q3) {
+            const double bv = Bz[b3 * Q3 + q3];
+            const dcomplex v = F[q3 * Q2 * Q1 + q21];
+            sr += bv * v.re; si += bv * v.im;
+        }
+        t2[i] = {sr, si};
+    }
+    __syncthreads();
+    // transposed y: t1[(b3 n2 + b2) Q1 + q1] = sum_q2 By[b2,q2] t2[(b3 Q2 + q2) Q1 + q1]
+    for (int i = tid; i < n3 * n2 * Q1; i += nt) {
+        const int q1 = i % Q1, b2 = (i / Q1) % n2, b3 = i / (Q1 * n2);
+        double sr = 0, si = 0;
+        for (int q2 = 0; q2 < Q2; ++q2) {
+            const double bv = By[b2 * Q2 + q2];
+            const dcomplex v = t2[(b3 * Q2 + q2) * Q1 + q1];
+            sr += bv * v.re; si += bv * v.im;
+        }
+        t1[i] = {sr, si};
+    }
+    __syncthreads();
+    // transposed x and scatter-add
+    for (int i = tid; i < n1 * n2 * n3; i += nt) {
+        const int b1 = i % n1, r = i / n1;
+        const int b2 = r % n2, b3 = r / n2;
+        double sr = 0, si = 0;
+        for (int q1 = 0; q1 < Q1; ++q1) {
+            const double bv = Bx[b1 * Q1 + q1];
+            const dcomplex v = t1[r * Q1 + q1];
+            sr += bv * v.re; si += bv * v.im;
+        }
+        double *dst = reinterpret_cast<double *>(y + ((o1 + b1) + g0 * ((o2 + b2) + g1 * (long)(o3 + b3))));
+        unsafeAtomicAdd(dst, sr);
+        unsafeAtomicAdd(dst + 1, si);
+    }
+}
+
+// ------------------------------------------------------ potential, 3D marching
+// One workgroup owns a TE x TE tile of elements in (x, y) and marches through
+// all element layers in z. The (x, y) stages of the sum factorisation are done
+// once per coefficient plane (not once per element), each thread keeps the z
+// lines of CPT fixed (qx, qy) point columns in registers: a rolling window of
+// the P+1 input planes an element needs and of the P+1 output planes it
+// touches. Per element layer:
+//   A  load plane ez+P, x stage -> T1 (LDS), y stage -> register window
+//   B  z stage at the Q points of element ez, times s V(r) w, transposed z
+//      stage into the output window
+//   C  output plane ez is complete: transposed y then x stages through LDS,
+//      scaled by g, atomically added to y (neighbouring tiles share p planes
+//      of functions in x and y).
+// Requires simple interior knots (element e starts at function e) and dim 3.
+template <int P, int Q, int TE, int CPT>
+__global__ void __launch_bounds__((TE * Q) * (TE * Q) / CPT)
+pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double gi,
+             const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
+{
+    constexpr int NP = P + 1, TQ = TE * Q, NCOL = TQ * TQ, NT = NCOL / CPT, TW = TE + P;
+    __shared__ dcomplex cin[TW * TW];   // input plane  [jy][jx]
+    __shared__ dcomplex T1[TW * TQ];    // x stage      [jy][qx]
+    __shared__ dcomplex G2p[NCOL];      // output plane at the points [qy][qx]
+    __shared__ dcomplex H1[TW * TQ];    // transposed y [jy][qx]
+    __shared__ double Bxs[TE * NP * Q], Bys[TE * NP * Q], Bzs[NP * Q], z2s[Q], wzs[Q];
+
+    const int tid = threadIdx.x;
+    const int ex0 = blockIdx.x * TE, ey0 = blockIdx.y * TE;
+    const int n0 = ax.n, n1 = ay.n, Ez = az.E;
+    const long plane = (long)n0 * n1;
+
+    for (int i = tid; i < TE * NP * Q; i += NT) {
+        const int el = i / (NP * Q), r = i % (NP * Q);
+        Bxs[i] = ex0 + el < ax.E ? ax.B0[(long)(ex0 + el) * NP * Q + r] : 0.0;
+        Bys[i] = ey0 + el < ay.E ? ay.B0[(long)(ey0 + el) * NP * Q + r] : 0.0;
+    }
+    // per column constants
+    double xy2[CPT], wxy[CPT], By[CPT][NP];
+    bool valid[CPT];
+    int cqx[CPT], cey[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int col = tid + c * NT;
+        const int qx = col % TQ, qy = col / TQ;
+        const int exl = qx / Q, q1 = qx % Q, eyl = qy / Q, q2 = qy % Q;
+        const int ex = ex0 + exl, ey = ey0 + eyl;
+        valid[c] = ex < ax.E && ey < ay.E;
+        cqx[c] = qx;
+        cey[c] = eyl;
+        const int exs = valid[c] ? ex : 0, eys = valid[c] ? ey : 0;
+        const double xv = ax.X[(long)exs * Q + q1], yv = ay.X[(long)eys * Q + q2];
+        xy2[c] = xv * xv + yv * yv;
+        wxy[c] = ax.W[(long)exs * Q + q1] * ay.W[(long)eys * Q + q2];
+#pragma unroll
+        for (int a = 0; a < NP; ++a) By[c][a] = ay.B0[((long)eys * NP + a) * Q + q2];
+    }
+    double tw[CPT][NP][2];   // input window: planes ez .. ez+P at my columns
+    double gw[CPT][NP][2];   // output window
+#pragma unroll
+    for (int c = 0; c < CPT; ++c)
+#pragma unroll
+        for (int a = 0; a < NP; ++a) tw[c][a][0] = tw[c][a][1] = gw[c][a][0] = gw[c][a][1] = 0.0;
+
+    // A: plane jz -> window slot `slot` (also stages the z tables of element ez when ez >= 0)
+    auto stage_in = [&](int jz, int slot, int ez) {
+        for (int i = tid; i < TW * TW; i += NT) {
+            const int jx = ex0 + i % TW, jy = ey0 + i / TW;
+            cin[i] = (jx < n0 && jy < n1) ? x[jx + (long)n0 * jy + plane * jz] : dcomplex{0.0, 0.0};
+        }
+        if (ez >= 0) {
+            for (int i = tid; i < NP * Q; i += NT) Bzs[i] = az.B0[(long)ez * NP * Q + i];
+            for (int i = tid; i < Q; i += NT) {
+                const double zv = az.X[(long)ez * Q + i];
+                z2s[i] = zv * zv;
+                wzs[i] = az.W[(long)ez * Q + i];
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < TW * TQ; i += NT) {
+            const int qx = i % TQ, jy = i / TQ, exl = qx / Q, q1 = qx % Q;
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int a = 0; a < NP; ++a) {
+                const double b = Bxs[(exl * NP + a) * Q + q1];
+                const dcomplex v = cin[jy * TW + exl + a];
+                sr += b * v.re;
+                si += b * v.im;
+            }
+            T1[i] = {sr, si};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int a = 0; a < NP; ++a) {
+                const dcomplex v = T1[(cey[c] + a) * TQ + cqx[c]];
+                sr += By[c][a] * v.re;
+                si += By[c][a] * v.im;
+            }
+            tw[c][slot][0] = sr;
+            tw[c][slot][1] = si;
+        }
+    };
+
+    // C: emit output plane jz from gw[.][0], then shift the output window
+    auto emit = [&](int jz) {
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const double r = gw[c][0][0], i = gw[c][0][1];
+            G2p[tid + c * NT] = {gr * r - gi * i, gr * i + gi * r};
+#pragma unroll
+            for (int a = 0; a < P; ++a) { gw[c][a][0] = gw[c][a + 1][0]; gw[c][a][1] = gw[c][a + 1][1]; }
+            gw[c][P][0] = gw[c][P][1] = 0.0;
+        }
+        __syncthreads();
+        for (int i = tid; i < TW * TQ; i += NT) {
+            const int qx = i % TQ, jy = i / TQ;
+            double sr = 0.0, si = 0.0;
+#pragma unroll 1
+            for (int eyl = 0; eyl < TE; ++eyl) {
+                const int a = jy - eyl;
+                if (a < 0 || a > P) continue;
+#pragma unroll
+                for (int q2 = 0; q2 < Q; ++q2) {
+                    const double b = Bys[(eyl * NP + a) * Q + q2];
+                    const dcomplex v = G2p[(eyl * Q + q2) * TQ + qx];
+                    sr += b * v.re;
+                    si += b * v.im;
+                }
+            }
+            H1[i] = {sr, si};
+        }
+        __syncthreads();
+        for (int i = tid; i < TW * TW; i += NT) {
+            const int jxl = i % TW, jyl = i / TW;
+            const int gx = ex0 + jxl, gy = ey0 + jyl;
+            if (gx >= n0 || gy >= n1) continue;
+            double sr = 0.0, si = 0.0;
+#pragma unroll 1
+            for (int exl = 0; exl < TE; ++exl) {
+                const int a = jxl - exl;
+                if (a < 0 || a > P) continue;
+#pragma unroll
+                for (int q1 = 0; q1 < Q; ++q1) {
+                    const double b = Bxs[(exl * NP + a) * Q + q1];
+                    const dcomplex v = H1[jyl * TQ + exl * Q + q1];
+                    sr += b * v.re;
+                    si += b * v.im;
+                }
+            }
+            double *dst = reinterpret_cast<double *>(y + gx + (long)n0 * gy + plane * jz);
+            unsafeAtomicAdd(dst, sr);
+            unsafeAtomicAdd(dst + 1, si);
+        }
+        __syncthreads();
+    };
+
+    for (int s = 0; s < P; ++s) stage_in(s, s, -1);
+    for (int ez = 0; ez < Ez; ++ez) {
+        stage_in(ez + P, P, ez);
+        // B: z stage, coefficient, transposed z stage
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            if (!valid[c]) continue;
+#pragma unroll 2
+            for (int qz = 0; qz < Q; ++qz) {
+                double fr = 0.0, fi = 0.0;
+#pragma unroll
+                for (int a = 0; a < NP; ++a) {
+                    const double b = Bzs[a * Q + qz];
+                    fr += b * tw[c][a][0];
+                    fi += b * tw[c][a][1];
+                }
+                const double v = pp.s * pot_value_rsq(pp, xy2[c] + z2s[qz]) * (wxy[c] * wzs[qz]);
+                fr *= v;
+                fi *= v;
+#pragma unroll
+                for (int a = 0; a < NP; ++a) {
+                    const double b = Bzs[a * Q + qz];
+                    gw[c][a][0] += b * fr;
+                    gw[c][a][1] += b * fi;
+                }
+            }
+        }
+        // shift the input window
+#pragma unroll
+        for (int c = 0; c < CPT; ++c)
+#pragma unroll
+            for (int a = 0; a < P; ++a) { tw[c][a][0] = tw[c][a + 1][0]; tw[c][a][1] = tw[c][a + 1][1]; }
+        emit(ez);
+    }
+    for (int k = 1; k <= P; ++k) emit(Ez - 1 + k);
+}
+
+// ------------------------------------------------------------------ separable
+// x pass: t = Mx x, u = Bx x (one thread per output)
+__global__ void sep_x_kernel(int n0, long lines, int p, const double *__restrict__ M, const dcomplex *__restrict__ B,
+                             const dcomplex *__restrict__ x, dcomplex *__restrict__ t, dcomplex *__restrict__ u)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= lines * n0) return;
+    const int i = (int)(I % n0);
+    const long base = I - i;
+    const int bw = 2 * p + 1;
+    const int lo = i - p < 0 ? 0 : i - p, hi = i + p > n0 - 1 ? n0 - 1 : i + p;
+    double tr = 0, ti = 0, ur = 0, ui = 0;
+    for (int J = lo; J <= hi; ++J) {
+        const dcomplex v = x[base + J];
+        const double m = M[i * bw + (J - i + p)];
+        const dcomplex b = B[i * bw + (J - i + p)];
+        tr += m * v.re; ti += m * v.im;
+        ur += b.re * v.re - b.im * v.im;
+        ui += b.re * v.im + b.im * v.re;
+    }
+    t[I] = {tr, ti};
+    u[I] = {ur, ui};
+}
+
+// y pass: a = My u + By t, b = My t
+__global__ void sep_y_kernel(int n0, int n1, long N, int p, const double *__restrict__ M,
+                             const dcomplex *__restrict__ B, const dcomplex *__restrict__ t,
+                             const dcomplex *__restrict__ u, dcomplex *__restrict__ a, dcomplex *__restrict__ b)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const int i = (int)(I % n0);
+    const int j = (int)((I / n0) % n1);
+    const long k = I / ((long)n0 * n1);
+    const int bw = 2 * p + 1;
+    const int lo = j - p < 0 ? 0 : j - p, hi = j + p > n1 - 1 ? n1 - 1 : j + p;
+    double ar = 0, ai = 0, br = 0, bi = 0;
+    for (int J = lo; J <= hi; ++J) {
+        const long src = i + (long)n0 * (J + (long)n1 * k);
+        const dcomplex tv = t[src], uv = u[src];
+        const double m = M[j * bw + (J - j + p)];
+        const dcomplex bb = B[j * bw + (J - j + p)];
+        ar += m * uv.re + bb.re * tv.re - bb.im * tv.im;
+        ai += m * uv.im + bb.re * tv.im + bb.im * tv.re;
+        br += m * tv.re; bi += m * tv.im;
+    }
+    a[I] = {ar, ai};
+    b[I] = {br, bi};
+}
+
+// z pass: y = Mz a + Bz b
+__global__ void sep_z_kernel(long plane, int n2, int p, const double *__restrict__ M, const dcomplex *__restrict__ B,
+                             const dcomplex *__restrict__ a, const dcomplex *__restrict__ b, dcomplex *__restrict__ y)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= plane * n2) return;
+    const long m = I % plane;
+    const int k = (int)(I / plane);
+    const int bw = 2 * p + 1;
+    const int lo = k - p < 0 ? 0 : k - p, hi = k + p > n2 - 1 ? n2 - 1 : k + p;
+    double yr = 0, yi = 0;
+    for (int K = lo; K <= hi; ++K) {
+        const dcomplex av = a[m + plane * K], bv = b[m + plane * K];
+        const double mm = M[k * bw + (K - k + p)];
+        const dcomplex bb = B[k * bw + (K - k + p)];
+        yr += mm * av.re + bb.re * bv.re - bb.im * bv.im;
+        yi += mm * av.im + bb.re * bv.im + bb.im * bv.re;
+    }
+    y[I] = {yr, yi};
+}
+
+__global__ void wall_fix_kernel(int n0, int n1, long N, const unsigned char *w0, const unsigned char *w1,
+                                const unsigned char *w2, const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const int i = (int)(I % n0), j = (int)((I / n0) % n1);
+    const long k = I / ((long)n0 * n1);
+    if (w0[i] | w1[j] | w2[k]) y[I] = x[I];
+}
+
+// ------------------------------------------------------------- reductions
+constexpr int kRedThreads = 256;
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh)
+{
+    // wave64 butterfly then across waves; fixed order => deterministic
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+    __syncthreads();
+    return s;
+}
+
+// partial[(i * gridDim.x) + block] = sum over the block's chunk of conj(V_i) w, K vectors per pass
+template <int K>
+__global__ void __launch_bounds__(kRedThreads) multi_dot_kernel(const dcomplex *__restrict__ V, long ld, int i0,
+                                                                int nv, const dcomplex *__restrict__ w, long N,
+                                                                dcomplex *__restrict__ partial)
+{
+    __shared__ double sh[2 * K][kRedThreads / 64];
+    double acc[2 * K];
+#pragma unroll
+    for (int k = 0; k < 2 * K; ++k) acc[k] = 0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        const dcomplex wv = w[I];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (i0 + k < nv) {
+                const dcomplex v = V[(long)(i0 + k) * ld + I];
+                acc[2 * k] += v.re * wv.re + v.im * wv.im;
+                acc[2 * k + 1] += v.re * wv.im - v.im * wv.re;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * K; ++k) {
+        double v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0) sh[k][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        const int k = threadIdx.x;
+        if (i0 + k < nv) {
+            double sr = 0, si = 0;
+            for (int w2 = 0; w2 < kRedThreads / 64; ++w2) { sr += sh[2 * k][w2]; si += sh[2 * k + 1][w2]; }
+            partial[(long)(i0 + k) * gridDim.x + blockIdx.x] = {sr, si};
+        }
+    }
+}
+
+// out[i] = sum_b partial[i * nb + b], one block per i, fixed order
+__global__ void reduce_complex_kernel(const dcomplex *__restrict__ partial, int nb, dcomplex *__restrict__ out)
+{
+    __shared__ double sh[kRedThreads / 64];
+    const int i = blockIdx.x;
+    double sr = 0, si = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        sr += partial[(long)i * nb + b].re;
+        si += partial[(long)i * nb + b].im;
+    }
+    sr = block_sum(sr, sh);
+    si = block_sum(si, sh);
+    if (threadIdx.x == 0) out[i] = {sr, si};
+}
+
+__global__ void reduce_real_kernel(const double *__restrict__ partial, int nb, double *__restrict__ out)
+{
+    __shared__ double sh[kRedThreads / 64];
+    double s = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+__global__ void __launch_bounds__(kRedThreads) multi_axpy_norm_kernel(const dcomplex *__restrict__ V, long ld, int nv,
+                                                                      const dcomplex *__restrict__ h,
+                                                                      dcomplex *__restrict__ w, long N,
+                                                                      double *__restrict__ partial)
+{
+    __shared__ dcomplex hs[64];
+    __shared__ double sh[kRedThreads / 64];
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) hs[i] = h[i];
+    __syncthreads();
+    double acc = 0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        dcomplex v = w[I];
+        for (int i = 0; i < nv; ++i) {
+            const dcomplex b = V[(long)i * ld + I];
+            v.re -= hs[i].re * b.re - hs[i].im * b.im;
+            v.im -= hs[i].re * b.im + hs[i].im * b.re;
+        }
+        w[I] = v;
+        acc += v.re * v.re + v.im * v.im;
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ void multi_axpy_kernel(const dcomplex *__restrict__ Z, long ld, int nv, const dcomplex *__restrict__ c,
+                                  dcomplex *__restrict__ x, long N)
+{
+    __shared__ dcomplex cs[64];
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) cs[i] = c[i];
+    __syncthreads();
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        dcomplex v = x[I];
+        for (int i = 0; i < nv; ++i) {
+            const dcomplex b = Z[(long)i * ld + I];
+            v.re += cs[i].re * b.re - cs[i].im * b.im;
+            v.im += cs[i].re * b.im + cs[i].im * b.re;
+        }
+        x[I] = v;
+    }
+}
+
+__global__ void axpby_kernel(dcomplex a, const dcomplex *__restrict__ x, dcomplex b, dcomplex *__restrict__ y, long N)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        const dcomplex xv = x[I], yv = y[I];
+        y[I] = {a.re * xv.re - a.im * xv.im + b.re * yv.re - b.im * yv.im,
+                a.re * xv.im + a.im * xv.re + b.re * yv.im + b.im * yv.re};
+    }
+}
+
+__global__ void scale_invsqrt_kernel(const dcomplex *__restrict__ x, const double *__restrict__ n2,
+                                     dcomplex *__restrict__ y, long N)
+{
+    const double s = 1.0 / sqrt(n2[0]);
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        const dcomplex v = x[I];
+        y[I] = {v.re * s, v.im * s};
+    }
+}
+
+__global__ void __launch_bounds__(kRedThreads) norm2_kernel(const dcomplex *__restrict__ x, long N,
+                                                            double *__restrict__ partial)
+{
+    __shared__ double sh[kRedThreads / 64];
+    double acc = 0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        const dcomplex v = x[I];
+        acc += v.re * v.re + v.im * v.im;
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// --------------------------------------------------------------- dense axis
+// y[.., i, ..] = sum_k T(i, k) x[.., k, ..] along axis d; one thread per output.
+__global__ void dense_axis_kernel(int d, int n0, int n1, int n2, const double *__restrict__ T, int trans,
+                                  const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
+{
+    const long N = (long)n0 * n1 * n2;
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const int i0 = (int)(I % n0), i1 = (int)((I / n0) % n1);
+    const int i2 = (int)(I / ((long)n0 * n1));
+    int n, i;
+    long base, stride;
+    if (d == 0) { n = n0; i = i0; stride = 1; base = I - i0; }
+    else if (d == 1) { n = n1; i = i1; stride = n0; base = I - (long)i1 * n0; }
+    else { n = n2; i = i2; stride = (long)n0 * n1; base = I - (long)i2 * n0 * n1; }
+    double sr = 0, si = 0;
+    for (int k = 0; k < n; ++k) {
+        const double tv = trans ? T[(long)k * n + i] : T[(long)i * n + k];
+        const dcomplex v = x[base + k * stride];
+        sr += tv * v.re; si += tv * v.im;
+    }
+    y[I] = {sr, si};
+}
+
+__global__ void diag_scale_kernel(int n0, int n1, long N, const double *l0, const double *l1, const double *l2,
+                                  const unsigned char *w0, const unsigned char *w1, const unsigned char *w2,
+                                  dcomplex alpha, dcomplex g, const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const int i = (int)(I % n0), j = (int)((I / n0) % n1);
+    const long k = I / ((long)n0 * n1);
+    const dcomplex v = x[I];
+    if (w0[i] | w1[j] | w2[k]) { y[I] = v; return; }
+    const double lam = l0[i] + l1[j] + l2[k];
+    const double dr = alpha.re + g.re * lam, di = alpha.im + g.im * lam;
+    const double inv = 1.0 / (dr * dr + di * di);
+    y[I] = {(v.re * dr + v.im * di) * inv, (v.im * dr - v.re * di) * inv};
+}
+
+// --------------------------------------------------------------- sparse axis
+__global__ void csr_axis_kernel(int d, int a0, int a1, int a2, int m, const int *__restrict__ rowptr,
+                                const int *__restrict__ col, const double *__restrict__ val,
+                                const dcomplex *__restrict__ in, dcomplex *__restrict__ out, int accumulate)
+{
+    // output dims: axis d replaced by m
+    const int o0 = d == 0 ? m : a0, o1 = d == 1 ? m : a1, o2 = d == 2 ? m : a2;
+    const long N = (long)o0 * o1 * o2;
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const int i0 = (int)(I % o0), i1 = (int)((I / o0) % o1);
+    const int i2 = (int)(I / ((long)o0 * o1));
+    const int r = d == 0 ? i0 : (d == 1 ? i1 : i2);
+    double sr = 0, si = 0;
+    for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+        const int c = col[e];
+        const int j0 = d == 0 ? c : i0, j1 = d == 1 ? c : i1, j2 = d == 2 ? c : i2;
+        const dcomplex v = in[j0 + (long)a0 * (j1 + (long)a1 * j2)];
+        sr += val[e] * v.re; si += val[e] * v.im;
+    }
+    if (accumulate) { sr += out[I].re; si += out[I].im; }
+    out[I] = {sr, si};
+}
+
+inline unsigned grid_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+inline unsigned grid_capped(long n, int bs, unsigned cap)
+{
+    const unsigned g = grid_for(n, bs);
+    return g < cap ? g : cap;
+}
+
+}  // namespace
+
+// ===================================================================== launchers
+void launch_pot_apply_generic(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
+                              const dcomplex *x, dcomplex *y)
+{
+    const int n1 = ax[0].p + 1, n2 = ax[1].p + 1, n3 = ax[2].p + 1;
+    const int Q1 = ax[0].Q, Q2 = ax[1].Q, Q3 = ax[2].Q;
+    size_t dbl = (size_t)(n1 * Q1 + n2 * Q2 + n3 * Q3 + 2 * (Q1 + Q2 + Q3));
+    dbl = (dbl + 1) & ~(size_t)1;
+    const size_t cpx = (size_t)n1 * n2 * n3 + (size_t)n3 * n2 * Q1 + (size_t)n3 * Q2 * Q1 + (size_t)Q3 * Q2 * Q1;
+    const size_t bytes = dbl * sizeof(double) + cpx * sizeof(dcomplex);
+    const long nel = (long)ax[0].E * ax[1].E * ax[2].E;
+    hipLaunchKernelGGL(pot_apply_generic_kernel, dim3((unsigned)nel), dim3(128), bytes, st, ax[0], ax[1], ax[2], pot,
+                       gr, gi, x, y);
+}
+
+template <int P, int Q, int TE, int CPT>
+static void launch_pot3d(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
+                         const dcomplex *x, dcomplex *y)
+{
+    constexpr int NT = (TE * Q) * (TE * Q) / CPT;
+    const dim3 grid((unsigned)((ax[0].E + TE - 1) / TE), (unsigned)((ax[1].E + TE - 1) / TE));
+    hipLaunchKernelGGL((pot3d_kernel<P, Q, TE, CPT>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr, gi, x, y);
+}
+
+bool launch_pot_apply_3d(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
+                         const dcomplex *x, dcomplex *y)
+{
+    const int p = ax[0].p, Q = ax[0].Q;
+    for (int d = 1; d < 3; ++d)
+        if (ax[d].p != p || ax[d].Q != Q) return false;
+    if (p == 4 && Q == 8) { launch_pot3d<4, 8, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
+    if (p == 3 && Q == 8) { launch_pot3d<3, 8, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
+    if (p == 3 && Q == 5) { launch_pot3d<3, 5, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
+    return false;
+}
+
+void launch_sep_apply(hipStream_t st, const SepBands &sb, const dcomplex *x, dcomplex *y, dcomplex *tmp[4])
+{
+    const int n0 = sb.n[0], n1 = sb.n[1], n2 = sb.n[2];
+    const long N = (long)n0 * n1 * n2;
+    const int bs = 256;
+    hipLaunchKernelGGL(sep_x_kernel, dim3(grid_for(N, bs)), dim3(bs), 0, st, n0, N / n0, sb.p[0], sb.M[0], sb.B[0],
+                       x, tmp[0], tmp[1]);
+    hipLaunchKernelGGL(sep_y_kernel, dim3(grid_for(N, bs)), dim3(bs), 0, st, n0, n1, N, sb.p[1], sb.M[1], sb.B[1],
+                       tmp[0], tmp[1], tmp[2], tmp[3]);
+    hipLaunchKernelGGL(sep_z_kernel, dim3(grid_for(N, bs)), dim3(bs), 0, st, (long)n0 * n1, n2, sb.p[2], sb.M[2],
+                       sb.B[2], tmp[2], tmp[3], y);
+}
+
+void launch_wall_fix(hipStream_t st, const int n[3], const unsigned char *wall[3], const dcomplex *x, dcomplex *y)
+{
+    const long N = (long)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(wall_fix_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st, n[0], n[1], N, wall[0], wall[1],
+                       wall[2], x, y);
+}
+
+void launch_multi_dot(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *w, long N,
+                      dcomplex *partial, dcomplex *out)
+{
+    const unsigned nb = grid_capped(N, kRedThreads, kRedBlocks);
+    for (int i0 = 0; i0 < nv; i0 += 4)
+        hipLaunchKernelGGL(multi_dot_kernel<4>, dim3(nb), dim3(kRedThreads), 0, st, V, ld, i0, nv, w, N, partial);
+    hipLaunchKernelGGL(reduce_complex_kernel, dim3(nv), dim3(kRedThreads), 0, st, partial, (int)nb, out);
+}
+
+void launch_multi_axpy_norm(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *h, dcomplex *w,
+                            long N, double *partial, double *out_norm2)
+{
+    const unsigned nb = grid_capped(N, kRedThreads, kRedBlocks);
+    hipLaunchKernelGGL(multi_axpy_norm_kernel, dim3(nb), dim3(kRedThreads), 0, st, V, ld, nv, h, w, N, partial);
+    hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial, (int)nb, out_norm2);
+}
+
+void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N)
+{
+    hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, Z, ld, nv, c, x, N);
+}
+
+void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dcomplex *y, long N)
+{
+    hipLaunchKernelGGL(axpby_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, a, x, b, y, N);
+}
+
+void launch_scale_invsqrt(hipStream_t st, const dcomplex *x, const double *norm2, dcomplex *y, long N)
+{
+    hipLaunchKernelGGL(scale_invsqrt_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, x, norm2, y, N);
+}
+
+void launch_norm2(hipStream_t st, const dcomplex *x, long N, double *partial, double *out)
+{
+    const unsigned nb = grid_capped(N, kRedThreads, kRedBlocks);
+    hipLaunchKernelGGL(norm2_kernel, dim3(nb), dim3(kRedThreads), 0, st, x, N, partial);
+    hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial, (int)nb, out);
+}
+
+void launch_dense_axis(hipStream_t st, int d, const int n[3], const double *T, bool trans, const dcomplex *x,
+                       dcomplex *y)
+{
+    const long N = (long)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(dense_axis_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st, d, n[0], n[1], n[2], T,
+                       trans ? 1 : 0, x, y);
+}
+
+void launch_diag_scale(hipStream_t st, const int n[3], const double *lam[3], const unsigned char *wall[3],
+                       dcomplex alpha, dcomplex g, const dcomplex *x, dcomplex *y)
+{
+    const long N = (long)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(diag_scale_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st, n[0], n[1], N, lam[0], lam[1],
+                       lam[2], wall[0], wall[1], wall[2], alpha, g, x, y);
+}
+
+void launch_csr_axis(hipStream_t st, int d, const int nin[3], int m, const int *rowptr, const int *col,
+                     const double *val, const dcomplex *in, dcomplex *out, bool accumulate)
+{
+    const long N = (long)(d == 0 ? m : nin[0]) * (d == 1 ? m : nin[1]) * (d == 2 ? m : nin[2]);
+    hipLaunchKernelGGL(csr_axis_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st, d, nin[0], nin[1], nin[2], m,
+                       rowptr, col, val, in, out, accumulate ? 1 : 0);
+}
+
+}  // namespace tdz

@@ -1,0 +1,134 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Tolerances: floating-point (complex128). Operator terms agree to 1e-13
+relative (rounding only: both sides compute the same integrals with the same
+Gauss rule, in different orders); solves agree to the solver tolerance.
+"""
+import numpy as np
+import pytest
+
+from tests.problems import HO_1D, HOLE_3D, MANO_3D, SOFTCORE_2D, SOFTCORE_3D
+
+pytestmark = pytest.mark.gpu
+
+PROBLEMS = {"softcore3d": SOFTCORE_3D, "mano3d": MANO_3D, "softcore2d": SOFTCORE_2D, "ho1d": HO_1D,
+            "hole3d": HOLE_3D}
+
+
+def _rand(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module", params=list(PROBLEMS))
+def pair(request):
+    import tdsez_amd as T
+    from oracle import Oracle
+    desc = PROBLEMS[request.param]
+    o = Oracle(desc)
+    ctx = T.Context(desc)
+    assert ctx.n == o.n and ctx.N == o.N
+    yield request.param, desc, o, ctx
+    ctx.close()
+
+
+def _wall_mask(o, desc):
+    if not desc.get("dirichlet"):
+        return np.zeros(o.N, dtype=bool)
+    n0, n1, n2 = o.n
+    I = np.arange(o.N)
+    i, j, k = I % n0, (I // n0) % n1, I // (n0 * n1)
+    m = np.zeros(o.N, dtype=bool)
+    for d, (idx, n) in enumerate(((i, n0), (j, n1), (k, n2))):
+        if d < desc["dim"]:
+            m |= (idx == 0) | (idx == n - 1)
+    return m
+
+
+@pytest.mark.parametrize("term", range(7))
+def test_terms(pair, term):
+    import tdsez_amd as T
+    name, desc, o, ctx = pair
+    if term >= T.TERM_DX + desc["dim"]:
+        pytest.skip("axis not active")
+    x = _rand(o.N, 10 + term)
+    wall = _wall_mask(o, desc)
+    x[wall] = 0.0
+    ref = o.term(term, x)
+    dx, dy = ctx.vec(x), ctx.vec()
+    ctx.term(term, dx, dy)
+    got = dy.download()
+    got[wall] = 0.0
+    ref[wall] = 0.0
+    assert _rel(got, ref) < 1e-13, (name, term, _rel(got, ref))
+
+
+@pytest.mark.parametrize("alpha,g,sE", [(1.0, 0.01j, (0.0, 0.0, 0.03)), (0.3 - 0.2j, 0.7 + 0.1j, (0.02, -0.01, 0.05)),
+                                        (1.0, -0.01j, (0.0, 0.0, 0.0))])
+def test_full_operator(pair, alpha, g, sE):
+    name, desc, o, ctx = pair
+    sE = np.array(sE)
+    sE[desc["dim"]:] = 0.0
+    x = _rand(o.N, 3)
+    ref = o.apply(alpha, g, sE, x)
+    ctx.set_shift(alpha, g, sE)
+    dx, dy = ctx.vec(x), ctx.vec()
+    ctx.apply(dx, dy)
+    assert _rel(dy.download(), ref) < 1e-13, name
+
+
+def test_separable_preconditioner(pair):
+    import tdsez_amd as T
+    name, desc, o, ctx = pair
+    alpha, g = 1.0, 0.01j
+    r = _rand(o.N, 5)
+    ref = o.fd_apply(alpha, g, r)
+    ctx.set_shift(alpha, g, (0, 0, 0))
+    ctx.pc_setup(T.PC_SEPARABLE)
+    dr, dz = ctx.vec(r), ctx.vec()
+    ctx.pc_apply(dr, dz)
+    assert _rel(dz.download(), ref) < 1e-10, name
+
+
+@pytest.mark.parametrize("pc", [1, 2])
+def test_ksp_solve_true_residual(pair, pc):
+    import tdsez_amd as T
+    name, desc, o, ctx = pair
+    if pc == T.PC_MG and ctx.levels < 2:
+        pytest.skip("grid cannot be coarsened")
+    alpha, g = 1.0, 0.02j
+    sE = np.array([0.0, 0.0, 0.02])
+    sE[desc["dim"]:] = 0.0
+    xs = _rand(o.N, 7)
+    xs[_wall_mask(o, desc)] = 0.0
+    b = o.apply(alpha, g, sE, xs)
+    ctx.set_shift(alpha, g, sE)
+    ctx.pc_setup(pc)
+    db, dx = ctx.vec(b), ctx.vec()
+    st = ctx.solve(db, dx, rtol=1e-12, maxit=300)
+    x = dx.download()
+    res = np.linalg.norm(b - o.apply(alpha, g, sE, x)) / np.linalg.norm(b)
+    assert res < 1e-11, (name, pc, res, st.its)
+    assert st.its < 60, (name, pc, st.its)
+
+
+def test_cn_steps_match_oracle(pair):
+    name, desc, o, ctx = pair
+    import tdsez_amd as T
+    dt, nsteps = 0.02, 4
+    psi0 = o.project_gaussian(centre=(0.5, -0.3, 0.2)[:desc["dim"]], width=(1.2, 1.0, 1.1)[:desc["dim"]],
+                              momentum=(0.3, 0.0, -0.2)[:desc["dim"]])
+    sE = np.zeros((nsteps, 3))
+    sE[:, desc["dim"] - 1] = 0.05 * np.sin(0.3 * (np.arange(nsteps) + 0.5) * dt)
+    ref, its_ref, _ = o.cn_steps(psi0, dt, sE, rtol=1e-13)
+    ctx.pc_setup(T.PC_SEPARABLE)
+    d = ctx.vec(psi0)
+    for s in range(nsteps):
+        ctx.step_cn(d, dt, sE[s], rtol=1e-13)
+    got = d.download()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err < 1e-11, (name, err)
