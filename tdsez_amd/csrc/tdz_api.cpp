@@ -17,6 +17,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -83,6 +84,7 @@ struct Level {
     dcomplex *r = nullptr, *z = nullptr, *t = nullptr;
     // dim 3 with element e starting at function e on every axis (tiled kernel applies)
     bool simple = false;
+    int pot_variant = -1;   // tile shape of the tiled kernel picked by the autotuner (-1: generic)
 };
 
 }  // namespace
@@ -239,7 +241,8 @@ void pot_apply(tdz_ctx *c, Level &L, cplx g, const dcomplex *x, dcomplex *y)
         }
         HIPCHK(hipEventRecord(c->ev[c->ev_used].first, c->st));
     }
-    if (!(L.simple && c->use_tiled && tdz::launch_pot_apply_3d(c->st, ax, pp, g.real(), g.imag(), x, y)))
+    if (!(L.simple && c->use_tiled &&
+          tdz::launch_pot_apply_3d(c->st, ax, pp, g.real(), g.imag(), x, y, L.pot_variant)))
         tdz::launch_pot_apply_generic(c->st, ax, pp, g.real(), g.imag(), x, y);
     if (timed) HIPCHK(hipEventRecord(c->ev[c->ev_used++].second, c->st));
 }
@@ -253,6 +256,46 @@ void level_apply(tdz_ctx *c, Level &L, const dcomplex *x, dcomplex *y)
         const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
         tdz::launch_wall_fix(c->st, L.n, w, x, y);
     }
+}
+
+// Autotuner: time every tile-shape variant of the tiled potential kernel on this
+// level's grid (one warm-up and two timed launches each, HIP events on the
+// context stream) and keep the fastest. TDZ_POT_VARIANT=k forces variant k.
+void autotune_pot(tdz_ctx *c, Level &L)
+{
+    L.pot_variant = -1;
+    if (!L.simple || !c->ph.pot_kind) return;
+    const int nv = tdz::pot3d_variants(L.ax[0].p, L.ax[0].Q);
+    if (nv == 0) return;
+    if (const char *f = std::getenv("TDZ_POT_VARIANT")) {
+        const int k = std::atoi(f);
+        L.pot_variant = (k >= 0 && k < nv) ? k : -1;
+        return;
+    }
+    if (nv == 1) { L.pot_variant = 0; return; }
+    dcomplex *x = dalloc<dcomplex>(L.N), *y = dalloc<dcomplex>(L.N);
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipMemsetAsync(x, 0, L.N * sizeof(dcomplex), c->st));
+    HIPCHK(hipMemsetAsync(y, 0, L.N * sizeof(dcomplex), c->st));
+    tdz::AxisDev ax[3] = {axis_dev(L, 0), axis_dev(L, 1), axis_dev(L, 2)};
+    tdz::PotParams pp{c->ph.pot_kind, c->ph.s, c->ph.pot[0], c->ph.pot[1]};
+    float best = 1e30f;
+    for (int v = 0; v < nv; ++v) {
+        tdz::launch_pot_apply_3d(c->st, ax, pp, 1.0, 0.0, x, y, v);
+        HIPCHK(hipEventRecord(a, c->st));
+        for (int r = 0; r < 2; ++r) tdz::launch_pot_apply_3d(c->st, ax, pp, 1.0, 0.0, x, y, v);
+        HIPCHK(hipEventRecord(b, c->st));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        if (ms < best) { best = ms; L.pot_variant = v; }
+    }
+    HIPCHK(hipEventDestroy(a));
+    HIPCHK(hipEventDestroy(b));
+    HIPCHK(hipFree(x));
+    HIPCHK(hipFree(y));
 }
 
 void setup_fd(tdz_ctx *c, Level &L)
@@ -589,6 +632,7 @@ void build(tdz_ctx *c, const tdz_problem &pr)
         }
     }
     for (auto &L : c->lv) upload_level_tables(c, *L);
+    for (auto &L : c->lv) autotune_pot(c, *L);
     const long N = c->lv[0]->N;
     for (int i = 0; i < 4; ++i) c->tmp[i] = track(c, dalloc<dcomplex>(N));
     c->w1 = track(c, dalloc<dcomplex>(N));

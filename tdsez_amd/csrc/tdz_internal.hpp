@@ -79,7 +79,9 @@ void launch_pot_apply_generic(hipStream_t st, const AxisDev ax[3], PotParams pot
 // Same for dim 3 with simple interior knots (tiled z-marching kernel); returns
 // false when no instantiation covers (p, Q).
 bool launch_pot_apply_3d(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
-                         const dcomplex *x, dcomplex *y);
+                         const dcomplex *x, dcomplex *y, int variant);
+// number of tile-shape variants of the tiled kernel for (p, Q) (0: none)
+int pot3d_variants(int p, int Q);
 // Separable Kronecker apply y = Mz My Bx x + Mz By Mx x + Bz My Mx x, bands on device:
 // M_d real [n_d*(2p+1)], B_d complex [n_d*(2p+1)]; tmp = 4 scratch vectors.
 struct SepBands {
@@ -111,6 +113,9 @@ void launch_norm2(hipStream_t st, const dcomplex *x, long N, double *partial, do
 // Dense axis transform y = T x along axis d (T real n x n, row-major; trans: T^T).
 void launch_dense_axis(hipStream_t st, int d, const int n[3], const double *T, bool trans, const dcomplex *x,
                        dcomplex *y);
+// One thread per output, T read through the caches (reference kernel for tests).
+void launch_dense_axis_simple(hipStream_t st, int d, const int n[3], const double *T, bool trans,
+                              const dcomplex *x, dcomplex *y);
 // y[i,j,k] = x[i,j,k] / (alpha + g (lx[i] + ly[j] + lz[k])), walls: y = x
 void launch_diag_scale(hipStream_t st, const int n[3], const double *lam[3], const unsigned char *wall[3],
                        dcomplex alpha, dcomplex g, const dcomplex *x, dcomplex *y);

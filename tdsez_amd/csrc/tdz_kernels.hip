@@ -25,12 +25,13 @@ __device__ __forceinline__ double pot_value(const PotParams &pp, double r2)
     return 0.0;
 }
 
-// same values through the FP64 reciprocal square root (no division)
-__device__ __forceinline__ double pot_value_rsq(const PotParams &pp, double r2)
+// 1/sqrt(r) for r > 0: the hardware estimate refined by one second-order
+// Newton-Raphson (Householder) step, y = y0 + y0 e (1/2 + 3/8 e), e = 1 - r y0^2.
+__device__ __forceinline__ double rsqrt_pos(double r)
 {
-    if (pp.kind == 1) return 0.5 * pp.a0 * r2;
-    if (pp.kind == 2) return -pp.a0 * rsqrt(r2 + pp.a1);
-    return 0.0;
+    const double y0 = __builtin_amdgcn_rsq(r);
+    const double e = fma(-r * y0, y0, 1.0);
+    return fma(y0 * e, fma(e, 0.375, 0.5), y0);
 }
 
 // ------------------------------------------------------------------ potential
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, Axis
 //      scaled by g, atomically added to y (neighbouring tiles share p planes
 //      of functions in x and y).
 // Requires simple interior knots (element e starts at function e) and dim 3.
-template <int P, int Q, int TE, int CPT>
+template <int P, int Q, int TE, int CPT, int KIND>
 __global__ void __launch_bounds__((TE * Q) * (TE * Q) / CPT)
 pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double gi,
              const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
@@ -187,7 +188,8 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
         Bxs[i] = ex0 + el < ax.E ? ax.B0[(long)(ex0 + el) * NP * Q + r] : 0.0;
         Bys[i] = ey0 + el < ay.E ? ay.B0[(long)(ey0 + el) * NP * Q + r] : 0.0;
     }
-    // per column constants
+    // per column constants: soft-core s V w = cst / sqrt(xy2 + z^2) wz with xy2 = x^2 + y^2 + a;
+    // harmonic s V w = cst (xy2 + z^2) wz with xy2 = x^2 + y^2
     double xy2[CPT], wxy[CPT], By[CPT][NP];
     bool valid[CPT];
     int cqx[CPT], cey[CPT];
@@ -202,8 +204,9 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
         cey[c] = eyl;
         const int exs = valid[c] ? ex : 0, eys = valid[c] ? ey : 0;
         const double xv = ax.X[(long)exs * Q + q1], yv = ay.X[(long)eys * Q + q2];
-        xy2[c] = xv * xv + yv * yv;
-        wxy[c] = ax.W[(long)exs * Q + q1] * ay.W[(long)eys * Q + q2];
+        const double w = ax.W[(long)exs * Q + q1] * ay.W[(long)eys * Q + q2];
+        xy2[c] = xv * xv + yv * yv + (KIND == 2 ? pp.a1 : 0.0);
+        wxy[c] = KIND == 2 ? -pp.s * pp.a0 * w : 0.5 * pp.s * pp.a0 * w;
 #pragma unroll
         for (int a = 0; a < NP; ++a) By[c][a] = ay.B0[((long)eys * NP + a) * Q + q2];
     }
@@ -325,7 +328,8 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
                     fr += b * tw[c][a][0];
                     fi += b * tw[c][a][1];
                 }
-                const double v = pp.s * pot_value_rsq(pp, xy2[c] + z2s[qz]) * (wxy[c] * wzs[qz]);
+                const double r2 = xy2[c] + z2s[qz];
+                const double v = (KIND == 2 ? rsqrt_pos(r2) : r2) * (wxy[c] * wzs[qz]);
                 fr *= v;
                 fi *= v;
 #pragma unroll
@@ -610,6 +614,76 @@ __global__ void dense_axis_kernel(int d, int n0, int n1, int n2, const double *_
     y[I] = {sr, si};
 }
 
+// Tiled form of the same transform as a batched complex x real GEMM:
+//   Out[b][i][l] = sum_k T(i, k) In[b][k][l],  element (b, k, l) at b*sB + k*sK + l*sL.
+// A 64 x 64 (i, l) output tile per workgroup, K in chunks of 16 staged in LDS,
+// each of the 256 threads accumulates a 4 x 4 block. kfast: the input tile is
+// read with k fastest (sK = 1), else l fastest (sL = 1).
+constexpr int GT = 64, GK = 16;
+__global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, long sK, long sL, int kfast,
+                                                        const double *__restrict__ T, int trans,
+                                                        const dcomplex *__restrict__ in, dcomplex *__restrict__ out)
+{
+    __shared__ double Ts[GK][GT + 1];      // [k][i]
+    __shared__ dcomplex Is[GK][GT + 1];    // [k][l]
+    const int tid = threadIdx.x;
+    const long l0 = (long)blockIdx.x * GT;
+    const int i0 = blockIdx.y * GT;
+    const long boff = (long)blockIdx.z * sB;
+    const int ti = tid / 16, tl = tid % 16;   // 16 x 16 threads, 4 x 4 outputs each
+    double ar[4][4], ai[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) ar[a][b] = ai[a][b] = 0.0;
+    for (int k0 = 0; k0 < n; k0 += GK) {
+        for (int e = tid; e < GK * GT; e += 256) {
+            // T tile: Ts[k][i] = T(i0 + i, k0 + k)
+            const int kk = e % GK, ii = e / GK;
+            const int gi = i0 + ii, gk = k0 + kk;
+            double v = 0.0;
+            if (gi < n && gk < n) v = trans ? T[(long)gk * n + gi] : T[(long)gi * n + gk];
+            Ts[kk][ii] = v;
+            // input tile
+            int kq, lq;
+            if (kfast) { kq = e % GK; lq = e / GK; } else { lq = e % GT; kq = e / GT; }
+            const int gk2 = k0 + kq;
+            const long gl = l0 + lq;
+            dcomplex u = {0.0, 0.0};
+            if (gk2 < n && gl < L) u = in[boff + gk2 * sK + gl * sL];
+            Is[kq][lq] = u;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < GK; ++kk) {
+            double tv[4];
+            dcomplex iv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) tv[a] = Ts[kk][ti + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) iv[b] = Is[kk][tl + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    ar[a][b] += tv[a] * iv[b].re;
+                    ai[a][b] += tv[a] * iv[b].im;
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int gi = i0 + ti + 16 * a;
+        if (gi >= n) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const long gl = l0 + tl + 16 * b;
+            if (gl < L) out[boff + gi * sK + gl * sL] = {ar[a][b], ai[a][b]};
+        }
+    }
+}
+
 __global__ void diag_scale_kernel(int n0, int n1, long N, const double *l0, const double *l1, const double *l2,
                                   const unsigned char *w0, const unsigned char *w1, const unsigned char *w2,
                                   dcomplex alpha, dcomplex g, const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
@@ -680,17 +754,41 @@ static void launch_pot3d(hipStream_t st, const AxisDev ax[3], PotParams pot, dou
 {
     constexpr int NT = (TE * Q) * (TE * Q) / CPT;
     const dim3 grid((unsigned)((ax[0].E + TE - 1) / TE), (unsigned)((ax[1].E + TE - 1) / TE));
-    hipLaunchKernelGGL((pot3d_kernel<P, Q, TE, CPT>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr, gi, x, y);
+    if (pot.kind == 2)
+        hipLaunchKernelGGL((pot3d_kernel<P, Q, TE, CPT, 2>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr, gi,
+                           x, y);
+    else
+        hipLaunchKernelGGL((pot3d_kernel<P, Q, TE, CPT, 1>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr, gi,
+                           x, y);
+}
+
+int pot3d_variants(int p, int Q)
+{
+    if (p == 4 && Q == 8) return 2;
+    if (p == 3 && Q == 8) return 2;
+    if (p == 3 && Q == 5) return 1;
+    return 0;
 }
 
 bool launch_pot_apply_3d(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
-                         const dcomplex *x, dcomplex *y)
+                         const dcomplex *x, dcomplex *y, int variant)
 {
     const int p = ax[0].p, Q = ax[0].Q;
     for (int d = 1; d < 3; ++d)
         if (ax[d].p != p || ax[d].Q != Q) return false;
-    if (p == 4 && Q == 8) { launch_pot3d<4, 8, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
-    if (p == 3 && Q == 8) { launch_pot3d<3, 8, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
+    if (pot.kind != 1 && pot.kind != 2) return false;
+    if (variant < 0 || variant >= pot3d_variants(p, Q)) return false;
+    // variant 0: 4 x 4 element tiles, 1024 threads; variant 1: 2 x 2 tiles, 256 threads
+    if (p == 4 && Q == 8) {
+        if (variant == 0) launch_pot3d<4, 8, 4, 1>(st, ax, pot, gr, gi, x, y);
+        else launch_pot3d<4, 8, 2, 1>(st, ax, pot, gr, gi, x, y);
+        return true;
+    }
+    if (p == 3 && Q == 8) {
+        if (variant == 0) launch_pot3d<3, 8, 4, 1>(st, ax, pot, gr, gi, x, y);
+        else launch_pot3d<3, 8, 2, 1>(st, ax, pot, gr, gi, x, y);
+        return true;
+    }
     if (p == 3 && Q == 5) { launch_pot3d<3, 5, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
     return false;
 }
@@ -756,6 +854,22 @@ void launch_norm2(hipStream_t st, const dcomplex *x, long N, double *partial, do
 
 void launch_dense_axis(hipStream_t st, int d, const int n[3], const double *T, bool trans, const dcomplex *x,
                        dcomplex *y)
+{
+    // batched GEMM view of the transform along axis d
+    const long n01 = (long)n[0] * n[1];
+    long L, sB, sK, sL;
+    unsigned B;
+    int kfast;
+    if (d == 0) { L = (long)n[1] * n[2]; sB = 0; sK = 1; sL = n[0]; B = 1; kfast = 1; }
+    else if (d == 1) { L = n[0]; sB = n01; sK = n[0]; sL = 1; B = (unsigned)n[2]; kfast = 0; }
+    else { L = n01; sB = 0; sK = n01; sL = 1; B = 1; kfast = 0; }
+    const dim3 grid((unsigned)((L + GT - 1) / GT), (unsigned)((n[d] + GT - 1) / GT), B);
+    hipLaunchKernelGGL(axis_gemm_kernel, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T, trans ? 1 : 0, x,
+                       y);
+}
+
+void launch_dense_axis_simple(hipStream_t st, int d, const int n[3], const double *T, bool trans,
+                              const dcomplex *x, dcomplex *y)
 {
     const long N = (long)n[0] * n[1] * n[2];
     hipLaunchKernelGGL(dense_axis_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st, d, n[0], n[1], n[2], T,
