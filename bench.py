@@ -4,7 +4,8 @@ Workload (the 3D soft-core configuration): V = -1/sqrt(r^2+2) on [-40,40]^3,
 128 uniform elements per axis, p = 4, Q = 8 Gauss points, a quadratic CAP in
 the outer 8 a.u. (strength 0.01), a z-polarised sin^2 pulse E0 = 0.05,
 omega = 0.057, 4 cycles; CN dt = 0.02; FGMRES to ||r||/||b|| < 1e-10 with the
-chosen preconditioner. A "step" is one CN step; inputs are resident in HBM.
+chosen preconditioner. The initial state is the lowest eigenstate of H, computed
+on the GPU at set-up. A "step" is one CN step; inputs are resident in HBM.
 
 Prints one JSON line (see the driver contract). The CPU baseline times the CPU
 oracle on one CN step of the same problem (bounded sample) on this host.
@@ -35,19 +36,6 @@ def pot_flops_per_element(p, Q):
     return 2 * 4 * (n ** 3 * Q + n ** 2 * Q ** 2 + n * Q ** 3) + 2 * Q ** 3
 
 
-def initial_state(desc, n):
-    """Separable Gaussian (width 1.5) sampled at the Greville abscissae, then M-normalised."""
-    p = desc["p"]
-    vals = []
-    for d in range(3):
-        E, lo, hi = desc["nelem"][d], desc["lo"][d], desc["hi"][d]
-        t = np.concatenate([[lo] * (p + 1), lo + (hi - lo) * np.arange(1, E) / E, [hi] * (p + 1)])
-        g = np.array([t[i + 1:i + p + 1].mean() for i in range(n[d])])
-        vals.append(np.exp(-g ** 2 / (2 * 1.5 ** 2)))
-    psi = np.einsum("k,j,i->kji", vals[2], vals[1], vals[0]).reshape(-1).astype(np.complex128)
-    return psi
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +45,8 @@ def main():
     ap.add_argument("--nel", type=int, default=128)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--init", default="ground", choices=["ground", "gaussian"],
+                    help="initial state: lowest eigenstate (default) or the projected Gaussian (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -68,10 +58,14 @@ def main():
     ctx = T.Context(desc)
     pc = {"sep": T.PC_SEPARABLE, "mg": T.PC_MG, "none": T.PC_NONE}[args.pc]
     ctx.pc_setup(pc)
-    psi = ctx.vec(initial_state(desc, ctx.n))
-    nrm = ctx.observables(psi)[0]
-    psi.upload(psi.download() / math.sqrt(nrm))
+    psi = ctx.vec()
+    ctx.project_gaussian(psi, (0.0, 0.0, 0.0), (1.5, 1.5, 1.5))
+    E0, res0, it0 = float("nan"), float("nan"), 0
+    if args.init == "ground":
+        E0, res0, it0 = ctx.ground_state(psi, sigma=-0.3, tol=1e-8, maxit=80)
     t_setup = time.perf_counter() - t_setup
+    print(f"# ground state E0 = {E0:.10f} (eigen-residual {res0:.1e}, {it0} inverse iterations)",
+          file=sys.stderr, flush=True)
     pulse = T.make_pulse(E0=(0.0, 0.0, 0.05), omega=0.057, ncycles=4.0)
     dt = 0.02
     print(f"# setup {t_setup:.2f} s, N = {ctx.N}, levels = {ctx.levels}, pc = {args.pc}", file=sys.stderr, flush=True)
@@ -93,19 +87,27 @@ def main():
     flops = pot_flops_per_element(p, Q) * nel ** 3
     avg_s = (kms / 1e3) / max(nlaunch, 1)
     achieved = flops / avg_s if nlaunch else 0.0
-    roofline = {"bound": "mfma", "kernel": "pot_apply_generic_kernel", "achieved": achieved / 1e12,
+    roofline = {"bound": "mfma", "kernel": "pot3d_kernel", "achieved": achieved / 1e12,
                 "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
                 "launches": nlaunch, "avg_ms": avg_s * 1e3, "flops_per_launch": flops}
+    # HBM bytes per launch from the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
+    # (copy-kernel calibrated); only meaningful for the 128^3 workload they were taken on
+    tj = os.path.join(ROOT, "profiles", "pot3d_softcore128_traffic.json")
+    if nel == 128 and os.path.exists(tj):
+        with open(tj) as f:
+            tr = json.load(f)
+        roofline["traffic"] = tr["traffic_bytes"]
+        roofline["traffic_source"] = "profiles/pot3d_softcore128_traffic.json (" + tr["kernel"] + ")"
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(desc, dt, pulse, args)
+        cpu = cpu_baseline(desc, dt, pulse, args, psi.download())
 
     line = {
         "metric": "CN steps/s, 3D p=4 128^3 grid: learned+mixed PC vs plain FP64 MG at |r|<1e-10",
         "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "complex128 (fp64)", "data": "synthetic (Gaussian initial state)",
+        "vs_baseline": None, "dtype": "complex128 (fp64)", "data": "synthetic: lowest eigenstate of H (GPU inverse iteration), E0 = %.8f" % E0,
         "config": {"workload": f"3D soft-core atom {nel}^3 elements p=4 Q=8, CN dt=0.02, rtol={args.rtol}",
                    "preconditioner": {"sep": "fp64 separable (fast diagonalisation)", "mg": "plain fp64 GMG",
                                       "none": "none"}[args.pc],
@@ -117,7 +119,7 @@ def main():
     ctx.close()
 
 
-def cpu_baseline(desc, dt, pulse, args):
+def cpu_baseline(desc, dt, pulse, args, psi):
     """Time the CPU oracle on one CN step of the same problem (bounded sample)."""
     try:
         import oracle
@@ -125,8 +127,6 @@ def cpu_baseline(desc, dt, pulse, args):
     except Exception as e:  # noqa: BLE001
         return {"error": f"oracle unavailable: {e}"}
     o = Oracle(desc)
-    n = o.n
-    psi = initial_state(desc, n)
     t = args.warmup * dt + 0.5 * dt
     E = 0.05 * math.sin(math.pi * t / (4 * 2 * math.pi / 0.057)) ** 2 * math.sin(0.057 * t)
     sE = np.array([[0.0, 0.0, E]])

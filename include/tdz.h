@@ -117,6 +117,28 @@ int tdz_step_cn(tdz_ctx *ctx, double *psi, double dt, const double sE_mid[3], do
  * step; its (optional, host) the Krylov iterations per step. */
 int tdz_evolve_cn(tdz_ctx *ctx, double *psi, double t0, double dt, int nsteps, const tdz_pulse *pulse,
                   double rtol, int maxit, double *obs, int *its);
+/* Initial states. tdz_project_gaussian: L2 projection of the packet
+ * prod_a (2 pi w_a^2)^(-1/4) exp(-(x_a-c_a)^2/(4 w_a^2) + i k_a x_a), normalised so
+ * psi^H M psi = 1 (src/core.cpp:1188-1262). tdz_ground_state: lowest eigenstate of
+ * H = K + sV (CAP excluded, as before src/propagator.cpp:51-69) by shifted inverse
+ * iteration from psi, shift sigma below the ground energy, until the relative
+ * eigen-residual < tol (replaces the SLEPc solve of src/core.cpp:758-1083). */
+int tdz_project_gaussian(tdz_ctx *ctx, const double centre[3], const double width[3], const double momentum[3],
+                         double *psi);
+int tdz_ground_state(tdz_ctx *ctx, double *psi, double sigma, double tol, int maxit, double *energy,
+                     double *residual, int *iterations);
+/* One Pade(2,2) step psi <- R22(-i h M^-1 H_E) psi: stages
+ * (M + g_k H_E) y = (M + conj(g_k) H_E) x, g_{1,2} = h(-+sqrt3 + 3i)/12, each
+ * started from x plus the order-3 binomial extrapolation of the increments of
+ * earlier calls with the same slot (Pade22Step, src/propagator.cpp:564-676). */
+int tdz_step_pade22(tdz_ctx *ctx, double *psi, double h, const double sE[3], int slot, double rtol, int maxit,
+                    tdz_ksp_stats *stats);
+/* nsteps of the rational propagators (EvolveRational, src/propagator.cpp:682-769):
+ * magnus = 0: Pade(2,2) with the midpoint field; magnus = 1: commutator-free
+ * Magnus 4, two Pade(2,2) half steps with fields 2(a2 E1 + a1 E2), 2(a1 E1 + a2 E2)
+ * at the Gauss nodes. obs/its as for tdz_evolve_cn (its: sum over the stages). */
+int tdz_evolve_rational(tdz_ctx *ctx, double *psi, double t0, double dt, int nsteps, const tdz_pulse *pulse,
+                        int magnus, double rtol, int maxit, double *obs, int *its);
 /* Observables [norm psi^H M psi, Re psi^H H psi, Im psi^H H psi, <x>, <y>, <z>]
  * (MonitorProducts, src/manager.cpp:584-672; src/monitor.cpp:1145-1194). */
 int tdz_observables(tdz_ctx *ctx, const double *psi, double out[6]);

@@ -28,7 +28,8 @@ EXPORTS = [
     "tdz_vec_alloc", "tdz_vec_free", "tdz_vec_upload", "tdz_vec_download", "tdz_sync",
     "tdz_set_shift", "tdz_op_apply", "tdz_term_apply", "tdz_pc_setup", "tdz_pc_apply",
     "tdz_ksp_solve", "tdz_step_cn", "tdz_evolve_cn", "tdz_observables", "tdz_pulse_coupling",
-    "tdz_timing", "tdz_timing_stats",
+    "tdz_timing", "tdz_timing_stats", "tdz_step_pade22", "tdz_evolve_rational",
+    "tdz_project_gaussian", "tdz_ground_state",
 ]
 
 
@@ -117,6 +118,10 @@ def lib():
         L.tdz_evolve_cn.argtypes = [vp, vp, d, d, i, C.POINTER(Pulse), d, i, dp, ip]
         L.tdz_observables.argtypes = [vp, vp, dp]
         L.tdz_pulse_coupling.argtypes = [vp, C.POINTER(Pulse), d, dp]
+        L.tdz_step_pade22.argtypes = [vp, vp, d, dp, i, d, i, C.POINTER(KspStats)]
+        L.tdz_evolve_rational.argtypes = [vp, vp, d, d, i, C.POINTER(Pulse), i, d, i, dp, ip]
+        L.tdz_project_gaussian.argtypes = [vp, dp, dp, dp, vp]
+        L.tdz_ground_state.argtypes = [vp, vp, d, d, i, dp, dp, ip]
         L.tdz_timing.argtypes = [vp, i]
         L.tdz_timing_stats.argtypes = [vp, dp, C.POINTER(C.c_long)]
         _lib = L
@@ -225,6 +230,33 @@ class Context:
                                  rtol, maxit, _dp(obs) if observe else None,
                                  its.ctypes.data_as(C.POINTER(C.c_int))))
         return its, obs
+
+    def step_pade22(self, psi, h, sE=(0, 0, 0), slot=0, rtol=1e-12, maxit=1000):
+        st = KspStats()
+        s = np.ascontiguousarray(sE, dtype=np.float64)
+        _chk(lib().tdz_step_pade22(self.h, psi.ptr, h, _dp(s), slot, rtol, maxit, C.byref(st)))
+        return st
+
+    def evolve_rational(self, psi, t0, dt, nsteps, pulse=None, magnus=True, rtol=1e-12, maxit=1000,
+                        observe=False):
+        its = np.zeros(nsteps, dtype=np.int32)
+        obs = np.zeros((nsteps, 7)) if observe else None
+        _chk(lib().tdz_evolve_rational(self.h, psi.ptr, t0, dt, nsteps,
+                                       C.byref(pulse) if pulse is not None else None, 1 if magnus else 0, rtol,
+                                       maxit, _dp(obs) if observe else None,
+                                       its.ctypes.data_as(C.POINTER(C.c_int))))
+        return its, obs
+
+    def project_gaussian(self, psi, centre, width, momentum=(0, 0, 0)):
+        c = np.ascontiguousarray(list(centre) + [0.0] * (3 - len(centre)), dtype=np.float64)
+        w = np.ascontiguousarray(list(width) + [1.0] * (3 - len(width)), dtype=np.float64)
+        k = np.ascontiguousarray(list(momentum) + [0.0] * (3 - len(momentum)), dtype=np.float64)
+        _chk(lib().tdz_project_gaussian(self.h, _dp(c), _dp(w), _dp(k), psi.ptr))
+
+    def ground_state(self, psi, sigma, tol=1e-9, maxit=100):
+        E, r, it = C.c_double(0), C.c_double(0), C.c_int(0)
+        _chk(lib().tdz_ground_state(self.h, psi.ptr, sigma, tol, maxit, C.byref(E), C.byref(r), C.byref(it)))
+        return E.value, r.value, it.value
 
     def observables(self, psi):
         out = np.zeros(6)

@@ -107,6 +107,11 @@ struct tdz_ctx {
     std::vector<void *> allocs;     // everything to free
     std::vector<double *> user_vecs;
     int refresh_base = -1, rebuilds = 0;
+    bool cap_on = true;         // the CAP is part of H (off during the eigensolve, as src/propagator.cpp:51-69)
+    // Pade(2,2) stage guesses: per slot (2 x stage) the last increments y - x
+    dcomplex *hist = nullptr;   // [4 slots][3] vectors
+    int hist_count[4] = {0, 0, 0, 0};
+    dcomplex *xin = nullptr;    // stage input copy
     bool use_tiled = true;   // tiled 3D potential kernel where it applies
     // kernel timing (measurement hook)
     bool timing = false;
@@ -198,7 +203,7 @@ void update_bands(tdz_ctx *c, Level &L)
         const bool active = d < c->prob.dim;
         for (size_t k = 0; k < sz; ++k) {
             cplx T = 0.0;
-            if (active) T = inv2m * L.S[d][k] + cplx(0, -1) * L.Wc[d][k] + c->sE[d] * L.Xb[d][k];
+            if (active) T = inv2m * L.S[d][k] + (c->cap_on ? cplx(0, -1) * L.Wc[d][k] : 0.0) + c->sE[d] * L.Xb[d][k];
             cplx b = c->g * T;
             if (d == 0) b += c->alpha * L.M[d][k];
             B[k] = dc(b);
@@ -387,6 +392,29 @@ void pc_apply_dev(tdz_ctx *c, const dcomplex *r, dcomplex *z)
     case TDZ_PC_SEPARABLE: fd_apply(c, L, r, z, c->w1, c->w2); break;
     case TDZ_PC_MG: vcycle(c, 0, r, z); break;
     default: HIPCHK(hipMemcpyAsync(z, r, L.N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st)); break;
+    }
+}
+
+// <x, y> on device (deterministic), result on host
+cplx dev_dot(tdz_ctx *c, const dcomplex *x, const dcomplex *y, long N)
+{
+    tdz::launch_multi_dot(c->st, x, N, 1, y, N, c->dpart, c->dh);
+    dcomplex h{0, 0};
+    HIPCHK(hipMemcpyAsync(&h, c->dh, sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return {h.re, h.im};
+}
+
+// y = M x on the fine level (walls: identity)
+void mass_apply(tdz_ctx *c, const dcomplex *x, dcomplex *y)
+{
+    Level &L = *c->lv[0];
+    tdz::SepBands mb = sep_bands(L);
+    for (int d = 0; d < 3; ++d) mb.B[d] = L.dBmass[d];
+    tdz::launch_sep_apply(c->st, mb, x, y, c->tmp);
+    if (c->prob.dirichlet) {
+        const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+        tdz::launch_wall_fix(c->st, L.n, w, x, y);
     }
 }
 
@@ -855,6 +883,228 @@ int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], doub
         tdz::launch_axpby(c->st, {-2.0, 0}, c->w1, {2.0, 0}, c->Vb, L.N);
         set_shift(c, 1.0, g, sE_mid);         // solve (M + i dt/2 H_E) psi' = rhs, guess psi
         fgmres(c, c->rhs, x, rtol, maxit, stats, true);
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr int kGuessOrder = 3;
+
+// psi <- R22(-i h M^-1 H_E) psi (src/propagator.cpp:564-676). Stage k solves
+// (M + g_k H_E) y = (M + conj(g_k) H_E) x from the guess x + extrapolated increment
+// (binomial weights of order <= 3 over this slot's earlier increments).
+void pade22_step(tdz_ctx *c, dcomplex *x, double h, const double sE[3], int slot, double rtol, int maxit,
+                 tdz_ksp_stats *stats)
+{
+    if (!c->pc_ready) pc_setup(c, c->pc_kind);
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    if (!c->hist) {
+        c->hist = track(c, dalloc<dcomplex>((size_t)4 * kGuessOrder * N));
+        c->xin = track(c, dalloc<dcomplex>(N));
+    }
+    const double s3 = std::sqrt(3.0);
+    const cplx gk[2] = {h * cplx(-s3, 3.0) / 12.0, h * cplx(s3, 3.0) / 12.0};
+    static const double w[3][3] = {{1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
+    tdz_ksp_stats tot{};
+    for (int k = 0; k < 2; ++k) {
+        const int sl = 2 * slot + k;
+        dcomplex *H = c->hist + (size_t)sl * kGuessOrder * N;
+        set_shift(c, 1.0, std::conj(gk[k]), sE);
+        level_apply(c, L, x, c->rhs);
+        HIPCHK(hipMemcpyAsync(c->xin, x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+        const int q = std::min(kGuessOrder, c->hist_count[sl]);
+        for (int j = 0; j < q; ++j) tdz::launch_axpby(c->st, {w[q - 1][j], 0}, H + (size_t)j * N, {1, 0}, x, N);
+        set_shift(c, 1.0, gk[k], sE);
+        tdz_ksp_stats st{};
+        fgmres(c, c->rhs, x, rtol, maxit, &st);
+        // history: shift, newest increment y - x first
+        for (int j = kGuessOrder - 1; j > 0; --j)
+            HIPCHK(hipMemcpyAsync(H + (size_t)j * N, H + (size_t)(j - 1) * N, N * sizeof(dcomplex),
+                                  hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(H, x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+        tdz::launch_axpby(c->st, {-1, 0}, c->xin, {1, 0}, H, N);
+        c->hist_count[sl] = std::min(c->hist_count[sl] + 1, kGuessOrder);
+        tot.its += st.its;
+        tot.restarts += st.restarts;
+        tot.res = std::max(tot.res, st.res);
+        tot.bnorm = st.bnorm;
+    }
+    tot.pc_rebuilds = c->rebuilds;
+    if (stats) *stats = tot;
+}
+
+// L2 projection of a Gaussian packet (src/core.cpp:1188-1262): b_i = int B_i psi0,
+// c = M^-1 b, c /= sqrt(c^H M c). psi0 is separable, so b is a Kronecker product
+// of 1D integrals and M^-1 is applied exactly by fast diagonalisation with g = 0.
+void project_gaussian(tdz_ctx *c, const double centre[3], const double width[3], const double mom[3],
+                      dcomplex *out)
+{
+    Level &L = *c->lv[0];
+    std::vector<cplx> bd[3];
+    for (int d = 0; d < 3; ++d) {
+        const tdz::Axis1D &A = L.ax[d];
+        bd[d].assign(A.n, 0.0);
+        if (d >= c->prob.dim) { bd[d][0] = 1.0; continue; }
+        const int nen = A.p + 1;
+        for (int e = 0; e < A.E; ++e)
+            for (int q = 0; q < A.Q; ++q) {
+                const double x = A.X[(size_t)e * A.Q + q], w = A.W[(size_t)e * A.Q + q];
+                const double s = width[d], u = x - centre[d];
+                const cplx v = std::pow(2.0 * M_PI * s * s, -0.25) *
+                               std::exp(cplx(-u * u / (4.0 * s * s), mom[d] * x));
+                for (int a = 0; a < nen; ++a)
+                    bd[d][A.off[e] + a] += w * A.B0[((size_t)e * nen + a) * A.Q + q] * v;
+            }
+    }
+    std::vector<dcomplex> b(L.N);
+    const int n0 = L.n[0], n1 = L.n[1];
+    for (int k = 0; k < L.n[2]; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n0; ++i)
+                b[i + (size_t)n0 * (j + (size_t)n1 * k)] = dc(bd[0][i] * bd[1][j] * bd[2][k]);
+    HIPCHK(hipMemcpyAsync(c->w1, b.data(), L.N * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+    setup_fd(c, L);
+    const cplx a0 = c->alpha, g0 = c->g;
+    double s0[3] = {c->sE[0], c->sE[1], c->sE[2]};
+    c->alpha = 1.0;
+    c->g = 0.0;
+    fd_apply(c, L, c->w1, out, c->tmp[0], c->tmp[1]);
+    c->alpha = a0;
+    c->g = g0;
+    (void)s0;
+    mass_apply(c, out, c->w2);
+    const double nrm = std::sqrt(dev_dot(c, out, c->w2, L.N).real());
+    tdz::launch_axpby(c->st, {1.0 / nrm, 0}, out, {0, 0}, out, L.N);
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
+// Lowest eigenstate of H = K + sV (no CAP) by shifted inverse iteration with
+// Rayleigh-quotient shift updates: (H - sigma M) x_{k+1} = M x_k by FGMRES with the
+// separable preconditioner of (-sigma) M + (K + V_sep). Stops when the relative
+// eigen-residual ||H x - E M x|| / ||M x|| < tol. Replaces the SLEPc shift-invert
+// solve (src/core.cpp:758-1083) for the ground state only.
+void ground_state(tdz_ctx *c, dcomplex *x, double sigma, double tol, int maxit, double *E_out, double *res_out,
+                  int *its_out)
+{
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    const int pc0 = c->pc_kind;
+    const cplx a0 = c->alpha, g0 = c->g;
+    double s0[3] = {c->sE[0], c->sE[1], c->sE[2]};
+    const double zero[3] = {0, 0, 0};
+    c->cap_on = false;
+    c->pc_kind = TDZ_PC_SEPARABLE;
+    setup_fd(c, L);
+    auto rayleigh = [&](double &E, double &res) {
+        mass_apply(c, x, c->w2);                       // w2 = M x
+        set_shift(c, 0.0, 1.0, zero);
+        level_apply(c, L, x, c->w1);                   // w1 = H x
+        const double xm = dev_dot(c, x, c->w2, N).real();
+        E = dev_dot(c, x, c->w1, N).real() / xm;
+        tdz::launch_axpby(c->st, {-E, 0}, c->w2, {1, 0}, c->w1, N);   // w1 = H x - E M x
+        res = std::sqrt(dev_norm2(c, c->w1, N) / dev_norm2(c, c->w2, N));
+    };
+    double E = 0, res = 1;
+    int it = 0;
+    try {
+        rayleigh(E, res);
+        for (; it < maxit && res > tol; ++it) {
+            const double shift = it < 3 ? sigma : std::min(sigma, E - 10.0 * res);
+            mass_apply(c, x, c->rhs);                  // rhs = M x
+            set_shift(c, -shift, 1.0, zero);
+            tdz_ksp_stats st{};
+            fgmres(c, c->rhs, x, 1e-11, 500, &st);
+            mass_apply(c, x, c->w2);
+            const double nrm = std::sqrt(dev_dot(c, x, c->w2, N).real());
+            tdz::launch_axpby(c->st, {1.0 / nrm, 0}, x, {0, 0}, x, N);
+            rayleigh(E, res);
+        }
+    } catch (...) {
+        c->cap_on = true;
+        c->pc_kind = pc0;
+        set_shift(c, a0, g0, s0);
+        throw;
+    }
+    c->cap_on = true;
+    c->pc_kind = pc0;
+    set_shift(c, a0, g0, s0);
+    if (E_out) *E_out = E;
+    if (res_out) *res_out = res;
+    if (its_out) *its_out = it;
+    if (res > tol) throw Error("ground state: inverse iteration did not converge");
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdz_project_gaussian(tdz_ctx *c, const double centre[3], const double width[3], const double momentum[3],
+                         double *psi)
+{
+    return guarded([&] { project_gaussian(c, centre, width, momentum, reinterpret_cast<dcomplex *>(psi)); });
+}
+
+int tdz_ground_state(tdz_ctx *c, double *psi, double sigma, double tol, int maxit, double *energy,
+                     double *residual, int *iterations)
+{
+    return guarded([&] {
+        ground_state(c, reinterpret_cast<dcomplex *>(psi), sigma, tol, maxit, energy, residual, iterations);
+    });
+}
+
+int tdz_step_pade22(tdz_ctx *c, double *psi, double h, const double sE[3], int slot, double rtol, int maxit,
+                    tdz_ksp_stats *stats)
+{
+    return guarded([&] {
+        if (slot < 0 || slot > 1) throw Error("slot must be 0 or 1");
+        pade22_step(c, reinterpret_cast<dcomplex *>(psi), h, sE, slot, rtol, maxit, stats);
+    });
+}
+
+int tdz_evolve_rational(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, const tdz_pulse *pulse,
+                        int magnus, double rtol, int maxit, double *obs, int *its)
+{
+    return guarded([&] {
+        dcomplex *x = reinterpret_cast<dcomplex *>(psi);
+        // commutator-free Magnus 4 (src/propagator.cpp:729-758): Gauss nodes c1,2, weights a1,2
+        const double s3 = std::sqrt(3.0);
+        const double c1 = 0.5 - s3 / 6.0, c2 = 0.5 + s3 / 6.0;
+        const double a1 = (3.0 - 2.0 * s3) / 12.0, a2 = (3.0 + 2.0 * s3) / 12.0;
+        for (int s = 0; s < nsteps; ++s) {
+            const double t = t0 + s * dt;
+            tdz_ksp_stats st{};
+            int total = 0;
+            if (!magnus) {
+                double E[3] = {0, 0, 0};
+                if (pulse) tdz_pulse_coupling(c, pulse, t + 0.5 * dt, E);
+                pade22_step(c, x, dt, E, 0, rtol, maxit, &st);
+                total = st.its;
+            } else {
+                double E1[3] = {0, 0, 0}, E2[3] = {0, 0, 0}, Ea[3], Eb[3];
+                if (pulse) {
+                    tdz_pulse_coupling(c, pulse, t + c1 * dt, E1);
+                    tdz_pulse_coupling(c, pulse, t + c2 * dt, E2);
+                }
+                for (int a = 0; a < 3; ++a) {
+                    Ea[a] = 2.0 * (a2 * E1[a] + a1 * E2[a]);   // applied first
+                    Eb[a] = 2.0 * (a1 * E1[a] + a2 * E2[a]);
+                }
+                pade22_step(c, x, 0.5 * dt, Ea, 0, rtol, maxit, &st);
+                total = st.its;
+                pade22_step(c, x, 0.5 * dt, Eb, 1, rtol, maxit, &st);
+                total += st.its;
+            }
+            if (its) its[s] = total;
+            if (obs) {
+                double o[6];
+                if (tdz_observables(c, psi, o)) throw Error(g_err);
+                obs[7 * s] = t + dt;
+                for (int k = 0; k < 6; ++k) obs[7 * s + 1 + k] = o[k];
+            }
+        }
     });
 }
 
