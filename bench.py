@@ -1,14 +1,21 @@
-"""Benchmark: Crank-Nicolson steps/s of the 3D soft-core atom at 128^3, p = 4.
+"""Benchmark: Crank-Nicolson steps/s of the 3D soft-core atom at 128^3, p = 4,
+learned mixed-precision preconditioner against plain FP64 multigrid.
 
 Workload (the 3D soft-core configuration): V = -1/sqrt(r^2+2) on [-40,40]^3,
 128 uniform elements per axis, p = 4, Q = 8 Gauss points, a quadratic CAP in
 the outer 8 a.u. (strength 0.01), a z-polarised sin^2 pulse E0 = 0.05,
-omega = 0.057, 4 cycles; CN dt = 0.02; FGMRES to ||r||/||b|| < 1e-10 with the
-chosen preconditioner. The initial state is the lowest eigenstate of H, computed
-on the GPU at set-up. A "step" is one CN step; inputs are resident in HBM.
+omega = 0.057, 4 cycles; CN dt = 0.02; FGMRES to ||r||/||b|| < 1e-10. The
+initial state is the lowest eigenstate of H, computed on the GPU at set-up. A
+"step" is one CN step; inputs are resident in HBM.
 
-Prints one JSON line (see the driver contract). The CPU baseline times the CPU
-oracle on one CN step of the same problem (bounded sample) on this host.
+Default run: the same steps from the same state twice, first with plain FP64
+geometric multigrid (the baseline), then with the learned mixed-precision
+preconditioner (network trained at set-up on vectors harvested from two CN
+steps; set-up time is not in the timed region). value = steps/s of the learned
+run; the baseline's steps/s, the speed-up and the L2 difference of the two final
+wavefunctions go into the same line. Prints one JSON line (see the driver
+contract). The CPU baseline times the CPU oracle on one CN step of the same
+problem (bounded sample) on this host.
 """
 import argparse
 import json
@@ -31,6 +38,15 @@ def problem(nel=128):
                 pot_kind=2, pot=(1.0, 2.0), cap_kind=1, cap=(0.01, 8.0), mg_levels=3, mg_smooth=1)
 
 
+PCS = {"sep": "fp64 separable (fast diagonalisation)", "mg": "plain fp64 GMG",
+       "sep32": "separable solve in complex64", "mg32": "mixed-precision GMG (complex64 V-cycle)",
+       "corr32": "complex64 first-order corrected separable solve (lumped potential)",
+       "corrx32": "complex64 first-order corrected separable solve (matrix-free potential)",
+       "learned32": "learned mixed precision: complex64 corrected separable solve + BF16-MFMA network remainder, "
+                    "FP64 operator/orthogonalisation/residual, plain FP64 GMG fallback",
+       "none": "none"}
+
+
 def pot_flops_per_element(p, Q):
     n = p + 1
     return 2 * 4 * (n ** 3 * Q + n ** 2 * Q ** 2 + n * Q ** 3) + 2 * Q ** 3
@@ -39,9 +55,11 @@ def pot_flops_per_element(p, Q):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--pc", default="mg", choices=["sep", "mg", "none"])
+    ap.add_argument("--pc", default="learned32", choices=list(PCS))
+    ap.add_argument("--no-baseline", action="store_true", help="skip the plain FP64 GMG run")
+    ap.add_argument("--train-iters", type=int, default=150)
     ap.add_argument("--nel", type=int, default=128)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -56,32 +74,74 @@ def main():
     desc = problem(args.nel)
     t_setup = time.perf_counter()
     ctx = T.Context(desc)
-    pc = {"sep": T.PC_SEPARABLE, "mg": T.PC_MG, "none": T.PC_NONE}[args.pc]
-    ctx.pc_setup(pc)
+    kinds = {"sep": T.PC_SEPARABLE, "mg": T.PC_MG, "none": T.PC_NONE, "sep32": T.PC_SEPARABLE_MIXED,
+             "mg32": T.PC_MG_MIXED, "corr32": T.PC_CORRECTED, "corrx32": T.PC_CORRECTED_EXACT,
+             "learned32": T.PC_LEARNED}
+    ctx.pc_setup(T.PC_MG if ctx.levels > 1 else T.PC_SEPARABLE)
     psi = ctx.vec()
     ctx.project_gaussian(psi, (0.0, 0.0, 0.0), (1.5, 1.5, 1.5))
     E0, res0, it0 = float("nan"), float("nan"), 0
     if args.init == "ground":
         E0, res0, it0 = ctx.ground_state(psi, sigma=-0.3, tol=1e-8, maxit=80)
-    t_setup = time.perf_counter() - t_setup
     print(f"# ground state E0 = {E0:.10f} (eigen-residual {res0:.1e}, {it0} inverse iterations)",
           file=sys.stderr, flush=True)
     pulse = T.make_pulse(E0=(0.0, 0.0, 0.05), omega=0.057, ncycles=4.0)
     dt = 0.02
+    x0 = psi.download()
+
+    def run(kind, timing):
+        """W untimed + K timed CN steps from x0; returns (steps/s, its, final psi, kernel ms, launches)."""
+        ctx.pc_setup(kind)
+        d = ctx.vec(x0)
+        ctx.evolve_cn(d, 0.0, dt, args.warmup, pulse, rtol=args.rtol, maxit=500)
+        ctx.sync()
+        if timing:
+            ctx.timing(True)
+        t0 = time.perf_counter()
+        its, _ = ctx.evolve_cn(d, args.warmup * dt, dt, args.steps, pulse, rtol=args.rtol, maxit=500)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        kms, nl = ctx.timing_stats() if timing else (0.0, 0)
+        ctx.timing(False)
+        out = d.download()
+        d.free()
+        return args.steps / el, its, out, kms, nl
+
+    base = None
+    if args.pc == "learned32" and not args.no_baseline:
+        sps_b, its_b, psi_b, _, _ = run(T.PC_MG, False)
+        base = {"preconditioner": PCS["mg"], "steps_per_s": sps_b, "ms_per_step": 1e3 / sps_b,
+                "krylov_its_per_step": float(np.mean(its_b))}
+        print(f"# baseline FP64 GMG: {sps_b:.2f} steps/s, its/step {its_b.tolist()}", file=sys.stderr, flush=True)
+    learn = None
+    if args.pc == "learned32":
+        tl = time.perf_counter()
+        l0, l1 = ctx.learn_train(psi, 0.0, dt, pulse, harvest_steps=2, max_samples=6, iters=args.train_iters,
+                                 lr=3e-3, rtol=args.rtol)
+        base_its, _, _ = ctx.learn_status()
+        learn = {"loss_physics_init": l0, "loss_trained": l1, "train_s": time.perf_counter() - tl,
+                 "its_without_network": base_its}
+        print(f"# trained: loss/sample {l0:.3e} -> {l1:.3e} in {learn['train_s']:.1f} s", file=sys.stderr, flush=True)
+    t_setup = time.perf_counter() - t_setup
     print(f"# setup {t_setup:.2f} s, N = {ctx.N}, levels = {ctx.levels}, pc = {args.pc}", file=sys.stderr, flush=True)
 
-    its_w, _ = ctx.evolve_cn(psi, 0.0, dt, args.warmup, pulse, rtol=args.rtol, maxit=500)
-    ctx.sync()
-    ctx.timing(True)
-    t0 = time.perf_counter()
-    its, _ = ctx.evolve_cn(psi, args.warmup * dt, dt, args.steps, pulse, rtol=args.rtol, maxit=500)
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    kms, nlaunch = ctx.timing_stats()
-    ctx.timing(False)
-    steps_per_s = args.steps / elapsed
+    steps_per_s, its, psi_end, kms, nlaunch = run(kinds[args.pc], True)
+    elapsed = args.steps / steps_per_s
     print(f"# timed {args.steps} steps in {elapsed:.3f} s; Krylov its/step {its.tolist()}", file=sys.stderr,
           flush=True)
+    if learn is not None:
+        _, fallbacks, active = ctx.learn_status()
+        learn.update({"fallbacks": fallbacks, "network_active": bool(active)})
+    if base is not None:
+        def mnorm(x):
+            dx, dy = ctx.vec(x), ctx.vec()
+            ctx.term(T.TERM_M, dx, dy)
+            v = float(np.sqrt(abs(np.vdot(x, dy.download()).real)))
+            dx.free()
+            dy.free()
+            return v
+        base["speedup"] = steps_per_s / base["steps_per_s"]
+        base["psi_rel_l2_diff"] = mnorm(psi_end - psi_b) / mnorm(psi_b)
 
     p, Q, nel = desc["p"], desc["nquad"], args.nel
     flops = pot_flops_per_element(p, Q) * nel ** 3
@@ -101,17 +161,20 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(desc, dt, pulse, args, psi.download())
+        cpu = cpu_baseline(desc, dt, pulse, args, x0)
 
     line = {
         "metric": "CN steps/s, 3D p=4 128^3 grid: learned+mixed PC vs plain FP64 MG at |r|<1e-10",
         "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "complex128 (fp64)", "data": "synthetic: lowest eigenstate of H (GPU inverse iteration), E0 = %.8f" % E0,
+        "vs_baseline": None, "dtype": "complex128 (fp64 operator/Krylov; complex64 + bf16 preconditioner)"
+        if args.pc in ("learned32", "sep32", "mg32", "corr32", "corrx32") else "complex128 (fp64)", "data": "synthetic: lowest eigenstate of H (GPU inverse iteration), E0 = %.8f" % E0,
         "config": {"workload": f"3D soft-core atom {nel}^3 elements p=4 Q=8, CN dt=0.02, rtol={args.rtol}",
-                   "preconditioner": {"sep": "fp64 separable (fast diagonalisation)", "mg": "plain fp64 GMG",
-                                      "none": "none"}[args.pc],
-                   "dofs": ctx.N, "mg_levels": ctx.levels, "krylov_its_per_step": float(np.mean(its))},
+                   "preconditioner": PCS[args.pc],
+                   "dofs": ctx.N, "mg_levels": ctx.levels, "krylov_its_per_step": float(np.mean(its)),
+                   "setup_s": t_setup},
+        "fp64_mg_baseline": base,
+        "learned": learn,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

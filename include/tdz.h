@@ -34,7 +34,14 @@ enum { TDZ_CAP_NONE = 0, TDZ_CAP_QUADRATIC = 1 /* cap[0] (cap[1]-s)^2 within cap
        TDZ_CAP_MANOLOPOULOS = 2 /* kmin = cap[0] (src/assembly/assembly.cpp:170-208) */ };
 /* Preconditioners. */
 enum { TDZ_PC_NONE = 0, TDZ_PC_SEPARABLE = 1 /* fast-diagonalisation inverse of alpha M + g (K + V_sep) */,
-       TDZ_PC_MG = 2 /* geometric B-spline multigrid V-cycle */ };
+       TDZ_PC_MG = 2 /* geometric B-spline multigrid V-cycle */,
+       TDZ_PC_SEPARABLE_MIXED = 3 /* the separable solve in complex64 */,
+       TDZ_PC_MG_MIXED = 4 /* the V-cycle in complex64: smoother, level operators, transfers */,
+       TDZ_PC_CORRECTED = 5 /* complex64 first-order corrected separable solve: S^-1 (r - dA S^-1 r), dA = the
+                               CAP and field terms exactly plus the potential lumped at the Greville points */,
+       TDZ_PC_CORRECTED_EXACT = 6 /* the same with the potential product by the complex64 matrix-free kernel */,
+       TDZ_PC_LEARNED = 7 /* TDZ_PC_CORRECTED plus the learned remainder of the potential product (BF16 MFMA
+                             CNN); falls back to plain multigrid when Krylov iterations regress */ };
 /* Single operator terms for tdz_term_apply. */
 enum { TDZ_TERM_M = 0, TDZ_TERM_K = 1, TDZ_TERM_V = 2, TDZ_TERM_CAP = 3, TDZ_TERM_DX = 4, TDZ_TERM_DY = 5,
        TDZ_TERM_DZ = 6 };
@@ -144,6 +151,33 @@ int tdz_evolve_rational(tdz_ctx *ctx, double *psi, double t0, double dt, int nst
 int tdz_observables(tdz_ctx *ctx, const double *psi, double out[6]);
 /* Field sE_a(t) = s E_a(t) of a pulse (TDSEZParser::Coupling, include/tdsez_parser.hpp:190-194). */
 int tdz_pulse_coupling(const tdz_ctx *ctx, const tdz_pulse *pulse, double t, double sE[3]);
+
+/* Learned correction (no reference counterpart: the reference has no learned
+ * component). The network input is u = [y, G_x, G_y, G_z] (4 complex vectors of
+ * N), y = S^-1 r the separable solve of the preconditioner's residual and
+ * G_d = X_d y - x_d .* M y its first moments about the Greville points; the target
+ * is t = V y - Vg .* M y (the part of the potential product the lumped form
+ * misses, by the matrix-free kernel). tdz_learn_train harvests (u, t) pairs from
+ * the solver's own preconditioned vectors during harvest_steps CN steps from psi
+ * (psi is not modified), then fits the network by Adam with the FP32
+ * forward/backward kernels; loss per sample |N(u) - t|^2 / |t|^2 (1 = the loss of
+ * a zero output), reported as the mean over the samples before and after the fit.
+ * It also records the Krylov iterations of the preconditioner without the network
+ * (the regression threshold of the fallback).
+ * tdz_learn_weights gets (set = 0) or sets (set = 1) W1 [27][4][16], W2 [27][16][16];
+ * tdz_learn_fields copies the positional fields [N][16]; tdz_learn_features
+ * computes u [4][N] (host, complex interleaved) for a device vector y;
+ * tdz_learn_eval runs the FP32 forward/backward and the BF16 MFMA forward on one
+ * host pair (u [4][N], t [N], complex interleaved) (host outputs: loss, dW1, dW2,
+ * out_f32 and out_mfma [2N] re then im). */
+int tdz_learn_train(tdz_ctx *ctx, const double *psi, double t0, double dt, const tdz_pulse *pulse, int harvest_steps,
+                    int max_samples, int iters, double lr, double rtol, double *loss_before, double *loss_after);
+int tdz_learn_weights(tdz_ctx *ctx, float *W1, float *W2, int set);
+int tdz_learn_fields(tdz_ctx *ctx, float *P);
+int tdz_learn_features(tdz_ctx *ctx, const double *y, double *u);
+int tdz_learn_eval(tdz_ctx *ctx, const double *u, const double *target, double *loss, float *dW1, float *dW2,
+                   double *out_f32, double *out_mfma);
+int tdz_learn_status(tdz_ctx *ctx, int *base_its, int *fallbacks, int *active);
 
 /* Measurement hooks (no reference counterpart; used by bench.py): HIP events
  * recorded on the context's stream around every launch of the fine-level

@@ -17,10 +17,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtdz.so")
 
-PC_NONE, PC_SEPARABLE, PC_MG = 0, 1, 2
+(PC_NONE, PC_SEPARABLE, PC_MG, PC_SEPARABLE_MIXED, PC_MG_MIXED, PC_CORRECTED, PC_CORRECTED_EXACT,
+ PC_LEARNED) = range(8)
 TERM_M, TERM_K, TERM_V, TERM_CAP, TERM_DX, TERM_DY, TERM_DZ = range(7)
 POT_NONE, POT_HARMONIC, POT_SOFTCORE = 0, 1, 2
 CAP_NONE, CAP_QUADRATIC, CAP_MANOLOPOULOS = 0, 1, 2
+CNN_INPUTS, CNN_CHANNELS = 4, 16   # network input fields (y and its first moments), hidden channels
 
 # every symbol include/tdz.h declares
 EXPORTS = [
@@ -30,6 +32,7 @@ EXPORTS = [
     "tdz_ksp_solve", "tdz_step_cn", "tdz_evolve_cn", "tdz_observables", "tdz_pulse_coupling",
     "tdz_timing", "tdz_timing_stats", "tdz_step_pade22", "tdz_evolve_rational",
     "tdz_project_gaussian", "tdz_ground_state",
+    "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
 ]
 
 
@@ -122,6 +125,13 @@ def lib():
         L.tdz_evolve_rational.argtypes = [vp, vp, d, d, i, C.POINTER(Pulse), i, d, i, dp, ip]
         L.tdz_project_gaussian.argtypes = [vp, dp, dp, dp, vp]
         L.tdz_ground_state.argtypes = [vp, vp, d, d, i, dp, dp, ip]
+        fp = C.POINTER(C.c_float)
+        L.tdz_learn_train.argtypes = [vp, vp, d, d, C.POINTER(Pulse), i, i, i, d, d, dp, dp]
+        L.tdz_learn_weights.argtypes = [vp, fp, fp, i]
+        L.tdz_learn_fields.argtypes = [vp, fp]
+        L.tdz_learn_features.argtypes = [vp, vp, dp]
+        L.tdz_learn_eval.argtypes = [vp, dp, dp, dp, fp, fp, dp, dp]
+        L.tdz_learn_status.argtypes = [vp, ip, ip, ip]
         L.tdz_timing.argtypes = [vp, i]
         L.tdz_timing_stats.argtypes = [vp, dp, C.POINTER(C.c_long)]
         _lib = L
@@ -257,6 +267,55 @@ class Context:
         E, r, it = C.c_double(0), C.c_double(0), C.c_int(0)
         _chk(lib().tdz_ground_state(self.h, psi.ptr, sigma, tol, maxit, C.byref(E), C.byref(r), C.byref(it)))
         return E.value, r.value, it.value
+
+    # ---- learned correction
+    def learn_train(self, psi, t0, dt, pulse=None, harvest_steps=2, max_samples=6, iters=200, lr=3e-3,
+                    rtol=1e-10):
+        l0, l1 = C.c_double(0), C.c_double(0)
+        _chk(lib().tdz_learn_train(self.h, psi.ptr, t0, dt, C.byref(pulse) if pulse is not None else None,
+                                   harvest_steps, max_samples, iters, lr, rtol, C.byref(l0), C.byref(l1)))
+        return l0.value, l1.value
+
+    def learn_weights(self, W1=None, W2=None):
+        fp = C.POINTER(C.c_float)
+        if W1 is None:
+            W1 = np.zeros((27, CNN_INPUTS, 16), np.float32)
+            W2 = np.zeros((27, 16, 16), np.float32)
+            _chk(lib().tdz_learn_weights(self.h, W1.ctypes.data_as(fp), W2.ctypes.data_as(fp), 0))
+            return W1, W2
+        W1 = np.ascontiguousarray(W1, np.float32)
+        W2 = np.ascontiguousarray(W2, np.float32)
+        _chk(lib().tdz_learn_weights(self.h, W1.ctypes.data_as(fp), W2.ctypes.data_as(fp), 1))
+
+    def learn_fields(self):
+        P = np.zeros((self.N, 16), np.float32)
+        _chk(lib().tdz_learn_fields(self.h, P.ctypes.data_as(C.POINTER(C.c_float))))
+        return P
+
+    def learn_features(self, y):
+        """Network input [y, G_x, G_y, G_z] (complex64 arithmetic) of a device vector y."""
+        u = np.zeros((CNN_INPUTS, self.N), np.complex128)
+        _chk(lib().tdz_learn_features(self.h, y.ptr, _dp(u.view(np.float64))))
+        return u
+
+    def learn_eval(self, u, target):
+        """Loss, weight gradients, FP32 and BF16-MFMA outputs for one (u [4][N], t [N]) pair."""
+        fp = C.POINTER(C.c_float)
+        loss = C.c_double(0)
+        u = np.ascontiguousarray(u, np.complex128).reshape(CNN_INPUTS, self.N)
+        t = np.ascontiguousarray(target, np.complex128).reshape(self.N)
+        dW1 = np.zeros((27, CNN_INPUTS, 16), np.float32)
+        dW2 = np.zeros((27, 16, 16), np.float32)
+        of = np.zeros(2 * self.N)
+        om = np.zeros(2 * self.N)
+        _chk(lib().tdz_learn_eval(self.h, _dp(u.view(np.float64)), _dp(t.view(np.float64)), C.byref(loss),
+                                  dW1.ctypes.data_as(fp), dW2.ctypes.data_as(fp), _dp(of), _dp(om)))
+        return loss.value, dW1, dW2, of, om
+
+    def learn_status(self):
+        b, f, a = C.c_int(0), C.c_int(0), C.c_int(0)
+        _chk(lib().tdz_learn_status(self.h, C.byref(b), C.byref(f), C.byref(a)))
+        return b.value, f.value, a.value
 
     def observables(self, psi):
         out = np.zeros(6)

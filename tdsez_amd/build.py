@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libtdz.so")
-SOURCES = ["tdz_kernels.hip", "tdz_api.cpp", "tdz_basis.cpp"]
+SOURCES = ["tdz_kernels.hip", "tdz_learn.hip", "tdz_api.cpp", "tdz_basis.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -28,7 +29,9 @@
 #include "tdz_internal.hpp"
 
 using cplx = std::complex<double>;
+using tdz::cpx;
 using tdz::dcomplex;
+using tdz::fcomplex;
 
 namespace {
 
@@ -55,38 +58,94 @@ T *dalloc(size_t n)
     return static_cast<T *>(p);
 }
 
-// Sparse 1D matrix in CSR on device.
+// Sparse 1D matrix in CSR on device (values in both precisions).
 struct CSR {
     int rows = 0;
     int *rp = nullptr, *col = nullptr;
     double *val = nullptr;
+    float *valf = nullptr;
+};
+
+// Device data of one level in one precision.
+template <typename R>
+struct Prec {
+    R *B0[3] = {}, *X[3] = {}, *W[3] = {};   // basis tables, points, weights
+    R *M[3] = {};                            // real mass bands
+    cpx<R> *B[3] = {};                       // complex combined bands (current shift)
+    cpx<R> *Bmass[3] = {};                   // bands of M alone (M on x, zero on y and z)
+    R *V[3] = {}, *lam[3] = {};              // separable preconditioner
+    cpx<R> *r = nullptr, *z = nullptr, *t = nullptr;   // multigrid work vectors
+    bool ready = false;
 };
 
 struct Level {
     tdz::Axis1D ax[3];
     int n[3] = {1, 1, 1};
     long N = 1;
-    // device tables
-    double *dB0[3] = {}, *dX[3] = {}, *dW[3] = {};
     int *dOff[3] = {};
-    double *dM[3] = {};
-    dcomplex *dB[3] = {};
-    dcomplex *dBmass[3] = {};   // bands of M alone (M on x, zero on y and z)
     unsigned char *dWall[3] = {};
     // host bands of this level (Galerkin on coarse levels)
     std::vector<double> M[3], S[3], Xb[3], Wc[3], Vs[3];
-    // separable preconditioner
-    double *dV[3] = {}, *dLam[3] = {};
+    // host eigen-decomposition of the separable preconditioner
+    std::vector<double> hV[3], hLam[3];
     bool fd_ready = false;
+    Prec<double> d;
+    Prec<float> f;
     // transfer to the next coarser level: P (this x coarse) and R = P^T
     CSR P[3], R[3];
-    // multigrid work vectors (coarse levels: r, z; all: t)
-    dcomplex *r = nullptr, *z = nullptr, *t = nullptr;
     // dim 3 with element e starting at function e on every axis (tiled kernel applies)
     bool simple = false;
     int pot_variant = -1;   // tile shape of the tiled kernel picked by the autotuner (-1: generic)
 };
 
+template <typename R> Prec<R> &prec(Level &L);
+template <> Prec<double> &prec<double>(Level &L) { return L.d; }
+template <> Prec<float> &prec<float>(Level &L) { return L.f; }
+
+template <typename R> const R *csr_val(const CSR &s);
+template <> const double *csr_val<double>(const CSR &s) { return s.val; }
+template <> const float *csr_val<float>(const CSR &s) { return s.valf; }
+
+// fine-level scratch of one precision
+template <typename R>
+struct Scratch {
+    cpx<R> *tmp[4] = {};
+    cpx<R> *w1 = nullptr, *w2 = nullptr;
+    cpx<R> *u1 = nullptr, *u2 = nullptr;
+};
+
+}  // namespace
+
+namespace {
+// The learned part of the preconditioner: a small 3D CNN that reproduces the part of
+// the potential product V y the lumped form Vg .* M y misses (see tdz_learn.hip).
+struct LearnState {
+    bool ready = false;
+    float *P = nullptr;                  // [N][C] positional fields
+    std::vector<double> fscale;          // [C] maximum of each field before scaling to 1
+    float *W1 = nullptr, *W2 = nullptr;  // device weights [27][CI][C], [27][C][C]
+    float *gr[3] = {};                   // Greville coordinates per axis (float)
+    fcomplex *Xc[3] = {}, *Zc[3] = {};   // position band and zero band per axis (complex64)
+    fcomplex *feat = nullptr;            // [CI][N] network input: y and its first-moment features
+    tdz::CnnScales sin{{1.f, 1.f, 1.f, 1.f}};   // input scales (unit mean square over the samples)
+    double sout = 1.0;                   // target scale: the network outputs sout * t
+    std::vector<float> hW1, hW2;         // host weights
+    unsigned short *mb = nullptr;        // inference work [2N][C] bf16
+    float *out = nullptr;                // [2N]
+    fcomplex *nvec = nullptr;            // g * network output
+    // training
+    float *m = nullptr, *dov = nullptr, *dh = nullptr, *partial = nullptr, *dW1 = nullptr, *dW2 = nullptr;
+    float *lpart = nullptr;              // loss partial sums
+    std::vector<fcomplex *> ys, ts;      // harvested samples: network input [CI][N] and target t
+    std::vector<double> tnorm2;          // |t|^2 per sample
+    int max_samples = 0;
+    bool harvesting = false;
+    bool trained = false;
+    int base_its = -1;    // iterations of the same preconditioner without the network
+    int fallbacks = 0;
+    bool fallen = false;  // the network regressed: the fallback preconditioner is in use
+    int fallback_kind = TDZ_PC_MG;
+};
 }  // namespace
 
 struct tdz_ctx {
@@ -96,8 +155,10 @@ struct tdz_ctx {
     std::vector<std::unique_ptr<Level>> lv;
     cplx alpha = 1.0, g = 0.0;
     double sE[3] = {0, 0, 0};
-    dcomplex *tmp[4] = {};
-    dcomplex *w1 = nullptr, *w2 = nullptr, *rhs = nullptr;
+    Scratch<double> sd;
+    Scratch<float> sf;
+    bool mixed_ready = false;   // single-precision copies allocated
+    dcomplex *rhs = nullptr;
     int restart = 30;
     dcomplex *Vb = nullptr, *Zb = nullptr;
     dcomplex *dpart = nullptr, *dh = nullptr, *dh2 = nullptr;
@@ -113,6 +174,11 @@ struct tdz_ctx {
     int hist_count[4] = {0, 0, 0, 0};
     dcomplex *xin = nullptr;    // stage input copy
     bool use_tiled = true;   // tiled 3D potential kernel where it applies
+    // first-order corrected separable preconditioner: the non-separable part of s V at the
+    // Greville points (lumped product) and the bands of g (CAP + sE.D), fine level, complex64
+    float *dVg = nullptr;
+    fcomplex *dBcorr[3] = {};
+    LearnState learn;
     // kernel timing (measurement hook)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -121,11 +187,24 @@ struct tdz_ctx {
 
 namespace {
 
+template <typename R> Scratch<R> &scratch(tdz_ctx *c);
+template <> Scratch<double> &scratch<double>(tdz_ctx *c) { return c->sd; }
+template <> Scratch<float> &scratch<float>(tdz_ctx *c) { return c->sf; }
+
 template <typename T>
 T *track(tdz_ctx *c, T *p)
 {
     c->allocs.push_back(p);
     return p;
+}
+
+// free a tracked allocation before the context ends
+void untrack(tdz_ctx *c, void *p)
+{
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+    if (it == c->allocs.end()) return;
+    c->allocs.erase(it);
+    HIPCHK(hipFree(p));
 }
 
 template <typename T>
@@ -135,6 +214,8 @@ T *upload(tdz_ctx *c, const std::vector<T> &h)
     if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     return d;
 }
+
+std::vector<float> to_float(const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); }
 
 // B = P^T A P for a band A (fine level l) and sparse P (fine x coarse), returned as a coarse band.
 std::vector<double> galerkin_band(const std::vector<double> &A, int nf, int p,
@@ -170,26 +251,66 @@ CSR make_csr(tdz_ctx *c, const std::vector<std::vector<std::pair<int, double>>> 
     s.rp = upload(c, rp);
     s.col = upload(c, col);
     s.val = upload(c, val);
+    s.valf = upload(c, to_float(val));
     return s;
+}
+
+std::vector<dcomplex> mass_only_band(const Level &L, int d)
+{
+    std::vector<dcomplex> bm(L.M[d].size(), dcomplex{0.0, 0.0});
+    if (d == 0)
+        for (size_t k = 0; k < bm.size(); ++k) bm[k] = {L.M[d][k], 0.0};
+    return bm;
 }
 
 void upload_level_tables(tdz_ctx *c, Level &L)
 {
     for (int d = 0; d < 3; ++d) {
         const tdz::Axis1D &A = L.ax[d];
-        L.dB0[d] = upload(c, A.B0);
-        L.dX[d] = upload(c, A.X);
-        L.dW[d] = upload(c, A.W);
+        L.d.B0[d] = upload(c, A.B0);
+        L.d.X[d] = upload(c, A.X);
+        L.d.W[d] = upload(c, A.W);
         L.dOff[d] = upload(c, A.off);
-        L.dM[d] = upload(c, L.M[d]);
+        L.d.M[d] = upload(c, L.M[d]);
         std::vector<unsigned char> w(A.isWall.begin(), A.isWall.end());
         L.dWall[d] = upload(c, w);
-        L.dB[d] = track(c, dalloc<dcomplex>(L.M[d].size()));
-        std::vector<dcomplex> bm(L.M[d].size(), dcomplex{0.0, 0.0});
-        if (d == 0)
-            for (size_t k = 0; k < bm.size(); ++k) bm[k] = {L.M[d][k], 0.0};
-        L.dBmass[d] = upload(c, bm);
+        L.d.B[d] = track(c, dalloc<dcomplex>(L.M[d].size()));
+        L.d.Bmass[d] = upload(c, mass_only_band(L, d));
     }
+    L.d.t = track(c, dalloc<dcomplex>(L.N));
+    if (&L != c->lv[0].get()) {
+        L.d.r = track(c, dalloc<dcomplex>(L.N));
+        L.d.z = track(c, dalloc<dcomplex>(L.N));
+    }
+    L.d.ready = true;
+}
+
+// Single-precision copies of one level (tables, bands, work vectors), for the
+// mixed-precision preconditioners. The separable preconditioner's eigenvectors
+// are copied by setup_fd.
+void upload_level_tables_f(tdz_ctx *c, Level &L)
+{
+    if (L.f.ready) return;
+    for (int d = 0; d < 3; ++d) {
+        const tdz::Axis1D &A = L.ax[d];
+        L.f.B0[d] = upload(c, to_float(A.B0));
+        L.f.X[d] = upload(c, to_float(A.X));
+        L.f.W[d] = upload(c, to_float(A.W));
+        L.f.M[d] = upload(c, to_float(L.M[d]));
+        L.f.B[d] = track(c, dalloc<fcomplex>(L.M[d].size()));
+        std::vector<dcomplex> bm = mass_only_band(L, d);
+        std::vector<fcomplex> bf(bm.size());
+        for (size_t k = 0; k < bm.size(); ++k) bf[k] = {(float)bm[k].re, (float)bm[k].im};
+        L.f.Bmass[d] = upload(c, bf);
+        if (L.fd_ready) {
+            L.f.V[d] = upload(c, to_float(L.hV[d]));
+            L.f.lam[d] = upload(c, to_float(L.hLam[d]));
+        }
+    }
+    L.f.t = track(c, dalloc<fcomplex>(L.N));
+    L.f.r = track(c, dalloc<fcomplex>(L.N));
+    L.f.z = track(c, dalloc<fcomplex>(L.N));
+    L.f.ready = true;
 }
 
 // Complex bands of alpha M + g (K + CAP + sE.D) split over the three axes:
@@ -208,35 +329,64 @@ void update_bands(tdz_ctx *c, Level &L)
             if (d == 0) b += c->alpha * L.M[d][k];
             B[k] = dc(b);
         }
-        HIPCHK(hipMemcpyAsync(L.dB[d], B.data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(L.d.B[d], B.data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+        if (c->dBcorr[d] && &L == c->lv[0].get()) {
+            std::vector<fcomplex> Bc(sz, fcomplex{0.f, 0.f});
+            if (active)
+                for (size_t k = 0; k < sz; ++k) {
+                    const cplx b = c->g * ((c->cap_on ? cplx(0, -1) * L.Wc[d][k] : 0.0) + c->sE[d] * L.Xb[d][k]);
+                    Bc[k] = {(float)b.real(), (float)b.imag()};
+                }
+            HIPCHK(hipMemcpyAsync(c->dBcorr[d], Bc.data(), sz * sizeof(fcomplex), hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
+        if (L.f.ready) {
+            std::vector<fcomplex> Bf(sz);
+            for (size_t k = 0; k < sz; ++k) Bf[k] = {(float)B[k].re, (float)B[k].im};
+            HIPCHK(hipMemcpyAsync(L.f.B[d], Bf.data(), sz * sizeof(fcomplex), hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
     }
     HIPCHK(hipStreamSynchronize(c->st));
 }
 
-tdz::AxisDev axis_dev(const Level &L, int d)
+void set_shift_bands(tdz_ctx *c)
 {
-    const tdz::Axis1D &A = L.ax[d];
-    return {A.E, A.p, A.Q, A.n, L.dB0[d], L.dX[d], L.dW[d], L.dOff[d]};
+    for (auto &L : c->lv) update_bands(c, *L);
 }
 
-tdz::SepBands sep_bands(const Level &L)
+template <typename R>
+tdz::AxisDevT<R> axis_dev(Level &L, int d)
 {
-    tdz::SepBands sb;
+    const tdz::Axis1D &A = L.ax[d];
+    Prec<R> &P = prec<R>(L);
+    return {A.E, A.p, A.Q, A.n, P.B0[d], P.X[d], P.W[d], L.dOff[d]};
+}
+
+template <typename R>
+tdz::SepBandsT<R> sep_bands(Level &L)
+{
+    tdz::SepBandsT<R> sb;
+    Prec<R> &P = prec<R>(L);
     for (int d = 0; d < 3; ++d) {
         sb.n[d] = L.n[d];
         sb.p[d] = L.ax[d].p;
-        sb.M[d] = L.dM[d];
-        sb.B[d] = L.dB[d];
+        sb.M[d] = P.M[d];
+        sb.B[d] = P.B[d];
     }
     return sb;
 }
 
+template <typename R>
+inline cpx<R> cx(cplx z) { return {(R)z.real(), (R)z.imag()}; }
+
 // y += g (s V) x on level L: the tiled 3D kernel where it applies, else the generic one.
-void pot_apply(tdz_ctx *c, Level &L, cplx g, const dcomplex *x, dcomplex *y)
+template <typename R>
+void pot_apply(tdz_ctx *c, Level &L, cplx g, const cpx<R> *x, cpx<R> *y)
 {
-    tdz::AxisDev ax[3] = {axis_dev(L, 0), axis_dev(L, 1), axis_dev(L, 2)};
+    tdz::AxisDevT<R> ax[3] = {axis_dev<R>(L, 0), axis_dev<R>(L, 1), axis_dev<R>(L, 2)};
     tdz::PotParams pp{c->ph.pot_kind, c->ph.s, c->ph.pot[0], c->ph.pot[1]};
-    const bool timed = c->timing && &L == c->lv[0].get();
+    const bool timed = std::is_same<R, double>::value && c->timing && &L == c->lv[0].get();
     if (timed) {
         if (c->ev_used == c->ev.size()) {
             hipEvent_t a, b;
@@ -247,19 +397,20 @@ void pot_apply(tdz_ctx *c, Level &L, cplx g, const dcomplex *x, dcomplex *y)
         HIPCHK(hipEventRecord(c->ev[c->ev_used].first, c->st));
     }
     if (!(L.simple && c->use_tiled &&
-          tdz::launch_pot_apply_3d(c->st, ax, pp, g.real(), g.imag(), x, y, L.pot_variant)))
-        tdz::launch_pot_apply_generic(c->st, ax, pp, g.real(), g.imag(), x, y);
+          tdz::launch_pot_apply_3d<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y, L.pot_variant)))
+        tdz::launch_pot_apply_generic<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y);
     if (timed) HIPCHK(hipEventRecord(c->ev[c->ev_used++].second, c->st));
 }
 
 // y = A_l x with the current shift (separable bands + matrix-free potential).
-void level_apply(tdz_ctx *c, Level &L, const dcomplex *x, dcomplex *y)
+template <typename R>
+void level_apply(tdz_ctx *c, Level &L, const cpx<R> *x, cpx<R> *y)
 {
-    tdz::launch_sep_apply(c->st, sep_bands(L), x, y, c->tmp);
-    if (c->ph.pot_kind && c->g != 0.0) pot_apply(c, L, c->g, x, y);
+    tdz::launch_sep_apply<R>(c->st, sep_bands<R>(L), x, y, scratch<R>(c).tmp);
+    if (c->ph.pot_kind && c->g != 0.0) pot_apply<R>(c, L, c->g, x, y);
     if (c->prob.dirichlet) {
         const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-        tdz::launch_wall_fix(c->st, L.n, w, x, y);
+        tdz::launch_wall_fix<R>(c->st, L.n, w, x, y);
     }
 }
 
@@ -284,13 +435,13 @@ void autotune_pot(tdz_ctx *c, Level &L)
     HIPCHK(hipEventCreate(&b));
     HIPCHK(hipMemsetAsync(x, 0, L.N * sizeof(dcomplex), c->st));
     HIPCHK(hipMemsetAsync(y, 0, L.N * sizeof(dcomplex), c->st));
-    tdz::AxisDev ax[3] = {axis_dev(L, 0), axis_dev(L, 1), axis_dev(L, 2)};
+    tdz::AxisDev ax[3] = {axis_dev<double>(L, 0), axis_dev<double>(L, 1), axis_dev<double>(L, 2)};
     tdz::PotParams pp{c->ph.pot_kind, c->ph.s, c->ph.pot[0], c->ph.pot[1]};
     float best = 1e30f;
     for (int v = 0; v < nv; ++v) {
-        tdz::launch_pot_apply_3d(c->st, ax, pp, 1.0, 0.0, x, y, v);
+        tdz::launch_pot_apply_3d<double>(c->st, ax, pp, 1.0, 0.0, x, y, v);
         HIPCHK(hipEventRecord(a, c->st));
-        for (int r = 0; r < 2; ++r) tdz::launch_pot_apply_3d(c->st, ax, pp, 1.0, 0.0, x, y, v);
+        for (int r = 0; r < 2; ++r) tdz::launch_pot_apply_3d<double>(c->st, ax, pp, 1.0, 0.0, x, y, v);
         HIPCHK(hipEventRecord(b, c->st));
         HIPCHK(hipEventSynchronize(b));
         float ms = 0;
@@ -320,77 +471,252 @@ void setup_fd(tdz_ctx *c, Level &L)
             V = {1.0};
             lam = {0.0};
         }
-        L.dV[d] = upload(c, V);
-        L.dLam[d] = upload(c, lam);
+        L.hV[d] = V;
+        L.hLam[d] = lam;
+        L.d.V[d] = upload(c, V);
+        L.d.lam[d] = upload(c, lam);
+        if (L.f.ready) {
+            L.f.V[d] = upload(c, to_float(V));
+            L.f.lam[d] = upload(c, to_float(lam));
+        }
     }
     L.fd_ready = true;
 }
 
 // z = (alpha M + g (K + V_sep))^-1 r by fast diagonalisation (walls: identity);
 // V_sep is the part of s V that is a sum of 1D terms. s1, s2: scratch.
-void fd_apply(tdz_ctx *c, Level &L, const dcomplex *r, dcomplex *z, dcomplex *s1, dcomplex *s2)
+template <typename R>
+void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<R> *s2)
 {
-    tdz::launch_dense_axis(c->st, 0, L.n, L.dV[0], true, r, s1);
-    tdz::launch_dense_axis(c->st, 1, L.n, L.dV[1], true, s1, s2);
-    tdz::launch_dense_axis(c->st, 2, L.n, L.dV[2], true, s2, s1);
-    const double *lam[3] = {L.dLam[0], L.dLam[1], L.dLam[2]};
+    Prec<R> &P = prec<R>(L);
+    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1);
+    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2);
+    tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], true, s2, s1);
+    const R *lam[3] = {P.lam[0], P.lam[1], P.lam[2]};
     const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-    tdz::launch_diag_scale(c->st, L.n, lam, w, dc(c->alpha), dc(c->g), s1, s2);
-    tdz::launch_dense_axis(c->st, 0, L.n, L.dV[0], false, s2, s1);
-    tdz::launch_dense_axis(c->st, 1, L.n, L.dV[1], false, s1, s2);
-    tdz::launch_dense_axis(c->st, 2, L.n, L.dV[2], false, s2, z);
-    if (c->prob.dirichlet) tdz::launch_wall_fix(c->st, L.n, w, r, z);
+    tdz::launch_diag_scale<R>(c->st, L.n, lam, w, cx<R>(c->alpha), cx<R>(c->g), s1, s2);
+    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, s2, s1);
+    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, s1, s2);
+    tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], false, s2, z);
+    if (c->prob.dirichlet) tdz::launch_wall_fix<R>(c->st, L.n, w, r, z);
 }
 
 // z = restriction-to-coarse / prolongation along three axes (CSR passes)
-void transfer(tdz_ctx *c, const CSR ops[3], const int nin[3], const int nout[3], const dcomplex *in, dcomplex *out,
-              dcomplex *s1, dcomplex *s2, bool accumulate)
+template <typename R>
+void transfer(tdz_ctx *c, const CSR ops[3], const int nin[3], const int nout[3], const cpx<R> *in, cpx<R> *out,
+              cpx<R> *s1, cpx<R> *s2, bool accumulate)
 {
     int a[3] = {nin[0], nin[1], nin[2]};
-    tdz::launch_csr_axis(c->st, 0, a, nout[0], ops[0].rp, ops[0].col, ops[0].val, in, s1, false);
+    tdz::launch_csr_axis<R>(c->st, 0, a, nout[0], ops[0].rp, ops[0].col, csr_val<R>(ops[0]), in, s1, false);
     a[0] = nout[0];
-    tdz::launch_csr_axis(c->st, 1, a, nout[1], ops[1].rp, ops[1].col, ops[1].val, s1, s2, false);
+    tdz::launch_csr_axis<R>(c->st, 1, a, nout[1], ops[1].rp, ops[1].col, csr_val<R>(ops[1]), s1, s2, false);
     a[1] = nout[1];
-    tdz::launch_csr_axis(c->st, 2, a, nout[2], ops[2].rp, ops[2].col, ops[2].val, s2, out, accumulate);
+    tdz::launch_csr_axis<R>(c->st, 2, a, nout[2], ops[2].rp, ops[2].col, csr_val<R>(ops[2]), s2, out, accumulate);
 }
 
-// Richardson sweep z += S^-1 (r - A z) on level L (resid in L.t)
-void smooth(tdz_ctx *c, Level &L, const dcomplex *r, dcomplex *z, dcomplex *s1, dcomplex *s2, dcomplex *s3)
+// Richardson sweep z += S^-1 (r - A z) on level L (residual in the level's t)
+template <typename R>
+void smooth(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<R> *s2, cpx<R> *s3)
 {
-    level_apply(c, L, z, L.t);
-    tdz::launch_axpby(c->st, {1, 0}, r, {-1, 0}, L.t, L.N);  // t = r - A z
-    fd_apply(c, L, L.t, s3, s1, s2);
-    tdz::launch_axpby(c->st, {1, 0}, s3, {1, 0}, z, L.N);   // z += S^-1 t
+    cpx<R> *t = prec<R>(L).t;
+    const cpx<R> one{1, 0}, mone{-1, 0};
+    level_apply<R>(c, L, z, t);
+    tdz::launch_axpby_t<R>(c->st, one, r, mone, t, L.N);  // t = r - A z
+    fd_apply<R>(c, L, t, s3, s1, s2);
+    tdz::launch_axpby_t<R>(c->st, one, s3, one, z, L.N);   // z += S^-1 t
 }
 
-// V-cycle z = B_l r on level l (z is overwritten).
-void vcycle(tdz_ctx *c, int l, const dcomplex *r, dcomplex *z)
+// V-cycle z = B_l r on level l (z is overwritten), in precision R.
+template <typename R>
+void vcycle(tdz_ctx *c, int l, const cpx<R> *r, cpx<R> *z)
 {
     Level &L = *c->lv[l];
     const int nu = std::max(1, c->prob.mg_smooth);
     // scratch: w1, w2 (fine-level sized) for the transforms; tmp[3] holds a smoother
     // correction (level_apply, the only other user of tmp, has finished by then)
-    dcomplex *s1 = c->w1, *s2 = c->w2;
-    fd_apply(c, L, r, z, s1, s2);
+    Scratch<R> &S = scratch<R>(c);
+    cpx<R> *s1 = S.w1, *s2 = S.w2;
+    const cpx<R> one{1, 0}, mone{-1, 0};
+    fd_apply<R>(c, L, r, z, s1, s2);
     const bool coarsest = l + 1 == (int)c->lv.size();
     const int pre = coarsest ? nu + 1 : nu;
-    for (int s = 1; s < pre; ++s) smooth(c, L, r, z, s1, s2, c->tmp[3]);
+    for (int s = 1; s < pre; ++s) smooth<R>(c, L, r, z, s1, s2, S.tmp[3]);
     if (coarsest) return;
     Level &Cl = *c->lv[l + 1];
-    level_apply(c, L, z, L.t);
-    tdz::launch_axpby(c->st, {1, 0}, r, {-1, 0}, L.t, L.N);      // t = r - A z
-    transfer(c, L.R, L.n, Cl.n, L.t, Cl.r, s1, s2, false);         // r_c = P^T t
-    vcycle(c, l + 1, Cl.r, Cl.z);
-    transfer(c, L.P, Cl.n, L.n, Cl.z, z, s1, s2, true);            // z += P z_c
-    for (int s = 0; s < nu; ++s) smooth(c, L, r, z, s1, s2, c->tmp[3]);
+    cpx<R> *t = prec<R>(L).t;
+    level_apply<R>(c, L, z, t);
+    tdz::launch_axpby_t<R>(c->st, one, r, mone, t, L.N);                   // t = r - A z
+    transfer<R>(c, L.R, L.n, Cl.n, t, prec<R>(Cl).r, s1, s2, false);       // r_c = P^T t
+    vcycle<R>(c, l + 1, prec<R>(Cl).r, prec<R>(Cl).z);
+    transfer<R>(c, L.P, Cl.n, L.n, prec<R>(Cl).z, z, s1, s2, true);        // z += P z_c
+    for (int s = 0; s < nu; ++s) smooth<R>(c, L, r, z, s1, s2, S.tmp[3]);
+}
+
+// Single-precision data for the mixed-precision preconditioners.
+void setup_mixed(tdz_ctx *c)
+{
+    if (c->mixed_ready) return;
+    const long N = c->lv[0]->N;
+    for (int i = 0; i < 4; ++i) c->sf.tmp[i] = track(c, dalloc<fcomplex>(N));
+    c->sf.w1 = track(c, dalloc<fcomplex>(N));
+    c->sf.w2 = track(c, dalloc<fcomplex>(N));
+    for (auto &L : c->lv) upload_level_tables_f(c, *L);
+    c->mixed_ready = true;
+    set_shift_bands(c);
+}
+
+// Lumped non-separable potential at the Greville points (s V minus its separable part)
+// and the correction bands, for the first-order corrected preconditioner.
+void setup_corrected(tdz_ctx *c)
+{
+    if (c->dVg) return;
+    Level &L = *c->lv[0];
+    std::vector<double> gr[3];
+    for (int d = 0; d < 3; ++d)
+        gr[d] = d < c->prob.dim ? tdz::greville(L.ax[d].t, L.ax[d].p) : std::vector<double>{0.0};
+    std::vector<float> vg(L.N, 0.f);
+    if (c->ph.pot_kind == 2)
+        for (int k = 0; k < L.n[2]; ++k)
+            for (int j = 0; j < L.n[1]; ++j)
+                for (int i = 0; i < L.n[0]; ++i) {
+                    const double r2 = gr[0][i] * gr[0][i] + gr[1][j] * gr[1][j] + gr[2][k] * gr[2][k];
+                    vg[i + (size_t)L.n[0] * (j + (size_t)L.n[1] * k)] =
+                        (float)(-c->ph.s * c->ph.pot[0] / std::sqrt(r2 + c->ph.pot[1]));
+                }
+    c->dVg = upload(c, vg);
+    for (int d = 0; d < 3; ++d) c->dBcorr[d] = track(c, dalloc<fcomplex>(L.M[d].size()));
+    const long N = L.N;
+    c->sf.u1 = track(c, dalloc<fcomplex>(N));
+    c->sf.u2 = track(c, dalloc<fcomplex>(N));
+    set_shift_bands(c);
+}
+
+// z ~ A^-1 r by the first-order corrected separable solve, complex64:
+//   y = S^-1 r,  z = S^-1 (r - g Vg .* M y - g (CAP + sE.D) y - n(y))
+// S = alpha M + g (K + V_sep) (fast diagonalisation); n(y) is the learned remainder
+// of the potential product (null: lumped product only).
+// Size of the terms the corrected solve treats as a perturbation of S, relative to
+// S >= M: |g| (max W + sum_a |sE_a| max|x_a| + max|s V - V_sep|). The first-order
+// correction only helps while this is well below 1.
+double correction_bound(tdz_ctx *c)
+{
+    Level &L = *c->lv[0];
+    double w = 0, f = 0;
+    for (int d = 0; d < c->prob.dim; ++d) {
+        double wm = 0;
+        for (double v : L.ax[d].cap) wm = std::max(wm, std::fabs(v));
+        if (c->cap_on) w += wm;
+        f += std::fabs(c->sE[d]) * std::max(std::fabs(L.ax[d].lo), std::fabs(L.ax[d].hi));
+    }
+    double v = 0;
+    if (c->ph.pot_kind == 2) v = std::fabs(c->ph.pot[0]) / std::sqrt(std::max(c->ph.pot[1], 1e-300));
+    return std::abs(c->g) * (w + f + v);
+}
+
+// Network input on the fine level: feat_0 = y, feat_{1+d} = X_d y - x_d .* M y (first
+// moments of y about each Greville point along axis d; X_d the position band on axis d,
+// mass bands on the others). My = M y is given.
+void learned_features(tdz_ctx *c, const fcomplex *y, const fcomplex *My, fcomplex *feat)
+{
+    Level &L = *c->lv[0];
+    LearnState &ls = c->learn;
+    const long N = L.N;
+    HIPCHK(hipMemcpyAsync(feat, y, N * sizeof(fcomplex), hipMemcpyDeviceToDevice, c->st));
+    for (int d = 0; d < 3; ++d) {
+        fcomplex *G = feat + (size_t)(1 + d) * N;
+        if (d >= c->prob.dim) {
+            HIPCHK(hipMemsetAsync(G, 0, N * sizeof(fcomplex), c->st));
+            continue;
+        }
+        tdz::SepBandsT<float> xb = sep_bands<float>(L);
+        for (int e = 0; e < 3; ++e) xb.B[e] = e == d ? ls.Xc[e] : ls.Zc[e];
+        tdz::launch_sep_apply<float>(c->st, xb, y, G, c->sf.tmp);
+        tdz::moment_feature(c->st, L.n, d, ls.gr[d], My, G);
+    }
+}
+
+void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, const fcomplex *n, bool exact_v = false,
+                     bool use_net = false)
+{
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    tdz::launch_demote(c->st, r, L.f.r, N);
+    if (correction_bound(c) > 0.25) {   // perturbation too large: the separable solve alone
+        fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);
+        tdz::launch_promote(c->st, L.f.z, z, N);
+        return;
+    }
+    fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);                // y
+    tdz::SepBandsT<float> mb = sep_bands<float>(L);
+    for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
+    tdz::launch_sep_apply<float>(c->st, mb, L.f.z, L.f.t, c->sf.tmp);       // M y
+    tdz::SepBandsT<float> cb = sep_bands<float>(L);
+    for (int d = 0; d < 3; ++d) cb.B[d] = c->dBcorr[d];
+    tdz::launch_sep_apply<float>(c->st, cb, L.f.z, c->sf.u1, c->sf.tmp);    // g (CAP + sE.D) y
+    if (exact_v && c->ph.pot_kind) {   // potential product by the matrix-free kernel (reference for the learned one)
+        HIPCHK(hipMemsetAsync(c->sf.u2, 0, N * sizeof(fcomplex), c->st));
+        pot_apply<float>(c, L, c->g, L.f.z, c->sf.u2);
+        HIPCHK(hipMemsetAsync(L.f.t, 0, N * sizeof(fcomplex), c->st));
+        n = c->sf.u2;
+    }
+    LearnState &ls = c->learn;
+    const bool harvest = ls.harvesting && (int)ls.ys.size() < ls.max_samples && c->ph.pot_kind;
+    if (harvest || (use_net && ls.trained)) learned_features(c, L.f.z, L.f.t, ls.feat);
+    if (harvest) {
+        // sample: input features of y = S^-1 r, target t = V y - Vg .* M y (matrix-free potential product)
+        constexpr int CI = tdz::kCnnInputs;
+        fcomplex *ys = track(c, dalloc<fcomplex>((size_t)CI * N)), *ts = track(c, dalloc<fcomplex>(N));
+        HIPCHK(hipMemcpyAsync(ys, ls.feat, CI * N * sizeof(fcomplex), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemsetAsync(c->sf.u2, 0, N * sizeof(fcomplex), c->st));
+        pot_apply<float>(c, L, 1.0, L.f.z, c->sf.u2);
+        tdz::lumped_residual(c->st, N, c->sf.u2, L.f.t, c->dVg, ts);
+        std::vector<fcomplex> h(N);
+        HIPCHK(hipMemcpyAsync(h.data(), ts, N * sizeof(fcomplex), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        double t2 = 0;
+        for (const auto &v : h) t2 += (double)v.re * v.re + (double)v.im * v.im;
+        ls.ys.push_back(ys);
+        ls.ts.push_back(ts);
+        ls.tnorm2.push_back(t2);
+    }
+    if (use_net && ls.trained) {
+        const int n3[3] = {L.n[0], L.n[1], L.n[2]};
+        tdz::scale_inputs(c->st, N, ls.sin, ls.feat);
+        tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.P, ls.mb, ls.out, cx<float>(c->g / ls.sout),
+                            ls.nvec);
+        n = ls.nvec;
+    }
+    tdz::launch_corr_combine(c->st, L.f.r, L.f.t, c->sf.u1, n, c->dVg, cx<float>(c->g), c->sf.u1, N);
+    fd_apply<float>(c, L, c->sf.u1, L.f.z, c->sf.w1, c->sf.w2);
+    tdz::launch_promote(c->st, L.f.z, z, N);
 }
 
 void pc_apply_dev(tdz_ctx *c, const dcomplex *r, dcomplex *z)
 {
     Level &L = *c->lv[0];
     switch (c->pc_kind) {
-    case TDZ_PC_SEPARABLE: fd_apply(c, L, r, z, c->w1, c->w2); break;
-    case TDZ_PC_MG: vcycle(c, 0, r, z); break;
+    case TDZ_PC_SEPARABLE: fd_apply<double>(c, L, r, z, c->sd.w1, c->sd.w2); break;
+    case TDZ_PC_MG: vcycle<double>(c, 0, r, z); break;
+    case TDZ_PC_SEPARABLE_MIXED:
+        tdz::launch_demote(c->st, r, L.f.r, L.N);
+        fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);
+        tdz::launch_promote(c->st, L.f.z, z, L.N);
+        break;
+    case TDZ_PC_CORRECTED: corrected_apply(c, r, z, nullptr); break;
+    case TDZ_PC_CORRECTED_EXACT: corrected_apply(c, r, z, nullptr, true); break;
+    case TDZ_PC_LEARNED:
+        if (c->learn.fallen) {
+            if (c->learn.fallback_kind == TDZ_PC_MG) vcycle<double>(c, 0, r, z);
+            else fd_apply<double>(c, L, r, z, c->sd.w1, c->sd.w2);
+        } else {
+            corrected_apply(c, r, z, nullptr, false, true);
+        }
+        break;
+    case TDZ_PC_MG_MIXED:
+        tdz::launch_demote(c->st, r, L.f.r, L.N);
+        vcycle<float>(c, 0, L.f.r, L.f.z);
+        tdz::launch_promote(c->st, L.f.z, z, L.N);
+        break;
     default: HIPCHK(hipMemcpyAsync(z, r, L.N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st)); break;
     }
 }
@@ -409,12 +735,12 @@ cplx dev_dot(tdz_ctx *c, const dcomplex *x, const dcomplex *y, long N)
 void mass_apply(tdz_ctx *c, const dcomplex *x, dcomplex *y)
 {
     Level &L = *c->lv[0];
-    tdz::SepBands mb = sep_bands(L);
-    for (int d = 0; d < 3; ++d) mb.B[d] = L.dBmass[d];
-    tdz::launch_sep_apply(c->st, mb, x, y, c->tmp);
+    tdz::SepBandsT<double> mb = sep_bands<double>(L);
+    for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
+    tdz::launch_sep_apply<double>(c->st, mb, x, y, c->sd.tmp);
     if (c->prob.dirichlet) {
         const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-        tdz::launch_wall_fix(c->st, L.n, w, x, y);
+        tdz::launch_wall_fix<double>(c->st, L.n, w, x, y);
     }
 }
 
@@ -431,8 +757,10 @@ double dev_norm2(tdz_ctx *c, const dcomplex *x, long N)
 // re-orthogonalisation (two fused passes), complex Givens rotations. Stops when
 // the residual estimate |g_{j+1}| <= rtol ||b|| (true residual for right PC).
 // With r0_in_v0 the caller has already stored the initial residual b - A x in V_0.
+// Returns the iteration count; without convergence in maxit iterations it throws, or
+// with allow_fail returns -1 (x then holds the last iterate).
 int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, tdz_ksp_stats *st,
-           bool r0_in_v0 = false)
+           bool r0_in_v0 = false, bool allow_fail = false)
 {
     Level &L = *c->lv[0];
     const long N = L.N;
@@ -515,6 +843,7 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
     HIPCHK(hipStreamSynchronize(c->st));
     if (st) *st = {its, restarts, res, bnorm, c->rebuilds};
     if (!converged) {
+        if (allow_fail) return -1;
         char msg[160];
         std::snprintf(msg, sizeof msg, "KSP did not converge: %d iterations, relative residual %.3e", its, res);
         throw Error(msg);
@@ -527,17 +856,27 @@ void set_shift(tdz_ctx *c, cplx alpha, cplx g, const double sE[3])
     c->alpha = alpha;
     c->g = g;
     for (int a = 0; a < 3; ++a) c->sE[a] = sE ? sE[a] : 0.0;
-    for (auto &L : c->lv) update_bands(c, *L);
+    set_shift_bands(c);
 }
 
 void pc_setup(tdz_ctx *c, int kind)
 {
-    if (kind < TDZ_PC_NONE || kind > TDZ_PC_MG) throw Error("unknown preconditioner kind");
-    if (kind == TDZ_PC_MG && c->lv.size() < 2) throw Error("multigrid needs at least two levels");
+    if (kind < TDZ_PC_NONE || kind > TDZ_PC_LEARNED) throw Error("unknown preconditioner kind");
+    const bool mg = kind == TDZ_PC_MG || kind == TDZ_PC_MG_MIXED;
+    if (mg && c->lv.size() < 2) throw Error("multigrid needs at least two levels");
     c->pc_kind = kind;
-    if (kind == TDZ_PC_SEPARABLE) setup_fd(c, *c->lv[0]);
-    if (kind == TDZ_PC_MG)
+    if (mg)
         for (auto &L : c->lv) setup_fd(c, *L);
+    else if (kind != TDZ_PC_NONE)
+        setup_fd(c, *c->lv[0]);
+    const bool corr = kind == TDZ_PC_CORRECTED || kind == TDZ_PC_CORRECTED_EXACT || kind == TDZ_PC_LEARNED;
+    if (kind == TDZ_PC_LEARNED) {
+        if (!c->learn.trained) throw Error("the learned preconditioner has not been trained (tdz_learn_train)");
+        c->learn.fallback_kind = c->lv.size() >= 2 ? TDZ_PC_MG : TDZ_PC_SEPARABLE;
+        for (auto &L : c->lv) setup_fd(c, *L);
+    }
+    if (kind == TDZ_PC_SEPARABLE_MIXED || kind == TDZ_PC_MG_MIXED || corr) setup_mixed(c);
+    if (corr) setup_corrected(c);
     c->pc_ready = true;
 }
 
@@ -662,18 +1001,10 @@ void build(tdz_ctx *c, const tdz_problem &pr)
     for (auto &L : c->lv) upload_level_tables(c, *L);
     for (auto &L : c->lv) autotune_pot(c, *L);
     const long N = c->lv[0]->N;
-    for (int i = 0; i < 4; ++i) c->tmp[i] = track(c, dalloc<dcomplex>(N));
-    c->w1 = track(c, dalloc<dcomplex>(N));
-    c->w2 = track(c, dalloc<dcomplex>(N));
+    for (int i = 0; i < 4; ++i) c->sd.tmp[i] = track(c, dalloc<dcomplex>(N));
+    c->sd.w1 = track(c, dalloc<dcomplex>(N));
+    c->sd.w2 = track(c, dalloc<dcomplex>(N));
     c->rhs = track(c, dalloc<dcomplex>(N));
-    for (int l = 0; l < nl; ++l) {
-        Level &L = *c->lv[l];
-        L.t = track(c, dalloc<dcomplex>(L.N));
-        if (l > 0) {
-            L.r = track(c, dalloc<dcomplex>(L.N));
-            L.z = track(c, dalloc<dcomplex>(L.N));
-        }
-    }
     c->restart = pr.restart > 0 ? pr.restart : (N > 100000 ? 50 : 30);
     if (c->restart > 60) throw Error("restart must be <= 60");
     c->Vb = track(c, dalloc<dcomplex>((size_t)(c->restart + 1) * N));
@@ -817,17 +1148,17 @@ int tdz_term_apply(tdz_ctx *c, int term, const double *x, double *y)
                 if (term >= TDZ_TERM_DX && d == term - TDZ_TERM_DX && act) b = L.Xb[d][k];
                 B[d][k] = dc(b);
             }
-            HIPCHK(hipMemcpyAsync(L.dB[d], B[d].data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipMemcpyAsync(L.d.B[d], B[d].data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
         }
         if (term == TDZ_TERM_V) {
             HIPCHK(hipMemsetAsync(yd, 0, L.N * sizeof(dcomplex), c->st));
             if (c->ph.pot_kind) pot_apply(c, L, 1.0, xd, yd);
         } else {
-            tdz::launch_sep_apply(c->st, sep_bands(L), xd, yd, c->tmp);
+            tdz::launch_sep_apply<double>(c->st, sep_bands<double>(L), xd, yd, c->sd.tmp);
         }
         if (c->prob.dirichlet) {
             const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-            tdz::launch_wall_fix(c->st, L.n, w, xd, yd);
+            tdz::launch_wall_fix<double>(c->st, L.n, w, xd, yd);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->st));
@@ -859,6 +1190,37 @@ int tdz_ksp_solve(tdz_ctx *c, const double *b, double *x, double rtol, int maxit
     });
 }
 
+}  // extern "C"
+
+namespace {
+// Time-step solve with the current preconditioner. The learned preconditioner has
+// regressed when a solve needs more iterations than the same preconditioner without
+// the network took on the training steps: such a solve is stopped at that count, plain
+// multigrid takes over for the rest of the run and finishes it from the last iterate,
+// so the answer is always judged by the same FP64 residual.
+void step_solve(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, tdz_ksp_stats *st,
+                bool r0_in_v0)
+{
+    LearnState &ls = c->learn;
+    if (c->pc_kind != TDZ_PC_LEARNED || ls.fallen || ls.base_its < 0) {
+        fgmres(c, b, x, rtol, maxit, st, r0_in_v0);
+        return;
+    }
+    tdz_ksp_stats s1{};
+    if (fgmres(c, b, x, rtol, std::min(maxit, ls.base_its), &s1, r0_in_v0, true) >= 0) {
+        *st = s1;
+        return;
+    }
+    ls.fallen = true;
+    ++ls.fallbacks;
+    fgmres(c, b, x, rtol, std::max(maxit - s1.its, 1), st, false);
+    st->its += s1.its;
+    st->restarts += s1.restarts;
+}
+}  // namespace
+
+extern "C" {
+
 int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], double rtol, int maxit,
                 tdz_ksp_stats *stats)
 {
@@ -872,17 +1234,19 @@ int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], doub
         // (M + i dt/2 H_E) psi = 2 M psi - rhs, so the initial residual of the guess psi
         // is r0 = 2 (rhs - M psi): one mass apply instead of a full operator apply
         // (the same identity as the stage right-hand sides of src/propagator.cpp:589-624)
-        tdz::SepBands mb = sep_bands(L);
-        for (int d = 0; d < 3; ++d) mb.B[d] = L.dBmass[d];
-        tdz::launch_sep_apply(c->st, mb, x, c->w1, c->tmp);
+        tdz::SepBandsT<double> mb = sep_bands<double>(L);
+        for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
+        tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
         if (c->prob.dirichlet) {
             const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-            tdz::launch_wall_fix(c->st, L.n, w, x, c->w1);
+            tdz::launch_wall_fix<double>(c->st, L.n, w, x, c->sd.w1);
         }
         HIPCHK(hipMemcpyAsync(c->Vb, c->rhs, L.N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
-        tdz::launch_axpby(c->st, {-2.0, 0}, c->w1, {2.0, 0}, c->Vb, L.N);
+        tdz::launch_axpby(c->st, {-2.0, 0}, c->sd.w1, {2.0, 0}, c->Vb, L.N);
         set_shift(c, 1.0, g, sE_mid);         // solve (M + i dt/2 H_E) psi' = rhs, guess psi
-        fgmres(c, c->rhs, x, rtol, maxit, stats, true);
+        tdz_ksp_stats st{};
+        step_solve(c, c->rhs, x, rtol, maxit, &st, true);
+        if (stats) *stats = st;
     });
 }
 
@@ -919,7 +1283,7 @@ void pade22_step(tdz_ctx *c, dcomplex *x, double h, const double sE[3], int slot
         for (int j = 0; j < q; ++j) tdz::launch_axpby(c->st, {w[q - 1][j], 0}, H + (size_t)j * N, {1, 0}, x, N);
         set_shift(c, 1.0, gk[k], sE);
         tdz_ksp_stats st{};
-        fgmres(c, c->rhs, x, rtol, maxit, &st);
+        step_solve(c, c->rhs, x, rtol, maxit, &st, false);
         // history: shift, newest increment y - x first
         for (int j = kGuessOrder - 1; j > 0; --j)
             HIPCHK(hipMemcpyAsync(H + (size_t)j * N, H + (size_t)(j - 1) * N, N * sizeof(dcomplex),
@@ -965,18 +1329,18 @@ void project_gaussian(tdz_ctx *c, const double centre[3], const double width[3],
         for (int j = 0; j < n1; ++j)
             for (int i = 0; i < n0; ++i)
                 b[i + (size_t)n0 * (j + (size_t)n1 * k)] = dc(bd[0][i] * bd[1][j] * bd[2][k]);
-    HIPCHK(hipMemcpyAsync(c->w1, b.data(), L.N * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->sd.w1, b.data(), L.N * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
     setup_fd(c, L);
     const cplx a0 = c->alpha, g0 = c->g;
     double s0[3] = {c->sE[0], c->sE[1], c->sE[2]};
     c->alpha = 1.0;
     c->g = 0.0;
-    fd_apply(c, L, c->w1, out, c->tmp[0], c->tmp[1]);
+    fd_apply(c, L, c->sd.w1, out, c->sd.tmp[0], c->sd.tmp[1]);
     c->alpha = a0;
     c->g = g0;
     (void)s0;
-    mass_apply(c, out, c->w2);
-    const double nrm = std::sqrt(dev_dot(c, out, c->w2, L.N).real());
+    mass_apply(c, out, c->sd.w2);
+    const double nrm = std::sqrt(dev_dot(c, out, c->sd.w2, L.N).real());
     tdz::launch_axpby(c->st, {1.0 / nrm, 0}, out, {0, 0}, out, L.N);
     HIPCHK(hipStreamSynchronize(c->st));
 }
@@ -999,13 +1363,13 @@ void ground_state(tdz_ctx *c, dcomplex *x, double sigma, double tol, int maxit, 
     c->pc_kind = TDZ_PC_SEPARABLE;
     setup_fd(c, L);
     auto rayleigh = [&](double &E, double &res) {
-        mass_apply(c, x, c->w2);                       // w2 = M x
+        mass_apply(c, x, c->sd.w2);                       // w2 = M x
         set_shift(c, 0.0, 1.0, zero);
-        level_apply(c, L, x, c->w1);                   // w1 = H x
-        const double xm = dev_dot(c, x, c->w2, N).real();
-        E = dev_dot(c, x, c->w1, N).real() / xm;
-        tdz::launch_axpby(c->st, {-E, 0}, c->w2, {1, 0}, c->w1, N);   // w1 = H x - E M x
-        res = std::sqrt(dev_norm2(c, c->w1, N) / dev_norm2(c, c->w2, N));
+        level_apply(c, L, x, c->sd.w1);                   // w1 = H x
+        const double xm = dev_dot(c, x, c->sd.w2, N).real();
+        E = dev_dot(c, x, c->sd.w1, N).real() / xm;
+        tdz::launch_axpby(c->st, {-E, 0}, c->sd.w2, {1, 0}, c->sd.w1, N);   // w1 = H x - E M x
+        res = std::sqrt(dev_norm2(c, c->sd.w1, N) / dev_norm2(c, c->sd.w2, N));
     };
     double E = 0, res = 1;
     int it = 0;
@@ -1017,8 +1381,8 @@ void ground_state(tdz_ctx *c, dcomplex *x, double sigma, double tol, int maxit, 
             set_shift(c, -shift, 1.0, zero);
             tdz_ksp_stats st{};
             fgmres(c, c->rhs, x, 1e-11, 500, &st);
-            mass_apply(c, x, c->w2);
-            const double nrm = std::sqrt(dev_dot(c, x, c->w2, N).real());
+            mass_apply(c, x, c->sd.w2);
+            const double nrm = std::sqrt(dev_dot(c, x, c->sd.w2, N).real());
             tdz::launch_axpby(c->st, {1.0 / nrm, 0}, x, {0, 0}, x, N);
             rayleigh(E, res);
         }
@@ -1035,6 +1399,241 @@ void ground_state(tdz_ctx *c, dcomplex *x, double sigma, double tol, int maxit, 
     if (res_out) *res_out = res;
     if (its_out) *its_out = it;
     if (res > tol) throw Error("ground state: inverse iteration did not converge");
+}
+
+// ------------------------------------------------------------------ learned part
+// Positional fields of the network at the Greville points of the fine level: 1, V,
+// h dV, h^2 d2V (6), V^2, x/L (3), r^2/L^2 for the soft-core potential
+// V = -s Z / sqrt(r^2 + a); each channel scaled to unit maximum over the grid.
+void learned_fields(tdz_ctx *c, std::vector<float> &P, std::vector<double> &scale)
+{
+    Level &L = *c->lv[0];
+    constexpr int C = tdz::kCnnChannels;
+    std::vector<double> gr[3];
+    double h[3] = {0, 0, 0}, Lx[3] = {1, 1, 1};
+    for (int d = 0; d < 3; ++d) {
+        if (d < c->prob.dim) {
+            gr[d] = tdz::greville(L.ax[d].t, L.ax[d].p);
+            h[d] = (L.ax[d].hi - L.ax[d].lo) / L.ax[d].E;
+            Lx[d] = std::max(std::fabs(L.ax[d].lo), std::fabs(L.ax[d].hi));
+        } else {
+            gr[d] = {0.0};
+        }
+    }
+    const double sZ = c->ph.s * c->ph.pot[0], a = c->ph.pot_kind == 2 ? c->ph.pot[1] : 1.0;
+    std::vector<double> f((size_t)L.N * C, 0.0);
+    for (int k = 0; k < L.n[2]; ++k)
+        for (int j = 0; j < L.n[1]; ++j)
+            for (int i = 0; i < L.n[0]; ++i) {
+                const double x[3] = {gr[0][i], gr[1][j], gr[2][k]};
+                const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+                const double rho = 1.0 / std::sqrt(r2 + a), rho3 = rho * rho * rho, rho5 = rho3 * rho * rho;
+                double *o = &f[(size_t)(i + (size_t)L.n[0] * (j + (size_t)L.n[1] * k)) * C];
+                const double V = -sZ * rho;
+                o[0] = 1.0;
+                o[1] = V;
+                for (int d = 0; d < 3; ++d) o[2 + d] = sZ * x[d] * rho3 * h[d];
+                for (int d = 0; d < 3; ++d) o[5 + d] = sZ * (rho3 - 3.0 * x[d] * x[d] * rho5) * h[d] * h[d];
+                o[8] = -3.0 * sZ * x[0] * x[1] * rho5 * h[0] * h[1];
+                o[9] = -3.0 * sZ * x[0] * x[2] * rho5 * h[0] * h[2];
+                o[10] = -3.0 * sZ * x[1] * x[2] * rho5 * h[1] * h[2];
+                o[11] = V * V;
+                for (int d = 0; d < 3; ++d) o[12 + d] = x[d] / Lx[d];
+                o[15] = r2 / (Lx[0] * Lx[0] + Lx[1] * Lx[1] + Lx[2] * Lx[2]);
+            }
+    std::vector<double> mx(C, 0.0);
+    for (size_t v = 0; v < (size_t)L.N; ++v)
+        for (int q = 0; q < C; ++q) mx[q] = std::max(mx[q], std::fabs(f[v * C + q]));
+    P.resize(f.size());
+    for (size_t v = 0; v < (size_t)L.N; ++v)
+        for (int q = 0; q < C; ++q) P[v * C + q] = (float)(mx[q] > 0 ? f[v * C + q] / mx[q] : 0.0);
+    scale = mx;
+}
+
+void init_weights(tdz_ctx *c);
+
+void setup_learned(tdz_ctx *c)
+{
+    LearnState &ls = c->learn;
+    if (ls.ready) return;
+    setup_mixed(c);
+    setup_corrected(c);
+    Level &L = *c->lv[0];
+    constexpr int C = tdz::kCnnChannels, CI = tdz::kCnnInputs;
+    const long N = L.N;
+    std::vector<float> P;
+    learned_fields(c, P, ls.fscale);
+    ls.P = upload(c, P);
+    ls.W1 = track(c, dalloc<float>(27 * CI * C));
+    ls.W2 = track(c, dalloc<float>(27 * C * C));
+    ls.mb = track(c, dalloc<unsigned short>((size_t)2 * N * C));
+    ls.out = track(c, dalloc<float>((size_t)2 * N));
+    ls.nvec = track(c, dalloc<fcomplex>(N));
+    ls.feat = track(c, dalloc<fcomplex>((size_t)CI * N));
+    for (int d = 0; d < 3; ++d) {
+        const size_t sz = L.M[d].size();
+        std::vector<fcomplex> xb(sz, fcomplex{0.f, 0.f}), zb(sz, fcomplex{0.f, 0.f});
+        for (size_t k = 0; k < sz; ++k) xb[k].re = (float)L.Xb[d][k];
+        ls.Xc[d] = upload(c, xb);
+        ls.Zc[d] = upload(c, zb);
+        std::vector<float> g(L.n[d], 0.f);
+        if (d < c->prob.dim) {
+            const std::vector<double> gd = tdz::greville(L.ax[d].t, L.ax[d].p);
+            for (int i = 0; i < L.n[d]; ++i) g[i] = (float)gd[i];
+        }
+        ls.gr[d] = upload(c, g);
+    }
+    ls.ready = true;
+    init_weights(c);
+}
+
+// Initial weights: the first-order Taylor closure of the potential product,
+//   V y - V(x_v) M y ~ sum_d dV/dx_d(x_v) (X_d y - x_d M y),
+// routed through hidden channel 2 + d (its positional field is h_d dV/dx_d scaled to
+// unit maximum) with the current input and target scales, plus small deterministic
+// noise on every other weight so that no channel starts without a gradient.
+void init_weights(tdz_ctx *c)
+{
+    LearnState &ls = c->learn;
+    Level &L = *c->lv[0];
+    constexpr int C = tdz::kCnnChannels, CI = tdz::kCnnInputs;
+    ls.hW1.assign(27 * CI * C, 0.f);
+    ls.hW2.assign(27 * C * C, 0.f);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    auto rnd = [&]() {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        return ((st >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0;
+    };
+    for (auto &w : ls.hW1) w = (float)(rnd() * 1e-2);
+    for (auto &w : ls.hW2) w = (float)(rnd() * 1e-3);
+    constexpr int centre = 13;   // tap (0, 0, 0)
+    for (int d = 0; d < c->prob.dim; ++d) {
+        const double h = (L.ax[d].hi - L.ax[d].lo) / L.ax[d].E;
+        ls.hW1[(centre * CI + 1 + d) * C + 2 + d] = 1.f;
+        ls.hW2[(centre * C + 2 + d) * C + 0] = (float)(ls.fscale[2 + d] / h * ls.sout / ls.sin.v[1 + d]);
+    }
+    HIPCHK(hipMemcpy(ls.W1, ls.hW1.data(), ls.hW1.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ls.W2, ls.hW2.data(), ls.hW2.size() * 4, hipMemcpyHostToDevice));
+}
+
+void setup_training_buffers(tdz_ctx *c)
+{
+    LearnState &ls = c->learn;
+    if (ls.m) return;
+    constexpr int C = tdz::kCnnChannels;
+    const long N = c->lv[0]->N;
+    ls.m = track(c, dalloc<float>((size_t)2 * N * C));
+    ls.dov = track(c, dalloc<float>((size_t)2 * N * C));
+    ls.dh = track(c, dalloc<float>((size_t)2 * N * C));
+    ls.partial = track(c, dalloc<float>((size_t)256 * 27 * C * C));
+    ls.dW1 = track(c, dalloc<float>(27 * tdz::kCnnInputs * C));
+    ls.dW2 = track(c, dalloc<float>(27 * C * C));
+}
+
+// Normalised loss sum_s |N(y_s) - t_s|^2 / |t_s|^2 and its gradient (summed over the
+// samples), FP32 forward/backward kernels; gradients returned on the host.
+double learned_loss_grad(tdz_ctx *c, std::vector<double> &g1, std::vector<double> &g2, bool want_grad)
+{
+    LearnState &ls = c->learn;
+    Level &L = *c->lv[0];
+    constexpr int C = tdz::kCnnChannels;
+    const long N = L.N;
+    const int n3[3] = {L.n[0], L.n[1], L.n[2]};
+    g1.assign(27 * tdz::kCnnInputs * C, 0.0);
+    g2.assign(27 * C * C, 0.0);
+    std::vector<float> h1(27 * tdz::kCnnInputs * C), h2(27 * C * C), hout;
+    double loss = 0;
+    const int nchunk = 256;
+    for (size_t s = 0; s < ls.ys.size(); ++s) {
+        tdz::cnn_forward_f32(c->st, n3, ls.ys[s], ls.W1, ls.W2, ls.P, ls.m, ls.out);
+        // loss on the host from the outputs (2N floats) and the target
+        hout.resize((size_t)2 * N);
+        std::vector<fcomplex> tt(N);
+        HIPCHK(hipMemcpyAsync(hout.data(), ls.out, hout.size() * 4, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(tt.data(), ls.ts[s], N * sizeof(fcomplex), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        double l = 0;
+        for (long v = 0; v < N; ++v) {
+            const double dr = hout[v] - tt[v].re, di = hout[N + v] - tt[v].im;
+            l += dr * dr + di * di;
+        }
+        loss += l / ls.tnorm2[s];
+        if (!want_grad) continue;
+        tdz::cnn_backward_f32(c->st, n3, ls.ys[s], ls.ts[s], (float)(1.0 / ls.tnorm2[s]), ls.W2, ls.P, ls.m, ls.out,
+                              ls.dov, ls.dh, ls.partial, nchunk, ls.dW1, ls.dW2);
+        HIPCHK(hipMemcpyAsync(h1.data(), ls.dW1, h1.size() * 4, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(h2.data(), ls.dW2, h2.size() * 4, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (size_t i = 0; i < h1.size(); ++i) g1[i] += h1[i];
+        for (size_t i = 0; i < h2.size(); ++i) g2[i] += h2[i];
+    }
+    return loss;
+}
+
+// Input scales (unit mean square of each input field over the samples) and the target
+// scale, applied to the stored samples; inference applies the same scales.
+void normalise_samples(tdz_ctx *c)
+{
+    LearnState &ls = c->learn;
+    const long N = c->lv[0]->N;
+    constexpr int CI = tdz::kCnnInputs;
+    std::vector<fcomplex> h((size_t)CI * N);
+    double su[CI] = {0, 0, 0, 0}, stt = 0;
+    for (size_t s = 0; s < ls.ys.size(); ++s) {
+        HIPCHK(hipMemcpy(h.data(), ls.ys[s], h.size() * sizeof(fcomplex), hipMemcpyDeviceToHost));
+        for (int k = 0; k < CI; ++k)
+            for (long v = 0; v < N; ++v) {
+                const fcomplex z = h[(size_t)k * N + v];
+                su[k] += (double)z.re * z.re + (double)z.im * z.im;
+            }
+        stt += ls.tnorm2[s];
+    }
+    const double cnt = (double)ls.ys.size() * N;
+    for (int k = 0; k < CI; ++k) ls.sin.v[k] = su[k] > 0 ? (float)std::sqrt(cnt / su[k]) : 1.f;
+    ls.sout = stt > 0 ? std::sqrt(cnt / stt) : 1.0;
+    std::vector<fcomplex> th(N);
+    for (size_t s = 0; s < ls.ys.size(); ++s) {
+        tdz::scale_inputs(c->st, N, ls.sin, ls.ys[s]);
+        HIPCHK(hipMemcpyAsync(th.data(), ls.ts[s], N * sizeof(fcomplex), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (auto &z : th) z = {(float)(z.re * ls.sout), (float)(z.im * ls.sout)};
+        HIPCHK(hipMemcpy(ls.ts[s], th.data(), N * sizeof(fcomplex), hipMemcpyHostToDevice));
+        ls.tnorm2[s] *= ls.sout * ls.sout;
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
+// Adam on the harvested samples.
+void learned_fit(tdz_ctx *c, int iters, double lr, double *loss0, double *loss1)
+{
+    LearnState &ls = c->learn;
+    setup_training_buffers(c);
+    std::vector<double> g1, g2, m1(ls.hW1.size(), 0.0), v1(ls.hW1.size(), 0.0), m2(ls.hW2.size(), 0.0),
+        v2(ls.hW2.size(), 0.0);
+    const double b1 = 0.9, b2 = 0.999, eps = 1e-12;
+    double first = -1, last = 0;
+    for (int it = 1; it <= iters; ++it) {
+        last = learned_loss_grad(c, g1, g2, true);
+        if (first < 0) first = last;
+        const double c1 = 1.0 - std::pow(b1, it), c2 = 1.0 - std::pow(b2, it);
+        auto step = [&](std::vector<float> &w, const std::vector<double> &g, std::vector<double> &m,
+                        std::vector<double> &v) {
+            for (size_t i = 0; i < w.size(); ++i) {
+                m[i] = b1 * m[i] + (1 - b1) * g[i];
+                v[i] = b2 * v[i] + (1 - b2) * g[i] * g[i];
+                w[i] -= (float)(lr * (m[i] / c1) / (std::sqrt(v[i] / c2) + eps));
+            }
+        };
+        step(ls.hW1, g1, m1, v1);
+        step(ls.hW2, g2, m2, v2);
+        HIPCHK(hipMemcpy(ls.W1, ls.hW1.data(), ls.hW1.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ls.W2, ls.hW2.data(), ls.hW2.size() * 4, hipMemcpyHostToDevice));
+    }
+    last = learned_loss_grad(c, g1, g2, false);
+    // reported per sample: 1 is the loss of a network that outputs zero
+    const double ns = (double)ls.ys.size();
+    if (loss0) *loss0 = first / ns;
+    if (loss1) *loss1 = last / ns;
 }
 
 }  // namespace
@@ -1126,16 +1725,16 @@ int tdz_observables(tdz_ctx *c, const double *psi, double out[6])
         auto dot = [&](int term_or_h) {
             // term_or_h < 0: full H = K + sV + CAP
             if (term_or_h >= 0) {
-                if (tdz_term_apply(c, term_or_h, psi, reinterpret_cast<double *>(c->w1))) throw Error(g_err);
+                if (tdz_term_apply(c, term_or_h, psi, reinterpret_cast<double *>(c->sd.w1))) throw Error(g_err);
             } else {
                 const cplx a0 = c->alpha, g0 = c->g;
                 double s0[3] = {c->sE[0], c->sE[1], c->sE[2]};
                 const double zero[3] = {0, 0, 0};
                 set_shift(c, 0.0, 1.0, zero);
-                level_apply(c, L, reinterpret_cast<const dcomplex *>(psi), c->w1);
+                level_apply(c, L, reinterpret_cast<const dcomplex *>(psi), c->sd.w1);
                 set_shift(c, a0, g0, s0);
             }
-            HIPCHK(hipMemcpyAsync(hv.data(), c->w1, N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(hv.data(), c->sd.w1, N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
             HIPCHK(hipStreamSynchronize(c->st));
             cplx s = 0;
             for (long i = 0; i < N; ++i) s += std::conj(cplx(hp[i].re, hp[i].im)) * cplx(hv[i].re, hv[i].im);
@@ -1165,8 +1764,9 @@ int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, con
             if (c->refresh_base < 0) c->refresh_base = st.its;
             else if (st.its > 2 * c->refresh_base + 5) {
                 c->refresh_base = -1;
+                // the separable solves depend on (alpha, g) only and every level operator
+                // follows the field through set_shift, so a rebuild has nothing to redo
                 ++c->rebuilds;
-                for (auto &L : c->lv) L->fd_ready = false;
                 pc_setup(c, c->pc_kind);
             }
             if (its) its[s] = st.its;
@@ -1178,6 +1778,163 @@ int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, con
             }
         }
     });
+}
+
+int tdz_learn_train(tdz_ctx *c, const double *psi, double t0, double dt, const tdz_pulse *pulse, int harvest_steps,
+                    int max_samples, int iters, double lr, double rtol, double *loss_before, double *loss_after)
+{
+    return guarded([&] {
+        if (c->ph.pot_kind == 0) throw Error("no potential: nothing to learn");
+        setup_learned(c);
+        LearnState &ls = c->learn;
+        Level &L = *c->lv[0];
+        const long N = L.N;
+        const int kind0 = c->pc_kind;
+        // a new training run starts from fresh samples and unit scales
+        for (fcomplex *p : ls.ys) untrack(c, p);
+        for (fcomplex *p : ls.ts) untrack(c, p);
+        ls.ys.clear();
+        ls.ts.clear();
+        ls.tnorm2.clear();
+        ls.sin = tdz::CnnScales{{1.f, 1.f, 1.f, 1.f}};
+        ls.sout = 1.0;
+        // harvest from a copy of psi with the preconditioner the network augments
+        dcomplex *x = dalloc<dcomplex>(N);
+        HIPCHK(hipMemcpy(x, psi, N * sizeof(dcomplex), hipMemcpyDeviceToDevice));
+        pc_setup(c, TDZ_PC_CORRECTED);
+        ls.max_samples = max_samples;
+        ls.harvesting = true;
+        int base = 0;
+        try {
+            for (int s = 0; s < harvest_steps; ++s) {
+                double sE[3] = {0, 0, 0};
+                if (pulse) tdz_pulse_coupling(c, pulse, t0 + (s + 0.5) * dt, sE);
+                tdz_ksp_stats st{};
+                if (tdz_step_cn(c, reinterpret_cast<double *>(x), dt, sE, rtol, 1000, &st)) throw Error(g_err);
+                base = std::max(base, st.its);
+            }
+        } catch (...) {
+            ls.harvesting = false;
+            hipFree(x);
+            throw;
+        }
+        ls.harvesting = false;
+        HIPCHK(hipFree(x));
+        ls.base_its = base;
+        if (ls.ys.empty()) throw Error("no samples harvested");
+        normalise_samples(c);
+        init_weights(c);
+        learned_fit(c, iters, lr, loss_before, loss_after);
+        ls.trained = true;
+        ls.fallen = false;
+        pc_setup(c, kind0 == TDZ_PC_NONE ? TDZ_PC_LEARNED : kind0);
+    });
+}
+
+int tdz_learn_weights(tdz_ctx *c, float *W1, float *W2, int set)
+{
+    return guarded([&] {
+        setup_learned(c);
+        LearnState &ls = c->learn;
+        if (set) {
+            std::copy(W1, W1 + ls.hW1.size(), ls.hW1.begin());
+            std::copy(W2, W2 + ls.hW2.size(), ls.hW2.begin());
+            HIPCHK(hipMemcpy(ls.W1, ls.hW1.data(), ls.hW1.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(ls.W2, ls.hW2.data(), ls.hW2.size() * 4, hipMemcpyHostToDevice));
+            ls.trained = true;
+        } else {
+            std::copy(ls.hW1.begin(), ls.hW1.end(), W1);
+            std::copy(ls.hW2.begin(), ls.hW2.end(), W2);
+        }
+    });
+}
+
+int tdz_learn_fields(tdz_ctx *c, float *P)
+{
+    return guarded([&] {
+        setup_learned(c);
+        HIPCHK(hipMemcpy(P, c->learn.P, (size_t)c->lv[0]->N * tdz::kCnnChannels * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int tdz_learn_features(tdz_ctx *c, const double *y, double *feat)
+{
+    return guarded([&] {
+        setup_learned(c);
+        Level &L = *c->lv[0];
+        const long N = L.N;
+        constexpr int CI = tdz::kCnnInputs;
+        fcomplex *My = c->sf.u1;
+        tdz::launch_demote(c->st, reinterpret_cast<const dcomplex *>(y), L.f.z, N);
+        tdz::SepBandsT<float> mb = sep_bands<float>(L);
+        for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
+        tdz::launch_sep_apply<float>(c->st, mb, L.f.z, My, c->sf.tmp);
+        learned_features(c, L.f.z, My, c->learn.feat);
+        std::vector<fcomplex> h((size_t)CI * N);
+        HIPCHK(hipMemcpyAsync(h.data(), c->learn.feat, h.size() * sizeof(fcomplex), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (size_t k = 0; k < h.size(); ++k) {
+            feat[2 * k] = h[k].re;
+            feat[2 * k + 1] = h[k].im;
+        }
+    });
+}
+
+int tdz_learn_eval(tdz_ctx *c, const double *u, const double *target, double *loss, float *dW1, float *dW2,
+                   double *out_f32, double *out_mfma)
+{
+    return guarded([&] {
+        setup_learned(c);
+        setup_training_buffers(c);
+        LearnState &ls = c->learn;
+        Level &L = *c->lv[0];
+        const long N = L.N;
+        constexpr int CI = tdz::kCnnInputs;
+        std::vector<fcomplex> uh((size_t)CI * N), th(N);
+        for (size_t k = 0; k < uh.size(); ++k) uh[k] = {(float)u[2 * k], (float)u[2 * k + 1]};
+        double t2 = 0;
+        for (long k = 0; k < N; ++k) {
+            th[k] = {(float)target[2 * k], (float)target[2 * k + 1]};
+            t2 += (double)th[k].re * th[k].re + (double)th[k].im * th[k].im;
+        }
+        fcomplex *y32 = upload(c, uh), *t32 = upload(c, th);
+        auto ys = ls.ys, ts = ls.ts;
+        auto tn = ls.tnorm2;
+        ls.ys = {y32};
+        ls.ts = {t32};
+        ls.tnorm2 = {t2 > 0 ? t2 : 1.0};
+        std::vector<double> g1, g2;
+        const double l = learned_loss_grad(c, g1, g2, true);
+        ls.ys = ys;
+        ls.ts = ts;
+        ls.tnorm2 = tn;
+        if (loss) *loss = l;
+        if (dW1) for (size_t i = 0; i < g1.size(); ++i) dW1[i] = (float)g1[i];
+        if (dW2) for (size_t i = 0; i < g2.size(); ++i) dW2[i] = (float)g2[i];
+        std::vector<float> o((size_t)2 * N);
+        if (out_f32) {
+            HIPCHK(hipMemcpy(o.data(), ls.out, o.size() * 4, hipMemcpyDeviceToHost));   // forward of the loss
+            for (size_t i = 0; i < o.size(); ++i) out_f32[i] = o[i];
+        }
+        if (out_mfma) {
+            const int n3[3] = {L.n[0], L.n[1], L.n[2]};
+            tdz::cnn_infer_mfma(c->st, n3, y32, ls.W1, ls.W2, ls.P, ls.mb, ls.out, fcomplex{1.f, 0.f}, nullptr);
+            HIPCHK(hipStreamSynchronize(c->st));
+            HIPCHK(hipMemcpy(o.data(), ls.out, o.size() * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < o.size(); ++i) out_mfma[i] = o[i];
+        }
+        untrack(c, y32);
+        untrack(c, t32);
+    });
+}
+
+int tdz_learn_status(tdz_ctx *c, int *base_its, int *fallbacks, int *active)
+{
+    if (!c) { g_err = "null context"; return 1; }
+    if (base_its) *base_its = c->learn.base_its;
+    if (fallbacks) *fallbacks = c->learn.fallbacks;
+    if (active) *active = c->pc_kind == TDZ_PC_LEARNED && c->learn.trained && !c->learn.fallen;
+    return 0;
 }
 
 int tdz_timing(tdz_ctx *c, int enable)

@@ -56,16 +56,23 @@ void gen_eig_bands(const Axis1D &A, const std::vector<double> &Tband, std::vecto
                    std::vector<double> &lam);
 
 // ------------------------------------------------------------ device side
-struct alignas(16) dcomplex { double re, im; };
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+    R re, im;
+};
+using dcomplex = cpx<double>;
+using fcomplex = cpx<float>;
 
 // Device copy of the per-axis tables the kernels read.
-struct AxisDev {
+template <typename R>
+struct AxisDevT {
     int E, p, Q, n;
-    const double *B0;   // [E][p+1][Q]
-    const double *X;    // [E*Q]
-    const double *W;    // [E*Q]
+    const R *B0;        // [E][p+1][Q]
+    const R *X;         // [E*Q]
+    const R *W;         // [E*Q]
     const int *off;     // [E]
 };
+using AxisDev = AxisDevT<double>;
 
 struct PotParams {
     int kind;
@@ -74,24 +81,30 @@ struct PotParams {
 };
 
 // y += g * (s V) x over all elements (generic sum-factorised kernel).
-void launch_pot_apply_generic(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
-                              const dcomplex *x, dcomplex *y);
+template <typename R>
+void launch_pot_apply_generic(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
+                              cpx<R> *y);
 // Same for dim 3 with simple interior knots (tiled z-marching kernel); returns
-// false when no instantiation covers (p, Q).
-bool launch_pot_apply_3d(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
-                         const dcomplex *x, dcomplex *y, int variant);
+// false when no instantiation covers (p, Q) or variant.
+template <typename R>
+bool launch_pot_apply_3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
+                         cpx<R> *y, int variant);
 // number of tile-shape variants of the tiled kernel for (p, Q) (0: none)
 int pot3d_variants(int p, int Q);
 // Separable Kronecker apply y = Mz My Bx x + Mz By Mx x + Bz My Mx x, bands on device:
 // M_d real [n_d*(2p+1)], B_d complex [n_d*(2p+1)]; tmp = 4 scratch vectors.
-struct SepBands {
+template <typename R>
+struct SepBandsT {
     int n[3], p[3];
-    const double *M[3];
-    const dcomplex *B[3];
+    const R *M[3];
+    const cpx<R> *B[3];
 };
-void launch_sep_apply(hipStream_t st, const SepBands &sb, const dcomplex *x, dcomplex *y, dcomplex *tmp[4]);
+using SepBands = SepBandsT<double>;
+template <typename R>
+void launch_sep_apply(hipStream_t st, const SepBandsT<R> &sb, const cpx<R> *x, cpx<R> *y, cpx<R> *tmp[4]);
 // y[I] = x[I] where the wall mask flags I (1D flags per axis on device).
-void launch_wall_fix(hipStream_t st, const int n[3], const unsigned char *wall[3], const dcomplex *x, dcomplex *y);
+template <typename R>
+void launch_wall_fix(hipStream_t st, const int n[3], const unsigned char *wall[3], const cpx<R> *x, cpx<R> *y);
 
 // Deterministic two-stage reductions. partial must hold kMaxBlocks*(nv) entries.
 constexpr int kRedBlocks = 512;
@@ -105,24 +118,57 @@ void launch_multi_axpy_norm(hipStream_t st, const dcomplex *V, long ld, int nv, 
 void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N);
 // y = a x + b y (scalars on host)
 void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dcomplex *y, long N);
+template <typename R>
+void launch_axpby_t(hipStream_t st, cpx<R> a, const cpx<R> *x, cpx<R> b, cpx<R> *y, long N);
+// precision conversions
+void launch_demote(hipStream_t st, const dcomplex *x, fcomplex *y, long N);
+void launch_promote(hipStream_t st, const fcomplex *x, dcomplex *y, long N);
+// q = r - g Vg .* My - c - n (n may be null), complex64
+void launch_corr_combine(hipStream_t st, const fcomplex *r, const fcomplex *My, const fcomplex *c,
+                         const fcomplex *n, const float *Vg, fcomplex g, fcomplex *q, long N);
 // y = x * s where s = 1/sqrt(*norm2) read on device
 void launch_scale_invsqrt(hipStream_t st, const dcomplex *x, const double *norm2, dcomplex *y, long N);
 // out = ||x||^2 on device
 void launch_norm2(hipStream_t st, const dcomplex *x, long N, double *partial, double *out);
 
 // Dense axis transform y = T x along axis d (T real n x n, row-major; trans: T^T).
-void launch_dense_axis(hipStream_t st, int d, const int n[3], const double *T, bool trans, const dcomplex *x,
-                       dcomplex *y);
+template <typename R>
+void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool trans, const cpx<R> *x, cpx<R> *y);
 // One thread per output, T read through the caches (reference kernel for tests).
-void launch_dense_axis_simple(hipStream_t st, int d, const int n[3], const double *T, bool trans,
-                              const dcomplex *x, dcomplex *y);
+template <typename R>
+void launch_dense_axis_simple(hipStream_t st, int d, const int n[3], const R *T, bool trans, const cpx<R> *x,
+                              cpx<R> *y);
 // y[i,j,k] = x[i,j,k] / (alpha + g (lx[i] + ly[j] + lz[k])), walls: y = x
-void launch_diag_scale(hipStream_t st, const int n[3], const double *lam[3], const unsigned char *wall[3],
-                       dcomplex alpha, dcomplex g, const dcomplex *x, dcomplex *y);
+template <typename R>
+void launch_diag_scale(hipStream_t st, const int n[3], const R *lam[3], const unsigned char *wall[3], cpx<R> alpha,
+                       cpx<R> g, const cpx<R> *x, cpx<R> *y);
 
 // Sparse 1D transfer along axis d: out (axis d of size m) = A in (axis d of size nin),
 // A in CSR (rowptr[m+1], col, val); other axes unchanged. accumulate: out += A in.
+template <typename R>
 void launch_csr_axis(hipStream_t st, int d, const int nin[3], int m, const int *rowptr, const int *col,
-                     const double *val, const dcomplex *in, dcomplex *out, bool accumulate);
+                     const R *val, const cpx<R> *in, cpx<R> *out, bool accumulate);
+
+// ---------------------------------------------------------------- learned part
+constexpr int kCnnChannels = 16;
+constexpr int kCnnInputs = 4;   // y and its three first-moment features
+// FP32 forward: m [2N][C] work, out [2N] (re then im component)
+void cnn_forward_f32(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
+                     const float *P, float *m, float *out);
+// FP32 backward of the loss sum (out - t)^2 * inv_t2 over both components
+void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const fcomplex *target, float inv_t2,
+                      const float *W2, const float *P, const float *m, const float *out, float *dov, float *dh,
+                      float *partial, int nchunk, float *dW1, float *dW2);
+// BF16 MFMA forward; mb [2N][C] bf16 work, out [2N]; if nout: nout = g (out_re + i out_im)
+void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
+                    const float *P, unsigned short *mb, float *out, fcomplex g, fcomplex *nout);
+void cnn_to_complex(hipStream_t st, long N, const float *out, fcomplex g, fcomplex *n);
+// t = pv - Vg .* My
+void lumped_residual(hipStream_t st, long N, const fcomplex *pv, const fcomplex *My, const float *Vg, fcomplex *t);
+// u[c] *= v[c], c < kCnnInputs (in place)
+struct CnnScales { float v[kCnnInputs]; };
+void scale_inputs(hipStream_t st, long N, const CnnScales &s, fcomplex *u);
+// G -= x_d .* My (x_d: Greville coordinate along axis d)
+void moment_feature(hipStream_t st, const int n[3], int d, const float *gr, const fcomplex *My, fcomplex *G);
 
 }  // namespace tdz

@@ -18,11 +18,12 @@ namespace tdz {
 
 namespace {
 
-__device__ __forceinline__ double pot_value(const PotParams &pp, double r2)
+template <typename R>
+__device__ __forceinline__ R pot_value(const PotParams &pp, R r2)
 {
-    if (pp.kind == 1) return 0.5 * pp.a0 * r2;
-    if (pp.kind == 2) return -pp.a0 / sqrt(r2 + pp.a1);
-    return 0.0;
+    if (pp.kind == 1) return R(0.5) * (R)pp.a0 * r2;
+    if (pp.kind == 2) return -(R)pp.a0 / sqrt(r2 + (R)pp.a1);
+    return R(0);
 }
 
 // 1/sqrt(r) for r > 0: the hardware estimate refined by one second-order
@@ -34,14 +35,19 @@ __device__ __forceinline__ double rsqrt_pos(double r)
     return fma(y0 * e, fma(e, 0.375, 0.5), y0);
 }
 
+// single precision (preconditioner): the hardware estimate, about 1 ulp
+__device__ __forceinline__ float rsqrt_pos(float r) { return __builtin_amdgcn_rsqf(r); }
+
 // ------------------------------------------------------------------ potential
 // One element per workgroup. LDS layout (doubles): tables Bx|By|Bz, then the
 // complex stages c[n3][n2][n1], t1[n3][n2][Q1], t2[n3][Q2][Q1], F[Q3][Q2][Q1].
-__global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, AxisDev a1, AxisDev a2, PotParams pp,
-                                                                double gr, double gi, const dcomplex *__restrict__ x,
-                                                                dcomplex *__restrict__ y)
+template <typename R>
+__global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDevT<R> a0, AxisDevT<R> a1, AxisDevT<R> a2, PotParams pp,
+                                                                R gr, R gi, const cpx<R> *__restrict__ x,
+                                                                cpx<R> *__restrict__ y)
 {
-    extern __shared__ double smem[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    R *smem = reinterpret_cast<R *>(smem_raw);
     const int n1 = a0.p + 1, n2 = a1.p + 1, n3 = a2.p + 1;
     const int Q1 = a0.Q, Q2 = a1.Q, Q3 = a2.Q;
     const long eid = blockIdx.x;
@@ -51,15 +57,15 @@ __global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, Axis
     const int o1 = a0.off[e1], o2 = a1.off[e2], o3 = a2.off[e3];
     const long g0 = a0.n, g1 = a1.n;
 
-    double *Bx = smem, *By = Bx + n1 * Q1, *Bz = By + n2 * Q2;
-    double *Xq = Bz + n3 * Q3, *Yq = Xq + Q1, *Zq = Yq + Q2;  // point coordinates
-    double *Wx = Zq + Q3, *Wy = Wx + Q1, *Wz = Wy + Q2;        // weights
+    R *Bx = smem, *By = Bx + n1 * Q1, *Bz = By + n2 * Q2;
+    R *Xq = Bz + n3 * Q3, *Yq = Xq + Q1, *Zq = Yq + Q2;  // point coordinates
+    R *Wx = Zq + Q3, *Wy = Wx + Q1, *Wz = Wy + Q2;        // weights
     size_t off = (size_t)(n1 * Q1 + n2 * Q2 + n3 * Q3 + 2 * (Q1 + Q2 + Q3));
     off = (off + 1) & ~(size_t)1;  // 16-byte alignment for the complex arrays
-    dcomplex *c = reinterpret_cast<dcomplex *>(smem + off);
-    dcomplex *t1 = c + n1 * n2 * n3;
-    dcomplex *t2 = t1 + n3 * n2 * Q1;
-    dcomplex *F = t2 + n3 * Q2 * Q1;
+    cpx<R> *c = reinterpret_cast<cpx<R> *>(smem + off);
+    cpx<R> *t1 = c + n1 * n2 * n3;
+    cpx<R> *t2 = t1 + n3 * n2 * Q1;
+    cpx<R> *F = t2 + n3 * Q2 * Q1;
 
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int i = tid; i < n1 * Q1; i += nt) Bx[i] = a0.B0[(size_t)e1 * n1 * Q1 + i];
@@ -76,10 +82,10 @@ __global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, Axis
     // stage x: t1[(b3 n2 + b2) Q1 + q1]
     for (int i = tid; i < n3 * n2 * Q1; i += nt) {
         const int q1 = i % Q1, r = i / Q1;
-        double sr = 0, si = 0;
+        R sr = 0, si = 0;
         for (int b1 = 0; b1 < n1; ++b1) {
-            const double bv = Bx[b1 * Q1 + q1];
-            const dcomplex v = c[r * n1 + b1];
+            const R bv = Bx[b1 * Q1 + q1];
+            const cpx<R> v = c[r * n1 + b1];
             sr += bv * v.re; si += bv * v.im;
         }
         t1[i] = {sr, si};
@@ -88,10 +94,10 @@ __global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, Axis
     // stage y: t2[(b3 Q2 + q2) Q1 + q1]
     for (int i = tid; i < n3 * Q2 * Q1; i += nt) {
         const int q1 = i % Q1, q2 = (i / Q1) % Q2, b3 = i / (Q1 * Q2);
-        double sr = 0, si = 0;
+        R sr = 0, si = 0;
         for (int b2 = 0; b2 < n2; ++b2) {
-            const double bv = By[b2 * Q2 + q2];
-            const dcomplex v = t1[(b3 * n2 + b2) * Q1 + q1];
+            const R bv = By[b2 * Q2 + q2];
+            const cpx<R> v = t1[(b3 * n2 + b2) * Q1 + q1];
             sr += bv * v.re; si += bv * v.im;
         }
         t2[i] = {sr, si};
@@ -100,25 +106,25 @@ __global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, Axis
     // stage z and the point coefficient g s V(r) w
     for (int i = tid; i < Q3 * Q2 * Q1; i += nt) {
         const int q1 = i % Q1, q2 = (i / Q1) % Q2, q3 = i / (Q1 * Q2);
-        double sr = 0, si = 0;
+        R sr = 0, si = 0;
         for (int b3 = 0; b3 < n3; ++b3) {
-            const double bv = Bz[b3 * Q3 + q3];
-            const dcomplex v = t2[(b3 * Q2 + q2) * Q1 + q1];
+            const R bv = Bz[b3 * Q3 + q3];
+            const cpx<R> v = t2[(b3 * Q2 + q2) * Q1 + q1];
             sr += bv * v.re; si += bv * v.im;
         }
-        const double r2 = Xq[q1] * Xq[q1] + Yq[q2] * Yq[q2] + Zq[q3] * Zq[q3];
-        const double cv = pp.s * pot_value(pp, r2) * (Wx[q1] * Wy[q2] * Wz[q3]);
-        const double cr = gr * cv, ci = gi * cv;
+        const R r2 = Xq[q1] * Xq[q1] + Yq[q2] * Yq[q2] + Zq[q3] * Zq[q3];
+        const R cv = (R)pp.s * pot_value(pp, r2) * (Wx[q1] * Wy[q2] * Wz[q3]);
+        const R cr = gr * cv, ci = gi * cv;
         F[i] = {cr * sr - ci * si, cr * si + ci * sr};
     }
     __syncthreads();
     // transposed z: t2[(b3 Q2 + q2) Q1 + q1] = sum_q3 Bz[b3,q3] F[(q3 Q2 + q2) Q1 + q1]
     for (int i = tid; i < n3 * Q2 * Q1; i += nt) {
         const int q21 = i % (Q2 * Q1), b3 = i / (Q2 * Q1);
-        double sr = 0, si = 0;
+        R sr = 0, si = 0;
         for (int q3 = 0; q3 < Q3; ++q3) {
-            const double bv = Bz[b3 * Q3 + q3];
-            const dcomplex v = F[q3 * Q2 * Q1 + q21];
+            const R bv = Bz[b3 * Q3 + q3];
+            const cpx<R> v = F[q3 * Q2 * Q1 + q21];
             sr += bv * v.re; si += bv * v.im;
         }
         t2[i] = {sr, si};
@@ -127,10 +133,10 @@ __global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, Axis
     // transposed y: t1[(b3 n2 + b2) Q1 + q1] = sum_q2 By[b2,q2] t2[(b3 Q2 + q2) Q1 + q1]
     for (int i = tid; i < n3 * n2 * Q1; i += nt) {
         const int q1 = i % Q1, b2 = (i / Q1) % n2, b3 = i / (Q1 * n2);
-        double sr = 0, si = 0;
+        R sr = 0, si = 0;
         for (int q2 = 0; q2 < Q2; ++q2) {
-            const double bv = By[b2 * Q2 + q2];
-            const dcomplex v = t2[(b3 * Q2 + q2) * Q1 + q1];
+            const R bv = By[b2 * Q2 + q2];
+            const cpx<R> v = t2[(b3 * Q2 + q2) * Q1 + q1];
             sr += bv * v.re; si += bv * v.im;
         }
         t1[i] = {sr, si};
@@ -140,13 +146,13 @@ __global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, Axis
     for (int i = tid; i < n1 * n2 * n3; i += nt) {
         const int b1 = i % n1, r = i / n1;
         const int b2 = r % n2, b3 = r / n2;
-        double sr = 0, si = 0;
+        R sr = 0, si = 0;
         for (int q1 = 0; q1 < Q1; ++q1) {
-            const double bv = Bx[b1 * Q1 + q1];
-            const dcomplex v = t1[r * Q1 + q1];
+            const R bv = Bx[b1 * Q1 + q1];
+            const cpx<R> v = t1[r * Q1 + q1];
             sr += bv * v.re; si += bv * v.im;
         }
-        double *dst = reinterpret_cast<double *>(y + ((o1 + b1) + g0 * ((o2 + b2) + g1 * (long)(o3 + b3))));
+        R *dst = reinterpret_cast<R *>(y + ((o1 + b1) + g0 * ((o2 + b2) + g1 * (long)(o3 + b3))));
         unsafeAtomicAdd(dst, sr);
         unsafeAtomicAdd(dst + 1, si);
     }
@@ -166,17 +172,17 @@ __global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDev a0, Axis
 //      scaled by g, atomically added to y (neighbouring tiles share p planes
 //      of functions in x and y).
 // Requires simple interior knots (element e starts at function e) and dim 3.
-template <int P, int Q, int TE, int CPT, int KIND>
+template <typename R, int P, int Q, int TE, int CPT, int KIND>
 __global__ void __launch_bounds__((TE * Q) * (TE * Q) / CPT)
-pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double gi,
-             const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
+pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr, R gi,
+             const cpx<R> *__restrict__ x, cpx<R> *__restrict__ y)
 {
     constexpr int NP = P + 1, TQ = TE * Q, NCOL = TQ * TQ, NT = NCOL / CPT, TW = TE + P;
-    __shared__ dcomplex cin[TW * TW];   // input plane  [jy][jx]
-    __shared__ dcomplex T1[TW * TQ];    // x stage      [jy][qx]
-    __shared__ dcomplex G2p[NCOL];      // output plane at the points [qy][qx]
-    __shared__ dcomplex H1[TW * TQ];    // transposed y [jy][qx]
-    __shared__ double Bxs[TE * NP * Q], Bys[TE * NP * Q], Bzs[NP * Q], z2s[Q], wzs[Q];
+    __shared__ cpx<R> cin[TW * TW];   // input plane  [jy][jx]
+    __shared__ cpx<R> T1[TW * TQ];    // x stage      [jy][qx]
+    __shared__ cpx<R> G2p[NCOL];      // output plane at the points [qy][qx]
+    __shared__ cpx<R> H1[TW * TQ];    // transposed y [jy][qx]
+    __shared__ R Bxs[TE * NP * Q], Bys[TE * NP * Q], Bzs[NP * Q], z2s[Q], wzs[Q];
 
     const int tid = threadIdx.x;
     const int ex0 = blockIdx.x * TE, ey0 = blockIdx.y * TE;
@@ -190,7 +196,7 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
     }
     // per column constants: soft-core s V w = cst / sqrt(xy2 + z^2) wz with xy2 = x^2 + y^2 + a;
     // harmonic s V w = cst (xy2 + z^2) wz with xy2 = x^2 + y^2
-    double xy2[CPT], wxy[CPT], By[CPT][NP];
+    R xy2[CPT], wxy[CPT], By[CPT][NP];
     bool valid[CPT];
     int cqx[CPT], cey[CPT];
 #pragma unroll
@@ -203,15 +209,15 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
         cqx[c] = qx;
         cey[c] = eyl;
         const int exs = valid[c] ? ex : 0, eys = valid[c] ? ey : 0;
-        const double xv = ax.X[(long)exs * Q + q1], yv = ay.X[(long)eys * Q + q2];
-        const double w = ax.W[(long)exs * Q + q1] * ay.W[(long)eys * Q + q2];
+        const R xv = ax.X[(long)exs * Q + q1], yv = ay.X[(long)eys * Q + q2];
+        const R w = ax.W[(long)exs * Q + q1] * ay.W[(long)eys * Q + q2];
         xy2[c] = xv * xv + yv * yv + (KIND == 2 ? pp.a1 : 0.0);
         wxy[c] = KIND == 2 ? -pp.s * pp.a0 * w : 0.5 * pp.s * pp.a0 * w;
 #pragma unroll
         for (int a = 0; a < NP; ++a) By[c][a] = ay.B0[((long)eys * NP + a) * Q + q2];
     }
-    double tw[CPT][NP][2];   // input window: planes ez .. ez+P at my columns
-    double gw[CPT][NP][2];   // output window
+    R tw[CPT][NP][2];   // input window: planes ez .. ez+P at my columns
+    R gw[CPT][NP][2];   // output window
 #pragma unroll
     for (int c = 0; c < CPT; ++c)
 #pragma unroll
@@ -221,12 +227,12 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
     auto stage_in = [&](int jz, int slot, int ez) {
         for (int i = tid; i < TW * TW; i += NT) {
             const int jx = ex0 + i % TW, jy = ey0 + i / TW;
-            cin[i] = (jx < n0 && jy < n1) ? x[jx + (long)n0 * jy + plane * jz] : dcomplex{0.0, 0.0};
+            cin[i] = (jx < n0 && jy < n1) ? x[jx + (long)n0 * jy + plane * jz] : cpx<R>{0.0, 0.0};
         }
         if (ez >= 0) {
             for (int i = tid; i < NP * Q; i += NT) Bzs[i] = az.B0[(long)ez * NP * Q + i];
             for (int i = tid; i < Q; i += NT) {
-                const double zv = az.X[(long)ez * Q + i];
+                const R zv = az.X[(long)ez * Q + i];
                 z2s[i] = zv * zv;
                 wzs[i] = az.W[(long)ez * Q + i];
             }
@@ -234,11 +240,11 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
         __syncthreads();
         for (int i = tid; i < TW * TQ; i += NT) {
             const int qx = i % TQ, jy = i / TQ, exl = qx / Q, q1 = qx % Q;
-            double sr = 0.0, si = 0.0;
+            R sr = 0.0, si = 0.0;
 #pragma unroll
             for (int a = 0; a < NP; ++a) {
-                const double b = Bxs[(exl * NP + a) * Q + q1];
-                const dcomplex v = cin[jy * TW + exl + a];
+                const R b = Bxs[(exl * NP + a) * Q + q1];
+                const cpx<R> v = cin[jy * TW + exl + a];
                 sr += b * v.re;
                 si += b * v.im;
             }
@@ -247,10 +253,10 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-            double sr = 0.0, si = 0.0;
+            R sr = 0.0, si = 0.0;
 #pragma unroll
             for (int a = 0; a < NP; ++a) {
-                const dcomplex v = T1[(cey[c] + a) * TQ + cqx[c]];
+                const cpx<R> v = T1[(cey[c] + a) * TQ + cqx[c]];
                 sr += By[c][a] * v.re;
                 si += By[c][a] * v.im;
             }
@@ -263,7 +269,7 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
     auto emit = [&](int jz) {
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-            const double r = gw[c][0][0], i = gw[c][0][1];
+            const R r = gw[c][0][0], i = gw[c][0][1];
             G2p[tid + c * NT] = {gr * r - gi * i, gr * i + gi * r};
 #pragma unroll
             for (int a = 0; a < P; ++a) { gw[c][a][0] = gw[c][a + 1][0]; gw[c][a][1] = gw[c][a + 1][1]; }
@@ -272,15 +278,15 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
         __syncthreads();
         for (int i = tid; i < TW * TQ; i += NT) {
             const int qx = i % TQ, jy = i / TQ;
-            double sr = 0.0, si = 0.0;
+            R sr = 0.0, si = 0.0;
 #pragma unroll 1
             for (int eyl = 0; eyl < TE; ++eyl) {
                 const int a = jy - eyl;
                 if (a < 0 || a > P) continue;
 #pragma unroll
                 for (int q2 = 0; q2 < Q; ++q2) {
-                    const double b = Bys[(eyl * NP + a) * Q + q2];
-                    const dcomplex v = G2p[(eyl * Q + q2) * TQ + qx];
+                    const R b = Bys[(eyl * NP + a) * Q + q2];
+                    const cpx<R> v = G2p[(eyl * Q + q2) * TQ + qx];
                     sr += b * v.re;
                     si += b * v.im;
                 }
@@ -292,20 +298,20 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
             const int jxl = i % TW, jyl = i / TW;
             const int gx = ex0 + jxl, gy = ey0 + jyl;
             if (gx >= n0 || gy >= n1) continue;
-            double sr = 0.0, si = 0.0;
+            R sr = 0.0, si = 0.0;
 #pragma unroll 1
             for (int exl = 0; exl < TE; ++exl) {
                 const int a = jxl - exl;
                 if (a < 0 || a > P) continue;
 #pragma unroll
                 for (int q1 = 0; q1 < Q; ++q1) {
-                    const double b = Bxs[(exl * NP + a) * Q + q1];
-                    const dcomplex v = H1[jyl * TQ + exl * Q + q1];
+                    const R b = Bxs[(exl * NP + a) * Q + q1];
+                    const cpx<R> v = H1[jyl * TQ + exl * Q + q1];
                     sr += b * v.re;
                     si += b * v.im;
                 }
             }
-            double *dst = reinterpret_cast<double *>(y + gx + (long)n0 * gy + plane * jz);
+            R *dst = reinterpret_cast<R *>(y + gx + (long)n0 * gy + plane * jz);
             unsafeAtomicAdd(dst, sr);
             unsafeAtomicAdd(dst + 1, si);
         }
@@ -321,20 +327,20 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
             if (!valid[c]) continue;
 #pragma unroll 2
             for (int qz = 0; qz < Q; ++qz) {
-                double fr = 0.0, fi = 0.0;
+                R fr = 0.0, fi = 0.0;
 #pragma unroll
                 for (int a = 0; a < NP; ++a) {
-                    const double b = Bzs[a * Q + qz];
+                    const R b = Bzs[a * Q + qz];
                     fr += b * tw[c][a][0];
                     fi += b * tw[c][a][1];
                 }
-                const double r2 = xy2[c] + z2s[qz];
-                const double v = (KIND == 2 ? rsqrt_pos(r2) : r2) * (wxy[c] * wzs[qz]);
+                const R r2 = xy2[c] + z2s[qz];
+                const R v = (KIND == 2 ? rsqrt_pos(r2) : r2) * (wxy[c] * wzs[qz]);
                 fr *= v;
                 fi *= v;
 #pragma unroll
                 for (int a = 0; a < NP; ++a) {
-                    const double b = Bzs[a * Q + qz];
+                    const R b = Bzs[a * Q + qz];
                     gw[c][a][0] += b * fr;
                     gw[c][a][1] += b * fi;
                 }
@@ -352,8 +358,9 @@ pot3d_kernel(AxisDev ax, AxisDev ay, AxisDev az, PotParams pp, double gr, double
 
 // ------------------------------------------------------------------ separable
 // x pass: t = Mx x, u = Bx x (one thread per output)
-__global__ void sep_x_kernel(int n0, long lines, int p, const double *__restrict__ M, const dcomplex *__restrict__ B,
-                             const dcomplex *__restrict__ x, dcomplex *__restrict__ t, dcomplex *__restrict__ u)
+template <typename R>
+__global__ void sep_x_kernel(int n0, long lines, int p, const R *__restrict__ M, const cpx<R> *__restrict__ B,
+                             const cpx<R> *__restrict__ x, cpx<R> *__restrict__ t, cpx<R> *__restrict__ u)
 {
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= lines * n0) return;
@@ -361,11 +368,11 @@ __global__ void sep_x_kernel(int n0, long lines, int p, const double *__restrict
     const long base = I - i;
     const int bw = 2 * p + 1;
     const int lo = i - p < 0 ? 0 : i - p, hi = i + p > n0 - 1 ? n0 - 1 : i + p;
-    double tr = 0, ti = 0, ur = 0, ui = 0;
+    R tr = 0, ti = 0, ur = 0, ui = 0;
     for (int J = lo; J <= hi; ++J) {
-        const dcomplex v = x[base + J];
-        const double m = M[i * bw + (J - i + p)];
-        const dcomplex b = B[i * bw + (J - i + p)];
+        const cpx<R> v = x[base + J];
+        const R m = M[i * bw + (J - i + p)];
+        const cpx<R> b = B[i * bw + (J - i + p)];
         tr += m * v.re; ti += m * v.im;
         ur += b.re * v.re - b.im * v.im;
         ui += b.re * v.im + b.im * v.re;
@@ -375,9 +382,10 @@ __global__ void sep_x_kernel(int n0, long lines, int p, const double *__restrict
 }
 
 // y pass: a = My u + By t, b = My t
-__global__ void sep_y_kernel(int n0, int n1, long N, int p, const double *__restrict__ M,
-                             const dcomplex *__restrict__ B, const dcomplex *__restrict__ t,
-                             const dcomplex *__restrict__ u, dcomplex *__restrict__ a, dcomplex *__restrict__ b)
+template <typename R>
+__global__ void sep_y_kernel(int n0, int n1, long N, int p, const R *__restrict__ M,
+                             const cpx<R> *__restrict__ B, const cpx<R> *__restrict__ t,
+                             const cpx<R> *__restrict__ u, cpx<R> *__restrict__ a, cpx<R> *__restrict__ b)
 {
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= N) return;
@@ -386,12 +394,12 @@ __global__ void sep_y_kernel(int n0, int n1, long N, int p, const double *__rest
     const long k = I / ((long)n0 * n1);
     const int bw = 2 * p + 1;
     const int lo = j - p < 0 ? 0 : j - p, hi = j + p > n1 - 1 ? n1 - 1 : j + p;
-    double ar = 0, ai = 0, br = 0, bi = 0;
+    R ar = 0, ai = 0, br = 0, bi = 0;
     for (int J = lo; J <= hi; ++J) {
         const long src = i + (long)n0 * (J + (long)n1 * k);
-        const dcomplex tv = t[src], uv = u[src];
-        const double m = M[j * bw + (J - j + p)];
-        const dcomplex bb = B[j * bw + (J - j + p)];
+        const cpx<R> tv = t[src], uv = u[src];
+        const R m = M[j * bw + (J - j + p)];
+        const cpx<R> bb = B[j * bw + (J - j + p)];
         ar += m * uv.re + bb.re * tv.re - bb.im * tv.im;
         ai += m * uv.im + bb.re * tv.im + bb.im * tv.re;
         br += m * tv.re; bi += m * tv.im;
@@ -401,8 +409,9 @@ __global__ void sep_y_kernel(int n0, int n1, long N, int p, const double *__rest
 }
 
 // z pass: y = Mz a + Bz b
-__global__ void sep_z_kernel(long plane, int n2, int p, const double *__restrict__ M, const dcomplex *__restrict__ B,
-                             const dcomplex *__restrict__ a, const dcomplex *__restrict__ b, dcomplex *__restrict__ y)
+template <typename R>
+__global__ void sep_z_kernel(long plane, int n2, int p, const R *__restrict__ M, const cpx<R> *__restrict__ B,
+                             const cpx<R> *__restrict__ a, const cpx<R> *__restrict__ b, cpx<R> *__restrict__ y)
 {
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= plane * n2) return;
@@ -410,19 +419,20 @@ __global__ void sep_z_kernel(long plane, int n2, int p, const double *__restrict
     const int k = (int)(I / plane);
     const int bw = 2 * p + 1;
     const int lo = k - p < 0 ? 0 : k - p, hi = k + p > n2 - 1 ? n2 - 1 : k + p;
-    double yr = 0, yi = 0;
+    R yr = 0, yi = 0;
     for (int K = lo; K <= hi; ++K) {
-        const dcomplex av = a[m + plane * K], bv = b[m + plane * K];
-        const double mm = M[k * bw + (K - k + p)];
-        const dcomplex bb = B[k * bw + (K - k + p)];
+        const cpx<R> av = a[m + plane * K], bv = b[m + plane * K];
+        const R mm = M[k * bw + (K - k + p)];
+        const cpx<R> bb = B[k * bw + (K - k + p)];
         yr += mm * av.re + bb.re * bv.re - bb.im * bv.im;
         yi += mm * av.im + bb.re * bv.im + bb.im * bv.re;
     }
     y[I] = {yr, yi};
 }
 
+template <typename R>
 __global__ void wall_fix_kernel(int n0, int n1, long N, const unsigned char *w0, const unsigned char *w1,
-                                const unsigned char *w2, const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
+                                const unsigned char *w2, const cpx<R> *__restrict__ x, cpx<R> *__restrict__ y)
 {
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= N) return;
@@ -590,10 +600,59 @@ __global__ void __launch_bounds__(kRedThreads) norm2_kernel(const dcomplex *__re
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
+template <typename R>
+__global__ void axpby_t_kernel(cpx<R> a, const cpx<R> *__restrict__ x, cpx<R> b, cpx<R> *__restrict__ y, long N)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        const cpx<R> xv = x[I], yv = y[I];
+        y[I] = {a.re * xv.re - a.im * xv.im + b.re * yv.re - b.im * yv.im,
+                a.re * xv.im + a.im * xv.re + b.re * yv.im + b.im * yv.re};
+    }
+}
+
+// y (complex64) = x (complex128), rounded
+__global__ void demote_kernel(const dcomplex *__restrict__ x, fcomplex *__restrict__ y, long N)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        const dcomplex v = x[I];
+        y[I] = {(float)v.re, (float)v.im};
+    }
+}
+
+// y (complex128) = x (complex64)
+__global__ void promote_kernel(const fcomplex *__restrict__ x, dcomplex *__restrict__ y, long N)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        const fcomplex v = x[I];
+        y[I] = {(double)v.re, (double)v.im};
+    }
+}
+
+// q = r - g Vg .* My - c - n (n optional): the right-hand side of the second
+// separable solve of the first-order corrected preconditioner
+__global__ void corr_combine_kernel(const fcomplex *__restrict__ r, const fcomplex *__restrict__ My,
+                                    const fcomplex *__restrict__ c, const fcomplex *__restrict__ n,
+                                    const float *__restrict__ Vg, fcomplex g, fcomplex *__restrict__ q, long N)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        const fcomplex m = My[I], cv = c[I], rv = r[I];
+        const float v = Vg[I];
+        float qr = rv.re - v * (g.re * m.re - g.im * m.im) - cv.re;
+        float qi = rv.im - v * (g.re * m.im + g.im * m.re) - cv.im;
+        if (n) { qr -= n[I].re; qi -= n[I].im; }
+        q[I] = {qr, qi};
+    }
+}
+
 // --------------------------------------------------------------- dense axis
 // y[.., i, ..] = sum_k T(i, k) x[.., k, ..] along axis d; one thread per output.
-__global__ void dense_axis_kernel(int d, int n0, int n1, int n2, const double *__restrict__ T, int trans,
-                                  const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
+template <typename R>
+__global__ void dense_axis_kernel(int d, int n0, int n1, int n2, const R *__restrict__ T, int trans,
+                                  const cpx<R> *__restrict__ x, cpx<R> *__restrict__ y)
 {
     const long N = (long)n0 * n1 * n2;
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -605,10 +664,10 @@ __global__ void dense_axis_kernel(int d, int n0, int n1, int n2, const double *_
     if (d == 0) { n = n0; i = i0; stride = 1; base = I - i0; }
     else if (d == 1) { n = n1; i = i1; stride = n0; base = I - (long)i1 * n0; }
     else { n = n2; i = i2; stride = (long)n0 * n1; base = I - (long)i2 * n0 * n1; }
-    double sr = 0, si = 0;
+    R sr = 0, si = 0;
     for (int k = 0; k < n; ++k) {
-        const double tv = trans ? T[(long)k * n + i] : T[(long)i * n + k];
-        const dcomplex v = x[base + k * stride];
+        const R tv = trans ? T[(long)k * n + i] : T[(long)i * n + k];
+        const cpx<R> v = x[base + k * stride];
         sr += tv * v.re; si += tv * v.im;
     }
     y[I] = {sr, si};
@@ -620,18 +679,19 @@ __global__ void dense_axis_kernel(int d, int n0, int n1, int n2, const double *_
 // each of the 256 threads accumulates a 4 x 4 block. kfast: the input tile is
 // read with k fastest (sK = 1), else l fastest (sL = 1).
 constexpr int GT = 64, GK = 16;
+template <typename R>
 __global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, long sK, long sL, int kfast,
-                                                        const double *__restrict__ T, int trans,
-                                                        const dcomplex *__restrict__ in, dcomplex *__restrict__ out)
+                                                        const R *__restrict__ T, int trans,
+                                                        const cpx<R> *__restrict__ in, cpx<R> *__restrict__ out)
 {
-    __shared__ double Ts[GK][GT + 1];      // [k][i]
-    __shared__ dcomplex Is[GK][GT + 1];    // [k][l]
+    __shared__ R Ts[GK][GT + 1];      // [k][i]
+    __shared__ cpx<R> Is[GK][GT + 1];    // [k][l]
     const int tid = threadIdx.x;
     const long l0 = (long)blockIdx.x * GT;
     const int i0 = blockIdx.y * GT;
     const long boff = (long)blockIdx.z * sB;
     const int ti = tid / 16, tl = tid % 16;   // 16 x 16 threads, 4 x 4 outputs each
-    double ar[4][4], ai[4][4];
+    R ar[4][4], ai[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -641,7 +701,7 @@ __global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, 
             // T tile: Ts[k][i] = T(i0 + i, k0 + k)
             const int kk = e % GK, ii = e / GK;
             const int gi = i0 + ii, gk = k0 + kk;
-            double v = 0.0;
+            R v = 0.0;
             if (gi < n && gk < n) v = trans ? T[(long)gk * n + gi] : T[(long)gi * n + gk];
             Ts[kk][ii] = v;
             // input tile
@@ -649,15 +709,15 @@ __global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, 
             if (kfast) { kq = e % GK; lq = e / GK; } else { lq = e % GT; kq = e / GT; }
             const int gk2 = k0 + kq;
             const long gl = l0 + lq;
-            dcomplex u = {0.0, 0.0};
+            cpx<R> u = {0.0, 0.0};
             if (gk2 < n && gl < L) u = in[boff + gk2 * sK + gl * sL];
             Is[kq][lq] = u;
         }
         __syncthreads();
 #pragma unroll 4
         for (int kk = 0; kk < GK; ++kk) {
-            double tv[4];
-            dcomplex iv[4];
+            R tv[4];
+            cpx<R> iv[4];
 #pragma unroll
             for (int a = 0; a < 4; ++a) tv[a] = Ts[kk][ti + 16 * a];
 #pragma unroll
@@ -684,26 +744,28 @@ __global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, 
     }
 }
 
-__global__ void diag_scale_kernel(int n0, int n1, long N, const double *l0, const double *l1, const double *l2,
+template <typename R>
+__global__ void diag_scale_kernel(int n0, int n1, long N, const R *l0, const R *l1, const R *l2,
                                   const unsigned char *w0, const unsigned char *w1, const unsigned char *w2,
-                                  dcomplex alpha, dcomplex g, const dcomplex *__restrict__ x, dcomplex *__restrict__ y)
+                                  cpx<R> alpha, cpx<R> g, const cpx<R> *__restrict__ x, cpx<R> *__restrict__ y)
 {
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= N) return;
     const int i = (int)(I % n0), j = (int)((I / n0) % n1);
     const long k = I / ((long)n0 * n1);
-    const dcomplex v = x[I];
+    const cpx<R> v = x[I];
     if (w0[i] | w1[j] | w2[k]) { y[I] = v; return; }
-    const double lam = l0[i] + l1[j] + l2[k];
-    const double dr = alpha.re + g.re * lam, di = alpha.im + g.im * lam;
-    const double inv = 1.0 / (dr * dr + di * di);
+    const R lam = l0[i] + l1[j] + l2[k];
+    const R dr = alpha.re + g.re * lam, di = alpha.im + g.im * lam;
+    const R inv = R(1) / (dr * dr + di * di);
     y[I] = {(v.re * dr + v.im * di) * inv, (v.im * dr - v.re * di) * inv};
 }
 
 // --------------------------------------------------------------- sparse axis
+template <typename R>
 __global__ void csr_axis_kernel(int d, int a0, int a1, int a2, int m, const int *__restrict__ rowptr,
-                                const int *__restrict__ col, const double *__restrict__ val,
-                                const dcomplex *__restrict__ in, dcomplex *__restrict__ out, int accumulate)
+                                const int *__restrict__ col, const R *__restrict__ val,
+                                const cpx<R> *__restrict__ in, cpx<R> *__restrict__ out, int accumulate)
 {
     // output dims: axis d replaced by m
     const int o0 = d == 0 ? m : a0, o1 = d == 1 ? m : a1, o2 = d == 2 ? m : a2;
@@ -713,11 +775,11 @@ __global__ void csr_axis_kernel(int d, int a0, int a1, int a2, int m, const int 
     const int i0 = (int)(I % o0), i1 = (int)((I / o0) % o1);
     const int i2 = (int)(I / ((long)o0 * o1));
     const int r = d == 0 ? i0 : (d == 1 ? i1 : i2);
-    double sr = 0, si = 0;
+    R sr = 0, si = 0;
     for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) {
         const int c = col[e];
         const int j0 = d == 0 ? c : i0, j1 = d == 1 ? c : i1, j2 = d == 2 ? c : i2;
-        const dcomplex v = in[j0 + (long)a0 * (j1 + (long)a1 * j2)];
+        const cpx<R> v = in[j0 + (long)a0 * (j1 + (long)a1 * j2)];
         sr += val[e] * v.re; si += val[e] * v.im;
     }
     if (accumulate) { sr += out[I].re; si += out[I].im; }
@@ -734,32 +796,33 @@ inline unsigned grid_capped(long n, int bs, unsigned cap)
 }  // namespace
 
 // ===================================================================== launchers
-void launch_pot_apply_generic(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
-                              const dcomplex *x, dcomplex *y)
+template <typename R>
+void launch_pot_apply_generic(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
+                              cpx<R> *y)
 {
     const int n1 = ax[0].p + 1, n2 = ax[1].p + 1, n3 = ax[2].p + 1;
     const int Q1 = ax[0].Q, Q2 = ax[1].Q, Q3 = ax[2].Q;
-    size_t dbl = (size_t)(n1 * Q1 + n2 * Q2 + n3 * Q3 + 2 * (Q1 + Q2 + Q3));
-    dbl = (dbl + 1) & ~(size_t)1;
-    const size_t cpx = (size_t)n1 * n2 * n3 + (size_t)n3 * n2 * Q1 + (size_t)n3 * Q2 * Q1 + (size_t)Q3 * Q2 * Q1;
-    const size_t bytes = dbl * sizeof(double) + cpx * sizeof(dcomplex);
+    size_t nr = (size_t)(n1 * Q1 + n2 * Q2 + n3 * Q3 + 2 * (Q1 + Q2 + Q3));
+    nr = (nr + 1) & ~(size_t)1;
+    const size_t nc = (size_t)n1 * n2 * n3 + (size_t)n3 * n2 * Q1 + (size_t)n3 * Q2 * Q1 + (size_t)Q3 * Q2 * Q1;
+    const size_t bytes = nr * sizeof(R) + nc * sizeof(cpx<R>);
     const long nel = (long)ax[0].E * ax[1].E * ax[2].E;
-    hipLaunchKernelGGL(pot_apply_generic_kernel, dim3((unsigned)nel), dim3(128), bytes, st, ax[0], ax[1], ax[2], pot,
-                       gr, gi, x, y);
+    hipLaunchKernelGGL(pot_apply_generic_kernel<R>, dim3((unsigned)nel), dim3(128), bytes, st, ax[0], ax[1], ax[2],
+                       pot, gr, gi, x, y);
 }
 
-template <int P, int Q, int TE, int CPT>
-static void launch_pot3d(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
-                         const dcomplex *x, dcomplex *y)
+template <typename R, int P, int Q, int TE, int CPT>
+static void launch_pot3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
+                         cpx<R> *y)
 {
     constexpr int NT = (TE * Q) * (TE * Q) / CPT;
     const dim3 grid((unsigned)((ax[0].E + TE - 1) / TE), (unsigned)((ax[1].E + TE - 1) / TE));
     if (pot.kind == 2)
-        hipLaunchKernelGGL((pot3d_kernel<P, Q, TE, CPT, 2>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr, gi,
-                           x, y);
+        hipLaunchKernelGGL((pot3d_kernel<R, P, Q, TE, CPT, 2>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr,
+                           gi, x, y);
     else
-        hipLaunchKernelGGL((pot3d_kernel<P, Q, TE, CPT, 1>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr, gi,
-                           x, y);
+        hipLaunchKernelGGL((pot3d_kernel<R, P, Q, TE, CPT, 1>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr,
+                           gi, x, y);
 }
 
 int pot3d_variants(int p, int Q)
@@ -770,8 +833,9 @@ int pot3d_variants(int p, int Q)
     return 0;
 }
 
-bool launch_pot_apply_3d(hipStream_t st, const AxisDev ax[3], PotParams pot, double gr, double gi,
-                         const dcomplex *x, dcomplex *y, int variant)
+template <typename R>
+bool launch_pot_apply_3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
+                         cpx<R> *y, int variant)
 {
     const int p = ax[0].p, Q = ax[0].Q;
     for (int d = 1; d < 3; ++d)
@@ -780,36 +844,38 @@ bool launch_pot_apply_3d(hipStream_t st, const AxisDev ax[3], PotParams pot, dou
     if (variant < 0 || variant >= pot3d_variants(p, Q)) return false;
     // variant 0: 4 x 4 element tiles, 1024 threads; variant 1: 2 x 2 tiles, 256 threads
     if (p == 4 && Q == 8) {
-        if (variant == 0) launch_pot3d<4, 8, 4, 1>(st, ax, pot, gr, gi, x, y);
-        else launch_pot3d<4, 8, 2, 1>(st, ax, pot, gr, gi, x, y);
+        if (variant == 0) launch_pot3d<R, 4, 8, 4, 1>(st, ax, pot, gr, gi, x, y);
+        else launch_pot3d<R, 4, 8, 2, 1>(st, ax, pot, gr, gi, x, y);
         return true;
     }
     if (p == 3 && Q == 8) {
-        if (variant == 0) launch_pot3d<3, 8, 4, 1>(st, ax, pot, gr, gi, x, y);
-        else launch_pot3d<3, 8, 2, 1>(st, ax, pot, gr, gi, x, y);
+        if (variant == 0) launch_pot3d<R, 3, 8, 4, 1>(st, ax, pot, gr, gi, x, y);
+        else launch_pot3d<R, 3, 8, 2, 1>(st, ax, pot, gr, gi, x, y);
         return true;
     }
-    if (p == 3 && Q == 5) { launch_pot3d<3, 5, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
+    if (p == 3 && Q == 5) { launch_pot3d<R, 3, 5, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
     return false;
 }
 
-void launch_sep_apply(hipStream_t st, const SepBands &sb, const dcomplex *x, dcomplex *y, dcomplex *tmp[4])
+template <typename R>
+void launch_sep_apply(hipStream_t st, const SepBandsT<R> &sb, const cpx<R> *x, cpx<R> *y, cpx<R> *tmp[4])
 {
     const int n0 = sb.n[0], n1 = sb.n[1], n2 = sb.n[2];
     const long N = (long)n0 * n1 * n2;
     const int bs = 256;
-    hipLaunchKernelGGL(sep_x_kernel, dim3(grid_for(N, bs)), dim3(bs), 0, st, n0, N / n0, sb.p[0], sb.M[0], sb.B[0],
+    hipLaunchKernelGGL(sep_x_kernel<R>, dim3(grid_for(N, bs)), dim3(bs), 0, st, n0, N / n0, sb.p[0], sb.M[0], sb.B[0],
                        x, tmp[0], tmp[1]);
-    hipLaunchKernelGGL(sep_y_kernel, dim3(grid_for(N, bs)), dim3(bs), 0, st, n0, n1, N, sb.p[1], sb.M[1], sb.B[1],
+    hipLaunchKernelGGL(sep_y_kernel<R>, dim3(grid_for(N, bs)), dim3(bs), 0, st, n0, n1, N, sb.p[1], sb.M[1], sb.B[1],
                        tmp[0], tmp[1], tmp[2], tmp[3]);
-    hipLaunchKernelGGL(sep_z_kernel, dim3(grid_for(N, bs)), dim3(bs), 0, st, (long)n0 * n1, n2, sb.p[2], sb.M[2],
+    hipLaunchKernelGGL(sep_z_kernel<R>, dim3(grid_for(N, bs)), dim3(bs), 0, st, (long)n0 * n1, n2, sb.p[2], sb.M[2],
                        sb.B[2], tmp[2], tmp[3], y);
 }
 
-void launch_wall_fix(hipStream_t st, const int n[3], const unsigned char *wall[3], const dcomplex *x, dcomplex *y)
+template <typename R>
+void launch_wall_fix(hipStream_t st, const int n[3], const unsigned char *wall[3], const cpx<R> *x, cpx<R> *y)
 {
     const long N = (long)n[0] * n[1] * n[2];
-    hipLaunchKernelGGL(wall_fix_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st, n[0], n[1], N, wall[0], wall[1],
+    hipLaunchKernelGGL(wall_fix_kernel<R>, dim3(grid_for(N, 256)), dim3(256), 0, st, n[0], n[1], N, wall[0], wall[1],
                        wall[2], x, y);
 }
 
@@ -840,6 +906,29 @@ void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dco
     hipLaunchKernelGGL(axpby_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, a, x, b, y, N);
 }
 
+template <typename R>
+void launch_axpby_t(hipStream_t st, cpx<R> a, const cpx<R> *x, cpx<R> b, cpx<R> *y, long N)
+{
+    hipLaunchKernelGGL(axpby_t_kernel<R>, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, a, x, b, y, N);
+}
+
+void launch_demote(hipStream_t st, const dcomplex *x, fcomplex *y, long N)
+{
+    hipLaunchKernelGGL(demote_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, x, y, N);
+}
+
+void launch_promote(hipStream_t st, const fcomplex *x, dcomplex *y, long N)
+{
+    hipLaunchKernelGGL(promote_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, x, y, N);
+}
+
+void launch_corr_combine(hipStream_t st, const fcomplex *r, const fcomplex *My, const fcomplex *c,
+                         const fcomplex *n, const float *Vg, fcomplex g, fcomplex *q, long N)
+{
+    hipLaunchKernelGGL(corr_combine_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, r, My, c, n, Vg, g, q,
+                       N);
+}
+
 void launch_scale_invsqrt(hipStream_t st, const dcomplex *x, const double *norm2, dcomplex *y, long N)
 {
     hipLaunchKernelGGL(scale_invsqrt_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, x, norm2, y, N);
@@ -852,8 +941,8 @@ void launch_norm2(hipStream_t st, const dcomplex *x, long N, double *partial, do
     hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial, (int)nb, out);
 }
 
-void launch_dense_axis(hipStream_t st, int d, const int n[3], const double *T, bool trans, const dcomplex *x,
-                       dcomplex *y)
+template <typename R>
+void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool trans, const cpx<R> *x, cpx<R> *y)
 {
     // batched GEMM view of the transform along axis d
     const long n01 = (long)n[0] * n[1];
@@ -864,32 +953,55 @@ void launch_dense_axis(hipStream_t st, int d, const int n[3], const double *T, b
     else if (d == 1) { L = n[0]; sB = n01; sK = n[0]; sL = 1; B = (unsigned)n[2]; kfast = 0; }
     else { L = n01; sB = 0; sK = n01; sL = 1; B = 1; kfast = 0; }
     const dim3 grid((unsigned)((L + GT - 1) / GT), (unsigned)((n[d] + GT - 1) / GT), B);
-    hipLaunchKernelGGL(axis_gemm_kernel, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T, trans ? 1 : 0, x,
+    hipLaunchKernelGGL(axis_gemm_kernel<R>, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T, trans ? 1 : 0, x,
                        y);
 }
 
-void launch_dense_axis_simple(hipStream_t st, int d, const int n[3], const double *T, bool trans,
-                              const dcomplex *x, dcomplex *y)
+template <typename R>
+void launch_dense_axis_simple(hipStream_t st, int d, const int n[3], const R *T, bool trans, const cpx<R> *x,
+                              cpx<R> *y)
 {
     const long N = (long)n[0] * n[1] * n[2];
-    hipLaunchKernelGGL(dense_axis_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st, d, n[0], n[1], n[2], T,
+    hipLaunchKernelGGL(dense_axis_kernel<R>, dim3(grid_for(N, 256)), dim3(256), 0, st, d, n[0], n[1], n[2], T,
                        trans ? 1 : 0, x, y);
 }
 
-void launch_diag_scale(hipStream_t st, const int n[3], const double *lam[3], const unsigned char *wall[3],
-                       dcomplex alpha, dcomplex g, const dcomplex *x, dcomplex *y)
+template <typename R>
+void launch_diag_scale(hipStream_t st, const int n[3], const R *lam[3], const unsigned char *wall[3], cpx<R> alpha,
+                       cpx<R> g, const cpx<R> *x, cpx<R> *y)
 {
     const long N = (long)n[0] * n[1] * n[2];
-    hipLaunchKernelGGL(diag_scale_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st, n[0], n[1], N, lam[0], lam[1],
+    hipLaunchKernelGGL(diag_scale_kernel<R>, dim3(grid_for(N, 256)), dim3(256), 0, st, n[0], n[1], N, lam[0], lam[1],
                        lam[2], wall[0], wall[1], wall[2], alpha, g, x, y);
 }
 
+template <typename R>
 void launch_csr_axis(hipStream_t st, int d, const int nin[3], int m, const int *rowptr, const int *col,
-                     const double *val, const dcomplex *in, dcomplex *out, bool accumulate)
+                     const R *val, const cpx<R> *in, cpx<R> *out, bool accumulate)
 {
     const long N = (long)(d == 0 ? m : nin[0]) * (d == 1 ? m : nin[1]) * (d == 2 ? m : nin[2]);
-    hipLaunchKernelGGL(csr_axis_kernel, dim3(grid_for(N, 256)), dim3(256), 0, st, d, nin[0], nin[1], nin[2], m,
+    hipLaunchKernelGGL(csr_axis_kernel<R>, dim3(grid_for(N, 256)), dim3(256), 0, st, d, nin[0], nin[1], nin[2], m,
                        rowptr, col, val, in, out, accumulate ? 1 : 0);
 }
+
+#define TDZ_INSTANTIATE(R)                                                                                          \
+    template void launch_pot_apply_generic<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,  \
+                                              cpx<R> *);                                                           \
+    template bool launch_pot_apply_3d<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,       \
+                                         cpx<R> *, int);                                                           \
+    template void launch_sep_apply<R>(hipStream_t, const SepBandsT<R> &, const cpx<R> *, cpx<R> *, cpx<R> *[4]);   \
+    template void launch_wall_fix<R>(hipStream_t, const int[3], const unsigned char *[3], const cpx<R> *,          \
+                                     cpx<R> *);                                                                    \
+    template void launch_axpby_t<R>(hipStream_t, cpx<R>, const cpx<R> *, cpx<R>, cpx<R> *, long);                  \
+    template void launch_dense_axis<R>(hipStream_t, int, const int[3], const R *, bool, const cpx<R> *, cpx<R> *); \
+    template void launch_dense_axis_simple<R>(hipStream_t, int, const int[3], const R *, bool, const cpx<R> *,     \
+                                              cpx<R> *);                                                           \
+    template void launch_diag_scale<R>(hipStream_t, const int[3], const R *[3], const unsigned char *[3], cpx<R>,  \
+                                       cpx<R>, const cpx<R> *, cpx<R> *);                                          \
+    template void launch_csr_axis<R>(hipStream_t, int, const int[3], int, const int *, const int *, const R *,     \
+                                     const cpx<R> *, cpx<R> *, bool);
+
+TDZ_INSTANTIATE(double)
+TDZ_INSTANTIATE(float)
 
 }  // namespace tdz

@@ -94,11 +94,11 @@ def test_separable_preconditioner(pair):
     assert _rel(dz.download(), ref) < 1e-10, name
 
 
-@pytest.mark.parametrize("pc", [1, 2])
+@pytest.mark.parametrize("pc", [1, 2, 3, 4, 5, 6])
 def test_ksp_solve_true_residual(pair, pc):
     import tdsez_amd as T
     name, desc, o, ctx = pair
-    if pc == T.PC_MG and ctx.levels < 2:
+    if pc in (T.PC_MG, T.PC_MG_MIXED) and ctx.levels < 2:
         pytest.skip("grid cannot be coarsened")
     alpha, g = 1.0, 0.02j
     sE = np.array([0.0, 0.0, 0.02])
@@ -132,3 +132,23 @@ def test_cn_steps_match_oracle(pair):
     got = d.download()
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert err < 1e-11, (name, err)
+
+
+@pytest.mark.parametrize("kinds", [(1, 3), (2, 4)])
+def test_mixed_precision_preconditioner_matches_fp64(pair, kinds):
+    """The complex64 preconditioners compute the same map as the FP64 ones to
+    single-precision accuracy."""
+    import tdsez_amd as T
+    name, desc, o, ctx = pair
+    if kinds[0] == T.PC_MG and ctx.levels < 2:
+        pytest.skip("grid cannot be coarsened")
+    ctx.set_shift(1.0, 0.02j, (0, 0, 0))
+    r = _rand(o.N, 9)
+    r[_wall_mask(o, desc)] = 0.0
+    dr, z64, z32 = ctx.vec(r), ctx.vec(), ctx.vec()
+    ctx.pc_setup(kinds[0])
+    ctx.pc_apply(dr, z64)
+    ctx.pc_setup(kinds[1])
+    ctx.pc_apply(dr, z32)
+    a, b = z64.download(), z32.download()
+    assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-4, (name, np.linalg.norm(a - b) / np.linalg.norm(a))

@@ -1,0 +1,187 @@
+"""The learned potential-product network: oracle gradients pinned by finite
+differences (CPU); GPU input features, forward (FP32 and BF16 MFMA) and GPU
+gradients against the oracle; training gain, the ablation against the method
+without the network, and the plain-multigrid fallback (GPU)."""
+import numpy as np
+import pytest
+
+from oracle import cnn
+
+C, CI = 16, 4
+
+
+def _rand_net(seed, n=(5, 4, 6)):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((CI,) + n[::-1])
+    t = rng.standard_normal(n[::-1])
+    W1 = rng.standard_normal((27, CI, C)) * 0.3
+    W2 = rng.standard_normal((27, C, C)) * 0.05
+    P1 = rng.standard_normal((C,) + n[::-1])
+    P2 = rng.standard_normal((C,) + n[::-1])
+    return u, t, W1, W2, P1, P2
+
+
+def test_oracle_gradients_finite_difference():
+    u, t, W1, W2, P1, P2 = _rand_net(0)
+    loss, dW1, dW2, _ = cnn.loss_and_grad(u, t, W1, W2, P1, P2)
+    rng = np.random.default_rng(1)
+    for _ in range(6):
+        i, c, a = rng.integers(27), rng.integers(CI), rng.integers(C)
+        e = 1e-6
+        Wp, Wm = W1.copy(), W1.copy()
+        Wp[i, c, a] += e
+        Wm[i, c, a] -= e
+        fd = (cnn.loss_and_grad(u, t, Wp, W2, P1, P2)[0] - cnn.loss_and_grad(u, t, Wm, W2, P1, P2)[0]) / (2 * e)
+        assert abs(fd - dW1[i, c, a]) < 1e-6 * max(1.0, abs(fd)), (fd, dW1[i, c, a])
+        i, a, b = rng.integers(27), rng.integers(C), rng.integers(C)
+        Wp, Wm = W2.copy(), W2.copy()
+        Wp[i, a, b] += e
+        Wm[i, a, b] -= e
+        fd = (cnn.loss_and_grad(u, t, W1, Wp, P1, P2)[0] - cnn.loss_and_grad(u, t, W1, Wm, P1, P2)[0]) / (2 * e)
+        assert abs(fd - dW2[i, a, b]) < 1e-6 * max(1.0, abs(fd)), (fd, dW2[i, a, b])
+
+
+def test_oracle_shift_convention():
+    u = np.arange(24, dtype=float).reshape(2, 3, 4)
+    s = cnn.shift(u, 0, 0, 1)          # s(v) = u(v + x)
+    assert s[0, 0, 0] == u[0, 0, 1] and s[0, 0, 3] == 0.0
+    s = cnn.shift(u, 1, 0, 0)
+    assert s[0, 1, 2] == u[1, 1, 2] and np.all(s[1] == 0.0)
+
+
+SMALL = dict(dim=3, nelem=(6, 5, 7), lo=(-8.0, -7.0, -9.0), hi=(8.0, 7.0, 9.0), p=4, nquad=8, mass=1.0,
+             charge=-1.0, pot_kind=2, pot=(1.0, 2.0), cap_kind=1, cap=(0.01, 3.0))
+
+
+@pytest.mark.gpu
+def test_gpu_features_match_oracle():
+    """u = [y, X_d y - x_d M y] in complex64 against the oracle's FP64 operators."""
+    import tdsez_amd as T
+    from oracle import Oracle, greville
+    o, ctx = Oracle(SMALL), T.Context(SMALL)
+    rng = np.random.default_rng(4)
+    y = rng.standard_normal(ctx.N) + 1j * rng.standard_normal(ctx.N)
+    got = ctx.learn_features(ctx.vec(y))
+    gr = [greville(o.knots(d), SMALL["p"]) for d in range(3)]
+    ref = cnn.features(o, y, gr)
+    for c in range(CI):
+        err = np.linalg.norm(got[c] - ref[c]) / np.linalg.norm(ref[c])
+        assert err < 1e-5, (c, err)
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_forward_and_gradients_match_oracle():
+    import tdsez_amd as T
+    ctx = T.Context(SMALL)
+    n0, n1, n2 = ctx.n
+    rng = np.random.default_rng(3)
+    W1 = (rng.standard_normal((27, CI, C)) * 0.3).astype(np.float32)
+    W2 = (rng.standard_normal((27, C, C)) * 0.05).astype(np.float32)
+    ctx.learn_weights(W1, W2)
+    P = ctx.learn_fields().astype(np.float64)                 # [N][C]
+    Pg = P.T.reshape(C, n2, n1, n0)
+    u = rng.standard_normal((CI, ctx.N)) + 1j * rng.standard_normal((CI, ctx.N))
+    t = rng.standard_normal(ctx.N) + 1j * rng.standard_normal(ctx.N)
+    loss, dW1, dW2, of, om = ctx.learn_eval(u, t)
+    u32 = u.astype(np.complex64).astype(np.complex128)
+    t32 = t.astype(np.complex64).astype(np.complex128)
+    T2 = np.sum(np.abs(t32) ** 2)
+    ref_loss, g1, g2, outs = 0.0, 0.0, 0.0, []
+    for comp in (np.real, np.imag):
+        uu = comp(u32).reshape(CI, n2, n1, n0)
+        tt = comp(t32).reshape(n2, n1, n0)
+        out, _ = cnn.forward(uu, W1.astype(np.float64), W2.astype(np.float64), Pg, Pg)
+        outs.append(out.reshape(-1))
+        l, a1, a2, _ = cnn.loss_and_grad(uu, tt, W1.astype(np.float64), W2.astype(np.float64), Pg, Pg)
+        # the oracle normalises per component; the GPU by |t|^2 over both components
+        s = np.sum(tt ** 2) / T2
+        ref_loss += l * s
+        g1 = g1 + a1 * s
+        g2 = g2 + a2 * s
+    ref_out = np.concatenate(outs)
+    assert np.linalg.norm(of - ref_out) / np.linalg.norm(ref_out) < 1e-5
+    assert abs(loss - ref_loss) < 1e-5 * ref_loss
+    assert np.linalg.norm(dW1 - g1) / np.linalg.norm(g1) < 1e-4
+    assert np.linalg.norm(dW2 - g2) / np.linalg.norm(g2) < 1e-4
+    # BF16 MFMA inference: same map to bfloat16 accuracy
+    assert np.linalg.norm(om - ref_out) / np.linalg.norm(ref_out) < 2e-2
+    ctx.close()
+
+
+def _mnorm(ctx, x):
+    """L2 norm of the wavefunction with coefficients x: sqrt(x^H M x)."""
+    import tdsez_amd as T
+    dx, dy = ctx.vec(x), ctx.vec()
+    ctx.term(T.TERM_M, dx, dy)
+    return float(np.sqrt(np.vdot(x, dy.download()).real))
+
+
+MID = dict(dim=3, nelem=(24, 24, 24), lo=(-7.5,) * 3, hi=(7.5,) * 3, p=4, nquad=8, mass=1.0, charge=-1.0,
+           pot_kind=2, pot=(1.0, 2.0), cap_kind=1, cap=(0.01, 2.5), mg_levels=2)
+
+
+@pytest.mark.gpu
+def test_learned_preconditioner_gain_and_fallback():
+    """Train on the solver's own vectors. The trained network must capture most of the
+    potential product the lumped form misses, contract residuals more than the method
+    without it (the ablation), never need more Krylov iterations, and leave the
+    solution unchanged (the FP64 residual judges). A broken network regresses and
+    plain multigrid takes over with the same answer."""
+    import tdsez_amd as T
+    ctx = T.Context(MID)
+    psi = ctx.vec()
+    ctx.project_gaussian(psi, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    pulse = T.make_pulse(E0=(0.0, 0.0, 0.05), omega=0.057, ncycles=4.0)
+    rtol = 1e-13   # the regression threshold is measured at the tolerance of the runs below
+    l0, l1 = ctx.learn_train(psi, 0.0, 0.02, pulse, harvest_steps=2, max_samples=6, iters=150, lr=3e-3, rtol=rtol)
+    print("loss per sample before/after the fit", l0, l1)
+    assert l1 < l0 and l1 < 0.5, (l0, l1)
+    # ablation: one preconditioner apply on the initial CN residuals -2 g H_E phi of
+    # states the network has not seen, residual of the CN operator
+    g, sE = 0.01j, (0.0, 0.0, 0.03)
+    rs = []
+    for centre, momentum in (((0.5, -0.3, 0.2), (0.3, 0.0, -0.2)), ((-0.8, 0.4, 0.0), (0.0, 0.5, 0.4))):
+        phi, hphi = ctx.vec(), ctx.vec()
+        ctx.project_gaussian(phi, centre, (1.2, 1.0, 1.1), momentum)
+        ctx.set_shift(0.0, 1.0, sE)
+        ctx.apply(phi, hphi)
+        rs.append(-2.0 * g * hphi.download())
+    ctx.set_shift(1.0, g, sE)
+    contraction = {}
+    for kind in (T.PC_CORRECTED, T.PC_LEARNED):
+        ctx.pc_setup(kind)
+        worst = 0.0
+        for r in rs:
+            dr, dz, dw = ctx.vec(r), ctx.vec(), ctx.vec()
+            ctx.pc_apply(dr, dz)
+            ctx.apply(dz, dw)
+            worst = max(worst, np.linalg.norm(r - dw.download()) / np.linalg.norm(r))
+        contraction[kind] = worst
+    print("contraction without / with the network", contraction)
+    assert contraction[T.PC_LEARNED] < 0.8 * contraction[T.PC_CORRECTED], contraction
+    x0 = psi.download()
+    runs = {}
+    for kind in (T.PC_CORRECTED, T.PC_LEARNED):
+        ctx.pc_setup(kind)
+        d = ctx.vec(x0)
+        its, _ = ctx.evolve_cn(d, 0.0, 0.02, 5, pulse, rtol=rtol)
+        runs[kind] = (its, d.download())
+    its_c, xc = runs[T.PC_CORRECTED]
+    its_l, xl = runs[T.PC_LEARNED]
+    print("iterations without / with the network", its_c, its_l)
+    assert its_l.max() <= its_c.max(), (its_l, its_c)
+    assert ctx.learn_status()[1] == 0
+    # same wavefunction in L2 (the coefficient vector itself is only fixed to
+    # cond(M) times the solver tolerance)
+    assert _mnorm(ctx, xl - xc) / _mnorm(ctx, xc) < 1e-10
+    # a broken network regresses: plain multigrid takes over and the answer is unchanged
+    W1, W2 = ctx.learn_weights()
+    ctx.learn_weights(W1 * 50.0, W2)
+    ctx.pc_setup(T.PC_LEARNED)
+    d = ctx.vec(x0)
+    its_b, _ = ctx.evolve_cn(d, 0.0, 0.02, 5, pulse, rtol=rtol)
+    _, fallbacks, active = ctx.learn_status()
+    assert fallbacks >= 1 and active == 0
+    assert _mnorm(ctx, d.download() - xc) / _mnorm(ctx, xc) < 1e-10
+    ctx.close()
