@@ -135,7 +135,7 @@ struct LearnState {
     fcomplex *nvec = nullptr;            // g * network output
     // training
     float *m = nullptr, *dov = nullptr, *dh = nullptr, *partial = nullptr, *dW1 = nullptr, *dW2 = nullptr;
-    float *lpart = nullptr;              // loss partial sums
+    double *lpart = nullptr;             // loss partial sums [kLossBlocks]
     std::vector<fcomplex *> ys, ts;      // harvested samples: network input [CI][N] and target t
     std::vector<double> tnorm2;          // |t|^2 per sample
     int max_samples = 0;
@@ -1527,6 +1527,7 @@ void setup_training_buffers(tdz_ctx *c)
     ls.dh = track(c, dalloc<float>((size_t)2 * N * C));
     ls.partial = track(c, dalloc<float>((size_t)256 * 27 * C * C));
     ls.dW1 = track(c, dalloc<float>(27 * tdz::kCnnInputs * C));
+    ls.lpart = track(c, dalloc<double>(tdz::kLossBlocks));
     ls.dW2 = track(c, dalloc<float>(27 * C * C));
 }
 
@@ -1541,22 +1542,17 @@ double learned_loss_grad(tdz_ctx *c, std::vector<double> &g1, std::vector<double
     const int n3[3] = {L.n[0], L.n[1], L.n[2]};
     g1.assign(27 * tdz::kCnnInputs * C, 0.0);
     g2.assign(27 * C * C, 0.0);
-    std::vector<float> h1(27 * tdz::kCnnInputs * C), h2(27 * C * C), hout;
+    std::vector<float> h1(27 * tdz::kCnnInputs * C), h2(27 * C * C);
     double loss = 0;
     const int nchunk = 256;
     for (size_t s = 0; s < ls.ys.size(); ++s) {
         tdz::cnn_forward_f32(c->st, n3, ls.ys[s], ls.W1, ls.W2, ls.P, ls.m, ls.out);
-        // loss on the host from the outputs (2N floats) and the target
-        hout.resize((size_t)2 * N);
-        std::vector<fcomplex> tt(N);
-        HIPCHK(hipMemcpyAsync(hout.data(), ls.out, hout.size() * 4, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipMemcpyAsync(tt.data(), ls.ts[s], N * sizeof(fcomplex), hipMemcpyDeviceToHost, c->st));
+        tdz::cnn_loss(c->st, N, ls.out, ls.ts[s], ls.lpart);
+        std::vector<double> lp(tdz::kLossBlocks);
+        HIPCHK(hipMemcpyAsync(lp.data(), ls.lpart, lp.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         double l = 0;
-        for (long v = 0; v < N; ++v) {
-            const double dr = hout[v] - tt[v].re, di = hout[N + v] - tt[v].im;
-            l += dr * dr + di * di;
-        }
+        for (double v : lp) l += v;
         loss += l / ls.tnorm2[s];
         if (!want_grad) continue;
         tdz::cnn_backward_f32(c->st, n3, ls.ys[s], ls.ts[s], (float)(1.0 / ls.tnorm2[s]), ls.W2, ls.P, ls.m, ls.out,

@@ -163,6 +163,9 @@ void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const f
 void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
                     const float *P, unsigned short *mb, float *out, fcomplex g, fcomplex *nout);
 void cnn_to_complex(hipStream_t st, long N, const float *out, fcomplex g, fcomplex *n);
+// partial[b] (b < kLossBlocks, FP64) sums to sum (out - t)^2 over both components
+constexpr int kLossBlocks = 256;
+void cnn_loss(hipStream_t st, long N, const float *out, const fcomplex *target, double *partial);
 // t = pv - Vg .* My
 void lumped_residual(hipStream_t st, long N, const fcomplex *pv, const fcomplex *My, const float *Vg, fcomplex *t);
 // u[c] *= v[c], c < kCnnInputs (in place)

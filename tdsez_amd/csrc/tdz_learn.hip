@@ -149,48 +149,127 @@ __global__ void __launch_bounds__(256) cnn_dm_kernel(int n0, int n1, int n2, con
     for (int a = 0; a < C; ++a) dh[I * C + a] = P[v * C + a] * g[a];
 }
 
-// Weight gradients as partial sums per block: block (t, chunk) accumulates over its voxel chunk.
-// dW2[t][a][b] = sum_{comp, v} m[comp][v + t][a] do[comp][v][b]     (256 threads = (a, b))
-__global__ void __launch_bounds__(256) cnn_dw2_kernel(int n0, int n1, int n2, int nchunk,
-                                                      const float *__restrict__ m, const float *__restrict__ dov,
-                                                      float *__restrict__ partial)
+// Weight gradients on MFMA (v_mfma_f32_16x16x4_f32, FP32 in and out). Operand layout
+// (wave64): lane l holds A[l%16][l/16], B[l/16][l%16] and D[4(l/16)+q][l%16], q < 4.
+// One MFMA consumes 4 consecutive entries I of the flattened [comp][v] index (k = l/16).
+// Block b covers a contiguous range of such groups, its 4 waves take every 4th group;
+// the waves' accumulators are summed through LDS in wave order and the block writes
+// its partial sums, reduced over the blocks afterwards in a fixed order (deterministic).
+typedef float floatx4g __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void voxel_of(long I, long N, int n0, int n1, int &comp, int &i, int &j, int &k)
 {
-    const int t = blockIdx.x, chunk = blockIdx.y;
-    const int a = threadIdx.x / C, b = threadIdx.x % C;
-    const long N = (long)n0 * n1 * n2;
-    const long per = (2 * N + nchunk - 1) / nchunk;
-    const long I0 = (long)chunk * per, I1 = I0 + per < 2 * N ? I0 + per : 2 * N;
-    float s = 0.f;
-    for (long I = I0; I < I1; ++I) {
-        const int comp = (int)(I / N);
-        const long v = I % N;
-        const int i = (int)(v % n0), j = (int)((v / n0) % n1), k = (int)(v / ((long)n0 * n1));
-        const long q = nb(i, j, k, t, n0, n1, n2);
-        if (q < 0) continue;
-        s += m[((long)comp * N + q) * C + a] * dov[I * C + b];
-    }
-    partial[((long)chunk * NTAP + t) * C * C + a * C + b] = s;
+    comp = (int)(I / N);
+    const long v = I % N;
+    i = (int)(v % n0);
+    j = (int)((v / n0) % n1);
+    k = (int)(v / ((long)n0 * n1));
 }
 
-// dW1[t][c][a] = sum_{comp, v} u_c,comp(v + t) dh[comp][v][a]   (block (chunk, c): C threads per tap)
-__global__ void __launch_bounds__(NTAP * C) cnn_dw1_kernel(int n0, int n1, int n2, int nchunk,
-                                                           const fcomplex *__restrict__ u,
-                                                           const float *__restrict__ dh, float *__restrict__ partial)
+// dW2[t][a][b] = sum_I m[comp][v + t][a] do[I][b]: A[a][k] = m(v_k + t)[a], B[k][b] = do[I_k][b]
+__global__ void __launch_bounds__(256) cnn_dw2_mfma_kernel(int n0, int n1, int n2, const float *__restrict__ m,
+                                                           const float *__restrict__ dov,
+                                                           float *__restrict__ partial)
 {
-    const int t = threadIdx.x / C, a = threadIdx.x % C, chunk = blockIdx.x, c = blockIdx.y;
+    __shared__ float red[NTAP * C * C];
     const long N = (long)n0 * n1 * n2;
-    const long per = (2 * N + nchunk - 1) / nchunk;
-    const long I0 = (long)chunk * per, I1 = I0 + per < 2 * N ? I0 + per : 2 * N;
-    float s = 0.f;
-    for (long I = I0; I < I1; ++I) {
-        const int comp = (int)(I / N);
-        const long v = I % N;
-        const int i = (int)(v % n0), j = (int)((v / n0) % n1), k = (int)(v / ((long)n0 * n1));
-        const long q = nb(i, j, k, t, n0, n1, n2);
-        if (q < 0) continue;
-        s += comp_of(u[(long)c * N + q], comp) * dh[I * C + a];
+    const long groups = (2 * N + 3) / 4;
+    const long per = (groups + gridDim.x - 1) / gridDim.x;
+    const long g0 = (long)blockIdx.x * per, g1 = g0 + per < groups ? g0 + per : groups;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int col = lane & 15, kk = lane >> 4;
+    floatx4g acc[NTAP];
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) acc[t] = {0.f, 0.f, 0.f, 0.f};
+    for (long g = g0 + w; g < g1; g += 4) {
+        const long I = 4 * g + kk;
+        float b = 0.f;
+        int comp = 0, i = 0, j = 0, k = 0;
+        const bool in = I < 2 * N;
+        if (in) {
+            b = dov[I * C + col];
+            voxel_of(I, N, n0, n1, comp, i, j, k);
+        }
+#pragma unroll
+        for (int t = 0; t < NTAP; ++t) {
+            float a = 0.f;
+            if (in) {
+                const long q = nb(i, j, k, t, n0, n1, n2);
+                if (q >= 0) a = m[((long)comp * N + q) * C + col];
+            }
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+        }
     }
-    partial[((long)chunk * NTAP + t) * CI * C + c * C + a] = s;
+    // lane holds D[4 kk + q][col] = (a = 4 kk + q, b = col)
+    for (int ww = 0; ww < 4; ++ww) {
+        if (w == ww)
+#pragma unroll
+            for (int t = 0; t < NTAP; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    float &r = red[(t * C + 4 * kk + q) * C + col];
+                    r = ww == 0 ? acc[t][q] : r + acc[t][q];
+                }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < NTAP * C * C; e += blockDim.x) partial[(long)blockIdx.x * NTAP * C * C + e] = red[e];
+}
+
+// dW1[t][c][a] = sum_I u_c,comp(v + t) dh[I][a]. Rows of A pair 4 taps with the CI = 4
+// channels: A[(t', c)][k] = u_c(v_k + tap 4 s + t'), 7 steps s cover the 27 taps.
+__global__ void __launch_bounds__(256) cnn_dw1_mfma_kernel(int n0, int n1, int n2, const fcomplex *__restrict__ u,
+                                                           const float *__restrict__ dh,
+                                                           float *__restrict__ partial)
+{
+    static_assert(CI == 4, "4 taps x CI channels fill the 16 rows of A");
+    constexpr int S = (NTAP + 3) / 4;
+    __shared__ float red[NTAP * CI * C];
+    const long N = (long)n0 * n1 * n2;
+    const long groups = (2 * N + 3) / 4;
+    const long per = (groups + gridDim.x - 1) / gridDim.x;
+    const long g0 = (long)blockIdx.x * per, g1 = g0 + per < groups ? g0 + per : groups;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int col = lane & 15, kk = lane >> 4;
+    const int tq = col >> 2, c = col & 3;   // this lane's A row: tap offset t' and channel
+    floatx4g acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = {0.f, 0.f, 0.f, 0.f};
+    for (long g = g0 + w; g < g1; g += 4) {
+        const long I = 4 * g + kk;
+        float b = 0.f;
+        int comp = 0, i = 0, j = 0, k = 0;
+        const bool in = I < 2 * N;
+        if (in) {
+            b = dh[I * C + col];
+            voxel_of(I, N, n0, n1, comp, i, j, k);
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int t = 4 * s + tq;
+            float a = 0.f;
+            if (in && t < NTAP) {
+                const long q = nb(i, j, k, t, n0, n1, n2);
+                if (q >= 0) a = comp_of(u[(long)c * N + q], comp);
+            }
+            acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[s], 0, 0, 0);
+        }
+    }
+    // lane holds D[4 kk + q][col] = (tap 4 s + kk, channel q, hidden channel col)
+    for (int ww = 0; ww < 4; ++ww) {
+        if (w == ww)
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int t = 4 * s + kk;
+                if (t >= NTAP) continue;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    float &r = red[(t * CI + q) * C + col];
+                    r = ww == 0 ? acc[s][q] : r + acc[s][q];
+                }
+            }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < NTAP * CI * C; e += blockDim.x) partial[(long)blockIdx.x * NTAP * CI * C + e] = red[e];
 }
 
 // out[i] = sum_c partial[c * n + i] (fixed order)
@@ -214,109 +293,189 @@ __device__ __forceinline__ short to_bf16(float f)
     return (short)(u >> 16);
 }
 
-// Layer 1 on MFMA: one wave per 16 consecutive voxels along x (one component).
-// A[i][k] = u_c(v_i + tap t), k = t CI + c (zero for k >= 27 CI), B[k][a] = W1[t][c][a];
-// D[i][a] -> m = P[v_i][a] D[i][a] stored as bf16 at mb[(comp N + v) C + a].
-// Operand layout of v_mfma_f32_16x16x16_bf16 (wave64): lane l holds A[l%16][4(l/16)+q],
-// B[4(l/16)+q][l%16], q < 4, and D[4(l/16)+q][l%16].
-__global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2, long ntiles,
-                                                          const fcomplex *__restrict__ u,
+// Tiles of the MFMA inference: a block of 256 threads (4 waves) owns 16 (x) x 4 (y)
+// x 4 (z) voxels of one component; wave w owns z-plane w of the tile and its four
+// 16-voxel rows along x. The block first stages the tile plus a one-voxel halo
+// (18 x 6 x 6) in LDS, so every input value is read from HBM/L2 once per block.
+constexpr int TX = 16, TY = 4, TZ = 4;
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+
+struct TileIdx {
+    int comp, x0, y0, z0;
+};
+
+// blockIdx -> tile, XCD-aware: consecutive dispatches go round-robin over the 8 XCDs,
+// so block b is remapped to tile (b % 8) * (nb / 8) + b / 8; neighbouring tiles (which
+// share halo voxels) then land on the same XCD and its L2.
+__device__ __forceinline__ TileIdx tile_of(int n0, int n1, int n2)
+{
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8;
+    const int t = b < 8 * per ? (b % 8) * per + b / 8 : b;
+    const int xt = (n0 + TX - 1) / TX, yt = (n1 + TY - 1) / TY, zt = (n2 + TZ - 1) / TZ;
+    TileIdx r;
+    r.x0 = (t % xt) * TX;
+    r.y0 = ((t / xt) % yt) * TY;
+    r.z0 = ((t / (xt * yt)) % zt) * TZ;
+    r.comp = t / (xt * yt * zt);
+    return r;
+}
+
+__device__ __forceinline__ int tap_of_halo(int lx, int ly, int lz) { return (lz * HY + ly) * HX + lx; }
+
+// Layer 1 on MFMA, v_mfma_f32_16x16x16_bf16. Operand layout (wave64): lane l holds
+// A[l%16][4(l/16)+q], B[4(l/16)+q][l%16], q < 4, and D[4(l/16)+q][l%16].
+// A[i][k] = u_c(v_i + tap t), k = t CI + c (zero for k >= 27 CI); B[k][a] = W1[t][c][a];
+// D[i][a] -> m = P[v_i][a] D[i][a], stored as bf16 at mb[(comp N + v) C + a].
+__global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2, const fcomplex *__restrict__ u,
                                                           const float *__restrict__ W1, const float *__restrict__ P,
                                                           unsigned short *__restrict__ mb)
 {
     static_assert(CI == 4, "a lane's 4 consecutive k must be the CI channels of one tap");
-    __shared__ short wb[K1 * C];   // [k][a], k padded to K1
-    for (int e = threadIdx.x; e < K1 * C; e += blockDim.x) wb[e] = e < NTAP * CI * C ? to_bf16(W1[e]) : (short)0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const long tile = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (tile >= ntiles) return;
+    __shared__ float4 us[HV];   // halo tile, the CI channels of one component per voxel
+    const TileIdx T = tile_of(n0, n1, n2);
     const long N = (long)n0 * n1 * n2;
-    const int xt = (n0 + 15) / 16;
-    const long lines = (long)n1 * n2;
-    const int comp = (int)(tile / (lines * xt));
-    const long rem = tile % (lines * xt);
-    const long line = rem / xt;
-    const int x0 = (int)(rem % xt) * 16;
-    const int j = (int)(line % n1), k = (int)(line / n1);
+    for (int e = threadIdx.x; e < CI * HV; e += blockDim.x) {
+        const int c = e / HV, h = e % HV;
+        const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
+        float val = 0.f;
+        if (gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2)
+            val = comp_of(u[(long)c * N + gx + (long)n0 * (gy + (long)n1 * gz)], T.comp);
+        reinterpret_cast<float *>(us)[h * CI + c] = val;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row = lane & 15, kb = (lane >> 4) * 4;
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    // this lane's B operand for the 7 K steps: W1[t][q][row], t = 4 step + kb / 4
+    bf16x4 bw[K1 / 16];
 #pragma unroll
-    for (int kk = 0; kk < K1; kk += 16) {
-        bf16x4 a, b;
-        const int t = (kk + kb) / CI;   // this lane's tap; q runs over its CI channels
-        const int i = x0 + row;
-        long idx = -1;
-        if (t < NTAP && i < n0) idx = nb(i, j, k, t, n0, n1, n2);
+    for (int s = 0; s < K1 / 16; ++s) {
+        const int t = 4 * s + kb / 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bw[s][q] = t < NTAP ? to_bf16(W1[(t * CI + q) * C + row]) : (short)0;
+    }
+    __syncthreads();
+    floatx4 acc[TY];
+#pragma unroll
+    for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < K1 / 16; ++s) {
+        const int t = 4 * s + kb / 4;
+        const int tt = t < NTAP ? t : 13;   // padded k: any in-tile voxel, B is zero there
+        const int dx = tap_dx(tt), dy = tap_dy(tt), dz = tap_dz(tt);
+#pragma unroll
+        for (int ry = 0; ry < TY; ++ry) {
+            const float4 v = us[tap_of_halo(row + 1 + dx, ry + 1 + dy, w + 1 + dz)];
+            bf16x4 a;
+            a[0] = to_bf16(v.x);
+            a[1] = to_bf16(v.y);
+            a[2] = to_bf16(v.z);
+            a[3] = to_bf16(v.w);
+            acc[ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[s], acc[ry], 0, 0, 0);
+        }
+    }
+    // lane holds D[kb + q][row]: voxel x0 + kb + q, hidden channel row
+    const int gz = T.z0 + w;
+#pragma unroll
+    for (int ry = 0; ry < TY; ++ry) {
+        const int gy = T.y0 + ry;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            a[q] = to_bf16(idx >= 0 ? comp_of(u[(long)q * N + idx], comp) : 0.f);
-            b[q] = wb[(kk + kb + q) * C + row];
+            const int gx = T.x0 + kb + q;
+            if (gx >= n0 || gy >= n1 || gz >= n2) continue;
+            const long v = gx + (long)n0 * (gy + (long)n1 * gz);
+            mb[((long)T.comp * N + v) * C + row] = (unsigned short)to_bf16(P[v * C + row] * acc[ry][q]);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    // lane holds D[kb + q][row]: voxel x0 + kb + q, channel row
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int i = x0 + kb + q;
-        if (i >= n0) continue;
-        const long v = i + (long)n0 * (j + (long)n1 * k);
-        mb[((long)comp * N + v) * C + row] = (unsigned short)to_bf16(P[v * C + row] * acc[q]);
     }
 }
 
 // Layer 2 on MFMA: per tap t, A[i][a] = m[v_i + t][a], B[a][b] = W2[t][a][b]; the 27
-// products accumulate into D[i][b]; out(v_i) = sum_b P[v_i][b] D[i][b] (reduced over
-// the 16 lanes that hold row i's columns).
-__global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2, long ntiles,
+// products accumulate into D[i][b]; out(v_i) = sum_b P[v_i][b] D[i][b] (reduced over the
+// 16 lanes that hold row i's columns). The halo tile of m lives in LDS with a 40-byte
+// voxel stride (16 bf16 channels + 4 padding), which spreads the 16 lanes of an 8-byte
+// A-operand read over distinct banks.
+constexpr int MS = 20;   // LDS stride of one voxel of m, in bf16 elements
+__global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2,
                                                           const unsigned short *__restrict__ mb,
                                                           const float *__restrict__ W2, const float *__restrict__ P,
                                                           float *__restrict__ out)
 {
-    __shared__ short wb[NTAP * C * C];   // [t][a][b]
-    for (int e = threadIdx.x; e < NTAP * C * C; e += blockDim.x) wb[e] = to_bf16(W2[e]);
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const long tile = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (tile >= ntiles) return;
+    __shared__ uint2 ms[HV * MS / 4];
+    const TileIdx T = tile_of(n0, n1, n2);
     const long N = (long)n0 * n1 * n2;
-    const int xt = (n0 + 15) / 16;
-    const long lines = (long)n1 * n2;
-    const int comp = (int)(tile / (lines * xt));
-    const long rem = tile % (lines * xt);
-    const long line = rem / xt;
-    const int x0 = (int)(rem % xt) * 16;
-    const int j = (int)(line % n1), k = (int)(line / n1);
-    const int row = lane & 15, kb = (lane >> 4) * 4;
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int t = 0; t < NTAP; ++t) {
-        bf16x4 a, b;
-        const int i = x0 + row;
-        long idx = -1;
-        if (i < n0) idx = nb(i, j, k, t, n0, n1, n2);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            a[q] = idx >= 0 ? (short)mb[((long)comp * N + idx) * C + kb + q] : (short)0;
-            b[q] = wb[(t * C + kb + q) * C + row];
+    // stage: 4 x 8 bytes per voxel (channels 4p .. 4p+3)
+    for (int e = threadIdx.x; e < 4 * HV; e += blockDim.x) {
+        const int h = e >> 2, part = e & 3;
+        const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
+        uint2 val = {0u, 0u};
+        if (gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2) {
+            const long v = gx + (long)n0 * (gy + (long)n1 * gz);
+            val = reinterpret_cast<const uint2 *>(mb + ((long)T.comp * N + v) * C)[part];
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        ms[(h * MS) / 4 + part] = val;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row = lane & 15, kb = (lane >> 4) * 4;
+    // this lane's B operand of every tap: W2[t][kb + q][row]
+    bf16x4 bw[NTAP];
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bw[t][q] = to_bf16(W2[(t * C + kb + q) * C + row]);
+    __syncthreads();
+    floatx4 acc[TY];
+#pragma unroll
+    for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) {
+        const int dx = tap_dx(t), dy = tap_dy(t), dz = tap_dz(t);
+#pragma unroll
+        for (int ry = 0; ry < TY; ++ry) {
+            const int h = tap_of_halo(row + 1 + dx, ry + 1 + dy, w + 1 + dz);
+            const uint2 raw = ms[(h * MS) / 4 + kb / 4];
+            bf16x4 a;
+            a[0] = (short)(raw.x & 0xFFFFu);
+            a[1] = (short)(raw.x >> 16);
+            a[2] = (short)(raw.y & 0xFFFFu);
+            a[3] = (short)(raw.y >> 16);
+            acc[ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[t], acc[ry], 0, 0, 0);
+        }
     }
     // lane holds D[kb + q][row] (voxel x0 + kb + q, output channel row)
+    const int gz = T.z0 + w;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int i = x0 + kb + q;
-        float s = 0.f;
-        if (i < n0) {
-            const long v = i + (long)n0 * (j + (long)n1 * k);
-            s = P[v * C + row] * acc[q];
-        }
-        // sum over the 16 lanes of this group (channels)
-        for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-        if (row == 0 && i < n0) {
-            const long v = i + (long)n0 * (j + (long)n1 * k);
-            out[(long)comp * N + v] = s;
+    for (int ry = 0; ry < TY; ++ry) {
+        const int gy = T.y0 + ry;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int gx = T.x0 + kb + q;
+            const bool in = gx < n0 && gy < n1 && gz < n2;
+            const long v = in ? gx + (long)n0 * (gy + (long)n1 * gz) : 0;
+            float s = in ? P[v * C + row] * acc[ry][q] : 0.f;
+            for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+            if (row == 0 && in) out[(long)T.comp * N + v] = s;
         }
     }
+}
+
+// partial[b] = sum over this block's grid-stride share of (out - target)^2 (FP64)
+__global__ void __launch_bounds__(256) cnn_loss_kernel(long N, const float *__restrict__ out,
+                                                       const fcomplex *__restrict__ target,
+                                                       double *__restrict__ partial)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < 2 * N; I += (long)gridDim.x * blockDim.x) {
+        const int comp = (int)(I / N);
+        const double d = (double)out[I] - (double)comp_of(target[I % N], comp);
+        s += d * d;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
 // n[v] = g * (out_re[v] + i out_im[v]) as complex64
@@ -393,13 +552,11 @@ void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const f
 {
     const long N = (long)n[0] * n[1] * n[2];
     hipLaunchKernelGGL(cnn_dout_kernel, dim3(gridn(2 * N, 256)), dim3(256), 0, st, N, out, target, P, inv_t2, dov);
-    hipLaunchKernelGGL(cnn_dw2_kernel, dim3(NTAP, nchunk), dim3(256), 0, st, n[0], n[1], n[2], nchunk, m, dov,
-                       partial);
+    hipLaunchKernelGGL(cnn_dw2_mfma_kernel, dim3(nchunk), dim3(256), 0, st, n[0], n[1], n[2], m, dov, partial);
     hipLaunchKernelGGL(sum_partials_kernel, dim3(gridn(NTAP * C * C, 256)), dim3(256), 0, st, partial, nchunk,
                        NTAP * C * C, dW2);
     hipLaunchKernelGGL(cnn_dm_kernel, dim3(gridn(2 * N, 256)), dim3(256), 0, st, n[0], n[1], n[2], dov, W2, P, dh);
-    hipLaunchKernelGGL(cnn_dw1_kernel, dim3(nchunk, CI), dim3(NTAP * C), 0, st, n[0], n[1], n[2], nchunk, u, dh,
-                       partial);
+    hipLaunchKernelGGL(cnn_dw1_mfma_kernel, dim3(nchunk), dim3(256), 0, st, n[0], n[1], n[2], u, dh, partial);
     hipLaunchKernelGGL(sum_partials_kernel, dim3(gridn(NTAP * CI * C, 256)), dim3(256), 0, st, partial, nchunk,
                        NTAP * CI * C, dW1);
 }
@@ -408,12 +565,17 @@ void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const flo
                     const float *P, unsigned short *mb, float *out, fcomplex g, fcomplex *nout)
 {
     const long N = (long)n[0] * n[1] * n[2];
-    const long ntiles = 2L * ((n[0] + 15) / 16) * n[1] * n[2];
-    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3(gridn(ntiles, 4)), dim3(256), 0, st, n[0], n[1], n[2], ntiles, u,
-                       W1, P, mb);
-    hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3(gridn(ntiles, 4)), dim3(256), 0, st, n[0], n[1], n[2], ntiles, mb,
-                       W2, P, out);
+    const long ntiles = 2L * ((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ);
+    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n[0], n[1], n[2], u, W1, P,
+                       mb);
+    hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n[0], n[1], n[2], mb, W2, P,
+                       out);
     if (nout) hipLaunchKernelGGL(cnn_to_complex_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, N, out, g, nout);
+}
+
+void cnn_loss(hipStream_t st, long N, const float *out, const fcomplex *target, double *partial)
+{
+    hipLaunchKernelGGL(cnn_loss_kernel, dim3(kLossBlocks), dim3(256), 0, st, N, out, target, partial);
 }
 
 void cnn_to_complex(hipStream_t st, long N, const float *out, fcomplex g, fcomplex *n)
