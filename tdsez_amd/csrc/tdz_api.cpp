@@ -125,7 +125,7 @@ struct LearnState {
     std::vector<double> fscale;          // [C] maximum of each field before scaling to 1
     float *W1 = nullptr, *W2 = nullptr;  // device weights [27][CI][C], [27][C][C]
     float *gr[3] = {};                   // Greville coordinates per axis (float)
-    fcomplex *Xc[3] = {}, *Zc[3] = {};   // position band and zero band per axis (complex64)
+    float *Xr[3] = {};                   // position band per axis (float)
     fcomplex *feat = nullptr;            // [CI][N] network input: y and its first-moment features
     tdz::CnnScales sin{{1.f, 1.f, 1.f, 1.f}};   // input scales (unit mean square over the samples)
     double sout = 1.0;                   // target scale: the network outputs sout * t
@@ -159,6 +159,10 @@ struct tdz_ctx {
     Scratch<float> sf;
     bool mixed_ready = false;   // single-precision copies allocated
     dcomplex *rhs = nullptr;
+    // chained CN right-hand side: A psi of the last solve of an evolve run
+    dcomplex *ax = nullptr, *zerov = nullptr, *dDip[3] = {};
+    bool ax_valid = false;
+    double ax_dt = 0, ax_sE[3] = {0, 0, 0};
     int restart = 30;
     dcomplex *Vb = nullptr, *Zb = nullptr;
     dcomplex *dpart = nullptr, *dh = nullptr, *dh2 = nullptr;
@@ -613,26 +617,21 @@ double correction_bound(tdz_ctx *c)
     return std::abs(c->g) * (w + f + v);
 }
 
-// Network input on the fine level: feat_0 = y, feat_{1+d} = X_d y - x_d .* M y (first
-// moments of y about each Greville point along axis d; X_d the position band on axis d,
-// mass bands on the others). My = M y is given.
-void learned_features(tdz_ctx *c, const fcomplex *y, const fcomplex *My, fcomplex *feat)
+// Bands of the fused products on the fine level (complex64 path).
+tdz::FusedBands fused_bands(tdz_ctx *c)
 {
     Level &L = *c->lv[0];
     LearnState &ls = c->learn;
-    const long N = L.N;
-    HIPCHK(hipMemcpyAsync(feat, y, N * sizeof(fcomplex), hipMemcpyDeviceToDevice, c->st));
+    tdz::FusedBands fb;
     for (int d = 0; d < 3; ++d) {
-        fcomplex *G = feat + (size_t)(1 + d) * N;
-        if (d >= c->prob.dim) {
-            HIPCHK(hipMemsetAsync(G, 0, N * sizeof(fcomplex), c->st));
-            continue;
-        }
-        tdz::SepBandsT<float> xb = sep_bands<float>(L);
-        for (int e = 0; e < 3; ++e) xb.B[e] = e == d ? ls.Xc[e] : ls.Zc[e];
-        tdz::launch_sep_apply<float>(c->st, xb, y, G, c->sf.tmp);
-        tdz::moment_feature(c->st, L.n, d, ls.gr[d], My, G);
+        fb.n[d] = L.n[d];
+        fb.p[d] = L.ax[d].p;
+        fb.M[d] = L.f.M[d];
+        fb.X[d] = ls.Xr[d];
+        fb.C[d] = c->dBcorr[d];
+        fb.g[d] = ls.gr[d];
     }
+    return fb;
 }
 
 void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, const fcomplex *n, bool exact_v = false,
@@ -647,21 +646,26 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, const fcomplex 
         return;
     }
     fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);                // y
-    tdz::SepBandsT<float> mb = sep_bands<float>(L);
-    for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
-    tdz::launch_sep_apply<float>(c->st, mb, L.f.z, L.f.t, c->sf.tmp);       // M y
-    tdz::SepBandsT<float> cb = sep_bands<float>(L);
-    for (int d = 0; d < 3; ++d) cb.B[d] = c->dBcorr[d];
-    tdz::launch_sep_apply<float>(c->st, cb, L.f.z, c->sf.u1, c->sf.tmp);    // g (CAP + sE.D) y
+    LearnState &ls = c->learn;
+    const bool harvest = ls.harvesting && (int)ls.ys.size() < ls.max_samples && c->ph.pot_kind;
+    const bool net = use_net && ls.trained;
+    if (harvest || net) {   // M y, g (CAP + sE.D) y and the network input in three passes
+        fcomplex *tmp[7] = {c->sf.tmp[0], c->sf.tmp[1], c->sf.tmp[2], c->sf.tmp[3], c->sf.w1, c->sf.w2, c->sf.u2};
+        tdz::fused_products(c->st, fused_bands(c), ls.sin, L.f.z, tmp, L.f.t, c->sf.u1, ls.feat);
+    } else {
+        tdz::SepBandsT<float> mb = sep_bands<float>(L);
+        for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
+        tdz::launch_sep_apply<float>(c->st, mb, L.f.z, L.f.t, c->sf.tmp);       // M y
+        tdz::SepBandsT<float> cb = sep_bands<float>(L);
+        for (int d = 0; d < 3; ++d) cb.B[d] = c->dBcorr[d];
+        tdz::launch_sep_apply<float>(c->st, cb, L.f.z, c->sf.u1, c->sf.tmp);    // g (CAP + sE.D) y
+    }
     if (exact_v && c->ph.pot_kind) {   // potential product by the matrix-free kernel (reference for the learned one)
         HIPCHK(hipMemsetAsync(c->sf.u2, 0, N * sizeof(fcomplex), c->st));
         pot_apply<float>(c, L, c->g, L.f.z, c->sf.u2);
         HIPCHK(hipMemsetAsync(L.f.t, 0, N * sizeof(fcomplex), c->st));
         n = c->sf.u2;
     }
-    LearnState &ls = c->learn;
-    const bool harvest = ls.harvesting && (int)ls.ys.size() < ls.max_samples && c->ph.pot_kind;
-    if (harvest || (use_net && ls.trained)) learned_features(c, L.f.z, L.f.t, ls.feat);
     if (harvest) {
         // sample: input features of y = S^-1 r, target t = V y - Vg .* M y (matrix-free potential product)
         constexpr int CI = tdz::kCnnInputs;
@@ -679,9 +683,8 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, const fcomplex 
         ls.ts.push_back(ts);
         ls.tnorm2.push_back(t2);
     }
-    if (use_net && ls.trained) {
+    if (net) {
         const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-        tdz::scale_inputs(c->st, N, ls.sin, ls.feat);
         tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.P, ls.mb, ls.out, cx<float>(c->g / ls.sout),
                             ls.nvec);
         n = ls.nvec;
@@ -759,8 +762,12 @@ double dev_norm2(tdz_ctx *c, const dcomplex *x, long N)
 // With r0_in_v0 the caller has already stored the initial residual b - A x in V_0.
 // Returns the iteration count; without convergence in maxit iterations it throws, or
 // with allow_fail returns -1 (x then holds the last iterate).
+// With ax_out, a converged solve also returns A x = b - r there, r the final residual
+// taken from the Arnoldi relation of the last cycle: r = V_{j+1} (beta e_1 - Hbar y)
+// = gamma_{j+1} V_{j+1} Q^H e_{j+1} (Q the Givens rotations), at the cost of one
+// (j+1)-term vector update instead of an operator apply.
 int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, tdz_ksp_stats *st,
-           bool r0_in_v0 = false, bool allow_fail = false)
+           bool r0_in_v0 = false, bool allow_fail = false, dcomplex *ax_out = nullptr)
 {
     Level &L = *c->lv[0];
     const long N = L.N;
@@ -773,6 +780,7 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
     double res = 0;
     if (bnorm == 0.0) {
         HIPCHK(hipMemsetAsync(x, 0, N * sizeof(dcomplex), c->st));
+        if (ax_out) HIPCHK(hipMemsetAsync(ax_out, 0, N * sizeof(dcomplex), c->st));
         if (st) *st = {0, 0, 0.0, 0.0, c->rebuilds};
         return 0;
     }
@@ -785,7 +793,14 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
         r0_in_v0 = false;
         const double beta = std::sqrt(dev_norm2(c, Vj(0), N));
         res = beta / bnorm;
-        if (res <= rtol) { converged = true; break; }
+        if (res <= rtol) {
+            converged = true;
+            if (ax_out) {   // V_0 holds the residual b - A x itself
+                HIPCHK(hipMemcpyAsync(ax_out, b, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+                tdz::launch_axpby(c->st, {-1, 0}, Vj(0), {1, 0}, ax_out, N);
+            }
+            break;
+        }
         if (its >= maxit) break;
         ++restarts;
         tdz::launch_axpby(c->st, {1.0 / beta, 0}, Vj(0), {0, 0}, Vj(0), N);
@@ -838,6 +853,23 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
         for (int i = 0; i < j; ++i) yd[i] = dc(yv[i]);
         HIPCHK(hipMemcpyAsync(c->dh, yd.data(), j * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
         tdz::launch_multi_axpy(c->st, c->pc_kind == TDZ_PC_NONE ? c->Vb : c->Zb, N, j, c->dh, x, N);
+        if (converged && ax_out) {
+            // q = Q^H e_j: the inverse rotations, newest first
+            std::vector<cplx> q(j + 1, cplx(0));
+            q[j] = 1.0;
+            for (int i = j - 1; i >= 0; --i) {
+                const cplx a = q[i], bb = q[i + 1];
+                q[i] = cs[i] * a - sn[i] * bb;
+                q[i + 1] = std::conj(sn[i]) * a + cs[i] * bb;
+            }
+            std::vector<dcomplex> cr(j + 1);
+            for (int i = 0; i <= j; ++i) cr[i] = dc(-gv[j] * q[i]);
+            // the host vector must outlive the asynchronous copy: synchronise before leaving
+            HIPCHK(hipMemcpyAsync(c->dh2, cr.data(), (j + 1) * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipMemcpyAsync(ax_out, b, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+            tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N);
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
         if (converged || its >= maxit) break;
     }
     HIPCHK(hipStreamSynchronize(c->st));
@@ -1199,23 +1231,91 @@ namespace {
 // multigrid takes over for the rest of the run and finishes it from the last iterate,
 // so the answer is always judged by the same FP64 residual.
 void step_solve(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, tdz_ksp_stats *st,
-                bool r0_in_v0)
+                bool r0_in_v0, dcomplex *ax_out = nullptr)
 {
     LearnState &ls = c->learn;
     if (c->pc_kind != TDZ_PC_LEARNED || ls.fallen || ls.base_its < 0) {
-        fgmres(c, b, x, rtol, maxit, st, r0_in_v0);
+        fgmres(c, b, x, rtol, maxit, st, r0_in_v0, false, ax_out);
         return;
     }
     tdz_ksp_stats s1{};
-    if (fgmres(c, b, x, rtol, std::min(maxit, ls.base_its), &s1, r0_in_v0, true) >= 0) {
+    if (fgmres(c, b, x, rtol, std::min(maxit, ls.base_its), &s1, r0_in_v0, true, ax_out) >= 0) {
         *st = s1;
         return;
     }
     ls.fallen = true;
     ++ls.fallbacks;
-    fgmres(c, b, x, rtol, std::max(maxit - s1.its, 1), st, false);
+    fgmres(c, b, x, rtol, std::max(maxit - s1.its, 1), st, false, false, ax_out);
     st->its += s1.its;
     st->restarts += s1.restarts;
+}
+
+// One CN step psi <- (M + g H_E)^-1 (M - g H_E) psi, g = i dt/2 (src/propagator.cpp:470-540).
+// Unchained, the right-hand side is an operator apply. Chained (the previous step of
+// the same run solved (M + g H_E') psi = b' with this dt and left A psi = b' - r' in
+// c->ax), the identity
+//   (M - g H_E) psi = 2 M psi - (M + g H_E') psi - g sum_a (sE_a - sE'_a) D_a psi
+// gives it from one mass apply and one dipole apply; both the identity and A psi are
+// exact up to rounding, so the step is the same.
+void cn_step(tdz_ctx *c, dcomplex *x, double dt, const double sE_mid[3], double rtol, int maxit, tdz_ksp_stats *st,
+             bool chained)
+{
+    if (!c->pc_ready) pc_setup(c, c->pc_kind);
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    const cplx g(0.0, 0.5 * dt);
+    if (!c->ax) c->ax = track(c, dalloc<dcomplex>(N));
+    chained = chained && c->ax_valid && c->ax_dt == dt;
+    tdz::SepBandsT<double> mb = sep_bands<double>(L);
+    for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
+    const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+    if (chained) {
+        tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);   // M psi
+        if (c->prob.dirichlet) tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
+        // rhs = 2 M psi - A' psi
+        HIPCHK(hipMemcpyAsync(c->rhs, c->sd.w1, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+        tdz::launch_axpby(c->st, {-1, 0}, c->ax, {2, 0}, c->rhs, N);
+        bool field = false;
+        for (int a = 0; a < c->prob.dim; ++a) field = field || sE_mid[a] != c->ax_sE[a];
+        if (field) {   // rhs -= g sum_a dsE_a D_a psi: one Kronecker apply with scaled position bands
+            tdz::SepBandsT<double> db = sep_bands<double>(L);
+            for (int d = 0; d < 3; ++d) {
+                const size_t sz = L.M[d].size();
+                if (!c->dDip[d]) c->dDip[d] = track(c, dalloc<dcomplex>(sz));
+                std::vector<dcomplex> B(sz, dcomplex{0, 0});
+                if (d < c->prob.dim)
+                    for (size_t k = 0; k < sz; ++k) B[k] = dc(-g * (sE_mid[d] - c->ax_sE[d]) * L.Xb[d][k]);
+                HIPCHK(hipMemcpyAsync(c->dDip[d], B.data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+                HIPCHK(hipStreamSynchronize(c->st));
+                db.B[d] = c->dDip[d];
+            }
+            tdz::launch_sep_apply<double>(c->st, db, x, c->sd.w2, c->sd.tmp);
+            if (c->prob.dirichlet) {   // the walls carry the identity rows of M - g H: no field term
+                if (!c->zerov) {
+                    c->zerov = track(c, dalloc<dcomplex>(N));
+                    HIPCHK(hipMemsetAsync(c->zerov, 0, N * sizeof(dcomplex), c->st));
+                }
+                tdz::launch_wall_fix<double>(c->st, L.n, wall, c->zerov, c->sd.w2);
+            }
+            tdz::launch_axpby(c->st, {1, 0}, c->sd.w2, {1, 0}, c->rhs, N);
+        }
+    } else {
+        set_shift(c, 1.0, -g, sE_mid);        // rhs = (M - i dt/2 H_E) psi
+        level_apply(c, L, x, c->rhs);
+        tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
+        if (c->prob.dirichlet) tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
+    }
+    // (M + i dt/2 H_E) psi = 2 M psi - rhs, so the initial residual of the guess psi
+    // is r0 = 2 (rhs - M psi): one mass apply instead of a full operator apply
+    // (the same identity as the stage right-hand sides of src/propagator.cpp:589-624)
+    HIPCHK(hipMemcpyAsync(c->Vb, c->rhs, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+    tdz::launch_axpby(c->st, {-2.0, 0}, c->sd.w1, {2.0, 0}, c->Vb, N);
+    set_shift(c, 1.0, g, sE_mid);         // solve (M + i dt/2 H_E) psi' = rhs, guess psi
+    c->ax_valid = false;
+    step_solve(c, c->rhs, x, rtol, maxit, st, true, c->ax);
+    c->ax_valid = true;
+    c->ax_dt = dt;
+    for (int a = 0; a < 3; ++a) c->ax_sE[a] = sE_mid[a];
 }
 }  // namespace
 
@@ -1225,27 +1325,8 @@ int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], doub
                 tdz_ksp_stats *stats)
 {
     return guarded([&] {
-        if (!c->pc_ready) pc_setup(c, c->pc_kind);
-        dcomplex *x = reinterpret_cast<dcomplex *>(psi);
-        Level &L = *c->lv[0];
-        const cplx g(0.0, 0.5 * dt);
-        set_shift(c, 1.0, -g, sE_mid);        // rhs = (M - i dt/2 H_E) psi
-        level_apply(c, L, x, c->rhs);
-        // (M + i dt/2 H_E) psi = 2 M psi - rhs, so the initial residual of the guess psi
-        // is r0 = 2 (rhs - M psi): one mass apply instead of a full operator apply
-        // (the same identity as the stage right-hand sides of src/propagator.cpp:589-624)
-        tdz::SepBandsT<double> mb = sep_bands<double>(L);
-        for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
-        tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
-        if (c->prob.dirichlet) {
-            const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-            tdz::launch_wall_fix<double>(c->st, L.n, w, x, c->sd.w1);
-        }
-        HIPCHK(hipMemcpyAsync(c->Vb, c->rhs, L.N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
-        tdz::launch_axpby(c->st, {-2.0, 0}, c->sd.w1, {2.0, 0}, c->Vb, L.N);
-        set_shift(c, 1.0, g, sE_mid);         // solve (M + i dt/2 H_E) psi' = rhs, guess psi
         tdz_ksp_stats st{};
-        step_solve(c, c->rhs, x, rtol, maxit, &st, true);
+        cn_step(c, reinterpret_cast<dcomplex *>(psi), dt, sE_mid, rtol, maxit, &st, false);
         if (stats) *stats = st;
     });
 }
@@ -1472,10 +1553,7 @@ void setup_learned(tdz_ctx *c)
     ls.feat = track(c, dalloc<fcomplex>((size_t)CI * N));
     for (int d = 0; d < 3; ++d) {
         const size_t sz = L.M[d].size();
-        std::vector<fcomplex> xb(sz, fcomplex{0.f, 0.f}), zb(sz, fcomplex{0.f, 0.f});
-        for (size_t k = 0; k < sz; ++k) xb[k].re = (float)L.Xb[d][k];
-        ls.Xc[d] = upload(c, xb);
-        ls.Zc[d] = upload(c, zb);
+        ls.Xr[d] = upload(c, to_float(L.Xb[d]));
         std::vector<float> g(L.n[d], 0.f);
         if (d < c->prob.dim) {
             const std::vector<double> gd = tdz::greville(L.ax[d].t, L.ax[d].p);
@@ -1754,7 +1832,7 @@ int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, con
             double sE[3] = {0, 0, 0};
             if (pulse) tdz_pulse_coupling(c, pulse, tm, sE);
             tdz_ksp_stats st{};
-            if (tdz_step_cn(c, psi, dt, sE, rtol, maxit, &st)) throw Error(g_err);
+            cn_step(c, reinterpret_cast<dcomplex *>(psi), dt, sE, rtol, maxit, &st, s > 0);
             // refresh rule (include/tdsez_propagator.hpp:17-31): the preconditioner is rebuilt
             // from the current operator when a solve needs > 2 base + 5 iterations
             if (c->refresh_base < 0) c->refresh_base = st.its;
@@ -1860,12 +1938,10 @@ int tdz_learn_features(tdz_ctx *c, const double *y, double *feat)
         Level &L = *c->lv[0];
         const long N = L.N;
         constexpr int CI = tdz::kCnnInputs;
-        fcomplex *My = c->sf.u1;
         tdz::launch_demote(c->st, reinterpret_cast<const dcomplex *>(y), L.f.z, N);
-        tdz::SepBandsT<float> mb = sep_bands<float>(L);
-        for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
-        tdz::launch_sep_apply<float>(c->st, mb, L.f.z, My, c->sf.tmp);
-        learned_features(c, L.f.z, My, c->learn.feat);
+        fcomplex *tmp[7] = {c->sf.tmp[0], c->sf.tmp[1], c->sf.tmp[2], c->sf.tmp[3], c->sf.w1, c->sf.w2, c->sf.u2};
+        const tdz::CnnScales unit{{1.f, 1.f, 1.f, 1.f}};
+        tdz::fused_products(c->st, fused_bands(c), unit, L.f.z, tmp, L.f.t, c->sf.u1, c->learn.feat);
         std::vector<fcomplex> h((size_t)CI * N);
         HIPCHK(hipMemcpyAsync(h.data(), c->learn.feat, h.size() * sizeof(fcomplex), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));

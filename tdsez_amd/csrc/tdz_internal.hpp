@@ -171,7 +171,16 @@ void lumped_residual(hipStream_t st, long N, const fcomplex *pv, const fcomplex 
 // u[c] *= v[c], c < kCnnInputs (in place)
 struct CnnScales { float v[kCnnInputs]; };
 void scale_inputs(hipStream_t st, long N, const CnnScales &s, fcomplex *u);
-// G -= x_d .* My (x_d: Greville coordinate along axis d)
-void moment_feature(hipStream_t st, const int n[3], int d, const float *gr, const fcomplex *My, fcomplex *G);
+// Fused products of the network-corrected solve (complex64): My, corr (the C bands'
+// Kronecker sum applied to y) and, if u is not null, the scaled network input
+// u = [s0 y, s1 G_x, s2 G_y, s3 G_z], G_d = X_d y - x_d My. tmp: 7 work vectors.
+struct FusedBands {
+    int n[3], p[3];
+    const float *M[3], *X[3];    // real mass and position bands
+    const fcomplex *C[3];        // complex bands of the C term
+    const float *g[3];           // Greville coordinates
+};
+void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, const fcomplex *y, fcomplex *tmp[7],
+                    fcomplex *My, fcomplex *corr, fcomplex *u);
 
 }  // namespace tdz

@@ -12,6 +12,8 @@
 //   csr_axis          : Boehm prolongation / restriction along one axis.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "tdz_internal.hpp"
 
 namespace tdz {
@@ -744,6 +746,79 @@ __global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, 
     }
 }
 
+// FP32 variant on MFMA (v_mfma_f32_16x16x4_f32; lane l holds A[l%16][l/16],
+// B[l/16][l%16], D[4(l/16)+q][l%16]): the same 64 x 64 output tile per block, staged
+// through LDS as above; wave (wi, wl) of the 2 x 2 waves owns a 32 x 32 quadrant as
+// 2 x 2 MFMA tiles, real and imaginary parts as two real GEMMs with the same A = T.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) axis_gemm_mfma_kernel(int n, long L, long sB, long sK, long sL, int kfast,
+                                                             const float *__restrict__ T, int trans,
+                                                             const fcomplex *__restrict__ in,
+                                                             fcomplex *__restrict__ out)
+{
+    __shared__ float Ts[GK][GT];      // [k][i]
+    __shared__ float Ir[GK][GT], Ii[GK][GT];   // [k][l]
+    const int tid = threadIdx.x;
+    const long l0 = (long)blockIdx.x * GT;
+    const int i0 = blockIdx.y * GT;
+    const long boff = (long)blockIdx.z * sB;
+    const int lane = tid & 63, wv = tid >> 6, wi = wv >> 1, wl = wv & 1;
+    const int r16 = lane & 15, k4 = lane >> 4;
+    f32x4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) accr[a][b] = acci[a][b] = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < n; k0 += GK) {
+        for (int e = tid; e < GK * GT; e += 256) {
+            const int kk = e % GK, ii = e / GK;
+            const int gi = i0 + ii, gk = k0 + kk;
+            float v = 0.f;
+            if (gi < n && gk < n) v = trans ? T[(long)gk * n + gi] : T[(long)gi * n + gk];
+            Ts[kk][ii] = v;
+            int kq, lq;
+            if (kfast) { kq = e % GK; lq = e / GK; } else { lq = e % GT; kq = e / GT; }
+            const int gk2 = k0 + kq;
+            const long gl = l0 + lq;
+            fcomplex u = {0.f, 0.f};
+            if (gk2 < n && gl < L) u = in[boff + gk2 * sK + gl * sL];
+            Ir[kq][lq] = u.re;
+            Ii[kq][lq] = u.im;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < GK / 4; ++s) {
+            const int kk = 4 * s + k4;
+            float av[2], br[2], bi[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) av[a] = Ts[kk][32 * wi + 16 * a + r16];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                br[b] = Ir[kk][32 * wl + 16 * b + r16];
+                bi[b] = Ii[kk][32 * wl + 16 * b + r16];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], br[b], accr[a][b], 0, 0, 0);
+                    acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bi[b], acci[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gi = i0 + 32 * wi + 16 * a + 4 * k4 + q;
+                const long gl = l0 + 32 * wl + 16 * b + r16;
+                if (gi < n && gl < L) out[boff + gi * sK + gl * sL] = {accr[a][b][q], acci[a][b][q]};
+            }
+}
+
 template <typename R>
 __global__ void diag_scale_kernel(int n0, int n1, long N, const R *l0, const R *l1, const R *l2,
                                   const unsigned char *w0, const unsigned char *w1, const unsigned char *w2,
@@ -953,8 +1028,12 @@ void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool t
     else if (d == 1) { L = n[0]; sB = n01; sK = n[0]; sL = 1; B = (unsigned)n[2]; kfast = 0; }
     else { L = n01; sB = 0; sK = n01; sL = 1; B = 1; kfast = 0; }
     const dim3 grid((unsigned)((L + GT - 1) / GT), (unsigned)((n[d] + GT - 1) / GT), B);
-    hipLaunchKernelGGL(axis_gemm_kernel<R>, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T, trans ? 1 : 0, x,
-                       y);
+    if constexpr (std::is_same<R, float>::value)
+        hipLaunchKernelGGL(axis_gemm_mfma_kernel, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T,
+                           trans ? 1 : 0, x, y);
+    else
+        hipLaunchKernelGGL(axis_gemm_kernel<R>, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T,
+                           trans ? 1 : 0, x, y);
 }
 
 template <typename R>

@@ -487,6 +487,115 @@ __global__ void cnn_to_complex_kernel(long N, const float *__restrict__ out, fco
     n[v] = {g.re * re - g.im * im, g.re * im + g.im * re};
 }
 
+// ---------------------------------------------------------------- fused products
+// The corrected solve with the network needs, for y = S^-1 r (complex64),
+//   My, corr = (Cx (x) My (x) Mz + Mx (x) Cy (x) Mz + Mx (x) My (x) Cz) y,
+//   G_d = X_d y - x_d My (X_x = Xx (x) My (x) Mz etc., x_d the Greville coordinate),
+// and the scaled network input u = [s0 y, s1 G_x, s2 G_y, s3 G_z]. Three passes share
+// the partial products (1 -> 3 -> 4 -> 6 vectors) instead of four Kronecker applies:
+//   x: P = Mx y, Q = Xx y, R = Cx y
+//   y: PP = My P, PX = Xy P, QQ = My Q, S = My R + Cy P
+//   z: My = Mz PP, X_x y = Mz QQ, X_y y = Mz PX, X_z y = Xz PP, corr = Mz S + Cz PP
+// Bands are stored row-major [i][2p+1] (real M, X; complex C).
+__global__ void feat_x_kernel(int n0, long lines, int p, const float *__restrict__ M, const float *__restrict__ X,
+                              const fcomplex *__restrict__ Cb, const fcomplex *__restrict__ y,
+                              fcomplex *__restrict__ P, fcomplex *__restrict__ Q, fcomplex *__restrict__ R)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= lines * n0) return;
+    const int i = (int)(I % n0);
+    const long base = I - i;
+    const int bw = 2 * p + 1;
+    const int lo = i - p < 0 ? 0 : i - p, hi = i + p > n0 - 1 ? n0 - 1 : i + p;
+    float pr = 0, pi = 0, qr = 0, qi = 0, rr = 0, ri = 0;
+    for (int J = lo; J <= hi; ++J) {
+        const fcomplex v = y[base + J];
+        const int e = i * bw + (J - i + p);
+        const float m = M[e], x = X[e];
+        const fcomplex cb = Cb[e];
+        pr += m * v.re; pi += m * v.im;
+        qr += x * v.re; qi += x * v.im;
+        rr += cb.re * v.re - cb.im * v.im;
+        ri += cb.re * v.im + cb.im * v.re;
+    }
+    P[I] = {pr, pi};
+    Q[I] = {qr, qi};
+    R[I] = {rr, ri};
+}
+
+__global__ void feat_y_kernel(int n0, int n1, long N, int p, const float *__restrict__ M, const float *__restrict__ X,
+                              const fcomplex *__restrict__ Cb, const fcomplex *__restrict__ P,
+                              const fcomplex *__restrict__ Q, const fcomplex *__restrict__ R,
+                              fcomplex *__restrict__ PP, fcomplex *__restrict__ PX, fcomplex *__restrict__ QQ,
+                              fcomplex *__restrict__ S)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const int i = (int)(I % n0);
+    const int j = (int)((I / n0) % n1);
+    const long k = I / ((long)n0 * n1);
+    const int bw = 2 * p + 1;
+    const int lo = j - p < 0 ? 0 : j - p, hi = j + p > n1 - 1 ? n1 - 1 : j + p;
+    float a0 = 0, a1 = 0, b0 = 0, b1 = 0, c0 = 0, c1 = 0, d0 = 0, d1 = 0;
+    for (int J = lo; J <= hi; ++J) {
+        const long src = i + (long)n0 * (J + (long)n1 * k);
+        const fcomplex pv = P[src], qv = Q[src], rv = R[src];
+        const int e = j * bw + (J - j + p);
+        const float m = M[e], x = X[e];
+        const fcomplex cb = Cb[e];
+        a0 += m * pv.re; a1 += m * pv.im;
+        b0 += x * pv.re; b1 += x * pv.im;
+        c0 += m * qv.re; c1 += m * qv.im;
+        d0 += m * rv.re + cb.re * pv.re - cb.im * pv.im;
+        d1 += m * rv.im + cb.re * pv.im + cb.im * pv.re;
+    }
+    PP[I] = {a0, a1};
+    PX[I] = {b0, b1};
+    QQ[I] = {c0, c1};
+    S[I] = {d0, d1};
+}
+
+// z pass; also writes u (the scaled network input, [CI][N]) when u is not null
+__global__ void feat_z_kernel(int n0, int n1, int n2, int p, const float *__restrict__ M, const float *__restrict__ X,
+                              const fcomplex *__restrict__ Cb, const float *__restrict__ gx,
+                              const float *__restrict__ gy, const float *__restrict__ gz, CnnScales sc,
+                              const fcomplex *__restrict__ y, const fcomplex *__restrict__ PP,
+                              const fcomplex *__restrict__ PX, const fcomplex *__restrict__ QQ,
+                              const fcomplex *__restrict__ S, fcomplex *__restrict__ My, fcomplex *__restrict__ corr,
+                              fcomplex *__restrict__ u)
+{
+    const long plane = (long)n0 * n1, N = plane * n2;
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const long mm = I % plane;
+    const int k = (int)(I / plane);
+    const int bw = 2 * p + 1;
+    const int lo = k - p < 0 ? 0 : k - p, hi = k + p > n2 - 1 ? n2 - 1 : k + p;
+    float m0 = 0, m1 = 0, xx0 = 0, xx1 = 0, xy0 = 0, xy1 = 0, xz0 = 0, xz1 = 0, c0 = 0, c1 = 0;
+    for (int K = lo; K <= hi; ++K) {
+        const long src = mm + plane * K;
+        const fcomplex pp = PP[src], px = PX[src], qq = QQ[src], sv = S[src];
+        const int e = k * bw + (K - k + p);
+        const float m = M[e], x = X[e];
+        const fcomplex cb = Cb[e];
+        m0 += m * pp.re; m1 += m * pp.im;
+        xx0 += m * qq.re; xx1 += m * qq.im;
+        xy0 += m * px.re; xy1 += m * px.im;
+        xz0 += x * pp.re; xz1 += x * pp.im;
+        c0 += m * sv.re + cb.re * pp.re - cb.im * pp.im;
+        c1 += m * sv.im + cb.re * pp.im + cb.im * pp.re;
+    }
+    My[I] = {m0, m1};
+    corr[I] = {c0, c1};
+    if (!u) return;
+    const float x0 = gx[I % n0], x1 = gy[(I / n0) % n1], x2 = gz[k];
+    const fcomplex yv = y[I];
+    u[I] = {sc.v[0] * yv.re, sc.v[0] * yv.im};
+    u[N + I] = {sc.v[1] * (xx0 - x0 * m0), sc.v[1] * (xx1 - x0 * m1)};
+    u[2 * N + I] = {sc.v[2] * (xy0 - x1 * m0), sc.v[2] * (xy1 - x1 * m1)};
+    u[3 * N + I] = {sc.v[3] * (xz0 - x2 * m0), sc.v[3] * (xz1 - x2 * m1)};
+}
+
 // u[c N + v] *= s.v[c] for the CI input fields
 __global__ void scale_inputs_kernel(long N, CnnScales s, fcomplex *__restrict__ u)
 {
@@ -494,18 +603,6 @@ __global__ void scale_inputs_kernel(long N, CnnScales s, fcomplex *__restrict__ 
     if (I >= CI * N) return;
     const float f = s.v[I / N];
     u[I] = {u[I].re * f, u[I].im * f};
-}
-
-// first-moment features: G[v] -= x_d(v) My[v], where G holds X_d y on entry
-__global__ void moment_feature_kernel(int n0, int n1, int n2, int d, const float *__restrict__ gr,
-                                      const fcomplex *__restrict__ My, fcomplex *__restrict__ G)
-{
-    const long N = (long)n0 * n1 * n2;
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
-    const int ii = d == 0 ? (int)(v % n0) : d == 1 ? (int)((v / n0) % n1) : (int)(v / ((long)n0 * n1));
-    const float x = gr[ii];
-    G[v] = {G[v].re - x * My[v].re, G[v].im - x * My[v].im};
 }
 
 // t = pv - Vg .* My: the part of the potential product the lumped form misses
@@ -522,15 +619,22 @@ inline unsigned gridn(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
 
+void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, const fcomplex *y, fcomplex *tmp[7],
+                    fcomplex *My, fcomplex *corr, fcomplex *u)
+{
+    const int n0 = fb.n[0], n1 = fb.n[1], n2 = fb.n[2];
+    const long N = (long)n0 * n1 * n2;
+    hipLaunchKernelGGL(feat_x_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, n0, N / n0, fb.p[0], fb.M[0], fb.X[0],
+                       fb.C[0], y, tmp[0], tmp[1], tmp[2]);
+    hipLaunchKernelGGL(feat_y_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, n0, n1, N, fb.p[1], fb.M[1], fb.X[1],
+                       fb.C[1], tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], tmp[6]);
+    hipLaunchKernelGGL(feat_z_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, n0, n1, n2, fb.p[2], fb.M[2], fb.X[2],
+                       fb.C[2], fb.g[0], fb.g[1], fb.g[2], sc, y, tmp[3], tmp[4], tmp[5], tmp[6], My, corr, u);
+}
+
 void scale_inputs(hipStream_t st, long N, const CnnScales &s, fcomplex *u)
 {
     hipLaunchKernelGGL(scale_inputs_kernel, dim3(gridn(CI * N, 256)), dim3(256), 0, st, N, s, u);
-}
-
-void moment_feature(hipStream_t st, const int n[3], int d, const float *gr, const fcomplex *My, fcomplex *G)
-{
-    const long N = (long)n[0] * n[1] * n[2];
-    hipLaunchKernelGGL(moment_feature_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, n[0], n[1], n[2], d, gr, My, G);
 }
 
 void lumped_residual(hipStream_t st, long N, const fcomplex *pv, const fcomplex *My, const float *Vg, fcomplex *t)
