@@ -149,6 +149,19 @@ int tdz_evolve_rational(tdz_ctx *ctx, double *psi, double t0, double dt, int nst
 /* Observables [norm psi^H M psi, Re psi^H H psi, Im psi^H H psi, <x>, <y>, <z>]
  * (MonitorProducts, src/manager.cpp:584-672; src/monitor.cpp:1145-1194). */
 int tdz_observables(tdz_ctx *ctx, const double *psi, double out[6]);
+/* One row of each trace of the observables file for the device state psi at time t
+ * (column layouts of src/monitor.cpp:1281-1330, written by tdz_io_trace_append):
+ * dip [1 + 3 dim] = [t, E_a, q <D_a>, acceleration_a (0: not computed)],
+ * energy [7] = [t, Kin, Pot, Int, Tot, <1/m>, Norm] (real parts; Int = s E.<D>,
+ * Tot = <H> + Int), ac [3] = [t, Re, Im psi0^H M psi] (zeros when psi0 is null).
+ * E is the field E_a(t) itself (not s E). Any output may be null. */
+/* Guess history of the Pade/Magnus stage solves (4 slots x 3 increments of N complex
+ * values; CheckpointGuess, src/propagator.cpp:1546-1590): set = 0 copies the counts and,
+ * if hist is not null, the increments to hist[((slot * 3) + j) * N] (host); set = 1
+ * restores them. */
+int tdz_pade_history(tdz_ctx *ctx, int set, int counts[4], double *hist);
+int tdz_trace_row(tdz_ctx *ctx, const double *psi, const double *psi0, double t, const double E[3], double *dip,
+                  double *energy, double *ac);
 /* Field sE_a(t) = s E_a(t) of a pulse (TDSEZParser::Coupling, include/tdsez_parser.hpp:190-194). */
 int tdz_pulse_coupling(const tdz_ctx *ctx, const tdz_pulse *pulse, double t, double sE[3]);
 

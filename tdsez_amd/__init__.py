@@ -32,7 +32,7 @@ EXPORTS = [
     "tdz_ksp_solve", "tdz_step_cn", "tdz_evolve_cn", "tdz_observables", "tdz_pulse_coupling",
     "tdz_timing", "tdz_timing_stats", "tdz_step_pade22", "tdz_evolve_rational",
     "tdz_project_gaussian", "tdz_ground_state",
-    "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
+    "tdz_trace_row", "tdz_pade_history", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
 ]
 
 
@@ -126,6 +126,8 @@ def lib():
         L.tdz_project_gaussian.argtypes = [vp, dp, dp, dp, vp]
         L.tdz_ground_state.argtypes = [vp, vp, d, d, i, dp, dp, ip]
         fp = C.POINTER(C.c_float)
+        L.tdz_trace_row.argtypes = [vp, vp, vp, d, dp, dp, dp, dp]
+        L.tdz_pade_history.argtypes = [vp, i, ip, dp]
         L.tdz_learn_train.argtypes = [vp, vp, d, d, C.POINTER(Pulse), i, i, i, d, d, dp, dp]
         L.tdz_learn_weights.argtypes = [vp, fp, fp, i]
         L.tdz_learn_fields.argtypes = [vp, fp]
@@ -316,6 +318,27 @@ class Context:
         b, f, a = C.c_int(0), C.c_int(0), C.c_int(0)
         _chk(lib().tdz_learn_status(self.h, C.byref(b), C.byref(f), C.byref(a)))
         return b.value, f.value, a.value
+
+    def trace_row(self, psi, t, E=(0.0, 0.0, 0.0), psi0=None):
+        """(dipoles row, energies row, autocorrelation row) in the observables-file layout."""
+        dim = self.desc["dim"]
+        dip, en, ac = np.zeros(1 + 3 * dim), np.zeros(7), np.zeros(3)
+        Ea = np.ascontiguousarray(list(E) + [0.0] * (3 - len(E)), dtype=np.float64)
+        _chk(lib().tdz_trace_row(self.h, psi.ptr, psi0.ptr if psi0 is not None else None, t, _dp(Ea), _dp(dip),
+                                 _dp(en), _dp(ac)))
+        return dip, en, ac
+
+    def pade_history(self, counts=None, hist=None):
+        """Get (no arguments) or set the Pade/Magnus guess history: counts [4], hist [4, 3, N]."""
+        ip = C.POINTER(C.c_int)
+        if counts is None:
+            cnt = np.zeros(4, np.int32)
+            h = np.zeros((4, 3, self.N), np.complex128)
+            _chk(lib().tdz_pade_history(self.h, 0, cnt.ctypes.data_as(ip), _dp(h.view(np.float64))))
+            return cnt, h
+        cnt = np.ascontiguousarray(counts, np.int32)
+        h = np.ascontiguousarray(hist, np.complex128)
+        _chk(lib().tdz_pade_history(self.h, 1, cnt.ctypes.data_as(ip), _dp(h.view(np.float64))))
 
     def observables(self, psi):
         out = np.zeros(6)

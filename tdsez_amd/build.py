@@ -35,5 +35,32 @@ def build(verbose=False):
     return LIB
 
 
+HDF5_ROOT = os.environ.get("TDZ_HDF5_ROOT", "/opt/conda")
+IO_LIB = os.path.join(HERE, "libtdz_io.so")
+
+
+def build_io(verbose=False):
+    """Host library of include/tdz_io.h (HDF5 checkpoints and traces); returns None
+    when no HDF5 installation is found (the propagator does not need it)."""
+    inc = os.path.join(HDF5_ROOT, "include")
+    lib = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        sys.stderr.write(f"tdsez_amd: no hdf5.h under {inc}; libtdz_io.so not built\n")
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           "-I" + inc, os.path.join(HERE, "csrc", "tdz_io.cpp"), "-L" + lib, "-Wl,-rpath," + lib, "-lhdf5",
+           "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building libtdz_io.so")
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_io(verbose=True))

@@ -1547,7 +1547,7 @@ void setup_learned(tdz_ctx *c)
     ls.P = upload(c, P);
     ls.W1 = track(c, dalloc<float>(27 * CI * C));
     ls.W2 = track(c, dalloc<float>(27 * C * C));
-    ls.mb = track(c, dalloc<unsigned short>((size_t)2 * N * C));
+    ls.mb = track(c, dalloc<unsigned short>(tdz::cnn_infer_work_bytes(N) / 2));
     ls.out = track(c, dalloc<float>((size_t)2 * N));
     ls.nvec = track(c, dalloc<fcomplex>(N));
     ls.feat = track(c, dalloc<fcomplex>((size_t)CI * N));
@@ -1788,37 +1788,114 @@ int tdz_pulse_coupling(const tdz_ctx *c, const tdz_pulse *pulse, double t, doubl
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// psi^H T psi for T = M, H (K + sV + CAP, no field), D_x, D_y, D_z, K, sV, in that order
+// (v[0..6]), and v[7] = psi0^H M psi when psi0 is given (the autocorrelation)
+// (MonitorProducts, src/monitor.cpp:1145-1194). Vectors are reduced on the host in FP64.
+void expectations(tdz_ctx *c, const double *psi, const double *psi0, cplx *v)
+{
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    std::vector<dcomplex> hp(N), hv(N), h0;
+    HIPCHK(hipMemcpyAsync(hp.data(), psi, N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+    if (psi0) {
+        h0.resize(N);
+        HIPCHK(hipMemcpyAsync(h0.data(), psi0, N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    auto dot = [&](int term_or_h, const std::vector<dcomplex> &left) {
+        if (term_or_h >= 0) {
+            if (tdz_term_apply(c, term_or_h, psi, reinterpret_cast<double *>(c->sd.w1))) throw Error(g_err);
+        } else {
+            const cplx a0 = c->alpha, g0 = c->g;
+            double s0[3] = {c->sE[0], c->sE[1], c->sE[2]};
+            const double zero[3] = {0, 0, 0};
+            set_shift(c, 0.0, 1.0, zero);
+            level_apply(c, L, reinterpret_cast<const dcomplex *>(psi), c->sd.w1);
+            set_shift(c, a0, g0, s0);
+        }
+        HIPCHK(hipMemcpyAsync(hv.data(), c->sd.w1, N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        cplx s = 0;
+        for (long i = 0; i < N; ++i) s += std::conj(cplx(left[i].re, left[i].im)) * cplx(hv[i].re, hv[i].im);
+        return s;
+    };
+    v[0] = dot(TDZ_TERM_M, hp);
+    v[1] = dot(-1, hp);
+    for (int a = 0; a < 3; ++a) v[2 + a] = a < c->prob.dim ? dot(TDZ_TERM_DX + a, hp) : cplx(0);
+    v[5] = dot(TDZ_TERM_K, hp);
+    v[6] = c->ph.pot_kind ? dot(TDZ_TERM_V, hp) : cplx(0);
+    v[7] = psi0 ? dot(TDZ_TERM_M, h0) : cplx(0);
+}
+}  // namespace
+
+extern "C" {
+
 int tdz_observables(tdz_ctx *c, const double *psi, double out[6])
 {
     return guarded([&] {
-        Level &L = *c->lv[0];
-        const long N = L.N;
-        std::vector<dcomplex> hp(N), hv(N);
-        HIPCHK(hipMemcpyAsync(hp.data(), psi, N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
-        auto dot = [&](int term_or_h) {
-            // term_or_h < 0: full H = K + sV + CAP
-            if (term_or_h >= 0) {
-                if (tdz_term_apply(c, term_or_h, psi, reinterpret_cast<double *>(c->sd.w1))) throw Error(g_err);
-            } else {
-                const cplx a0 = c->alpha, g0 = c->g;
-                double s0[3] = {c->sE[0], c->sE[1], c->sE[2]};
-                const double zero[3] = {0, 0, 0};
-                set_shift(c, 0.0, 1.0, zero);
-                level_apply(c, L, reinterpret_cast<const dcomplex *>(psi), c->sd.w1);
-                set_shift(c, a0, g0, s0);
+        cplx v[6];
+        expectations(c, psi, nullptr, v);
+        out[0] = v[0].real();
+        out[1] = v[1].real();
+        out[2] = v[1].imag();
+        for (int a = 0; a < 3; ++a) out[3 + a] = v[2 + a].real();
+    });
+}
+
+int tdz_pade_history(tdz_ctx *c, int set, int counts[4], double *hist)
+{
+    return guarded([&] {
+        const long N = c->lv[0]->N;
+        const size_t bytes = (size_t)4 * kGuessOrder * N * sizeof(dcomplex);
+        if (!set) {
+            for (int s = 0; s < 4; ++s) counts[s] = c->hist ? c->hist_count[s] : 0;
+            if (c->hist && hist) HIPCHK(hipMemcpy(hist, c->hist, bytes, hipMemcpyDeviceToHost));
+            return;
+        }
+        if (!c->hist) {
+            c->hist = track(c, dalloc<dcomplex>((size_t)4 * kGuessOrder * N));
+            c->xin = track(c, dalloc<dcomplex>(N));
+        }
+        for (int s = 0; s < 4; ++s) c->hist_count[s] = std::max(0, std::min(counts[s], kGuessOrder));
+        if (hist) HIPCHK(hipMemcpy(c->hist, hist, bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int tdz_trace_row(tdz_ctx *c, const double *psi, const double *psi0, double t, const double E[3], double *dip,
+                  double *energy, double *ac)
+{
+    return guarded([&] {
+        cplx v[8];
+        expectations(c, psi, psi0, v);
+        const int dim = c->prob.dim;
+        const double q = -c->ph.s;   // charge
+        if (dip) {
+            dip[0] = t;
+            for (int a = 0; a < dim; ++a) {
+                dip[1 + a] = E ? E[a] : 0.0;
+                dip[1 + dim + a] = q * v[2 + a].real();
+                dip[1 + 2 * dim + a] = 0.0;   // acceleration is not computed
             }
-            HIPCHK(hipMemcpyAsync(hv.data(), c->sd.w1, N * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
-            HIPCHK(hipStreamSynchronize(c->st));
-            cplx s = 0;
-            for (long i = 0; i < N; ++i) s += std::conj(cplx(hp[i].re, hp[i].im)) * cplx(hv[i].re, hv[i].im);
-            return s;
-        };
-        out[0] = dot(TDZ_TERM_M).real();
-        const cplx e = dot(-1);
-        out[1] = e.real();
-        out[2] = e.imag();
-        for (int a = 0; a < 3; ++a) out[3 + a] = a < c->prob.dim ? dot(TDZ_TERM_DX + a).real() : 0.0;
+        }
+        if (energy) {
+            double inter = 0;
+            for (int a = 0; a < dim; ++a) inter += c->ph.s * (E ? E[a] : 0.0) * v[2 + a].real();
+            energy[0] = t;
+            energy[1] = v[5].real();                // kinetic
+            energy[2] = v[6].real();                // potential s V
+            energy[3] = inter;                      // s E . <D>
+            energy[4] = v[1].real() + inter;        // <H> + interaction
+            energy[5] = v[0].real() / c->ph.mass;   // <1/m> (constant mass)
+            energy[6] = v[0].real();                // norm
+        }
+        if (ac) {
+            ac[0] = t;
+            ac[1] = v[7].real();
+            ac[2] = v[7].imag();
+        }
     });
 }
 

@@ -159,7 +159,9 @@ void cnn_forward_f32(hipStream_t st, const int n[3], const fcomplex *u, const fl
 void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const fcomplex *target, float inv_t2,
                       const float *W2, const float *P, const float *m, const float *out, float *dov, float *dh,
                       float *partial, int nchunk, float *dW1, float *dW2);
-// BF16 MFMA forward; mb [2N][C] bf16 work, out [2N]; if nout: nout = g (out_re + i out_im)
+// BF16 MFMA forward; mb work of cnn_infer_work_bytes(N) bytes ([2N][C] bf16 plus the
+// packed weights), out [2N]; if nout: nout = g (out_re + i out_im)
+inline size_t cnn_infer_work_bytes(long N) { return (size_t)2 * N * kCnnChannels * 2 + 64 * 64 * 8; }
 void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
                     const float *P, unsigned short *mb, float *out, fcomplex g, fcomplex *nout);
 void cnn_to_complex(hipStream_t st, long N, const float *out, fcomplex g, fcomplex *n);

@@ -698,22 +698,33 @@ __global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, 
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) ar[a][b] = ai[a][b] = 0.0;
+    constexpr int NE = GK * GT / 256;
     for (int k0 = 0; k0 < n; k0 += GK) {
-        for (int e = tid; e < GK * GT; e += 256) {
-            // T tile: Ts[k][i] = T(i0 + i, k0 + k)
+        // T tile Ts[k][i] = T(i0 + i, k0 + k) and the input tile; all loads of the chunk
+        // in flight before the LDS stores
+        R tv[NE];
+        cpx<R> uv[NE];
+#pragma unroll
+        for (int r = 0; r < NE; ++r) {
+            const int e = tid + 256 * r;
             const int kk = e % GK, ii = e / GK;
             const int gi = i0 + ii, gk = k0 + kk;
-            R v = 0.0;
-            if (gi < n && gk < n) v = trans ? T[(long)gk * n + gi] : T[(long)gi * n + gk];
-            Ts[kk][ii] = v;
-            // input tile
+            tv[r] = 0.0;
+            if (gi < n && gk < n) tv[r] = trans ? T[(long)gk * n + gi] : T[(long)gi * n + gk];
             int kq, lq;
             if (kfast) { kq = e % GK; lq = e / GK; } else { lq = e % GT; kq = e / GT; }
             const int gk2 = k0 + kq;
             const long gl = l0 + lq;
-            cpx<R> u = {0.0, 0.0};
-            if (gk2 < n && gl < L) u = in[boff + gk2 * sK + gl * sL];
-            Is[kq][lq] = u;
+            uv[r] = {0.0, 0.0};
+            if (gk2 < n && gl < L) uv[r] = in[boff + gk2 * sK + gl * sL];
+        }
+#pragma unroll
+        for (int r = 0; r < NE; ++r) {
+            const int e = tid + 256 * r;
+            Ts[e % GK][e / GK] = tv[r];
+            int kq, lq;
+            if (kfast) { kq = e % GK; lq = e / GK; } else { lq = e % GT; kq = e / GT; }
+            Is[kq][lq] = uv[r];
         }
         __syncthreads();
 #pragma unroll 4
@@ -769,21 +780,33 @@ __global__ void __launch_bounds__(256) axis_gemm_mfma_kernel(int n, long L, long
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) accr[a][b] = acci[a][b] = {0.f, 0.f, 0.f, 0.f};
+    constexpr int NE = GK * GT / 256;
     for (int k0 = 0; k0 < n; k0 += GK) {
-        for (int e = tid; e < GK * GT; e += 256) {
+        // all loads of the chunk in flight before the LDS stores
+        float tv[NE];
+        fcomplex uv[NE];
+#pragma unroll
+        for (int r = 0; r < NE; ++r) {
+            const int e = tid + 256 * r;
             const int kk = e % GK, ii = e / GK;
             const int gi = i0 + ii, gk = k0 + kk;
-            float v = 0.f;
-            if (gi < n && gk < n) v = trans ? T[(long)gk * n + gi] : T[(long)gi * n + gk];
-            Ts[kk][ii] = v;
+            tv[r] = 0.f;
+            if (gi < n && gk < n) tv[r] = trans ? T[(long)gk * n + gi] : T[(long)gi * n + gk];
             int kq, lq;
             if (kfast) { kq = e % GK; lq = e / GK; } else { lq = e % GT; kq = e / GT; }
             const int gk2 = k0 + kq;
             const long gl = l0 + lq;
-            fcomplex u = {0.f, 0.f};
-            if (gk2 < n && gl < L) u = in[boff + gk2 * sK + gl * sL];
-            Ir[kq][lq] = u.re;
-            Ii[kq][lq] = u.im;
+            uv[r] = {0.f, 0.f};
+            if (gk2 < n && gl < L) uv[r] = in[boff + gk2 * sK + gl * sL];
+        }
+#pragma unroll
+        for (int r = 0; r < NE; ++r) {
+            const int e = tid + 256 * r;
+            Ts[e % GK][e / GK] = tv[r];
+            int kq, lq;
+            if (kfast) { kq = e % GK; lq = e / GK; } else { lq = e % GT; kq = e / GT; }
+            Ir[kq][lq] = uv[r].re;
+            Ii[kq][lq] = uv[r].im;
         }
         __syncthreads();
 #pragma unroll

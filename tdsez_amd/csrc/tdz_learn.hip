@@ -294,11 +294,14 @@ __device__ __forceinline__ short to_bf16(float f)
 }
 
 // Tiles of the MFMA inference: a block of 256 threads (4 waves) owns 16 (x) x 4 (y)
-// x 4 (z) voxels of one component; wave w owns z-plane w of the tile and its four
-// 16-voxel rows along x. The block first stages the tile plus a one-voxel halo
-// (18 x 6 x 6) in LDS, so every input value is read from HBM/L2 once per block.
-constexpr int TX = 16, TY = 4, TZ = 4;
+// x 12 (z) voxels of one component; wave w owns z-planes 3w .. 3w+2 of the tile, each
+// as four 16-voxel rows along x. The block first stages the tile plus a one-voxel halo
+// (18 x 6 x 14) in LDS, so every input value is read from HBM/L2 about twice per tile
+// volume instead of 27 times, and the bf16 weights (packed once per call in the lanes'
+// operand order) are loaded once per block.
+constexpr int TX = 16, TY = 4, TZ = 12, WZ = TZ / 4;
 constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+constexpr int NPACK = NTAP + K1 / 16;   // packed operand slots: 27 layer-2 taps, then the layer-1 K steps
 
 struct TileIdx {
     int comp, x0, y0, z0;
@@ -323,67 +326,109 @@ __device__ __forceinline__ TileIdx tile_of(int n0, int n1, int n2)
 
 __device__ __forceinline__ int tap_of_halo(int lx, int ly, int lz) { return (lz * HY + ly) * HX + lx; }
 
+__device__ __forceinline__ bf16x4 unpack4(uint2 raw)
+{
+    bf16x4 a;
+    a[0] = (short)(raw.x & 0xFFFFu);
+    a[1] = (short)(raw.x >> 16);
+    a[2] = (short)(raw.y & 0xFFFFu);
+    a[3] = (short)(raw.y >> 16);
+    return a;
+}
+
+__device__ __forceinline__ uint2 pack4(float a, float b, float c, float d)
+{
+    return {(unsigned)(unsigned short)to_bf16(a) | ((unsigned)(unsigned short)to_bf16(b) << 16),
+            (unsigned)(unsigned short)to_bf16(c) | ((unsigned)(unsigned short)to_bf16(d) << 16)};
+}
+
+// wp[slot * 64 + lane]: the lane's 4 bf16 B-operand values. Slots t < 27: W2[t][kb+q][row];
+// slots 27 + s: W1[t][q][row] with t = 4 s + kb / 4 (zero past the last tap).
+__global__ void cnn_pack_kernel(const float *__restrict__ W1, const float *__restrict__ W2, uint2 *__restrict__ wp)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NPACK * 64) return;
+    const int slot = e / 64, lane = e % 64, row = lane & 15, kb = (lane >> 4) * 4;
+    float v[4];
+    if (slot < NTAP) {
+        for (int q = 0; q < 4; ++q) v[q] = W2[(slot * C + kb + q) * C + row];
+    } else {
+        const int t = 4 * (slot - NTAP) + kb / 4;
+        for (int q = 0; q < 4; ++q) v[q] = t < NTAP ? W1[(t * CI + q) * C + row] : 0.f;
+    }
+    wp[e] = pack4(v[0], v[1], v[2], v[3]);
+}
+
 // Layer 1 on MFMA, v_mfma_f32_16x16x16_bf16. Operand layout (wave64): lane l holds
 // A[l%16][4(l/16)+q], B[4(l/16)+q][l%16], q < 4, and D[4(l/16)+q][l%16].
 // A[i][k] = u_c(v_i + tap t), k = t CI + c (zero for k >= 27 CI); B[k][a] = W1[t][c][a];
 // D[i][a] -> m = P[v_i][a] D[i][a], stored as bf16 at mb[(comp N + v) C + a].
 __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2, const fcomplex *__restrict__ u,
-                                                          const float *__restrict__ W1, const float *__restrict__ P,
+                                                          const uint2 *__restrict__ wp, const float *__restrict__ P,
                                                           unsigned short *__restrict__ mb)
 {
     static_assert(CI == 4, "a lane's 4 consecutive k must be the CI channels of one tap");
     __shared__ float4 us[HV];   // halo tile, the CI channels of one component per voxel
     const TileIdx T = tile_of(n0, n1, n2);
     const long N = (long)n0 * n1 * n2;
-    for (int e = threadIdx.x; e < CI * HV; e += blockDim.x) {
-        const int c = e / HV, h = e % HV;
-        const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
-        float val = 0.f;
-        if (gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2)
-            val = comp_of(u[(long)c * N + gx + (long)n0 * (gy + (long)n1 * gz)], T.comp);
-        reinterpret_cast<float *>(us)[h * CI + c] = val;
+    // stage in two phases (all loads in flight, then all LDS stores): a load-store loop
+    // would wait for each load in turn
+    constexpr int NE = (CI * HV + 255) / 256;
+    float val[NE];
+#pragma unroll
+    for (int r = 0; r < NE; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        val[r] = 0.f;
+        if (e < CI * HV) {
+            const int c = e / HV, h = e % HV;
+            const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
+            if (gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2)
+                val[r] = comp_of(u[(long)c * N + gx + (long)n0 * (gy + (long)n1 * gz)], T.comp);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NE; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        if (e < CI * HV) reinterpret_cast<float *>(us)[(e % HV) * CI + e / HV] = val[r];
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row = lane & 15, kb = (lane >> 4) * 4;
-    // this lane's B operand for the 7 K steps: W1[t][q][row], t = 4 step + kb / 4
     bf16x4 bw[K1 / 16];
 #pragma unroll
-    for (int s = 0; s < K1 / 16; ++s) {
-        const int t = 4 * s + kb / 4;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) bw[s][q] = t < NTAP ? to_bf16(W1[(t * CI + q) * C + row]) : (short)0;
-    }
+    for (int s = 0; s < K1 / 16; ++s) bw[s] = unpack4(wp[(NTAP + s) * 64 + lane]);
     __syncthreads();
-    floatx4 acc[TY];
+    for (int pz = 0; pz < WZ; ++pz) {
+        const int lz = w * WZ + pz, gz = T.z0 + lz;
+        floatx4 acc[TY];
 #pragma unroll
-    for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
+        for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int s = 0; s < K1 / 16; ++s) {
-        const int t = 4 * s + kb / 4;
-        const int tt = t < NTAP ? t : 13;   // padded k: any in-tile voxel, B is zero there
-        const int dx = tap_dx(tt), dy = tap_dy(tt), dz = tap_dz(tt);
+        for (int s = 0; s < K1 / 16; ++s) {
+            const int t = 4 * s + kb / 4;
+            const int tt = t < NTAP ? t : 13;   // padded k: any in-tile voxel, B is zero there
+            const int dx = tap_dx(tt), dy = tap_dy(tt), dz = tap_dz(tt);
+#pragma unroll
+            for (int ry = 0; ry < TY; ++ry) {
+                const float4 v = us[tap_of_halo(row + 1 + dx, ry + 1 + dy, lz + 1 + dz)];
+                bf16x4 a;
+                a[0] = to_bf16(v.x);
+                a[1] = to_bf16(v.y);
+                a[2] = to_bf16(v.z);
+                a[3] = to_bf16(v.w);
+                acc[ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[s], acc[ry], 0, 0, 0);
+            }
+        }
+        // lane holds D[kb + q][row]: voxel x0 + kb + q, hidden channel row
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) {
-            const float4 v = us[tap_of_halo(row + 1 + dx, ry + 1 + dy, w + 1 + dz)];
-            bf16x4 a;
-            a[0] = to_bf16(v.x);
-            a[1] = to_bf16(v.y);
-            a[2] = to_bf16(v.z);
-            a[3] = to_bf16(v.w);
-            acc[ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[s], acc[ry], 0, 0, 0);
-        }
-    }
-    // lane holds D[kb + q][row]: voxel x0 + kb + q, hidden channel row
-    const int gz = T.z0 + w;
+            const int gy = T.y0 + ry;
 #pragma unroll
-    for (int ry = 0; ry < TY; ++ry) {
-        const int gy = T.y0 + ry;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int gx = T.x0 + kb + q;
-            if (gx >= n0 || gy >= n1 || gz >= n2) continue;
-            const long v = gx + (long)n0 * (gy + (long)n1 * gz);
-            mb[((long)T.comp * N + v) * C + row] = (unsigned short)to_bf16(P[v * C + row] * acc[ry][q]);
+            for (int q = 0; q < 4; ++q) {
+                const int gx = T.x0 + kb + q;
+                if (gx >= n0 || gy >= n1 || gz >= n2) continue;
+                const long v = gx + (long)n0 * (gy + (long)n1 * gz);
+                mb[((long)T.comp * N + v) * C + row] = (unsigned short)to_bf16(P[v * C + row] * acc[ry][q]);
+            }
         }
     }
 }
@@ -396,63 +441,68 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
 constexpr int MS = 20;   // LDS stride of one voxel of m, in bf16 elements
 __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2,
                                                           const unsigned short *__restrict__ mb,
-                                                          const float *__restrict__ W2, const float *__restrict__ P,
+                                                          const uint2 *__restrict__ wp, const float *__restrict__ P,
                                                           float *__restrict__ out)
 {
     __shared__ uint2 ms[HV * MS / 4];
     const TileIdx T = tile_of(n0, n1, n2);
     const long N = (long)n0 * n1 * n2;
-    // stage: 4 x 8 bytes per voxel (channels 4p .. 4p+3)
-    for (int e = threadIdx.x; e < 4 * HV; e += blockDim.x) {
-        const int h = e >> 2, part = e & 3;
-        const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
-        uint2 val = {0u, 0u};
-        if (gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2) {
-            const long v = gx + (long)n0 * (gy + (long)n1 * gz);
-            val = reinterpret_cast<const uint2 *>(mb + ((long)T.comp * N + v) * C)[part];
+    // stage 4 x 8 bytes per voxel (channels 4p .. 4p+3) in two phases (all loads in
+    // flight, then all LDS stores)
+    constexpr int NE = (4 * HV + 255) / 256;
+    uint2 val[NE];
+#pragma unroll
+    for (int r = 0; r < NE; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        val[r] = {0u, 0u};
+        if (e < 4 * HV) {
+            const int h = e >> 2, part = e & 3;
+            const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
+            if (gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2) {
+                const long v = gx + (long)n0 * (gy + (long)n1 * gz);
+                val[r] = reinterpret_cast<const uint2 *>(mb + ((long)T.comp * N + v) * C)[part];
+            }
         }
-        ms[(h * MS) / 4 + part] = val;
+    }
+#pragma unroll
+    for (int r = 0; r < NE; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        if (e < 4 * HV) ms[((e >> 2) * MS) / 4 + (e & 3)] = val[r];
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row = lane & 15, kb = (lane >> 4) * 4;
-    // this lane's B operand of every tap: W2[t][kb + q][row]
     bf16x4 bw[NTAP];
 #pragma unroll
-    for (int t = 0; t < NTAP; ++t)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) bw[t][q] = to_bf16(W2[(t * C + kb + q) * C + row]);
+    for (int t = 0; t < NTAP; ++t) bw[t] = unpack4(wp[t * 64 + lane]);
     __syncthreads();
-    floatx4 acc[TY];
+    for (int pz = 0; pz < WZ; ++pz) {
+        const int lz = w * WZ + pz, gz = T.z0 + lz;
+        floatx4 acc[TY];
 #pragma unroll
-    for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
+        for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < NTAP; ++t) {
-        const int dx = tap_dx(t), dy = tap_dy(t), dz = tap_dz(t);
+        for (int t = 0; t < NTAP; ++t) {
+            const int dx = tap_dx(t), dy = tap_dy(t), dz = tap_dz(t);
+#pragma unroll
+            for (int ry = 0; ry < TY; ++ry) {
+                const int h = tap_of_halo(row + 1 + dx, ry + 1 + dy, lz + 1 + dz);
+                acc[ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(unpack4(ms[(h * MS) / 4 + kb / 4]), bw[t],
+                                                                   acc[ry], 0, 0, 0);
+            }
+        }
+        // lane holds D[kb + q][row] (voxel x0 + kb + q, output channel row)
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) {
-            const int h = tap_of_halo(row + 1 + dx, ry + 1 + dy, w + 1 + dz);
-            const uint2 raw = ms[(h * MS) / 4 + kb / 4];
-            bf16x4 a;
-            a[0] = (short)(raw.x & 0xFFFFu);
-            a[1] = (short)(raw.x >> 16);
-            a[2] = (short)(raw.y & 0xFFFFu);
-            a[3] = (short)(raw.y >> 16);
-            acc[ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[t], acc[ry], 0, 0, 0);
-        }
-    }
-    // lane holds D[kb + q][row] (voxel x0 + kb + q, output channel row)
-    const int gz = T.z0 + w;
+            const int gy = T.y0 + ry;
 #pragma unroll
-    for (int ry = 0; ry < TY; ++ry) {
-        const int gy = T.y0 + ry;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int gx = T.x0 + kb + q;
-            const bool in = gx < n0 && gy < n1 && gz < n2;
-            const long v = in ? gx + (long)n0 * (gy + (long)n1 * gz) : 0;
-            float s = in ? P[v * C + row] * acc[ry][q] : 0.f;
-            for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-            if (row == 0 && in) out[(long)T.comp * N + v] = s;
+            for (int q = 0; q < 4; ++q) {
+                const int gx = T.x0 + kb + q;
+                const bool in = gx < n0 && gy < n1 && gz < n2;
+                const long v = in ? gx + (long)n0 * (gy + (long)n1 * gz) : 0;
+                float s = in ? P[v * C + row] * acc[ry][q] : 0.f;
+                for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+                if (row == 0 && in) out[(long)T.comp * N + v] = s;
+            }
         }
     }
 }
@@ -670,9 +720,11 @@ void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const flo
 {
     const long N = (long)n[0] * n[1] * n[2];
     const long ntiles = 2L * ((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ);
-    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n[0], n[1], n[2], u, W1, P,
-                       mb);
-    hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n[0], n[1], n[2], mb, W2, P,
+    // the packed weights go behind the 2 N C bf16 of m (the work buffer holds both)
+    uint2 *wp = reinterpret_cast<uint2 *>(mb + 2 * N * C);
+    hipLaunchKernelGGL(cnn_pack_kernel, dim3(gridn(NPACK * 64, 256)), dim3(256), 0, st, W1, W2, wp);
+    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n[0], n[1], n[2], u, wp, P, mb);
+    hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n[0], n[1], n[2], mb, wp, P,
                        out);
     if (nout) hipLaunchKernelGGL(cnn_to_complex_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, N, out, g, nout);
 }
