@@ -192,6 +192,27 @@ int tdz_learn_eval(tdz_ctx *ctx, const double *u, const double *target, double *
                    double *out_f32, double *out_mfma);
 int tdz_learn_status(tdz_ctx *ctx, int *base_its, int *fallbacks, int *active);
 
+/* Batched scan (no reference counterpart: the reference runs a scan as separate
+ * jobs). nruns independent problems (any grids, degrees and pulses) advance together:
+ * one CN step of every run is captured once in a HIP graph (one branch per run: the
+ * field bands of the step from a precomputed s E(t_n + dt/2) table, the right-hand
+ * side, kiters FGMRES iterations with the Hessenberg problem on the device, the
+ * update) and each step is one graph launch. A run whose residual estimate after
+ * the captured iterations is above rtol finishes that step on the host FGMRES
+ * (counted in tdz_batch_fixups), so every step meets the same FP64 test as
+ * tdz_step_cn. pc_kind: TDZ_PC_SEPARABLE, TDZ_PC_SEPARABLE_MIXED or TDZ_PC_NONE;
+ * 1 <= kiters <= 8 and <= restart. nsteps bounds the steps the field table covers.
+ * tdz_batch_context gives a run's context (for set-up, e.g. tdz_ground_state on psi)
+ * and its state vector (device, owned by the batch). its and res ([nsteps][nruns])
+ * receive the iterations and relative residuals of each step. */
+typedef struct tdz_batch tdz_batch;
+int tdz_batch_create(const tdz_problem *probs, const tdz_pulse *pulses, int nruns, double dt, int nsteps,
+                     int pc_kind, int kiters, tdz_batch **out);
+int tdz_batch_context(tdz_batch *b, int run, tdz_ctx **ctx, double **psi);
+int tdz_batch_evolve(tdz_batch *b, int nsteps, double rtol, int maxit, int *its, double *res);
+int tdz_batch_fixups(const tdz_batch *b, long *fixups);
+int tdz_batch_destroy(tdz_batch *b);
+
 /* Measurement hooks (no reference counterpart; used by bench.py): HIP events
  * recorded on the context's stream around every launch of the fine-level
  * potential kernel while enabled; stats returns their summed duration and count. */

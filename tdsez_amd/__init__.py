@@ -32,7 +32,8 @@ EXPORTS = [
     "tdz_ksp_solve", "tdz_step_cn", "tdz_evolve_cn", "tdz_observables", "tdz_pulse_coupling",
     "tdz_timing", "tdz_timing_stats", "tdz_step_pade22", "tdz_evolve_rational",
     "tdz_project_gaussian", "tdz_ground_state",
-    "tdz_trace_row", "tdz_pade_history", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
+    "tdz_trace_row", "tdz_pade_history", "tdz_batch_create", "tdz_batch_context", "tdz_batch_evolve",
+    "tdz_batch_fixups", "tdz_batch_destroy", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
 ]
 
 
@@ -127,6 +128,11 @@ def lib():
         L.tdz_ground_state.argtypes = [vp, vp, d, d, i, dp, dp, ip]
         fp = C.POINTER(C.c_float)
         L.tdz_trace_row.argtypes = [vp, vp, vp, d, dp, dp, dp, dp]
+        L.tdz_batch_create.argtypes = [C.POINTER(Problem), C.POINTER(Pulse), i, d, i, i, i, C.POINTER(vp)]
+        L.tdz_batch_context.argtypes = [vp, i, C.POINTER(vp), C.POINTER(vp)]
+        L.tdz_batch_evolve.argtypes = [vp, i, d, i, ip, dp]
+        L.tdz_batch_fixups.argtypes = [vp, C.POINTER(C.c_long)]
+        L.tdz_batch_destroy.argtypes = [vp]
         L.tdz_pade_history.argtypes = [vp, i, ip, dp]
         L.tdz_learn_train.argtypes = [vp, vp, d, d, C.POINTER(Pulse), i, i, i, d, d, dp, dp]
         L.tdz_learn_weights.argtypes = [vp, fp, fp, i]
@@ -360,3 +366,69 @@ class Context:
         ms, n = C.c_double(0), C.c_long(0)
         _chk(lib().tdz_timing_stats(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class _BorrowedVec(DeviceVec):
+    """A device vector owned by a batch (not freed by Python)."""
+
+    def __init__(self, ctx, ptr):
+        self.ctx = ctx
+        self.ptr = ptr
+
+    def free(self):
+        self.ptr = None
+
+
+class _BorrowedContext(Context):
+    """A run's context inside a batch (destroyed with the batch)."""
+
+    def __init__(self, desc, h):
+        self.desc = dict(desc)
+        self.h = h
+        self.N = int(lib().tdz_size(self.h))
+        n = (C.c_int * 3)()
+        _chk(lib().tdz_dims(self.h, n))
+        self.n = tuple(n)
+        self.levels = int(lib().tdz_levels(self.h))
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Many independent runs advanced by one captured HIP graph per step (tdz_batch_*)."""
+
+    def __init__(self, descs, pulses, dt, nsteps, pc=PC_SEPARABLE_MIXED, kiters=4):
+        n = len(descs)
+        probs = (Problem * n)(*[make_problem(dd) for dd in descs])
+        pls = (Pulse * n)(*pulses) if pulses is not None else None
+        h = C.c_void_p()
+        _chk(lib().tdz_batch_create(probs, pls, n, dt, nsteps, pc, kiters, C.byref(h)))
+        self.h = h
+        self.nruns = n
+        self.runs, self.psi = [], []
+        for r in range(n):
+            ch, ph = C.c_void_p(), C.c_void_p()
+            _chk(lib().tdz_batch_context(self.h, r, C.byref(ch), C.byref(ph)))
+            ctx = _BorrowedContext(descs[r], ch)
+            self.runs.append(ctx)
+            self.psi.append(_BorrowedVec(ctx, ph))
+
+    def evolve(self, nsteps, rtol=1e-10, maxit=500):
+        its = np.zeros((nsteps, self.nruns), np.int32)
+        res = np.zeros((nsteps, self.nruns))
+        _chk(lib().tdz_batch_evolve(self.h, nsteps, rtol, maxit, its.ctypes.data_as(C.POINTER(C.c_int)), _dp(res)))
+        return its, res
+
+    def fixups(self):
+        f = C.c_long(0)
+        _chk(lib().tdz_batch_fixups(self.h, C.byref(f)))
+        return f.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tdz_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

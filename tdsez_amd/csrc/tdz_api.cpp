@@ -94,7 +94,8 @@ struct Level {
     // transfer to the next coarser level: P (this x coarse) and R = P^T
     CSR P[3], R[3];
     // dim 3 with element e starting at function e on every axis (tiled kernel applies)
-    bool simple = false;
+    bool simple = false;    // 3D, simple knots on every axis: the tiled 3D potential kernel
+    bool simple2 = false;   // 2D, simple knots: the tiled 2D potential kernel
     int pot_variant = -1;   // tile shape of the tiled kernel picked by the autotuner (-1: generic)
 };
 
@@ -159,6 +160,17 @@ struct tdz_ctx {
     Scratch<float> sf;
     bool mixed_ready = false;   // single-precision copies allocated
     dcomplex *rhs = nullptr;
+    // captured CN step (batched scan): device-side FGMRES state, this run's field per
+    // step, step counter, residual estimate per step, the field-dependent bands
+    struct Captured {
+        tdz::GmState *gm = nullptr;
+        double *bnorm2 = nullptr, *sE = nullptr, *res = nullptr;
+        int *step = nullptr;
+        dcomplex *coef = nullptr, *T0[3] = {}, *bandR[3] = {}, *bandL[3] = {};
+        double *Xb[3] = {}, *M0 = nullptr;
+        int nsteps = 0, k = 0;
+        double dt = 0;
+    } cap;
     // chained CN right-hand side: A psi of the last solve of an evolve run
     dcomplex *ax = nullptr, *zerov = nullptr, *dDip[3] = {};
     bool ax_valid = false;
@@ -400,9 +412,11 @@ void pot_apply(tdz_ctx *c, Level &L, cplx g, const cpx<R> *x, cpx<R> *y)
         }
         HIPCHK(hipEventRecord(c->ev[c->ev_used].first, c->st));
     }
-    if (!(L.simple && c->use_tiled &&
-          tdz::launch_pot_apply_3d<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y, L.pot_variant)))
-        tdz::launch_pot_apply_generic<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y);
+    const bool tiled =
+        c->use_tiled &&
+        ((L.simple && tdz::launch_pot_apply_3d<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y, L.pot_variant)) ||
+         (L.simple2 && tdz::launch_pot_apply_2d<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y, L.pot_variant)));
+    if (!tiled) tdz::launch_pot_apply_generic<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y);
     if (timed) HIPCHK(hipEventRecord(c->ev[c->ev_used++].second, c->st));
 }
 
@@ -424,8 +438,8 @@ void level_apply(tdz_ctx *c, Level &L, const cpx<R> *x, cpx<R> *y)
 void autotune_pot(tdz_ctx *c, Level &L)
 {
     L.pot_variant = -1;
-    if (!L.simple || !c->ph.pot_kind) return;
-    const int nv = tdz::pot3d_variants(L.ax[0].p, L.ax[0].Q);
+    if (!(L.simple || L.simple2) || !c->ph.pot_kind) return;
+    const int nv = L.simple ? tdz::pot3d_variants(L.ax[0].p, L.ax[0].Q) : tdz::pot2d_variants(L.ax[0].p, L.ax[0].Q);
     if (nv == 0) return;
     if (const char *f = std::getenv("TDZ_POT_VARIANT")) {
         const int k = std::atoi(f);
@@ -442,10 +456,14 @@ void autotune_pot(tdz_ctx *c, Level &L)
     tdz::AxisDev ax[3] = {axis_dev<double>(L, 0), axis_dev<double>(L, 1), axis_dev<double>(L, 2)};
     tdz::PotParams pp{c->ph.pot_kind, c->ph.s, c->ph.pot[0], c->ph.pot[1]};
     float best = 1e30f;
+    auto launch = [&](int v) {
+        if (L.simple) tdz::launch_pot_apply_3d<double>(c->st, ax, pp, 1.0, 0.0, x, y, v);
+        else tdz::launch_pot_apply_2d<double>(c->st, ax, pp, 1.0, 0.0, x, y, v);
+    };
     for (int v = 0; v < nv; ++v) {
-        tdz::launch_pot_apply_3d<double>(c->st, ax, pp, 1.0, 0.0, x, y, v);
+        launch(v);
         HIPCHK(hipEventRecord(a, c->st));
-        for (int r = 0; r < 2; ++r) tdz::launch_pot_apply_3d<double>(c->st, ax, pp, 1.0, 0.0, x, y, v);
+        for (int r = 0; r < 2; ++r) launch(v);
         HIPCHK(hipEventRecord(b, c->st));
         HIPCHK(hipEventSynchronize(b));
         float ms = 0;
@@ -972,9 +990,11 @@ void build(tdz_ctx *c, const tdz_problem &pr)
             L->Vs[d] = L->ax[d].Vs;
         }
         L->N = (long)L->n[0] * L->n[1] * L->n[2];
-        L->simple = pr.dim == 3;
-        for (int d = 0; d < 3; ++d)
-            for (int e = 0; e < L->ax[d].E; ++e) L->simple = L->simple && L->ax[d].off[e] == e;
+        bool uniform = true;
+        for (int d = 0; d < pr.dim; ++d)
+            for (int e = 0; e < L->ax[d].E; ++e) uniform = uniform && L->ax[d].off[e] == e;
+        L->simple = pr.dim == 3 && uniform;
+        L->simple2 = pr.dim == 2 && uniform;
         c->lv.push_back(std::move(L));
     }
     // ---- transfers and Galerkin bands (P^T A P per axis; exact for separable terms)
@@ -2108,6 +2128,264 @@ int tdz_timing_stats(tdz_ctx *c, double *total_ms, long *launches)
         *total_ms = t;
         *launches = (long)c->ev_used;
     });
+}
+
+}  // extern "C"
+
+// ===================================================================== batched scan
+namespace {
+// Device data of a run for captured CN steps: the static part of each axis band
+// T0 = S/(2m) - i W (complex), the position bands, the field s E(t_n + dt/2) of every
+// step, and the work of the device-side FGMRES(k).
+void setup_capture(tdz_ctx *c, double dt, const std::vector<double> &sE, int nsteps, int k)
+{
+    Level &L = *c->lv[0];
+    auto &C = c->cap;
+    if (k < 1 || k > tdz::kGmMax) throw Error("captured FGMRES length must be in [1, 8]");
+    if (k > c->restart) throw Error("captured FGMRES length exceeds the Krylov basis");
+    C.k = k;
+    C.dt = dt;
+    C.nsteps = nsteps;
+    C.gm = track(c, dalloc<tdz::GmState>(1));
+    C.bnorm2 = track(c, dalloc<double>(1));
+    C.coef = track(c, dalloc<dcomplex>(tdz::kGmMax));
+    C.step = track(c, dalloc<int>(1));
+    C.res = track(c, dalloc<double>(nsteps + 1));
+    HIPCHK(hipMemset(C.step, 0, sizeof(int)));
+    HIPCHK(hipMemset(C.res, 0, (nsteps + 1) * sizeof(double)));
+    C.sE = upload(c, sE);
+    const double inv2m = 0.5 / c->ph.mass;
+    for (int d = 0; d < 3; ++d) {
+        const size_t sz = L.M[d].size();
+        std::vector<dcomplex> t0(sz, dcomplex{0, 0});
+        std::vector<double> xb(sz, 0.0);
+        if (d < c->prob.dim)
+            for (size_t q = 0; q < sz; ++q) {
+                t0[q] = {inv2m * L.S[d][q], c->cap_on ? -L.Wc[d][q] : 0.0};
+                xb[q] = L.Xb[d][q];
+            }
+        C.T0[d] = upload(c, t0);
+        C.Xb[d] = upload(c, xb);
+        C.bandR[d] = track(c, dalloc<dcomplex>(sz));
+        C.bandL[d] = track(c, dalloc<dcomplex>(sz));
+    }
+    C.M0 = upload(c, L.M[0]);
+}
+
+// Records one CN step of psi into the context's stream (which is capturing): field
+// bands of this step, rhs = (M - g H_E) psi, r0 = 2 (rhs - M psi), k FGMRES iterations
+// with the Hessenberg problem on the device, the solution update and the step counter.
+// No host synchronisation and no memset inside.
+void record_cn_step(tdz_ctx *c, dcomplex *x)
+{
+    Level &L = *c->lv[0];
+    auto &C = c->cap;
+    const long N = L.N;
+    const int k = C.k;
+    const cplx g(0.0, 0.5 * C.dt);
+    for (int d = 0; d < 3; ++d) {
+        const long sz = (long)L.M[d].size();
+        tdz::launch_field_band(c->st, sz, d == 0 ? C.M0 : nullptr, C.T0[d], C.Xb[d], dc(-g), C.sE, C.step, d,
+                               C.bandR[d]);
+        tdz::launch_field_band(c->st, sz, d == 0 ? C.M0 : nullptr, C.T0[d], C.Xb[d], dc(g), C.sE, C.step, d,
+                               C.bandL[d]);
+    }
+    dcomplex *own[3] = {L.d.B[0], L.d.B[1], L.d.B[2]};
+    const cplx g0 = c->g;
+    for (int d = 0; d < 3; ++d) L.d.B[d] = C.bandR[d];
+    c->g = -g;
+    level_apply(c, L, x, c->rhs);
+    tdz::SepBandsT<double> mb = sep_bands<double>(L);
+    for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
+    tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
+    const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+    if (c->prob.dirichlet) tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
+    dcomplex *V0 = c->Vb;
+    HIPCHK(hipMemcpyAsync(V0, c->rhs, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+    tdz::launch_axpby(c->st, {-2.0, 0}, c->sd.w1, {2.0, 0}, V0, N);
+    tdz::launch_norm2(c->st, c->rhs, N, c->dpartr, C.bnorm2);
+    tdz::launch_norm2(c->st, V0, N, c->dpartr, c->dnorm);
+    tdz::launch_gm_start(c->st, c->dnorm, C.bnorm2, C.gm);
+    tdz::launch_scale_invsqrt(c->st, V0, c->dnorm, V0, N);
+    for (int d = 0; d < 3; ++d) L.d.B[d] = C.bandL[d];
+    c->g = g;
+    for (int j = 0; j < k; ++j) {
+        dcomplex *Vj = c->Vb + (long)j * N, *Zj = c->Zb + (long)j * N, *w = c->Vb + (long)(j + 1) * N;
+        pc_apply_dev(c, Vj, Zj);
+        level_apply(c, L, Zj, w);
+        tdz::launch_multi_dot(c->st, c->Vb, N, j + 1, w, N, c->dpart, c->dh);
+        tdz::launch_multi_axpy_norm(c->st, c->Vb, N, j + 1, c->dh, w, N, c->dpartr, c->dnorm);
+        tdz::launch_multi_dot(c->st, c->Vb, N, j + 1, w, N, c->dpart, c->dh2);
+        tdz::launch_multi_axpy_norm(c->st, c->Vb, N, j + 1, c->dh2, w, N, c->dpartr, c->dnorm);
+        tdz::launch_gm_arnoldi(c->st, j, k, c->dh, c->dh2, c->dnorm, C.gm);
+        tdz::launch_scale_invsqrt(c->st, w, c->dnorm, w, N);
+    }
+    tdz::launch_gm_solve(c->st, k, k, C.gm, C.coef, C.res, C.step);
+    tdz::launch_multi_axpy(c->st, c->Zb, N, k, C.coef, x, N);
+    tdz::launch_step_advance(c->st, C.step);
+    for (int d = 0; d < 3; ++d) L.d.B[d] = own[d];
+    c->g = g0;
+}
+}  // namespace
+
+struct tdz_batch {
+    std::vector<tdz_ctx *> runs;
+    std::vector<dcomplex *> psi;
+    std::vector<int> step;    // host copy of each run's step counter
+    std::vector<std::vector<double>> sE;
+    hipStream_t st = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double dt = 0;
+    int nsteps = 0, k = 0;
+    long fixups = 0;
+    double *hres = nullptr;   // pinned: every run's residual estimate of the last step
+};
+
+namespace {
+void batch_free(tdz_batch *b)
+{
+    if (b->st) hipStreamSynchronize(b->st);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (auto e : b->join) hipEventDestroy(e);
+    if (b->fork) hipEventDestroy(b->fork);
+    for (tdz_ctx *c : b->runs) tdz_destroy(c);
+    if (b->hres) hipHostFree(b->hres);
+    if (b->st) hipStreamDestroy(b->st);
+    delete b;
+}
+
+// One graph: fork from the batch stream to every run's stream, one recorded CN step
+// per run, join back. The runs are independent branches of the graph.
+void capture_batch(tdz_batch *b)
+{
+    HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    try {
+        HIPCHK(hipEventRecord(b->fork, b->st));
+        for (size_t r = 0; r < b->runs.size(); ++r) {
+            tdz_ctx *c = b->runs[r];
+            HIPCHK(hipStreamWaitEvent(c->st, b->fork, 0));
+            record_cn_step(c, b->psi[r]);
+            HIPCHK(hipEventRecord(b->join[r], c->st));
+            HIPCHK(hipStreamWaitEvent(b->st, b->join[r], 0));
+        }
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(b->st, &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(b->st, &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+}
+}  // namespace
+
+extern "C" {
+
+int tdz_batch_create(const tdz_problem *probs, const tdz_pulse *pulses, int nruns, double dt, int nsteps,
+                     int pc_kind, int kiters, tdz_batch **out)
+{
+    if (!probs || !out || nruns < 1 || nsteps < 1 || dt <= 0) { g_err = "bad batch arguments"; return 1; }
+    auto *b = new tdz_batch();
+    const int rc = guarded([&] {
+        if (pc_kind != TDZ_PC_SEPARABLE && pc_kind != TDZ_PC_SEPARABLE_MIXED && pc_kind != TDZ_PC_NONE)
+            throw Error("the captured step takes the separable preconditioners (fp64 or complex64) or none");
+        b->dt = dt;
+        b->nsteps = nsteps;
+        b->k = kiters;
+        HIPCHK(hipStreamCreate(&b->st));
+        HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        for (int r = 0; r < nruns; ++r) {
+            tdz_ctx *c = nullptr;
+            if (tdz_create(&probs[r], &c)) throw Error(std::string("run ") + std::to_string(r) + ": " + g_err);
+            b->runs.push_back(c);
+            pc_setup(c, pc_kind);
+            dcomplex *p = track(c, dalloc<dcomplex>(c->lv[0]->N));
+            HIPCHK(hipMemset(p, 0, c->lv[0]->N * sizeof(dcomplex)));
+            b->psi.push_back(p);
+            std::vector<double> e((size_t)3 * (nsteps + 1), 0.0);
+            for (int s = 0; s <= nsteps; ++s)
+                if (pulses) tdz_pulse_coupling(c, &pulses[r], (s + 0.5) * dt, &e[3 * (size_t)s]);
+            for (int s = 0; s <= nsteps; ++s)
+                for (int a = c->prob.dim; a < 3; ++a) e[3 * (size_t)s + a] = 0.0;
+            b->sE.push_back(e);
+            setup_capture(c, dt, e, nsteps, kiters);
+            hipEvent_t j;
+            HIPCHK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+            b->join.push_back(j);
+        }
+        b->step.assign(nruns, 0);
+        HIPCHK(hipHostMalloc((void **)&b->hres, nruns * sizeof(double), hipHostMallocDefault));
+        for (tdz_ctx *c : b->runs) HIPCHK(hipStreamSynchronize(c->st));
+    });
+    if (rc) {
+        batch_free(b);
+        *out = nullptr;
+        return rc;
+    }
+    *out = b;
+    return 0;
+}
+
+int tdz_batch_destroy(tdz_batch *b)
+{
+    if (b) batch_free(b);
+    return 0;
+}
+
+int tdz_batch_context(tdz_batch *b, int run, tdz_ctx **ctx, double **psi)
+{
+    return guarded([&] {
+        if (!b || run < 0 || run >= (int)b->runs.size()) throw Error("no such run");
+        if (ctx) *ctx = b->runs[run];
+        if (psi) *psi = reinterpret_cast<double *>(b->psi[run]);
+    });
+}
+
+int tdz_batch_evolve(tdz_batch *b, int nsteps, double rtol, int maxit, int *its, double *res)
+{
+    return guarded([&] {
+        const int R = (int)b->runs.size();
+        if (b->step[0] + nsteps > b->nsteps) throw Error("batch: past the last precomputed field step");
+        for (tdz_ctx *c : b->runs) HIPCHK(hipStreamSynchronize(c->st));
+        if (!b->exec) capture_batch(b);
+        double *r = b->hres;
+        for (int s = 0; s < nsteps; ++s) {
+            HIPCHK(hipGraphLaunch(b->exec, b->st));
+            for (int q = 0; q < R; ++q)
+                HIPCHK(hipMemcpyAsync(&r[q], b->runs[q]->cap.res + b->step[q], sizeof(double),
+                                      hipMemcpyDeviceToHost, b->st));
+            HIPCHK(hipStreamSynchronize(b->st));
+            for (int q = 0; q < R; ++q) {
+                tdz_ctx *c = b->runs[q];
+                int it = b->k;
+                if (!(r[q] <= rtol)) {
+                    // the captured cycle fell short: finish this run's step on the host solver
+                    // from the graph's iterate and right-hand side, with this step's field
+                    const double *e = &b->sE[q][3 * (size_t)b->step[q]];
+                    set_shift(c, 1.0, cplx(0.0, 0.5 * b->dt), e);
+                    tdz_ksp_stats st{};
+                    fgmres(c, c->rhs, b->psi[q], rtol, maxit, &st);
+                    it += st.its;
+                    r[q] = st.res;
+                    ++b->fixups;
+                }
+                if (its) its[(size_t)s * R + q] = it;
+                if (res) res[(size_t)s * R + q] = r[q];
+                ++b->step[q];
+            }
+        }
+    });
+}
+
+int tdz_batch_fixups(const tdz_batch *b, long *fixups)
+{
+    if (!b || !fixups) { g_err = "null argument"; return 1; }
+    *fixups = b->fixups;
+    return 0;
 }
 
 }  // extern "C"

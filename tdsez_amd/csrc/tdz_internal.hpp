@@ -91,6 +91,12 @@ bool launch_pot_apply_3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot,
                          cpx<R> *y, int variant);
 // number of tile-shape variants of the tiled kernel for (p, Q) (0: none)
 int pot3d_variants(int p, int Q);
+// 2D grids with simple knots, p in 3..5, Q = 8: tiled kernel, variant = tile shape
+// (0: 1x1, 1: 2x1, 2: 2x2, 3: 4x2, 4: 4x4 elements); false if not applicable.
+template <typename R>
+bool launch_pot_apply_2d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
+                         cpx<R> *y, int variant);
+int pot2d_variants(int p, int Q);
 // Separable Kronecker apply y = Mz My Bx x + Mz By Mx x + Bz My Mx x, bands on device:
 // M_d real [n_d*(2p+1)], B_d complex [n_d*(2p+1)]; tmp = 4 scratch vectors.
 template <typename R>
@@ -148,6 +154,25 @@ void launch_diag_scale(hipStream_t st, const int n[3], const R *lam[3], const un
 template <typename R>
 void launch_csr_axis(hipStream_t st, int d, const int nin[3], int m, const int *rowptr, const int *col,
                      const R *val, const cpx<R> *in, cpx<R> *out, bool accumulate);
+
+// ---------------------------------------------------------------- device FGMRES(k)
+constexpr int kGmMax = 8;   // longest captured cycle
+struct GmState {
+    dcomplex H[(kGmMax + 1) * kGmMax];
+    double cs[kGmMax];
+    dcomplex sn[kGmMax];
+    dcomplex gv[kGmMax + 1];
+    double bnorm;
+    int done;
+};
+void launch_gm_start(hipStream_t st, const double *beta2, const double *bnorm2, GmState *g);
+// column j of an m-column cycle from h = h1 + h2 (the two CGS passes) and ||w||^2
+void launch_gm_arnoldi(hipStream_t st, int j, int m, const dcomplex *h1, const dcomplex *h2, const double *wn2,
+                       GmState *g);
+void launch_gm_solve(hipStream_t st, int k, int m, GmState *g, dcomplex *y, double *res, const int *step);
+void launch_step_advance(hipStream_t st, int *step);
+void launch_field_band(hipStream_t st, long n, const double *M, const dcomplex *T0, const double *X, dcomplex g,
+                       const double *sE, const int *step, int d, dcomplex *B);
 
 // ---------------------------------------------------------------- learned part
 constexpr int kCnnChannels = 16;

@@ -160,6 +160,129 @@ __global__ void __launch_bounds__(128) pot_apply_generic_kernel(AxisDevT<R> a0, 
     }
 }
 
+// ------------------------------------------------------ potential, 2D tiles
+// One workgroup owns TXE x TYE elements of a 2D grid with simple knots (element e
+// starts at function e on both axes) and one thread per quadrature point:
+//   load the (TXE+P) x (TYE+P) coefficient patch, x stage -> T1 [rows][x points],
+//   y stage at every point times g s V(r) w -> F, transposed y stage -> T2, transposed
+//   x stage -> patch contributions, atomically added to y (neighbouring tiles share P
+//   functions per axis). Tile shapes are picked per grid by the autotuner.
+template <typename R, int P, int Q, int TXE, int TYE, int KIND>
+__global__ void __launch_bounds__(TXE * Q * TYE * Q) pot2d_kernel(AxisDevT<R> a0, AxisDevT<R> a1, PotParams pp,
+                                                                  R gr, R gi, const cpx<R> *__restrict__ x,
+                                                                  cpx<R> *__restrict__ y)
+{
+    constexpr int NB = P + 1, CX = TXE + P, CY = TYE + P, PX = TXE * Q, PY = TYE * Q, NT = PX * PY;
+    __shared__ R Bx[TXE][NB][Q], By[TYE][NB][Q];
+    __shared__ R Xq[PX], Wx[PX], Yq[PY], Wy[PY];
+    __shared__ cpx<R> c[CY][CX];
+    __shared__ cpx<R> T1[CY][PX];
+    __shared__ cpx<R> F[PY][PX];
+    const int tid = threadIdx.x;
+    const int ex0 = blockIdx.x * TXE, ey0 = blockIdx.y * TYE;
+    const int Ex = a0.E, Ey = a1.E;
+    const long n0 = a0.n, n1 = a1.n;
+    for (int i = tid; i < TXE * NB * Q; i += NT) {
+        const int lx = i / (NB * Q), r = i % (NB * Q);
+        Bx[lx][r / Q][r % Q] = ex0 + lx < Ex ? a0.B0[(size_t)(ex0 + lx) * NB * Q + r] : (R)0;
+    }
+    for (int i = tid; i < TYE * NB * Q; i += NT) {
+        const int ly = i / (NB * Q), r = i % (NB * Q);
+        By[ly][r / Q][r % Q] = ey0 + ly < Ey ? a1.B0[(size_t)(ey0 + ly) * NB * Q + r] : (R)0;
+    }
+    for (int i = tid; i < PX; i += NT) {
+        const int e = ex0 + i / Q;
+        Xq[i] = e < Ex ? a0.X[(size_t)e * Q + i % Q] : (R)0;
+        Wx[i] = e < Ex ? a0.W[(size_t)e * Q + i % Q] : (R)0;   // zero weight: no contribution
+    }
+    for (int i = tid; i < PY; i += NT) {
+        const int e = ey0 + i / Q;
+        Yq[i] = e < Ey ? a1.X[(size_t)e * Q + i % Q] : (R)0;
+        Wy[i] = e < Ey ? a1.W[(size_t)e * Q + i % Q] : (R)0;
+    }
+    for (int i = tid; i < CY * CX; i += NT) {
+        const int jy = i / CX, ix = i % CX;
+        const long gx = ex0 + ix, gy = ey0 + jy;
+        c[jy][ix] = (gx < n0 && gy < n1) ? x[gx + n0 * gy] : cpx<R>{0, 0};
+    }
+    __syncthreads();
+    // x stage: T1[j][px] = sum_b Bx[lx][b][q] c[j][lx + b]
+    for (int i = tid; i < CY * PX; i += NT) {
+        const int j = i / PX, px = i % PX, lx = px / Q, q = px % Q;
+        R sr = 0, si = 0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const R bv = Bx[lx][b][q];
+            const cpx<R> v = c[j][lx + b];
+            sr += bv * v.re;
+            si += bv * v.im;
+        }
+        T1[j][px] = {sr, si};
+    }
+    __syncthreads();
+    // y stage and the point coefficient g s V(r) w: one point per thread
+    {
+        const int py = tid / PX, px = tid % PX, ly = py / Q, q = py % Q;
+        R sr = 0, si = 0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const R bv = By[ly][b][q];
+            const cpx<R> v = T1[ly + b][px];
+            sr += bv * v.re;
+            si += bv * v.im;
+        }
+        const R r2 = Xq[px] * Xq[px] + Yq[py] * Yq[py];
+        R vv;
+        if (KIND == 2) vv = -(R)pp.a0 / sqrt(r2 + (R)pp.a1);
+        else vv = (R)0.5 * (R)pp.a0 * r2;
+        const R cv = (R)pp.s * vv * Wx[px] * Wy[py];
+        const R cr = gr * cv, ci = gi * cv;
+        F[py][px] = {cr * sr - ci * si, cr * si + ci * sr};
+    }
+    __syncthreads();
+    // transposed y: T1[j][px] = sum_{ly + b = j} sum_q By[ly][b][q] F[ly Q + q][px]
+    for (int i = tid; i < CY * PX; i += NT) {
+        const int j = i / PX, px = i % PX;
+        R sr = 0, si = 0;
+#pragma unroll
+        for (int ly = 0; ly < TYE; ++ly) {
+            const int b = j - ly;
+            if (b < 0 || b > P) continue;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const R bv = By[ly][b][q];
+                const cpx<R> v = F[ly * Q + q][px];
+                sr += bv * v.re;
+                si += bv * v.im;
+            }
+        }
+        T1[j][px] = {sr, si};
+    }
+    __syncthreads();
+    // transposed x and scatter-add: y[patch (j, i)] += sum_{lx + b = i} sum_q Bx[lx][b][q] T1[j][lx Q + q]
+    for (int k = tid; k < CY * CX; k += NT) {
+        const int j = k / CX, i = k % CX;
+        const long gx = ex0 + i, gy = ey0 + j;
+        if (gx >= n0 || gy >= n1) continue;
+        R sr = 0, si = 0;
+#pragma unroll
+        for (int lx = 0; lx < TXE; ++lx) {
+            const int b = i - lx;
+            if (b < 0 || b > P) continue;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const R bv = Bx[lx][b][q];
+                const cpx<R> v = T1[j][lx * Q + q];
+                sr += bv * v.re;
+                si += bv * v.im;
+            }
+        }
+        R *dst = reinterpret_cast<R *>(y + (gx + n0 * gy));
+        unsafeAtomicAdd(dst, sr);
+        unsafeAtomicAdd(dst + 1, si);
+    }
+}
+
 // ------------------------------------------------------ potential, 3D marching
 // One workgroup owns a TE x TE tile of elements in (x, y) and marches through
 // all element layers in z. The (x, y) stages of the sum factorisation are done
@@ -577,10 +700,10 @@ __global__ void axpby_kernel(dcomplex a, const dcomplex *__restrict__ x, dcomple
     }
 }
 
-__global__ void scale_invsqrt_kernel(const dcomplex *__restrict__ x, const double *__restrict__ n2,
-                                     dcomplex *__restrict__ y, long N)
+// y = x / sqrt(n2) (in place allowed; a zero norm leaves zero)
+__global__ void scale_invsqrt_kernel(const dcomplex *x, const double *__restrict__ n2, dcomplex *y, long N)
 {
-    const double s = 1.0 / sqrt(n2[0]);
+    const double s = n2[0] > 0.0 ? 1.0 / sqrt(n2[0]) : 0.0;
     const long stride = (long)gridDim.x * blockDim.x;
     for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
         const dcomplex v = x[I];
@@ -699,11 +822,11 @@ __global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, 
 #pragma unroll
         for (int b = 0; b < 4; ++b) ar[a][b] = ai[a][b] = 0.0;
     constexpr int NE = GK * GT / 256;
-    for (int k0 = 0; k0 < n; k0 += GK) {
-        // T tile Ts[k][i] = T(i0 + i, k0 + k) and the input tile; all loads of the chunk
-        // in flight before the LDS stores
-        R tv[NE];
-        cpx<R> uv[NE];
+    // T tile Ts[k][i] = T(i0 + i, k0 + k) and the input tile; software pipeline: the
+    // global loads of chunk k0 + GK are in flight while chunk k0 is multiplied out of LDS
+    R tv[NE];
+    cpx<R> uv[NE];
+    auto load = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < NE; ++r) {
             const int e = tid + 256 * r;
@@ -718,6 +841,9 @@ __global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, 
             uv[r] = {0.0, 0.0};
             if (gk2 < n && gl < L) uv[r] = in[boff + gk2 * sK + gl * sL];
         }
+    };
+    load(0);
+    for (int k0 = 0; k0 < n; k0 += GK) {
 #pragma unroll
         for (int r = 0; r < NE; ++r) {
             const int e = tid + 256 * r;
@@ -727,6 +853,7 @@ __global__ void __launch_bounds__(256) axis_gemm_kernel(int n, long L, long sB, 
             Is[kq][lq] = uv[r];
         }
         __syncthreads();
+        if (k0 + GK < n) load(k0 + GK);
 #pragma unroll 4
         for (int kk = 0; kk < GK; ++kk) {
             R tv[4];
@@ -781,10 +908,11 @@ __global__ void __launch_bounds__(256) axis_gemm_mfma_kernel(int n, long L, long
 #pragma unroll
         for (int b = 0; b < 2; ++b) accr[a][b] = acci[a][b] = {0.f, 0.f, 0.f, 0.f};
     constexpr int NE = GK * GT / 256;
-    for (int k0 = 0; k0 < n; k0 += GK) {
-        // all loads of the chunk in flight before the LDS stores
-        float tv[NE];
-        fcomplex uv[NE];
+    // software pipeline: the global loads of chunk k0 + GK are in flight while chunk k0
+    // is multiplied out of LDS
+    float tv[NE];
+    fcomplex uv[NE];
+    auto load = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < NE; ++r) {
             const int e = tid + 256 * r;
@@ -799,6 +927,9 @@ __global__ void __launch_bounds__(256) axis_gemm_mfma_kernel(int n, long L, long
             uv[r] = {0.f, 0.f};
             if (gk2 < n && gl < L) uv[r] = in[boff + gk2 * sK + gl * sL];
         }
+    };
+    load(0);
+    for (int k0 = 0; k0 < n; k0 += GK) {
 #pragma unroll
         for (int r = 0; r < NE; ++r) {
             const int e = tid + 256 * r;
@@ -809,6 +940,7 @@ __global__ void __launch_bounds__(256) axis_gemm_mfma_kernel(int n, long L, long
             Ii[kq][lq] = uv[r].im;
         }
         __syncthreads();
+        if (k0 + GK < n) load(k0 + GK);
 #pragma unroll
         for (int s = 0; s < GK / 4; ++s) {
             const int kk = 4 * s + k4;
@@ -921,6 +1053,51 @@ static void launch_pot3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot,
     else
         hipLaunchKernelGGL((pot3d_kernel<R, P, Q, TE, CPT, 1>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr,
                            gi, x, y);
+}
+
+template <typename R, int P, int Q, int TXE, int TYE>
+static void launch_pot2d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
+                         cpx<R> *y)
+{
+    const dim3 grid((unsigned)((ax[0].E + TXE - 1) / TXE), (unsigned)((ax[1].E + TYE - 1) / TYE));
+    constexpr int NT = TXE * Q * TYE * Q;
+    if (pot.kind == 2)
+        hipLaunchKernelGGL((pot2d_kernel<R, P, Q, TXE, TYE, 2>), grid, dim3(NT), 0, st, ax[0], ax[1], pot, gr, gi, x,
+                           y);
+    else
+        hipLaunchKernelGGL((pot2d_kernel<R, P, Q, TXE, TYE, 1>), grid, dim3(NT), 0, st, ax[0], ax[1], pot, gr, gi, x,
+                           y);
+}
+
+// tile shapes (elements) of the 2D kernel: 1x1 (64 threads), 2x1, 2x2, 4x2, 4x4 (1024)
+constexpr int kPot2dVariants = 5;
+int pot2d_variants(int p, int Q) { return (Q == 8 && p >= 3 && p <= 5) ? kPot2dVariants : 0; }
+
+template <typename R, int P>
+static void launch_pot2d_p(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
+                           cpx<R> *y, int variant)
+{
+    switch (variant) {
+    case 0: launch_pot2d<R, P, 8, 1, 1>(st, ax, pot, gr, gi, x, y); break;
+    case 1: launch_pot2d<R, P, 8, 2, 1>(st, ax, pot, gr, gi, x, y); break;
+    case 2: launch_pot2d<R, P, 8, 2, 2>(st, ax, pot, gr, gi, x, y); break;
+    case 3: launch_pot2d<R, P, 8, 4, 2>(st, ax, pot, gr, gi, x, y); break;
+    default: launch_pot2d<R, P, 8, 4, 4>(st, ax, pot, gr, gi, x, y); break;
+    }
+}
+
+template <typename R>
+bool launch_pot_apply_2d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
+                         cpx<R> *y, int variant)
+{
+    const int p = ax[0].p, Q = ax[0].Q;
+    if (ax[1].p != p || ax[1].Q != Q) return false;
+    if (pot.kind != 1 && pot.kind != 2) return false;
+    if (variant < 0 || variant >= pot2d_variants(p, Q)) return false;
+    if (p == 3) launch_pot2d_p<R, 3>(st, ax, pot, gr, gi, x, y, variant);
+    else if (p == 4) launch_pot2d_p<R, 4>(st, ax, pot, gr, gi, x, y, variant);
+    else launch_pot2d_p<R, 5>(st, ax, pot, gr, gi, x, y, variant);
+    return true;
 }
 
 int pot3d_variants(int p, int Q)
@@ -1086,10 +1263,122 @@ void launch_csr_axis(hipStream_t st, int d, const int nin[3], int m, const int *
                        rowptr, col, val, in, out, accumulate ? 1 : 0);
 }
 
+// ------------------------------------------------------------ device-side FGMRES(k)
+// The small Hessenberg least-squares problem of a fixed-length FGMRES cycle on the
+// device (one thread), so that a whole time step can be captured in a HIP graph
+// without host round trips; same rotations as the host solver.
+namespace {
+__device__ __forceinline__ dcomplex cmul(dcomplex a, dcomplex b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ dcomplex cconj(dcomplex a) { return {a.re, -a.im}; }
+__device__ __forceinline__ dcomplex cadd(dcomplex a, dcomplex b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ dcomplex csub(dcomplex a, dcomplex b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ double cabs2(dcomplex a) { return a.re * a.re + a.im * a.im; }
+
+__global__ void gm_start_kernel(const double *beta2, const double *bnorm2, GmState *g)
+{
+    g->gv[0] = {sqrt(beta2[0]), 0.0};
+    for (int i = 1; i <= kGmMax; ++i) g->gv[i] = {0.0, 0.0};
+    g->bnorm = sqrt(bnorm2[0]);
+    g->done = 0;
+}
+
+__global__ void gm_arnoldi_kernel(int j, int m, const dcomplex *h1, const dcomplex *h2, const double *wn2, GmState *g)
+{
+    dcomplex *H = g->H;   // [(m + 1) x m] row-major, column j
+    for (int i = 0; i <= j; ++i) H[i * m + j] = cadd(h1[i], h2[i]);
+    const double hn = sqrt(wn2[0]);
+    H[(j + 1) * m + j] = {hn, 0.0};
+    for (int i = 0; i < j; ++i) {
+        const dcomplex a = H[i * m + j], b = H[(i + 1) * m + j];
+        const double c = g->cs[i];
+        const dcomplex sn = g->sn[i];
+        H[i * m + j] = cadd({c * a.re, c * a.im}, cmul(sn, b));
+        const dcomplex t = cmul(cconj(sn), a);
+        H[(i + 1) * m + j] = {c * b.re - t.re, c * b.im - t.im};
+    }
+    const dcomplex h1j = H[j * m + j], h2j = H[(j + 1) * m + j];
+    const double rho = sqrt(cabs2(h1j) + cabs2(h2j));
+    const double a1 = sqrt(cabs2(h1j));
+    if (a1 == 0.0) {
+        g->cs[j] = 0.0;
+        g->sn[j] = {1.0, 0.0};
+    } else {
+        g->cs[j] = a1 / rho;
+        const dcomplex ph = {h1j.re / a1, h1j.im / a1};
+        const dcomplex t = cmul(ph, cconj(h2j));
+        g->sn[j] = {t.re / rho, t.im / rho};
+    }
+    const double c = g->cs[j];
+    const dcomplex sn = g->sn[j];
+    H[j * m + j] = cadd({c * h1j.re, c * h1j.im}, cmul(sn, h2j));
+    H[(j + 1) * m + j] = {0.0, 0.0};
+    const dcomplex gj = g->gv[j];
+    const dcomplex t = cmul(cconj(sn), gj);
+    g->gv[j + 1] = {-t.re, -t.im};
+    g->gv[j] = {c * gj.re, c * gj.im};
+}
+
+// y = R^-1 g over the first k columns; res[*step] = |g_k| / ||b||
+__global__ void gm_solve_kernel(int k, int m, GmState *g, dcomplex *y, double *res, const int *step)
+{
+    const dcomplex *H = g->H;
+    for (int i = k - 1; i >= 0; --i) {
+        dcomplex s = g->gv[i];
+        for (int l = i + 1; l < k; ++l) s = csub(s, cmul(H[i * m + l], y[l]));
+        const dcomplex d = H[i * m + i];
+        const double dd = cabs2(d);
+        y[i] = dd > 0.0 ? dcomplex{(s.re * d.re + s.im * d.im) / dd, (s.im * d.re - s.re * d.im) / dd}
+                        : dcomplex{0.0, 0.0};
+    }
+    res[*step] = g->bnorm > 0.0 ? sqrt(cabs2(g->gv[k])) / g->bnorm : 0.0;
+}
+
+__global__ void step_advance_kernel(int *step) { ++step[0]; }
+
+// B[i] = (M ? M[i] : 0) + g (T0[i] + sE[3 step + d] X[i]) (band of one axis for this step's field)
+__global__ void field_band_kernel(long n, const double *__restrict__ M, const dcomplex *__restrict__ T0,
+                                  const double *__restrict__ X, dcomplex g, const double *__restrict__ sE,
+                                  const int *__restrict__ step, int d, dcomplex *__restrict__ B)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double e = sE[3 * (long)step[0] + d];
+    const dcomplex t = {T0[i].re + e * X[i], T0[i].im};
+    const dcomplex gt = cmul(g, t);
+    B[i] = {(M ? M[i] : 0.0) + gt.re, gt.im};
+}
+}  // namespace
+
+void launch_gm_start(hipStream_t st, const double *beta2, const double *bnorm2, GmState *g)
+{
+    hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(1), 0, st, beta2, bnorm2, g);
+}
+void launch_gm_arnoldi(hipStream_t st, int j, int m, const dcomplex *h1, const dcomplex *h2, const double *wn2,
+                       GmState *g)
+{
+    hipLaunchKernelGGL(gm_arnoldi_kernel, dim3(1), dim3(1), 0, st, j, m, h1, h2, wn2, g);
+}
+void launch_gm_solve(hipStream_t st, int k, int m, GmState *g, dcomplex *y, double *res, const int *step)
+{
+    hipLaunchKernelGGL(gm_solve_kernel, dim3(1), dim3(1), 0, st, k, m, g, y, res, step);
+}
+void launch_step_advance(hipStream_t st, int *step)
+{
+    hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(1), 0, st, step);
+}
+void launch_field_band(hipStream_t st, long n, const double *M, const dcomplex *T0, const double *X, dcomplex g,
+                       const double *sE, const int *step, int d, dcomplex *B)
+{
+    hipLaunchKernelGGL(field_band_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, M, T0, X, g, sE,
+                       step, d, B);
+}
+
 #define TDZ_INSTANTIATE(R)                                                                                          \
     template void launch_pot_apply_generic<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,  \
                                               cpx<R> *);                                                           \
     template bool launch_pot_apply_3d<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,       \
+                                         cpx<R> *, int);                                                           \
+    template bool launch_pot_apply_2d<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,       \
                                          cpx<R> *, int);                                                           \
     template void launch_sep_apply<R>(hipStream_t, const SepBandsT<R> &, const cpx<R> *, cpx<R> *, cpx<R> *[4]);   \
     template void launch_wall_fix<R>(hipStream_t, const int[3], const unsigned char *[3], const cpx<R> *,          \

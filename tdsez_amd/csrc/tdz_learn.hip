@@ -399,6 +399,16 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
     __syncthreads();
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
+        // the plane's positional values for this lane's outputs, loaded ahead of the MFMAs
+        float pv[TY][4];
+#pragma unroll
+        for (int ry = 0; ry < TY; ++ry)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gx = T.x0 + kb + q, gy = T.y0 + ry;
+                pv[ry][q] = (gx < n0 && gy < n1 && gz < n2) ? P[(gx + (long)n0 * (gy + (long)n1 * gz)) * C + row]
+                                                            : 0.f;
+            }
         floatx4 acc[TY];
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
@@ -427,7 +437,7 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
                 const int gx = T.x0 + kb + q;
                 if (gx >= n0 || gy >= n1 || gz >= n2) continue;
                 const long v = gx + (long)n0 * (gy + (long)n1 * gz);
-                mb[((long)T.comp * N + v) * C + row] = (unsigned short)to_bf16(P[v * C + row] * acc[ry][q]);
+                mb[((long)T.comp * N + v) * C + row] = (unsigned short)to_bf16(pv[ry][q] * acc[ry][q]);
             }
         }
     }
@@ -477,6 +487,15 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
     __syncthreads();
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
+        float pv[TY][4];   // positional values of this lane's outputs, loaded ahead of the MFMAs
+#pragma unroll
+        for (int ry = 0; ry < TY; ++ry)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gx = T.x0 + kb + q, gy = T.y0 + ry;
+                pv[ry][q] = (gx < n0 && gy < n1 && gz < n2) ? P[(gx + (long)n0 * (gy + (long)n1 * gz)) * C + row]
+                                                            : 0.f;
+            }
         floatx4 acc[TY];
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
@@ -490,19 +509,28 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
                                                                    acc[ry], 0, 0, 0);
             }
         }
-        // lane holds D[kb + q][row] (voxel x0 + kb + q, output channel row)
+        // lane holds D[kb + q][row] (voxel x0 + kb + q, output channel row). out(v) sums the
+        // 16 lanes of a group: recursive halving over the 16 values (ry, q) leaves lane row
+        // with the sum for k = 4 ry + q = row (15 exchanges instead of 64).
+        float v16[16];
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry) {
-            const int gy = T.y0 + ry;
+        for (int ry = 0; ry < TY; ++ry)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int gx = T.x0 + kb + q;
-                const bool in = gx < n0 && gy < n1 && gz < n2;
-                const long v = in ? gx + (long)n0 * (gy + (long)n1 * gz) : 0;
-                float s = in ? P[v * C + row] * acc[ry][q] : 0.f;
-                for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-                if (row == 0 && in) out[(long)T.comp * N + v] = s;
+            for (int q = 0; q < 4; ++q) v16[4 * ry + q] = pv[ry][q] * acc[ry][q];
+#pragma unroll
+        for (int m = 8; m > 0; m >>= 1) {
+            const bool up = (row & m) != 0;
+#pragma unroll
+            for (int i = 0; i < m; ++i) {
+                const float keep = up ? v16[i + m] : v16[i];
+                const float send = up ? v16[i] : v16[i + m];
+                v16[i] = keep + __shfl_xor(send, m, 16);
             }
+        }
+        {
+            const int ry = row >> 2, q = row & 3;
+            const int gx = T.x0 + kb + q, gy = T.y0 + ry;
+            if (gx < n0 && gy < n1 && gz < n2) out[(long)T.comp * N + gx + (long)n0 * (gy + (long)n1 * gz)] = v16[0];
         }
     }
 }
