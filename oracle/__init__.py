@@ -23,6 +23,9 @@ class Params(C.Structure):
         ("mass", C.c_double), ("charge", C.c_double),
         ("pot_kind", C.c_int), ("pot", C.c_double * 4),
         ("cap_kind", C.c_int), ("cap", C.c_double * 4),
+        ("knots", C.POINTER(C.c_double)), ("nknots", C.c_int), ("nlayers", C.c_int),
+        ("layer_bounds", C.POINTER(C.c_double)), ("layer_mass", C.POINTER(C.c_double)),
+        ("layer_v", C.POINTER(C.c_double)),
     ]
 
 
@@ -45,6 +48,22 @@ def make_params(desc):
     p.cap_kind = desc.get("cap_kind", 0)
     for i, v in enumerate(desc.get("cap", ())):
         p.cap[i] = v
+    # layered axis (the last active one): optional knot vector and layers
+    # desc["knots"]: knot vector; desc["layers"]: (bounds [n+1], masses [n], potentials [n])
+    keep = []
+    if desc.get("knots") is not None:
+        k = np.ascontiguousarray(desc["knots"], np.float64)
+        keep.append(k)
+        p.knots = k.ctypes.data_as(C.POINTER(C.c_double))
+        p.nknots = k.size
+    if desc.get("layers") is not None:
+        b, m, v = (np.ascontiguousarray(a, np.float64) for a in desc["layers"])
+        keep += [b, m, v]
+        p.nlayers = m.size
+        p.layer_bounds = b.ctypes.data_as(C.POINTER(C.c_double))
+        p.layer_mass = m.ctypes.data_as(C.POINTER(C.c_double))
+        p.layer_v = v.ctypes.data_as(C.POINTER(C.c_double))
+    p._keep = keep   # the arrays must outlive the create call
     return p
 
 

@@ -54,6 +54,16 @@ typedef struct {
     double pot[4];
     int cap_kind;        // 0 none, 1 quadratic eta*(w-s)^2, 2 Manolopoulos(kmin)
     double cap[4];
+    // layered axis (the last active one): an optional knot vector (repeated interior
+    // knots give C^(p-k) continuity; multiplicity p = C0 interfaces) and optional
+    // piecewise-constant layers [bounds[i], bounds[i+1]) of effective mass and potential
+    // offset (BenDaniel-Duke kinetic term 1/2 grad (1/m) grad; the offset enters as s V)
+    const double *knots;
+    int nknots;
+    int nlayers;
+    const double *layer_bounds;
+    const double *layer_mass;
+    const double *layer_v;
 } orc_params;
 }
 
@@ -135,6 +145,9 @@ struct Axis {
     std::vector<double> M, Xb, G, A, S, Wc, Vs;  // bands n x (2p+1); Vs: separable part of s V
     std::vector<double> capq;  // CAP profile W_d at the points [E*Q]
     std::vector<char> isWall;  // [n]
+    std::vector<int> off;      // [E] first function of each element
+    // layered axis only: bands of 1/m, (1/m) B'B' and the layer potential V_l
+    std::vector<double> Mw, Sw, Vl;
 };
 
 struct Problem {
@@ -143,6 +156,20 @@ struct Problem {
     Axis ax[3];
     size_t N = 1;
     double s = 1;  // carrier sign
+    bool layered = false;
+    int lax = 0;   // the layered axis (dim - 1)
+    std::vector<double> lb, lm, lv;   // layer bounds, masses, potentials
+
+    int layer_of(double u) const
+    {
+        int i = 0;
+        while (i + 1 < (int)lm.size() && u >= lb[i + 1]) ++i;
+        return i;
+    }
+    double inv_mass(int d, double u) const
+    {
+        return (layered && d == lax) ? 1.0 / lm[layer_of(u)] : 1.0 / prm.mass;
+    }
 
     double pot(double x, double y, double z) const
     {
@@ -179,33 +206,44 @@ void setup_axis(Problem &P, int d)
         A.E = 1; A.p = 0; A.Q = 1; A.n = 1; A.active = false;
         A.X = {0.0}; A.W = {1.0}; A.N0 = {1.0}; A.N1 = {0.0};
         A.M = {1.0}; A.Xb = {0.0}; A.G = {0.0}; A.A = {0.0}; A.S = {0.0}; A.Wc = {0.0}; A.Vs = {0.0};
-        A.capq = {0.0}; A.isWall = {0};
+        A.capq = {0.0}; A.isWall = {0}; A.off = {0};
         return;
     }
     A.active = true;
-    A.E = r.nelem[d]; A.p = r.p; A.Q = r.nquad; A.lo = r.lo[d]; A.hi = r.hi[d];
-    A.n = A.E + A.p;
+    A.p = r.p; A.Q = r.nquad;
     A.wall = r.dirichlet != 0;
-    const int p = A.p, E = A.E, Q = A.Q, nen = p + 1, bw = 2 * p + 1;
-    // open uniform knots, continuity p-1 (IGAAxisInitUniform(..., p-1))
-    A.t.assign(E + 2 * p + 1, 0.0);
-    for (int i = 0; i <= p; ++i) { A.t[i] = A.lo; A.t[E + p + i] = A.hi; }
-    for (int i = 1; i < E; ++i) A.t[p + i] = A.lo + (A.hi - A.lo) * i / E;
+    const int p = A.p, Q = A.Q, nen = p + 1, bw = 2 * p + 1;
+    if (d == P.dim - 1 && r.knots && r.nknots > 2 * p + 1) {
+        A.t.assign(r.knots, r.knots + r.nknots);
+    } else {   // open uniform knots, continuity p-1 (IGAAxisInitUniform(..., p-1))
+        const int E0 = r.nelem[d];
+        A.t.assign(E0 + 2 * p + 1, 0.0);
+        for (int i = 0; i <= p; ++i) { A.t[i] = r.lo[d]; A.t[E0 + p + i] = r.hi[d]; }
+        for (int i = 1; i < E0; ++i) A.t[p + i] = r.lo[d] + (r.hi[d] - r.lo[d]) * i / E0;
+    }
+    A.lo = A.t.front(); A.hi = A.t.back();
+    A.n = (int)A.t.size() - p - 1;
+    A.off.clear();
+    for (int sp = p; sp < A.n; ++sp)
+        if (A.t[sp + 1] > A.t[sp]) A.off.push_back(sp - p);   // non-empty span sp: functions sp-p .. sp
+    A.E = (int)A.off.size();
+    const int E = A.E;
     std::vector<double> gx(Q), gw(Q);
     gauss_legendre(Q, gx.data(), gw.data());
     A.X.resize((size_t)E * Q); A.W.resize((size_t)E * Q);
     A.N0.assign((size_t)E * nen * Q, 0.0); A.N1 = A.N0;
     A.capq.assign((size_t)E * Q, 0.0);
     for (int e = 0; e < E; ++e) {
-        const double u0 = A.t[p + e], u1 = A.t[p + e + 1], J = 0.5 * (u1 - u0);
+        const int sp = A.off[e] + p;
+        const double u0 = A.t[sp], u1 = A.t[sp + 1], J = 0.5 * (u1 - u0);
         for (int q = 0; q < Q; ++q) {
             const double x = 0.5 * (u0 + u1) + J * gx[q];
             A.X[e * Q + q] = x;
             A.W[e * Q + q] = gw[q] * J;
             A.capq[e * Q + q] = P.cap_profile(A, x);
             for (int a = 0; a < nen; ++a) {
-                A.N0[((size_t)e * nen + a) * Q + q] = bspline(A.t, e + a, p, x, p + e);
-                A.N1[((size_t)e * nen + a) * Q + q] = bspline_d1(A.t, e + a, p, x, p + e);
+                A.N0[((size_t)e * nen + a) * Q + q] = bspline(A.t, A.off[e] + a, p, x, sp);
+                A.N1[((size_t)e * nen + a) * Q + q] = bspline_d1(A.t, A.off[e] + a, p, x, sp);
             }
         }
     }
@@ -215,6 +253,8 @@ void setup_axis(Problem &P, int d)
     const size_t sz = (size_t)A.n * bw;
     A.M.assign(sz, 0); A.Xb.assign(sz, 0); A.G.assign(sz, 0); A.S.assign(sz, 0); A.Wc.assign(sz, 0);
     A.Vs.assign(sz, 0);
+    const bool lay = P.layered && d == P.lax;
+    if (lay) { A.Mw.assign(sz, 0); A.Sw.assign(sz, 0); A.Vl.assign(sz, 0); }
     // separable part of s V per axis: the harmonic potential is a sum of 1D terms
     const double vsc = r.pot_kind == 1 ? P.s * 0.5 * r.pot[0] : 0.0;
     for (int e = 0; e < E; ++e)
@@ -222,7 +262,7 @@ void setup_axis(Problem &P, int d)
             const double w = A.W[e * Q + q], x = A.X[e * Q + q], cq = A.capq[e * Q + q];
             for (int a = 0; a < nen; ++a)
                 for (int b = 0; b < nen; ++b) {
-                    const size_t k = (size_t)(e + a) * bw + (b - a + p);
+                    const size_t k = (size_t)(A.off[e] + a) * bw + (b - a + p);
                     const double Na = A.N0[((size_t)e * nen + a) * Q + q], Nb = A.N0[((size_t)e * nen + b) * Q + q];
                     const double Da = A.N1[((size_t)e * nen + a) * Q + q], Db = A.N1[((size_t)e * nen + b) * Q + q];
                     A.M[k] += w * Na * Nb;
@@ -231,6 +271,12 @@ void setup_axis(Problem &P, int d)
                     A.S[k] += w * Da * Db;
                     A.Wc[k] += w * cq * Na * Nb;
                     A.Vs[k] += w * vsc * x * x * Na * Nb;
+                    if (lay) {
+                        const int l = P.layer_of(x);
+                        A.Mw[k] += w / P.lm[l] * Na * Nb;
+                        A.Sw[k] += w / P.lm[l] * Da * Db;
+                        A.Vl[k] += w * P.lv[l] * Na * Nb;
+                    }
                 }
         }
     A.A.assign(sz, 0);
@@ -325,7 +371,7 @@ void pot_apply(const Problem &P, cplx c, bool acc, const cplx *x, cplx *y)
                         // stage x: t1[a3][a2][q1]
                         for (int a3 = 0; a3 < n3; ++a3)
                             for (int a2 = 0; a2 < n2; ++a2) {
-                                const cplx *cr = x + (e1 + (size_t)g0 * ((e2 + a2) + (size_t)g1 * (e3 + a3)));
+                                const cplx *cr = x + (X.off[e1] + (size_t)g0 * ((Y.off[e2] + a2) + (size_t)g1 * (Z.off[e3] + a3)));
                                 cplx *tr = t1.data() + ((size_t)a3 * n2 + a2) * Q1;
                                 for (int q1 = 0; q1 < Q1; ++q1) tr[q1] = 0;
                                 for (int a1 = 0; a1 < n1; ++a1) {
@@ -383,7 +429,7 @@ void pot_apply(const Problem &P, cplx c, bool acc, const cplx *x, cplx *y)
                             }
                         for (int a3 = 0; a3 < n3; ++a3)
                             for (int a2 = 0; a2 < n2; ++a2) {
-                                cplx *yr = y + (e1 + (size_t)g0 * ((e2 + a2) + (size_t)g1 * (e3 + a3)));
+                                cplx *yr = y + (X.off[e1] + (size_t)g0 * ((Y.off[e2] + a2) + (size_t)g1 * (Z.off[e3] + a3)));
                                 const cplx *tr = t1.data() + ((size_t)a3 * n2 + a2) * Q1;
                                 for (int a1 = 0; a1 < n1; ++a1) {
                                     cplx s = 0;
@@ -409,13 +455,26 @@ void term_apply(const Problem &P, int term, cplx c, bool acc, const cplx *x, cpl
         for (int a = 0; a < P.dim; ++a) {
             const std::vector<double> *U[3] = {Mb[0], Mb[1], Mb[2]};
             U[a] = &P.ax[a].S;
-            kron_term(P, U, c * (0.5 / P.prm.mass), !first, x, y);
+            double f = 0.5 / P.prm.mass;
+            if (P.layered) {   // 1/m(z) rides on the layered axis: Mw there, Sw for its own term
+                f = 0.5;
+                if (a == P.lax) U[a] = &P.ax[a].Sw;
+                else U[P.lax] = &P.ax[P.lax].Mw;
+            }
+            kron_term(P, U, c * f, !first, x, y);
             first = false;
         }
         if (first) std::fill(y, y + P.N, cplx(0));
         break;
     }
-    case tV: pot_apply(P, c, acc, x, y); break;
+    case tV:
+        pot_apply(P, c, acc, x, y);
+        if (P.layered) {   // s V_l(z): separable, on the layered axis
+            const std::vector<double> *U[3] = {Mb[0], Mb[1], Mb[2]};
+            U[P.lax] = &P.ax[P.lax].Vl;
+            kron_term(P, U, c * P.s, true, x, y);
+        }
+        break;
     case tCAP: {  // CAP = -i sum_d W_d (x) prod M_e (src/assembly/cap.cpp:126-131)
         bool first = !acc;
         for (int a = 0; a < P.dim; ++a) {
@@ -781,7 +840,17 @@ void *orc_create(const orc_params *prm)
     P->prm = *prm;
     P->dim = prm->dim;
     P->s = -prm->charge;
+    P->lax = prm->dim - 1;
+    if (prm->nlayers > 0 && prm->layer_bounds && prm->layer_mass && prm->layer_v) {
+        P->layered = true;
+        P->lb.assign(prm->layer_bounds, prm->layer_bounds + prm->nlayers + 1);
+        P->lm.assign(prm->layer_mass, prm->layer_mass + prm->nlayers);
+        P->lv.assign(prm->layer_v, prm->layer_v + prm->nlayers);
+    }
+
     for (int d = 0; d < 3; ++d) setup_axis(*P, d);
+    P->prm.knots = nullptr;   // copied into the axis; the caller's arrays are not kept
+    P->prm.layer_bounds = P->prm.layer_mass = P->prm.layer_v = nullptr;
     P->N = (size_t)P->ax[0].n * P->ax[1].n * P->ax[2].n;
     return P;
 }
@@ -883,17 +952,23 @@ int orc_dense(void *h, int term, double *out)
                                 for (int kb = 0; kb < na; ++kb) {
                                     const int a1 = ka % n1, a2 = (ka / n1) % n2, a3 = ka / (n1 * n2);
                                     const int b1 = kb % n1, b2 = (kb / n1) % n2, b3 = kb / (n1 * n2);
-                                    const size_t I = idx3(P, e1 + a1, e2 + a2, e3 + a3), J = idx3(P, e1 + b1, e2 + b2, e3 + b3);
+                                    const int o1 = P.ax[0].off[e1], o2 = P.ax[1].off[e2], o3 = P.ax[2].off[e3];
+                                    const size_t I = idx3(P, o1 + a1, o2 + a2, o3 + a3), J = idx3(P, o1 + b1, o2 + b2, o3 + b3);
                                     cplx v = 0;
                                     switch (term) {
                                     case tM: v = w * N0[ka] * N0[kb]; break;
                                     case tK: {
                                         double s = 0;
                                         for (int d = 0; d < P.dim; ++d) s += D[d][ka] * D[d][kb];
-                                        v = 0.5 / P.prm.mass * w * s;
+                                        v = 0.5 * P.inv_mass(P.lax, r[P.lax]) * w * s;
                                         break;
                                     }
-                                    case tV: v = P.s * P.pot(r[0], r[1], r[2]) * w * N0[ka] * N0[kb]; break;
+                                    case tV: {
+                                        double vv = P.pot(r[0], r[1], r[2]);
+                                        if (P.layered) vv += P.lv[P.layer_of(r[P.lax])];
+                                        v = P.s * vv * w * N0[ka] * N0[kb];
+                                        break;
+                                    }
                                     case tCAP: v = cplx(0, -1) * cap * w * N0[ka] * N0[kb]; break;
                                     default: v = r[term - tDx] * w * N0[ka] * N0[kb]; break;
                                     }
@@ -1013,7 +1088,7 @@ int orc_project_gaussian(void *h, const double *centre, const double *width, con
                 const double x = A.X[e * A.Q + q], w = A.W[e * A.Q + q];
                 const double s = width[d], u = x - centre[d];
                 const cplx v = std::pow(2.0 * M_PI * s * s, -0.25) * std::exp(cplx(-u * u / (4.0 * s * s), momentum[d] * x));
-                for (int a = 0; a <= A.p; ++a) bd[d][e + a] += w * A.N0[((size_t)e * (A.p + 1) + a) * A.Q + q] * v;
+                for (int a = 0; a <= A.p; ++a) bd[d][A.off[e] + a] += w * A.N0[((size_t)e * (A.p + 1) + a) * A.Q + q] * v;
             }
     }
     for (int k = 0; k < Z.n; ++k)

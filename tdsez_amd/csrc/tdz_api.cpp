@@ -89,6 +89,7 @@ struct Level {
     // host eigen-decomposition of the separable preconditioner
     std::vector<double> hV[3], hLam[3];
     bool fd_ready = false;
+    bool stale = false;     // coarse level whose bands wait for the next V-cycle
     Prec<double> d;
     Prec<float> f;
     // transfer to the next coarser level: P (this x coarse) and R = P^T
@@ -366,9 +367,35 @@ void update_bands(tdz_ctx *c, Level &L)
     HIPCHK(hipStreamSynchronize(c->st));
 }
 
+// Coarse levels are only read by the V-cycle: while the preconditioner does not use
+// them their bands are left stale and brought up to date by refresh_coarse.
+bool uses_coarse(const tdz_ctx *c)
+{
+    return c->pc_kind == TDZ_PC_MG || c->pc_kind == TDZ_PC_MG_MIXED ||
+           (c->pc_kind == TDZ_PC_LEARNED && c->learn.fallen);
+}
+
 void set_shift_bands(tdz_ctx *c)
 {
-    for (auto &L : c->lv) update_bands(c, *L);
+    const bool coarse = uses_coarse(c);
+    for (size_t l = 0; l < c->lv.size(); ++l) {
+        Level &L = *c->lv[l];
+        if (l > 0 && !coarse) {
+            L.stale = true;
+            continue;
+        }
+        update_bands(c, L);
+        L.stale = false;
+    }
+}
+
+void refresh_coarse(tdz_ctx *c)
+{
+    for (size_t l = 1; l < c->lv.size(); ++l)
+        if (c->lv[l]->stale) {
+            update_bands(c, *c->lv[l]);
+            c->lv[l]->stale = false;
+        }
 }
 
 template <typename R>
@@ -717,7 +744,10 @@ void pc_apply_dev(tdz_ctx *c, const dcomplex *r, dcomplex *z)
     Level &L = *c->lv[0];
     switch (c->pc_kind) {
     case TDZ_PC_SEPARABLE: fd_apply<double>(c, L, r, z, c->sd.w1, c->sd.w2); break;
-    case TDZ_PC_MG: vcycle<double>(c, 0, r, z); break;
+    case TDZ_PC_MG:
+        refresh_coarse(c);
+        vcycle<double>(c, 0, r, z);
+        break;
     case TDZ_PC_SEPARABLE_MIXED:
         tdz::launch_demote(c->st, r, L.f.r, L.N);
         fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);
@@ -727,7 +757,10 @@ void pc_apply_dev(tdz_ctx *c, const dcomplex *r, dcomplex *z)
     case TDZ_PC_CORRECTED_EXACT: corrected_apply(c, r, z, nullptr, true); break;
     case TDZ_PC_LEARNED:
         if (c->learn.fallen) {
-            if (c->learn.fallback_kind == TDZ_PC_MG) vcycle<double>(c, 0, r, z);
+            if (c->learn.fallback_kind == TDZ_PC_MG) {
+                refresh_coarse(c);
+                vcycle<double>(c, 0, r, z);
+            }
             else fd_apply<double>(c, L, r, z, c->sd.w1, c->sd.w2);
         } else {
             corrected_apply(c, r, z, nullptr, false, true);
@@ -735,6 +768,7 @@ void pc_apply_dev(tdz_ctx *c, const dcomplex *r, dcomplex *z)
         break;
     case TDZ_PC_MG_MIXED:
         tdz::launch_demote(c->st, r, L.f.r, L.N);
+        refresh_coarse(c);
         vcycle<float>(c, 0, L.f.r, L.f.z);
         tdz::launch_promote(c->st, L.f.z, z, L.N);
         break;
@@ -1856,7 +1890,7 @@ extern "C" {
 int tdz_observables(tdz_ctx *c, const double *psi, double out[6])
 {
     return guarded([&] {
-        cplx v[6];
+        cplx v[8];   // expectations() fills 8 values
         expectations(c, psi, nullptr, v);
         out[0] = v[0].real();
         out[1] = v[1].real();
