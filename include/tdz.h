@@ -64,6 +64,19 @@ typedef struct {
     int mg_levels;       /* MGLevels: 0 automatic, else the number of levels */
     int mg_smooth;       /* MGSmooth: smoothing sweeps per level (default 1) */
     int restart;         /* Krylov restart; 0 = reference rule (30, 50 above 1e5 rows) */
+    /* z axis of a layered structure (dim 3). knots: an open knot vector for z (nknots
+     * values; interior knots repeated p times give C0 interfaces), or null for uniform.
+     * nlayers > 0: piecewise-constant layers [layer_bounds[i], layer_bounds[i+1]) of
+     * effective mass layer_mass[i] and potential offset layer_v[i] (enters as s V):
+     * BenDaniel-Duke kinetic term 1/2 grad.(1/m(z)) grad. With layers the CAP, fields
+     * along x or y and the harmonic potential are not supported. The arrays are read
+     * by tdz_create only. */
+    const double *knots;
+    int nknots;
+    int nlayers;
+    const double *layer_bounds;
+    const double *layer_mass;
+    const double *layer_v;
 } tdz_problem;
 
 typedef struct {

@@ -888,8 +888,9 @@ void orc_axis_tables(void *h, int d, double *pts, double *wts, double *N0, doubl
 // which: 0 M, 1 X, 2 G, 3 A = G - G^T, 4 S, 5 W (CAP profile band)
 void orc_band(void *h, int d, int which, double *out)
 {
+    // which: 0 M, 1 X, 2 G, 3 A, 4 S, 5 W; on the layered axis 6 Mw, 7 Sw, 8 V_l
     const Axis &A = static_cast<Problem *>(h)->ax[d];
-    const std::vector<double> *b[6] = {&A.M, &A.Xb, &A.G, &A.A, &A.S, &A.Wc};
+    const std::vector<double> *b[9] = {&A.M, &A.Xb, &A.G, &A.A, &A.S, &A.Wc, &A.Mw, &A.Sw, &A.Vl};
     std::copy(b[which]->begin(), b[which]->end(), out);
 }
 

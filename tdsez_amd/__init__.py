@@ -45,6 +45,9 @@ class Problem(C.Structure):
         ("pot_kind", C.c_int), ("pot", C.c_double * 4),
         ("cap_kind", C.c_int), ("cap", C.c_double * 4),
         ("mg_levels", C.c_int), ("mg_smooth", C.c_int), ("restart", C.c_int),
+        ("knots", C.POINTER(C.c_double)), ("nknots", C.c_int), ("nlayers", C.c_int),
+        ("layer_bounds", C.POINTER(C.c_double)), ("layer_mass", C.POINTER(C.c_double)),
+        ("layer_v", C.POINTER(C.c_double)),
     ]
 
 
@@ -78,6 +81,21 @@ def make_problem(desc):
     p.mg_levels = desc.get("mg_levels", 0)
     p.mg_smooth = desc.get("mg_smooth", 1)
     p.restart = desc.get("restart", 0)
+    # layered z axis: desc["knots"] (knot vector), desc["layers"] = (bounds, masses, potentials)
+    keep = []
+    if desc.get("knots") is not None:
+        k = np.ascontiguousarray(desc["knots"], np.float64)
+        keep.append(k)
+        p.knots = k.ctypes.data_as(C.POINTER(C.c_double))
+        p.nknots = k.size
+    if desc.get("layers") is not None:
+        b, m, v = (np.ascontiguousarray(a, np.float64) for a in desc["layers"])
+        keep += [b, m, v]
+        p.nlayers = m.size
+        p.layer_bounds = b.ctypes.data_as(C.POINTER(C.c_double))
+        p.layer_mass = m.ctypes.data_as(C.POINTER(C.c_double))
+        p.layer_v = v.ctypes.data_as(C.POINTER(C.c_double))
+    p._keep = keep   # the arrays must outlive tdz_create
     return p
 
 
@@ -400,7 +418,8 @@ class Batch:
 
     def __init__(self, descs, pulses, dt, nsteps, pc=PC_SEPARABLE_MIXED, kiters=4):
         n = len(descs)
-        probs = (Problem * n)(*[make_problem(dd) for dd in descs])
+        made = [make_problem(dd) for dd in descs]   # keeps any knot/layer arrays alive
+        probs = (Problem * n)(*made)
         pls = (Pulse * n)(*pulses) if pulses is not None else None
         h = C.c_void_p()
         _chk(lib().tdz_batch_create(probs, pls, n, dt, nsteps, pc, kiters, C.byref(h)))

@@ -74,6 +74,8 @@ struct Prec {
     cpx<R> *B[3] = {};                       // complex combined bands (current shift)
     cpx<R> *Bmass[3] = {};                   // bands of M alone (M on x, zero on y and z)
     R *V[3] = {}, *lam[3] = {};              // separable preconditioner
+    R *Mw = nullptr;                         // layered z axis: 1/m-weighted mass band
+    cpx<R> *lwork = nullptr;                 // layered z axis: work of the z-line solves
     cpx<R> *r = nullptr, *z = nullptr, *t = nullptr;   // multigrid work vectors
     bool ready = false;
 };
@@ -86,6 +88,9 @@ struct Level {
     unsigned char *dWall[3] = {};
     // host bands of this level (Galerkin on coarse levels)
     std::vector<double> M[3], S[3], Xb[3], Wc[3], Vs[3];
+    // layered z axis: bands of 1/m, (1/m) B'B' and the layer potential (Galerkin on coarse levels)
+    bool layered = false;
+    std::vector<double> Mw, Sw, Vl;
     // host eigen-decomposition of the separable preconditioner
     std::vector<double> hV[3], hLam[3];
     bool fd_ready = false;
@@ -161,6 +166,9 @@ struct tdz_ctx {
     Scratch<float> sf;
     bool mixed_ready = false;   // single-precision copies allocated
     dcomplex *rhs = nullptr;
+    // layered structure along z: bounds, effective masses, potential offsets
+    bool layered = false;
+    std::vector<double> lay_b, lay_m, lay_v;
     // captured CN step (batched scan): device-side FGMRES state, this run's field per
     // step, step counter, residual estimate per step, the field-dependent bands
     struct Captured {
@@ -272,10 +280,12 @@ CSR make_csr(tdz_ctx *c, const std::vector<std::vector<std::pair<int, double>>> 
     return s;
 }
 
+// M alone as one Kronecker term: M on x (zero y and z bands), or on z for a layered
+// structure, whose x and y terms carry the 1/m-weighted z band
 std::vector<dcomplex> mass_only_band(const Level &L, int d)
 {
     std::vector<dcomplex> bm(L.M[d].size(), dcomplex{0.0, 0.0});
-    if (d == 0)
+    if (d == (L.layered ? 2 : 0))
         for (size_t k = 0; k < bm.size(); ++k) bm[k] = {L.M[d][k], 0.0};
     return bm;
 }
@@ -294,6 +304,7 @@ void upload_level_tables(tdz_ctx *c, Level &L)
         L.d.B[d] = track(c, dalloc<dcomplex>(L.M[d].size()));
         L.d.Bmass[d] = upload(c, mass_only_band(L, d));
     }
+    if (L.layered) L.d.Mw = upload(c, L.Mw);
     L.d.t = track(c, dalloc<dcomplex>(L.N));
     if (&L != c->lv[0].get()) {
         L.d.r = track(c, dalloc<dcomplex>(L.N));
@@ -324,6 +335,10 @@ void upload_level_tables_f(tdz_ctx *c, Level &L)
             L.f.lam[d] = upload(c, to_float(L.hLam[d]));
         }
     }
+    if (L.layered) {
+        L.f.Mw = upload(c, to_float(L.Mw));
+        if (L.fd_ready) L.f.lwork = track(c, dalloc<fcomplex>((size_t)L.N * (2 * L.ax[2].p + 1)));
+    }
     L.f.t = track(c, dalloc<fcomplex>(L.N));
     L.f.r = track(c, dalloc<fcomplex>(L.N));
     L.f.z = track(c, dalloc<fcomplex>(L.N));
@@ -340,10 +355,18 @@ void update_bands(tdz_ctx *c, Level &L)
         std::vector<dcomplex> B(sz);
         const bool active = d < c->prob.dim;
         for (size_t k = 0; k < sz; ++k) {
-            cplx T = 0.0;
-            if (active) T = inv2m * L.S[d][k] + (c->cap_on ? cplx(0, -1) * L.Wc[d][k] : 0.0) + c->sE[d] * L.Xb[d][k];
-            cplx b = c->g * T;
-            if (d == 0) b += c->alpha * L.M[d][k];
+            cplx b;
+            if (L.layered) {
+                // x, y: g S/2 (their z factor is the 1/m band, Mz2); z: alpha M + g (Sw/2 + s V_l + sE_z X)
+                if (d < 2) b = c->g * (0.5 * L.S[d][k]);
+                else b = c->alpha * L.M[2][k] + c->g * (0.5 * L.Sw[k] + c->ph.s * L.Vl[k] + c->sE[2] * L.Xb[2][k]);
+            } else {
+                cplx T = 0.0;
+                if (active)
+                    T = inv2m * L.S[d][k] + (c->cap_on ? cplx(0, -1) * L.Wc[d][k] : 0.0) + c->sE[d] * L.Xb[d][k];
+                b = c->g * T;
+                if (d == 0) b += c->alpha * L.M[d][k];
+            }
             B[k] = dc(b);
         }
         HIPCHK(hipMemcpyAsync(L.d.B[d], B.data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
@@ -417,6 +440,7 @@ tdz::SepBandsT<R> sep_bands(Level &L)
         sb.M[d] = P.M[d];
         sb.B[d] = P.B[d];
     }
+    sb.Mz2 = L.layered ? P.Mw : nullptr;
     return sb;
 }
 
@@ -510,9 +534,13 @@ void setup_fd(tdz_ctx *c, Level &L)
     for (int d = 0; d < 3; ++d) {
         std::vector<double> V, lam;
         const tdz::Axis1D &A = L.ax[d];
-        if (d < c->prob.dim) {
+        if (L.layered && d == 2) {   // z: solved along lines, no eigenbasis
+            V = {1.0};
+            lam = {0.0};
+        } else if (d < c->prob.dim) {
             std::vector<double> T(L.S[d].size());
-            for (size_t k = 0; k < T.size(); ++k) T[k] = inv2m * L.S[d][k] + L.Vs[d][k];
+            for (size_t k = 0; k < T.size(); ++k)
+                T[k] = L.layered ? 0.5 * L.S[d][k] : inv2m * L.S[d][k] + L.Vs[d][k];
             tdz::Axis1D Ab = A;
             Ab.M = L.M[d];
             tdz::gen_eig_bands(Ab, T, V, lam);
@@ -529,15 +557,32 @@ void setup_fd(tdz_ctx *c, Level &L)
             L.f.lam[d] = upload(c, to_float(lam));
         }
     }
+    if (L.layered) {
+        L.d.lwork = track(c, dalloc<dcomplex>((size_t)L.N * (2 * L.ax[2].p + 1)));
+        if (L.f.ready) L.f.lwork = track(c, dalloc<fcomplex>((size_t)L.N * (2 * L.ax[2].p + 1)));
+    }
     L.fd_ready = true;
 }
 
 // z = (alpha M + g (K + V_sep))^-1 r by fast diagonalisation (walls: identity);
 // V_sep is the part of s V that is a sum of 1D terms. s1, s2: scratch.
+// Layered structure: x and y by their eigenbases, z by a banded solve per (x, y) line of
+// the whole separable operator there (the current z band, i.e. with the field along z,
+// plus g (lambda_x + lambda_y) times the 1/m band): the exact inverse without a
+// non-separable potential.
 template <typename R>
 void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<R> *s2)
 {
     Prec<R> &P = prec<R>(L);
+    if (L.layered) {
+        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1);
+        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2);
+        tdz::launch_zline_solve<R>(c->st, L.n, L.ax[2].p, P.lam[0], P.lam[1], P.B[2], P.Mw, cx<R>(c->g), s2, s1,
+                                   P.lwork);
+        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, s1, s2);
+        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, s2, z);
+        return;
+    }
     tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1);
     tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2);
     tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], true, s2, s1);
@@ -937,6 +982,8 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
 
 void set_shift(tdz_ctx *c, cplx alpha, cplx g, const double sE[3])
 {
+    if (c->layered && sE && (sE[0] != 0.0 || sE[1] != 0.0))
+        throw Error("a layered structure takes fields along z only");
     c->alpha = alpha;
     c->g = g;
     for (int a = 0; a < 3; ++a) c->sE[a] = sE ? sE[a] : 0.0;
@@ -948,6 +995,8 @@ void pc_setup(tdz_ctx *c, int kind)
     if (kind < TDZ_PC_NONE || kind > TDZ_PC_LEARNED) throw Error("unknown preconditioner kind");
     const bool mg = kind == TDZ_PC_MG || kind == TDZ_PC_MG_MIXED;
     if (mg && c->lv.size() < 2) throw Error("multigrid needs at least two levels");
+    if (c->layered && (kind == TDZ_PC_CORRECTED || kind == TDZ_PC_CORRECTED_EXACT || kind == TDZ_PC_LEARNED))
+        throw Error("a layered structure is separable: its separable solve is exact (no correction to learn)");
     c->pc_kind = kind;
     if (mg)
         for (auto &L : c->lv) setup_fd(c, *L);
@@ -973,6 +1022,35 @@ double pulse_field(const tdz_pulse &pl, int a, double t)
     return pl.E0[a] * s * s * std::sin(pl.omega * t + pl.cep);
 }
 
+// Bands of the layered z axis from its tables: Mw = int (1/m) B_i B_j, Sw = int (1/m)
+// B_i' B_j', Vl = int V_l B_i B_j, with the layer of each Gauss point (the BenDaniel-Duke
+// kinetic term 1/2 grad.(1/m) grad and the band offsets; H semantics of
+// src/assembly/sumfact.cpp:496-502 with cK = w/(2m)).
+void layer_bands(const tdz_ctx *c, const tdz::Axis1D &A, std::vector<double> &Mw, std::vector<double> &Sw,
+                 std::vector<double> &Vl)
+{
+    const int p = A.p, nen = p + 1, bw = 2 * p + 1, Q = A.Q;
+    Mw.assign((size_t)A.n * bw, 0.0);
+    Sw = Mw;
+    Vl = Mw;
+    for (int e = 0; e < A.E; ++e)
+        for (int q = 0; q < Q; ++q) {
+            const double x = A.X[(size_t)e * Q + q], w = A.W[(size_t)e * Q + q];
+            int l = 0;
+            while (l + 1 < (int)c->lay_m.size() && x >= c->lay_b[l + 1]) ++l;
+            const double im = 1.0 / c->lay_m[l], v = c->lay_v[l];
+            for (int a = 0; a < nen; ++a)
+                for (int b = 0; b < nen; ++b) {
+                    const size_t k = (size_t)(A.off[e] + a) * bw + (b - a + p);
+                    const double Na = A.B0[((size_t)e * nen + a) * Q + q], Nb = A.B0[((size_t)e * nen + b) * Q + q];
+                    const double Da = A.B1[((size_t)e * nen + a) * Q + q], Db = A.B1[((size_t)e * nen + b) * Q + q];
+                    Mw[k] += w * im * Na * Nb;
+                    Sw[k] += w * im * Da * Db;
+                    Vl[k] += w * v * Na * Nb;
+                }
+        }
+}
+
 void build(tdz_ctx *c, const tdz_problem &pr)
 {
     if (pr.dim < 1 || pr.dim > 3) throw Error("dim must be 1, 2 or 3");
@@ -995,6 +1073,36 @@ void build(tdz_ctx *c, const tdz_problem &pr)
     std::vector<std::array<std::vector<double>, 3>> knots;
     std::array<std::vector<double>, 3> k0;
     for (int d = 0; d < pr.dim; ++d) k0[d] = tdz::uniform_knots(pr.nelem[d], pr.lo[d], pr.hi[d], pr.p);
+    if (pr.knots && pr.nknots > 0) {   // the z axis of a layered structure (last active axis)
+        std::vector<double> t(pr.knots, pr.knots + pr.nknots);
+        const int p = pr.p;
+        if ((int)t.size() < 2 * p + 2) throw Error("knot vector too short for the degree");
+        for (size_t i = 1; i < t.size(); ++i)
+            if (t[i] < t[i - 1]) throw Error("knot vector must be nondecreasing");
+        for (int i = 1; i <= p; ++i)
+            if (t[i] != t[0] || t[t.size() - 1 - i] != t.back()) throw Error("knot vector must be open");
+        for (size_t i = p + 1; i + p + 1 < t.size(); ++i) {
+            size_t m = 1;
+            while (i + m < t.size() - p - 1 && t[i + m] == t[i]) ++m;
+            if ((int)m > p) throw Error("interior knot multiplicity above p (discontinuous basis)");
+        }
+        k0[pr.dim - 1] = t;
+    }
+    if (pr.nlayers > 0) {
+        if (pr.dim != 3) throw Error("layers need dim = 3 (the layered axis is z)");
+        if (!pr.layer_bounds || !pr.layer_mass || !pr.layer_v) throw Error("layer arrays missing");
+        if (pr.cap_kind) throw Error("a CAP with layers is not supported");
+        if (pr.pot_kind == 1) throw Error("the harmonic potential with layers is not supported");
+        if (pr.dirichlet) throw Error("Dirichlet walls with layers are not supported");
+        c->layered = true;
+        c->lay_b.assign(pr.layer_bounds, pr.layer_bounds + pr.nlayers + 1);
+        c->lay_m.assign(pr.layer_mass, pr.layer_mass + pr.nlayers);
+        c->lay_v.assign(pr.layer_v, pr.layer_v + pr.nlayers);
+        for (int i = 0; i < pr.nlayers; ++i) {
+            if (!(c->lay_m[i] > 0)) throw Error("layer masses must be positive");
+            if (!(c->lay_b[i + 1] > c->lay_b[i])) throw Error("layer bounds must increase");
+        }
+    }
     knots.push_back(k0);
     const int maxLevels = pr.mg_levels > 0 ? pr.mg_levels : 20;
     while ((int)knots.size() < maxLevels) {
@@ -1024,6 +1132,8 @@ void build(tdz_ctx *c, const tdz_problem &pr)
             L->Vs[d] = L->ax[d].Vs;
         }
         L->N = (long)L->n[0] * L->n[1] * L->n[2];
+        L->layered = c->layered;
+        if (c->layered && l == 0) layer_bands(c, L->ax[2], L->Mw, L->Sw, L->Vl);
         bool uniform = true;
         for (int d = 0; d < pr.dim; ++d)
             for (int e = 0; e < L->ax[d].E; ++e) uniform = uniform && L->ax[d].off[e] == e;
@@ -1066,6 +1176,11 @@ void build(tdz_ctx *c, const tdz_problem &pr)
                 Cl.Xb[d] = galerkin_band(F.Xb[d], nf, pr.p, P, nc);
                 Cl.Wc[d] = galerkin_band(F.Wc[d], nf, pr.p, P, nc);
                 Cl.Vs[d] = galerkin_band(F.Vs[d], nf, pr.p, P, nc);
+                if (c->layered && d == 2) {
+                    Cl.Mw = galerkin_band(F.Mw, nf, pr.p, P, nc);
+                    Cl.Sw = galerkin_band(F.Sw, nf, pr.p, P, nc);
+                    Cl.Vl = galerkin_band(F.Vl, nf, pr.p, P, nc);
+                }
                 if (pr.dirichlet)
                     for (int i = 0; i < nc; ++i)
                         if (Cl.ax[d].isWall[i]) {
@@ -1100,6 +1215,8 @@ void build(tdz_ctx *c, const tdz_problem &pr)
     c->dh2 = track(c, dalloc<dcomplex>(c->restart + 1));
     c->dpartr = track(c, dalloc<double>(tdz::kRedBlocks));
     c->dnorm = track(c, dalloc<double>(1));
+    c->prob.knots = nullptr;   // the caller's arrays are not kept
+    c->prob.layer_bounds = c->prob.layer_mass = c->prob.layer_v = nullptr;
     const double zero[3] = {0, 0, 0};
     set_shift(c, 1.0, 0.0, zero);
 }
@@ -1228,19 +1345,28 @@ int tdz_term_apply(tdz_ctx *c, int term, const double *x, double *y)
             const bool act = d < c->prob.dim;
             for (size_t k = 0; k < sz; ++k) {
                 cplx b = 0;
-                if (term == TDZ_TERM_M && d == 0) b = L.M[d][k];
-                if (term == TDZ_TERM_K && act) b = inv2m * L.S[d][k];
-                if (term == TDZ_TERM_CAP && act) b = cplx(0, -1) * L.Wc[d][k];
+                if (L.layered) {   // x and y terms carry the 1/m z band (Mz2) except D_x, D_y
+                    if (term == TDZ_TERM_M && d == 2) b = L.M[2][k];
+                    if (term == TDZ_TERM_K) b = d < 2 ? 0.5 * L.S[d][k] : 0.5 * L.Sw[k];
+                    if (term == TDZ_TERM_V && d == 2) b = c->ph.s * L.Vl[k];
+                } else {
+                    if (term == TDZ_TERM_M && d == 0) b = L.M[d][k];
+                    if (term == TDZ_TERM_K && act) b = inv2m * L.S[d][k];
+                    if (term == TDZ_TERM_CAP && act) b = cplx(0, -1) * L.Wc[d][k];
+                }
                 if (term >= TDZ_TERM_DX && d == term - TDZ_TERM_DX && act) b = L.Xb[d][k];
                 B[d][k] = dc(b);
             }
             HIPCHK(hipMemcpyAsync(L.d.B[d], B[d].data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
         }
+        tdz::SepBandsT<double> sb = sep_bands<double>(L);
+        if (term >= TDZ_TERM_DX) sb.Mz2 = nullptr;
         if (term == TDZ_TERM_V) {
-            HIPCHK(hipMemsetAsync(yd, 0, L.N * sizeof(dcomplex), c->st));
+            if (L.layered) tdz::launch_sep_apply<double>(c->st, sb, xd, yd, c->sd.tmp);   // s V_l(z)
+            else HIPCHK(hipMemsetAsync(yd, 0, L.N * sizeof(dcomplex), c->st));
             if (c->ph.pot_kind) pot_apply(c, L, 1.0, xd, yd);
         } else {
-            tdz::launch_sep_apply<double>(c->st, sep_bands<double>(L), xd, yd, c->sd.tmp);
+            tdz::launch_sep_apply<double>(c->st, sb, xd, yd, c->sd.tmp);
         }
         if (c->prob.dirichlet) {
             const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};

@@ -104,6 +104,9 @@ struct SepBandsT {
     int n[3], p[3];
     const R *M[3];
     const cpx<R> *B[3];
+    // layered z axis: the z factor of the x and y terms (the 1/m(z)-weighted mass band);
+    // null: M[2]
+    const R *Mz2 = nullptr;
 };
 using SepBands = SepBandsT<double>;
 template <typename R>
@@ -154,6 +157,14 @@ void launch_diag_scale(hipStream_t st, const int n[3], const R *lam[3], const un
 template <typename R>
 void launch_csr_axis(hipStream_t st, int d, const int nin[3], int m, const int *rowptr, const int *col,
                      const R *val, const cpx<R> *in, cpx<R> *out, bool accumulate);
+
+// x = A_ij^-1 rhs along z for every (i, j) line, A_ij = Bz + g (lx[i] + ly[j]) Mw (banded,
+// half-width p; banded elimination without pivoting, one thread per line; work holds
+// n0 n1 n2 (2p+1) values). The separable solve of a layered structure in the x/y
+// eigenbasis.
+template <typename R>
+void launch_zline_solve(hipStream_t st, const int n[3], int p, const R *lx, const R *ly, const cpx<R> *Bz,
+                        const R *Mw, cpx<R> g, const cpx<R> *rhs, cpx<R> *x, cpx<R> *work);
 
 // ---------------------------------------------------------------- device FGMRES(k)
 constexpr int kGmMax = 8;   // longest captured cycle

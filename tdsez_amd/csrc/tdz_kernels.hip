@@ -538,6 +538,7 @@ template <typename R>
 __global__ void sep_z_kernel(long plane, int n2, int p, const R *__restrict__ M, const cpx<R> *__restrict__ B,
                              const cpx<R> *__restrict__ a, const cpx<R> *__restrict__ b, cpx<R> *__restrict__ y)
 {
+    // M: the z factor of the x and y terms (the mass band, or the 1/m-weighted one)
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= plane * n2) return;
     const long m = I % plane;
@@ -1142,8 +1143,8 @@ void launch_sep_apply(hipStream_t st, const SepBandsT<R> &sb, const cpx<R> *x, c
                        x, tmp[0], tmp[1]);
     hipLaunchKernelGGL(sep_y_kernel<R>, dim3(grid_for(N, bs)), dim3(bs), 0, st, n0, n1, N, sb.p[1], sb.M[1], sb.B[1],
                        tmp[0], tmp[1], tmp[2], tmp[3]);
-    hipLaunchKernelGGL(sep_z_kernel<R>, dim3(grid_for(N, bs)), dim3(bs), 0, st, (long)n0 * n1, n2, sb.p[2], sb.M[2],
-                       sb.B[2], tmp[2], tmp[3], y);
+    hipLaunchKernelGGL(sep_z_kernel<R>, dim3(grid_for(N, bs)), dim3(bs), 0, st, (long)n0 * n1, n2, sb.p[2],
+                       sb.Mz2 ? sb.Mz2 : sb.M[2], sb.B[2], tmp[2], tmp[3], y);
 }
 
 template <typename R>
@@ -1373,6 +1374,79 @@ void launch_field_band(hipStream_t st, long n, const double *M, const dcomplex *
                        step, d, B);
 }
 
+// ------------------------------------------------------------ z-line solves
+namespace {
+template <typename R>
+__device__ __forceinline__ cpx<R> cmulR(cpx<R> a, cpx<R> b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> cdivR(cpx<R> a, cpx<R> b)
+{
+    const R d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+
+template <typename R>
+__global__ void zline_solve_kernel(int n0, int n1, int n2, int p, const R *__restrict__ lx, const R *__restrict__ ly,
+                                   const cpx<R> *__restrict__ Bz, const R *__restrict__ Mw, cpx<R> g,
+                                   const cpx<R> *__restrict__ rhs, cpx<R> *__restrict__ x, cpx<R> *__restrict__ work)
+{
+    const long nl = (long)n0 * n1;
+    const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nl) return;
+    const int i = (int)(line % n0), j = (int)(line / n0);
+    const R lam = lx[i] + ly[j];
+    const cpx<R> cg = {g.re * lam, g.im * lam};
+    const int bw = 2 * p + 1;
+    auto W = [&](int k, int m) -> cpx<R> & { return work[((long)k * bw + m) * nl + line]; };
+    for (int k = 0; k < n2; ++k) {
+        for (int m = 0; m < bw; ++m) {
+            const int col = k - p + m;
+            cpx<R> v = {0, 0};
+            if (col >= 0 && col < n2) {
+                const cpx<R> b = Bz[(long)k * bw + m];
+                const R w = Mw[(long)k * bw + m];
+                v = {b.re + cg.re * w, b.im + cg.im * w};
+            }
+            W(k, m) = v;
+        }
+        x[line + nl * k] = rhs[line + nl * k];
+    }
+    for (int k = 0; k < n2; ++k) {   // elimination below the diagonal (no fill-in without pivoting)
+        const cpx<R> piv = W(k, p);
+        const cpx<R> xk = x[line + nl * k];
+        const int rmax = k + p < n2 - 1 ? k + p : n2 - 1;
+        for (int r = k + 1; r <= rmax; ++r) {
+            const cpx<R> f = cdivR(W(r, k - r + p), piv);
+            for (int m2 = 1; m2 <= p && k + m2 < n2; ++m2) {
+                const cpx<R> t = cmulR(f, W(k, p + m2));
+                cpx<R> &d = W(r, k + m2 - r + p);
+                d = {d.re - t.re, d.im - t.im};
+            }
+            const cpx<R> t = cmulR(f, xk);
+            cpx<R> &xr = x[line + nl * r];
+            xr = {xr.re - t.re, xr.im - t.im};
+        }
+    }
+    for (int k = n2 - 1; k >= 0; --k) {   // back substitution
+        cpx<R> s = x[line + nl * k];
+        for (int m2 = 1; m2 <= p && k + m2 < n2; ++m2) {
+            const cpx<R> t = cmulR(W(k, p + m2), x[line + nl * (k + m2)]);
+            s = {s.re - t.re, s.im - t.im};
+        }
+        x[line + nl * k] = cdivR(s, W(k, p));
+    }
+}
+}  // namespace
+
+template <typename R>
+void launch_zline_solve(hipStream_t st, const int n[3], int p, const R *lx, const R *ly, const cpx<R> *Bz,
+                        const R *Mw, cpx<R> g, const cpx<R> *rhs, cpx<R> *x, cpx<R> *work)
+{
+    const long nl = (long)n[0] * n[1];
+    hipLaunchKernelGGL(zline_solve_kernel<R>, dim3((unsigned)((nl + 63) / 64)), dim3(64), 0, st, n[0], n[1], n[2], p,
+                       lx, ly, Bz, Mw, g, rhs, x, work);
+}
+
 #define TDZ_INSTANTIATE(R)                                                                                          \
     template void launch_pot_apply_generic<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,  \
                                               cpx<R> *);                                                           \
@@ -1390,7 +1464,9 @@ void launch_field_band(hipStream_t st, long n, const double *M, const dcomplex *
     template void launch_diag_scale<R>(hipStream_t, const int[3], const R *[3], const unsigned char *[3], cpx<R>,  \
                                        cpx<R>, const cpx<R> *, cpx<R> *);                                          \
     template void launch_csr_axis<R>(hipStream_t, int, const int[3], int, const int *, const int *, const R *,     \
-                                     const cpx<R> *, cpx<R> *, bool);
+                                     const cpx<R> *, cpx<R> *, bool);                                              \
+    template void launch_zline_solve<R>(hipStream_t, const int[3], int, const R *, const R *, const cpx<R> *,      \
+                                        const R *, cpx<R>, const cpx<R> *, cpx<R> *, cpx<R> *);
 
 TDZ_INSTANTIATE(double)
 TDZ_INSTANTIATE(float)
