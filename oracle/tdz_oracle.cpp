@@ -513,7 +513,7 @@ void op_apply(const Problem &P, cplx alpha, cplx g, const double sE[3], const cp
     const cplx *x = xm.data();
     term_apply(P, tM, alpha, false, x, y);
     term_apply(P, tK, g, true, x, y);
-    if (P.prm.pot_kind) term_apply(P, tV, g, true, x, y);
+    if (P.prm.pot_kind || P.layered) term_apply(P, tV, g, true, x, y);
     if (P.prm.cap_kind) term_apply(P, tCAP, g, true, x, y);
     for (int a = 0; a < P.dim; ++a)
         if (sE[a] != 0.0) term_apply(P, tDx + a, g * sE[a], true, x, y);

@@ -105,14 +105,16 @@ def test_layered_steps_match_oracle(magnus):
         for s in range(nsteps):
             t = s * dt
             E1, E2 = ctx.coupling(pulse, t + c1 * dt), ctx.coupling(pulse, t + c2 * dt)
-            ref, _ = o.pade22_step(ref, 0.5 * dt, 2 * (a2 * E1 + a1 * E2), rtol=1e-13, pc=0)
-            ref, _ = o.pade22_step(ref, 0.5 * dt, 2 * (a1 * E1 + a2 * E2), rtol=1e-13, pc=0)
+            ref, _ = o.pade22_step(ref, 0.5 * dt, 2 * (a2 * E1 + a1 * E2), rtol=1e-13, restart=100, maxit=5000)
+            ref, _ = o.pade22_step(ref, 0.5 * dt, 2 * (a1 * E1 + a2 * E2), rtol=1e-13, restart=100, maxit=5000)
         its, _ = ctx.evolve_rational(d, 0.0, dt, nsteps, pulse, magnus=True, rtol=1e-13)
     else:
         sE = np.array([ctx.coupling(pulse, (s + 0.5) * dt) for s in range(nsteps)])
-        ref, _, _ = o.cn_steps(psi0, dt, sE, rtol=1e-13, pc=0, restart=100, maxit=5000)
+        # the oracle's separable preconditioner assumes one mass: an inexact but convergent one here
+        ref, _, _ = o.cn_steps(psi0, dt, sE, rtol=1e-13, restart=100, maxit=5000)
         its, _ = ctx.evolve_cn(d, 0.0, dt, nsteps, pulse, rtol=1e-13)
     err = _l2(o, d.download() - ref) / _l2(o, ref)
     assert err < 1e-11, err
-    assert its.max() <= 2, its
+    # one FGMRES iteration per solve: CN has one solve per step, Magnus-4 four stage solves
+    assert its.max() <= (4 if magnus else 2), its
     ctx.close()

@@ -94,3 +94,15 @@ def test_equal_layers_are_a_constant_mass():
     ka, kb = a.term(1, x), b.term(1, x)
     assert np.abs(ka - kb).max() < 1e-13 * np.abs(kb).max()
     assert np.abs(a.term(2, x)).max() == 0.0
+
+
+def test_layered_operator_composition():
+    """alpha M + g (K + sV_l + sE_z D_z) assembled densely equals the oracle's apply."""
+    desc = layered()
+    o = Oracle(desc)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    alpha, g, sE = 0.7 + 0.1j, 0.3j, np.array([0.0, 0.0, 0.03])
+    A = alpha * o.dense(0) + g * (o.dense(1) + o.dense(2) + sE[2] * o.dense(6))
+    y = o.apply(alpha, g, sE, x)
+    assert np.abs(A @ x - y).max() < 1e-13 * np.abs(y).max()
