@@ -95,6 +95,9 @@ struct Level {
     std::vector<double> hV[3], hLam[3];
     bool fd_ready = false;
     bool stale = false;     // coarse level whose bands wait for the next V-cycle
+    // host staging of the band uploads (kept so that no copy waits on its source)
+    std::vector<dcomplex> hB[3];
+    std::vector<fcomplex> hBf[3], hBc[3];
     Prec<double> d;
     Prec<float> f;
     // transfer to the next coarser level: P (this x coarse) and R = P^T
@@ -352,7 +355,8 @@ void update_bands(tdz_ctx *c, Level &L)
     const double inv2m = 0.5 / c->ph.mass;
     for (int d = 0; d < 3; ++d) {
         const size_t sz = L.M[d].size();
-        std::vector<dcomplex> B(sz);
+        std::vector<dcomplex> &B = L.hB[d];
+        B.resize(sz);
         const bool active = d < c->prob.dim;
         for (size_t k = 0; k < sz; ++k) {
             cplx b;
@@ -371,23 +375,23 @@ void update_bands(tdz_ctx *c, Level &L)
         }
         HIPCHK(hipMemcpyAsync(L.d.B[d], B.data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
         if (c->dBcorr[d] && &L == c->lv[0].get()) {
-            std::vector<fcomplex> Bc(sz, fcomplex{0.f, 0.f});
+            std::vector<fcomplex> &Bc = L.hBc[d];
+            Bc.assign(sz, fcomplex{0.f, 0.f});
             if (active)
                 for (size_t k = 0; k < sz; ++k) {
                     const cplx b = c->g * ((c->cap_on ? cplx(0, -1) * L.Wc[d][k] : 0.0) + c->sE[d] * L.Xb[d][k]);
                     Bc[k] = {(float)b.real(), (float)b.imag()};
                 }
             HIPCHK(hipMemcpyAsync(c->dBcorr[d], Bc.data(), sz * sizeof(fcomplex), hipMemcpyHostToDevice, c->st));
-            HIPCHK(hipStreamSynchronize(c->st));
         }
         if (L.f.ready) {
-            std::vector<fcomplex> Bf(sz);
+            std::vector<fcomplex> &Bf = L.hBf[d];
+            Bf.resize(sz);
             for (size_t k = 0; k < sz; ++k) Bf[k] = {(float)B[k].re, (float)B[k].im};
             HIPCHK(hipMemcpyAsync(L.f.B[d], Bf.data(), sz * sizeof(fcomplex), hipMemcpyHostToDevice, c->st));
-            HIPCHK(hipStreamSynchronize(c->st));
         }
     }
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamSynchronize(c->st));   // the staging vectors are reused by the next update
 }
 
 // Coarse levels are only read by the V-cycle: while the preconditioner does not use
