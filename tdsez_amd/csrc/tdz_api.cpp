@@ -74,6 +74,7 @@ struct Prec {
     cpx<R> *B[3] = {};                       // complex combined bands (current shift)
     cpx<R> *Bmass[3] = {};                   // bands of M alone (M on x, zero on y and z)
     R *V[3] = {}, *lam[3] = {};              // separable preconditioner
+    R *Vf[3][2] = {};                        // complex64: V and V^T in MFMA fragment order (or null)
     R *Mw = nullptr;                         // layered z axis: 1/m-weighted mass band
     cpx<R> *lwork = nullptr;                 // layered z axis: work of the z-line solves
     cpx<R> *r = nullptr, *z = nullptr, *t = nullptr;   // multigrid work vectors
@@ -319,6 +320,8 @@ void upload_level_tables(tdz_ctx *c, Level &L)
 // Single-precision copies of one level (tables, bands, work vectors), for the
 // mixed-precision preconditioners. The separable preconditioner's eigenvectors
 // are copied by setup_fd.
+void upload_fd_float(tdz_ctx *c, Level &L, int d);
+
 void upload_level_tables_f(tdz_ctx *c, Level &L)
 {
     if (L.f.ready) return;
@@ -333,10 +336,7 @@ void upload_level_tables_f(tdz_ctx *c, Level &L)
         std::vector<fcomplex> bf(bm.size());
         for (size_t k = 0; k < bm.size(); ++k) bf[k] = {(float)bm[k].re, (float)bm[k].im};
         L.f.Bmass[d] = upload(c, bf);
-        if (L.fd_ready) {
-            L.f.V[d] = upload(c, to_float(L.hV[d]));
-            L.f.lam[d] = upload(c, to_float(L.hLam[d]));
-        }
+        if (L.fd_ready) upload_fd_float(c, L, d);
     }
     if (L.layered) {
         L.f.Mw = upload(c, to_float(L.Mw));
@@ -531,6 +531,20 @@ void autotune_pot(tdz_ctx *c, Level &L)
     HIPCHK(hipFree(y));
 }
 
+// complex64 copies of the axis-d eigen-decomposition, with the fragment-ordered V and
+// V^T of the register-resident MFMA transform (axes it takes)
+void upload_fd_float(tdz_ctx *c, Level &L, int d)
+{
+    L.f.V[d] = upload(c, to_float(L.hV[d]));
+    L.f.lam[d] = upload(c, to_float(L.hLam[d]));
+    const int n = L.n[d];
+    for (int tr = 0; tr < 2; ++tr) {
+        std::vector<float> fr;
+        if (L.hV[d].size() == (size_t)n * n) fr = tdz::axis_fragments(n, L.hV[d], tr == 1);
+        L.f.Vf[d][tr] = fr.empty() ? nullptr : upload(c, fr);
+    }
+}
+
 void setup_fd(tdz_ctx *c, Level &L)
 {
     if (L.fd_ready) return;
@@ -556,10 +570,7 @@ void setup_fd(tdz_ctx *c, Level &L)
         L.hLam[d] = lam;
         L.d.V[d] = upload(c, V);
         L.d.lam[d] = upload(c, lam);
-        if (L.f.ready) {
-            L.f.V[d] = upload(c, to_float(V));
-            L.f.lam[d] = upload(c, to_float(lam));
-        }
+        if (L.f.ready) upload_fd_float(c, L, d);
     }
     if (L.layered) {
         L.d.lwork = track(c, dalloc<dcomplex>((size_t)L.N * (2 * L.ax[2].p + 1)));
@@ -579,23 +590,31 @@ void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<
 {
     Prec<R> &P = prec<R>(L);
     if (L.layered) {
-        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1);
-        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2);
+        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1, P.Vf[0][1]);
+        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2, P.Vf[1][1]);
         tdz::launch_zline_solve<R>(c->st, L.n, L.ax[2].p, P.lam[0], P.lam[1], P.B[2], P.Mw, cx<R>(c->g), s2, s1,
                                    P.lwork);
-        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, s1, s2);
-        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, s2, z);
+        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, s1, s2, P.Vf[0][0]);
+        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, s2, z, P.Vf[1][0]);
         return;
     }
-    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1);
-    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2);
-    tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], true, s2, s1);
+    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1, P.Vf[0][1]);
+    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2, P.Vf[1][1]);
     const R *lam[3] = {P.lam[0], P.lam[1], P.lam[2]};
     const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-    tdz::launch_diag_scale<R>(c->st, L.n, lam, w, cx<R>(c->alpha), cx<R>(c->g), s1, s2);
-    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, s2, s1);
-    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, s1, s2);
-    tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], false, s2, z);
+    cpx<R> *d = s2, *e = s1;   // the diagonal solve's output and the other scratch vector
+    bool fused = false;
+    if constexpr (std::is_same<R, float>::value) {   // z pass and diagonal solve in one kernel
+        fused = tdz::launch_dense_axis_diag(c->st, L.n, P.Vf[2][1], lam, w, cx<R>(c->alpha), cx<R>(c->g), s2, s1);
+        if (fused) d = s1, e = s2;
+    }
+    if (!fused) {
+        tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], true, s2, s1, P.Vf[2][1]);
+        tdz::launch_diag_scale<R>(c->st, L.n, lam, w, cx<R>(c->alpha), cx<R>(c->g), s1, s2);
+    }
+    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, d, e, P.Vf[0][0]);
+    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, e, d, P.Vf[1][0]);
+    tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], false, d, z, P.Vf[2][0]);
     if (c->prob.dirichlet) tdz::launch_wall_fix<R>(c->st, L.n, w, r, z);
 }
 

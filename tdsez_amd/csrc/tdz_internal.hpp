@@ -141,9 +141,20 @@ void launch_scale_invsqrt(hipStream_t st, const dcomplex *x, const double *norm2
 void launch_norm2(hipStream_t st, const dcomplex *x, long N, double *partial, double *out);
 
 // Dense axis transform y = T x along axis d (T real n x n, row-major; trans: T^T).
+// Tfrag: for R = float, the matrix in the fragment order of axis_fragments (the
+// register-resident MFMA transform; null: the LDS-tiled kernel); ignored for double
 template <typename R>
-void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool trans, const cpx<R> *x, cpx<R> *y);
+void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool trans, const cpx<R> *x, cpx<R> *y,
+                       const R *Tfrag = nullptr);
+// k-steps of 4 of the register-resident transform for an axis of n (0: axis too long)
+int axis_frag_steps(int n);
+// n x n row-major T (or its transpose) as the transform's A fragments (empty: too long)
+std::vector<float> axis_fragments(int n, const std::vector<double> &T, bool trans);
 // One thread per output, T read through the caches (reference kernel for tests).
+// complex64 forward transform along z fused with the diagonal solve (Tfrag: fragments
+// of T^T; then divide by alpha + g (l0 + l1 + l2) off the walls); false if not applicable
+bool launch_dense_axis_diag(hipStream_t st, const int n[3], const float *Tfrag, const float *lam[3],
+                            const unsigned char *wall[3], fcomplex alpha, fcomplex g, const fcomplex *x, fcomplex *y);
 template <typename R>
 void launch_dense_axis_simple(hipStream_t st, int d, const int n[3], const R *T, bool trans, const cpx<R> *x,
                               cpx<R> *y);

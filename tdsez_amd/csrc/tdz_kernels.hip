@@ -975,6 +975,108 @@ __global__ void __launch_bounds__(256) axis_gemm_mfma_kernel(int n, long L, long
             }
 }
 
+// FP32 transform along one axis on MFMA (v_mfma_f32_16x16x4_f32; real and imaginary
+// parts as two GEMMs sharing A), one wave per (16-column tile, group of G 16-row tiles)
+// of the output, no LDS and no barriers. The matrix comes in fragment order (built on
+// the host by axis_fragments): float4 s4 of lane l of row tile it holds
+// A(16 it + l % 16, 4 (4 s4 + j) + l / 16), j < 4, zero past n, so a wave reads its rows
+// with KS/4 fully coalesced 16-byte loads. The wave loads its 16 input columns as B
+// fragments (KS k-steps of 4, branch-free: the index is clamped and the value zeroed
+// past n), then runs the K = n reduction in 4G independent accumulator chains. Columns
+// are numbered across the batch: column c starts at (c / Lin) sB + (c % Lin) sL, k
+// steps by sK. Work item j = 4 t + wave (t the XCD-remapped block index) is column tile
+// j / ngr, row group j % ngr: the row groups of one column tile run on one CU and
+// neighbouring column tiles on one XCD, which share the input lines through L1 and L2.
+struct DiagF {   // epilogue of the last forward pass: divide by alpha + g (l0 + l1 + l2) off the walls
+    const float *lam[3];
+    const unsigned char *wall[3];
+    fcomplex alpha, g;
+    int n0;
+};
+
+template <int KS, int G, bool DIAG>
+__global__ void __launch_bounds__(256) axis_wave_mfma_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
+                                                             int nit, const float4 *__restrict__ Tf,
+                                                             const fcomplex *__restrict__ in,
+                                                             fcomplex *__restrict__ out, DiagF dg)
+{
+    constexpr int KS4 = (KS + 3) / 4;
+    const int ngr = (nit + G - 1) / G;
+    const int nb = gridDim.x, b = blockIdx.x, per = nb / 8;
+    const long t = b < 8 * per ? (long)(b % 8) * per + b / 8 : b;
+    const long item = 4 * t + (threadIdx.x >> 6);
+    const long nct = (ncol + 15) / 16;
+    if (item >= nct * ngr) return;   // whole wave: no barriers below
+    const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const int ig = (int)(item % ngr);
+    const long col = (item / ngr) * 16 + r16;
+    const bool cv = col < ncol;
+    const long base = cv ? (col / Lin) * sB + (col % Lin) * sL : 0;
+    float a[G][KS4 * 4];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int it = min(ig * G + g, nit - 1);   // past the last row tile: a copy, not stored
+#pragma unroll
+        for (int s4 = 0; s4 < KS4; ++s4) {
+            const float4 v = Tf[(it * KS4 + s4) * 64 + lane];
+            a[g][4 * s4] = v.x;
+            a[g][4 * s4 + 1] = v.y;
+            a[g][4 * s4 + 2] = v.z;
+            a[g][4 * s4 + 3] = v.w;
+        }
+    }
+    fcomplex bv[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + k4;
+        bv[s] = in[base + (long)min(k, n - 1) * sK];
+        if (!cv || k >= n) bv[s] = {0.f, 0.f};
+    }
+    f32x4 ar[G][2], ai[G][2];
+#pragma unroll
+    for (int g = 0; g < G; ++g) ar[g][0] = ar[g][1] = ai[g][0] = ai[g][1] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            ar[g][s & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[g][s], bv[s].re, ar[g][s & 1], 0, 0, 0);
+            ai[g][s & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[g][s], bv[s].im, ai[g][s & 1], 0, 0, 0);
+        }
+    // lane holds D[4 k4 + q][r16] of each row tile: output row 16 it + 4 k4 + q of column col
+    if (!cv) return;
+    float la = 0.f;
+    bool wl = false;
+    if constexpr (DIAG) {   // z pass: column col = i0 + n0 i1, row i = i2
+        const int i0 = (int)(col % dg.n0), i1 = (int)(col / dg.n0);
+        la = dg.lam[0][i0] + dg.lam[1][i1];
+        wl = (dg.wall[0][i0] | dg.wall[1][i1]) != 0;
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int it = ig * G + g;
+        if (it >= nit) break;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * it + 4 * k4 + q;
+            if (i >= n) continue;
+            fcomplex v = {ar[g][0][q] + ar[g][1][q], ai[g][0][q] + ai[g][1][q]};
+            if constexpr (DIAG) {
+                if (!wl && !dg.wall[2][i]) {
+                    const float lam = la + dg.lam[2][i];
+                    const float dr = dg.alpha.re + dg.g.re * lam, di = dg.alpha.im + dg.g.im * lam;
+                    const float inv = 1.f / (dr * dr + di * di);
+                    v = {(v.re * dr + v.im * di) * inv, (v.im * dr - v.re * di) * inv};
+                }
+            }
+            out[base + (long)i * sK] = v;
+        }
+    }
+}
+
+// largest axis length the register-resident transform takes; longer axes use the
+// LDS-tiled kernel
+constexpr int kWaveAxisMax = 160;
+
 template <typename R>
 __global__ void diag_scale_kernel(int n0, int n1, long N, const R *l0, const R *l1, const R *l2,
                                   const unsigned char *w0, const unsigned char *w1, const unsigned char *w2,
@@ -1217,8 +1319,65 @@ void launch_norm2(hipStream_t st, const dcomplex *x, long N, double *partial, do
     hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial, (int)nb, out);
 }
 
+int axis_frag_steps(int n)
+{
+    const int ns = (n + 3) / 4;
+    if (n > kWaveAxisMax) return 0;
+    return ns <= 8 ? 8 : ns <= 16 ? 16 : ns <= 24 ? 24 : ns <= 33 ? 33 : 40;
+}
+
+std::vector<float> axis_fragments(int n, const std::vector<double> &T, bool trans)
+{
+    const int KS = axis_frag_steps(n);
+    if (!KS) return {};
+    const int KS4 = (KS + 3) / 4, nit = (n + 15) / 16;
+    std::vector<float> f((size_t)nit * KS4 * 256, 0.f);
+    for (int it = 0; it < nit; ++it)
+        for (int s4 = 0; s4 < KS4; ++s4)
+            for (int l = 0; l < 64; ++l)
+                for (int j = 0; j < 4; ++j) {
+                    const int row = 16 * it + l % 16, k = 4 * (4 * s4 + j) + l / 16;
+                    if (row < n && k < n)
+                        f[(((size_t)it * KS4 + s4) * 64 + l) * 4 + j] =
+                            (float)(trans ? T[(size_t)k * n + row] : T[(size_t)row * n + k]);
+                }
+    return f;
+}
+
+namespace {
+// one launch of the fragment-ordered transform: KS from the axis length, G row tiles
+// per wave (3, fewer on short axes)
+template <bool DIAG>
+void launch_wave_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, long sL, long sK, const float *Tf,
+                      const fcomplex *x, fcomplex *y, const DiagF &dg)
+{
+    const int nit = (nd + 15) / 16, G = nit < 3 ? nit : 3, ngr = (nit + G - 1) / G;
+    const unsigned nblk = (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
+    const float4 *T4 = reinterpret_cast<const float4 *>(Tf);
+#define TDZ_WAVE(KS, GG)                                                                                           \
+    hipLaunchKernelGGL((axis_wave_mfma_kernel<KS, GG, DIAG>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, sB, sL, sK, \
+                       nit, T4, x, y, dg)
+#define TDZ_WAVE_G(KS)              \
+    do {                            \
+        if (G == 1) TDZ_WAVE(KS, 1); \
+        else if (G == 2) TDZ_WAVE(KS, 2); \
+        else TDZ_WAVE(KS, 3);       \
+    } while (0)
+    switch (axis_frag_steps(nd)) {
+    case 8: TDZ_WAVE_G(8); break;
+    case 16: TDZ_WAVE_G(16); break;
+    case 24: TDZ_WAVE_G(24); break;
+    case 33: TDZ_WAVE_G(33); break;
+    default: TDZ_WAVE_G(40); break;
+    }
+#undef TDZ_WAVE_G
+#undef TDZ_WAVE
+}
+}  // namespace
+
 template <typename R>
-void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool trans, const cpx<R> *x, cpx<R> *y)
+void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool trans, const cpx<R> *x, cpx<R> *y,
+                       const R *Tfrag)
 {
     // batched GEMM view of the transform along axis d
     const long n01 = (long)n[0] * n[1];
@@ -1229,12 +1388,26 @@ void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool t
     else if (d == 1) { L = n[0]; sB = n01; sK = n[0]; sL = 1; B = (unsigned)n[2]; kfast = 0; }
     else { L = n01; sB = 0; sK = n01; sL = 1; B = 1; kfast = 0; }
     const dim3 grid((unsigned)((L + GT - 1) / GT), (unsigned)((n[d] + GT - 1) / GT), B);
-    if constexpr (std::is_same<R, float>::value)
+    if constexpr (std::is_same<R, float>::value) {
+        if (Tfrag && axis_frag_steps(n[d])) {   // columns across the batch: (c / Lin) sB + (c % Lin) sL
+            launch_wave_axis<false>(st, n[d], L * B, L, sB, sL, sK, Tfrag, x, y, DiagF{});
+            return;
+        }
         hipLaunchKernelGGL(axis_gemm_mfma_kernel, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T,
                            trans ? 1 : 0, x, y);
-    else
+    } else
         hipLaunchKernelGGL(axis_gemm_kernel<R>, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T,
                            trans ? 1 : 0, x, y);
+}
+
+bool launch_dense_axis_diag(hipStream_t st, const int n[3], const float *Tfrag, const float *lam[3],
+                            const unsigned char *wall[3], fcomplex alpha, fcomplex g, const fcomplex *x, fcomplex *y)
+{
+    if (!Tfrag || !axis_frag_steps(n[2])) return false;
+    const long ncol = (long)n[0] * n[1];
+    const DiagF dg{{lam[0], lam[1], lam[2]}, {wall[0], wall[1], wall[2]}, alpha, g, n[0]};
+    launch_wave_axis<true>(st, n[2], ncol, ncol, 0, 1, ncol, Tfrag, x, y, dg);
+    return true;
 }
 
 template <typename R>
@@ -1458,7 +1631,8 @@ void launch_zline_solve(hipStream_t st, const int n[3], int p, const R *lx, cons
     template void launch_wall_fix<R>(hipStream_t, const int[3], const unsigned char *[3], const cpx<R> *,          \
                                      cpx<R> *);                                                                    \
     template void launch_axpby_t<R>(hipStream_t, cpx<R>, const cpx<R> *, cpx<R>, cpx<R> *, long);                  \
-    template void launch_dense_axis<R>(hipStream_t, int, const int[3], const R *, bool, const cpx<R> *, cpx<R> *); \
+    template void launch_dense_axis<R>(hipStream_t, int, const int[3], const R *, bool, const cpx<R> *, cpx<R> *,   \
+                                        const R *);                                                               \
     template void launch_dense_axis_simple<R>(hipStream_t, int, const int[3], const R *, bool, const cpx<R> *,     \
                                               cpx<R> *);                                                           \
     template void launch_diag_scale<R>(hipStream_t, const int[3], const R *[3], const unsigned char *[3], cpx<R>,  \
