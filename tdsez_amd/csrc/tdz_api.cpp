@@ -133,6 +133,7 @@ namespace {
 struct LearnState {
     bool ready = false;
     float *P = nullptr;                  // [N][C] positional fields
+    unsigned short *Pb = nullptr;        // the same as bf16, field-major (inference)
     std::vector<double> fscale;          // [C] maximum of each field before scaling to 1
     float *W1 = nullptr, *W2 = nullptr;  // device weights [27][CI][C], [27][C][C]
     float *gr[3] = {};                   // Greville coordinates per axis (float)
@@ -798,7 +799,7 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, const fcomplex 
     }
     if (net) {
         const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-        tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.P, ls.mb, ls.out, cx<float>(c->g / ls.sout),
+        tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, cx<float>(c->g / ls.sout),
                             ls.nvec);
         n = ls.nvec;
     }
@@ -1748,6 +1749,10 @@ void setup_learned(tdz_ctx *c)
     std::vector<float> P;
     learned_fields(c, P, ls.fscale);
     ls.P = upload(c, P);
+    {
+        const int n3[3] = {L.n[0], L.n[1], L.n[2]};
+        ls.Pb = upload(c, tdz::cnn_fields_bf16(n3, P));
+    }
     ls.W1 = track(c, dalloc<float>(27 * CI * C));
     ls.W2 = track(c, dalloc<float>(27 * C * C));
     ls.mb = track(c, dalloc<unsigned short>(tdz::cnn_infer_work_bytes(N) / 2));
@@ -2270,7 +2275,7 @@ int tdz_learn_eval(tdz_ctx *c, const double *u, const double *target, double *lo
         }
         if (out_mfma) {
             const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-            tdz::cnn_infer_mfma(c->st, n3, y32, ls.W1, ls.W2, ls.P, ls.mb, ls.out, fcomplex{1.f, 0.f}, nullptr);
+            tdz::cnn_infer_mfma(c->st, n3, y32, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{1.f, 0.f}, nullptr);
             HIPCHK(hipStreamSynchronize(c->st));
             HIPCHK(hipMemcpy(o.data(), ls.out, o.size() * 4, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < o.size(); ++i) out_mfma[i] = o[i];

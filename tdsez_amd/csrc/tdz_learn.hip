@@ -12,6 +12,9 @@
 //                       the preconditioner), v_mfma_f32_16x16x16_bf16
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <vector>
+
 #include "tdz_internal.hpp"
 
 namespace tdz {
@@ -359,37 +362,65 @@ __global__ void cnn_pack_kernel(const float *__restrict__ W1, const float *__res
     wp[e] = pack4(v[0], v[1], v[2], v[3]);
 }
 
-// Layer 1 on MFMA, v_mfma_f32_16x16x16_bf16. Operand layout (wave64): lane l holds
-// A[l%16][4(l/16)+q], B[4(l/16)+q][l%16], q < 4, and D[4(l/16)+q][l%16].
+// The inference reads the positional fields as bf16 in groups of 4 voxels along x:
+// Pb[((z n1 + y) (n0p / 4) + x / 4) 64 + 4 a + x % 4] for field a (n0p: n0 rounded up to
+// a multiple of 4, padding zero). A lane's 4 consecutive voxels x0 + kb .. x0 + kb + 3
+// (kb a multiple of 4) of its field `row` are one aligned 8-byte load, and the 64 lanes
+// of a wave read 512 contiguous bytes.
+__device__ __forceinline__ void load_fields(const unsigned short *__restrict__ Pb, int n0, int n0p, int n1, int n2,
+                                            int row, int gx, int gy, int gz, float pv[4])
+{
+    if (gx < n0 && gy < n1 && gz < n2) {   // gx + 3 < n0p: n0p and gx are multiples of 4
+        const uint2 raw = *reinterpret_cast<const uint2 *>(
+            Pb + (((long)gz * n1 + gy) * (n0p / 4) + gx / 4) * 64 + 4 * row);
+        pv[0] = __uint_as_float(raw.x << 16);
+        pv[1] = __uint_as_float(raw.x & 0xFFFF0000u);
+        pv[2] = __uint_as_float(raw.y << 16);
+        pv[3] = __uint_as_float(raw.y & 0xFFFF0000u);
+    } else {
+        pv[0] = pv[1] = pv[2] = pv[3] = 0.f;
+    }
+}
+
+// Layer 1 on MFMA, v_mfma_f32_16x16x16_bf16, both components per block (the inputs are
+// read once). Operand layout (wave64): lane l holds A[l%16][4(l/16)+q], B[4(l/16)+q][l%16],
+// q < 4, and D[4(l/16)+q][l%16].
 // A[i][k] = u_c(v_i + tap t), k = t CI + c (zero for k >= 27 CI); B[k][a] = W1[t][c][a];
 // D[i][a] -> m = P[v_i][a] D[i][a], stored as bf16 at mb[(comp N + v) C + a].
+// The halo tile is staged in LDS as bf16 (the CI channels of a voxel are one 8-byte
+// A operand, no conversion per MFMA).
 __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2, const fcomplex *__restrict__ u,
-                                                          const uint2 *__restrict__ wp, const float *__restrict__ P,
+                                                          const uint2 *__restrict__ wp,
+                                                          const unsigned short *__restrict__ Pb, int n0p,
                                                           unsigned short *__restrict__ mb)
 {
     static_assert(CI == 4, "a lane's 4 consecutive k must be the CI channels of one tap");
-    __shared__ float4 us[HV];   // halo tile, the CI channels of one component per voxel
+    __shared__ uint2 usv[2][HV];   // [comp][voxel]: the CI channels as bf16
+    unsigned short *us = reinterpret_cast<unsigned short *>(usv);
     const TileIdx T = tile_of(n0, n1, n2);
     const long N = (long)n0 * n1 * n2;
-    // stage in two phases (all loads in flight, then all LDS stores): a load-store loop
-    // would wait for each load in turn
+    // stage in two phases (all loads in flight, then all LDS stores)
     constexpr int NE = (CI * HV + 255) / 256;
-    float val[NE];
+    fcomplex val[NE];
 #pragma unroll
     for (int r = 0; r < NE; ++r) {
         const int e = threadIdx.x + 256 * r;
-        val[r] = 0.f;
+        val[r] = {0.f, 0.f};
         if (e < CI * HV) {
             const int c = e / HV, h = e % HV;
             const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
             if (gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2)
-                val[r] = comp_of(u[(long)c * N + gx + (long)n0 * (gy + (long)n1 * gz)], T.comp);
+                val[r] = u[(long)c * N + gx + (long)n0 * (gy + (long)n1 * gz)];
         }
     }
 #pragma unroll
     for (int r = 0; r < NE; ++r) {
         const int e = threadIdx.x + 256 * r;
-        if (e < CI * HV) reinterpret_cast<float *>(us)[(e % HV) * CI + e / HV] = val[r];
+        if (e < CI * HV) {
+            const int c = e / HV, h = e % HV;
+            us[h * CI + c] = (unsigned short)to_bf16(val[r].re);
+            us[(HV + h) * CI + c] = (unsigned short)to_bf16(val[r].im);
+        }
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row = lane & 15, kb = (lane >> 4) * 4;
@@ -397,21 +428,16 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
 #pragma unroll
     for (int s = 0; s < K1 / 16; ++s) bw[s] = unpack4(wp[(NTAP + s) * 64 + lane]);
     __syncthreads();
+    const uint2 *us0 = usv[0], *us1 = usv[1];
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
         // the plane's positional values for this lane's outputs, loaded ahead of the MFMAs
         float pv[TY][4];
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry)
+        for (int ry = 0; ry < TY; ++ry) load_fields(Pb, n0, n0p, n1, n2, row, T.x0 + kb, T.y0 + ry, gz, pv[ry]);
+        floatx4 acc[2][TY];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int gx = T.x0 + kb + q, gy = T.y0 + ry;
-                pv[ry][q] = (gx < n0 && gy < n1 && gz < n2) ? P[(gx + (long)n0 * (gy + (long)n1 * gz)) * C + row]
-                                                            : 0.f;
-            }
-        floatx4 acc[TY];
-#pragma unroll
-        for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
+        for (int ry = 0; ry < TY; ++ry) acc[0][ry] = acc[1][ry] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < K1 / 16; ++s) {
             const int t = 4 * s + kb / 4;
@@ -419,13 +445,9 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
             const int dx = tap_dx(tt), dy = tap_dy(tt), dz = tap_dz(tt);
 #pragma unroll
             for (int ry = 0; ry < TY; ++ry) {
-                const float4 v = us[tap_of_halo(row + 1 + dx, ry + 1 + dy, lz + 1 + dz)];
-                bf16x4 a;
-                a[0] = to_bf16(v.x);
-                a[1] = to_bf16(v.y);
-                a[2] = to_bf16(v.z);
-                a[3] = to_bf16(v.w);
-                acc[ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[s], acc[ry], 0, 0, 0);
+                const int h = tap_of_halo(row + 1 + dx, ry + 1 + dy, lz + 1 + dz);
+                acc[0][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(unpack4(us0[h]), bw[s], acc[0][ry], 0, 0, 0);
+                acc[1][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(unpack4(us1[h]), bw[s], acc[1][ry], 0, 0, 0);
             }
         }
         // lane holds D[kb + q][row]: voxel x0 + kb + q, hidden channel row
@@ -437,7 +459,8 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
                 const int gx = T.x0 + kb + q;
                 if (gx >= n0 || gy >= n1 || gz >= n2) continue;
                 const long v = gx + (long)n0 * (gy + (long)n1 * gz);
-                mb[((long)T.comp * N + v) * C + row] = (unsigned short)to_bf16(pv[ry][q] * acc[ry][q]);
+                mb[v * C + row] = (unsigned short)to_bf16(pv[ry][q] * acc[0][ry][q]);
+                mb[(N + v) * C + row] = (unsigned short)to_bf16(pv[ry][q] * acc[1][ry][q]);
             }
         }
     }
@@ -451,7 +474,8 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
 constexpr int MS = 20;   // LDS stride of one voxel of m, in bf16 elements
 __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2,
                                                           const unsigned short *__restrict__ mb,
-                                                          const uint2 *__restrict__ wp, const float *__restrict__ P,
+                                                          const uint2 *__restrict__ wp,
+                                                          const unsigned short *__restrict__ Pb, int n0p,
                                                           float *__restrict__ out)
 {
     __shared__ uint2 ms[HV * MS / 4];
@@ -485,30 +509,41 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
 #pragma unroll
     for (int t = 0; t < NTAP; ++t) bw[t] = unpack4(wp[t * 64 + lane]);
     __syncthreads();
+    // Halo-row-major order: each A fragment (16 voxels of one halo row, shifted by dx) is
+    // read from LDS once and feeds the up to 9 output rows (ry, pz) it is a tap of, so
+    // the wave reads (TY + 2)(WZ + 2) 3 fragments instead of 27 TY WZ.
+    floatx4 acc[WZ][TY];
+#pragma unroll
+    for (int pz = 0; pz < WZ; ++pz)
+#pragma unroll
+        for (int ry = 0; ry < TY; ++ry) acc[pz][ry] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int hz = 0; hz < WZ + 2; ++hz)
+#pragma unroll
+        for (int hy = 0; hy < TY + 2; ++hy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int h = tap_of_halo(row + 1 + dx, hy, w * WZ + hz);
+                const bf16x4 a = unpack4(ms[(h * MS) / 4 + kb / 4]);
+#pragma unroll
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int pz = hz - 1 - dz;   // output plane w WZ + pz reads halo plane w WZ + hz
+                    if (pz < 0 || pz >= WZ) continue;
+#pragma unroll
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int ry = hy - 1 - dy;
+                        if (ry < 0 || ry >= TY) continue;
+                        const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                        acc[pz][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[t], acc[pz][ry], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
-        float pv[TY][4];   // positional values of this lane's outputs, loaded ahead of the MFMAs
+        float pv[TY][4];   // positional values of this lane's outputs
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int gx = T.x0 + kb + q, gy = T.y0 + ry;
-                pv[ry][q] = (gx < n0 && gy < n1 && gz < n2) ? P[(gx + (long)n0 * (gy + (long)n1 * gz)) * C + row]
-                                                            : 0.f;
-            }
-        floatx4 acc[TY];
-#pragma unroll
-        for (int ry = 0; ry < TY; ++ry) acc[ry] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int t = 0; t < NTAP; ++t) {
-            const int dx = tap_dx(t), dy = tap_dy(t), dz = tap_dz(t);
-#pragma unroll
-            for (int ry = 0; ry < TY; ++ry) {
-                const int h = tap_of_halo(row + 1 + dx, ry + 1 + dy, lz + 1 + dz);
-                acc[ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(unpack4(ms[(h * MS) / 4 + kb / 4]), bw[t],
-                                                                   acc[ry], 0, 0, 0);
-            }
-        }
+        for (int ry = 0; ry < TY; ++ry) load_fields(Pb, n0, n0p, n1, n2, row, T.x0 + kb, T.y0 + ry, gz, pv[ry]);
         // lane holds D[kb + q][row] (voxel x0 + kb + q, output channel row). out(v) sums the
         // 16 lanes of a group: recursive halving over the 16 values (ry, q) leaves lane row
         // with the sum for k = 4 ry + q = row (15 exchanges instead of 64).
@@ -516,14 +551,18 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) v16[4 * ry + q] = pv[ry][q] * acc[ry][q];
+            for (int q = 0; q < 4; ++q) v16[4 * ry + q] = pv[ry][q] * acc[pz][ry][q];
+        // The lane-dependent choice between v16[i] and v16[i + m] is a bit mask (one
+        // v_bfi per value): written as a select, the compiler turns it into a
+        // runtime index into v16 (a 16-way compare/select chain per value).
 #pragma unroll
         for (int m = 8; m > 0; m >>= 1) {
-            const bool up = (row & m) != 0;
+            const unsigned um = (row & m) ? 0xFFFFFFFFu : 0u;
 #pragma unroll
             for (int i = 0; i < m; ++i) {
-                const float keep = up ? v16[i + m] : v16[i];
-                const float send = up ? v16[i] : v16[i + m];
+                const unsigned lo = __float_as_uint(v16[i]), hi = __float_as_uint(v16[i + m]);
+                const float keep = __uint_as_float((hi & um) | (lo & ~um));
+                const float send = __uint_as_float((lo & um) | (hi & ~um));
                 v16[i] = keep + __shfl_xor(send, m, 16);
             }
         }
@@ -743,17 +782,40 @@ void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const f
                        NTAP * CI * C, dW1);
 }
 
+int cnn_fields_row(int n0) { return (n0 + 3) / 4 * 4; }
+
+std::vector<unsigned short> cnn_fields_bf16(const int n[3], const std::vector<float> &P)
+{
+    const int n0p = cnn_fields_row(n[0]);
+    std::vector<unsigned short> out((size_t)C * n[2] * n[1] * n0p, 0);
+    for (int a = 0; a < C; ++a)
+        for (int z = 0; z < n[2]; ++z)
+            for (int y = 0; y < n[1]; ++y)
+                for (int x = 0; x < n[0]; ++x) {
+                    const float f = P[(x + (size_t)n[0] * (y + (size_t)n[1] * z)) * C + a];
+                    unsigned bits;
+                    std::memcpy(&bits, &f, 4);
+                    bits += 0x7FFFu + ((bits >> 16) & 1u);   // round to nearest even
+                    out[(((size_t)z * n[1] + y) * (n0p / 4) + x / 4) * 64 + 4 * a + x % 4] =
+                        (unsigned short)(bits >> 16);
+                }
+    return out;
+}
+
 void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
-                    const float *P, unsigned short *mb, float *out, fcomplex g, fcomplex *nout)
+                    const unsigned short *Pb, unsigned short *mb, float *out, fcomplex g, fcomplex *nout)
 {
     const long N = (long)n[0] * n[1] * n[2];
-    const long ntiles = 2L * ((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ);
+    const long tiles = (long)((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ);
+    const int n0p = cnn_fields_row(n[0]);
     // the packed weights go behind the 2 N C bf16 of m (the work buffer holds both)
     uint2 *wp = reinterpret_cast<uint2 *>(mb + 2 * N * C);
     hipLaunchKernelGGL(cnn_pack_kernel, dim3(gridn(NPACK * 64, 256)), dim3(256), 0, st, W1, W2, wp);
-    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n[0], n[1], n[2], u, wp, P, mb);
-    hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)ntiles), dim3(256), 0, st, n[0], n[1], n[2], mb, wp, P,
-                       out);
+    // layer 1: one block per tile (both components); layer 2: one per tile and component
+    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)tiles), dim3(256), 0, st, n[0], n[1], n[2], u, wp, Pb, n0p,
+                       mb);
+    hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)(2 * tiles)), dim3(256), 0, st, n[0], n[1], n[2], mb, wp,
+                       Pb, n0p, out);
     if (nout) hipLaunchKernelGGL(cnn_to_complex_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, N, out, g, nout);
 }
 
