@@ -63,6 +63,9 @@ def main():
     ap.add_argument("--nel", type=int, default=128)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--predictor", type=int, default=2, choices=[0, 1, 2],
+                    help="initial guess of the CN solves, both legs: 0 the current state (as TSTHETA), "
+                         "1/2 the binomial extrapolation of the last 2/3 states")
     ap.add_argument("--init", default="ground", choices=["ground", "gaussian"],
                     help="initial state: lowest eigenstate (default) or the projected Gaussian (profiling runs)")
     ap.add_argument("--workload", default="atom128", choices=["atom128", "scan", "hetero"],
@@ -99,9 +102,10 @@ def main():
     dt = 0.02
     x0 = psi.download()
 
-    def run(kind, timing):
+    def run(kind, timing, order=None):
         """W untimed + K timed CN steps from x0; returns (steps/s, its, final psi, kernel ms, launches)."""
         ctx.pc_setup(kind)
+        ctx.set_cn_predictor(args.predictor if order is None else order)
         d = ctx.vec(x0)
         ctx.evolve_cn(d, 0.0, dt, args.warmup, pulse, rtol=args.rtol, maxit=500)
         ctx.sync()
@@ -153,6 +157,18 @@ def main():
         base["speedup"] = steps_per_s / base["steps_per_s"]
         base["psi_rel_l2_diff"] = mnorm(psi_end - psi_b) / mnorm(psi_b)
 
+    plain = None
+    if args.predictor > 0:   # the same legs with the current state as guess
+        sps0, its0, _, _, _ = run(kinds[args.pc], False, 0)
+        plain = {"steps_per_s": sps0, "krylov_its_per_step": float(np.mean(its0))}
+        if base is not None:
+            sps0b, its0b, _, _, _ = run(T.PC_MG, False, 0)
+            plain.update({"fp64_mg_steps_per_s": sps0b, "fp64_mg_krylov_its_per_step": float(np.mean(its0b)),
+                          "speedup": sps0 / sps0b})
+        print(f"# without predictor: {sps0:.2f} steps/s (its {np.mean(its0):.1f})"
+              + (f", FP64 GMG {plain['fp64_mg_steps_per_s']:.2f} steps/s" if base is not None else ""),
+              file=sys.stderr, flush=True)
+
     p, Q, nel = desc["p"], desc["nquad"], args.nel
     flops = pot_flops_per_element(p, Q) * nel ** 3
     avg_s = (kms / 1e3) / max(nlaunch, 1)
@@ -181,9 +197,12 @@ def main():
         if args.pc in ("learned32", "sep32", "mg32", "corr32", "corrx32") else "complex128 (fp64)", "data": "synthetic: lowest eigenstate of H (GPU inverse iteration), E0 = %.8f" % E0,
         "config": {"workload": f"3D soft-core atom {nel}^3 elements p=4 Q=8, CN dt=0.02, rtol={args.rtol}",
                    "preconditioner": PCS[args.pc],
+                   "cn_guess": ("current state" if args.predictor == 0 else
+                                f"binomial extrapolation of the last {args.predictor + 1} states (both legs)"),
                    "dofs": ctx.N, "mg_levels": ctx.levels, "krylov_its_per_step": float(np.mean(its)),
                    "setup_s": t_setup},
         "fp64_mg_baseline": base,
+        "without_predictor": plain,
         "learned": learn,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -137,6 +137,13 @@ int tdz_step_cn(tdz_ctx *ctx, double *psi, double dt, const double sE_mid[3], do
  * step; its (optional, host) the Krylov iterations per step. */
 int tdz_evolve_cn(tdz_ctx *ctx, double *psi, double t0, double dt, int nsteps, const tdz_pulse *pulse,
                   double rtol, int maxit, double *obs, int *its);
+/* Initial guess of the CN solves of tdz_evolve_cn: order 0 (default) starts each
+ * solve from the current state, as TSTHETA does (src/propagator.cpp:149); order 1 or
+ * 2 from the binomial extrapolation of the last 2 or 3 states (the guess of the
+ * stage solves, src/propagator.cpp:629-638), whose residual the library forms
+ * without an operator apply. Every solve still meets rtol on the true residual; the
+ * history carries over to the next tdz_evolve_cn call when psi is unchanged. */
+int tdz_set_cn_predictor(tdz_ctx *ctx, int order);
 /* Initial states. tdz_project_gaussian: L2 projection of the packet
  * prod_a (2 pi w_a^2)^(-1/4) exp(-(x_a-c_a)^2/(4 w_a^2) + i k_a x_a), normalised so
  * psi^H M psi = 1 (src/core.cpp:1188-1262). tdz_ground_state: lowest eigenstate of

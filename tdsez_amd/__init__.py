@@ -29,7 +29,7 @@ EXPORTS = [
     "tdz_create", "tdz_destroy", "tdz_last_error", "tdz_size", "tdz_dims", "tdz_levels",
     "tdz_vec_alloc", "tdz_vec_free", "tdz_vec_upload", "tdz_vec_download", "tdz_sync",
     "tdz_set_shift", "tdz_op_apply", "tdz_term_apply", "tdz_pc_setup", "tdz_pc_apply",
-    "tdz_ksp_solve", "tdz_step_cn", "tdz_evolve_cn", "tdz_observables", "tdz_pulse_coupling",
+    "tdz_ksp_solve", "tdz_step_cn", "tdz_evolve_cn", "tdz_set_cn_predictor", "tdz_observables", "tdz_pulse_coupling",
     "tdz_timing", "tdz_timing_stats", "tdz_step_pade22", "tdz_evolve_rational",
     "tdz_project_gaussian", "tdz_ground_state",
     "tdz_trace_row", "tdz_pade_history", "tdz_batch_create", "tdz_batch_context", "tdz_batch_evolve",
@@ -138,6 +138,7 @@ def lib():
         L.tdz_ksp_solve.argtypes = [vp, vp, vp, d, i, C.POINTER(KspStats)]
         L.tdz_step_cn.argtypes = [vp, vp, d, dp, d, i, C.POINTER(KspStats)]
         L.tdz_evolve_cn.argtypes = [vp, vp, d, d, i, C.POINTER(Pulse), d, i, dp, ip]
+        L.tdz_set_cn_predictor.argtypes = [vp, i]
         L.tdz_observables.argtypes = [vp, vp, dp]
         L.tdz_pulse_coupling.argtypes = [vp, C.POINTER(Pulse), d, dp]
         L.tdz_step_pade22.argtypes = [vp, vp, d, dp, i, d, i, C.POINTER(KspStats)]
@@ -258,6 +259,10 @@ class Context:
         s = np.ascontiguousarray(sE_mid, dtype=np.float64)
         _chk(lib().tdz_step_cn(self.h, psi.ptr, dt, _dp(s), rtol, maxit, C.byref(st)))
         return st
+
+    def set_cn_predictor(self, order):
+        """Initial guess of the CN solves: 0 the current state, 1/2 the binomial extrapolation."""
+        _chk(lib().tdz_set_cn_predictor(self.h, int(order)))
 
     def evolve_cn(self, psi, t0, dt, nsteps, pulse=None, rtol=1e-12, maxit=1000, observe=False):
         its = np.zeros(nsteps, dtype=np.int32)

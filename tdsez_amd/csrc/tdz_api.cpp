@@ -185,6 +185,16 @@ struct tdz_ctx {
         int nsteps = 0, k = 0;
         double dt = 0;
     } cap;
+    // CN predictor: the last states of a run with their field-free operator products and
+    // dipole products, for the extrapolated initial guess and its residual
+    struct Predictor {
+        int order = 0;              // extrapolation order of the guess (0: the current state)
+        int count = 0, head = 0;    // states held (at most 3), slot of the newest
+        double dt = 0;
+        unsigned axes = 0;          // field axes whose dipole products are kept
+        dcomplex *psi[3] = {}, *P[3] = {}, *D[3][3] = {};   // state; (M + g H_0) state; D_a state
+        dcomplex *Xband[3] = {}, *Zband[3] = {};   // complex position bands and zero bands (dipole applies)
+    } pred;
     // chained CN right-hand side: A psi of the last solve of an evolve run
     dcomplex *ax = nullptr, *zerov = nullptr, *dDip[3] = {};
     bool ax_valid = false;
@@ -1468,6 +1478,7 @@ void cn_step(tdz_ctx *c, dcomplex *x, double dt, const double sE_mid[3], double 
     Level &L = *c->lv[0];
     const long N = L.N;
     const cplx g(0.0, 0.5 * dt);
+    c->pred.count = 0;   // the state moves on without the predictor's bookkeeping
     if (!c->ax) c->ax = track(c, dalloc<dcomplex>(N));
     chained = chained && c->ax_valid && c->ax_dt == dt;
     tdz::SepBandsT<double> mb = sep_bands<double>(L);
@@ -1521,9 +1532,159 @@ void cn_step(tdz_ctx *c, dcomplex *x, double dt, const double sE_mid[3], double 
     c->ax_dt = dt;
     for (int a = 0; a < 3; ++a) c->ax_sE[a] = sE_mid[a];
 }
+
+// ------------------------------------------------------------------ CN predictor
+// y = D_a x, the dipole along axis a: the position band on axis a and the mass bands on
+// the others; zero on Dirichlet walls, whose rows of the propagator matrices carry no field.
+void dipole_apply(tdz_ctx *c, int a, const dcomplex *x, dcomplex *y)
+{
+    Level &L = *c->lv[0];
+    auto &p = c->pred;
+    for (int d = 0; d < 3; ++d)
+        if (!p.Zband[d]) p.Zband[d] = upload(c, std::vector<dcomplex>(L.M[d].size(), dcomplex{0, 0}));
+    if (!p.Xband[a]) {
+        std::vector<dcomplex> B(L.Xb[a].size());
+        for (size_t k = 0; k < B.size(); ++k) B[k] = {L.Xb[a][k], 0.0};
+        p.Xband[a] = upload(c, B);
+    }
+    tdz::SepBandsT<double> db = sep_bands<double>(L);
+    db.Mz2 = nullptr;   // layered structures: the dipole carries the plain mass along z
+    for (int d = 0; d < 3; ++d) db.B[d] = d == a ? p.Xband[a] : p.Zband[d];
+    tdz::launch_sep_apply<double>(c->st, db, x, y, c->sd.tmp);
+    if (c->prob.dirichlet) {
+        if (!c->zerov) c->zerov = upload(c, std::vector<dcomplex>(L.N, dcomplex{0, 0}));
+        const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+        tdz::launch_wall_fix<double>(c->st, L.n, wall, c->zerov, y);
+    }
+}
+
+// Push state x into the predictor history with D_a x and P = (M + g H_0) x, the operator
+// at the run's g without field: P = ax - g sum_a sE_a D_a x when ax = (M + g H_sE) x is
+// known (the solve's A x), else one operator apply.
+void pred_push(tdz_ctx *c, const dcomplex *x, const dcomplex *ax, const double sE[3])
+{
+    auto &p = c->pred;
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    const cplx g(0.0, 0.5 * p.dt);
+    const int h = (p.head + 1) % 3;
+    if (!p.psi[h]) {
+        p.psi[h] = track(c, dalloc<dcomplex>(N));
+        p.P[h] = track(c, dalloc<dcomplex>(N));
+    }
+    HIPCHK(hipMemcpyAsync(p.psi[h], x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+    for (int a = 0; a < 3; ++a)
+        if (p.axes >> a & 1) {
+            if (!p.D[h][a]) p.D[h][a] = track(c, dalloc<dcomplex>(N));
+            dipole_apply(c, a, x, p.D[h][a]);
+        }
+    if (ax) {
+        tdz::LinComb lc;
+        lc.add({1, 0}, ax);
+        for (int a = 0; a < 3; ++a)
+            if (p.axes >> a & 1) lc.add(dc(-g * sE[a]), p.D[h][a]);
+        tdz::launch_lincomb(c->st, lc, p.P[h], N);
+    } else {
+        const double zero[3] = {0, 0, 0};
+        set_shift(c, 1.0, g, zero);
+        level_apply(c, L, x, p.P[h]);
+    }
+    p.head = h;
+    p.count = std::min(p.count + 1, 3);
+}
+
+// True when x is bit for bit the newest state of the history (an evolve call that
+// continues the run of the previous one).
+bool pred_continues(tdz_ctx *c, const dcomplex *x, double dt)
+{
+    auto &p = c->pred;
+    if (p.count == 0 || p.dt != dt) return false;
+    const long N = c->lv[0]->N;
+    tdz::LinComb d;
+    d.add({1, 0}, x);
+    d.add({-1, 0}, p.psi[p.head]);
+    tdz::launch_lincomb(c->st, d, c->sd.w2, N);
+    tdz::launch_norm2(c->st, c->sd.w2, N, c->dpartr, c->dnorm);
+    double v = -1;
+    HIPCHK(hipMemcpyAsync(&v, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return v == 0.0;
+}
+
+// One CN step with an extrapolated initial guess. With the states psi_j (j = 0 the
+// current one) of the history, the guess is the binomial extrapolation
+// x0 = sum_j w_j psi_j, w = (1), (2, -1) or (3, -3, 1) by order; its residual
+//   r0 = b - A x0,  A = M + g H_sE,  b = (M - g H_sE) psi_0 = 2 M psi_0 - A psi_0,
+// follows from A psi_j = P_j + g sum_a sE_a D_a psi_j without an operator apply, so
+// the step costs one mass apply and one dipole apply per field axis besides the solve.
+// The solve is the same (M + g H_sE) psi' = b to the same tolerance as without the
+// guess (src/propagator.cpp:78-251 starts from psi_0; the stage solves of
+// src/propagator.cpp:629-638 extrapolate the same way).
+void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double rtol, int maxit, tdz_ksp_stats *st)
+{
+    if (!c->pc_ready) pc_setup(c, c->pc_kind);
+    auto &p = c->pred;
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    const cplx g(0.0, 0.5 * dt);
+    c->ax_valid = false;
+    if (!c->ax) c->ax = track(c, dalloc<dcomplex>(N));
+    for (int a = 0; a < c->prob.dim; ++a)
+        if (sE[a] != 0 && !(p.axes >> a & 1)) {   // a new field axis: restart the history
+            p.axes |= 1u << a;
+            p.count = 0;
+        }
+    if (p.count == 0 || p.dt != dt) {
+        p.count = 0;
+        p.dt = dt;
+        pred_push(c, x, nullptr, sE);
+    }
+    auto slot = [&](int j) { return (p.head - j + 3) % 3; };
+    const int h0 = slot(0);
+    // M psi_0 (walls: identity rows)
+    tdz::SepBandsT<double> mb = sep_bands<double>(L);
+    for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
+    tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
+    if (c->prob.dirichlet) {
+        const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+        tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
+    }
+    tdz::LinComb rb;   // b = 2 M psi_0 - P_0 - g sE.D psi_0
+    rb.add({2, 0}, c->sd.w1);
+    rb.add({-1, 0}, p.P[h0]);
+    for (int a = 0; a < 3; ++a)
+        if (p.axes >> a & 1) rb.add(dc(-g * sE[a]), p.D[h0][a]);
+    tdz::launch_lincomb(c->st, rb, c->rhs, N);
+    static const double W[3][3] = {{1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
+    const int o = std::min(p.order, p.count - 1);
+    tdz::LinComb r0, x0;
+    r0.add({1, 0}, c->rhs);
+    for (int j = 0; j <= o; ++j) {
+        const int s = slot(j);
+        const double w = W[o][j];
+        r0.add({-w, 0}, p.P[s]);
+        for (int a = 0; a < 3; ++a)
+            if (p.axes >> a & 1) r0.add(dc(-w * g * sE[a]), p.D[s][a]);
+        x0.add({w, 0}, p.psi[s]);
+    }
+    tdz::launch_lincomb(c->st, r0, c->Vb, N);   // the solve's first basis vector before scaling
+    tdz::launch_lincomb(c->st, x0, x, N);
+    set_shift(c, 1.0, g, sE);
+    step_solve(c, c->rhs, x, rtol, maxit, st, true, c->ax);
+    pred_push(c, x, c->ax, sE);
+}
 }  // namespace
 
 extern "C" {
+
+int tdz_set_cn_predictor(tdz_ctx *c, int order)
+{
+    return guarded([&] {
+        if (order < 0 || order > 2) throw Error("CN predictor order must be 0, 1 or 2");
+        c->pred.order = order;
+        c->pred.count = 0;
+    });
+}
 
 int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], double rtol, int maxit,
                 tdz_ksp_stats *stats)
@@ -2112,12 +2273,15 @@ int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, con
 {
     return guarded([&] {
         if (!c->pc_ready) pc_setup(c, c->pc_kind);
+        dcomplex *x = reinterpret_cast<dcomplex *>(psi);
+        if (c->pred.order > 0 && !pred_continues(c, x, dt)) c->pred.count = 0;
         for (int s = 0; s < nsteps; ++s) {
             const double tm = t0 + (s + 0.5) * dt;
             double sE[3] = {0, 0, 0};
             if (pulse) tdz_pulse_coupling(c, pulse, tm, sE);
             tdz_ksp_stats st{};
-            cn_step(c, reinterpret_cast<dcomplex *>(psi), dt, sE, rtol, maxit, &st, s > 0);
+            if (c->pred.order > 0) cn_step_pred(c, x, dt, sE, rtol, maxit, &st);
+            else cn_step(c, x, dt, sE, rtol, maxit, &st, s > 0);
             // refresh rule (include/tdsez_propagator.hpp:17-31): the preconditioner is rebuilt
             // from the current operator when a solve needs > 2 base + 5 iterations
             if (c->refresh_base < 0) c->refresh_base = st.its;

@@ -127,6 +127,16 @@ void launch_multi_axpy_norm(hipStream_t st, const dcomplex *V, long ld, int nv, 
 void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N);
 // y = a x + b y (scalars on host)
 void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dcomplex *y, long N);
+// y = sum_k c_k v_k (k <= kMax terms; y may be one of the v_k: every element is read
+// before it is written)
+struct LinComb {
+    static constexpr int kMax = 16;
+    const dcomplex *v[kMax];
+    dcomplex c[kMax];
+    int k = 0;
+    void add(dcomplex coef, const dcomplex *vec) { c[k] = coef; v[k++] = vec; }
+};
+void launch_lincomb(hipStream_t st, const LinComb &lc, dcomplex *y, long N);
 template <typename R>
 void launch_axpby_t(hipStream_t st, cpx<R> a, const cpx<R> *x, cpx<R> b, cpx<R> *y, long N);
 // precision conversions

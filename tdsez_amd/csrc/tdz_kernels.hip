@@ -12,6 +12,7 @@
 //   csr_axis          : Boehm prolongation / restriction along one axis.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
 #include <type_traits>
 
 #include "tdz_internal.hpp"
@@ -1077,6 +1078,19 @@ __global__ void __launch_bounds__(256) axis_wave_mfma_kernel(int n, long ncol, l
 // LDS-tiled kernel
 constexpr int kWaveAxisMax = 160;
 
+__global__ void lincomb_kernel(LinComb lc, dcomplex *y, long N)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        double re = 0, im = 0;
+        for (int k = 0; k < lc.k; ++k) {
+            const dcomplex v = lc.v[k][i], c = lc.c[k];
+            re += c.re * v.re - c.im * v.im;
+            im += c.re * v.im + c.im * v.re;
+        }
+        y[i] = {re, im};
+    }
+}
+
 template <typename R>
 __global__ void diag_scale_kernel(int n0, int n1, long N, const R *l0, const R *l1, const R *l2,
                                   const unsigned char *w0, const unsigned char *w1, const unsigned char *w2,
@@ -1277,6 +1291,12 @@ void launch_multi_axpy_norm(hipStream_t st, const dcomplex *V, long ld, int nv, 
 void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N)
 {
     hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, Z, ld, nv, c, x, N);
+}
+
+void launch_lincomb(hipStream_t st, const LinComb &lc, dcomplex *y, long N)
+{
+    if (lc.k < 1 || lc.k > LinComb::kMax) throw std::runtime_error("lincomb: 1 .. kMax terms");
+    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, lc, y, N);
 }
 
 void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dcomplex *y, long N)
