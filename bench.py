@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--predictor", type=int, default=2, choices=[0, 1, 2],
                     help="initial guess of the CN solves, both legs: 0 the current state (as TSTHETA), "
                          "1/2 the binomial extrapolation of the last 2/3 states")
+    ap.add_argument("--no-plain-guess", action="store_true",
+                    help="skip the comparison runs that start each solve from the current state")
     ap.add_argument("--init", default="ground", choices=["ground", "gaussian"],
                     help="initial state: lowest eigenstate (default) or the projected Gaussian (profiling runs)")
     ap.add_argument("--workload", default="atom128", choices=["atom128", "scan", "hetero"],
@@ -158,7 +160,7 @@ def main():
         base["psi_rel_l2_diff"] = mnorm(psi_end - psi_b) / mnorm(psi_b)
 
     plain = None
-    if args.predictor > 0:   # the same legs with the current state as guess
+    if args.predictor > 0 and not args.no_plain_guess:   # the same legs with the current state as guess
         sps0, its0, _, _, _ = run(kinds[args.pc], False, 0)
         plain = {"steps_per_s": sps0, "krylov_its_per_step": float(np.mean(its0))}
         if base is not None:

@@ -12,6 +12,7 @@
 //   csr_axis          : Boehm prolongation / restriction along one axis.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <type_traits>
 
@@ -555,6 +556,117 @@ __global__ void sep_z_kernel(long plane, int n2, int p, const R *__restrict__ M,
         yi += mm * av.im + bb.re * bv.im + bb.im * bv.re;
     }
     y[I] = {yr, yi};
+}
+
+// Fused separable apply (FP64, 3D, equal orders P on the three axes), z-marching:
+//   y = Mz' My Bx x + Mz' By Mx x + Bz My Mx x
+// (Mz' the z factor of the x and y terms). A block owns a 16 x 16 tile of (i, j)
+// columns and a chunk of output planes k; it walks the input planes K that reach the
+// chunk. Per plane: the x band pass for the tile's columns on the 16 + 2P rows it
+// needs (plane tile with halo staged in LDS), then the y band pass for each (i, j), and
+// the z band scatters a = My u + By t, b = My t into a ring of 2P + 1 output planes
+// held in registers (slot of output k: (k + P) mod (2P + 1), compile-time in the
+// unrolled ring loop); output k - P is complete after plane K and is stored. One read of
+// x and one write of y per point, instead of the three passes' six vector transfers.
+template <int P>
+__global__ void __launch_bounds__(256) sep3_kernel(int n0, int n1, int n2, int kchunk, const double *__restrict__ Mx,
+                                                   const dcomplex *__restrict__ Bx, const double *__restrict__ My,
+                                                   const dcomplex *__restrict__ By, const double *__restrict__ Mz,
+                                                   const dcomplex *__restrict__ Bz, const dcomplex *__restrict__ x,
+                                                   dcomplex *__restrict__ y)
+{
+    constexpr int R = 2 * P + 1, T = 16, H = T + 2 * P;
+    __shared__ dcomplex Xs[H * H];            // plane tile with halo, [jj][ii]
+    __shared__ dcomplex Ts[H * T], Us[H * T];  // x pass on the halo rows, [jj][i]
+    const int tid = threadIdx.x, ti = tid % T, tj = tid / T;
+    const int tilesx = (n0 + T - 1) / T, tilesy = (n1 + T - 1) / T;
+    int b = blockIdx.x;
+    const int bx = b % tilesx;
+    b /= tilesx;
+    const int by = b % tilesy, bz = b / tilesy;
+    const int i0 = bx * T, j0 = by * T, k0 = bz * kchunk, k1 = min(k0 + kchunk, n2);
+    const int i = i0 + ti, j = j0 + tj;
+    const bool own = i < n0 && j < n1;
+    const int ic = min(i, n0 - 1), jc = min(j, n1 - 1);
+    double mx[R], my[R];
+    dcomplex bxv[R], byv[R], acc[R];
+#pragma unroll
+    for (int l = 0; l < R; ++l) {
+        mx[l] = Mx[ic * R + l];
+        bxv[l] = Bx[ic * R + l];
+        my[l] = My[jc * R + l];
+        byv[l] = By[jc * R + l];
+        acc[l] = {0.0, 0.0};
+    }
+    const long plane = (long)n0 * n1;
+    const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
+    // the plane tile of input plane K, each thread's share in registers: the next
+    // plane's loads are in flight while the current plane is computed
+    constexpr int NX = (H * H + 255) / 256;
+    dcomplex xv[NX];
+    auto load_plane = [&](int K) {
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + 256 * q;
+            const int ii = i0 - P + e % H, jj = j0 - P + e / H;
+            xv[q] = {0.0, 0.0};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) xv[q] = x[K * plane + (long)jj * n0 + ii];
+        }
+    };
+    load_plane(Kbeg);
+    for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int K = Kb + r;
+            if (K >= Kbeg && K < Kend) {   // block-uniform
+#pragma unroll
+                for (int q = 0; q < NX; ++q) {
+                    const int e = tid + 256 * q;
+                    if (e < H * H) Xs[e] = xv[q];
+                }
+                __syncthreads();
+                if (K + 1 < Kend) load_plane(K + 1);
+                for (int e = tid; e < H * T; e += 256) {   // e % T == ti: the same x band row
+                    const int jj = e / T;
+                    double tr = 0, tim = 0, ur = 0, ui = 0;
+#pragma unroll
+                    for (int l = 0; l < R; ++l) {
+                        const dcomplex v = Xs[jj * H + ti + l];
+                        tr += mx[l] * v.re;
+                        tim += mx[l] * v.im;
+                        ur += bxv[l].re * v.re - bxv[l].im * v.im;
+                        ui += bxv[l].re * v.im + bxv[l].im * v.re;
+                    }
+                    Ts[e] = {tr, tim};
+                    Us[e] = {ur, ui};
+                }
+                __syncthreads();
+                double ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+                for (int l = 0; l < R; ++l) {
+                    const dcomplex tv = Ts[(tj + l) * T + ti], uv = Us[(tj + l) * T + ti];
+                    ar += my[l] * uv.re + byv[l].re * tv.re - byv[l].im * tv.im;
+                    ai += my[l] * uv.im + byv[l].re * tv.im + byv[l].im * tv.re;
+                    br += my[l] * tv.re;
+                    bi += my[l] * tv.im;
+                }
+#pragma unroll
+                for (int d = -P; d <= P; ++d) {   // output k = K + d, band entry (k, K) = P - d
+                    const int k = K + d;
+                    if (k >= k0 && k < k1) {
+                        const double mz = Mz[k * R + P - d];
+                        const dcomplex bzv = Bz[k * R + P - d];
+                        dcomplex &o = acc[(r + d + P) % R];
+                        o.re += mz * ar + bzv.re * br - bzv.im * bi;
+                        o.im += mz * ai + bzv.re * bi + bzv.im * br;
+                    }
+                }
+            }
+            const int ko = K - P;   // complete: every input plane within P of it is done
+            if (ko >= k0 && ko < k1 && own) y[ko * plane + (long)j * n0 + i] = acc[r];
+            acc[r] = {0.0, 0.0};
+        }
+    }
 }
 
 template <typename R>
@@ -1255,6 +1367,35 @@ void launch_sep_apply(hipStream_t st, const SepBandsT<R> &sb, const cpx<R> *x, c
     const int n0 = sb.n[0], n1 = sb.n[1], n2 = sb.n[2];
     const long N = (long)n0 * n1 * n2;
     const int bs = 256;
+    if constexpr (std::is_same<R, double>::value) {
+        const int p = sb.p[0];
+        if (p == sb.p[1] && p == sb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p) {
+            // z chunks: one round of blocks at two per CU, each chunk at least 2P + 2 planes
+            static int ncu = 0;
+            if (!ncu) {
+                int dev = 0;
+                if (hipGetDevice(&dev) != hipSuccess ||
+                    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+                    ncu = 256;
+            }
+            const int tiles = ((n0 + 15) / 16) * ((n1 + 15) / 16);
+            const int chunks = std::max(1, std::min(2 * ncu / tiles, n2 / (2 * p + 2)));
+            const int kchunk = (n2 + chunks - 1) / chunks;
+            const unsigned nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
+            const double *Mz = sb.Mz2 ? sb.Mz2 : sb.M[2];
+#define TDZ_SEP3(PP)                                                                                             \
+    hipLaunchKernelGGL(sep3_kernel<PP>, dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, sb.M[0], sb.B[0], sb.M[1], \
+                       sb.B[1], Mz, sb.B[2], x, y)
+            switch (p) {
+            case 2: TDZ_SEP3(2); break;
+            case 3: TDZ_SEP3(3); break;
+            case 4: TDZ_SEP3(4); break;
+            default: TDZ_SEP3(5); break;
+            }
+#undef TDZ_SEP3
+            return;
+        }
+    }
     hipLaunchKernelGGL(sep_x_kernel<R>, dim3(grid_for(N, bs)), dim3(bs), 0, st, n0, N / n0, sb.p[0], sb.M[0], sb.B[0],
                        x, tmp[0], tmp[1]);
     hipLaunchKernelGGL(sep_y_kernel<R>, dim3(grid_for(N, bs)), dim3(bs), 0, st, n0, n1, N, sb.p[1], sb.M[1], sb.B[1],

@@ -56,8 +56,10 @@ def test_predicted_steps_match_oracle(pair, order):
     try:
         d = ctx.vec(psi0)
         its, _ = ctx.evolve_cn(d, 0.0, dt, nsteps, pulse, rtol=1e-13)
+        # each of the 8 solves meets rtol 1e-13 on its residual; the state error it leaves
+        # is that times the condition of M + i dt/2 H (a few here), summed over the steps
         err = _l2(o, d.download() - ref) / _l2(o, ref)
-        assert err < 1e-12, (name, order, err, its)
+        assert err < 1e-11, (name, order, err, its)
         # and the guess pays: no more iterations than starting from the current state
         d0 = ctx.vec(psi0)
         ctx.set_cn_predictor(0)
