@@ -1770,6 +1770,91 @@ __global__ void zline_solve_kernel(int n0, int n1, int n2, int p, const R *__res
         x[line + nl * k] = cdivR(s, W(k, p));
     }
 }
+
+// The same banded elimination with the active rows in registers: rows k .. k+P of the
+// matrix, columns k .. k+2P, slide down the line; row k+P+1 enters from the band tables
+// (shared by all lines, cached), the finished row k of U and the eliminated x_k go to
+// `work` and `x` once, and the back substitution reads them once. The operations and
+// their order are those of zline_solve_kernel; only the storage moves out of the
+// dependency chain (global-memory round trips per update made the line solves
+// latency-bound: 1.5 ms at 68 x 68 x 260).
+template <typename R, int P>
+__global__ void __launch_bounds__(64) zline_reg_kernel(int n0, int n1, int n2, const R *__restrict__ lx,
+                                                      const R *__restrict__ ly,
+                                 const cpx<R> *__restrict__ Bz, const R *__restrict__ Mw, cpx<R> g,
+                                 const cpx<R> *__restrict__ rhs, cpx<R> *__restrict__ x, cpx<R> *__restrict__ work)
+{
+    constexpr int BW = 2 * P + 1;
+    const long nl = (long)n0 * n1;
+    const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nl) return;
+    const int i = (int)(line % n0), j = (int)(line / n0);
+    const R lam = lx[i] + ly[j];
+    const cpx<R> cg = {g.re * lam, g.im * lam};
+    auto entry = [&](int r, int col) -> cpx<R> {   // A(r, col), zero outside the matrix and the band
+        if (r >= n2 || col < 0 || col >= n2 || col - r > P || r - col > P) return {0, 0};
+        const cpx<R> b = Bz[(long)r * BW + (col - r + P)];
+        const R w = Mw[(long)r * BW + (col - r + P)];
+        return {b.re + cg.re * w, b.im + cg.im * w};
+    };
+    cpx<R> W[P + 1][BW];   // W[rr][cc] = A(k + rr, k + cc)
+    cpx<R> xw[P + 1];      // x at rows k .. k+P
+#pragma unroll
+    for (int rr = 0; rr <= P; ++rr) {
+#pragma unroll
+        for (int cc = 0; cc < BW; ++cc) W[rr][cc] = entry(rr, cc);
+        xw[rr] = rr < n2 ? rhs[line + nl * rr] : cpx<R>{0, 0};
+    }
+    for (int k = 0; k < n2; ++k) {
+        const cpx<R> piv = W[0][0];
+#pragma unroll
+        for (int rr = 1; rr <= P; ++rr) {
+            if (k + rr >= n2) break;
+            const cpx<R> f = cdivR(W[rr][0], piv);
+#pragma unroll
+            for (int m2 = 1; m2 <= P; ++m2) {
+                if (k + m2 >= n2) break;
+                const cpx<R> t = cmulR(f, W[0][m2]);
+                W[rr][m2] = {W[rr][m2].re - t.re, W[rr][m2].im - t.im};
+            }
+            const cpx<R> t = cmulR(f, xw[0]);
+            xw[rr] = {xw[rr].re - t.re, xw[rr].im - t.im};
+        }
+        // row k of U (diagonal and P super-diagonals) and the eliminated x_k
+#pragma unroll
+        for (int m = 0; m <= P; ++m) work[((long)k * (P + 1) + m) * nl + line] = W[0][m];
+        x[line + nl * k] = xw[0];
+        // slide: rows k+1 .. k+P move up one row and left one column, row k+P+1 enters
+#pragma unroll
+        for (int rr = 0; rr < P; ++rr) {
+#pragma unroll
+            for (int cc = 0; cc < BW - 1; ++cc) W[rr][cc] = W[rr + 1][cc + 1];
+            W[rr][BW - 1] = {0, 0};
+            xw[rr] = xw[rr + 1];
+        }
+        const int rn = k + P + 1;
+#pragma unroll
+        for (int cc = 0; cc < BW; ++cc) W[P][cc] = entry(rn, k + 1 + cc);
+        xw[P] = rn < n2 ? rhs[line + nl * rn] : cpx<R>{0, 0};
+    }
+    cpx<R> xs[P];   // x at rows k+1 .. k+P of the back substitution
+#pragma unroll
+    for (int m = 0; m < P; ++m) xs[m] = {0, 0};
+    for (int k = n2 - 1; k >= 0; --k) {
+        cpx<R> s = x[line + nl * k];
+#pragma unroll
+        for (int m2 = 1; m2 <= P; ++m2) {
+            if (k + m2 >= n2) break;
+            const cpx<R> t = cmulR(work[((long)k * (P + 1) + m2) * nl + line], xs[m2 - 1]);
+            s = {s.re - t.re, s.im - t.im};
+        }
+        const cpx<R> xk = cdivR(s, work[((long)k * (P + 1)) * nl + line]);
+        x[line + nl * k] = xk;
+#pragma unroll
+        for (int m = P - 1; m > 0; --m) xs[m] = xs[m - 1];
+        xs[0] = xk;
+    }
+}
 }  // namespace
 
 template <typename R>
@@ -1777,8 +1862,17 @@ void launch_zline_solve(hipStream_t st, const int n[3], int p, const R *lx, cons
                         const R *Mw, cpx<R> g, const cpx<R> *rhs, cpx<R> *x, cpx<R> *work)
 {
     const long nl = (long)n[0] * n[1];
-    hipLaunchKernelGGL(zline_solve_kernel<R>, dim3((unsigned)((nl + 63) / 64)), dim3(64), 0, st, n[0], n[1], n[2], p,
-                       lx, ly, Bz, Mw, g, rhs, x, work);
+    const dim3 grid((unsigned)((nl + 63) / 64));
+    switch (p) {   // work holds n2 (p+1) entries per line here (of the n2 (2p+1) allocated)
+    case 1: hipLaunchKernelGGL((zline_reg_kernel<R, 1>), grid, dim3(64), 0, st, n[0], n[1], n[2], lx, ly, Bz, Mw, g, rhs, x, work); return;
+    case 2: hipLaunchKernelGGL((zline_reg_kernel<R, 2>), grid, dim3(64), 0, st, n[0], n[1], n[2], lx, ly, Bz, Mw, g, rhs, x, work); return;
+    case 3: hipLaunchKernelGGL((zline_reg_kernel<R, 3>), grid, dim3(64), 0, st, n[0], n[1], n[2], lx, ly, Bz, Mw, g, rhs, x, work); return;
+    case 4: hipLaunchKernelGGL((zline_reg_kernel<R, 4>), grid, dim3(64), 0, st, n[0], n[1], n[2], lx, ly, Bz, Mw, g, rhs, x, work); return;
+    case 5: hipLaunchKernelGGL((zline_reg_kernel<R, 5>), grid, dim3(64), 0, st, n[0], n[1], n[2], lx, ly, Bz, Mw, g, rhs, x, work); return;
+    default: break;
+    }
+    hipLaunchKernelGGL(zline_solve_kernel<R>, grid, dim3(64), 0, st, n[0], n[1], n[2], p, lx, ly, Bz, Mw, g, rhs, x,
+                       work);
 }
 
 #define TDZ_INSTANTIATE(R)                                                                                          \
