@@ -1331,8 +1331,8 @@ bool launch_pot_apply_2d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot,
 
 int pot3d_variants(int p, int Q)
 {
-    if (p == 4 && Q == 8) return 2;
-    if (p == 3 && Q == 8) return 2;
+    if (p == 4 && Q == 8) return 4;
+    if (p == 3 && Q == 8) return 4;
     if (p == 3 && Q == 5) return 1;
     return 0;
 }
@@ -1346,15 +1346,24 @@ bool launch_pot_apply_3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot,
         if (ax[d].p != p || ax[d].Q != Q) return false;
     if (pot.kind != 1 && pot.kind != 2) return false;
     if (variant < 0 || variant >= pot3d_variants(p, Q)) return false;
-    // variant 0: 4 x 4 element tiles, 1024 threads; variant 1: 2 x 2 tiles, 256 threads
+    // variant 0: 4 x 4 element tiles, 1024 threads; 1: 2 x 2 tiles, 256 threads; 2: 2 x 2
+    // tiles, 128 threads of 2 point columns; 3: 4 x 4 tiles, 256 threads of 4 columns
     if (p == 4 && Q == 8) {
-        if (variant == 0) launch_pot3d<R, 4, 8, 4, 1>(st, ax, pot, gr, gi, x, y);
-        else launch_pot3d<R, 4, 8, 2, 1>(st, ax, pot, gr, gi, x, y);
+        switch (variant) {
+        case 0: launch_pot3d<R, 4, 8, 4, 1>(st, ax, pot, gr, gi, x, y); break;
+        case 1: launch_pot3d<R, 4, 8, 2, 1>(st, ax, pot, gr, gi, x, y); break;
+        case 2: launch_pot3d<R, 4, 8, 2, 2>(st, ax, pot, gr, gi, x, y); break;
+        default: launch_pot3d<R, 4, 8, 4, 4>(st, ax, pot, gr, gi, x, y); break;
+        }
         return true;
     }
     if (p == 3 && Q == 8) {
-        if (variant == 0) launch_pot3d<R, 3, 8, 4, 1>(st, ax, pot, gr, gi, x, y);
-        else launch_pot3d<R, 3, 8, 2, 1>(st, ax, pot, gr, gi, x, y);
+        switch (variant) {
+        case 0: launch_pot3d<R, 3, 8, 4, 1>(st, ax, pot, gr, gi, x, y); break;
+        case 1: launch_pot3d<R, 3, 8, 2, 1>(st, ax, pot, gr, gi, x, y); break;
+        case 2: launch_pot3d<R, 3, 8, 2, 2>(st, ax, pot, gr, gi, x, y); break;
+        default: launch_pot3d<R, 3, 8, 4, 4>(st, ax, pot, gr, gi, x, y); break;
+        }
         return true;
     }
     if (p == 3 && Q == 5) { launch_pot3d<R, 3, 5, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
