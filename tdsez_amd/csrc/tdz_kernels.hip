@@ -442,7 +442,9 @@ pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr,
             unsafeAtomicAdd(dst, sr);
             unsafeAtomicAdd(dst + 1, si);
         }
-        __syncthreads();
+        // no barrier here: the next writes of H1 and G2p follow the next emit's first
+        // barrier, those of cin and the z tables (stage_in) only replace data whose
+        // last readers ran before this emit's barriers
     };
 
     for (int s = 0; s < P; ++s) stage_in(s, s, -1);
