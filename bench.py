@@ -75,7 +75,8 @@ def main():
                          "scan; hetero: the layered heterostructure with Magnus-4")
     ap.add_argument("--runs", type=int, default=64, help="scan: number of runs")
     ap.add_argument("--seed", type=int, default=7, help="scan: seed of the drawn grids and pulses")
-    ap.add_argument("--kiters", type=int, default=4, help="scan: FGMRES iterations captured per step")
+    ap.add_argument("--kiters", type=int, default=None,
+                    help="scan: FGMRES iterations captured per step (default 2 with the predictor, else 4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -236,13 +237,18 @@ def scan(args):
     dt = 0.02
     pulses = [T.make_pulse(E0=(E0, 0.0, 0.0), omega=0.057, ncycles=3.0, cep=cep) for _, E0, cep in runs]
     t0 = time.perf_counter()
+    if args.kiters is None:
+        args.kiters = 2 if args.predictor else 4
     b = T.Batch([d for d, _, _ in runs], pulses, dt, args.warmup + args.steps, pc=T.PC_SEPARABLE_MIXED,
                 kiters=args.kiters)
+    if args.predictor:
+        b.set_predictor(args.predictor)
     for r, ctx in enumerate(b.runs):
         ctx.project_gaussian(b.psi[r], (0.0, 0.0), (1.0, 1.0))
         ctx.ground_state(b.psi[r], sigma=-0.9, tol=1e-8, maxit=60)
     setup = time.perf_counter() - t0
     b.evolve(args.warmup, rtol=args.rtol)          # the first call captures the graph
+    fix0 = b.fixups()
     t0 = time.perf_counter()
     its, res = b.evolve(args.steps, rtol=args.rtol)
     el = time.perf_counter() - t0
@@ -255,7 +261,10 @@ def scan(args):
         "data": f"synthetic: {args.runs} runs drawn from seed {args.seed}, ground states at set-up",
         "config": {"workload": f"batched 2D scan, {args.runs} runs, {dofs} dofs in total, CN dt=0.02, "
                                f"rtol={args.rtol}, {args.kiters} captured FGMRES iterations per step",
-                   "setup_s": setup, "krylov_its_per_step": float(its.mean()), "host_completions": b.fixups(),
+                   "cn_guess": ("current state" if args.predictor == 0 else
+                                f"binomial extrapolation of the last {args.predictor + 1} states"),
+                   "setup_s": setup, "krylov_its_per_step": float(its.mean()),
+                   "host_completions": b.fixups() - fix0, "host_completions_warmup": fix0,
                    "max_residual": float(res.max())},
         "roofline": None, "cpu_baseline": None,
     }

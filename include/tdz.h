@@ -231,6 +231,11 @@ int tdz_batch_create(const tdz_problem *probs, const tdz_pulse *pulses, int nrun
 int tdz_batch_context(tdz_batch *b, int run, tdz_ctx **ctx, double **psi);
 int tdz_batch_evolve(tdz_batch *b, int nsteps, double rtol, int maxit, int *its, double *res);
 int tdz_batch_fixups(const tdz_batch *b, long *fixups);
+/* Initial guess of the captured steps (before the first tdz_batch_evolve): order 1 or 2
+ * starts each run's solve from the binomial extrapolation of its last 2 or 3 states
+ * (as tdz_set_cn_predictor; its residual then takes one more operator apply in the
+ * graph), order 0 (default) from the current state. */
+int tdz_batch_set_predictor(tdz_batch *b, int order);
 int tdz_batch_destroy(tdz_batch *b);
 
 /* Measurement hooks (no reference counterpart; used by bench.py): HIP events

@@ -29,7 +29,7 @@ EXPORTS = [
     "tdz_create", "tdz_destroy", "tdz_last_error", "tdz_size", "tdz_dims", "tdz_levels",
     "tdz_vec_alloc", "tdz_vec_free", "tdz_vec_upload", "tdz_vec_download", "tdz_sync",
     "tdz_set_shift", "tdz_op_apply", "tdz_term_apply", "tdz_pc_setup", "tdz_pc_apply",
-    "tdz_ksp_solve", "tdz_step_cn", "tdz_evolve_cn", "tdz_set_cn_predictor", "tdz_observables", "tdz_pulse_coupling",
+    "tdz_ksp_solve", "tdz_step_cn", "tdz_evolve_cn", "tdz_set_cn_predictor", "tdz_batch_set_predictor", "tdz_observables", "tdz_pulse_coupling",
     "tdz_timing", "tdz_timing_stats", "tdz_step_pade22", "tdz_evolve_rational",
     "tdz_project_gaussian", "tdz_ground_state",
     "tdz_trace_row", "tdz_pade_history", "tdz_batch_create", "tdz_batch_context", "tdz_batch_evolve",
@@ -139,6 +139,7 @@ def lib():
         L.tdz_step_cn.argtypes = [vp, vp, d, dp, d, i, C.POINTER(KspStats)]
         L.tdz_evolve_cn.argtypes = [vp, vp, d, d, i, C.POINTER(Pulse), d, i, dp, ip]
         L.tdz_set_cn_predictor.argtypes = [vp, i]
+        L.tdz_batch_set_predictor.argtypes = [vp, i]
         L.tdz_observables.argtypes = [vp, vp, dp]
         L.tdz_pulse_coupling.argtypes = [vp, C.POINTER(Pulse), d, dp]
         L.tdz_step_pade22.argtypes = [vp, vp, d, dp, i, d, i, C.POINTER(KspStats)]
@@ -448,6 +449,11 @@ class Batch:
         f = C.c_long(0)
         _chk(lib().tdz_batch_fixups(self.h, C.byref(f)))
         return f.value
+
+    def set_predictor(self, order):
+        """Initial guess of the captured steps (before the first evolve): 0 the state,
+        1/2 the binomial extrapolation of the last 2/3 states."""
+        _chk(lib().tdz_batch_set_predictor(self.h, int(order)))
 
     def close(self):
         if getattr(self, "h", None):

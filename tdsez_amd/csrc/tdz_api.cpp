@@ -184,6 +184,8 @@ struct tdz_ctx {
         double *Xb[3] = {}, *M0 = nullptr;
         int nsteps = 0, k = 0;
         double dt = 0;
+        int pred = 0;                          // extrapolation order of the guess (0: the state)
+        dcomplex *h1 = nullptr, *h2 = nullptr;   // the two states before the current one
     } cap;
     // CN predictor: the last states of a run with their field-free operator products and
     // dipole products, for the extrapolated initial guess and its residual
@@ -2547,14 +2549,27 @@ void record_cn_step(tdz_ctx *c, dcomplex *x)
     for (int d = 0; d < 3; ++d) L.d.B[d] = C.bandR[d];
     c->g = -g;
     level_apply(c, L, x, c->rhs);
-    tdz::SepBandsT<double> mb = sep_bands<double>(L);
-    for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
-    tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
-    const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-    if (c->prob.dirichlet) tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
     dcomplex *V0 = c->Vb;
-    HIPCHK(hipMemcpyAsync(V0, c->rhs, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
-    tdz::launch_axpby(c->st, {-2.0, 0}, c->sd.w1, {2.0, 0}, V0, N);
+    if (C.pred > 0) {
+        // guess: the extrapolation from this state and the two before (as many as the run
+        // has taken steps), then r0 = rhs - (M + g H) guess by one operator apply
+        tdz::launch_extrapolate(c->st, C.step, C.pred, x, C.h1, C.h2, c->sd.w2, N);
+        HIPCHK(hipMemcpyAsync(C.h2, C.h1, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(C.h1, x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(x, c->sd.w2, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+        for (int d = 0; d < 3; ++d) L.d.B[d] = C.bandL[d];
+        c->g = g;
+        level_apply(c, L, x, V0);
+        tdz::launch_axpby(c->st, {1.0, 0}, c->rhs, {-1.0, 0}, V0, N);
+    } else {   // r0 = 2 (rhs - M x): the guess is the state
+        tdz::SepBandsT<double> mb = sep_bands<double>(L);
+        for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
+        tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
+        const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+        if (c->prob.dirichlet) tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
+        HIPCHK(hipMemcpyAsync(V0, c->rhs, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+        tdz::launch_axpby(c->st, {-2.0, 0}, c->sd.w1, {2.0, 0}, V0, N);
+    }
     tdz::launch_norm2(c->st, c->rhs, N, c->dpartr, C.bnorm2);
     tdz::launch_norm2(c->st, V0, N, c->dpartr, c->dnorm);
     tdz::launch_gm_start(c->st, c->dnorm, C.bnorm2, C.gm);
@@ -2680,6 +2695,27 @@ int tdz_batch_create(const tdz_problem *probs, const tdz_pulse *pulses, int nrun
     }
     *out = b;
     return 0;
+}
+
+int tdz_batch_set_predictor(tdz_batch *b, int order)
+{
+    return guarded([&] {
+        if (!b) throw Error("null batch");
+        if (order < 0 || order > 2) throw Error("CN predictor order must be 0, 1 or 2");
+        if (b->exec) throw Error("batch: the step is already captured (set the predictor before the first evolve)");
+        for (tdz_ctx *c : b->runs) {
+            auto &C = c->cap;
+            C.pred = order;
+            if (order > 0 && !C.h1) {
+                const long N = c->lv[0]->N;
+                C.h1 = track(c, dalloc<dcomplex>(N));
+                C.h2 = track(c, dalloc<dcomplex>(N));
+                HIPCHK(hipMemsetAsync(C.h1, 0, N * sizeof(dcomplex), c->st));
+                HIPCHK(hipMemsetAsync(C.h2, 0, N * sizeof(dcomplex), c->st));
+                HIPCHK(hipStreamSynchronize(c->st));
+            }
+        }
+    });
 }
 
 int tdz_batch_destroy(tdz_batch *b)

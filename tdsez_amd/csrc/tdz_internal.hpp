@@ -203,6 +203,9 @@ void launch_gm_arnoldi(hipStream_t st, int j, int m, const dcomplex *h1, const d
                        GmState *g);
 void launch_gm_solve(hipStream_t st, int k, int m, GmState *g, dcomplex *y, double *res, const int *step);
 void launch_step_advance(hipStream_t st, int *step);
+// out = binomial extrapolation of order min(order, *step) from x, h1, h2 (device step)
+void launch_extrapolate(hipStream_t st, const int *step, int order, const dcomplex *x, const dcomplex *h1,
+                        const dcomplex *h2, dcomplex *out, long N);
 void launch_field_band(hipStream_t st, long n, const double *M, const dcomplex *T0, const double *X, dcomplex g,
                        const double *sE, const int *step, int d, dcomplex *B);
 

@@ -1205,6 +1205,22 @@ __global__ void lincomb_kernel(LinComb lc, dcomplex *y, long N)
     }
 }
 
+// out = the binomial extrapolation of order min(order, *step) from x (the current state)
+// and h1, h2 (the two before): weights (1), (2, -1), (3, -3, 1). The order comes from the
+// device step counter, so a captured step extrapolates only over states it has.
+__global__ void extrap_kernel(const int *__restrict__ step, int order, const dcomplex *__restrict__ x,
+                              const dcomplex *__restrict__ h1, const dcomplex *__restrict__ h2,
+                              dcomplex *__restrict__ out, long N)
+{
+    const int o = min(order, *step);
+    const double w0 = o == 0 ? 1.0 : (o == 1 ? 2.0 : 3.0), w1 = o == 0 ? 0.0 : (o == 1 ? -1.0 : -3.0),
+                 w2 = o == 2 ? 1.0 : 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        const dcomplex a = x[i], b = h1[i], c = h2[i];
+        out[i] = {w0 * a.re + w1 * b.re + w2 * c.re, w0 * a.im + w1 * b.im + w2 * c.im};
+    }
+}
+
 template <typename R>
 __global__ void diag_scale_kernel(int n0, int n1, long N, const R *l0, const R *l1, const R *l2,
                                   const unsigned char *w0, const unsigned char *w1, const unsigned char *w2,
@@ -1449,6 +1465,13 @@ void launch_lincomb(hipStream_t st, const LinComb &lc, dcomplex *y, long N)
 {
     if (lc.k < 1 || lc.k > LinComb::kMax) throw std::runtime_error("lincomb: 1 .. kMax terms");
     hipLaunchKernelGGL(lincomb_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, lc, y, N);
+}
+
+void launch_extrapolate(hipStream_t st, const int *step, int order, const dcomplex *x, const dcomplex *h1,
+                        const dcomplex *h2, dcomplex *out, long N)
+{
+    hipLaunchKernelGGL(extrap_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, step, order, x, h1, h2, out,
+                       N);
 }
 
 void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dcomplex *y, long N)

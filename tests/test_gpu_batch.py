@@ -41,15 +41,18 @@ SCAN = [(_scan_desc((16, 16), 3), dict(E0=(0.05, 0, 0), cep=0.0)),
         (_scan_desc((12, 14), 5), dict(E0=(0.02, 0, 0), cep=4.0))]
 
 
-@pytest.mark.parametrize("pc,kiters", [(1, 4), (3, 4), (3, 1)])
-def test_batch_matches_single_runs_and_oracle(pc, kiters):
-    """kiters = 1 leaves most steps to the host completion path."""
+@pytest.mark.parametrize("pc,kiters,pred", [(1, 4, 0), (3, 4, 0), (3, 1, 0), (3, 2, 2), (1, 3, 1)])
+def test_batch_matches_single_runs_and_oracle(pc, kiters, pred):
+    """kiters = 1 leaves most steps to the host completion path; pred > 0 starts each
+    captured solve from the extrapolated guess (the same systems, the same tolerance)."""
     import tdsez_amd as T
     from oracle import Oracle
     descs = [d for d, _ in SCAN]
     pulses = [T.make_pulse(E0=kw["E0"], omega=0.5, ncycles=1.0, cep=kw["cep"]) for _, kw in SCAN]
     dt, nsteps = 0.05, 12
     b = T.Batch(descs, pulses, dt, nsteps, pc=pc, kiters=kiters)
+    if pred:
+        b.set_predictor(pred)
     starts = []
     for r, ctx in enumerate(b.runs):
         ctx.project_gaussian(b.psi[r], (0.5, -0.3), (1.2, 1.0), (0.2, 0.1))
@@ -58,6 +61,9 @@ def test_batch_matches_single_runs_and_oracle(pc, kiters):
     assert res.max() <= 1e-13
     if kiters == 1:
         assert b.fixups() > 0
+    if pred:
+        with pytest.raises(RuntimeError):   # the step is captured now
+            b.set_predictor(0)
     for r, (desc, pulse) in enumerate(zip(descs, pulses)):
         got = b.psi[r].download()
         ctx = T.Context(desc)
