@@ -160,10 +160,22 @@ def main():
         base["speedup"] = steps_per_s / base["steps_per_s"]
         base["psi_rel_l2_diff"] = mnorm(psi_end - psi_b) / mnorm(psi_b)
 
+    netfree = None
+    if args.pc == "learned32" and not args.no_baseline:
+        # the same solve without the network remainder (first-order corrected separable
+        # solve with the lumped potential): what the network adds at this tolerance
+        sps_c, its_c, _, _, _ = run(T.PC_CORRECTED, False)
+        netfree = {"preconditioner": PCS["corr32"], "steps_per_s": sps_c, "krylov_its_per_step": float(np.mean(its_c))}
+        print(f"# network-free corrected solve: {sps_c:.2f} steps/s, its/step {np.mean(its_c):.1f}",
+              file=sys.stderr, flush=True)
+
     plain = None
     if args.predictor > 0 and not args.no_plain_guess:   # the same legs with the current state as guess
         sps0, its0, _, _, _ = run(kinds[args.pc], False, 0)
         plain = {"steps_per_s": sps0, "krylov_its_per_step": float(np.mean(its0))}
+        if netfree is not None:
+            sps0c, its0c, _, _, _ = run(T.PC_CORRECTED, False, 0)
+            plain.update({"network_free_steps_per_s": sps0c, "network_free_krylov_its_per_step": float(np.mean(its0c))})
         if base is not None:
             sps0b, its0b, _, _, _ = run(T.PC_MG, False, 0)
             plain.update({"fp64_mg_steps_per_s": sps0b, "fp64_mg_krylov_its_per_step": float(np.mean(its0b)),
@@ -207,6 +219,7 @@ def main():
         "fp64_mg_baseline": base,
         "without_predictor": plain,
         "learned": learn,
+        "network_free": netfree,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
