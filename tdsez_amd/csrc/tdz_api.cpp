@@ -195,7 +195,9 @@ struct tdz_ctx {
         double dt = 0;
         unsigned axes = 0;          // field axes whose dipole products are kept
         dcomplex *psi[3] = {}, *P[3] = {}, *D[3][3] = {};   // state; (M + g H_0) state; D_a state
+        dcomplex *Mx[3] = {};                                // M state
         dcomplex *Xband[3] = {}, *Zband[3] = {};   // complex position bands and zero bands (dipole applies)
+        double *Xz = nullptr;                      // the real z position band (fused mass + z dipole)
     } pred;
     // chained CN right-hand side: A psi of the last solve of an evolve run
     dcomplex *ax = nullptr, *zerov = nullptr, *dDip[3] = {};
@@ -1575,11 +1577,32 @@ void pred_push(tdz_ctx *c, const dcomplex *x, const dcomplex *ax, const double s
         p.P[h] = track(c, dalloc<dcomplex>(N));
     }
     HIPCHK(hipMemcpyAsync(p.psi[h], x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+    if (!p.Mx[h]) p.Mx[h] = track(c, dalloc<dcomplex>(N));
     for (int a = 0; a < 3; ++a)
-        if (p.axes >> a & 1) {
-            if (!p.D[h][a]) p.D[h][a] = track(c, dalloc<dcomplex>(N));
-            dipole_apply(c, a, x, p.D[h][a]);
+        if ((p.axes >> a & 1) && !p.D[h][a]) p.D[h][a] = track(c, dalloc<dcomplex>(N));
+    // M x and D_a x: one fused pass for a field along z alone, else one apply each
+    bool fused = false;
+    if (p.axes == 4u && c->prob.dim == 3 && L.ax[0].p == L.ax[2].p && L.ax[1].p == L.ax[2].p) {
+        if (!p.Xz) p.Xz = upload(c, L.Xb[2]);
+        const double *M[3] = {L.d.M[0], L.d.M[1], L.d.M[2]};
+        fused = tdz::launch_mass_dipole_z(c->st, L.n, L.ax[2].p, M, p.Xz, x, p.Mx[h], p.D[h][2]);
+        if (fused && c->prob.dirichlet) {   // walls: identity rows of M, no field
+            if (!c->zerov) c->zerov = upload(c, std::vector<dcomplex>(L.N, dcomplex{0, 0}));
+            const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+            tdz::launch_wall_fix<double>(c->st, L.n, wall, c->zerov, p.D[h][2]);
         }
+    }
+    if (!fused) {
+        for (int a = 0; a < 3; ++a)
+            if (p.axes >> a & 1) dipole_apply(c, a, x, p.D[h][a]);
+        tdz::SepBandsT<double> mb = sep_bands<double>(L);
+        for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
+        tdz::launch_sep_apply<double>(c->st, mb, x, p.Mx[h], c->sd.tmp);
+    }
+    if (c->prob.dirichlet) {
+        const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+        tdz::launch_wall_fix<double>(c->st, L.n, wall, x, p.Mx[h]);
+    }
     if (ax) {
         tdz::LinComb lc;
         lc.add({1, 0}, ax);
@@ -1643,24 +1666,14 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
     }
     auto slot = [&](int j) { return (p.head - j + 3) % 3; };
     const int h0 = slot(0);
-    // M psi_0 (walls: identity rows)
-    tdz::SepBandsT<double> mb = sep_bands<double>(L);
-    for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
-    tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
-    if (c->prob.dirichlet) {
-        const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-        tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
-    }
-    tdz::LinComb rb;   // b = 2 M psi_0 - P_0 - g sE.D psi_0
-    rb.add({2, 0}, c->sd.w1);
+    tdz::LinComb rb;   // b = 2 M psi_0 - P_0 - g sE.D psi_0 (M psi_0 kept at its push)
+    rb.add({2, 0}, p.Mx[h0]);
     rb.add({-1, 0}, p.P[h0]);
     for (int a = 0; a < 3; ++a)
         if (p.axes >> a & 1) rb.add(dc(-g * sE[a]), p.D[h0][a]);
-    tdz::launch_lincomb(c->st, rb, c->rhs, N);
     static const double W[3][3] = {{1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
     const int o = std::min(p.order, p.count - 1);
-    tdz::LinComb r0, x0;
-    r0.add({1, 0}, c->rhs);
+    tdz::LinComb r0, x0;   // r0 = b - sum_j w_j A psi_j;  x0 = sum_j w_j psi_j
     for (int j = 0; j <= o; ++j) {
         const int s = slot(j);
         const double w = W[o][j];
@@ -1669,8 +1682,8 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
             if (p.axes >> a & 1) r0.add(dc(-w * g * sE[a]), p.D[s][a]);
         x0.add({w, 0}, p.psi[s]);
     }
-    tdz::launch_lincomb(c->st, r0, c->Vb, N);   // the solve's first basis vector before scaling
-    tdz::launch_lincomb(c->st, x0, x, N);
+    // b, the solve's first basis vector before scaling, and the guess in one pass
+    tdz::launch_lincomb3(c->st, rb, r0, x0, c->rhs, c->Vb, x, N);
     set_shift(c, 1.0, g, sE);
     step_solve(c, c->rhs, x, rtol, maxit, st, true, c->ax);
     pred_push(c, x, c->ax, sE);

@@ -137,6 +137,13 @@ struct LinComb {
     void add(dcomplex coef, const dcomplex *vec) { c[k] = coef; v[k++] = vec; }
 };
 void launch_lincomb(hipStream_t st, const LinComb &lc, dcomplex *y, long N);
+// y1 = sum l1; y2 = y1 + sum l2; y3 = sum l3, one pass
+void launch_lincomb3(hipStream_t st, const LinComb &l1, const LinComb &l2, const LinComb &l3, dcomplex *y1,
+                     dcomplex *y2, dcomplex *y3, long N);
+// m = Mz My Mx x and dz = Xz My Mx x (FP64, real bands of order p on every axis, 3D);
+// false if not applicable
+bool launch_mass_dipole_z(hipStream_t st, const int n[3], int p, const double *M[3], const double *Xz,
+                          const dcomplex *x, dcomplex *m, dcomplex *dz);
 template <typename R>
 void launch_axpby_t(hipStream_t st, cpx<R> a, const cpx<R> *x, cpx<R> b, cpx<R> *y, long N);
 // precision conversions

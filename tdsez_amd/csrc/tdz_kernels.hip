@@ -1221,6 +1221,135 @@ __global__ void extrap_kernel(const int *__restrict__ step, int order, const dco
     }
 }
 
+// y1 = sum lc1; y2 = y1 + sum lc2; y3 = sum lc3 (the predictor's right-hand side, guess
+// residual and guess in one pass)
+__global__ void lincomb3_kernel(LinComb l1, LinComb l2, LinComb l3, dcomplex *y1, dcomplex *y2, dcomplex *y3, long N)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        double ar = 0, ai = 0, br = 0, bi = 0, cr = 0, ci = 0;
+        for (int k = 0; k < l1.k; ++k) {
+            const dcomplex v = l1.v[k][i], c = l1.c[k];
+            ar += c.re * v.re - c.im * v.im;
+            ai += c.re * v.im + c.im * v.re;
+        }
+        br = ar;
+        bi = ai;
+        for (int k = 0; k < l2.k; ++k) {
+            const dcomplex v = l2.v[k][i], c = l2.c[k];
+            br += c.re * v.re - c.im * v.im;
+            bi += c.re * v.im + c.im * v.re;
+        }
+        for (int k = 0; k < l3.k; ++k) {
+            const dcomplex v = l3.v[k][i], c = l3.c[k];
+            cr += c.re * v.re - c.im * v.im;
+            ci += c.re * v.im + c.im * v.re;
+        }
+        y1[i] = {ar, ai};
+        y2[i] = {br, bi};
+        y3[i] = {cr, ci};
+    }
+}
+
+// Mass and z dipole of the same vector in one z-marching pass (FP64, equal orders P):
+//   m = Mz My Mx x,  d = Xz My Mx x
+// (real bands; the x and y passes are shared). Same tiling, plane prefetch and output
+// rings as sep3_kernel.
+template <int P>
+__global__ void __launch_bounds__(256) mass_dip_kernel(int n0, int n1, int n2, int kchunk,
+                                                       const double *__restrict__ Mx, const double *__restrict__ My,
+                                                       const double *__restrict__ Mz, const double *__restrict__ Xz,
+                                                       const dcomplex *__restrict__ x, dcomplex *__restrict__ m,
+                                                       dcomplex *__restrict__ dz)
+{
+    constexpr int R = 2 * P + 1, T = 16, H = T + 2 * P;
+    __shared__ dcomplex Xs[H * H];
+    __shared__ dcomplex Ts[H * T];
+    const int tid = threadIdx.x, ti = tid % T, tj = tid / T;
+    const int tilesx = (n0 + T - 1) / T, tilesy = (n1 + T - 1) / T;
+    int b = blockIdx.x;
+    const int bx = b % tilesx;
+    b /= tilesx;
+    const int by = b % tilesy, bz = b / tilesy;
+    const int i0 = bx * T, j0 = by * T, k0 = bz * kchunk, k1 = min(k0 + kchunk, n2);
+    const int i = i0 + ti, j = j0 + tj;
+    const bool own = i < n0 && j < n1;
+    const int ic = min(i, n0 - 1), jc = min(j, n1 - 1);
+    double mx[R], my[R];
+    dcomplex am[R], ad[R];
+#pragma unroll
+    for (int l = 0; l < R; ++l) {
+        mx[l] = Mx[ic * R + l];
+        my[l] = My[jc * R + l];
+        am[l] = ad[l] = {0.0, 0.0};
+    }
+    const long plane = (long)n0 * n1;
+    const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
+    constexpr int NX = (H * H + 255) / 256;
+    dcomplex xv[NX];
+    auto load_plane = [&](int K) {
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + 256 * q;
+            const int ii = i0 - P + e % H, jj = j0 - P + e / H;
+            xv[q] = {0.0, 0.0};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) xv[q] = x[K * plane + (long)jj * n0 + ii];
+        }
+    };
+    load_plane(Kbeg);
+    for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int K = Kb + r;
+            if (K >= Kbeg && K < Kend) {   // block-uniform
+#pragma unroll
+                for (int q = 0; q < NX; ++q) {
+                    const int e = tid + 256 * q;
+                    if (e < H * H) Xs[e] = xv[q];
+                }
+                __syncthreads();
+                if (K + 1 < Kend) load_plane(K + 1);
+                for (int e = tid; e < H * T; e += 256) {   // e % T == ti
+                    const int jj = e / T;
+                    double tr = 0, tim = 0;
+#pragma unroll
+                    for (int l = 0; l < R; ++l) {
+                        const dcomplex v = Xs[jj * H + ti + l];
+                        tr += mx[l] * v.re;
+                        tim += mx[l] * v.im;
+                    }
+                    Ts[e] = {tr, tim};
+                }
+                __syncthreads();
+                double ar = 0, ai = 0;
+#pragma unroll
+                for (int l = 0; l < R; ++l) {
+                    const dcomplex tv = Ts[(tj + l) * T + ti];
+                    ar += my[l] * tv.re;
+                    ai += my[l] * tv.im;
+                }
+#pragma unroll
+                for (int d = -P; d <= P; ++d) {
+                    const int k = K + d;
+                    if (k >= k0 && k < k1) {
+                        const double mz = Mz[k * R + P - d], xz = Xz[k * R + P - d];
+                        dcomplex &o = am[(r + d + P) % R], &q = ad[(r + d + P) % R];
+                        o.re += mz * ar;
+                        o.im += mz * ai;
+                        q.re += xz * ar;
+                        q.im += xz * ai;
+                    }
+                }
+            }
+            const int ko = K - P;
+            if (ko >= k0 && ko < k1 && own) {
+                m[ko * plane + (long)j * n0 + i] = am[r];
+                dz[ko * plane + (long)j * n0 + i] = ad[r];
+            }
+            am[r] = ad[r] = {0.0, 0.0};
+        }
+    }
+}
+
 template <typename R>
 __global__ void diag_scale_kernel(int n0, int n1, long N, const R *l0, const R *l1, const R *l2,
                                   const unsigned char *w0, const unsigned char *w1, const unsigned char *w2,
@@ -1472,6 +1601,41 @@ void launch_extrapolate(hipStream_t st, const int *step, int order, const dcompl
 {
     hipLaunchKernelGGL(extrap_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, step, order, x, h1, h2, out,
                        N);
+}
+
+void launch_lincomb3(hipStream_t st, const LinComb &l1, const LinComb &l2, const LinComb &l3, dcomplex *y1,
+                     dcomplex *y2, dcomplex *y3, long N)
+{
+    hipLaunchKernelGGL(lincomb3_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, l1, l2, l3, y1, y2, y3, N);
+}
+
+bool launch_mass_dipole_z(hipStream_t st, const int n[3], int p, const double *M[3], const double *Xz,
+                          const dcomplex *x, dcomplex *m, dcomplex *dz)
+{
+    const int n0 = n[0], n1 = n[1], n2 = n[2];
+    if (p < 2 || p > 5 || n2 <= 2 * p) return false;
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+    }
+    const int tiles = ((n0 + 15) / 16) * ((n1 + 15) / 16);
+    const int chunks = std::max(1, std::min(2 * ncu / tiles, n2 / (2 * p + 2)));
+    const int kchunk = (n2 + chunks - 1) / chunks;
+    const unsigned nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
+#define TDZ_MD(PP)                                                                                              \
+    hipLaunchKernelGGL(mass_dip_kernel<PP>, dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, M[0], M[1], M[2], Xz, x, \
+                       m, dz)
+    switch (p) {
+    case 2: TDZ_MD(2); break;
+    case 3: TDZ_MD(3); break;
+    case 4: TDZ_MD(4); break;
+    default: TDZ_MD(5); break;
+    }
+#undef TDZ_MD
+    return true;
 }
 
 void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dcomplex *y, long N)

@@ -69,6 +69,30 @@ def test_predicted_steps_match_oracle(pair, order):
         ctx.set_cn_predictor(0)
 
 
+def test_predicted_steps_z_field_match_oracle(pair):
+    """A field along z alone: the mass and z-dipole products of each state come from one
+    fused pass (3D problems)."""
+    import tdsez_amd as T
+    name, desc, o, ctx = pair
+    dim = desc["dim"]
+    dt, nsteps = 0.02, 8
+    psi0 = o.project_gaussian(centre=(0.5, -0.3, 0.2)[:dim], width=(1.2, 1.0, 1.1)[:dim],
+                              momentum=(0.3, 0.0, -0.2)[:dim])
+    pulse = T.make_pulse(E0=(0.0, 0.0, 0.08), omega=0.3, ncycles=1.0)
+    sE = np.array([ctx.coupling(pulse, (s + 0.5) * dt) for s in range(nsteps)])
+    sE[:, dim:] = 0.0
+    ref, _, _ = o.cn_steps(psi0, dt, sE, rtol=1e-13)
+    ctx.pc_setup(T.PC_SEPARABLE)
+    ctx.set_cn_predictor(2)
+    try:
+        d = ctx.vec(psi0)
+        its, _ = ctx.evolve_cn(d, 0.0, dt, nsteps, pulse, rtol=1e-13)
+        err = _l2(o, d.download() - ref) / _l2(o, ref)
+        assert err < 1e-11, (name, err, its)
+    finally:
+        ctx.set_cn_predictor(0)
+
+
 def test_history_carries_over_and_resets(pair):
     import tdsez_amd as T
     name, desc, o, ctx = pair
