@@ -133,7 +133,7 @@ namespace {
 struct LearnState {
     bool ready = false;
     float *P = nullptr;                  // [N][C] positional fields
-    unsigned short *Pb = nullptr;        // the same as bf16, field-major (inference)
+    unsigned short *Pb = nullptr;        // the same as bf16 (inference)
     std::vector<double> fscale;          // [C] maximum of each field before scaling to 1
     float *W1 = nullptr, *W2 = nullptr;  // device weights [27][CI][C], [27][C][C]
     float *gr[3] = {};                   // Greville coordinates per axis (float)
@@ -1927,7 +1927,7 @@ void setup_learned(tdz_ctx *c)
     ls.P = upload(c, P);
     {
         const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-        ls.Pb = upload(c, tdz::cnn_fields_bf16(n3, P));
+        ls.Pb = upload(c, tdz::cnn_fields_bf16(P));
     }
     ls.W1 = track(c, dalloc<float>(27 * CI * C));
     ls.W2 = track(c, dalloc<float>(27 * C * C));

@@ -228,11 +228,9 @@ void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const f
                       float *partial, int nchunk, float *dW1, float *dW2);
 // BF16 MFMA forward; mb work of cnn_infer_work_bytes(N) bytes ([2N][C] bf16 plus the
 // packed weights), out [2N]; if nout: nout = g (out_re + i out_im). Pb: the positional
-// fields as bf16 in groups of 4 voxels along x, x rows padded to cnn_fields_row(n0)
-// (cnn_fields_bf16 of the [N][C] float fields).
+// fields as bf16, [N][C] (cnn_fields_bf16 of the [N][C] float fields).
 inline size_t cnn_infer_work_bytes(long N) { return (size_t)2 * N * kCnnChannels * 2 + 64 * 64 * 8; }
-int cnn_fields_row(int n0);
-std::vector<unsigned short> cnn_fields_bf16(const int n[3], const std::vector<float> &P);
+std::vector<unsigned short> cnn_fields_bf16(const std::vector<float> &P);
 void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
                     const unsigned short *Pb, unsigned short *mb, float *out, fcomplex g, fcomplex *nout);
 void cnn_to_complex(hipStream_t st, long N, const float *out, fcomplex g, fcomplex *n);

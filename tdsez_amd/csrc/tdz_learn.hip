@@ -290,12 +290,8 @@ __global__ void sum_partials_kernel(const float *__restrict__ partial, int nchun
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ short to_bf16(float f)
-{
-    unsigned u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
-    return (short)(u >> 16);
-}
+// round to nearest even: one v_cvt_pk_bf16_f32 on gfx950
+__device__ __forceinline__ short to_bf16(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
 // Tiles of the MFMA inference: a block of 256 threads (4 waves) owns 16 (x) x 4 (y)
 // x 12 (z) voxels of one component; wave w owns z-planes 3w .. 3w+2 of the tile, each
@@ -330,15 +326,8 @@ __device__ __forceinline__ TileIdx tile_of(int n0, int n1, int n2)
 
 __device__ __forceinline__ int tap_of_halo(int lx, int ly, int lz) { return (lz * HY + ly) * HX + lx; }
 
-__device__ __forceinline__ bf16x4 unpack4(uint2 raw)
-{
-    bf16x4 a;
-    a[0] = (short)(raw.x & 0xFFFFu);
-    a[1] = (short)(raw.x >> 16);
-    a[2] = (short)(raw.y & 0xFFFFu);
-    a[3] = (short)(raw.y >> 16);
-    return a;
-}
+// the 4 bf16 of an 8-byte word as an MFMA operand (element 0 in the low half of .x)
+__device__ __forceinline__ bf16x4 unpack4(uint2 raw) { return __builtin_bit_cast(bf16x4, raw); }
 
 __device__ __forceinline__ uint2 pack4(float a, float b, float c, float d)
 {
@@ -363,17 +352,15 @@ __global__ void cnn_pack_kernel(const float *__restrict__ W1, const float *__res
     wp[e] = pack4(v[0], v[1], v[2], v[3]);
 }
 
-// The inference reads the positional fields as bf16 in groups of 4 voxels along x:
-// Pb[((z n1 + y) (n0p / 4) + x / 4) 64 + 4 a + x % 4] for field a (n0p: n0 rounded up to
-// a multiple of 4, padding zero). A lane's 4 consecutive voxels x0 + kb .. x0 + kb + 3
-// (kb a multiple of 4) of its field `row` are one aligned 8-byte load, and the 64 lanes
-// of a wave read 512 contiguous bytes.
-__device__ __forceinline__ void load_fields(const unsigned short *__restrict__ Pb, int n0, int n0p, int n1, int n2,
-                                            int row, int gx, int gy, int gz, float pv[4])
+// The inference reads the positional fields as bf16, voxel-major: Pb[v C + a]. The
+// lane that holds channels kb .. kb + 3 of voxel (gx, gy, gz) reads their 4 fields as
+// one aligned 8-byte load; the 64 lanes of a wave (16 voxels along x, 4 channel groups)
+// read 512 contiguous bytes.
+__device__ __forceinline__ void load_fields(const unsigned short *__restrict__ Pb, int n0, int n1, int n2, int kb,
+                                            int gx, int gy, int gz, float pv[4])
 {
-    if (gx < n0 && gy < n1 && gz < n2) {   // gx + 3 < n0p: n0p and gx are multiples of 4
-        const uint2 raw = *reinterpret_cast<const uint2 *>(
-            Pb + (((long)gz * n1 + gy) * (n0p / 4) + gx / 4) * 64 + 4 * row);
+    if (gx < n0 && gy < n1 && gz < n2) {
+        const uint2 raw = *reinterpret_cast<const uint2 *>(Pb + (gx + (long)n0 * (gy + (long)n1 * gz)) * C + kb);
         pv[0] = __uint_as_float(raw.x << 16);
         pv[1] = __uint_as_float(raw.x & 0xFFFF0000u);
         pv[2] = __uint_as_float(raw.y << 16);
@@ -385,42 +372,41 @@ __device__ __forceinline__ void load_fields(const unsigned short *__restrict__ P
 
 // Layer 1 on MFMA, v_mfma_f32_16x16x16_bf16, both components per block (the inputs are
 // read once). Operand layout (wave64): lane l holds A[l%16][4(l/16)+q], B[4(l/16)+q][l%16],
-// q < 4, and D[4(l/16)+q][l%16].
-// A[i][k] = u_c(v_i + tap t), k = t CI + c (zero for k >= 27 CI); B[k][a] = W1[t][c][a];
-// D[i][a] -> m = P[v_i][a] D[i][a], stored as bf16 at mb[(comp N + v) C + a].
+// q < 4, and D[4(l/16)+q][l%16]. The weights are the A operand and the data the B
+// operand, so D is the transposed product: with k = t CI + c (zero for k >= 27 CI),
+// A[a][k] = W1[t][c][a], B[k][i] = u_c(v_i + tap t), D[a][i] = sum_k W1 u, and lane l
+// holds the 4 consecutive hidden channels a = 4(l/16) .. +3 of voxel v_{l%16}:
+// m = P[v_i][a] D[a][i] is one 8-byte bf16 store at mb[(comp N + v) C + a].
 // The halo tile is staged in LDS as bf16 (the CI channels of a voxel are one 8-byte
-// A operand, no conversion per MFMA).
+// operand, no conversion per MFMA).
 __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2, const fcomplex *__restrict__ u,
                                                           const uint2 *__restrict__ wp,
-                                                          const unsigned short *__restrict__ Pb, int n0p,
+                                                          const unsigned short *__restrict__ Pb,
                                                           unsigned short *__restrict__ mb)
 {
     static_assert(CI == 4, "a lane's 4 consecutive k must be the CI channels of one tap");
     __shared__ uint2 usv[2][HV];   // [comp][voxel]: the CI channels as bf16
-    unsigned short *us = reinterpret_cast<unsigned short *>(usv);
     const TileIdx T = tile_of(n0, n1, n2);
     const long N = (long)n0 * n1 * n2;
-    // stage in two phases (all loads in flight, then all LDS stores)
-    constexpr int NE = (CI * HV + 255) / 256;
-    fcomplex val[NE];
+    // stage one halo voxel (its CI channels) per thread and round, in two phases (all
+    // loads in flight, then all LDS stores)
+    constexpr int NV = (HV + 255) / 256;
+    fcomplex val[NV][CI];
 #pragma unroll
-    for (int r = 0; r < NE; ++r) {
-        const int e = threadIdx.x + 256 * r;
-        val[r] = {0.f, 0.f};
-        if (e < CI * HV) {
-            const int c = e / HV, h = e % HV;
-            const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
-            if (gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2)
-                val[r] = u[(long)c * N + gx + (long)n0 * (gy + (long)n1 * gz)];
-        }
+    for (int r = 0; r < NV; ++r) {
+        const int h = threadIdx.x + 256 * r;
+        const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
+        const bool in = h < HV && gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2;
+        const fcomplex *src = u + (in ? gx + (long)n0 * (gy + (long)n1 * gz) : 0);
+#pragma unroll
+        for (int c = 0; c < CI; ++c) val[r][c] = in ? src[(long)c * N] : fcomplex{0.f, 0.f};
     }
 #pragma unroll
-    for (int r = 0; r < NE; ++r) {
-        const int e = threadIdx.x + 256 * r;
-        if (e < CI * HV) {
-            const int c = e / HV, h = e % HV;
-            us[h * CI + c] = (unsigned short)to_bf16(val[r].re);
-            us[(HV + h) * CI + c] = (unsigned short)to_bf16(val[r].im);
+    for (int r = 0; r < NV; ++r) {
+        const int h = threadIdx.x + 256 * r;
+        if (h < HV) {
+            usv[0][h] = pack4(val[r][0].re, val[r][1].re, val[r][2].re, val[r][3].re);
+            usv[1][h] = pack4(val[r][0].im, val[r][1].im, val[r][2].im, val[r][3].im);
         }
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -430,79 +416,87 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
     for (int s = 0; s < K1 / 16; ++s) bw[s] = unpack4(wp[(NTAP + s) * 64 + lane]);
     __syncthreads();
     const uint2 *us0 = usv[0], *us1 = usv[1];
+    const int gx = T.x0 + row;
+#pragma unroll 1   // unrolled, the plane loop runs slower (code size and register pressure)
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
         // the plane's positional values for this lane's outputs, loaded ahead of the MFMAs
         float pv[TY][4];
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry) load_fields(Pb, n0, n0p, n1, n2, row, T.x0 + kb, T.y0 + ry, gz, pv[ry]);
+        for (int ry = 0; ry < TY; ++ry) load_fields(Pb, n0, n1, n2, kb, gx, T.y0 + ry, gz, pv[ry]);
         floatx4 acc[2][TY];
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) acc[0][ry] = acc[1][ry] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < K1 / 16; ++s) {
             const int t = 4 * s + kb / 4;
-            const int tt = t < NTAP ? t : 13;   // padded k: any in-tile voxel, B is zero there
+            const int tt = t < NTAP ? t : 13;   // padded k: any in-tile voxel, the weight is zero there
             const int dx = tap_dx(tt), dy = tap_dy(tt), dz = tap_dz(tt);
 #pragma unroll
             for (int ry = 0; ry < TY; ++ry) {
                 const int h = tap_of_halo(row + 1 + dx, ry + 1 + dy, lz + 1 + dz);
-                acc[0][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(unpack4(us0[h]), bw[s], acc[0][ry], 0, 0, 0);
-                acc[1][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(unpack4(us1[h]), bw[s], acc[1][ry], 0, 0, 0);
+                acc[0][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[s], unpack4(us0[h]), acc[0][ry], 0, 0, 0);
+                acc[1][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[s], unpack4(us1[h]), acc[1][ry], 0, 0, 0);
             }
         }
-        // lane holds D[kb + q][row]: voxel x0 + kb + q, hidden channel row
+        // lane holds D[kb + q][row]: hidden channel kb + q of voxel (x0 + row, y0 + ry)
+        if (gx < n0 && gz < n2) {
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry) {
-            const int gy = T.y0 + ry;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int gx = T.x0 + kb + q;
-                if (gx >= n0 || gy >= n1 || gz >= n2) continue;
+            for (int ry = 0; ry < TY; ++ry) {
+                const int gy = T.y0 + ry;
+                if (gy >= n1) continue;
                 const long v = gx + (long)n0 * (gy + (long)n1 * gz);
-                mb[v * C + row] = (unsigned short)to_bf16(pv[ry][q] * acc[0][ry][q]);
-                mb[(N + v) * C + row] = (unsigned short)to_bf16(pv[ry][q] * acc[1][ry][q]);
+                const float *p = pv[ry];
+                const floatx4 a0 = acc[0][ry], a1 = acc[1][ry];
+                *reinterpret_cast<uint2 *>(mb + v * C + kb) =
+                    pack4(p[0] * a0[0], p[1] * a0[1], p[2] * a0[2], p[3] * a0[3]);
+                *reinterpret_cast<uint2 *>(mb + (N + v) * C + kb) =
+                    pack4(p[0] * a1[0], p[1] * a1[1], p[2] * a1[2], p[3] * a1[3]);
             }
         }
     }
 }
 
-// Layer 2 on MFMA: per tap t, A[i][a] = m[v_i + t][a], B[a][b] = W2[t][a][b]; the 27
-// products accumulate into D[i][b]; out(v_i) = sum_b P[v_i][b] D[i][b] (reduced over the
-// 16 lanes that hold row i's columns). The halo tile of m lives in LDS with a 40-byte
+// Layer 2 on MFMA: per tap t, A[b][a] = W2[t][a][b], B[a][i] = m[v_i + t][a]; the 27
+// products accumulate into D[b][i] (lane l holds output channels 4(l/16) .. +3 of voxel
+// v_{l%16}); out(v_i) = sum_b P[v_i][b] D[b][i] (4 channels in the lane, then over the
+// 4 lanes that hold voxel i's channel groups). The halo tile of m lives in LDS with a 40-byte
 // voxel stride (16 bf16 channels + 4 padding), which spreads the 16 lanes of an 8-byte
 // A-operand read over distinct banks.
 constexpr int MS = 20;   // LDS stride of one voxel of m, in bf16 elements
 __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2,
                                                           const unsigned short *__restrict__ mb,
                                                           const uint2 *__restrict__ wp,
-                                                          const unsigned short *__restrict__ Pb, int n0p,
+                                                          const unsigned short *__restrict__ Pb,
                                                           float *__restrict__ out)
 {
     __shared__ uint2 ms[HV * MS / 4];
     const TileIdx T = tile_of(n0, n1, n2);
     const long N = (long)n0 * n1 * n2;
-    // stage 4 x 8 bytes per voxel (channels 4p .. 4p+3) in two phases (all loads in
-    // flight, then all LDS stores)
-    constexpr int NE = (4 * HV + 255) / 256;
-    uint2 val[NE];
+    // stage one halo voxel (its 16 channels, 2 x 16 bytes) per thread and round, in two
+    // phases (all loads in flight, then all LDS stores)
+    constexpr int NV = (HV + 255) / 256;
+    uint4 val[NV][2];
 #pragma unroll
-    for (int r = 0; r < NE; ++r) {
-        const int e = threadIdx.x + 256 * r;
-        val[r] = {0u, 0u};
-        if (e < 4 * HV) {
-            const int h = e >> 2, part = e & 3;
-            const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
-            if (gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2) {
-                const long v = gx + (long)n0 * (gy + (long)n1 * gz);
-                val[r] = reinterpret_cast<const uint2 *>(mb + ((long)T.comp * N + v) * C)[part];
-            }
-        }
+    for (int r = 0; r < NV; ++r) {
+        const int h = threadIdx.x + 256 * r;
+        const int gx = T.x0 - 1 + h % HX, gy = T.y0 - 1 + (h / HX) % HY, gz = T.z0 - 1 + h / (HX * HY);
+        const bool in = h < HV && gx >= 0 && gx < n0 && gy >= 0 && gy < n1 && gz >= 0 && gz < n2;
+        const uint4 *src = reinterpret_cast<const uint4 *>(
+            mb + ((long)T.comp * N + (in ? gx + (long)n0 * (gy + (long)n1 * gz) : 0)) * C);
+        val[r][0] = in ? src[0] : uint4{0u, 0u, 0u, 0u};
+        val[r][1] = in ? src[1] : uint4{0u, 0u, 0u, 0u};
     }
 #pragma unroll
-    for (int r = 0; r < NE; ++r) {
-        const int e = threadIdx.x + 256 * r;
-        if (e < 4 * HV) ms[((e >> 2) * MS) / 4 + (e & 3)] = val[r];
+    for (int r = 0; r < NV; ++r) {
+        const int h = threadIdx.x + 256 * r;
+        if (h < HV) {
+            uint2 *dst = ms + (h * MS) / 4;
+            dst[0] = {val[r][0].x, val[r][0].y};
+            dst[1] = {val[r][0].z, val[r][0].w};
+            dst[2] = {val[r][1].x, val[r][1].y};
+            dst[3] = {val[r][1].z, val[r][1].w};
+        }
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row = lane & 15, kb = (lane >> 4) * 4;
@@ -510,7 +504,7 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
 #pragma unroll
     for (int t = 0; t < NTAP; ++t) bw[t] = unpack4(wp[t * 64 + lane]);
     __syncthreads();
-    // Halo-row-major order: each A fragment (16 voxels of one halo row, shifted by dx) is
+    // Halo-row-major order: each B fragment (16 voxels of one halo row, shifted by dx) is
     // read from LDS once and feeds the up to 9 output rows (ry, pz) it is a tap of, so
     // the wave reads (TY + 2)(WZ + 2) 3 fragments instead of 27 TY WZ.
     floatx4 acc[WZ][TY];
@@ -535,43 +529,40 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
                         const int ry = hy - 1 - dy;
                         if (ry < 0 || ry >= TY) continue;
                         const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-                        acc[pz][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[t], acc[pz][ry], 0, 0, 0);
+                        acc[pz][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[t], a, acc[pz][ry], 0, 0, 0);
                     }
                 }
             }
+    static_assert(TY == 4, "the 4 channel groups of a voxel end with one row each");
+    const int gx = T.x0 + row, g = lane >> 4;
 #pragma unroll
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
-        float pv[TY][4];   // positional values of this lane's outputs
+        // lane holds D[kb + q][row]: output channel kb + q of voxel (x0 + row, y0 + ry)
+        float v4[TY];
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry) load_fields(Pb, n0, n0p, n1, n2, row, T.x0 + kb, T.y0 + ry, gz, pv[ry]);
-        // lane holds D[kb + q][row] (voxel x0 + kb + q, output channel row). out(v) sums the
-        // 16 lanes of a group: recursive halving over the 16 values (ry, q) leaves lane row
-        // with the sum for k = 4 ry + q = row (15 exchanges instead of 64).
-        float v16[16];
+        for (int ry = 0; ry < TY; ++ry) {
+            float pv[4];
+            load_fields(Pb, n0, n1, n2, kb, gx, T.y0 + ry, gz, pv);
+            v4[ry] = pv[0] * acc[pz][ry][0] + pv[1] * acc[pz][ry][1] + pv[2] * acc[pz][ry][2] +
+                     pv[3] * acc[pz][ry][3];
+        }
+        // sum over the 4 channel groups (lanes row, row + 16, row + 32, row + 48) by
+        // recursive halving over ry: group g ends with the sum for ry = g (3 exchanges).
+        // The lane-dependent choice is a bit mask (a select becomes a runtime index).
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v16[4 * ry + q] = pv[ry][q] * acc[pz][ry][q];
-        // The lane-dependent choice between v16[i] and v16[i + m] is a bit mask (one
-        // v_bfi per value): written as a select, the compiler turns it into a
-        // runtime index into v16 (a 16-way compare/select chain per value).
-#pragma unroll
-        for (int m = 8; m > 0; m >>= 1) {
-            const unsigned um = (row & m) ? 0xFFFFFFFFu : 0u;
+        for (int m = 2; m > 0; m >>= 1) {
+            const unsigned um = (g & m) ? 0xFFFFFFFFu : 0u;
 #pragma unroll
             for (int i = 0; i < m; ++i) {
-                const unsigned lo = __float_as_uint(v16[i]), hi = __float_as_uint(v16[i + m]);
+                const unsigned lo = __float_as_uint(v4[i]), hi = __float_as_uint(v4[i + m]);
                 const float keep = __uint_as_float((hi & um) | (lo & ~um));
                 const float send = __uint_as_float((lo & um) | (hi & ~um));
-                v16[i] = keep + __shfl_xor(send, m, 16);
+                v4[i] = keep + __shfl_xor(send, 16 * m);
             }
         }
-        {
-            const int ry = row >> 2, q = row & 3;
-            const int gx = T.x0 + kb + q, gy = T.y0 + ry;
-            if (gx < n0 && gy < n1 && gz < n2) out[(long)T.comp * N + gx + (long)n0 * (gy + (long)n1 * gz)] = v16[0];
-        }
+        const int gy = T.y0 + g;
+        if (gx < n0 && gy < n1 && gz < n2) out[(long)T.comp * N + gx + (long)n0 * (gy + (long)n1 * gz)] = v4[0];
     }
 }
 
@@ -951,23 +942,15 @@ void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const f
                        NTAP * CI * C, dW1);
 }
 
-int cnn_fields_row(int n0) { return (n0 + 3) / 4 * 4; }
-
-std::vector<unsigned short> cnn_fields_bf16(const int n[3], const std::vector<float> &P)
+std::vector<unsigned short> cnn_fields_bf16(const std::vector<float> &P)
 {
-    const int n0p = cnn_fields_row(n[0]);
-    std::vector<unsigned short> out((size_t)C * n[2] * n[1] * n0p, 0);
-    for (int a = 0; a < C; ++a)
-        for (int z = 0; z < n[2]; ++z)
-            for (int y = 0; y < n[1]; ++y)
-                for (int x = 0; x < n[0]; ++x) {
-                    const float f = P[(x + (size_t)n[0] * (y + (size_t)n[1] * z)) * C + a];
-                    unsigned bits;
-                    std::memcpy(&bits, &f, 4);
-                    bits += 0x7FFFu + ((bits >> 16) & 1u);   // round to nearest even
-                    out[(((size_t)z * n[1] + y) * (n0p / 4) + x / 4) * 64 + 4 * a + x % 4] =
-                        (unsigned short)(bits >> 16);
-                }
+    std::vector<unsigned short> out(P.size());
+    for (size_t e = 0; e < P.size(); ++e) {
+        unsigned bits;
+        std::memcpy(&bits, &P[e], 4);
+        bits += 0x7FFFu + ((bits >> 16) & 1u);   // round to nearest even
+        out[e] = (unsigned short)(bits >> 16);
+    }
     return out;
 }
 
@@ -976,15 +959,13 @@ void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const flo
 {
     const long N = (long)n[0] * n[1] * n[2];
     const long tiles = (long)((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ);
-    const int n0p = cnn_fields_row(n[0]);
     // the packed weights go behind the 2 N C bf16 of m (the work buffer holds both)
     uint2 *wp = reinterpret_cast<uint2 *>(mb + 2 * N * C);
     hipLaunchKernelGGL(cnn_pack_kernel, dim3(gridn(NPACK * 64, 256)), dim3(256), 0, st, W1, W2, wp);
     // layer 1: one block per tile (both components); layer 2: one per tile and component
-    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)tiles), dim3(256), 0, st, n[0], n[1], n[2], u, wp, Pb, n0p,
-                       mb);
+    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)tiles), dim3(256), 0, st, n[0], n[1], n[2], u, wp, Pb, mb);
     hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)(2 * tiles)), dim3(256), 0, st, n[0], n[1], n[2], mb, wp,
-                       Pb, n0p, out);
+                       Pb, out);
     if (nout) hipLaunchKernelGGL(cnn_to_complex_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, N, out, g, nout);
 }
 
