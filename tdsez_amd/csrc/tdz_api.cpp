@@ -99,6 +99,8 @@ struct Level {
     // host staging of the band uploads (kept so that no copy waits on its source)
     std::vector<dcomplex> hB[3];
     std::vector<fcomplex> hBf[3], hBc[3];
+    // device copies of the stiffness, CAP and position bands (bands updated on the device)
+    double *dS[3] = {}, *dW[3] = {}, *dX[3] = {};
     Prec<double> d;
     Prec<float> f;
     // transfer to the next coarser level: P (this x coarse) and R = P^T
@@ -206,7 +208,10 @@ struct tdz_ctx {
     int restart = 30;
     dcomplex *Vb = nullptr, *Zb = nullptr;
     dcomplex *dpart = nullptr, *dh = nullptr, *dh2 = nullptr;
-    double *dpartr = nullptr, *dnorm = nullptr;
+    double *dpartr = nullptr, *dnorm = nullptr;   // dnorm: 2 slots
+    // pinned staging of the solver's small uploads (host memory that outlives the copies)
+    dcomplex *hstage = nullptr;
+    int hstage_len = 0;
     int pc_kind = TDZ_PC_NONE;
     bool pc_ready = false;
     std::vector<void *> allocs;     // everything to free
@@ -368,6 +373,39 @@ void upload_level_tables_f(tdz_ctx *c, Level &L)
 void update_bands(tdz_ctx *c, Level &L)
 {
     const double inv2m = 0.5 / c->ph.mass;
+    bool on_device = !L.layered;
+    for (int d = 0; d < 3 && on_device; ++d) {
+        const size_t sz = L.M[d].size();
+        on_device = L.d.M[d] && L.d.B[d] && L.S[d].size() == sz && L.Wc[d].size() == sz && L.Xb[d].size() == sz;
+    }
+    if (on_device) {
+        // on the device: no host staging, no copies, no wait for the stream
+        tdz::BandUpdate bu;
+        for (int d = 0; d < 3; ++d) {
+            const size_t sz = L.M[d].size();
+            if (!L.dS[d]) {
+                L.dS[d] = upload(c, L.S[d]);
+                L.dW[d] = upload(c, L.Wc[d]);
+                L.dX[d] = upload(c, L.Xb[d]);
+            }
+            bu.M[d] = L.d.M[d];
+            bu.S[d] = L.dS[d];
+            bu.W[d] = L.dW[d];
+            bu.X[d] = L.dX[d];
+            bu.B[d] = L.d.B[d];
+            bu.Bc[d] = (c->dBcorr[d] && &L == c->lv[0].get()) ? c->dBcorr[d] : nullptr;
+            bu.Bf[d] = L.f.ready ? L.f.B[d] : nullptr;
+            bu.sz[d] = (int)sz;
+            bu.active[d] = d < c->prob.dim;
+            bu.sE[d] = c->sE[d];
+        }
+        bu.alpha = dc(c->alpha);
+        bu.g = dc(c->g);
+        bu.inv2m = inv2m;
+        bu.cap = c->cap_on ? 1 : 0;
+        tdz::launch_band_update(c->st, bu);
+        return;
+    }
     for (int d = 0; d < 3; ++d) {
         const size_t sz = L.M[d].size();
         std::vector<dcomplex> &B = L.hB[d];
@@ -901,6 +939,10 @@ double dev_norm2(tdz_ctx *c, const dcomplex *x, long N)
 // taken from the Arnoldi relation of the last cycle: r = V_{j+1} (beta e_1 - Hbar y)
 // = gamma_{j+1} V_{j+1} Q^H e_{j+1} (Q the Givens rotations), at the cost of one
 // (j+1)-term vector update instead of an operator apply.
+// The host waits for the device once per Arnoldi step (the projections) and once at the
+// start (||b|| and, with r0_in_v0, ||r0|| together); the solution update at the end is
+// queued without a wait (its coefficients are uploaded from pinned staging, which the
+// next solve rewrites only after its own first wait).
 int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, tdz_ksp_stats *st,
            bool r0_in_v0 = false, bool allow_fail = false, dcomplex *ax_out = nullptr)
 {
@@ -910,7 +952,26 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
     auto Vj = [&](int j) { return c->Vb + (long)j * N; };
     auto Zj = [&](int j) { return c->pc_kind == TDZ_PC_NONE ? Vj(j) : c->Zb + (long)j * N; };
     std::vector<cplx> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hv(m + 1), hv2(m + 1);
-    const double bnorm = std::sqrt(dev_norm2(c, b, N));
+    double bnorm, beta0 = -1;   // beta0 >= 0: ||V_0|| of the first cycle, fetched with ||b||
+    if (r0_in_v0) {
+        tdz::launch_norm2(c->st, b, N, c->dpartr, c->dnorm + 1);
+        tdz::launch_norm2(c->st, Vj(0), N, c->dpartr, c->dnorm);
+        double h2[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(h2, c->dnorm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        bnorm = std::sqrt(h2[1]);
+        beta0 = std::sqrt(h2[0]);
+    } else {
+        bnorm = std::sqrt(dev_norm2(c, b, N));
+    }
+    // staging: now past a wait, so no earlier upload still reads it
+    if (c->hstage_len < 2 * (m + 1)) {
+        if (c->hstage) HIPCHK(hipHostFree(c->hstage));
+        c->hstage = nullptr;
+        HIPCHK(hipHostMalloc((void **)&c->hstage, 2 * (m + 1) * sizeof(dcomplex), hipHostMallocDefault));
+        c->hstage_len = 2 * (m + 1);
+    }
+    dcomplex *ystage = c->hstage, *rstage = c->hstage + (m + 1);
     int its = 0, restarts = 0;
     double res = 0;
     if (bnorm == 0.0) {
@@ -926,13 +987,16 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             tdz::launch_axpby(c->st, {1, 0}, b, {-1, 0}, Vj(0), N);   // V0 = b - A x
         }
         r0_in_v0 = false;
-        const double beta = std::sqrt(dev_norm2(c, Vj(0), N));
+        const double beta = beta0 >= 0 ? beta0 : std::sqrt(dev_norm2(c, Vj(0), N));
+        beta0 = -1;
         res = beta / bnorm;
         if (res <= rtol) {
             converged = true;
             if (ax_out) {   // V_0 holds the residual b - A x itself
-                HIPCHK(hipMemcpyAsync(ax_out, b, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
-                tdz::launch_axpby(c->st, {-1, 0}, Vj(0), {1, 0}, ax_out, N);
+                tdz::LinComb ax;
+                ax.add({1, 0}, b);
+                ax.add({-1, 0}, Vj(0));
+                tdz::launch_lincomb(c->st, ax, ax_out, N);
             }
             break;
         }
@@ -984,9 +1048,8 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             for (int k = i + 1; k < j; ++k) s -= H[(size_t)i * m + k] * yv[k];
             yv[i] = s / H[(size_t)i * m + i];
         }
-        std::vector<dcomplex> yd(j);
-        for (int i = 0; i < j; ++i) yd[i] = dc(yv[i]);
-        HIPCHK(hipMemcpyAsync(c->dh, yd.data(), j * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+        for (int i = 0; i < j; ++i) ystage[i] = dc(yv[i]);
+        HIPCHK(hipMemcpyAsync(c->dh, ystage, j * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
         tdz::launch_multi_axpy(c->st, c->pc_kind == TDZ_PC_NONE ? c->Vb : c->Zb, N, j, c->dh, x, N);
         if (converged && ax_out) {
             // q = Q^H e_j: the inverse rotations, newest first
@@ -997,17 +1060,12 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
                 q[i] = cs[i] * a - sn[i] * bb;
                 q[i + 1] = std::conj(sn[i]) * a + cs[i] * bb;
             }
-            std::vector<dcomplex> cr(j + 1);
-            for (int i = 0; i <= j; ++i) cr[i] = dc(-gv[j] * q[i]);
-            // the host vector must outlive the asynchronous copy: synchronise before leaving
-            HIPCHK(hipMemcpyAsync(c->dh2, cr.data(), (j + 1) * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
-            HIPCHK(hipMemcpyAsync(ax_out, b, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
-            tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N);
-            HIPCHK(hipStreamSynchronize(c->st));
+            for (int i = 0; i <= j; ++i) rstage[i] = dc(-gv[j] * q[i]);
+            HIPCHK(hipMemcpyAsync(c->dh2, rstage, (j + 1) * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
+            tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N, b);   // ax_out = b + sum
         }
         if (converged || its >= maxit) break;
     }
-    HIPCHK(hipStreamSynchronize(c->st));
     if (st) *st = {its, restarts, res, bnorm, c->rebuilds};
     if (!converged) {
         if (allow_fail) return -1;
@@ -1252,7 +1310,7 @@ void build(tdz_ctx *c, const tdz_problem &pr)
     c->dh = track(c, dalloc<dcomplex>(c->restart + 1));
     c->dh2 = track(c, dalloc<dcomplex>(c->restart + 1));
     c->dpartr = track(c, dalloc<double>(tdz::kRedBlocks));
-    c->dnorm = track(c, dalloc<double>(1));
+    c->dnorm = track(c, dalloc<double>(2));
     c->prob.knots = nullptr;   // the caller's arrays are not kept
     c->prob.layer_bounds = c->prob.layer_mass = c->prob.layer_v = nullptr;
     const double zero[3] = {0, 0, 0};
@@ -1299,6 +1357,7 @@ int tdz_destroy(tdz_ctx *c)
     for (auto &e : c->ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
     for (void *p : c->allocs) hipFree(p);
     for (double *p : c->user_vecs) hipFree(p);
+    if (c->hstage) hipHostFree(c->hstage);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
     return 0;
@@ -1437,6 +1496,7 @@ int tdz_ksp_solve(tdz_ctx *c, const double *b, double *x, double rtol, int maxit
     return guarded([&] {
         if (!c->pc_ready) pc_setup(c, c->pc_kind);
         fgmres(c, reinterpret_cast<const dcomplex *>(b), reinterpret_cast<dcomplex *>(x), rtol, maxit, stats);
+        HIPCHK(hipStreamSynchronize(c->st));   // the results are final when the call returns
     });
 }
 
@@ -1708,6 +1768,7 @@ int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], doub
         tdz_ksp_stats st{};
         cn_step(c, reinterpret_cast<dcomplex *>(psi), dt, sE_mid, rtol, maxit, &st, false);
         if (stats) *stats = st;
+        HIPCHK(hipStreamSynchronize(c->st));   // the results are final when the call returns
     });
 }
 
@@ -2109,6 +2170,7 @@ int tdz_ground_state(tdz_ctx *c, double *psi, double sigma, double tol, int maxi
 {
     return guarded([&] {
         ground_state(c, reinterpret_cast<dcomplex *>(psi), sigma, tol, maxit, energy, residual, iterations);
+        HIPCHK(hipStreamSynchronize(c->st));   // the results are final when the call returns
     });
 }
 
@@ -2118,6 +2180,7 @@ int tdz_step_pade22(tdz_ctx *c, double *psi, double h, const double sE[3], int s
     return guarded([&] {
         if (slot < 0 || slot > 1) throw Error("slot must be 0 or 1");
         pade22_step(c, reinterpret_cast<dcomplex *>(psi), h, sE, slot, rtol, maxit, stats);
+        HIPCHK(hipStreamSynchronize(c->st));   // the results are final when the call returns
     });
 }
 
@@ -2162,6 +2225,7 @@ int tdz_evolve_rational(tdz_ctx *c, double *psi, double t0, double dt, int nstep
                 for (int k = 0; k < 6; ++k) obs[7 * s + 1 + k] = o[k];
             }
         }
+        HIPCHK(hipStreamSynchronize(c->st));   // the results are final when the call returns
     });
 }
 
@@ -2315,6 +2379,7 @@ int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, con
                 for (int k = 0; k < 6; ++k) obs[7 * s + 1 + k] = o[k];
             }
         }
+        HIPCHK(hipStreamSynchronize(c->st));   // the results are final when the call returns
     });
 }
 

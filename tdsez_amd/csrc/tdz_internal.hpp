@@ -123,8 +123,26 @@ void launch_multi_dot(hipStream_t st, const dcomplex *V, long ld, int nv, const 
 // w -= sum_i h_i V_i (h on device), and out_norm2 = ||w||^2 (on device, deterministic)
 void launch_multi_axpy_norm(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *h, dcomplex *w,
                             long N, double *partial, double *out_norm2);
-// x += sum_i c_i Z_i (c on device)
-void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N);
+// x = base + sum_i c_i Z_i (c on device; base null: x itself)
+void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N,
+                       const dcomplex *base = nullptr);
+// The bands of the shifted operator alpha M + g H (non-layered), from the device copies
+// of the axis bands, per axis d (active: d < dim):
+//   B_d = g (S_d / 2m - i W_d + sE_d X_d) (+ alpha M_d for d = 0; g (0) on inactive axes),
+//   Bc_d = g (-i W_d + sE_d X_d) as complex64 (the separable field part, if Bc_d),
+//   Bf_d = B_d as complex64 (if Bf_d).
+// W_d is the CAP band (its term only with cap). Evaluated in the same order as the host
+// formula, without contraction, so the bands are the same to the bit.
+struct BandUpdate {
+    const double *M[3] = {}, *S[3] = {}, *W[3] = {}, *X[3] = {};
+    dcomplex *B[3] = {};
+    fcomplex *Bc[3] = {}, *Bf[3] = {};
+    int sz[3] = {0, 0, 0}, active[3] = {0, 0, 0};
+    dcomplex alpha{0, 0}, g{0, 0};
+    double inv2m = 0, sE[3] = {0, 0, 0};
+    int cap = 0;
+};
+void launch_band_update(hipStream_t st, const BandUpdate &bu);
 // y = a x + b y (scalars on host)
 void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dcomplex *y, long N);
 // y = sum_k c_k v_k (k <= kMax terms; y may be one of the v_k: every element is read

@@ -789,14 +789,14 @@ __global__ void __launch_bounds__(kRedThreads) multi_axpy_norm_kernel(const dcom
 }
 
 __global__ void multi_axpy_kernel(const dcomplex *__restrict__ Z, long ld, int nv, const dcomplex *__restrict__ c,
-                                  dcomplex *__restrict__ x, long N)
+                                  const dcomplex *base, dcomplex *x, long N)
 {
     __shared__ dcomplex cs[64];
     for (int i = threadIdx.x; i < nv; i += blockDim.x) cs[i] = c[i];
     __syncthreads();
     const long stride = (long)gridDim.x * blockDim.x;
     for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
-        dcomplex v = x[I];
+        dcomplex v = base[I];
         for (int i = 0; i < nv; ++i) {
             const dcomplex b = Z[(long)i * ld + I];
             v.re += cs[i].re * b.re - cs[i].im * b.im;
@@ -1158,6 +1158,89 @@ __global__ void __launch_bounds__(256) axis_wave_mfma_kernel(int n, long ncol, l
             ai[g][s & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[g][s], bv[s].im, ai[g][s & 1], 0, 0, 0);
         }
     // lane holds D[4 k4 + q][r16] of each row tile: output row 16 it + 4 k4 + q of column col
+    if (!cv) return;
+    float la = 0.f;
+    bool wl = false;
+    if constexpr (DIAG) {   // z pass: column col = i0 + n0 i1, row i = i2
+        const int i0 = (int)(col % dg.n0), i1 = (int)(col / dg.n0);
+        la = dg.lam[0][i0] + dg.lam[1][i1];
+        wl = (dg.wall[0][i0] | dg.wall[1][i1]) != 0;
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int it = ig * G + g;
+        if (it >= nit) break;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * it + 4 * k4 + q;
+            if (i >= n) continue;
+            fcomplex v = {ar[g][0][q] + ar[g][1][q], ai[g][0][q] + ai[g][1][q]};
+            if constexpr (DIAG) {
+                if (!wl && !dg.wall[2][i]) {
+                    const float lam = la + dg.lam[2][i];
+                    const float dr = dg.alpha.re + dg.g.re * lam, di = dg.alpha.im + dg.g.im * lam;
+                    const float inv = 1.f / (dr * dr + di * di);
+                    v = {(v.re * dr + v.im * di) * inv, (v.im * dr - v.re * di) * inv};
+                }
+            }
+            out[base + (long)i * sK] = v;
+        }
+    }
+}
+
+// The same transform with the matrix fragments shared through LDS: a block is 4 waves
+// on 4 consecutive 16-column tiles and one group of G row tiles, whose fragments it
+// stages in LDS once (the waves then read them from LDS instead of each loading them
+// from L2 through its CU's L1, which bounds the per-wave variant). Block t
+// (XCD-remapped) is column-tile group t / ngr, row group t % ngr.
+template <int KS, int G, bool DIAG>
+__global__ void __launch_bounds__(256) axis_lds_mfma_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
+                                                            int nit, const float4 *__restrict__ Tf,
+                                                            const fcomplex *__restrict__ in,
+                                                            fcomplex *__restrict__ out, DiagF dg)
+{
+    constexpr int KS4 = (KS + 3) / 4;
+    __shared__ float4 As[G * KS4 * 64];
+    const int ngr = (nit + G - 1) / G;
+    const int nb = gridDim.x, b = blockIdx.x, per = nb / 8;
+    const long t = b < 8 * per ? (long)(b % 8) * per + b / 8 : b;
+    const int ig = (int)(t % ngr);
+    for (int e = threadIdx.x; e < G * KS4 * 64; e += 256) {
+        const int it = min(ig * G + e / (KS4 * 64), nit - 1);   // past the last row tile: a copy, not stored
+        As[e] = Tf[(long)it * KS4 * 64 + e % (KS4 * 64)];
+    }
+    __syncthreads();
+    const long nct = (ncol + 15) / 16;
+    const long ct = (t / ngr) * 4 + (threadIdx.x >> 6);
+    if (ct >= nct) return;   // whole wave, after the only barrier
+    const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const long col = ct * 16 + r16;
+    const bool cv = col < ncol;
+    const long base = cv ? (col / Lin) * sB + (col % Lin) * sL : 0;
+    fcomplex bv[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + k4;
+        bv[s] = in[base + (long)min(k, n - 1) * sK];
+        if (!cv || k >= n) bv[s] = {0.f, 0.f};
+    }
+    f32x4 ar[G][2], ai[G][2];
+#pragma unroll
+    for (int g = 0; g < G; ++g) ar[g][0] = ar[g][1] = ai[g][0] = ai[g][1] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s4 = 0; s4 < KS4; ++s4)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const float4 v = As[(g * KS4 + s4) * 64 + lane];
+            const float a4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s = 4 * s4 + j;
+                if (s >= KS) break;
+                ar[g][s & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[j], bv[s].re, ar[g][s & 1], 0, 0, 0);
+                ai[g][s & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[j], bv[s].im, ai[g][s & 1], 0, 0, 0);
+            }
+        }
     if (!cv) return;
     float la = 0.f;
     bool wl = false;
@@ -1585,9 +1668,46 @@ void launch_multi_axpy_norm(hipStream_t st, const dcomplex *V, long ld, int nv, 
     hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial, (int)nb, out_norm2);
 }
 
-void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N)
+void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N,
+                       const dcomplex *base)
 {
-    hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, Z, ld, nv, c, x, N);
+    hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, Z, ld, nv, c,
+                       base ? base : x, x, N);
+}
+
+// one thread per band entry, blockIdx.y the axis
+__global__ void band_update_kernel(BandUpdate bu)
+{
+#pragma clang fp contract(off)
+    const int d = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= bu.sz[d]) return;
+    const dcomplex g = bu.g;
+    // T = S / 2m + (-i W) + sE X, or 0 on an inactive axis
+    double tr = 0.0, ti = 0.0, fr = 0.0, fi = 0.0;
+    if (bu.active[d]) {
+        const double w = bu.cap ? bu.W[d][k] : 0.0, sx = bu.sE[d] * bu.X[d][k];
+        tr = bu.inv2m * bu.S[d][k] + 0.0 + sx;
+        ti = -w;
+        fr = 0.0 + sx;
+        fi = -w;
+    }
+    dcomplex b = {g.re * tr - g.im * ti, g.re * ti + g.im * tr};
+    if (d == 0) {
+        const double m = bu.M[0][k];
+        b.re += bu.alpha.re * m;
+        b.im += bu.alpha.im * m;
+    }
+    bu.B[d][k] = b;
+    if (bu.Bc[d]) bu.Bc[d][k] = {(float)(g.re * fr - g.im * fi), (float)(g.re * fi + g.im * fr)};
+    if (bu.Bf[d]) bu.Bf[d][k] = {(float)b.re, (float)b.im};
+}
+
+void launch_band_update(hipStream_t st, const BandUpdate &bu)
+{
+    const int mx = std::max(bu.sz[0], std::max(bu.sz[1], bu.sz[2]));
+    if (mx <= 0) return;
+    hipLaunchKernelGGL(band_update_kernel, dim3((unsigned)((mx + 255) / 256), 3), dim3(256), 0, st, bu);
 }
 
 void launch_lincomb(hipStream_t st, const LinComb &lc, dcomplex *y, long N)
@@ -1710,12 +1830,20 @@ template <bool DIAG>
 void launch_wave_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, long sL, long sK, const float *Tf,
                       const fcomplex *x, fcomplex *y, const DiagF &dg)
 {
-    const int nit = (nd + 15) / 16, G = nit < 3 ? nit : 3, ngr = (nit + G - 1) / G;
-    const unsigned nblk = (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
+    static const int gforce = [] { const char *e = std::getenv("TDZ_WAVE_G"); return e ? std::atoi(e) : 3; }();
+    static const bool lds = [] { const char *e = std::getenv("TDZ_AXIS_LDS"); return e && std::atoi(e) != 0; }();
+    const int nit = (nd + 15) / 16, G = nit < gforce ? nit : gforce, ngr = (nit + G - 1) / G;
+    const unsigned nblk = lds ? (unsigned)(((ncol + 15) / 16 + 3) / 4 * ngr) : (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
     const float4 *T4 = reinterpret_cast<const float4 *>(Tf);
 #define TDZ_WAVE(KS, GG)                                                                                           \
-    hipLaunchKernelGGL((axis_wave_mfma_kernel<KS, GG, DIAG>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, sB, sL, sK, \
-                       nit, T4, x, y, dg)
+    do {                                                                                                           \
+        if (lds)                                                                                                   \
+            hipLaunchKernelGGL((axis_lds_mfma_kernel<KS, GG, DIAG>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, sB, \
+                               sL, sK, nit, T4, x, y, dg);                                                         \
+        else                                                                                                       \
+            hipLaunchKernelGGL((axis_wave_mfma_kernel<KS, GG, DIAG>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, sB, \
+                               sL, sK, nit, T4, x, y, dg);                                                         \
+    } while (0)
 #define TDZ_WAVE_G(KS)              \
     do {                            \
         if (G == 1) TDZ_WAVE(KS, 1); \
