@@ -1188,89 +1188,6 @@ __global__ void __launch_bounds__(256) axis_wave_mfma_kernel(int n, long ncol, l
     }
 }
 
-// The same transform with the matrix fragments shared through LDS: a block is 4 waves
-// on 4 consecutive 16-column tiles and one group of G row tiles, whose fragments it
-// stages in LDS once (the waves then read them from LDS instead of each loading them
-// from L2 through its CU's L1, which bounds the per-wave variant). Block t
-// (XCD-remapped) is column-tile group t / ngr, row group t % ngr.
-template <int KS, int G, bool DIAG>
-__global__ void __launch_bounds__(256) axis_lds_mfma_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
-                                                            int nit, const float4 *__restrict__ Tf,
-                                                            const fcomplex *__restrict__ in,
-                                                            fcomplex *__restrict__ out, DiagF dg)
-{
-    constexpr int KS4 = (KS + 3) / 4;
-    __shared__ float4 As[G * KS4 * 64];
-    const int ngr = (nit + G - 1) / G;
-    const int nb = gridDim.x, b = blockIdx.x, per = nb / 8;
-    const long t = b < 8 * per ? (long)(b % 8) * per + b / 8 : b;
-    const int ig = (int)(t % ngr);
-    for (int e = threadIdx.x; e < G * KS4 * 64; e += 256) {
-        const int it = min(ig * G + e / (KS4 * 64), nit - 1);   // past the last row tile: a copy, not stored
-        As[e] = Tf[(long)it * KS4 * 64 + e % (KS4 * 64)];
-    }
-    __syncthreads();
-    const long nct = (ncol + 15) / 16;
-    const long ct = (t / ngr) * 4 + (threadIdx.x >> 6);
-    if (ct >= nct) return;   // whole wave, after the only barrier
-    const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
-    const long col = ct * 16 + r16;
-    const bool cv = col < ncol;
-    const long base = cv ? (col / Lin) * sB + (col % Lin) * sL : 0;
-    fcomplex bv[KS];
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const int k = 4 * s + k4;
-        bv[s] = in[base + (long)min(k, n - 1) * sK];
-        if (!cv || k >= n) bv[s] = {0.f, 0.f};
-    }
-    f32x4 ar[G][2], ai[G][2];
-#pragma unroll
-    for (int g = 0; g < G; ++g) ar[g][0] = ar[g][1] = ai[g][0] = ai[g][1] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s4 = 0; s4 < KS4; ++s4)
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const float4 v = As[(g * KS4 + s4) * 64 + lane];
-            const float a4[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int s = 4 * s4 + j;
-                if (s >= KS) break;
-                ar[g][s & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[j], bv[s].re, ar[g][s & 1], 0, 0, 0);
-                ai[g][s & 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[j], bv[s].im, ai[g][s & 1], 0, 0, 0);
-            }
-        }
-    if (!cv) return;
-    float la = 0.f;
-    bool wl = false;
-    if constexpr (DIAG) {   // z pass: column col = i0 + n0 i1, row i = i2
-        const int i0 = (int)(col % dg.n0), i1 = (int)(col / dg.n0);
-        la = dg.lam[0][i0] + dg.lam[1][i1];
-        wl = (dg.wall[0][i0] | dg.wall[1][i1]) != 0;
-    }
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const int it = ig * G + g;
-        if (it >= nit) break;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = 16 * it + 4 * k4 + q;
-            if (i >= n) continue;
-            fcomplex v = {ar[g][0][q] + ar[g][1][q], ai[g][0][q] + ai[g][1][q]};
-            if constexpr (DIAG) {
-                if (!wl && !dg.wall[2][i]) {
-                    const float lam = la + dg.lam[2][i];
-                    const float dr = dg.alpha.re + dg.g.re * lam, di = dg.alpha.im + dg.g.im * lam;
-                    const float inv = 1.f / (dr * dr + di * di);
-                    v = {(v.re * dr + v.im * di) * inv, (v.im * dr - v.re * di) * inv};
-                }
-            }
-            out[base + (long)i * sK] = v;
-        }
-    }
-}
-
 // largest axis length the register-resident transform takes; longer axes use the
 // LDS-tiled kernel
 constexpr int kWaveAxisMax = 160;
@@ -1830,20 +1747,12 @@ template <bool DIAG>
 void launch_wave_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, long sL, long sK, const float *Tf,
                       const fcomplex *x, fcomplex *y, const DiagF &dg)
 {
-    static const int gforce = [] { const char *e = std::getenv("TDZ_WAVE_G"); return e ? std::atoi(e) : 3; }();
-    static const bool lds = [] { const char *e = std::getenv("TDZ_AXIS_LDS"); return e && std::atoi(e) != 0; }();
-    const int nit = (nd + 15) / 16, G = nit < gforce ? nit : gforce, ngr = (nit + G - 1) / G;
-    const unsigned nblk = lds ? (unsigned)(((ncol + 15) / 16 + 3) / 4 * ngr) : (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
+    const int nit = (nd + 15) / 16, G = nit < 3 ? nit : 3, ngr = (nit + G - 1) / G;
+    const unsigned nblk = (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
     const float4 *T4 = reinterpret_cast<const float4 *>(Tf);
 #define TDZ_WAVE(KS, GG)                                                                                           \
-    do {                                                                                                           \
-        if (lds)                                                                                                   \
-            hipLaunchKernelGGL((axis_lds_mfma_kernel<KS, GG, DIAG>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, sB, \
-                               sL, sK, nit, T4, x, y, dg);                                                         \
-        else                                                                                                       \
-            hipLaunchKernelGGL((axis_wave_mfma_kernel<KS, GG, DIAG>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, sB, \
-                               sL, sK, nit, T4, x, y, dg);                                                         \
-    } while (0)
+    hipLaunchKernelGGL((axis_wave_mfma_kernel<KS, GG, DIAG>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, sB, sL, sK, \
+                       nit, T4, x, y, dg)
 #define TDZ_WAVE_G(KS)              \
     do {                            \
         if (G == 1) TDZ_WAVE(KS, 1); \
