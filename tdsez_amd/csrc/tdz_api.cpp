@@ -212,6 +212,8 @@ struct tdz_ctx {
     // pinned staging of the solver's small uploads (host memory that outlives the copies)
     dcomplex *hstage = nullptr;
     int hstage_len = 0;
+    // the caller queued dnorm[0] = ||V_0||^2 and dnorm[1] = ||b||^2 for the next solve
+    bool norms_queued = false;
     int pc_kind = TDZ_PC_NONE;
     bool pc_ready = false;
     std::vector<void *> allocs;     // everything to free
@@ -953,9 +955,13 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
     auto Zj = [&](int j) { return c->pc_kind == TDZ_PC_NONE ? Vj(j) : c->Zb + (long)j * N; };
     std::vector<cplx> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hv(m + 1), hv2(m + 1);
     double bnorm, beta0 = -1;   // beta0 >= 0: ||V_0|| of the first cycle, fetched with ||b||
+    const bool queued = c->norms_queued;
+    c->norms_queued = false;
     if (r0_in_v0) {
-        tdz::launch_norm2(c->st, b, N, c->dpartr, c->dnorm + 1);
-        tdz::launch_norm2(c->st, Vj(0), N, c->dpartr, c->dnorm);
+        if (!queued) {
+            tdz::launch_norm2(c->st, b, N, c->dpartr, c->dnorm + 1);
+            tdz::launch_norm2(c->st, Vj(0), N, c->dpartr, c->dnorm);
+        }
         double h2[2] = {0, 0};
         HIPCHK(hipMemcpyAsync(h2, c->dnorm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
@@ -972,6 +978,7 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
         c->hstage_len = 2 * (m + 1);
     }
     dcomplex *ystage = c->hstage, *rstage = c->hstage + (m + 1);
+    double last_hn = 0;   // ||w|| of the last Arnoldi step
     int its = 0, restarts = 0;
     double res = 0;
     if (bnorm == 0.0) {
@@ -1024,7 +1031,7 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = cplx(h1[i].re + h2[i].re, h1[i].im + h2[i].im);
             const double hn = std::sqrt(wn2);
             H[(size_t)(j + 1) * m + j] = hn;
-            if (hn > 0) tdz::launch_scale_invsqrt(c->st, w, c->dnorm, w, N);
+            last_hn = hn;
             for (int i = 0; i < j; ++i) {
                 const cplx a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
                 H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
@@ -1041,6 +1048,9 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             ++its;
             res = std::abs(gv[j + 1]) / bnorm;
             if (res <= rtol || hn == 0.0) { ++j; converged = true; break; }
+            // V_{j+1} = w / ||w|| for the next step (a converged solve leaves w unscaled: only
+            // the residual update below reads it, with its coefficient divided by ||w||)
+            tdz::launch_scale_invsqrt(c->st, w, c->dnorm, w, N);
         }
         std::vector<cplx> yv(j);
         for (int i = j - 1; i >= 0; --i) {
@@ -1061,6 +1071,7 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
                 q[i + 1] = std::conj(sn[i]) * a + cs[i] * bb;
             }
             for (int i = 0; i <= j; ++i) rstage[i] = dc(-gv[j] * q[i]);
+            if (last_hn > 0) rstage[j] = dc(cplx(rstage[j].re, rstage[j].im) / last_hn);   // V_j left unscaled
             HIPCHK(hipMemcpyAsync(c->dh2, rstage, (j + 1) * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
             tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N, b);   // ax_out = b + sum
         }
@@ -1742,8 +1753,11 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
             if (p.axes >> a & 1) r0.add(dc(-w * g * sE[a]), p.D[s][a]);
         x0.add({w, 0}, p.psi[s]);
     }
-    // b, the solve's first basis vector before scaling, and the guess in one pass
-    tdz::launch_lincomb3(c->st, rb, r0, x0, c->rhs, c->Vb, x, N);
+    // b, the solve's first basis vector before scaling, the guess and the norms of the
+    // first two in one pass (dpart is free between solves: 2 (restart + 1) kRedBlocks doubles)
+    tdz::launch_lincomb3_norms(c->st, rb, r0, x0, c->rhs, c->Vb, x, N, reinterpret_cast<double *>(c->dpart),
+                               2 * tdz::kRedBlocks, c->dnorm);
+    c->norms_queued = true;
     set_shift(c, 1.0, g, sE);
     step_solve(c, c->rhs, x, rtol, maxit, st, true, c->ax);
     pred_push(c, x, c->ax, sE);

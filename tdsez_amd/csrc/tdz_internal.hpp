@@ -158,6 +158,10 @@ void launch_lincomb(hipStream_t st, const LinComb &lc, dcomplex *y, long N);
 // y1 = sum l1; y2 = y1 + sum l2; y3 = sum l3, one pass
 void launch_lincomb3(hipStream_t st, const LinComb &l1, const LinComb &l2, const LinComb &l3, dcomplex *y1,
                      dcomplex *y2, dcomplex *y3, long N);
+// the same, with out[0] = ||y2||^2 and out[1] = ||y1||^2 (on device, deterministic;
+// partial: scratch of 2 nbmax doubles)
+void launch_lincomb3_norms(hipStream_t st, const LinComb &l1, const LinComb &l2, const LinComb &l3, dcomplex *y1,
+                           dcomplex *y2, dcomplex *y3, long N, double *partial, int nbmax, double *out);
 // m = Mz My Mx x and dz = Xz My Mx x (FP64, real bands of order p on every axis, 3D);
 // false if not applicable
 bool launch_mass_dipole_z(hipStream_t st, const int n[3], int p, const double *M[3], const double *Xz,

@@ -1250,6 +1250,46 @@ __global__ void lincomb3_kernel(LinComb l1, LinComb l2, LinComb l3, dcomplex *y1
     }
 }
 
+// lincomb3 with the block sums of |y1|^2 and |y2|^2: partial[b], partial[nb + b]
+__global__ void __launch_bounds__(kRedThreads) lincomb3_norm_kernel(LinComb l1, LinComb l2, LinComb l3, dcomplex *y1,
+                                                                    dcomplex *y2, dcomplex *y3, long N,
+                                                                    double *__restrict__ partial)
+{
+    __shared__ double sh[kRedThreads / 64];
+    double s1 = 0, s2 = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        double ar = 0, ai = 0, br = 0, bi = 0, cr = 0, ci = 0;
+        for (int k = 0; k < l1.k; ++k) {
+            const dcomplex v = l1.v[k][i], c = l1.c[k];
+            ar += c.re * v.re - c.im * v.im;
+            ai += c.re * v.im + c.im * v.re;
+        }
+        br = ar;
+        bi = ai;
+        for (int k = 0; k < l2.k; ++k) {
+            const dcomplex v = l2.v[k][i], c = l2.c[k];
+            br += c.re * v.re - c.im * v.im;
+            bi += c.re * v.im + c.im * v.re;
+        }
+        for (int k = 0; k < l3.k; ++k) {
+            const dcomplex v = l3.v[k][i], c = l3.c[k];
+            cr += c.re * v.re - c.im * v.im;
+            ci += c.re * v.im + c.im * v.re;
+        }
+        y1[i] = {ar, ai};
+        y2[i] = {br, bi};
+        y3[i] = {cr, ci};
+        s1 += ar * ar + ai * ai;
+        s2 += br * br + bi * bi;
+    }
+    s1 = block_sum(s1, sh);
+    s2 = block_sum(s2, sh);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s1;
+        partial[gridDim.x + blockIdx.x] = s2;
+    }
+}
+
 // Mass and z dipole of the same vector in one z-marching pass (FP64, equal orders P):
 //   m = Mz My Mx x,  d = Xz My Mx x
 // (real bands; the x and y passes are shared). Same tiling, plane prefetch and output
@@ -1644,6 +1684,15 @@ void launch_lincomb3(hipStream_t st, const LinComb &l1, const LinComb &l2, const
                      dcomplex *y2, dcomplex *y3, long N)
 {
     hipLaunchKernelGGL(lincomb3_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, l1, l2, l3, y1, y2, y3, N);
+}
+
+void launch_lincomb3_norms(hipStream_t st, const LinComb &l1, const LinComb &l2, const LinComb &l3, dcomplex *y1,
+                           dcomplex *y2, dcomplex *y3, long N, double *partial, int nbmax, double *out)
+{
+    const unsigned nb = std::min(grid_capped(N, kRedThreads, 4096), (unsigned)nbmax);
+    hipLaunchKernelGGL(lincomb3_norm_kernel, dim3(nb), dim3(kRedThreads), 0, st, l1, l2, l3, y1, y2, y3, N, partial);
+    hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial + nb, (int)nb, out);
+    hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial, (int)nb, out + 1);
 }
 
 bool launch_mass_dipole_z(hipStream_t st, const int n[3], int p, const double *M[3], const double *Xz,
