@@ -146,7 +146,6 @@ struct LearnState {
     std::vector<float> hW1, hW2;         // host weights
     unsigned short *mb = nullptr;        // inference work [2N][C] bf16
     float *out = nullptr;                // [2N]
-    fcomplex *nvec = nullptr;            // g * network output
     // training
     float *m = nullptr, *dov = nullptr, *dh = nullptr, *partial = nullptr, *dW1 = nullptr, *dW2 = nullptr;
     double *lpart = nullptr;             // loss partial sums [kLossBlocks]
@@ -851,13 +850,14 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, const fcomplex 
         ls.ts.push_back(ts);
         ls.tnorm2.push_back(t2);
     }
+    const float *no = nullptr;   // the network's output, scaled by g / sout in the combine
     if (net) {
         const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-        tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, cx<float>(c->g / ls.sout),
-                            ls.nvec);
-        n = ls.nvec;
+        tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{0.f, 0.f}, nullptr);
+        no = ls.out;
     }
-    tdz::launch_corr_combine(c->st, L.f.r, L.f.t, c->sf.u1, n, c->dVg, cx<float>(c->g), c->sf.u1, N);
+    tdz::launch_corr_combine(c->st, L.f.r, L.f.t, c->sf.u1, n, c->dVg, cx<float>(c->g), c->sf.u1, N, no,
+                             cx<float>(c->g / ls.sout));
     fd_apply<float>(c, L, c->sf.u1, L.f.z, c->sf.w1, c->sf.w2);
     tdz::launch_promote(c->st, L.f.z, z, N);
 }
@@ -2008,7 +2008,6 @@ void setup_learned(tdz_ctx *c)
     ls.W2 = track(c, dalloc<float>(27 * C * C));
     ls.mb = track(c, dalloc<unsigned short>(tdz::cnn_infer_work_bytes(N) / 2));
     ls.out = track(c, dalloc<float>((size_t)2 * N));
-    ls.nvec = track(c, dalloc<fcomplex>(N));
     ls.feat = track(c, dalloc<fcomplex>((size_t)CI * N));
     for (int d = 0; d < 3; ++d) {
         const size_t sz = L.M[d].size();

@@ -171,9 +171,11 @@ void launch_axpby_t(hipStream_t st, cpx<R> a, const cpx<R> *x, cpx<R> b, cpx<R> 
 // precision conversions
 void launch_demote(hipStream_t st, const dcomplex *x, fcomplex *y, long N);
 void launch_promote(hipStream_t st, const fcomplex *x, dcomplex *y, long N);
-// q = r - g Vg .* My - c - n (n may be null), complex64
+// q = r - g Vg .* My - c - n (n may be null), complex64; with no (the network's [2N]
+// output as two real halves) also - ng (no_re + i no_im)
 void launch_corr_combine(hipStream_t st, const fcomplex *r, const fcomplex *My, const fcomplex *c,
-                         const fcomplex *n, const float *Vg, fcomplex g, fcomplex *q, long N);
+                         const fcomplex *n, const float *Vg, fcomplex g, fcomplex *q, long N,
+                         const float *no = nullptr, fcomplex ng = {0.f, 0.f});
 // y = x * s where s = 1/sqrt(*norm2) read on device
 void launch_scale_invsqrt(hipStream_t st, const dcomplex *x, const double *norm2, dcomplex *y, long N);
 // out = ||x||^2 on device

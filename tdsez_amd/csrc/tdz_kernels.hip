@@ -875,8 +875,9 @@ __global__ void promote_kernel(const fcomplex *__restrict__ x, dcomplex *__restr
 // q = r - g Vg .* My - c - n (n optional): the right-hand side of the second
 // separable solve of the first-order corrected preconditioner
 __global__ void corr_combine_kernel(const fcomplex *__restrict__ r, const fcomplex *__restrict__ My,
-                                    const fcomplex *__restrict__ c, const fcomplex *__restrict__ n,
-                                    const float *__restrict__ Vg, fcomplex g, fcomplex *__restrict__ q, long N)
+                                    const fcomplex *c, const fcomplex *__restrict__ n,
+                                    const float *__restrict__ Vg, fcomplex g, fcomplex *q, long N,
+                                    const float *__restrict__ no, fcomplex ng)
 {
     const long stride = (long)gridDim.x * blockDim.x;
     for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
@@ -885,6 +886,11 @@ __global__ void corr_combine_kernel(const fcomplex *__restrict__ r, const fcompl
         float qr = rv.re - v * (g.re * m.re - g.im * m.im) - cv.re;
         float qi = rv.im - v * (g.re * m.im + g.im * m.re) - cv.im;
         if (n) { qr -= n[I].re; qi -= n[I].im; }
+        if (no) {
+            const float ore = no[I], oim = no[N + I];
+            qr -= ng.re * ore - ng.im * oim;
+            qi -= ng.re * oim + ng.im * ore;
+        }
         q[I] = {qr, qi};
     }
 }
@@ -1746,10 +1752,11 @@ void launch_promote(hipStream_t st, const fcomplex *x, dcomplex *y, long N)
 }
 
 void launch_corr_combine(hipStream_t st, const fcomplex *r, const fcomplex *My, const fcomplex *c,
-                         const fcomplex *n, const float *Vg, fcomplex g, fcomplex *q, long N)
+                         const fcomplex *n, const float *Vg, fcomplex g, fcomplex *q, long N, const float *no,
+                         fcomplex ng)
 {
     hipLaunchKernelGGL(corr_combine_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, r, My, c, n, Vg, g, q,
-                       N);
+                       N, no, ng);
 }
 
 void launch_scale_invsqrt(hipStream_t st, const dcomplex *x, const double *norm2, dcomplex *y, long N)

@@ -301,7 +301,11 @@ __device__ __forceinline__ short to_bf16(float f) { return __builtin_bit_cast(sh
 // operand order) are loaded once per block.
 constexpr int TX = 16, TY = 4, TZ = 12, WZ = TZ / 4;
 constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
-constexpr int NPACK = NTAP + K1 / 16;   // packed operand slots: 27 layer-2 taps, then the layer-1 K steps
+// packed weights (behind m in the work buffer): the layer-1 K steps as 8-byte operands,
+// then from byte kPack2Offset the layer-2 tap pairs as 16-byte operands (2 per (dz, dy))
+constexpr int NPACK1 = K1 / 16, NPACK2 = 2 * 9, NPACK = NPACK1 + NPACK2;
+constexpr int kPack2Offset = 4096;
+static_assert(NPACK1 * 64 * 8 <= kPack2Offset, "layer-1 slots fit before the layer-2 slots");
 
 struct TileIdx {
     int comp, x0, y0, z0;
@@ -335,21 +339,34 @@ __device__ __forceinline__ uint2 pack4(float a, float b, float c, float d)
             (unsigned)(unsigned short)to_bf16(c) | ((unsigned)(unsigned short)to_bf16(d) << 16)};
 }
 
-// wp[slot * 64 + lane]: the lane's 4 bf16 B-operand values. Slots t < 27: W2[t][kb+q][row];
-// slots 27 + s: W1[t][q][row] with t = 4 s + kb / 4 (zero past the last tap).
-__global__ void cnn_pack_kernel(const float *__restrict__ W1, const float *__restrict__ W2, uint2 *__restrict__ wp)
+// Layer 1, wp[s * 64 + lane] (s < NPACK1): the lane's 4 bf16 A-operand values W1[t][q][row],
+// t = 4 s + kb / 4 (zero past the last tap).
+// Layer 2, wp2[(2 p + half) * 64 + lane], p = 3 (dz + 1) + (dy + 1): the lane's 8 bf16
+// A-operand values of a 16x16x32 product that takes two taps of the same (dz, dy), the
+// lane group g = lane / 16 holding input channels 8 (g & 1) .. +7 of tap
+//   half 0: dx = -1 (g < 2), dx = 0 (g >= 2);   half 1: dx = +1 (g < 2), zero (g >= 2),
+// value W2[t][a][row].
+__global__ void cnn_pack_kernel(const float *__restrict__ W1, const float *__restrict__ W2, uint2 *__restrict__ wp,
+                                uint4 *__restrict__ wp2)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= NPACK * 64) return;
-    const int slot = e / 64, lane = e % 64, row = lane & 15, kb = (lane >> 4) * 4;
-    float v[4];
-    if (slot < NTAP) {
-        for (int q = 0; q < 4; ++q) v[q] = W2[(slot * C + kb + q) * C + row];
-    } else {
-        const int t = 4 * (slot - NTAP) + kb / 4;
+    const int slot = e / 64, lane = e % 64, row = lane & 15, g = lane >> 4;
+    if (slot < NPACK1) {
+        const int t = 4 * slot + g;
+        float v[4];
         for (int q = 0; q < 4; ++q) v[q] = t < NTAP ? W1[(t * CI + q) * C + row] : 0.f;
+        wp[slot * 64 + lane] = pack4(v[0], v[1], v[2], v[3]);
+        return;
     }
-    wp[e] = pack4(v[0], v[1], v[2], v[3]);
+    const int s2 = slot - NPACK1, p = s2 / 2, half = s2 % 2;
+    const int dx = half == 0 ? (g < 2 ? -1 : 0) : 1;
+    const bool zero = half == 1 && g >= 2;
+    const int t = 9 * (p / 3) + 3 * (p % 3) + dx + 1;
+    float v[8];
+    for (int j = 0; j < 8; ++j) v[j] = zero ? 0.f : W2[(t * C + 8 * (g & 1) + j) * C + row];
+    const uint2 lo = pack4(v[0], v[1], v[2], v[3]), hi = pack4(v[4], v[5], v[6], v[7]);
+    wp2[s2 * 64 + lane] = {lo.x, lo.y, hi.x, hi.y};
 }
 
 // The inference reads the positional fields as bf16, voxel-major: Pb[v C + a]. The
@@ -413,7 +430,7 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
     const int row = lane & 15, kb = (lane >> 4) * 4;
     bf16x4 bw[K1 / 16];
 #pragma unroll
-    for (int s = 0; s < K1 / 16; ++s) bw[s] = unpack4(wp[(NTAP + s) * 64 + lane]);
+    for (int s = 0; s < K1 / 16; ++s) bw[s] = unpack4(wp[s * 64 + lane]);
     __syncthreads();
     const uint2 *us0 = usv[0], *us1 = usv[1];
     const int gx = T.x0 + row;
@@ -457,20 +474,24 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
     }
 }
 
-// Layer 2 on MFMA: per tap t, A[b][a] = W2[t][a][b], B[a][i] = m[v_i + t][a]; the 27
-// products accumulate into D[b][i] (lane l holds output channels 4(l/16) .. +3 of voxel
-// v_{l%16}); out(v_i) = sum_b P[v_i][b] D[b][i] (4 channels in the lane, then over the
-// 4 lanes that hold voxel i's channel groups). The halo tile of m lives in LDS with a 40-byte
-// voxel stride (16 bf16 channels + 4 padding), which spreads the 16 lanes of an 8-byte
-// A-operand read over distinct banks.
-constexpr int MS = 20;   // LDS stride of one voxel of m, in bf16 elements
+// Layer 2 on MFMA, v_mfma_f32_16x16x32_bf16 (lane l holds A[l%16][8(l/16)+j],
+// B[8(l/16)+j][l%16], j < 8, and D[4(l/16)+q][l%16]): per (dz, dy) two products of K = 32,
+// each taking two taps of the same output row (dx = -1 and 0, then dx = +1 and zero
+// weights): A[b][k] = W2[t(k)][a(k)][b], B[k][i] = m[v_i + t(k)][a(k)], k < 16 the first
+// tap. The 27 taps accumulate into D[b][i] (lane l holds output channels 4(l/16) .. +3 of
+// voxel v_{l%16}); out(v_i) = sum_b P[v_i][b] D[b][i] (4 channels in the lane, then over
+// the 4 lanes that hold voxel i's channel groups). The halo tile of m lives in LDS with a
+// 48-byte voxel stride (16 bf16 channels + 8 padding): a lane's 8 channels are one aligned
+// 16-byte read, and the 16 lanes of a lane group start on distinct 4-bank groups.
+constexpr int MS = 24;   // LDS stride of one voxel of m, in bf16 elements
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2,
                                                           const unsigned short *__restrict__ mb,
-                                                          const uint2 *__restrict__ wp,
+                                                          const uint4 *__restrict__ wp2,
                                                           const unsigned short *__restrict__ Pb,
                                                           float *__restrict__ out)
 {
-    __shared__ uint2 ms[HV * MS / 4];
+    __shared__ uint4 ms[HV * MS / 8];
     const TileIdx T = tile_of(n0, n1, n2);
     const long N = (long)n0 * n1 * n2;
     // stage one halo voxel (its 16 channels, 2 x 16 bytes) per thread and round, in two
@@ -491,22 +512,20 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
     for (int r = 0; r < NV; ++r) {
         const int h = threadIdx.x + 256 * r;
         if (h < HV) {
-            uint2 *dst = ms + (h * MS) / 4;
-            dst[0] = {val[r][0].x, val[r][0].y};
-            dst[1] = {val[r][0].z, val[r][0].w};
-            dst[2] = {val[r][1].x, val[r][1].y};
-            dst[3] = {val[r][1].z, val[r][1].w};
+            ms[h * (MS / 8)] = val[r][0];
+            ms[h * (MS / 8) + 1] = val[r][1];
         }
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int row = lane & 15, kb = (lane >> 4) * 4;
-    bf16x4 bw[NTAP];
+    const int row = lane & 15, kb = (lane >> 4) * 4, g = lane >> 4;
+    bf16x8 bw[NPACK2];
 #pragma unroll
-    for (int t = 0; t < NTAP; ++t) bw[t] = unpack4(wp[t * 64 + lane]);
+    for (int s = 0; s < NPACK2; ++s) bw[s] = __builtin_bit_cast(bf16x8, wp2[s * 64 + lane]);
     __syncthreads();
-    // Halo-row-major order: each B fragment (16 voxels of one halo row, shifted by dx) is
-    // read from LDS once and feeds the up to 9 output rows (ry, pz) it is a tap of, so
-    // the wave reads (TY + 2)(WZ + 2) 3 fragments instead of 27 TY WZ.
+    // Halo-row-major order: the two B fragments of a halo row (16 voxels shifted by dx =
+    // -1 | 0 and by +1, 8 channels per lane) are read from LDS once and feed the up to 9
+    // output rows (ry, pz) they are taps of.
+    const int dxa = g < 2 ? -1 : 0, cq = (g & 1);   // first fragment's shift; 16-byte quarter of the voxel
     floatx4 acc[WZ][TY];
 #pragma unroll
     for (int pz = 0; pz < WZ; ++pz)
@@ -515,26 +534,26 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
 #pragma unroll
     for (int hz = 0; hz < WZ + 2; ++hz)
 #pragma unroll
-        for (int hy = 0; hy < TY + 2; ++hy)
+        for (int hy = 0; hy < TY + 2; ++hy) {
+            const int ha = tap_of_halo(row + 1 + dxa, hy, w * WZ + hz), hb = tap_of_halo(row + 2, hy, w * WZ + hz);
+            const bf16x8 fa = __builtin_bit_cast(bf16x8, ms[ha * (MS / 8) + cq]);
+            const bf16x8 fb = __builtin_bit_cast(bf16x8, ms[hb * (MS / 8) + cq]);
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int h = tap_of_halo(row + 1 + dx, hy, w * WZ + hz);
-                const bf16x4 a = unpack4(ms[(h * MS) / 4 + kb / 4]);
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int pz = hz - 1 - dz;   // output plane w WZ + pz reads halo plane w WZ + hz
+                if (pz < 0 || pz >= WZ) continue;
 #pragma unroll
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const int pz = hz - 1 - dz;   // output plane w WZ + pz reads halo plane w WZ + hz
-                    if (pz < 0 || pz >= WZ) continue;
-#pragma unroll
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const int ry = hy - 1 - dy;
-                        if (ry < 0 || ry >= TY) continue;
-                        const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-                        acc[pz][ry] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[t], a, acc[pz][ry], 0, 0, 0);
-                    }
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int ry = hy - 1 - dy;
+                    if (ry < 0 || ry >= TY) continue;
+                    const int p = 3 * (dz + 1) + (dy + 1);
+                    acc[pz][ry] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[2 * p], fa, acc[pz][ry], 0, 0, 0);
+                    acc[pz][ry] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[2 * p + 1], fb, acc[pz][ry], 0, 0, 0);
                 }
             }
+        }
     static_assert(TY == 4, "the 4 channel groups of a voxel end with one row each");
-    const int gx = T.x0 + row, g = lane >> 4;
+    const int gx = T.x0 + row;
 #pragma unroll
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
@@ -961,10 +980,11 @@ void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const flo
     const long tiles = (long)((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ);
     // the packed weights go behind the 2 N C bf16 of m (the work buffer holds both)
     uint2 *wp = reinterpret_cast<uint2 *>(mb + 2 * N * C);
-    hipLaunchKernelGGL(cnn_pack_kernel, dim3(gridn(NPACK * 64, 256)), dim3(256), 0, st, W1, W2, wp);
+    uint4 *wp2 = reinterpret_cast<uint4 *>(reinterpret_cast<char *>(wp) + kPack2Offset);
+    hipLaunchKernelGGL(cnn_pack_kernel, dim3(gridn(NPACK * 64, 256)), dim3(256), 0, st, W1, W2, wp, wp2);
     // layer 1: one block per tile (both components); layer 2: one per tile and component
     hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)tiles), dim3(256), 0, st, n[0], n[1], n[2], u, wp, Pb, mb);
-    hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)(2 * tiles)), dim3(256), 0, st, n[0], n[1], n[2], mb, wp,
+    hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)(2 * tiles)), dim3(256), 0, st, n[0], n[1], n[2], mb, wp2,
                        Pb, out);
     if (nout) hipLaunchKernelGGL(cnn_to_complex_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, N, out, g, nout);
 }
