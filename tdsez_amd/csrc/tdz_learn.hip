@@ -373,18 +373,20 @@ __global__ void cnn_pack_kernel(const float *__restrict__ W1, const float *__res
 // lane that holds channels kb .. kb + 3 of voxel (gx, gy, gz) reads their 4 fields as
 // one aligned 8-byte load; the 64 lanes of a wave (16 voxels along x, 4 channel groups)
 // read 512 contiguous bytes.
-__device__ __forceinline__ void load_fields(const unsigned short *__restrict__ Pb, int n0, int n1, int n2, int kb,
-                                            int gx, int gy, int gz, float pv[4])
+// Issued well ahead of their use (their latency is not covered by one plane's MFMAs).
+__device__ __forceinline__ uint2 load_fields(const unsigned short *__restrict__ Pb, int n0, int n1, int n2, int kb,
+                                             int gx, int gy, int gz)
 {
-    if (gx < n0 && gy < n1 && gz < n2) {
-        const uint2 raw = *reinterpret_cast<const uint2 *>(Pb + (gx + (long)n0 * (gy + (long)n1 * gz)) * C + kb);
-        pv[0] = __uint_as_float(raw.x << 16);
-        pv[1] = __uint_as_float(raw.x & 0xFFFF0000u);
-        pv[2] = __uint_as_float(raw.y << 16);
-        pv[3] = __uint_as_float(raw.y & 0xFFFF0000u);
-    } else {
-        pv[0] = pv[1] = pv[2] = pv[3] = 0.f;
-    }
+    if (gx < n0 && gy < n1 && gz < n2)
+        return *reinterpret_cast<const uint2 *>(Pb + (gx + (long)n0 * (gy + (long)n1 * gz)) * C + kb);
+    return {0u, 0u};
+}
+__device__ __forceinline__ void unpack_fields(uint2 raw, float pv[4])
+{
+    pv[0] = __uint_as_float(raw.x << 16);
+    pv[1] = __uint_as_float(raw.x & 0xFFFF0000u);
+    pv[2] = __uint_as_float(raw.y << 16);
+    pv[3] = __uint_as_float(raw.y & 0xFFFF0000u);
 }
 
 // Layer 1 on MFMA, v_mfma_f32_16x16x16_bf16, both components per block (the inputs are
@@ -434,13 +436,19 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
     __syncthreads();
     const uint2 *us0 = usv[0], *us1 = usv[1];
     const int gx = T.x0 + row;
+    // positional values of this lane's outputs, one plane ahead
+    uint2 pnext[TY];
+#pragma unroll
+    for (int ry = 0; ry < TY; ++ry) pnext[ry] = load_fields(Pb, n0, n1, n2, kb, gx, T.y0 + ry, T.z0 + w * WZ);
 #pragma unroll 1   // unrolled, the plane loop runs slower (code size and register pressure)
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
-        // the plane's positional values for this lane's outputs, loaded ahead of the MFMAs
         float pv[TY][4];
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry) load_fields(Pb, n0, n1, n2, kb, gx, T.y0 + ry, gz, pv[ry]);
+        for (int ry = 0; ry < TY; ++ry) unpack_fields(pnext[ry], pv[ry]);
+        if (pz + 1 < WZ)
+#pragma unroll
+            for (int ry = 0; ry < TY; ++ry) pnext[ry] = load_fields(Pb, n0, n1, n2, kb, gx, T.y0 + ry, gz + 1);
         floatx4 acc[2][TY];
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) acc[0][ry] = acc[1][ry] = {0.f, 0.f, 0.f, 0.f};
@@ -521,6 +529,13 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
     bf16x8 bw[NPACK2];
 #pragma unroll
     for (int s = 0; s < NPACK2; ++s) bw[s] = __builtin_bit_cast(bf16x8, wp2[s * 64 + lane]);
+    // positional values of this lane's outputs, in flight during the products
+    uint2 praw[WZ][TY];
+#pragma unroll
+    for (int pz = 0; pz < WZ; ++pz)
+#pragma unroll
+        for (int ry = 0; ry < TY; ++ry)
+            praw[pz][ry] = load_fields(Pb, n0, n1, n2, kb, T.x0 + row, T.y0 + ry, T.z0 + w * WZ + pz);
     __syncthreads();
     // Halo-row-major order: the two B fragments of a halo row (16 voxels shifted by dx =
     // -1 | 0 and by +1, 8 channels per lane) are read from LDS once and feed the up to 9
@@ -562,7 +577,7 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) {
             float pv[4];
-            load_fields(Pb, n0, n1, n2, kb, gx, T.y0 + ry, gz, pv);
+            unpack_fields(praw[pz][ry], pv);
             v4[ry] = pv[0] * acc[pz][ry][0] + pv[1] * acc[pz][ry][1] + pv[2] * acc[pz][ry][2] +
                      pv[3] * acc[pz][ry][3];
         }
