@@ -780,8 +780,10 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
         cy[l] = fb.C[1][jc * R + l];
     }
     fcomplex aM[R], aXX[R], aXY[R], aXZ[R], aC[R];
+    fcomplex yr[R];   // y at this thread's (i, j) of the last R staged planes (plane K in slot K mod R)
 #pragma unroll
-    for (int l = 0; l < R; ++l) aM[l] = aXX[l] = aXY[l] = aXZ[l] = aC[l] = {0.f, 0.f};
+    for (int l = 0; l < R; ++l) aM[l] = aXX[l] = aXY[l] = aXZ[l] = aC[l] = yr[l] = {0.f, 0.f};
+    const float g0 = fb.g[0][ic], g1 = fb.g[1][jc];
     const long plane = (long)n0 * n1;
     const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
     constexpr int NX = (H * H + 255) / 256;
@@ -808,6 +810,7 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
                 }
                 __syncthreads();
                 if (K + 1 < Kend) load_plane(K + 1);
+                if (u) yr[r] = Ys[(tj + P) * H + ti + P];
                 for (int e = tid; e < H * T; e += 256) {   // e % T == ti
                     const int jj = e / T;
                     float tr = 0, tim = 0, qr = 0, qi = 0, rr = 0, ri = 0;
@@ -869,8 +872,8 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
                 corr[I] = aC[r];
                 if (u) {
                     const long N = plane * n2;
-                    const float g0 = fb.g[0][i], g1 = fb.g[1][j], g2 = fb.g[2][ko];
-                    const fcomplex yc = y[I];
+                    const float g2 = fb.g[2][ko];
+                    const fcomplex yc = yr[(r + P + 1) % R];   // plane ko = K - P
                     u[I] = {sc.v[0] * yc.re, sc.v[0] * yc.im};
                     u[N + I] = {sc.v[1] * (aXX[r].re - g0 * m0.re), sc.v[1] * (aXX[r].im - g0 * m0.im)};
                     u[2 * N + I] = {sc.v[2] * (aXY[r].re - g1 * m0.re), sc.v[2] * (aXY[r].im - g1 * m0.im)};
