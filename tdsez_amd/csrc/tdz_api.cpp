@@ -970,14 +970,18 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
     } else {
         bnorm = std::sqrt(dev_norm2(c, b, N));
     }
-    // staging: now past a wait, so no earlier upload still reads it
-    if (c->hstage_len < 2 * (m + 1)) {
+    // staging: now past a wait, so no earlier copy still uses it. [0, 2(m+1)]: the
+    // projections and the norm read back (the device layout of dh, dh2, dnorm); then the
+    // solution and residual coefficients uploaded to dh, dh2
+    const int slen = 4 * (m + 1) + 1;
+    if (c->hstage_len < slen) {
         if (c->hstage) HIPCHK(hipHostFree(c->hstage));
         c->hstage = nullptr;
-        HIPCHK(hipHostMalloc((void **)&c->hstage, 2 * (m + 1) * sizeof(dcomplex), hipHostMallocDefault));
-        c->hstage_len = 2 * (m + 1);
+        HIPCHK(hipHostMalloc((void **)&c->hstage, slen * sizeof(dcomplex), hipHostMallocDefault));
+        c->hstage_len = slen;
     }
-    dcomplex *ystage = c->hstage, *rstage = c->hstage + (m + 1);
+    const dcomplex *hback = c->hstage;
+    dcomplex *ystage = c->hstage + 2 * (m + 1) + 1, *rstage = ystage + (m + 1);
     double last_hn = 0;   // ||w|| of the last Arnoldi step
     int its = 0, restarts = 0;
     double res = 0;
@@ -1022,12 +1026,11 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             tdz::launch_multi_axpy_norm(c->st, c->Vb, N, j + 1, c->dh, w, N, c->dpartr, c->dnorm);
             tdz::launch_multi_dot(c->st, c->Vb, N, j + 1, w, N, c->dpart, c->dh2);
             tdz::launch_multi_axpy_norm(c->st, c->Vb, N, j + 1, c->dh2, w, N, c->dpartr, c->dnorm);
-            std::vector<dcomplex> h1(j + 1), h2(j + 1);
-            double wn2 = 0;
-            HIPCHK(hipMemcpyAsync(h1.data(), c->dh, (j + 1) * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
-            HIPCHK(hipMemcpyAsync(h2.data(), c->dh2, (j + 1) * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
-            HIPCHK(hipMemcpyAsync(&wn2, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(c->hstage, c->dh, (2 * (m + 1) + 1) * sizeof(dcomplex), hipMemcpyDeviceToHost,
+                                  c->st));
             HIPCHK(hipStreamSynchronize(c->st));
+            const dcomplex *h1 = hback, *h2 = hback + (m + 1);
+            const double wn2 = reinterpret_cast<const double *>(hback + 2 * (m + 1))[0];
             for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = cplx(h1[i].re + h2[i].re, h1[i].im + h2[i].im);
             const double hn = std::sqrt(wn2);
             H[(size_t)(j + 1) * m + j] = hn;
@@ -1059,9 +1062,8 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             yv[i] = s / H[(size_t)i * m + i];
         }
         for (int i = 0; i < j; ++i) ystage[i] = dc(yv[i]);
-        HIPCHK(hipMemcpyAsync(c->dh, ystage, j * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
-        tdz::launch_multi_axpy(c->st, c->pc_kind == TDZ_PC_NONE ? c->Vb : c->Zb, N, j, c->dh, x, N);
-        if (converged && ax_out) {
+        const bool with_ax = converged && ax_out;
+        if (with_ax) {
             // q = Q^H e_j: the inverse rotations, newest first
             std::vector<cplx> q(j + 1, cplx(0));
             q[j] = 1.0;
@@ -1072,9 +1074,12 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             }
             for (int i = 0; i <= j; ++i) rstage[i] = dc(-gv[j] * q[i]);
             if (last_hn > 0) rstage[j] = dc(cplx(rstage[j].re, rstage[j].im) / last_hn);   // V_j left unscaled
-            HIPCHK(hipMemcpyAsync(c->dh2, rstage, (j + 1) * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
-            tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N, b);   // ax_out = b + sum
         }
+        // y to dh and, with A x, the residual coefficients to dh2 = dh + m + 1: one copy
+        HIPCHK(hipMemcpyAsync(c->dh, ystage, (with_ax ? (m + 1) + (j + 1) : j) * sizeof(dcomplex),
+                              hipMemcpyHostToDevice, c->st));
+        tdz::launch_multi_axpy(c->st, c->pc_kind == TDZ_PC_NONE ? c->Vb : c->Zb, N, j, c->dh, x, N);
+        if (with_ax) tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N, b);   // ax_out = b + sum
         if (converged || its >= maxit) break;
     }
     if (st) *st = {its, restarts, res, bnorm, c->rebuilds};
@@ -1318,10 +1323,11 @@ void build(tdz_ctx *c, const tdz_problem &pr)
     c->Vb = track(c, dalloc<dcomplex>((size_t)(c->restart + 1) * N));
     c->Zb = track(c, dalloc<dcomplex>((size_t)c->restart * N));
     c->dpart = track(c, dalloc<dcomplex>((size_t)(c->restart + 1) * tdz::kRedBlocks));
-    c->dh = track(c, dalloc<dcomplex>(c->restart + 1));
-    c->dh2 = track(c, dalloc<dcomplex>(c->restart + 1));
+    // dh, dh2 and dnorm in one block: an Arnoldi step reads them back with one copy
+    c->dh = track(c, dalloc<dcomplex>(2 * (c->restart + 1) + 1));
+    c->dh2 = c->dh + (c->restart + 1);
     c->dpartr = track(c, dalloc<double>(tdz::kRedBlocks));
-    c->dnorm = track(c, dalloc<double>(2));
+    c->dnorm = reinterpret_cast<double *>(c->dh + 2 * (c->restart + 1));
     c->prob.knots = nullptr;   // the caller's arrays are not kept
     c->prob.layer_bounds = c->prob.layer_mass = c->prob.layer_v = nullptr;
     const double zero[3] = {0, 0, 0};
@@ -1647,16 +1653,15 @@ void pred_push(tdz_ctx *c, const dcomplex *x, const dcomplex *ax, const double s
         p.psi[h] = track(c, dalloc<dcomplex>(N));
         p.P[h] = track(c, dalloc<dcomplex>(N));
     }
-    HIPCHK(hipMemcpyAsync(p.psi[h], x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
     if (!p.Mx[h]) p.Mx[h] = track(c, dalloc<dcomplex>(N));
     for (int a = 0; a < 3; ++a)
         if ((p.axes >> a & 1) && !p.D[h][a]) p.D[h][a] = track(c, dalloc<dcomplex>(N));
-    // M x and D_a x: one fused pass for a field along z alone, else one apply each
+    // x, M x and D_a x: one fused pass for a field along z alone, else a copy and one apply each
     bool fused = false;
     if (p.axes == 4u && c->prob.dim == 3 && L.ax[0].p == L.ax[2].p && L.ax[1].p == L.ax[2].p) {
         if (!p.Xz) p.Xz = upload(c, L.Xb[2]);
         const double *M[3] = {L.d.M[0], L.d.M[1], L.d.M[2]};
-        fused = tdz::launch_mass_dipole_z(c->st, L.n, L.ax[2].p, M, p.Xz, x, p.Mx[h], p.D[h][2]);
+        fused = tdz::launch_mass_dipole_z(c->st, L.n, L.ax[2].p, M, p.Xz, x, p.Mx[h], p.D[h][2], p.psi[h]);
         if (fused && c->prob.dirichlet) {   // walls: identity rows of M, no field
             if (!c->zerov) c->zerov = upload(c, std::vector<dcomplex>(L.N, dcomplex{0, 0}));
             const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
@@ -1664,6 +1669,7 @@ void pred_push(tdz_ctx *c, const dcomplex *x, const dcomplex *ax, const double s
         }
     }
     if (!fused) {
+        HIPCHK(hipMemcpyAsync(p.psi[h], x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
         for (int a = 0; a < 3; ++a)
             if (p.axes >> a & 1) dipole_apply(c, a, x, p.D[h][a]);
         tdz::SepBandsT<double> mb = sep_bands<double>(L);

@@ -162,10 +162,10 @@ void launch_lincomb3(hipStream_t st, const LinComb &l1, const LinComb &l2, const
 // partial: scratch of 2 nbmax doubles)
 void launch_lincomb3_norms(hipStream_t st, const LinComb &l1, const LinComb &l2, const LinComb &l3, dcomplex *y1,
                            dcomplex *y2, dcomplex *y3, long N, double *partial, int nbmax, double *out);
-// m = Mz My Mx x and dz = Xz My Mx x (FP64, real bands of order p on every axis, 3D);
-// false if not applicable
+// m = Mz My Mx x and dz = Xz My Mx x (FP64, real bands of order p on every axis, 3D), and
+// xc = x if xc; false if not applicable
 bool launch_mass_dipole_z(hipStream_t st, const int n[3], int p, const double *M[3], const double *Xz,
-                          const dcomplex *x, dcomplex *m, dcomplex *dz);
+                          const dcomplex *x, dcomplex *m, dcomplex *dz, dcomplex *xc = nullptr);
 template <typename R>
 void launch_axpby_t(hipStream_t st, cpx<R> a, const cpx<R> *x, cpx<R> b, cpx<R> *y, long N);
 // precision conversions

@@ -1305,7 +1305,7 @@ __global__ void __launch_bounds__(256) mass_dip_kernel(int n0, int n1, int n2, i
                                                        const double *__restrict__ Mx, const double *__restrict__ My,
                                                        const double *__restrict__ Mz, const double *__restrict__ Xz,
                                                        const dcomplex *__restrict__ x, dcomplex *__restrict__ m,
-                                                       dcomplex *__restrict__ dz)
+                                                       dcomplex *__restrict__ dz, dcomplex *__restrict__ xc)
 {
     constexpr int R = 2 * P + 1, T = 16, H = T + 2 * P;
     __shared__ dcomplex Xs[H * H];
@@ -1354,6 +1354,7 @@ __global__ void __launch_bounds__(256) mass_dip_kernel(int n0, int n1, int n2, i
                 }
                 __syncthreads();
                 if (K + 1 < Kend) load_plane(K + 1);
+                if (xc && own && K >= k0 && K < k1) xc[K * plane + (long)j * n0 + i] = Xs[(tj + P) * H + ti + P];
                 for (int e = tid; e < H * T; e += 256) {   // e % T == ti
                     const int jj = e / T;
                     double tr = 0, tim = 0;
@@ -1702,7 +1703,7 @@ void launch_lincomb3_norms(hipStream_t st, const LinComb &l1, const LinComb &l2,
 }
 
 bool launch_mass_dipole_z(hipStream_t st, const int n[3], int p, const double *M[3], const double *Xz,
-                          const dcomplex *x, dcomplex *m, dcomplex *dz)
+                          const dcomplex *x, dcomplex *m, dcomplex *dz, dcomplex *xc)
 {
     const int n0 = n[0], n1 = n[1], n2 = n[2];
     if (p < 2 || p > 5 || n2 <= 2 * p) return false;
@@ -1719,7 +1720,7 @@ bool launch_mass_dipole_z(hipStream_t st, const int n[3], int p, const double *M
     const unsigned nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
 #define TDZ_MD(PP)                                                                                              \
     hipLaunchKernelGGL(mass_dip_kernel<PP>, dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, M[0], M[1], M[2], Xz, x, \
-                       m, dz)
+                       m, dz, xc)
     switch (p) {
     case 2: TDZ_MD(2); break;
     case 3: TDZ_MD(3); break;
