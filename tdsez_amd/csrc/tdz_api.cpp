@@ -1759,12 +1759,13 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
             if (p.axes >> a & 1) r0.add(dc(-w * g * sE[a]), p.D[s][a]);
         x0.add({w, 0}, p.psi[s]);
     }
+    set_shift(c, 1.0, g, sE);
     // b, the solve's first basis vector before scaling, the guess and the norms of the
-    // first two in one pass (dpart is free between solves: 2 (restart + 1) kRedBlocks doubles)
+    // first two in one pass (dpart is free between solves: 2 (restart + 1) kRedBlocks
+    // doubles); the solve that follows consumes the queued norms
     tdz::launch_lincomb3_norms(c->st, rb, r0, x0, c->rhs, c->Vb, x, N, reinterpret_cast<double *>(c->dpart),
                                2 * tdz::kRedBlocks, c->dnorm);
     c->norms_queued = true;
-    set_shift(c, 1.0, g, sE);
     step_solve(c, c->rhs, x, rtol, maxit, st, true, c->ax);
     pred_push(c, x, c->ax, sE);
 }
