@@ -754,13 +754,15 @@ __global__ void reduce_complex_kernel(const dcomplex *__restrict__ partial, int 
     if (threadIdx.x == 0) out[i] = {sr, si};
 }
 
+// out[i] = sum_b partial[i nb + b], one block per i, fixed order
 __global__ void reduce_real_kernel(const double *__restrict__ partial, int nb, double *__restrict__ out)
 {
     __shared__ double sh[kRedThreads / 64];
+    const double *p = partial + (long)blockIdx.x * nb;
     double s = 0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[b];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += p[b];
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[0] = s;
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 __global__ void __launch_bounds__(kRedThreads) multi_axpy_norm_kernel(const dcomplex *__restrict__ V, long ld, int nv,
@@ -1256,7 +1258,7 @@ __global__ void lincomb3_kernel(LinComb l1, LinComb l2, LinComb l3, dcomplex *y1
     }
 }
 
-// lincomb3 with the block sums of |y1|^2 and |y2|^2: partial[b], partial[nb + b]
+// lincomb3 with the block sums of |y2|^2 and |y1|^2: partial[b], partial[nb + b]
 __global__ void __launch_bounds__(kRedThreads) lincomb3_norm_kernel(LinComb l1, LinComb l2, LinComb l3, dcomplex *y1,
                                                                     dcomplex *y2, dcomplex *y3, long N,
                                                                     double *__restrict__ partial)
@@ -1291,8 +1293,8 @@ __global__ void __launch_bounds__(kRedThreads) lincomb3_norm_kernel(LinComb l1, 
     s1 = block_sum(s1, sh);
     s2 = block_sum(s2, sh);
     if (threadIdx.x == 0) {
-        partial[blockIdx.x] = s1;
-        partial[gridDim.x + blockIdx.x] = s2;
+        partial[blockIdx.x] = s2;
+        partial[gridDim.x + blockIdx.x] = s1;
     }
 }
 
@@ -1698,8 +1700,7 @@ void launch_lincomb3_norms(hipStream_t st, const LinComb &l1, const LinComb &l2,
 {
     const unsigned nb = std::min(grid_capped(N, kRedThreads, 4096), (unsigned)nbmax);
     hipLaunchKernelGGL(lincomb3_norm_kernel, dim3(nb), dim3(kRedThreads), 0, st, l1, l2, l3, y1, y2, y3, N, partial);
-    hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial + nb, (int)nb, out);
-    hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial, (int)nb, out + 1);
+    hipLaunchKernelGGL(reduce_real_kernel, dim3(2), dim3(kRedThreads), 0, st, partial, (int)nb, out);   // out[0], out[1]
 }
 
 bool launch_mass_dipole_z(hipStream_t st, const int n[3], int p, const double *M[3], const double *Xz,
