@@ -144,6 +144,7 @@ struct LearnState {
     tdz::CnnScales sin{{1.f, 1.f, 1.f, 1.f}};   // input scales (unit mean square over the samples)
     double sout = 1.0;                   // target scale: the network outputs sout * t
     std::vector<float> hW1, hW2;         // host weights
+    unsigned wver = 0, packed_ver = ~0u;  // version of the device weights; the one packed for inference
     unsigned short *mb = nullptr;        // inference work [2N][C] bf16
     float *out = nullptr;                // [2N]
     // training
@@ -159,6 +160,14 @@ struct LearnState {
     bool fallen = false;  // the network regressed: the fallback preconditioner is in use
     int fallback_kind = TDZ_PC_MG;
 };
+
+// the host weights to the device; a new version for the inference's packed operands
+void upload_weights(LearnState &ls)
+{
+    HIPCHK(hipMemcpy(ls.W1, ls.hW1.data(), ls.hW1.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ls.W2, ls.hW2.data(), ls.hW2.size() * 4, hipMemcpyHostToDevice));
+    ++ls.wver;
+}
 }  // namespace
 
 struct tdz_ctx {
@@ -853,7 +862,10 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, const fcomplex 
     const float *no = nullptr;   // the network's output, scaled by g / sout in the combine
     if (net) {
         const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-        tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{0.f, 0.f}, nullptr);
+        // the weights' BF16 operand packing persists in the work buffer until they change
+        tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{0.f, 0.f}, nullptr,
+                            ls.packed_ver != ls.wver);
+        ls.packed_ver = ls.wver;
         no = ls.out;
     }
     tdz::launch_corr_combine(c->st, L.f.r, L.f.t, c->sf.u1, n, c->dVg, cx<float>(c->g), c->sf.u1, N, no,
@@ -2055,8 +2067,7 @@ void init_weights(tdz_ctx *c)
         ls.hW1[(centre * CI + 1 + d) * C + 2 + d] = 1.f;
         ls.hW2[(centre * C + 2 + d) * C + 0] = (float)(ls.fscale[2 + d] / h * ls.sout / ls.sin.v[1 + d]);
     }
-    HIPCHK(hipMemcpy(ls.W1, ls.hW1.data(), ls.hW1.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ls.W2, ls.hW2.data(), ls.hW2.size() * 4, hipMemcpyHostToDevice));
+    upload_weights(ls);
 }
 
 void setup_training_buffers(tdz_ctx *c)
@@ -2165,8 +2176,7 @@ void learned_fit(tdz_ctx *c, int iters, double lr, double *loss0, double *loss1)
         };
         step(ls.hW1, g1, m1, v1);
         step(ls.hW2, g2, m2, v2);
-        HIPCHK(hipMemcpy(ls.W1, ls.hW1.data(), ls.hW1.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(ls.W2, ls.hW2.data(), ls.hW2.size() * 4, hipMemcpyHostToDevice));
+        upload_weights(ls);
     }
     last = learned_loss_grad(c, g1, g2, false);
     // reported per sample: 1 is the loss of a network that outputs zero
@@ -2462,8 +2472,7 @@ int tdz_learn_weights(tdz_ctx *c, float *W1, float *W2, int set)
         if (set) {
             std::copy(W1, W1 + ls.hW1.size(), ls.hW1.begin());
             std::copy(W2, W2 + ls.hW2.size(), ls.hW2.begin());
-            HIPCHK(hipMemcpy(ls.W1, ls.hW1.data(), ls.hW1.size() * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(ls.W2, ls.hW2.data(), ls.hW2.size() * 4, hipMemcpyHostToDevice));
+            upload_weights(ls);
             ls.trained = true;
         } else {
             std::copy(ls.hW1.begin(), ls.hW1.end(), W1);
@@ -2539,7 +2548,8 @@ int tdz_learn_eval(tdz_ctx *c, const double *u, const double *target, double *lo
         }
         if (out_mfma) {
             const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-            tdz::cnn_infer_mfma(c->st, n3, y32, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{1.f, 0.f}, nullptr);
+            tdz::cnn_infer_mfma(c->st, n3, y32, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{1.f, 0.f}, nullptr, true);
+            ls.packed_ver = ls.wver;
             HIPCHK(hipStreamSynchronize(c->st));
             HIPCHK(hipMemcpy(o.data(), ls.out, o.size() * 4, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < o.size(); ++i) out_mfma[i] = o[i];

@@ -252,11 +252,13 @@ void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const f
                       float *partial, int nchunk, float *dW1, float *dW2);
 // BF16 MFMA forward; mb work of cnn_infer_work_bytes(N) bytes ([2N][C] bf16 plus the
 // packed weights), out [2N]; if nout: nout = g (out_re + i out_im). Pb: the positional
-// fields as bf16, [N][C] (cnn_fields_bf16 of the [N][C] float fields).
+// fields as bf16, [N][C] (cnn_fields_bf16 of the [N][C] float fields). pack: pack W1, W2
+// into their BF16 operand order in mb first (else mb holds them from an earlier call).
 inline size_t cnn_infer_work_bytes(long N) { return (size_t)2 * N * kCnnChannels * 2 + 64 * 64 * 8; }
 std::vector<unsigned short> cnn_fields_bf16(const std::vector<float> &P);
 void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
-                    const unsigned short *Pb, unsigned short *mb, float *out, fcomplex g, fcomplex *nout);
+                    const unsigned short *Pb, unsigned short *mb, float *out, fcomplex g, fcomplex *nout,
+                    bool pack = true);
 void cnn_to_complex(hipStream_t st, long N, const float *out, fcomplex g, fcomplex *n);
 // partial[b] (b < kLossBlocks, FP64) sums to sum (out - t)^2 over both components
 constexpr int kLossBlocks = 256;

@@ -992,14 +992,14 @@ std::vector<unsigned short> cnn_fields_bf16(const std::vector<float> &P)
 }
 
 void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
-                    const unsigned short *Pb, unsigned short *mb, float *out, fcomplex g, fcomplex *nout)
+                    const unsigned short *Pb, unsigned short *mb, float *out, fcomplex g, fcomplex *nout, bool pack)
 {
     const long N = (long)n[0] * n[1] * n[2];
     const long tiles = (long)((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ);
     // the packed weights go behind the 2 N C bf16 of m (the work buffer holds both)
     uint2 *wp = reinterpret_cast<uint2 *>(mb + 2 * N * C);
     uint4 *wp2 = reinterpret_cast<uint4 *>(reinterpret_cast<char *>(wp) + kPack2Offset);
-    hipLaunchKernelGGL(cnn_pack_kernel, dim3(gridn(NPACK * 64, 256)), dim3(256), 0, st, W1, W2, wp, wp2);
+    if (pack) hipLaunchKernelGGL(cnn_pack_kernel, dim3(gridn(NPACK * 64, 256)), dim3(256), 0, st, W1, W2, wp, wp2);
     // layer 1: one block per tile (both components); layer 2: one per tile and component
     hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)tiles), dim3(256), 0, st, n[0], n[1], n[2], u, wp, Pb, mb);
     hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)(2 * tiles)), dim3(256), 0, st, n[0], n[1], n[2], mb, wp2,
