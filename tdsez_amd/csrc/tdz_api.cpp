@@ -994,7 +994,9 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
     }
     const dcomplex *hback = c->hstage;
     dcomplex *ystage = c->hstage + 2 * (m + 1) + 1, *rstage = ystage + (m + 1);
-    double last_hn = 0;   // ||w|| of the last Arnoldi step
+    double last_hn = 0;        // ||w|| of the last Arnoldi step (after both Gram-Schmidt passes)
+    bool last_fused = false;   // its second pass deferred: w as stored still holds V h2
+    std::vector<dcomplex> last_h2(tdz::kCgsFusedMax);
     int its = 0, restarts = 0;
     double res = 0;
     if (bnorm == 0.0) {
@@ -1033,20 +1035,36 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             if (c->pc_kind != TDZ_PC_NONE) pc_apply_dev(c, Vj(j), Zj(j));
             dcomplex *w = Vj(j + 1);
             level_apply(c, L, Zj(j), w);
-            // CGS2: two passes of h = V^H w, w -= V h
-            tdz::launch_multi_dot(c->st, c->Vb, N, j + 1, w, N, c->dpart, c->dh);
-            tdz::launch_multi_axpy_norm(c->st, c->Vb, N, j + 1, c->dh, w, N, c->dpartr, c->dnorm);
-            tdz::launch_multi_dot(c->st, c->Vb, N, j + 1, w, N, c->dpart, c->dh2);
-            tdz::launch_multi_axpy_norm(c->st, c->Vb, N, j + 1, c->dh2, w, N, c->dpartr, c->dnorm);
+            // CGS2: two passes of h = V^H w, w -= V h. Up to kCgsFusedMax vectors the first
+            // pass also takes the second pass's projections h2 = V^H w1 and ||w1||^2; the
+            // second subtraction is deferred (||w1 - V h2||^2 = ||w1||^2 - ||h2||^2 for
+            // orthonormal V) into the normalisation, or into the residual coefficients of a
+            // converged solve
+            const int nv = j + 1;
+            const bool fused_cgs = nv <= tdz::kCgsFusedMax;
+            tdz::launch_multi_dot(c->st, c->Vb, N, nv, w, N, c->dpart, c->dh);
+            if (fused_cgs) {
+                tdz::launch_multi_axpy_norm_dot(c->st, c->Vb, N, nv, c->dh, w, N, c->dpartr, c->dpart, c->dnorm,
+                                                c->dh2);
+            } else {
+                tdz::launch_multi_axpy_norm(c->st, c->Vb, N, nv, c->dh, w, N, c->dpartr, c->dnorm);
+                tdz::launch_multi_dot(c->st, c->Vb, N, nv, w, N, c->dpart, c->dh2);
+                tdz::launch_multi_axpy_norm(c->st, c->Vb, N, nv, c->dh2, w, N, c->dpartr, c->dnorm);
+            }
             HIPCHK(hipMemcpyAsync(c->hstage, c->dh, (2 * (m + 1) + 1) * sizeof(dcomplex), hipMemcpyDeviceToHost,
                                   c->st));
             HIPCHK(hipStreamSynchronize(c->st));
             const dcomplex *h1 = hback, *h2 = hback + (m + 1);
             const double wn2 = reinterpret_cast<const double *>(hback + 2 * (m + 1))[0];
             for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = cplx(h1[i].re + h2[i].re, h1[i].im + h2[i].im);
-            const double hn = std::sqrt(wn2);
+            double hn2 = wn2;
+            if (fused_cgs)
+                for (int i = 0; i <= j; ++i) hn2 -= h2[i].re * h2[i].re + h2[i].im * h2[i].im;
+            const double hn = hn2 > 0 ? std::sqrt(hn2) : 0.0;
             H[(size_t)(j + 1) * m + j] = hn;
             last_hn = hn;
+            last_fused = fused_cgs;
+            if (fused_cgs) std::copy(h2, h2 + nv, last_h2.begin());
             for (int i = 0; i < j; ++i) {
                 const cplx a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
                 H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
@@ -1063,9 +1081,10 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             ++its;
             res = std::abs(gv[j + 1]) / bnorm;
             if (res <= rtol || hn == 0.0) { ++j; converged = true; break; }
-            // V_{j+1} = w / ||w|| for the next step (a converged solve leaves w unscaled: only
-            // the residual update below reads it, with its coefficient divided by ||w||)
-            tdz::launch_scale_invsqrt(c->st, w, c->dnorm, w, N);
+            // V_{j+1} = (w - V h2) / ||w - V h2|| for the next step (a converged solve leaves w
+            // as it is: only the residual update below reads it, with its coefficients adjusted)
+            if (fused_cgs) tdz::launch_correct_scale(c->st, c->Vb, N, nv, c->dh2, w, N, 1.0 / hn);
+            else tdz::launch_scale_invsqrt(c->st, w, c->dnorm, w, N);
         }
         std::vector<cplx> yv(j);
         for (int i = j - 1; i >= 0; --i) {
@@ -1085,7 +1104,13 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
                 q[i + 1] = std::conj(sn[i]) * a + cs[i] * bb;
             }
             for (int i = 0; i <= j; ++i) rstage[i] = dc(-gv[j] * q[i]);
-            if (last_hn > 0) rstage[j] = dc(cplx(rstage[j].re, rstage[j].im) / last_hn);   // V_j left unscaled
+            if (last_hn > 0) {   // V_j = (w - [V h2]) / ||.|| with w as stored
+                const cplx cw = cplx(rstage[j].re, rstage[j].im) / last_hn;
+                if (last_fused)
+                    for (int i = 0; i < j; ++i)
+                        rstage[i] = dc(cplx(rstage[i].re, rstage[i].im) - cw * cplx(last_h2[i].re, last_h2[i].im));
+                rstage[j] = dc(cw);
+            }
         }
         // y to dh and, with A x, the residual coefficients to dh2 = dh + m + 1: one copy
         HIPCHK(hipMemcpyAsync(c->dh, ystage, (with_ax ? (m + 1) + (j + 1) : j) * sizeof(dcomplex),

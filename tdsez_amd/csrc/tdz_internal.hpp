@@ -123,6 +123,14 @@ void launch_multi_dot(hipStream_t st, const dcomplex *V, long ld, int nv, const 
 // w -= sum_i h_i V_i (h on device), and out_norm2 = ||w||^2 (on device, deterministic)
 void launch_multi_axpy_norm(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *h, dcomplex *w,
                             long N, double *partial, double *out_norm2);
+// Classical Gram-Schmidt with the second pass deferred: w -= sum_i h_i V_i (nv <=
+// kCgsFusedMax, h on device), out_norm2 = ||w||^2 and out_dot[i] = <V_i, w> (the second
+// pass's projections) from one pass; launch_correct_scale then applies w = s (w - V out_dot)
+constexpr int kCgsFusedMax = 4;
+void launch_multi_axpy_norm_dot(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *h, dcomplex *w,
+                                long N, double *pnorm, dcomplex *pdot, double *out_norm2, dcomplex *out_dot);
+void launch_correct_scale(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *h, dcomplex *w, long N,
+                          double s);
 // x = base + sum_i c_i Z_i (c on device; base null: x itself)
 void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N,
                        const dcomplex *base = nullptr);

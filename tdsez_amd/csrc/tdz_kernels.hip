@@ -790,6 +790,91 @@ __global__ void __launch_bounds__(kRedThreads) multi_axpy_norm_kernel(const dcom
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
+// The first Gram-Schmidt pass fused with the projections of its result (the second pass's
+// dot products): w -= sum_i h_i V_i (nv <= K), with the block sums of ||w||^2 (pnorm[b])
+// and of conj(V_i) . w (pdot[i nb + b]); each V_i is read once.
+template <int K>
+__global__ void __launch_bounds__(kRedThreads) multi_axpy_norm_dot_kernel(const dcomplex *__restrict__ V, long ld,
+                                                                          int nv, const dcomplex *__restrict__ h,
+                                                                          dcomplex *__restrict__ w, long N,
+                                                                          double *__restrict__ pnorm,
+                                                                          dcomplex *__restrict__ pdot)
+{
+    __shared__ dcomplex hs[K];
+    __shared__ double sh[2 * K + 1][kRedThreads / 64];
+    if ((int)threadIdx.x < K) hs[threadIdx.x] = (int)threadIdx.x < nv ? h[threadIdx.x] : dcomplex{0.0, 0.0};
+    __syncthreads();
+    double acc = 0, dr[K], di[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) dr[k] = di[k] = 0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        dcomplex vb[K];
+        dcomplex v = w[I];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k < nv) {
+                vb[k] = V[(long)k * ld + I];
+                v.re -= hs[k].re * vb[k].re - hs[k].im * vb[k].im;
+                v.im -= hs[k].re * vb[k].im + hs[k].im * vb[k].re;
+            }
+        }
+        w[I] = v;
+        acc += v.re * v.re + v.im * v.im;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k < nv) {
+                dr[k] += vb[k].re * v.re + vb[k].im * v.im;
+                di[k] += vb[k].re * v.im - vb[k].im * v.re;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) sh[2 * K][wave] = acc;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double a = dr[k], b = di[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+        }
+        if (lane == 0) { sh[2 * k][wave] = a; sh[2 * k + 1][wave] = b; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w2 = 0; w2 < kRedThreads / 64; ++w2) s += sh[2 * K][w2];
+        pnorm[blockIdx.x] = s;
+    }
+    if ((int)threadIdx.x < nv) {
+        const int k = threadIdx.x;
+        double sr = 0, si = 0;
+        for (int w2 = 0; w2 < kRedThreads / 64; ++w2) { sr += sh[2 * k][w2]; si += sh[2 * k + 1][w2]; }
+        pdot[(long)k * gridDim.x + blockIdx.x] = {sr, si};
+    }
+}
+
+// w = s (w - sum_i h_i V_i), h on device: the deferred second Gram-Schmidt correction and
+// the normalisation in one pass
+__global__ void correct_scale_kernel(const dcomplex *__restrict__ V, long ld, int nv, const dcomplex *__restrict__ h,
+                                     dcomplex *__restrict__ w, long N, double s)
+{
+    __shared__ dcomplex hs[64];
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) hs[i] = h[i];
+    __syncthreads();
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        dcomplex v = w[I];
+        for (int i = 0; i < nv; ++i) {
+            const dcomplex b = V[(long)i * ld + I];
+            v.re -= hs[i].re * b.re - hs[i].im * b.im;
+            v.im -= hs[i].re * b.im + hs[i].im * b.re;
+        }
+        w[I] = {s * v.re, s * v.im};
+    }
+}
+
 __global__ void multi_axpy_kernel(const dcomplex *__restrict__ Z, long ld, int nv, const dcomplex *__restrict__ c,
                                   const dcomplex *base, dcomplex *x, long N)
 {
@@ -1632,6 +1717,23 @@ void launch_multi_axpy_norm(hipStream_t st, const dcomplex *V, long ld, int nv, 
     const unsigned nb = grid_capped(N, kRedThreads, kRedBlocks);
     hipLaunchKernelGGL(multi_axpy_norm_kernel, dim3(nb), dim3(kRedThreads), 0, st, V, ld, nv, h, w, N, partial);
     hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, partial, (int)nb, out_norm2);
+}
+
+void launch_multi_axpy_norm_dot(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *h, dcomplex *w,
+                                long N, double *pnorm, dcomplex *pdot, double *out_norm2, dcomplex *out_dot)
+{
+    if (nv < 1 || nv > kCgsFusedMax) throw std::runtime_error("multi_axpy_norm_dot: 1 .. kCgsFusedMax vectors");
+    const unsigned nb = grid_capped(N, kRedThreads, kRedBlocks);
+    hipLaunchKernelGGL(multi_axpy_norm_dot_kernel<kCgsFusedMax>, dim3(nb), dim3(kRedThreads), 0, st, V, ld, nv, h, w,
+                       N, pnorm, pdot);
+    hipLaunchKernelGGL(reduce_real_kernel, dim3(1), dim3(kRedThreads), 0, st, pnorm, (int)nb, out_norm2);
+    hipLaunchKernelGGL(reduce_complex_kernel, dim3(nv), dim3(kRedThreads), 0, st, pdot, (int)nb, out_dot);
+}
+
+void launch_correct_scale(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *h, dcomplex *w, long N,
+                          double s)
+{
+    hipLaunchKernelGGL(correct_scale_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, V, ld, nv, h, w, N, s);
 }
 
 void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N,
