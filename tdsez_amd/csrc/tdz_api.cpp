@@ -204,7 +204,8 @@ struct tdz_ctx {
         int count = 0, head = 0;    // states held (at most 3), slot of the newest
         double dt = 0;
         unsigned axes = 0;          // field axes whose dipole products are kept
-        dcomplex *psi[3] = {}, *P[3] = {}, *D[3][3] = {};   // state; (M + g H_0) state; D_a state
+        dcomplex *psi[3] = {}, *P[3] = {}, *D[3][3] = {};   // state; (M + g H_sEh) state; D_a state
+        double sEh[3][3] = {};                               // the field of each slot's P
         dcomplex *Mx[3] = {};                                // M state
         dcomplex *Xband[3] = {}, *Zband[3] = {};   // complex position bands and zero bands (dipole applies)
         double *Xz = nullptr;                      // the real z position band (fused mass + z dipole)
@@ -1676,16 +1677,11 @@ void dipole_apply(tdz_ctx *c, int a, const dcomplex *x, dcomplex *y)
     }
 }
 
-// Push state x into the predictor history with D_a x and P = (M + g H_0) x, the operator
-// at the run's g without field: P = ax - g sum_a sE_a D_a x when ax = (M + g H_sE) x is
-// known (the solve's A x), else one operator apply.
-void pred_push(tdz_ctx *c, const dcomplex *x, const dcomplex *ax, const double sE[3])
+// Buffers of history slot h.
+void pred_alloc(tdz_ctx *c, int h)
 {
     auto &p = c->pred;
-    Level &L = *c->lv[0];
-    const long N = L.N;
-    const cplx g(0.0, 0.5 * p.dt);
-    const int h = (p.head + 1) % 3;
+    const long N = c->lv[0]->N;
     if (!p.psi[h]) {
         p.psi[h] = track(c, dalloc<dcomplex>(N));
         p.P[h] = track(c, dalloc<dcomplex>(N));
@@ -1693,6 +1689,20 @@ void pred_push(tdz_ctx *c, const dcomplex *x, const dcomplex *ax, const double s
     if (!p.Mx[h]) p.Mx[h] = track(c, dalloc<dcomplex>(N));
     for (int a = 0; a < 3; ++a)
         if ((p.axes >> a & 1) && !p.D[h][a]) p.D[h][a] = track(c, dalloc<dcomplex>(N));
+}
+
+// Push state x into the predictor history (slot (head + 1) % 3) with M x, D_a x and
+// P = (M + g H_sE) x: with ax_in_slot the solve that produced x already left its A x in
+// that slot's P (under the field sE of its step), else P is one operator apply without
+// field.
+void pred_push(tdz_ctx *c, const dcomplex *x, bool ax_in_slot, const double sE[3])
+{
+    auto &p = c->pred;
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    const cplx g(0.0, 0.5 * p.dt);
+    const int h = (p.head + 1) % 3;
+    pred_alloc(c, h);
     // x, M x and D_a x: one fused pass for a field along z alone, else a copy and one apply each
     bool fused = false;
     if (p.axes == 4u && c->prob.dim == 3 && L.ax[0].p == L.ax[2].p && L.ax[1].p == L.ax[2].p) {
@@ -1717,16 +1727,13 @@ void pred_push(tdz_ctx *c, const dcomplex *x, const dcomplex *ax, const double s
         const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
         tdz::launch_wall_fix<double>(c->st, L.n, wall, x, p.Mx[h]);
     }
-    if (ax) {
-        tdz::LinComb lc;
-        lc.add({1, 0}, ax);
-        for (int a = 0; a < 3; ++a)
-            if (p.axes >> a & 1) lc.add(dc(-g * sE[a]), p.D[h][a]);
-        tdz::launch_lincomb(c->st, lc, p.P[h], N);
+    if (ax_in_slot) {
+        for (int a = 0; a < 3; ++a) p.sEh[h][a] = sE[a];
     } else {
         const double zero[3] = {0, 0, 0};
         set_shift(c, 1.0, g, zero);
         level_apply(c, L, x, p.P[h]);
+        for (int a = 0; a < 3; ++a) p.sEh[h][a] = 0.0;
     }
     p.head = h;
     p.count = std::min(p.count + 1, 3);
@@ -1754,8 +1761,9 @@ bool pred_continues(tdz_ctx *c, const dcomplex *x, double dt)
 // current one) of the history, the guess is the binomial extrapolation
 // x0 = sum_j w_j psi_j, w = (1), (2, -1) or (3, -3, 1) by order; its residual
 //   r0 = b - A x0,  A = M + g H_sE,  b = (M - g H_sE) psi_0 = 2 M psi_0 - A psi_0,
-// follows from A psi_j = P_j + g sum_a sE_a D_a psi_j without an operator apply, so
-// the step costs one mass apply and one dipole apply per field axis besides the solve.
+// follows from A psi_j = P_j + g sum_a (sE_a - sE^j_a) D_a psi_j (P_j = (M + g H_sE^j) psi_j,
+// the A x its own solve returned) without an operator apply, so the step costs one mass
+// apply and one dipole apply per field axis besides the solve.
 // The solve is the same (M + g H_sE) psi' = b to the same tolerance as without the
 // guess (src/propagator.cpp:78-251 starts from psi_0; the stage solves of
 // src/propagator.cpp:629-638 extrapolate the same way).
@@ -1766,8 +1774,7 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
     Level &L = *c->lv[0];
     const long N = L.N;
     const cplx g(0.0, 0.5 * dt);
-    c->ax_valid = false;
-    if (!c->ax) c->ax = track(c, dalloc<dcomplex>(N));
+    c->ax_valid = false;   // the chained right-hand side of cn_step does not follow this run
     for (int a = 0; a < c->prob.dim; ++a)
         if (sE[a] != 0 && !(p.axes >> a & 1)) {   // a new field axis: restart the history
             p.axes |= 1u << a;
@@ -1776,15 +1783,15 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
     if (p.count == 0 || p.dt != dt) {
         p.count = 0;
         p.dt = dt;
-        pred_push(c, x, nullptr, sE);
+        pred_push(c, x, false, sE);
     }
     auto slot = [&](int j) { return (p.head - j + 3) % 3; };
     const int h0 = slot(0);
-    tdz::LinComb rb;   // b = 2 M psi_0 - P_0 - g sE.D psi_0 (M psi_0 kept at its push)
+    tdz::LinComb rb;   // b = 2 M psi_0 - P_0 - g (sE - sE^0).D psi_0 (M psi_0 kept at its push)
     rb.add({2, 0}, p.Mx[h0]);
     rb.add({-1, 0}, p.P[h0]);
     for (int a = 0; a < 3; ++a)
-        if (p.axes >> a & 1) rb.add(dc(-g * sE[a]), p.D[h0][a]);
+        if (p.axes >> a & 1) rb.add(dc(-g * (sE[a] - p.sEh[h0][a])), p.D[h0][a]);
     static const double W[3][3] = {{1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
     const int o = std::min(p.order, p.count - 1);
     tdz::LinComb r0, x0;   // r0 = b - sum_j w_j A psi_j;  x0 = sum_j w_j psi_j
@@ -1793,7 +1800,7 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
         const double w = W[o][j];
         r0.add({-w, 0}, p.P[s]);
         for (int a = 0; a < 3; ++a)
-            if (p.axes >> a & 1) r0.add(dc(-w * g * sE[a]), p.D[s][a]);
+            if (p.axes >> a & 1) r0.add(dc(-w * g * (sE[a] - p.sEh[s][a])), p.D[s][a]);
         x0.add({w, 0}, p.psi[s]);
     }
     set_shift(c, 1.0, g, sE);
@@ -1803,8 +1810,12 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
     tdz::launch_lincomb3_norms(c->st, rb, r0, x0, c->rhs, c->Vb, x, N, reinterpret_cast<double *>(c->dpart),
                                2 * tdz::kRedBlocks, c->dnorm);
     c->norms_queued = true;
-    step_solve(c, c->rhs, x, rtol, maxit, st, true, c->ax);
-    pred_push(c, x, c->ax, sE);
+    // the solve leaves A x in the P of the slot the new state goes to (the oldest slot,
+    // whose P the pass above has already read)
+    const int hnext = (p.head + 1) % 3;
+    pred_alloc(c, hnext);
+    step_solve(c, c->rhs, x, rtol, maxit, st, true, p.P[hnext]);
+    pred_push(c, x, true, sE);
 }
 }  // namespace
 
