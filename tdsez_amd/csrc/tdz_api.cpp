@@ -218,7 +218,7 @@ struct tdz_ctx {
     dcomplex *Vb = nullptr, *Zb = nullptr;
     dcomplex *dpart = nullptr, *dh = nullptr, *dh2 = nullptr;
     double *dpartr = nullptr, *dnorm = nullptr;   // dnorm: 2 slots
-    // pinned staging of the solver's small uploads (host memory that outlives the copies)
+    // pinned staging of the solver's small transfers (host memory that outlives the copies)
     dcomplex *hstage = nullptr;
     int hstage_len = 0;
     // the caller queued dnorm[0] = ||V_0||^2 and dnorm[1] = ||b||^2 for the next solve
