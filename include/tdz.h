@@ -126,6 +126,36 @@ int tdz_pc_apply(tdz_ctx *ctx, const double *r, double *z);
 /* KSPSolve: FGMRES, right preconditioning, x holds the initial guess
  * (src/propagator.cpp:131-182). Not converging within maxit is an error. */
 int tdz_ksp_solve(tdz_ctx *ctx, const double *b, double *x, double rtol, int maxit, tdz_ksp_stats *stats);
+/* TSSetIFunction callback body (TDSEZIFunctionPol{X,Y,Z}, src/monitor.cpp:108-162):
+ * F = i M psidot - H psi - sum_a sE_a D_a psi with sE = s E(t) (H includes the CAP);
+ * F = 0 on Dirichlet wall DOFs. The shift set by tdz_set_shift is left unchanged. */
+int tdz_ifunction(tdz_ctx *ctx, const double sE[3], const double *psi, const double *psidot, double *F);
+
+/* Multigrid levels (level 0 the finest; the hierarchy of src/multigrid.cpp:239-268).
+ * tdz_mg_transfer: dir 0 restricts out = P^T in (level -> level+1), dir 1 prolongs
+ * out = P in (level+1 -> level), P the knot-insertion prolongation (the interpolation
+ * PCMGSetInterpolation receives, src/multigrid.cpp:295-298; restriction P^T). Device
+ * vectors of at least the level sizes (tdz_level_dims).
+ * tdz_level_apply: y = A_l x with the current shift: A_0 = tdz_op_apply's operator,
+ * coarse levels P^T A P (PCMGSetGalerkin(PC_MG_GALERKIN_BOTH), src/multigrid.cpp:277). */
+int tdz_level_dims(const tdz_ctx *ctx, int level, int n[3]);
+int tdz_mg_transfer(tdz_ctx *ctx, int level, int dir, const double *in, double *out);
+int tdz_level_apply(tdz_ctx *ctx, int level, const double *x, double *y);
+/* -tdsez_pc_refresh_its N (include/tdsez_propagator.hpp:22-24): besides the rule
+ * its > 2 base + 5, rebuild the preconditioner whenever a solve takes more than
+ * max_its iterations (max_its < 0: off, the default). Rebuilds are counted in
+ * tdz_ksp_stats.pc_rebuilds. */
+int tdz_set_pc_refresh(tdz_ctx *ctx, int max_its);
+/* Host utilities of the multigrid set-up (no device work): one coarsening of a knot
+ * vector U[m] into C (at most m values; *mc = its length, 0 when nothing can be removed;
+ * src/multigrid.cpp:25-40), and the dense knot-insertion matrix R [nf][nc] (row-major,
+ * nf = mf - p - 1, nc = mc - p - 1) with coarse B-spline j = sum_i R[i][j] fine B-spline i
+ * (src/multigrid.cpp:52-81). */
+int tdz_coarsen_knots(const double *U, int m, double *C, int *mc);
+int tdz_refinement_matrix(const double *C, int mc, const double *F, int mf, int p, double *R);
+/* The last warning of the calling thread (also printed to stderr), e.g. the strong-field
+ * warning of multigrid runs (tau max|E.r| > 1.5, src/multigrid.cpp:213-237). */
+const char *tdz_last_warning(void);
 /* One Crank-Nicolson step psi <- (M + i dt/2 H_E)^-1 (M - i dt/2 H_E) psi with
  * H_E at the midpoint field sE_mid (TSTHETA theta = 1/2 in src/propagator.cpp:94-101). */
 int tdz_step_cn(tdz_ctx *ctx, double *psi, double dt, const double sE_mid[3], double rtol, int maxit,
@@ -175,11 +205,15 @@ int tdz_observables(tdz_ctx *ctx, const double *psi, double out[6]);
  * energy [7] = [t, Kin, Pot, Int, Tot, <1/m>, Norm] (real parts; Int = s E.<D>,
  * Tot = <H> + Int), ac [3] = [t, Re, Im psi0^H M psi] (zeros when psi0 is null).
  * E is the field E_a(t) itself (not s E). Any output may be null. */
-/* Guess history of the Pade/Magnus stage solves (4 slots x 3 increments of N complex
- * values; CheckpointGuess, src/propagator.cpp:1546-1590): set = 0 copies the counts and,
- * if hist is not null, the increments to hist[((slot * 3) + j) * N] (host); set = 1
- * restores them. */
+/* Guess history of the Pade/Magnus stage solves (4 slots x order increments of N complex
+ * values, order of tdz_set_pade_guess, default 3; CheckpointGuess,
+ * src/propagator.cpp:1546-1590): set = 0 copies the counts and, if hist is not null, the
+ * increments to hist[((slot * order) + j) * N] (host); set = 1 restores them. */
 int tdz_pade_history(tdz_ctx *ctx, int set, int counts[4], double *hist);
+/* -pade_extrapolate_guess and -pade_guess_order (src/propagator.cpp:723-726): whether the
+ * stage solves start from the extrapolated increments, and how many (1..5, default 3).
+ * The history of tdz_pade_history then holds [4 slots][order][N]. */
+int tdz_set_pade_guess(tdz_ctx *ctx, int extrapolate, int order);
 int tdz_trace_row(tdz_ctx *ctx, const double *psi, const double *psi0, double t, const double E[3], double *dip,
                   double *energy, double *ac);
 /* Field sE_a(t) = s E_a(t) of a pulse (TDSEZParser::Coupling, include/tdsez_parser.hpp:190-194). */

@@ -33,7 +33,9 @@ EXPORTS = [
     "tdz_timing", "tdz_timing_stats", "tdz_step_pade22", "tdz_evolve_rational",
     "tdz_project_gaussian", "tdz_ground_state",
     "tdz_trace_row", "tdz_pade_history", "tdz_batch_create", "tdz_batch_context", "tdz_batch_evolve",
-    "tdz_batch_fixups", "tdz_batch_destroy", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
+    "tdz_batch_fixups", "tdz_batch_destroy", "tdz_ifunction", "tdz_level_dims", "tdz_mg_transfer",
+    "tdz_level_apply", "tdz_set_pc_refresh", "tdz_coarsen_knots", "tdz_refinement_matrix", "tdz_last_warning",
+    "tdz_set_pade_guess", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
 ]
 
 
@@ -162,12 +164,43 @@ def lib():
         L.tdz_learn_status.argtypes = [vp, ip, ip, ip]
         L.tdz_timing.argtypes = [vp, i]
         L.tdz_timing_stats.argtypes = [vp, dp, C.POINTER(C.c_long)]
+        L.tdz_ifunction.argtypes = [vp, dp, vp, vp, vp]
+        L.tdz_level_dims.argtypes = [vp, i, ip]
+        L.tdz_mg_transfer.argtypes = [vp, i, i, vp, vp]
+        L.tdz_level_apply.argtypes = [vp, i, vp, vp]
+        L.tdz_set_pc_refresh.argtypes = [vp, i]
+        L.tdz_coarsen_knots.argtypes = [dp, i, dp, ip]
+        L.tdz_refinement_matrix.argtypes = [dp, i, dp, i, i, dp]
+        L.tdz_last_warning.restype = C.c_char_p
+        L.tdz_set_pade_guess.argtypes = [vp, i, i]
         _lib = L
     return _lib
 
 
 class TdzError(RuntimeError):
     pass
+
+
+def coarsen_knots(U):
+    """One multigrid coarsening of a knot vector (None when nothing can be removed)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    out = np.zeros(U.size)
+    m = C.c_int(0)
+    _chk(lib().tdz_coarsen_knots(_dp(U), U.size, _dp(out), C.byref(m)))
+    return out[:m.value] if m.value else None
+
+
+def refinement_matrix(Cknots, Fknots, p):
+    """Dense knot-insertion matrix R [nf, nc]: coarse B-spline j = sum_i R[i, j] fine B-spline i."""
+    Ck = np.ascontiguousarray(Cknots, dtype=np.float64)
+    Fk = np.ascontiguousarray(Fknots, dtype=np.float64)
+    out = np.zeros((Fk.size - p - 1, Ck.size - p - 1))
+    _chk(lib().tdz_refinement_matrix(_dp(Ck), Ck.size, _dp(Fk), Fk.size, p, _dp(out)))
+    return out
+
+
+def last_warning():
+    return lib().tdz_last_warning().decode()
 
 
 def _chk(rc):
@@ -220,6 +253,7 @@ class Context:
         _chk(lib().tdz_dims(self.h, n))
         self.n = tuple(n)
         self.levels = int(lib().tdz_levels(self.h))
+        self.guess_order = 3
 
     def close(self):
         if getattr(self, "h", None):
@@ -260,6 +294,30 @@ class Context:
         s = np.ascontiguousarray(sE_mid, dtype=np.float64)
         _chk(lib().tdz_step_cn(self.h, psi.ptr, dt, _dp(s), rtol, maxit, C.byref(st)))
         return st
+
+    def ifunction(self, sE, psi, psidot, F):
+        """F = i M psidot - H psi - sE.D psi (the TS IFunction)."""
+        s = np.ascontiguousarray(sE, dtype=np.float64)
+        _chk(lib().tdz_ifunction(self.h, _dp(s), psi.ptr, psidot.ptr, F.ptr))
+
+    def level_dims(self, level):
+        n = (C.c_int * 3)()
+        _chk(lib().tdz_level_dims(self.h, level, n))
+        return tuple(n)
+
+    def mg_transfer(self, level, direction, x, y):
+        """direction 0: y = P^T x (level -> level+1); 1: y = P x (level+1 -> level)."""
+        _chk(lib().tdz_mg_transfer(self.h, level, direction, x.ptr, y.ptr))
+
+    def level_apply(self, level, x, y):
+        _chk(lib().tdz_level_apply(self.h, level, x.ptr, y.ptr))
+
+    def set_pc_refresh(self, max_its):
+        _chk(lib().tdz_set_pc_refresh(self.h, int(max_its)))
+
+    def set_pade_guess(self, extrapolate=True, order=3):
+        _chk(lib().tdz_set_pade_guess(self.h, int(bool(extrapolate)), int(order)))
+        self.guess_order = int(order)
 
     def set_cn_predictor(self, order):
         """Initial guess of the CN solves: 0 the current state, 1/2 the binomial extrapolation."""
@@ -359,11 +417,11 @@ class Context:
         return dip, en, ac
 
     def pade_history(self, counts=None, hist=None):
-        """Get (no arguments) or set the Pade/Magnus guess history: counts [4], hist [4, 3, N]."""
+        """Get (no arguments) or set the Pade/Magnus guess history: counts [4], hist [4, order, N]."""
         ip = C.POINTER(C.c_int)
         if counts is None:
             cnt = np.zeros(4, np.int32)
-            h = np.zeros((4, 3, self.N), np.complex128)
+            h = np.zeros((4, self.guess_order, self.N), np.complex128)
             _chk(lib().tdz_pade_history(self.h, 0, cnt.ctypes.data_as(ip), _dp(h.view(np.float64))))
             return cnt, h
         cnt = np.ascontiguousarray(counts, np.int32)

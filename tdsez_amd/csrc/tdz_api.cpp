@@ -36,6 +36,7 @@ using tdz::fcomplex;
 namespace {
 
 thread_local std::string g_err;
+thread_local std::string g_warn;   // the last warning of the calling thread
 
 struct Error : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -109,6 +110,10 @@ struct Level {
     bool simple = false;    // 3D, simple knots on every axis: the tiled 3D potential kernel
     bool simple2 = false;   // 2D, simple knots: the tiled 2D potential kernel
     int pot_variant = -1;   // tile shape of the tiled kernel picked by the autotuner (-1: generic)
+    // coarse levels: the exact Galerkin potential P^T (s V) P as an offset-major stencil
+    double *Sv = nullptr;
+    float *Svf = nullptr;
+    long Sv_len = 0;
 };
 
 template <typename R> Prec<R> &prec(Level &L);
@@ -228,10 +233,15 @@ struct tdz_ctx {
     std::vector<void *> allocs;     // everything to free
     std::vector<double *> user_vecs;
     int refresh_base = -1, rebuilds = 0;
+    int stage_base[2] = {-1, -1};   // refresh trackers of the two Pade stages
+    bool field_warned = false;   // the strong-field warning was given
+    int refresh_cap = -1;       // rebuild whenever a solve takes more iterations (-tdsez_pc_refresh_its); -1: off
     bool cap_on = true;         // the CAP is part of H (off during the eigensolve, as src/propagator.cpp:51-69)
     // Pade(2,2) stage guesses: per slot (2 x stage) the last increments y - x
     dcomplex *hist = nullptr;   // [4 slots][3] vectors
     int hist_count[4] = {0, 0, 0, 0};
+    int guess_order = 3, hist_order = 0;   // -pade_guess_order (1..5); order of the allocated history
+    bool guess_extrap = true;              // -pade_extrapolate_guess
     dcomplex *xin = nullptr;    // stage input copy
     bool use_tiled = true;   // tiled 3D potential kernel where it applies
     // first-order corrected separable preconditioner: the non-separable part of s V at the
@@ -539,12 +549,28 @@ void pot_apply(tdz_ctx *c, Level &L, cplx g, const cpx<R> *x, cpx<R> *y)
     if (timed) HIPCHK(hipEventRecord(c->ev[c->ev_used++].second, c->st));
 }
 
-// y = A_l x with the current shift (separable bands + matrix-free potential).
+void ensure_galerkin(tdz_ctx *c);
+
+template <typename R> const R *stencil_of(const Level &L);
+template <> const double *stencil_of<double>(const Level &L) { return L.Sv; }
+template <> const float *stencil_of<float>(const Level &L) { return L.Svf; }
+
+// y = A_l x with the current shift: separable bands (Galerkin products per axis on coarse
+// levels) plus the potential, matrix-free on the finest level and the exact Galerkin
+// stencil P^T (s V) P on the coarse ones.
 template <typename R>
 void level_apply(tdz_ctx *c, Level &L, const cpx<R> *x, cpx<R> *y)
 {
     tdz::launch_sep_apply<R>(c->st, sep_bands<R>(L), x, y, scratch<R>(c).tmp);
-    if (c->ph.pot_kind && c->g != 0.0) pot_apply<R>(c, L, c->g, x, y);
+    if (c->ph.pot_kind && c->g != 0.0) {
+        if (&L == c->lv[0].get()) {
+            pot_apply<R>(c, L, c->g, x, y);
+        } else {
+            if (!stencil_of<R>(L)) ensure_galerkin(c);
+            const int hp[3] = {L.ax[0].p, L.ax[1].p, L.ax[2].p};
+            tdz::launch_stencil_apply<R>(c->st, L.n, hp, stencil_of<R>(L), (R)c->g.real(), (R)c->g.imag(), x, y);
+        }
+    }
     if (c->prob.dirichlet) {
         const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
         tdz::launch_wall_fix<R>(c->st, L.n, w, x, y);
@@ -733,6 +759,118 @@ void vcycle(tdz_ctx *c, int l, const cpx<R> *r, cpx<R> *z)
     for (int s = 0; s < nu; ++s) smooth<R>(c, L, r, z, s1, s2, S.tmp[3]);
 }
 
+// ---- exact Galerkin potential of the coarse levels
+// Host pair table of level axis A against the finest level's Gauss points (axis F): for
+// every (function i, offset d) the products N_i N_{i+d} at the run of fine points where
+// both are nonzero (wall functions are not in the space: no entries).
+struct PairTableHost {
+    std::vector<int> q0, len, off;
+    std::vector<double> val;
+};
+
+PairTableHost pair_table(const tdz::Axis1D &F, const tdz::Axis1D &A)
+{
+    PairTableHost t;
+    if (!A.active) {   // unused axis: one function, one point of weight 1
+        t.q0 = {0}; t.len = {1}; t.off = {0}; t.val = {1.0};
+        return t;
+    }
+    const int p = A.p, n = A.n, D = 2 * p + 1, npts = F.E * F.Q;
+    std::vector<std::vector<double>> runs((size_t)D * n);
+    std::vector<int> start((size_t)D * n, -1);
+    std::vector<double> v(p + 1);
+    for (int m = 0; m < npts; ++m) {
+        int f0 = 0;
+        tdz::basis_values(A.t, p, F.X[m], f0, v.data());
+        for (int a = 0; a <= p; ++a) {
+            if (A.isWall[f0 + a]) continue;
+            for (int b = 0; b <= p; ++b) {
+                if (A.isWall[f0 + b]) continue;
+                const size_t k = (size_t)(b - a + p) * n + (f0 + a);
+                if (start[k] < 0) start[k] = m;
+                std::vector<double> &r = runs[k];
+                r.resize((size_t)(m - start[k]), 0.0);   // a gap (none for nested spaces) holds zeros
+                r.push_back(v[a] * v[b]);
+            }
+        }
+    }
+    for (size_t k = 0; k < runs.size(); ++k) {
+        t.q0.push_back(std::max(start[k], 0));
+        t.len.push_back((int)runs[k].size());
+        t.off.push_back((int)t.val.size());
+        t.val.insert(t.val.end(), runs[k].begin(), runs[k].end());
+    }
+    return t;
+}
+
+// complex64 copies of the coarse potential stencils (mixed-precision V-cycle)
+void galerkin_float(tdz_ctx *c)
+{
+    for (size_t l = 1; l < c->lv.size(); ++l) {
+        Level &L = *c->lv[l];
+        if (!L.Sv || L.Svf) continue;
+        L.Svf = track(c, dalloc<float>(L.Sv_len));
+        tdz::launch_stencil_demote(c->st, L.Sv, L.Svf, L.Sv_len);
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
+// Assemble P^T (s V) P for every coarse level once (the potential does not change in
+// time; the field and the other terms coarsen per axis).
+void ensure_galerkin(tdz_ctx *c)
+{
+    if (!c->ph.pot_kind || c->lv.size() < 2) return;
+    Level &F = *c->lv[0];
+    std::vector<double *> tmp;
+    try {
+        double *X[3], *W[3];
+        for (int d = 0; d < 3; ++d) {
+            X[d] = dalloc<double>(F.ax[d].X.size());
+            tmp.push_back(X[d]);
+            W[d] = dalloc<double>(F.ax[d].W.size());
+            tmp.push_back(W[d]);
+            HIPCHK(hipMemcpy(X[d], F.ax[d].X.data(), F.ax[d].X.size() * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(W[d], F.ax[d].W.data(), F.ax[d].W.size() * 8, hipMemcpyHostToDevice));
+        }
+        for (size_t l = 1; l < c->lv.size(); ++l) {
+            Level &L = *c->lv[l];
+            if (L.Sv) continue;
+            tdz::GalerkinPlan g;
+            long len = 1, per_z = 0;
+            for (int d = 0; d < 3; ++d) {
+                const PairTableHost h = pair_table(F.ax[d], L.ax[d]);
+                int *q0 = dalloc<int>(h.q0.size()), *ln = dalloc<int>(h.len.size()), *of = dalloc<int>(h.off.size());
+                double *val = dalloc<double>(std::max<size_t>(h.val.size(), 1));
+                tmp.insert(tmp.end(), {reinterpret_cast<double *>(q0), reinterpret_cast<double *>(ln),
+                                       reinterpret_cast<double *>(of), val});
+                HIPCHK(hipMemcpy(q0, h.q0.data(), h.q0.size() * 4, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(ln, h.len.data(), h.len.size() * 4, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(of, h.off.data(), h.off.size() * 4, hipMemcpyHostToDevice));
+                if (!h.val.empty()) HIPCHK(hipMemcpy(val, h.val.data(), h.val.size() * 8, hipMemcpyHostToDevice));
+                g.pairs[d] = {(int)h.q0.size(), q0, ln, of, val};
+                g.X[d] = X[d];
+                g.W[d] = W[d];
+                g.nq[d] = F.ax[d].E * F.ax[d].Q;
+                len *= (long)h.q0.size();
+            }
+            g.pot = {c->ph.pot_kind, c->ph.s, c->ph.pot[0], c->ph.pot[1]};
+            per_z = (long)g.nq[1] * g.pairs[0].npairs + (long)g.pairs[1].npairs * g.pairs[0].npairs;
+            const long work_len = per_z * std::min<long>(g.nq[2], std::max<long>(1, (128L << 20) / per_z));
+            double *work = dalloc<double>(work_len);
+            tmp.push_back(work);
+            L.Sv = track(c, dalloc<double>(len));
+            L.Sv_len = len;
+            tdz::galerkin_potential(c->st, g, L.Sv, work, work_len);
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
+    } catch (...) {
+        for (double *q : tmp) hipFree(q);
+        throw;
+    }
+    for (double *q : tmp) HIPCHK(hipFree(q));
+    if (c->mixed_ready) galerkin_float(c);
+}
+
 // Single-precision data for the mixed-precision preconditioners.
 void setup_mixed(tdz_ctx *c)
 {
@@ -743,6 +881,7 @@ void setup_mixed(tdz_ctx *c)
     c->sf.w2 = track(c, dalloc<fcomplex>(N));
     for (auto &L : c->lv) upload_level_tables_f(c, *L);
     c->mixed_ready = true;
+    galerkin_float(c);
     set_shift_bands(c);
 }
 
@@ -1148,6 +1287,7 @@ void pc_setup(tdz_ctx *c, int kind)
     if (c->layered && (kind == TDZ_PC_CORRECTED || kind == TDZ_PC_CORRECTED_EXACT || kind == TDZ_PC_LEARNED))
         throw Error("a layered structure is separable: its separable solve is exact (no correction to learn)");
     c->pc_kind = kind;
+    if (mg || (kind == TDZ_PC_LEARNED && c->lv.size() >= 2)) ensure_galerkin(c);
     if (mg)
         for (auto &L : c->lv) setup_fd(c, *L);
     else if (kind != TDZ_PC_NONE)
@@ -1201,6 +1341,38 @@ void layer_bands(const tdz_ctx *c, const tdz::Axis1D &A, std::vector<double> &Mw
         }
 }
 
+// Knot vectors of the multigrid levels, finest first (the hierarchy of
+// src/multigrid.cpp:239-268). A level coarsens every axis that can still lose element
+// boundaries and keep at least p+1 elements; the list ends when no axis coarsens, at
+// mg_levels levels, or, for the automatic depth (mg_levels = 0), after the first coarse
+// level with fewer than 2000 functions.
+std::vector<std::array<std::vector<double>, 3>> level_knots(const std::array<std::vector<double>, 3> &finest, int dim,
+                                                            int p, int mg_levels)
+{
+    std::vector<std::array<std::vector<double>, 3>> out{finest};
+    const size_t depth = mg_levels > 0 ? (size_t)mg_levels : 20;
+    auto functions = [&](const std::array<std::vector<double>, 3> &K) {
+        long n = 1;
+        for (int d = 0; d < dim; ++d) n *= (long)K[d].size() - p - 1;
+        return n;
+    };
+    while (out.size() < depth) {
+        if (mg_levels == 0 && out.size() > 1 && functions(out.back()) < 2000) break;
+        std::array<std::vector<double>, 3> next = out.back();
+        int coarsened = 0;
+        for (int d = 0; d < dim; ++d) {
+            std::vector<double> c;
+            if (tdz::coarsen_knots(out.back()[d], c) && tdz::knot_spans(c) > p) {
+                next[d].swap(c);
+                ++coarsened;
+            }
+        }
+        if (coarsened == 0) break;
+        out.push_back(std::move(next));
+    }
+    return out;
+}
+
 void build(tdz_ctx *c, const tdz_problem &pr)
 {
     if (pr.dim < 1 || pr.dim > 3) throw Error("dim must be 1, 2 or 3");
@@ -1219,7 +1391,7 @@ void build(tdz_ctx *c, const tdz_problem &pr)
     for (int i = 0; i < 4; ++i) { c->ph.pot[i] = pr.pot[i]; c->ph.cap[i] = pr.cap[i]; }
     HIPCHK(hipStreamCreate(&c->st));
 
-    // ---- level hierarchy (src/multigrid.cpp:239-268): coarsen while every axis keeps >= p+1 spans
+    // ---- level hierarchy
     std::vector<std::array<std::vector<double>, 3>> knots;
     std::array<std::vector<double>, 3> k0;
     for (int d = 0; d < pr.dim; ++d) k0[d] = tdz::uniform_knots(pr.nelem[d], pr.lo[d], pr.hi[d], pr.p);
@@ -1253,24 +1425,7 @@ void build(tdz_ctx *c, const tdz_problem &pr)
             if (!(c->lay_b[i + 1] > c->lay_b[i])) throw Error("layer bounds must increase");
         }
     }
-    knots.push_back(k0);
-    const int maxLevels = pr.mg_levels > 0 ? pr.mg_levels : 20;
-    while ((int)knots.size() < maxLevels) {
-        auto C = knots.back();
-        bool any = false;
-        long dofs = 1;
-        for (int d = 0; d < pr.dim; ++d) {
-            std::vector<double> Cd;
-            if (tdz::coarsen_knots(knots.back()[d], Cd) && tdz::knot_spans(Cd) >= pr.p + 1) {
-                C[d] = Cd;
-                any = true;
-            }
-            dofs *= (long)C[d].size() - pr.p - 1;
-        }
-        if (!any) break;
-        knots.push_back(C);
-        if (pr.mg_levels == 0 && dofs < 2000) break;
-    }
+    knots = level_knots(k0, pr.dim, pr.p, pr.mg_levels);
     const int nl = (int)knots.size();
     for (int l = 0; l < nl; ++l) {
         auto L = std::make_unique<Level>();
@@ -1297,16 +1452,11 @@ void build(tdz_ctx *c, const tdz_problem &pr)
         for (int d = 0; d < 3; ++d) {
             std::vector<std::vector<std::pair<int, double>>> P;
             if (d < pr.dim) {
-                P = tdz::boehm_prolongation(knots[l + 1][d], knots[l][d], pr.p);
-                // exactness: partition of unity and linear precision at Greville points (src/multigrid.cpp:283-292)
-                const auto gc = tdz::greville(knots[l + 1][d], pr.p), gf = tdz::greville(knots[l][d], pr.p);
-                double err = 0;
-                for (size_t i = 0; i < P.size(); ++i) {
-                    double s1 = 0, sx = 0;
-                    for (auto &e : P[i]) { s1 += e.second; sx += e.second * gc[e.first]; }
-                    err = std::max({err, std::fabs(s1 - 1), std::fabs(sx - gf[i]) / (1 + std::fabs(gf[i]))});
-                }
-                if (P.size() != gf.size() || err > 1e-10) throw Error("knot-insertion prolongation is not exact");
+                P = tdz::refinement_matrix(knots[l + 1][d], knots[l][d], pr.p);
+                // the coarse B-splines must be reproduced by the fine ones through P (the
+                // reference checks partition of unity and linear precision, src/multigrid.cpp:283-292)
+                const double err = tdz::refinement_error(knots[l + 1][d], knots[l][d], pr.p, P);
+                if (!(err <= 1e-11)) throw Error("knot-insertion prolongation is not exact");
                 // walls: the coarse wall function maps only to the fine wall function
                 if (pr.dirichlet) {
                     for (size_t i = 0; i < P.size(); ++i) {
@@ -1350,7 +1500,7 @@ void build(tdz_ctx *c, const tdz_problem &pr)
         }
     }
     for (auto &L : c->lv) upload_level_tables(c, *L);
-    for (auto &L : c->lv) autotune_pot(c, *L);
+    autotune_pot(c, *c->lv[0]);   // coarse levels apply the Galerkin stencil
     const long N = c->lv[0]->N;
     for (int i = 0; i < 4; ++i) c->sd.tmp[i] = track(c, dalloc<dcomplex>(N));
     c->sd.w1 = track(c, dalloc<dcomplex>(N));
@@ -1390,6 +1540,7 @@ int guarded(F &&f)
 extern "C" {
 
 const char *tdz_last_error(void) { return g_err.c_str(); }
+const char *tdz_last_warning(void) { return g_warn.c_str(); }
 
 int tdz_create(const tdz_problem *prob, tdz_ctx **ctx)
 {
@@ -1555,9 +1706,149 @@ int tdz_ksp_solve(tdz_ctx *c, const double *b, double *x, double rtol, int maxit
     });
 }
 
+int tdz_ifunction(tdz_ctx *c, const double sE[3], const double *psi, const double *psidot, double *F)
+{
+    return guarded([&] {
+        if (!psi || !psidot || !F) throw Error("null vector");
+        Level &L = *c->lv[0];
+        const long N = L.N;
+        const cplx a0 = c->alpha, g0 = c->g;
+        const double s0[3] = {c->sE[0], c->sE[1], c->sE[2]};
+        const dcomplex *x = reinterpret_cast<const dcomplex *>(psi), *xd = reinterpret_cast<const dcomplex *>(psidot);
+        dcomplex *y = reinterpret_cast<dcomplex *>(F);
+        set_shift(c, 0.0, -1.0, sE);   // F = -(H + sE.D) psi
+        tdz::launch_sep_apply<double>(c->st, sep_bands<double>(L), x, y, c->sd.tmp);
+        if (c->ph.pot_kind) pot_apply<double>(c, L, -1.0, x, y);
+        mass_apply(c, xd, c->sd.w1);    // + i M psidot
+        tdz::launch_axpby(c->st, {0, 1}, c->sd.w1, {1, 0}, y, N);
+        if (c->prob.dirichlet) {         // wall DOFs are not unknowns of the propagation: F = 0 there
+            if (!c->zerov) c->zerov = upload(c, std::vector<dcomplex>(N, dcomplex{0, 0}));
+            const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+            tdz::launch_wall_fix<double>(c->st, L.n, w, c->zerov, y);
+        }
+        HIPCHK(hipGetLastError());
+        set_shift(c, a0, g0, s0);
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int tdz_level_dims(const tdz_ctx *c, int level, int n[3])
+{
+    if (!c || !n || level < 0 || level >= (int)c->lv.size()) { g_err = "no such level"; return 1; }
+    for (int d = 0; d < 3; ++d) n[d] = c->lv[level]->n[d];
+    return 0;
+}
+
+int tdz_mg_transfer(tdz_ctx *c, int level, int dir, const double *in, double *out)
+{
+    return guarded([&] {
+        if (level < 0 || level + 1 >= (int)c->lv.size()) throw Error("no coarser level below this one");
+        if (dir != 0 && dir != 1) throw Error("dir must be 0 (restrict) or 1 (prolong)");
+        Level &F = *c->lv[level], &Cl = *c->lv[level + 1];
+        const dcomplex *a = reinterpret_cast<const dcomplex *>(in);
+        dcomplex *b = reinterpret_cast<dcomplex *>(out);
+        if (dir == 0) transfer<double>(c, F.R, F.n, Cl.n, a, b, c->sd.w1, c->sd.w2, false);
+        else transfer<double>(c, F.P, Cl.n, F.n, a, b, c->sd.w1, c->sd.w2, false);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int tdz_level_apply(tdz_ctx *c, int level, const double *x, double *y)
+{
+    return guarded([&] {
+        if (level < 0 || level >= (int)c->lv.size()) throw Error("no such level");
+        refresh_coarse(c);
+        level_apply<double>(c, *c->lv[level], reinterpret_cast<const dcomplex *>(x), reinterpret_cast<dcomplex *>(y));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int tdz_set_pc_refresh(tdz_ctx *c, int max_its)
+{
+    return guarded([&] {
+        c->refresh_cap = max_its < 0 ? -1 : max_its;
+        c->refresh_base = -1;
+    });
+}
+
+int tdz_coarsen_knots(const double *U, int m, double *C, int *mc)
+{
+    return guarded([&] {
+        if (!U || !C || !mc || m < 2) throw Error("bad knot vector");
+        std::vector<double> c;
+        const bool removed = tdz::coarsen_knots(std::vector<double>(U, U + m), c);
+        *mc = removed ? (int)c.size() : 0;
+        if (removed) std::copy(c.begin(), c.end(), C);
+    });
+}
+
+int tdz_refinement_matrix(const double *C, int mc, const double *F, int mf, int p, double *R)
+{
+    return guarded([&] {
+        if (!C || !F || !R || p < 1 || mc < 2 * p + 2 || mf < mc) throw Error("bad knot vectors");
+        const auto rows = tdz::refinement_matrix(std::vector<double>(C, C + mc), std::vector<double>(F, F + mf), p);
+        const int nf = mf - p - 1, nc = mc - p - 1;
+        std::fill(R, R + (size_t)nf * nc, 0.0);
+        for (int i = 0; i < nf; ++i)
+            for (auto &e : rows[i]) R[(size_t)i * nc + e.first] = e.second;
+    });
+}
+
 }  // extern "C"
 
 namespace {
+// Strong-field check of TDSEZSetupMultigrid (src/multigrid.cpp:213-237): once tau max|E.r|
+// exceeds 1.5 over the pulse the field term dominates M + i tau H_E and the coarse levels
+// no longer represent it. tau = dt/2 (CN), dt/sqrt(12) (Pade(2,2)), dt/(2 sqrt(12))
+// (Magnus 4, half steps). Warns once per context (stderr and tdz_last_warning).
+void strong_field_check(tdz_ctx *c, const tdz_pulse *pulse, double tau, double dt)
+{
+    const bool mg = c->pc_kind == TDZ_PC_MG || c->pc_kind == TDZ_PC_MG_MIXED ||
+                    (c->pc_kind == TDZ_PC_LEARNED && c->learn.fallback_kind == TDZ_PC_MG);
+    if (!mg || !pulse || c->field_warned || pulse->omega <= 0 || pulse->ncycles <= 0 || dt <= 0) return;
+    const double T = pulse->ncycles * 2.0 * M_PI / pulse->omega;
+    double Lmax[3] = {0, 0, 0};
+    for (int a = 0; a < c->prob.dim; ++a) Lmax[a] = std::max(std::fabs(c->prob.lo[a]), std::fabs(c->prob.hi[a]));
+    const long n = (long)std::ceil(4.0 * T / dt);
+    double worst = 0;
+    for (long i = 0; i <= n; ++i) {
+        const double t = T * (double)i / (double)std::max(n, 1L);
+        double v = 0;
+        for (int a = 0; a < c->prob.dim; ++a) v += std::fabs(pulse_field(*pulse, a, t)) * Lmax[a];
+        worst = std::max(worst, v);
+    }
+    if (tau * worst > 1.5) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "tau*max|E.r| = %.2g > 1.5: the field term dominates the propagator operator and multigrid "
+                      "may not converge (use the separable preconditioner or a smaller time step)",
+                      tau * worst);
+        g_warn = msg;
+        std::fprintf(stderr, "tdz: warning: %s\n", msg);
+        c->field_warned = true;
+    }
+}
+
+// Preconditioner refresh rule of TDSEZPCRefresh (include/tdsez_propagator.hpp:17-31): the
+// first solve after a build sets base; a solve above 2 base + 5 iterations, or above the
+// cap (-tdsez_pc_refresh_its) when one is set, asks for a rebuild before the next solve.
+bool refresh_due(tdz_ctx *c, int &base, int its)
+{
+    const bool over_cap = c->refresh_cap >= 0 && its > c->refresh_cap;
+    if (!over_cap) {
+        if (base < 0) {
+            base = its;
+            return false;
+        }
+        if (its <= 2 * base + 5) return false;
+    }
+    base = -1;
+    ++c->rebuilds;
+    return true;
+}
+
 // Time-step solve with the current preconditioner. The learned preconditioner has
 // regressed when a solve needs more iterations than the same preconditioner without
 // the network took on the training steps: such a solve is stopped at that count, plain
@@ -1845,43 +2136,56 @@ int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], doub
 
 namespace {
 
-constexpr int kGuessOrder = 3;
+// guess history buffers for the current order: 4 slots (2 stages x 2 Magnus halves)
+void pade_hist_alloc(tdz_ctx *c)
+{
+    const long N = c->lv[0]->N;
+    if (c->hist && c->hist_order == c->guess_order) return;
+    if (c->hist) untrack(c, c->hist);
+    c->hist = track(c, dalloc<dcomplex>((size_t)4 * c->guess_order * N));
+    c->hist_order = c->guess_order;
+    for (int s = 0; s < 4; ++s) c->hist_count[s] = 0;
+    if (!c->xin) c->xin = track(c, dalloc<dcomplex>(N));
+}
 
-// psi <- R22(-i h M^-1 H_E) psi (src/propagator.cpp:564-676). Stage k solves
-// (M + g_k H_E) y = (M + conj(g_k) H_E) x from the guess x + extrapolated increment
-// (binomial weights of order <= 3 over this slot's earlier increments).
 void pade22_step(tdz_ctx *c, dcomplex *x, double h, const double sE[3], int slot, double rtol, int maxit,
                  tdz_ksp_stats *stats)
 {
     if (!c->pc_ready) pc_setup(c, c->pc_kind);
     Level &L = *c->lv[0];
     const long N = L.N;
-    if (!c->hist) {
-        c->hist = track(c, dalloc<dcomplex>((size_t)4 * kGuessOrder * N));
-        c->xin = track(c, dalloc<dcomplex>(N));
-    }
+    pade_hist_alloc(c);
+    const int order = c->guess_order;
     const double s3 = std::sqrt(3.0);
     const cplx gk[2] = {h * cplx(-s3, 3.0) / 12.0, h * cplx(s3, 3.0) / 12.0};
-    static const double w[3][3] = {{1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
+    // x + sum_j w_j d_j, d_0 the newest increment: the order-q binomial extrapolation
+    auto binom = [](int q, int j) {
+        double b = 1;
+        for (int i = 0; i < j + 1; ++i) b = b * (q - i) / (i + 1);
+        return (j % 2 ? -1.0 : 1.0) * b;
+    };
     tdz_ksp_stats tot{};
     for (int k = 0; k < 2; ++k) {
         const int sl = 2 * slot + k;
-        dcomplex *H = c->hist + (size_t)sl * kGuessOrder * N;
+        dcomplex *H = c->hist + (size_t)sl * order * N;
         set_shift(c, 1.0, std::conj(gk[k]), sE);
         level_apply(c, L, x, c->rhs);
         HIPCHK(hipMemcpyAsync(c->xin, x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
-        const int q = std::min(kGuessOrder, c->hist_count[sl]);
-        for (int j = 0; j < q; ++j) tdz::launch_axpby(c->st, {w[q - 1][j], 0}, H + (size_t)j * N, {1, 0}, x, N);
+        const int q = c->guess_extrap ? std::min(order, c->hist_count[sl]) : 0;
+        for (int j = 0; j < q; ++j) tdz::launch_axpby(c->st, {binom(q, j), 0}, H + (size_t)j * N, {1, 0}, x, N);
         set_shift(c, 1.0, gk[k], sE);
         tdz_ksp_stats st{};
         step_solve(c, c->rhs, x, rtol, maxit, &st, false);
-        // history: shift, newest increment y - x first
-        for (int j = kGuessOrder - 1; j > 0; --j)
-            HIPCHK(hipMemcpyAsync(H + (size_t)j * N, H + (size_t)(j - 1) * N, N * sizeof(dcomplex),
-                                  hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(hipMemcpyAsync(H, x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
-        tdz::launch_axpby(c->st, {-1, 0}, c->xin, {1, 0}, H, N);
-        c->hist_count[sl] = std::min(c->hist_count[sl] + 1, kGuessOrder);
+        if (c->guess_extrap) {   // history: shift, newest increment y - x first
+            for (int j = order - 1; j > 0; --j)
+                HIPCHK(hipMemcpyAsync(H + (size_t)j * N, H + (size_t)(j - 1) * N, N * sizeof(dcomplex),
+                                      hipMemcpyDeviceToDevice, c->st));
+            HIPCHK(hipMemcpyAsync(H, x, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+            tdz::launch_axpby(c->st, {-1, 0}, c->xin, {1, 0}, H, N);
+            c->hist_count[sl] = std::min(c->hist_count[sl] + 1, order);
+        }
+        // each stage keeps its own refresh tracker (src/propagator.cpp:650-668)
+        if (refresh_due(c, c->stage_base[k], st.its)) pc_setup(c, c->pc_kind);
         tot.its += st.its;
         tot.restarts += st.restarts;
         tot.res = std::max(tot.res, st.res);
@@ -2257,6 +2561,8 @@ int tdz_evolve_rational(tdz_ctx *c, double *psi, double t0, double dt, int nstep
         dcomplex *x = reinterpret_cast<dcomplex *>(psi);
         // commutator-free Magnus 4 (src/propagator.cpp:729-758): Gauss nodes c1,2, weights a1,2
         const double s3 = std::sqrt(3.0);
+        if (!c->pc_ready) pc_setup(c, c->pc_kind);
+        strong_field_check(c, pulse, magnus ? dt / (4.0 * s3) : dt / (2.0 * s3), dt);   // dt/(2 sqrt12), dt/sqrt12
         const double c1 = 0.5 - s3 / 6.0, c2 = 0.5 + s3 / 6.0;
         const double a1 = (3.0 - 2.0 * s3) / 12.0, a2 = (3.0 + 2.0 * s3) / 12.0;
         for (int s = 0; s < nsteps; ++s) {
@@ -2275,7 +2581,8 @@ int tdz_evolve_rational(tdz_ctx *c, double *psi, double t0, double dt, int nstep
                     tdz_pulse_coupling(c, pulse, t + c2 * dt, E2);
                 }
                 for (int a = 0; a < 3; ++a) {
-                    Ea[a] = 2.0 * (a2 * E1[a] + a1 * E2[a]);   // applied first
+                    // the first half step (a2 > |a1|) weights the earlier Gauss node most, the second the later one
+                    Ea[a] = 2.0 * (a2 * E1[a] + a1 * E2[a]);
                     Eb[a] = 2.0 * (a1 * E1[a] + a2 * E2[a]);
                 }
                 pade22_step(c, x, 0.5 * dt, Ea, 0, rtol, maxit, &st);
@@ -2363,18 +2670,31 @@ int tdz_pade_history(tdz_ctx *c, int set, int counts[4], double *hist)
 {
     return guarded([&] {
         const long N = c->lv[0]->N;
-        const size_t bytes = (size_t)4 * kGuessOrder * N * sizeof(dcomplex);
         if (!set) {
             for (int s = 0; s < 4; ++s) counts[s] = c->hist ? c->hist_count[s] : 0;
-            if (c->hist && hist) HIPCHK(hipMemcpy(hist, c->hist, bytes, hipMemcpyDeviceToHost));
+            if (c->hist && hist)
+                HIPCHK(hipMemcpy(hist, c->hist, (size_t)4 * c->hist_order * N * sizeof(dcomplex),
+                                 hipMemcpyDeviceToHost));
+            else if (hist)
+                std::memset(hist, 0, (size_t)4 * c->guess_order * N * sizeof(dcomplex));
             return;
         }
-        if (!c->hist) {
-            c->hist = track(c, dalloc<dcomplex>((size_t)4 * kGuessOrder * N));
-            c->xin = track(c, dalloc<dcomplex>(N));
+        pade_hist_alloc(c);
+        for (int s = 0; s < 4; ++s) c->hist_count[s] = std::max(0, std::min(counts[s], c->guess_order));
+        if (hist)
+            HIPCHK(hipMemcpy(c->hist, hist, (size_t)4 * c->guess_order * N * sizeof(dcomplex), hipMemcpyHostToDevice));
+    });
+}
+
+int tdz_set_pade_guess(tdz_ctx *c, int extrapolate, int order)
+{
+    return guarded([&] {
+        if (order < 1 || order > 5) throw Error("Pade guess order must be 1..5");
+        c->guess_extrap = extrapolate != 0;
+        if (order != c->guess_order) {
+            c->guess_order = order;
+            for (int s = 0; s < 4; ++s) c->hist_count[s] = 0;
         }
-        for (int s = 0; s < 4; ++s) c->hist_count[s] = std::max(0, std::min(counts[s], kGuessOrder));
-        if (hist) HIPCHK(hipMemcpy(c->hist, hist, bytes, hipMemcpyHostToDevice));
     });
 }
 
@@ -2418,6 +2738,7 @@ int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, con
 {
     return guarded([&] {
         if (!c->pc_ready) pc_setup(c, c->pc_kind);
+        strong_field_check(c, pulse, 0.5 * dt, dt);
         dcomplex *x = reinterpret_cast<dcomplex *>(psi);
         if (c->pred.order > 0 && !pred_continues(c, x, dt)) c->pred.count = 0;
         for (int s = 0; s < nsteps; ++s) {
@@ -2427,16 +2748,9 @@ int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, con
             tdz_ksp_stats st{};
             if (c->pred.order > 0) cn_step_pred(c, x, dt, sE, rtol, maxit, &st);
             else cn_step(c, x, dt, sE, rtol, maxit, &st, s > 0);
-            // refresh rule (include/tdsez_propagator.hpp:17-31): the preconditioner is rebuilt
-            // from the current operator when a solve needs > 2 base + 5 iterations
-            if (c->refresh_base < 0) c->refresh_base = st.its;
-            else if (st.its > 2 * c->refresh_base + 5) {
-                c->refresh_base = -1;
-                // the separable solves depend on (alpha, g) only and every level operator
-                // follows the field through set_shift, so a rebuild has nothing to redo
-                ++c->rebuilds;
-                pc_setup(c, c->pc_kind);
-            }
+            // the separable solves depend on (alpha, g) only and every level operator follows
+            // the field through set_shift, so a rebuild has nothing to redo but is counted
+            if (refresh_due(c, c->refresh_base, st.its)) pc_setup(c, c->pc_kind);
             if (its) its[s] = st.its;
             if (obs) {
                 double o[6];
@@ -2901,6 +3215,10 @@ int tdz_batch_evolve(tdz_batch *b, int nsteps, double rtol, int maxit, int *its,
                     set_shift(c, 1.0, cplx(0.0, 0.5 * b->dt), e);
                     tdz_ksp_stats st{};
                     fgmres(c, c->rhs, b->psi[q], rtol, maxit, &st);
+                    // fgmres leaves its final update queued on the run's stream; the next
+                    // graph launch (batch stream) reads psi and rewrites the solver's
+                    // buffers, so the run's stream must drain first
+                    HIPCHK(hipStreamSynchronize(c->st));
                     it += st.its;
                     r[q] = st.res;
                     ++b->fixups;

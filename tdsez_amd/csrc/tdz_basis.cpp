@@ -7,9 +7,11 @@
 //   Gauss-Legendre rule per element, weight*detJac src/core.cpp:1793, src/tensorops.cpp:42
 //   1D bands M, S, X and the CAP band W_d          src/tensorops.cpp:40-57, src/assembly/cap.cpp:52-88
 //   CAP profiles (Manolopoulos; 1/m scale)         src/assembly/assembly.cpp:170-208, :1892-1907
-//   knot coarsening, Boehm insertion, Greville     src/multigrid.cpp:25-93
+//   knot coarsening rule, knot-insertion P         src/multigrid.cpp:25-93 (behaviour; own algorithms:
+//                                                  run-length scan, Oslo algorithm, collocation check)
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <stdexcept>
 
 #include "tdz_internal.hpp"
@@ -323,74 +325,126 @@ void build_axis(Axis1D &A, const std::vector<double> &knots, int p, int Q, bool 
         }
 }
 
+// One multigrid coarsening of a knot vector: scan the runs of equal knots in order and
+// remove every second element boundary (runs 1, 3, 5, ... counted from the left end) that
+// is a simple knot; repeated knots (C0 interfaces) and both ends stay. Returns false when
+// the scan removes nothing (the rule of src/multigrid.cpp:25-40).
 bool coarsen_knots(const std::vector<double> &U, std::vector<double> &C)
 {
-    std::vector<std::pair<double, int>> bp;  // breakpoints with multiplicity
-    for (double u : U) {
-        if (!bp.empty() && u == bp.back().first) ++bp.back().second;
-        else bp.push_back({u, 1});
-    }
     C.clear();
-    bool dropped = false;
-    for (size_t i = 0; i < bp.size(); ++i) {
-        const bool interior = i > 0 && i + 1 < bp.size();
-        if (interior && bp[i].second == 1 && (i & 1)) {
-            dropped = true;
-            continue;
-        }
-        C.insert(C.end(), bp[i].second, bp[i].first);
+    C.reserve(U.size());
+    bool removed = false;
+    size_t run = 0;   // ordinal of the current run of equal knots
+    for (size_t first = 0; first < U.size(); ++run) {
+        size_t past = first;
+        while (past < U.size() && U[past] == U[first]) ++past;
+        const bool simple = past - first == 1, at_end = past == U.size();
+        if (run % 2 == 1 && simple && !at_end) removed = true;
+        else C.insert(C.end(), U.begin() + first, U.begin() + past);
+        first = past;
     }
-    return dropped;
+    return removed;
 }
 
+// Non-empty knot spans (elements) of a knot vector: distinct values minus one.
 int knot_spans(const std::vector<double> &U)
 {
-    int n = 0;
-    for (size_t i = 0; i + 1 < U.size(); ++i) n += U[i + 1] > U[i];
-    return n;
+    std::vector<double> v(U);
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return (int)v.size() - 1;
 }
 
-std::vector<std::vector<std::pair<int, double>>> boehm_prolongation(const std::vector<double> &C,
-                                                                    const std::vector<double> &F, int p)
+// Knot-insertion (refinement) matrix of the nested spaces S_p(C) in S_p(F), F a
+// refinement of C: coarse B-spline j = sum_i R(i, j) fine B-spline i. Row i holds the
+// discrete B-splines of the Oslo algorithm (Cohen, Lyche and Riesenfeld 1980): the
+// blossoms of the coarse B-splines at the fine knots F[i+1..i+p], evaluated on the coarse
+// span mu with C[mu] <= F[i] < C[mu+1] by p convex-combination sweeps
+//   a_k(j) = (x_k - C[j]) / (C[j+k] - C[j]) a_{k-1}(j)
+//          + (C[j+k+1] - x_k) / (C[j+k+1] - C[j+1]) a_{k-1}(j+1),   x_k = F[i+k],
+// from a_0 = 1 at j = mu. Only the terms with a nonzero a_{k-1} are formed; their
+// denominators span [C[mu], C[mu+1]] and are positive. Rows are sparse (columns
+// mu-p..mu), O(p^2) per fine function.
+std::vector<std::vector<std::pair<int, double>>> refinement_matrix(const std::vector<double> &C,
+                                                                   const std::vector<double> &F, int p)
 {
-    // dense working rows over the coarse functions; inserted one knot at a time
-    std::vector<double> U = C;
-    const int nc = (int)C.size() - p - 1;
-    std::vector<std::vector<double>> P(nc, std::vector<double>(nc, 0.0));
-    for (int j = 0; j < nc; ++j) P[j][j] = 1.0;
-    std::vector<double> add;
-    std::set_difference(F.begin(), F.end(), C.begin(), C.end(), std::back_inserter(add));
-    for (double xi : add) {
-        const int n = (int)U.size() - p - 1;
-        int k = (int)(std::upper_bound(U.begin(), U.end(), xi) - U.begin()) - 1;
-        k = std::min(std::max(k, p), n - 1);
-        std::vector<std::vector<double>> R(n + 1);
-        for (int i = 0; i <= k - p; ++i) R[i] = P[i];
-        for (int i = k + 1; i <= n; ++i) R[i] = P[i - 1];
-        for (int i = k - p + 1; i <= k; ++i) {
-            const double a = (xi - U[i]) / (U[i + p] - U[i]);
-            R[i].assign(nc, 0.0);
-            for (int c = 0; c < nc; ++c) R[i][c] = a * P[i][c] + (1.0 - a) * P[i - 1][c];
+    const int nf = (int)F.size() - p - 1, nc = (int)C.size() - p - 1;
+    if (nf < nc || F.front() != C.front() || F.back() != C.back())
+        throw std::runtime_error("refinement_matrix: the fine knots do not refine the coarse ones");
+    std::vector<std::vector<std::pair<int, double>>> rows(nf);
+    std::vector<double> a(p + 1), b(p + 1);
+    for (int i = 0; i < nf; ++i) {
+        const int mu = (int)(std::upper_bound(C.begin(), C.end(), F[i]) - C.begin()) - 1;
+        if (mu < p || mu >= nc) throw std::runtime_error("refinement_matrix: knot vectors are not open");
+        // a[r] holds column mu - k + r after sweep k
+        a[0] = 1.0;
+        for (int k = 1; k <= p; ++k) {
+            const double x = F[i + k];
+            for (int r = 0; r <= k; ++r) {
+                const int j = mu - k + r;
+                double v = 0.0;
+                if (r >= 1) v += (x - C[j]) / (C[j + k] - C[j]) * a[r - 1];              // a_{k-1}(j)
+                if (r < k) v += (C[j + k + 1] - x) / (C[j + k + 1] - C[j + 1]) * a[r];  // a_{k-1}(j+1)
+                b[r] = v;
+            }
+            std::swap(a, b);
         }
-        P.swap(R);
-        U.insert(std::upper_bound(U.begin(), U.end(), xi), xi);
+        for (int r = 0; r <= p; ++r)
+            if (a[r] != 0.0) rows[i].push_back({mu - p + r, a[r]});
     }
-    std::vector<std::vector<std::pair<int, double>>> rows(P.size());
-    for (size_t i = 0; i < P.size(); ++i)
-        for (int c = 0; c < nc; ++c)
-            if (std::fabs(P[i][c]) >= 1e-14) rows[i].push_back({c, P[i][c]});
     return rows;
 }
 
+// Greville abscissae: the mean of the p interior knots of each B-spline.
 std::vector<double> greville(const std::vector<double> &U, int p)
 {
     const int n = (int)U.size() - p - 1;
-    std::vector<double> g(n, 0.0);
+    std::vector<double> g(n);
+    double window = std::accumulate(U.begin() + 1, U.begin() + 1 + p, 0.0);
     for (int i = 0; i < n; ++i) {
-        for (int j = 1; j <= p; ++j) g[i] += U[i + j];
-        g[i] /= std::max(p, 1);
+        g[i] = window / p;
+        if (i + 1 < n) window += U[i + 1 + p] - U[i + 1];
     }
     return g;
+}
+
+// Values of the p+1 B-splines of degree p on knots U that are nonzero at x, the first
+// one's index in first.
+void basis_values(const std::vector<double> &U, int p, double x, int &first, double *vals)
+{
+    const int n = (int)U.size() - p - 1;
+    int k = (int)(std::upper_bound(U.begin(), U.end(), x) - U.begin()) - 1;
+    k = std::max(p, std::min(k, n - 1));
+    std::vector<double> d(2 * (p + 1));
+    ders_basis(k, x, p, U, 0, d.data());
+    first = k - p;
+    std::copy(d.begin(), d.begin() + p + 1, vals);
+}
+
+// Largest deviation from N_c,j(x) = sum_i R(i, j) N_f,i(x) over three points in every
+// fine element (the two-scale identity of nested spline spaces, which determines R).
+double refinement_error(const std::vector<double> &C, const std::vector<double> &F, int p,
+                        const std::vector<std::vector<std::pair<int, double>>> &R)
+{
+    const int nc = (int)C.size() - p - 1;
+    std::vector<double> vf(p + 1), vc(p + 1), sum(nc, 0.0);
+    double err = 0.0;
+    for (size_t k = 0; k + 1 < F.size(); ++k) {
+        if (!(F[k + 1] > F[k])) continue;
+        for (double s : {0.1127, 0.5, 0.8873}) {
+            const double x = F[k] + s * (F[k + 1] - F[k]);
+            int f0, c0;
+            basis_values(F, p, x, f0, vf.data());
+            basis_values(C, p, x, c0, vc.data());
+            for (int a = 0; a <= p; ++a)
+                for (auto &e : R[f0 + a]) sum[e.first] += e.second * vf[a];
+            for (int j = std::max(0, c0 - p - 1); j < std::min(nc, c0 + 2 * p + 2); ++j) {
+                const double want = (j >= c0 && j <= c0 + p) ? vc[j - c0] : 0.0;
+                err = std::max(err, std::fabs(sum[j] - want));
+                sum[j] = 0.0;
+            }
+        }
+    }
+    return err;
 }
 
 void gen_eig_bands(const Axis1D &A, const std::vector<double> &Tband, std::vector<double> &V,

@@ -40,12 +40,19 @@ void build_axis(Axis1D &A, const std::vector<double> &knots, int p, int Q, bool 
 void build_degenerate_axis(Axis1D &A);
 // Open uniform knot vector, continuity p-1.
 std::vector<double> uniform_knots(int E, double lo, double hi, int p);
-// Drop every other simple interior breakpoint (src/multigrid.cpp:25-40); false if none dropped.
+// Remove every second simple interior element boundary (the coarsening rule of
+// src/multigrid.cpp:25-40); false if nothing was removed.
 bool coarsen_knots(const std::vector<double> &U, std::vector<double> &C);
 int knot_spans(const std::vector<double> &U);
-// Boehm knot-insertion prolongation rows (coarse index, weight), fine x coarse.
-std::vector<std::vector<std::pair<int, double>>> boehm_prolongation(const std::vector<double> &C,
-                                                                    const std::vector<double> &F, int p);
+// Knot-insertion matrix of S_p(C) in S_p(F) as sparse rows (coarse index, weight), fine x
+// coarse (Oslo algorithm).
+std::vector<std::vector<std::pair<int, double>>> refinement_matrix(const std::vector<double> &C,
+                                                                   const std::vector<double> &F, int p);
+// max |N_c,j(x) - sum_i R(i, j) N_f,i(x)| over sample points of every fine element
+double refinement_error(const std::vector<double> &C, const std::vector<double> &F, int p,
+                        const std::vector<std::vector<std::pair<int, double>>> &R);
+// the p+1 B-splines of degree p on U nonzero at x (first: index of the first)
+void basis_values(const std::vector<double> &U, int p, double x, int &first, double *vals);
 std::vector<double> greville(const std::vector<double> &U, int p);
 // Symmetric eigen-decomposition (Householder tridiagonalisation + implicit QL):
 // A (n x n row-major) is replaced by the eigenvectors as columns.
@@ -225,6 +232,30 @@ void launch_csr_axis(hipStream_t st, int d, const int nin[3], int m, const int *
 template <typename R>
 void launch_zline_solve(hipStream_t st, const int n[3], int p, const R *lx, const R *ly, const cpx<R> *Bz,
                         const R *Mw, cpx<R> g, const cpx<R> *rhs, cpx<R> *x, cpx<R> *work);
+
+// ---------------------------------------------------------------- exact Galerkin potential
+// Per axis: for every (function i, offset d) pair of a coarse level (index (d+p) n + i),
+// the products N_i N_{i+d} at the run of the finest level's Gauss points [q0, q0+len)
+// where both are nonzero (values from val + off).
+struct GalPairs {
+    int npairs = 0;
+    const int *q0 = nullptr, *len = nullptr, *off = nullptr;
+    const double *val = nullptr;
+};
+struct GalerkinPlan {
+    GalPairs pairs[3];
+    const double *X[3] = {}, *W[3] = {};   // finest level's Gauss points and weights per axis
+    int nq[3] = {1, 1, 1};
+    PotParams pot{};
+};
+// S = P^T (s V) P of one coarse level as an offset-major stencil (zeroed first by the
+// first slab); work: work_len doubles of scratch
+void galerkin_potential(hipStream_t st, const GalerkinPlan &g, double *S, double *work, long work_len);
+void launch_stencil_demote(hipStream_t st, const double *S, float *Sf, long n);
+// y += g S x for an offset-major stencil of half-widths p[a] (0 on unused axes)
+template <typename R>
+void launch_stencil_apply(hipStream_t st, const int n[3], const int p[3], const R *S, R gr, R gi, const cpx<R> *x,
+                          cpx<R> *y);
 
 // ---------------------------------------------------------------- device FGMRES(k)
 constexpr int kGmMax = 8;   // longest captured cycle

@@ -104,7 +104,7 @@ def read_checkpoint(path, ctx, psi, dt, propagator="cn"):
     if code == 0:
         x, step, t, _, _ = read_checkpoint_host(path, ctx.N, dt, 0)
     else:
-        x, step, t, cnt, hist = read_checkpoint_host(path, ctx.N, dt, code, 4, 3)
+        x, step, t, cnt, hist = read_checkpoint_host(path, ctx.N, dt, code, 4, ctx.guess_order)
         ctx.pade_history(cnt, hist)
     psi.upload(x)
     return step, t

@@ -1,0 +1,184 @@
+"""Geometric B-spline multigrid on the GPU against the oracle, through the C ABI.
+
+- transfers: the device P and R = P^T of every level against the Kronecker product of
+  the oracle's Boehm prolongations (src/multigrid.cpp:52-81, :118-162);
+- coarse operators: tdz_level_apply at levels 1 and 2 against P^T A_0 P with A_0 the
+  oracle's matrix-free operator (PCMGSetGalerkin(PC_MG_GALERKIN_BOTH),
+  src/multigrid.cpp:277), every term including the potential, CAP and field;
+- CN steps preconditioned by the 3-level V-cycle (FP64 and complex64) against the
+  oracle's explicit steps, and the preconditioner refresh rule and strong-field warning
+  (include/tdsez_propagator.hpp:17-31, src/multigrid.cpp:213-237,
+  tests/test_multigrid.py:67-94 of the reference).
+Tolerances: transfers and operators 1e-13 relative (rounding), CN steps 1e-12 in L2.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# config 2 shape with enough elements for a 3-level hierarchy (p = 4: 20 -> 10 -> 5)
+MG3D = dict(dim=3, nelem=(20, 20, 24), lo=(-10.0, -10.0, -12.0), hi=(10.0, 10.0, 12.0), p=4, nquad=8, mass=1.0,
+            charge=-1.0, pot_kind=2, pot=(1.0, 2.0), cap_kind=1, cap=(0.01, 3.0), mg_levels=3)
+# p = 3, Q = 5, Manolopoulos CAP, heavier carrier, uneven axes (16 -> 8 -> 4)
+MG3D_P3 = dict(dim=3, nelem=(16, 20, 16), lo=(-8.0, -9.0, -8.0), hi=(8.0, 9.0, 8.0), p=3, nquad=5, mass=1.7,
+               charge=-1.0, pot_kind=2, pot=(1.0, 1.0), cap_kind=2, cap=(1.2,), mg_levels=3)
+PROBLEMS = {"mg3d_p4": MG3D, "mg3d_p3": MG3D_P3}
+
+
+def _rand(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _kron_apply(P, x, nin):
+    """y = (P_z (x) P_y (x) P_x) x for x in x-fastest order of sizes nin."""
+    X = x.reshape(nin[2], nin[1], nin[0])
+    X = np.einsum("ia,kja->kji", P[0], X)
+    X = np.einsum("jb,kbi->kji", P[1], X)
+    X = np.einsum("kc,cji->kji", P[2], X)
+    return X.reshape(-1)
+
+
+@pytest.fixture(scope="module", params=list(PROBLEMS))
+def mg(request):
+    import tdsez_amd as T
+    import oracle
+    desc = PROBLEMS[request.param]
+    o = oracle.Oracle(desc)
+    ctx = T.Context(desc)
+    assert ctx.levels == 3, ctx.levels
+    p = desc["p"]
+    knots = [[o.knots(d) for d in range(3)]]
+    for _ in range(2):
+        knots.append([oracle.coarsen_knots(k) for k in knots[-1]])
+    # P[l][d]: level l+1 -> level l along axis d
+    P = [[oracle.prolongation(knots[l + 1][d], knots[l][d], p) for d in range(3)] for l in range(2)]
+    dims = [tuple(len(knots[l][d]) - p - 1 for d in range(3)) for l in range(3)]
+    for l in range(3):
+        assert ctx.level_dims(l) == dims[l], (l, ctx.level_dims(l), dims[l])
+    yield request.param, desc, o, ctx, P, dims
+    ctx.close()
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_transfers_match_oracle(mg, level):
+    name, desc, o, ctx, P, dims = mg
+    nf, nc = dims[level], dims[level + 1]
+    xc = _rand(int(np.prod(nc)), 1)
+    xf = _rand(int(np.prod(nf)), 2)
+    pad = lambda v: np.concatenate([v, np.zeros(ctx.N - v.size, complex)])
+    dxc, dxf, dy = ctx.vec(pad(xc)), ctx.vec(pad(xf)), ctx.vec()
+    ctx.mg_transfer(level, 1, dxc, dy)                       # prolong
+    ref = _kron_apply(P[level], xc, nc)
+    assert _rel(dy.download()[:ref.size], ref) < 1e-13, name
+    ctx.mg_transfer(level, 0, dxf, dy)                       # restrict
+    ref = _kron_apply([M.T for M in P[level]], xf, nf)
+    assert _rel(dy.download()[:ref.size], ref) < 1e-13, name
+
+
+@pytest.mark.parametrize("level", [1, 2])
+@pytest.mark.parametrize("alpha,g,sE", [(1.0, 0.01j, (0.0, 0.0, 0.05)), (0.3 - 0.2j, 0.7 + 0.1j, (0.02, -0.01, 0.05))])
+def test_galerkin_coarse_operator(mg, level, alpha, g, sE):
+    """The coarse operator is exactly P^T A P, potential included (no rediscretisation)."""
+    name, desc, o, ctx, P, dims = mg
+    nl = dims[level]
+    x = _rand(int(np.prod(nl)), 3)
+    # composite prolongation level -> 0
+    Pl = [np.eye(dims[0][d]) for d in range(3)]
+    for l in range(level):
+        Pl = [Pl[d] @ P[l][d] for d in range(3)]
+    sE = np.array(sE)
+    ref = _kron_apply([M.T for M in Pl], o.apply(alpha, g, sE, _kron_apply(Pl, x, nl)), dims[0])
+    ctx.set_shift(alpha, g, sE)
+    dx = ctx.vec(np.concatenate([x, np.zeros(ctx.N - x.size, complex)]))
+    dy = ctx.vec()
+    ctx.level_apply(level, dx, dy)
+    assert _rel(dy.download()[:x.size], ref) < 1e-13, (name, level, _rel(dy.download()[:x.size], ref))
+
+
+def test_fine_level_apply_is_the_operator(mg):
+    name, desc, o, ctx, P, dims = mg
+    x = _rand(o.N, 4)
+    sE = np.array([0.0, 0.01, 0.03])
+    ctx.set_shift(1.0, 0.01j, sE)
+    dx, dy = ctx.vec(x), ctx.vec()
+    ctx.level_apply(0, dx, dy)
+    assert _rel(dy.download(), o.apply(1.0, 0.01j, sE, x)) < 1e-13
+
+
+@pytest.mark.parametrize("pc_name", ["MG", "MG_MIXED"])
+def test_mg_cn_steps_match_oracle(mg, pc_name):
+    import tdsez_amd as T
+    name, desc, o, ctx, P, dims = mg
+    dt, nsteps = 0.02, 3
+    psi0 = o.project_gaussian(centre=(0.5, -0.3, 0.2), width=(1.2, 1.0, 1.1), momentum=(0.3, 0.0, -0.2))
+    sE = np.zeros((nsteps, 3))
+    sE[:, 2] = 0.05 * np.cos(0.3 * (np.arange(nsteps) + 0.5) * dt)
+    ref, _, _ = o.cn_steps(psi0, dt, sE, rtol=1e-13)
+    ctx.pc_setup(getattr(T, "PC_" + pc_name))
+    d = ctx.vec(psi0)
+    its = []
+    for s in range(nsteps):
+        its.append(ctx.step_cn(d, dt, sE[s], rtol=1e-13).its)
+    diff = d.download() - ref
+    err = np.sqrt(abs(np.vdot(diff, o.term(0, diff)))) / np.sqrt(abs(np.vdot(ref, o.term(0, ref))))
+    assert err < 1e-12, (name, pc_name, err)
+    assert max(its) <= 10, (name, pc_name, its)
+
+
+def test_refresh_cap_rebuilds_every_solve(mg):
+    """-tdsez_pc_refresh_its 0: every solve with an iteration asks for a rebuild; the answer
+    is the same (the preconditioner never changes the solution beyond the tolerance)."""
+    import tdsez_amd as T
+    name, desc, o, ctx, P, dims = mg
+    psi0 = o.project_gaussian(centre=(0.2, 0.0, -0.1), width=(1.0, 1.0, 1.0))
+    pulse = T.make_pulse((0.0, 0.0, 0.05), 0.057, 2.0)
+    ctx.pc_setup(T.PC_MG)
+    base = ctx.vec(psi0)
+    ctx.set_pc_refresh(-1)
+    its0, _ = ctx.evolve_cn(base, 0.0, 0.02, 3, pulse, rtol=1e-12)
+    st0 = ctx.step_cn(ctx.vec(psi0), 0.02, (0, 0, 0.01), rtol=1e-12).pc_rebuilds
+    capped = ctx.vec(psi0)
+    ctx.set_pc_refresh(0)
+    its1, _ = ctx.evolve_cn(capped, 0.0, 0.02, 3, pulse, rtol=1e-12)
+    st1 = ctx.step_cn(ctx.vec(psi0), 0.02, (0, 0, 0.01), rtol=1e-12).pc_rebuilds
+    ctx.set_pc_refresh(-1)
+    assert st1 - st0 == 3, (st0, st1, its1)
+    a, b = base.download(), capped.download()
+    assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-10
+
+
+def test_refresh_rule_two_base_plus_five(mg):
+    """A solve above 2 base + 5 iterations (base: the first solve after a build) rebuilds."""
+    import tdsez_amd as T
+    name, desc, o, ctx, P, dims = mg
+    psi0 = o.project_gaussian(centre=(0.2, 0.0, -0.1), width=(1.0, 1.0, 1.0))
+    ctx.pc_setup(T.PC_SEPARABLE)
+    ctx.set_pc_refresh(-1)
+    d = ctx.vec(psi0)
+    r0 = ctx.step_cn(d, 0.02, (0, 0, 0), rtol=1e-12).pc_rebuilds   # step_cn applies no rule
+    its_a, _ = ctx.evolve_cn(d, 0.0, 0.02, 1, None, rtol=1e-3)       # base: a loose solve
+    # a long step: the potential, which the separable solve leaves out, dominates
+    its_b, _ = ctx.evolve_cn(d, 0.02, 2.0, 1, None, rtol=1e-10, maxit=2000)
+    r1 = ctx.step_cn(ctx.vec(psi0), 0.02, (0, 0, 0), rtol=1e-6).pc_rebuilds
+    expect = 1 if its_b[0] > 2 * its_a[0] + 5 else 0
+    assert its_b[0] > 2 * its_a[0] + 5, (its_a, its_b)
+    assert r1 - r0 == expect, (r0, r1, its_a, its_b)
+
+
+def test_strong_field_warning():
+    """tau max|E.r| > 1.5 with multigrid warns (the reference's LaserX = 2.0, TimeStep = 0.2)."""
+    import tdsez_amd as T
+    desc = dict(MG3D, nelem=(20, 20, 20), lo=(-10.0,) * 3, hi=(10.0,) * 3)
+    ctx = T.Context(desc)
+    try:
+        ctx.pc_setup(T.PC_MG)
+        psi = ctx.vec(np.zeros(ctx.N, complex))
+        ctx.evolve_cn(psi, 0.0, 0.2, 0, T.make_pulse((0.0, 0.0, 2.0), 1.0, 1.0), rtol=1e-10)
+        assert "tau*max|E.r|" in T.last_warning()
+    finally:
+        ctx.close()

@@ -101,3 +101,56 @@ def test_ground_state_oscillator():
     obs = ctx.observables(d)
     assert abs(obs[0] - 1.0) < 1e-10 and abs(obs[1] - 0.5) < 1e-8
     ctx.close()
+
+
+def test_ifunction_matches_oracle():
+    """F = i M psidot - H psi - sE.D psi (TDSEZIFunctionPol*, src/monitor.cpp:108-162), and the
+    shift of the Jacobian (tdz_set_shift) is left as it was."""
+    o, ctx = _pair(SOFTCORE_3D)
+    rng = np.random.default_rng(21)
+    psi = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    psidot = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    sE = np.array([0.01, -0.02, 0.05])
+    ref = 1j * o.term(0, psidot) + o.apply(0.0, -1.0, sE, psi)
+    dp, dd, dF = ctx.vec(psi), ctx.vec(psidot), ctx.vec()
+    ctx.set_shift(1.0, 0.01j, (0.0, 0.0, 0.02))
+    ctx.ifunction(sE, dp, dd, dF)
+    F = dF.download()
+    assert np.abs(F - ref).max() / np.abs(ref).max() < 1e-13
+    # the Jacobian shift survives: A psi = M psi + 0.01i (H + 0.02 D_z) psi
+    dy = ctx.vec()
+    ctx.apply(dp, dy)
+    ref2 = o.apply(1.0, 0.01j, np.array([0.0, 0.0, 0.02]), psi)
+    assert np.abs(dy.download() - ref2).max() / np.abs(ref2).max() < 1e-13
+    # TSTHETA's residual at the CN solution: F(t_mid, (x0+x1)/2, (x1-x0)/dt) = 0
+    dt, sEm = 0.02, np.array([0.0, 0.0, 0.03])
+    x0 = o.project_gaussian((0.3, 0.0, -0.2), (1.0, 1.1, 0.9))
+    x1, _, _ = o.cn_steps(x0, dt, sEm.reshape(1, 3), rtol=1e-14)
+    ctx.ifunction(sEm, ctx.vec(0.5 * (x0 + x1)), ctx.vec((x1 - x0) / dt), dF)
+    assert np.linalg.norm(dF.download()) / np.linalg.norm(o.term(0, x0)) < 1e-9
+    ctx.close()
+
+
+@pytest.mark.parametrize("order,extrapolate", [(1, True), (3, True), (5, True), (3, False)])
+def test_pade_guess_options_keep_the_answer(order, extrapolate):
+    """-pade_guess_order / -pade_extrapolate_guess change the initial guess of the stage solves,
+    not their answer (src/propagator.cpp:620-646, :723-726)."""
+    import tdsez_amd as T
+    o, ctx = _pair(TINY)
+    psi = o.project_gaussian((0.3, 0.0, -0.2), (1.0, 1.1, 0.9))
+    h, nsteps = 0.05, 6
+    ref = psi
+    for s in range(nsteps):
+        sE = np.array([0.0, 0.0, 0.04 * math.sin(0.2 * (s + 0.5) * h)])
+        ref, _ = o.pade22_step(ref, h, sE, rtol=1e-14)
+    ctx.pc_setup(T.PC_SEPARABLE)
+    ctx.set_pade_guess(extrapolate, order)
+    d = ctx.vec(psi)
+    for s in range(nsteps):
+        ctx.step_pade22(d, h, (0.0, 0.0, 0.04 * math.sin(0.2 * (s + 0.5) * h)), rtol=1e-14)
+    cnt, hist = ctx.pade_history()
+    assert hist.shape == (4, order, ctx.N)
+    assert list(cnt[:2]) == ([min(order, nsteps)] * 2 if extrapolate else [0, 0])
+    err = np.linalg.norm(d.download() - ref) / np.linalg.norm(ref)
+    assert err < 1e-11, err
+    ctx.close()
