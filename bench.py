@@ -4,23 +4,30 @@ learned mixed-precision preconditioner against plain FP64 multigrid.
 Workload (the 3D soft-core configuration): V = -1/sqrt(r^2+2) on [-40,40]^3,
 128 uniform elements per axis, p = 4, Q = 8 Gauss points, a quadratic CAP in
 the outer 8 a.u. (strength 0.01), a z-polarised sin^2 pulse E0 = 0.05,
-omega = 0.057, 4 cycles; CN dt = 0.02; FGMRES to ||r||/||b|| < 1e-10. The
-initial state is the lowest eigenstate of H, computed on the GPU at set-up. A
-"step" is one CN step; inputs are resident in HBM.
+omega = 0.057, 4 cycles (T = 440.9 a.u.); CN dt = 0.02; FGMRES to
+||r||/||b|| < 1e-10. The initial state is the lowest eigenstate of H, computed on
+the GPU at set-up. A "step" is one CN step; inputs are resident in HBM.
 
-Default run: the same steps from the same state twice, first with plain FP64
-geometric multigrid (the baseline), then with the learned mixed-precision
-preconditioner (network trained at set-up on vectors harvested from two CN
-steps; set-up time is not in the timed region). value = steps/s of the learned
-run; the baseline's steps/s, the speed-up and the L2 difference of the two final
-wavefunctions go into the same line. Prints one JSON line (see the driver
-contract). The CPU baseline times the CPU oracle on one CN step of the same
-problem (bounded sample) on this host.
+Default run (--workload atom128): two timing windows, each timed for every leg from
+the same state, median of --repeats runs of W untimed + K timed steps:
+  * "early": from the eigenstate at t = 0 (field about 1e-8);
+  * "peak": centred on the field crest nearest the envelope's peak (t = 3.5 pi/omega =
+    192.9 a.u., |E| = 0.048), reached by propagating the eigenstate there untimed with
+    the learned preconditioner.
+Legs: plain FP64 geometric multigrid (the baseline), the learned mixed-precision
+preconditioner (network trained at set-up on vectors of two CN steps) and the same
+solve without the network. value = the learned leg's median steps/s in the peak
+window. --workload fullpulse propagates the whole pulse (22,046 steps) with FP64 GMG,
+complex64 GMG and the learned preconditioner and reports steps/s over the run and,
+per pulse cycle, Krylov iterations and the wavefunction difference against the FP64
+run. The CPU baseline times the CPU oracle (C++/OpenMP, -march=native build made on
+this host) on 3 CN steps of the peak window (bounded sample).
 """
 import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -31,6 +38,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
 PEAK_HBM = 8.0e12
+DT = 0.02
+OMEGA, NCYC, E0Z = 0.057, 4.0, 0.05
+T_PULSE = NCYC * 2 * math.pi / OMEGA
+T_CREST = 3.5 * math.pi / OMEGA   # the field crest nearest T/2 (the envelope's peak is a zero of the carrier)
 
 
 def problem(nel=128):
@@ -38,7 +49,7 @@ def problem(nel=128):
                 pot_kind=2, pot=(1.0, 2.0), cap_kind=1, cap=(0.01, 8.0), mg_levels=3, mg_smooth=1)
 
 
-PCS = {"sep": "fp64 separable (fast diagonalisation)", "mg": "plain fp64 GMG",
+PCS = {"sep": "fp64 separable (fast diagonalisation)", "mg": "plain fp64 GMG (exact Galerkin coarse levels)",
        "sep32": "separable solve in complex64", "mg32": "mixed-precision GMG (complex64 V-cycle)",
        "corr32": "complex64 first-order corrected separable solve (lumped potential)",
        "corrx32": "complex64 first-order corrected separable solve (matrix-free potential)",
@@ -52,27 +63,31 @@ def pot_flops_per_element(p, Q):
     return 2 * 4 * (n ** 3 * Q + n ** 2 * Q ** 2 + n * Q ** 3) + 2 * Q ** 3
 
 
+def log(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--pc", default="learned32", choices=list(PCS))
-    ap.add_argument("--no-baseline", action="store_true", help="skip the plain FP64 GMG run")
+    ap.add_argument("--steps", type=int, default=100, help="timed CN steps per window and repeat")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps before each timed run (predictor history)")
+    ap.add_argument("--repeats", type=int, default=3, help="timed runs per leg and window (median reported)")
+    ap.add_argument("--windows", default="early,peak", help="comma list of early, peak")
     ap.add_argument("--train-iters", type=int, default=150)
     ap.add_argument("--nel", type=int, default=128)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--predictor", type=int, default=2, choices=[0, 1, 2],
-                    help="initial guess of the CN solves, both legs: 0 the current state (as TSTHETA), "
-                         "1/2 the binomial extrapolation of the last 2/3 states")
-    ap.add_argument("--no-plain-guess", action="store_true",
-                    help="skip the comparison runs that start each solve from the current state")
-    ap.add_argument("--init", default="ground", choices=["ground", "gaussian"],
-                    help="initial state: lowest eigenstate (default) or the projected Gaussian (profiling runs)")
-    ap.add_argument("--workload", default="atom128", choices=["atom128", "scan", "hetero"],
-                    help="atom128: the 3D soft-core CN benchmark (default); scan: the batched 2D intensity/CEP "
-                         "scan; hetero: the layered heterostructure with Magnus-4")
+    ap.add_argument("--predictor", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="initial guess of the CN solves of the value's runs, every leg: 0 the current state "
+                         "(TSTHETA's, the reference's configuration), k > 0 the binomial extrapolation of the "
+                         "last k+1 states")
+    ap.add_argument("--alt-predictor", type=int, default=2, choices=[0, 1, 2, 3, 4],
+                    help="guess of the second set of runs reported beside the value (-1 to skip)")
+    ap.add_argument("--workload", default="atom128", choices=["atom128", "fullpulse", "scan", "hetero"],
+                    help="atom128: the 3D soft-core CN benchmark (default); fullpulse: the whole pulse with "
+                         "per-cycle reports; scan: the batched 2D intensity/CEP scan; hetero: the layered "
+                         "heterostructure with Magnus-4")
     ap.add_argument("--runs", type=int, default=64, help="scan: number of runs")
     ap.add_argument("--seed", type=int, default=7, help="scan: seed of the drawn grids and pulses")
     ap.add_argument("--kiters", type=int, default=None,
@@ -81,118 +96,156 @@ def main():
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-
     if args.workload == "scan":
         return scan(args)
     if args.workload == "hetero":
         return hetero(args)
-    import tdsez_amd as T
-    desc = problem(args.nel)
-    t_setup = time.perf_counter()
-    ctx = T.Context(desc)
-    kinds = {"sep": T.PC_SEPARABLE, "mg": T.PC_MG, "none": T.PC_NONE, "sep32": T.PC_SEPARABLE_MIXED,
-             "mg32": T.PC_MG_MIXED, "corr32": T.PC_CORRECTED, "corrx32": T.PC_CORRECTED_EXACT,
-             "learned32": T.PC_LEARNED}
-    ctx.pc_setup(T.PC_MG if ctx.levels > 1 else T.PC_SEPARABLE)
-    psi = ctx.vec()
-    ctx.project_gaussian(psi, (0.0, 0.0, 0.0), (1.5, 1.5, 1.5))
-    E0, res0, it0 = float("nan"), float("nan"), 0
-    if args.init == "ground":
-        E0, res0, it0 = ctx.ground_state(psi, sigma=-0.3, tol=1e-8, maxit=80)
-    print(f"# ground state E0 = {E0:.10f} (eigen-residual {res0:.1e}, {it0} inverse iterations)",
-          file=sys.stderr, flush=True)
-    pulse = T.make_pulse(E0=(0.0, 0.0, 0.05), omega=0.057, ncycles=4.0)
-    dt = 0.02
-    x0 = psi.download()
+    return atom(args)
 
-    def run(kind, timing, order=None):
-        """W untimed + K timed CN steps from x0; returns (steps/s, its, final psi, kernel ms, launches)."""
-        ctx.pc_setup(kind)
-        ctx.set_cn_predictor(args.predictor if order is None else order)
-        d = ctx.vec(x0)
-        ctx.evolve_cn(d, 0.0, dt, args.warmup, pulse, rtol=args.rtol, maxit=500)
+
+class Atom:
+    """The 128^3 soft-core context with its eigenstate and the trained network."""
+
+    def __init__(self, args):
+        import tdsez_amd as T
+        self.T, self.args = T, args
+        self.desc = problem(args.nel)
+        t0 = time.perf_counter()
+        self.ctx = ctx = T.Context(self.desc)
+        ctx.pc_setup(T.PC_MG)
+        psi = ctx.vec()
+        ctx.project_gaussian(psi, (0.0, 0.0, 0.0), (1.5, 1.5, 1.5))
+        self.E0, res0, it0 = ctx.ground_state(psi, sigma=-0.3, tol=1e-8, maxit=80)
+        log(f"ground state E0 = {self.E0:.10f} (eigen-residual {res0:.1e}, {it0} inverse iterations)")
+        self.pulse = T.make_pulse(E0=(0.0, 0.0, E0Z), omega=OMEGA, ncycles=NCYC)
+        self.x0 = psi.download()
+        tl = time.perf_counter()
+        l0, l1 = ctx.learn_train(psi, 0.0, DT, self.pulse, harvest_steps=2, max_samples=6, iters=args.train_iters,
+                                 lr=3e-3, rtol=args.rtol)
+        base_its, _, _ = ctx.learn_status()
+        self.learn = {"loss_physics_init": l0, "loss_trained": l1, "train_s": time.perf_counter() - tl,
+                      "its_without_network": base_its}
+        log(f"trained: loss/sample {l0:.3e} -> {l1:.3e} in {self.learn['train_s']:.1f} s")
+        psi.free()
+        self.setup_s = time.perf_counter() - t0
+        self.kinds = {"learned": T.PC_LEARNED, "fp64_mg": T.PC_MG, "network_free": T.PC_CORRECTED,
+                      "mixed_mg": T.PC_MG_MIXED}
+
+    def run(self, leg, x, t0, nsteps, warmup, order, timing=False):
+        """warmup untimed + nsteps timed CN steps from state x at t0; returns
+        (steps/s, its, final state, potential-kernel ms, launches)."""
+        ctx = self.ctx
+        ctx.pc_setup(self.kinds[leg])
+        ctx.set_cn_predictor(order)
+        d = ctx.vec(x)
+        if warmup:
+            ctx.evolve_cn(d, t0, DT, warmup, self.pulse, rtol=self.args.rtol, maxit=500)
         ctx.sync()
         if timing:
             ctx.timing(True)
-        t0 = time.perf_counter()
-        its, _ = ctx.evolve_cn(d, args.warmup * dt, dt, args.steps, pulse, rtol=args.rtol, maxit=500)
+        t1 = time.perf_counter()
+        its, _ = ctx.evolve_cn(d, t0 + warmup * DT, DT, nsteps, self.pulse, rtol=self.args.rtol, maxit=500)
         ctx.sync()
-        el = time.perf_counter() - t0
+        el = time.perf_counter() - t1
         kms, nl = ctx.timing_stats() if timing else (0.0, 0)
         ctx.timing(False)
         out = d.download()
         d.free()
-        return args.steps / el, its, out, kms, nl
+        return nsteps / el, its, out, kms, nl
 
-    base = None
-    if args.pc == "learned32" and not args.no_baseline:
-        sps_b, its_b, psi_b, _, _ = run(T.PC_MG, False)
-        base = {"preconditioner": PCS["mg"], "steps_per_s": sps_b, "ms_per_step": 1e3 / sps_b,
-                "krylov_its_per_step": float(np.mean(its_b))}
-        print(f"# baseline FP64 GMG: {sps_b:.2f} steps/s, its/step {its_b.tolist()}", file=sys.stderr, flush=True)
-    learn = None
-    if args.pc == "learned32":
-        tl = time.perf_counter()
-        l0, l1 = ctx.learn_train(psi, 0.0, dt, pulse, harvest_steps=2, max_samples=6, iters=args.train_iters,
-                                 lr=3e-3, rtol=args.rtol)
-        base_its, _, _ = ctx.learn_status()
-        learn = {"loss_physics_init": l0, "loss_trained": l1, "train_s": time.perf_counter() - tl,
-                 "its_without_network": base_its}
-        print(f"# trained: loss/sample {l0:.3e} -> {l1:.3e} in {learn['train_s']:.1f} s", file=sys.stderr, flush=True)
-    t_setup = time.perf_counter() - t_setup
-    print(f"# setup {t_setup:.2f} s, N = {ctx.N}, levels = {ctx.levels}, pc = {args.pc}", file=sys.stderr, flush=True)
+    def mnorm(self, x):
+        T, ctx = self.T, self.ctx
+        dx, dy = ctx.vec(x), ctx.vec()
+        ctx.term(T.TERM_M, dx, dy)
+        v = float(np.sqrt(abs(np.vdot(x, dy.download()).real)))
+        dx.free()
+        dy.free()
+        return v
 
-    steps_per_s, its, psi_end, kms, nlaunch = run(kinds[args.pc], True)
-    elapsed = args.steps / steps_per_s
-    print(f"# timed {args.steps} steps in {elapsed:.3f} s; Krylov its/step {its.tolist()}", file=sys.stderr,
-          flush=True)
-    if learn is not None:
-        _, fallbacks, active = ctx.learn_status()
-        learn.update({"fallbacks": fallbacks, "network_active": bool(active)})
-    if base is not None:
-        def mnorm(x):
-            dx, dy = ctx.vec(x), ctx.vec()
-            ctx.term(T.TERM_M, dx, dy)
-            v = float(np.sqrt(abs(np.vdot(x, dy.download()).real)))
-            dx.free()
-            dy.free()
-            return v
-        base["speedup"] = steps_per_s / base["steps_per_s"]
-        base["psi_rel_l2_diff"] = mnorm(psi_end - psi_b) / mnorm(psi_b)
+    def propagate(self, leg, x, t0, nsteps, chunk=1000, on_chunk=None, order=2):
+        """Untimed propagation (progress on stderr); returns the state after nsteps."""
+        ctx = self.ctx
+        ctx.pc_setup(self.kinds[leg])
+        ctx.set_cn_predictor(order)
+        d = ctx.vec(x)
+        done, its_all = 0, []
+        while done < nsteps:
+            k = min(chunk, nsteps - done)
+            its, _ = ctx.evolve_cn(d, t0 + done * DT, DT, k, self.pulse, rtol=self.args.rtol, maxit=500)
+            its_all.append(its)
+            done += k
+            if on_chunk:
+                on_chunk(done, d)
+            log(f"{leg}: {done}/{nsteps} steps, t = {t0 + done * DT:.2f}, mean its {np.mean(its):.2f}")
+        out = d.download()
+        d.free()
+        return out, np.concatenate(its_all) if its_all else np.zeros(0, np.int32)
 
-    netfree = None
-    if args.pc == "learned32" and not args.no_baseline:
-        # the same solve without the network remainder (first-order corrected separable
-        # solve with the lumped potential): what the network adds at this tolerance
-        sps_c, its_c, _, _, _ = run(T.PC_CORRECTED, False)
-        netfree = {"preconditioner": PCS["corr32"], "steps_per_s": sps_c, "krylov_its_per_step": float(np.mean(its_c))}
-        print(f"# network-free corrected solve: {sps_c:.2f} steps/s, its/step {np.mean(its_c):.1f}",
-              file=sys.stderr, flush=True)
 
-    plain = None
-    if args.predictor > 0 and not args.no_plain_guess:   # the same legs with the current state as guess
-        sps0, its0, _, _, _ = run(kinds[args.pc], False, 0)
-        plain = {"steps_per_s": sps0, "krylov_its_per_step": float(np.mean(its0))}
-        if netfree is not None:
-            sps0c, its0c, _, _, _ = run(T.PC_CORRECTED, False, 0)
-            plain.update({"network_free_steps_per_s": sps0c, "network_free_krylov_its_per_step": float(np.mean(its0c))})
-        if base is not None:
-            sps0b, its0b, _, _, _ = run(T.PC_MG, False, 0)
-            plain.update({"fp64_mg_steps_per_s": sps0b, "fp64_mg_krylov_its_per_step": float(np.mean(its0b)),
-                          "speedup": sps0 / sps0b})
-        print(f"# without predictor: {sps0:.2f} steps/s (its {np.mean(its0):.1f})"
-              + (f", FP64 GMG {plain['fp64_mg_steps_per_s']:.2f} steps/s" if base is not None else ""),
-              file=sys.stderr, flush=True)
+def atom(args):
+    if args.workload == "fullpulse":
+        return fullpulse(args)
+    A = Atom(args)
+    T, ctx = A.T, A.ctx
+    K, W, R = args.steps, args.warmup, args.repeats
+    windows = [w for w in args.windows.split(",") if w]
+    roof = None
+    states = {}
+    for win in windows:
+        if win == "early":
+            states[win] = (A.x0, 0.0)
+        else:
+            # the timed steps are centred on the field crest nearest the envelope's peak T/2
+            # (omega t = 3.5 pi: |E| = 0.048)
+            npre = int(round(T_CREST / DT)) - W - K // 2
+            t1 = time.perf_counter()
+            xw, _ = A.propagate("learned", A.x0, 0.0, npre, chunk=2000, order=2)
+            states[win] = (xw, npre * DT)
+            log(f"reached t = {npre * DT:.2f} in {time.perf_counter() - t1:.1f} s")
+    guesses = [args.predictor] + ([args.alt_predictor] if args.alt_predictor >= 0 and
+                                  args.alt_predictor != args.predictor else [])
+    res = {}
+    for order in guesses:
+        key = "current_state" if order == 0 else f"extrapolation_order_{order}"
+        res[key] = {}
+        for win in windows:
+            xw, tw = states[win]
+            r = {"t_start": tw + W * DT, "field_at_start": float(ctx.coupling(A.pulse, tw + W * DT)[2])}
+            finals = {}
+            for leg in ("fp64_mg", "learned", "network_free"):
+                sps, itsm = [], []
+                for rep in range(R):
+                    timing = leg == "learned" and rep == 0 and win == windows[-1] and order == args.predictor
+                    v, its, out, kms, nl = A.run(leg, xw, tw, K, W, order, timing)
+                    sps.append(v)
+                    itsm.append(float(np.mean(its)))
+                    if rep == 0:
+                        finals[leg] = out
+                    if timing:
+                        roof = (kms, nl)
+                r[leg] = {"steps_per_s": float(np.median(sps)), "runs_steps_per_s": sps,
+                          "krylov_its_per_step": float(np.mean(itsm))}
+                log(f"guess {order}, {win}: {leg} {np.median(sps):.2f} steps/s (runs {['%.1f' % v for v in sps]}), "
+                    f"its/step {np.mean(itsm):.2f}")
+            r["speedup_vs_fp64_mg"] = r["learned"]["steps_per_s"] / r["fp64_mg"]["steps_per_s"]
+            r["psi_rel_l2_diff_learned_vs_fp64"] = A.mnorm(finals["learned"] - finals["fp64_mg"]) / \
+                A.mnorm(finals["fp64_mg"])
+            res[key][win] = r
+    main_key = "current_state" if args.predictor == 0 else f"extrapolation_order_{args.predictor}"
+    main_win = windows[-1]
+    sps = res[main_key][main_win]["learned"]["steps_per_s"]
+    _, fallbacks, active = ctx.learn_status()
+    A.learn.update({"fallbacks": fallbacks, "network_active": bool(active)})
 
-    p, Q, nel = desc["p"], desc["nquad"], args.nel
+    p, Q, nel = A.desc["p"], A.desc["nquad"], args.nel
     flops = pot_flops_per_element(p, Q) * nel ** 3
+    kms, nlaunch = roof if roof else (0.0, 0)
     avg_s = (kms / 1e3) / max(nlaunch, 1)
     achieved = flops / avg_s if nlaunch else 0.0
-    roofline = {"bound": "mfma", "kernel": "pot3d_kernel", "achieved": achieved / 1e12,
-                "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
-                "launches": nlaunch, "avg_ms": avg_s * 1e3, "flops_per_launch": flops}
-    # HBM bytes per launch from the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
-    # (copy-kernel calibrated); only meaningful for the 128^3 workload they were taken on
+    roofline = {"bound": "mfma", "compute_unit": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6)",
+                "kernel": "pot3d_kernel", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64, "traffic": None, "launches": nlaunch, "avg_ms": avg_s * 1e3,
+                "flops_per_launch": flops}
     tj = os.path.join(ROOT, "profiles", "pot3d_softcore128_traffic.json")
     if nel == 128 and os.path.exists(tj):
         with open(tj) as f:
@@ -202,29 +255,92 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(desc, dt, pulse, args, x0)
+        xw, tw = states[main_win]
+        cpu = cpu_baseline(A, xw, tw + W * DT, args)
 
     line = {
         "metric": "CN steps/s, 3D p=4 128^3 grid: learned+mixed PC vs plain FP64 MG at |r|<1e-10",
-        "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "complex128 (fp64 operator/Krylov; complex64 + bf16 preconditioner)"
-        if args.pc in ("learned32", "sep32", "mg32", "corr32", "corrx32") else "complex128 (fp64)", "data": "synthetic: lowest eigenstate of H (GPU inverse iteration), E0 = %.8f" % E0,
-        "config": {"workload": f"3D soft-core atom {nel}^3 elements p=4 Q=8, CN dt=0.02, rtol={args.rtol}",
-                   "preconditioner": PCS[args.pc],
-                   "cn_guess": ("current state" if args.predictor == 0 else
-                                f"binomial extrapolation of the last {args.predictor + 1} states (both legs)"),
-                   "dofs": ctx.N, "mg_levels": ctx.levels, "krylov_its_per_step": float(np.mean(its)),
-                   "setup_s": t_setup},
-        "fp64_mg_baseline": base,
-        "without_predictor": plain,
-        "learned": learn,
-        "network_free": netfree,
+        "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": 1e3 / sps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex128 (fp64 operator/Krylov; complex64 + bf16 preconditioner)",
+        "data": "synthetic: lowest eigenstate of H (GPU inverse iteration), E0 = %.8f" % A.E0,
+        "config": {"workload": f"3D soft-core atom {nel}^3 elements p=4 Q=8, CN dt={DT}, rtol={args.rtol}, "
+                               f"{main_win} window (median of {R} runs of {W}+{K} steps)",
+                   "preconditioner": PCS["learned32"], "baseline_preconditioner": PCS["mg"],
+                   "cn_guess": ("current state (as TSTHETA)" if args.predictor == 0 else
+                                f"binomial extrapolation of the last {args.predictor + 1} states (every leg)"),
+                   "dofs": ctx.N, "mg_levels": ctx.levels, "setup_s": A.setup_s},
+        "speedup_vs_fp64_mg": res[main_key][main_win]["speedup_vs_fp64_mg"],
+        "runs": res,
+        "learned": A.learn,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def fullpulse(args):
+    """The whole pulse with FP64 GMG, complex64 GMG and the learned preconditioner: steps/s
+    over the run (steps / propagation time, as src/timing.cpp:229-233) and, after every
+    pulse cycle, Krylov iterations per step and the wavefunction's L2 difference from the
+    FP64 GMG run."""
+    A = Atom(args)
+    nsteps = int(math.ceil(T_PULSE / DT - 1e-9))
+    cyc = [int(round(nsteps * (c + 1) / NCYC)) for c in range(int(NCYC))]
+    legs = {}
+    ref_states = {}
+    order = args.predictor if args.predictor > 0 else max(args.alt_predictor, 0)
+    for leg in ("fp64_mg", "mixed_mg", "learned"):
+        ctx = A.ctx
+        ctx.pc_setup(A.kinds[leg])
+        ctx.set_cn_predictor(order)
+        d = ctx.vec(A.x0)
+        done, per = 0, []
+        t_run = 0.0
+        for c, end in enumerate(cyc):
+            ctx.sync()
+            t1 = time.perf_counter()
+            its_c = []
+            while done < end:
+                k = min(1000, end - done)
+                its, _ = ctx.evolve_cn(d, done * DT, DT, k, A.pulse, rtol=args.rtol, maxit=500)
+                its_c.append(its)
+                done += k
+            ctx.sync()
+            el = time.perf_counter() - t1
+            t_run += el
+            its_c = np.concatenate(its_c)
+            x = d.download()
+            entry = {"cycle": c + 1, "t": done * DT, "steps": int(its_c.size), "seconds": el,
+                     "krylov_its_per_step": float(its_c.mean()), "krylov_its_max": int(its_c.max())}
+            if leg == "fp64_mg":
+                ref_states[c] = x
+            else:
+                entry["psi_rel_l2_diff_vs_fp64"] = A.mnorm(x - ref_states[c]) / A.mnorm(ref_states[c])
+            per.append(entry)
+            log(f"{leg}: cycle {c + 1} done at t = {done * DT:.2f}: {entry}")
+        d.free()
+        legs[leg] = {"steps_per_s": nsteps / t_run, "seconds": t_run, "cycles": per}
+    _, fallbacks, active = A.ctx.learn_status()
+    A.learn.update({"fallbacks": fallbacks, "network_active": bool(active)})
+    sps = legs["learned"]["steps_per_s"]
+    line = {
+        "metric": "CN steps/s over the whole pulse, 3D p=4 128^3: learned+mixed PC vs plain FP64 MG at |r|<1e-10",
+        "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": nsteps, "warmup": 0, "ms_per_step": 1e3 / sps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex128 (fp64 operator/Krylov; complex64 + bf16 preconditioner)",
+        "data": "synthetic: lowest eigenstate of H (GPU inverse iteration), E0 = %.8f" % A.E0,
+        "config": {"workload": f"3D soft-core atom {args.nel}^3 p=4 Q=8, the whole {NCYC:.0f}-cycle pulse "
+                               f"({nsteps} CN steps of dt={DT}), rtol={args.rtol}",
+                   "cn_guess": ("current state (as TSTHETA)" if order == 0 else
+                                f"binomial extrapolation of the last {order + 1} states (every leg)"),
+                   "setup_s": A.setup_s},
+        "speedup_vs_fp64_mg": sps / legs["fp64_mg"]["steps_per_s"],
+        "legs": legs, "learned": A.learn, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    A.ctx.close()
 
 
 def scan_runs(nruns, seed):
@@ -250,12 +366,13 @@ def scan(args):
     dt = 0.02
     pulses = [T.make_pulse(E0=(E0, 0.0, 0.0), omega=0.057, ncycles=3.0, cep=cep) for _, E0, cep in runs]
     t0 = time.perf_counter()
+    pred = min(args.predictor if args.predictor > 0 else max(args.alt_predictor, 0), 2)
     if args.kiters is None:
-        args.kiters = 2 if args.predictor else 4
+        args.kiters = 2 if pred else 4
     b = T.Batch([d for d, _, _ in runs], pulses, dt, args.warmup + args.steps, pc=T.PC_SEPARABLE_MIXED,
                 kiters=args.kiters)
-    if args.predictor:
-        b.set_predictor(args.predictor)
+    if pred:
+        b.set_predictor(pred)
     for r, ctx in enumerate(b.runs):
         ctx.project_gaussian(b.psi[r], (0.0, 0.0), (1.0, 1.0))
         ctx.ground_state(b.psi[r], sigma=-0.9, tol=1e-8, maxit=60)
@@ -274,8 +391,8 @@ def scan(args):
         "data": f"synthetic: {args.runs} runs drawn from seed {args.seed}, ground states at set-up",
         "config": {"workload": f"batched 2D scan, {args.runs} runs, {dofs} dofs in total, CN dt=0.02, "
                                f"rtol={args.rtol}, {args.kiters} captured FGMRES iterations per step",
-                   "cn_guess": ("current state" if args.predictor == 0 else
-                                f"binomial extrapolation of the last {args.predictor + 1} states"),
+                   "cn_guess": ("current state" if pred == 0 else
+                                f"binomial extrapolation of the last {pred + 1} states"),
                    "setup_s": setup, "krylov_its_per_step": float(its.mean()),
                    "host_completions": b.fixups() - fix0, "host_completions_warmup": fix0,
                    "max_residual": float(res.max())},
@@ -356,23 +473,28 @@ def hetero(args):
     ctx.close()
 
 
-def cpu_baseline(desc, dt, pulse, args, psi):
-    """Time the CPU oracle on one CN step of the same problem (bounded sample)."""
+def cpu_baseline(A, x, t, args, nsteps=3):
+    """The CPU oracle (C++/OpenMP, the reference algorithm restated) on nsteps CN steps of the
+    same problem from the state of the timed window (bounded sample), built with
+    -march=native on this host when the compiler is there (portable build otherwise)."""
     try:
+        r = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "native"], capture_output=True, text=True,
+                           timeout=300)
         import oracle
+        native = r.returncode == 0 and oracle.use_native()
         from oracle import Oracle
     except Exception as e:  # noqa: BLE001
         return {"error": f"oracle unavailable: {e}"}
-    o = Oracle(desc)
-    t = args.warmup * dt + 0.5 * dt
-    E = 0.05 * math.sin(math.pi * t / (4 * 2 * math.pi / 0.057)) ** 2 * math.sin(0.057 * t)
-    sE = np.array([[0.0, 0.0, E]])
+    o = Oracle(A.desc)
+    sE = np.array([A.ctx.coupling(A.pulse, t + (s + 0.5) * DT) for s in range(nsteps)])
     t0 = time.perf_counter()
-    _, its, _ = o.cn_steps(psi, dt, sE, rtol=args.rtol, maxit=500, restart=50, pc=1)
+    _, its, _ = o.cn_steps(x, DT, sE, rtol=args.rtol, maxit=500, restart=50, pc=1)
     el = time.perf_counter() - t0
-    return {"value": 1.0 / el, "unit": "steps/s", "cores": oracle.lib().orc_threads(), "kind": "port",
-            "sample": f"1 CN step of the same {desc['nelem'][0]}^3 problem, FGMRES + separable PC "
-                      f"({int(its[0])} its), {el:.1f} s"}
+    return {"value": nsteps / el, "unit": "steps/s", "cores": oracle.lib().orc_threads(), "kind": "port",
+            "sample": f"{nsteps} CN steps of the same {A.desc['nelem'][0]}^3 problem from the timed window's start "
+                      f"state (t = {t:.2f}), FGMRES(50) + separable fast-diagonalisation preconditioner "
+                      f"({its.tolist()} its), {el:.1f} s, "
+                      + ("-march=native build" if native else "portable x86-64-v3 build")}
 
 
 if __name__ == "__main__":

@@ -146,6 +146,11 @@ int tdz_level_apply(tdz_ctx *ctx, int level, const double *x, double *y);
  * max_its iterations (max_its < 0: off, the default). Rebuilds are counted in
  * tdz_ksp_stats.pc_rebuilds. */
 int tdz_set_pc_refresh(tdz_ctx *ctx, int max_its);
+/* Correction sweeps of TDZ_PC_CORRECTED* and TDZ_PC_LEARNED (no reference counterpart):
+ * z <- S^-1 (r - dA~ z) applied sweeps times after z = S^-1 r (1..4, default 1), dA~ the
+ * terms the separable solve S leaves out (CAP and field exactly, the potential lumped,
+ * plus the network's remainder for TDZ_PC_LEARNED). */
+int tdz_pc_set_corrections(tdz_ctx *ctx, int sweeps);
 /* Host utilities of the multigrid set-up (no device work): one coarsening of a knot
  * vector U[m] into C (at most m values; *mc = its length, 0 when nothing can be removed;
  * src/multigrid.cpp:25-40), and the dense knot-insertion matrix R [nf][nc] (row-major,
@@ -168,12 +173,18 @@ int tdz_step_cn(tdz_ctx *ctx, double *psi, double dt, const double sE_mid[3], do
 int tdz_evolve_cn(tdz_ctx *ctx, double *psi, double t0, double dt, int nsteps, const tdz_pulse *pulse,
                   double rtol, int maxit, double *obs, int *its);
 /* Initial guess of the CN solves of tdz_evolve_cn: order 0 (default) starts each
- * solve from the current state, as TSTHETA does (src/propagator.cpp:149); order 1 or
- * 2 from the binomial extrapolation of the last 2 or 3 states (the guess of the
- * stage solves, src/propagator.cpp:629-638), whose residual the library forms
- * without an operator apply. Every solve still meets rtol on the true residual; the
- * history carries over to the next tdz_evolve_cn call when psi is unchanged. */
+ * solve from the current state, as TSTHETA does (src/propagator.cpp:149); order
+ * k = 1..4 from the binomial extrapolation of the last k+1 states (the guess of the
+ * stage solves, src/propagator.cpp:629-638), whose residual the library forms from
+ * stored products instead of an operator apply. Every solve still meets rtol on its
+ * residual; the history carries over to the next tdz_evolve_cn call when psi is
+ * unchanged. */
 int tdz_set_cn_predictor(tdz_ctx *ctx, int order);
+/* The predictor's stored products are recomputed exactly every sync_every steps (an
+ * operator apply per kept state): carried from solve to solve their rounding would grow
+ * like n^(order + 1/2). last_drift is the relative inconsistency found at the last
+ * recomputation; the interval adapts to keep it between 1e-3 and 1e-2 of rtol. */
+int tdz_cn_predictor_status(const tdz_ctx *ctx, int *sync_every, double *last_drift);
 /* Initial states. tdz_project_gaussian: L2 projection of the packet
  * prod_a (2 pi w_a^2)^(-1/4) exp(-(x_a-c_a)^2/(4 w_a^2) + i k_a x_a), normalised so
  * psi^H M psi = 1 (src/core.cpp:1188-1262). tdz_ground_state: lowest eigenstate of
@@ -239,6 +250,10 @@ int tdz_pulse_coupling(const tdz_ctx *ctx, const tdz_pulse *pulse, double t, dou
  * out_f32 and out_mfma [2N] re then im). */
 int tdz_learn_train(tdz_ctx *ctx, const double *psi, double t0, double dt, const tdz_pulse *pulse, int harvest_steps,
                     int max_samples, int iters, double lr, double rtol, double *loss_before, double *loss_after);
+/* Training set of tdz_learn_train: besides the harvested pairs, random_samples synthetic
+ * inputs y = S^-1 w (and S^-1 M S^-1 w) from random fields w drawn from seed, which cover
+ * the whole grid (default 6, seed 12345). */
+int tdz_learn_config(tdz_ctx *ctx, int random_samples, unsigned long long seed);
 int tdz_learn_weights(tdz_ctx *ctx, float *W1, float *W2, int set);
 int tdz_learn_fields(tdz_ctx *ctx, float *P);
 int tdz_learn_features(tdz_ctx *ctx, const double *y, double *u);

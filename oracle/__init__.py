@@ -14,6 +14,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# -march=native build of the same source, made on the host that runs it (make -C oracle native)
+NATIVE_PATH = os.path.join(_HERE, "_native", "liboracle.so")
 
 
 class Params(C.Structure):
@@ -68,6 +70,16 @@ def make_params(desc):
 
 
 _lib = None
+
+
+def use_native():
+    """Load the -march=native build (built on this host) instead of the portable one; must
+    come before the first use of the library. Returns False when it is not built."""
+    global LIB_PATH
+    if _lib is not None or not os.path.exists(NATIVE_PATH):
+        return False
+    LIB_PATH = NATIVE_PATH
+    return True
 
 
 def lib():

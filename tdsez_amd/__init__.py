@@ -35,7 +35,7 @@ EXPORTS = [
     "tdz_trace_row", "tdz_pade_history", "tdz_batch_create", "tdz_batch_context", "tdz_batch_evolve",
     "tdz_batch_fixups", "tdz_batch_destroy", "tdz_ifunction", "tdz_level_dims", "tdz_mg_transfer",
     "tdz_level_apply", "tdz_set_pc_refresh", "tdz_coarsen_knots", "tdz_refinement_matrix", "tdz_last_warning",
-    "tdz_set_pade_guess", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
+    "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
 ]
 
 
@@ -173,6 +173,9 @@ def lib():
         L.tdz_refinement_matrix.argtypes = [dp, i, dp, i, i, dp]
         L.tdz_last_warning.restype = C.c_char_p
         L.tdz_set_pade_guess.argtypes = [vp, i, i]
+        L.tdz_pc_set_corrections.argtypes = [vp, i]
+        L.tdz_learn_config.argtypes = [vp, i, C.c_ulonglong]
+        L.tdz_cn_predictor_status.argtypes = [vp, ip, dp]
         _lib = L
     return _lib
 
@@ -312,12 +315,24 @@ class Context:
     def level_apply(self, level, x, y):
         _chk(lib().tdz_level_apply(self.h, level, x.ptr, y.ptr))
 
+    def learn_config(self, random_samples=6, seed=12345):
+        _chk(lib().tdz_learn_config(self.h, int(random_samples), int(seed)))
+
+    def set_corrections(self, sweeps):
+        _chk(lib().tdz_pc_set_corrections(self.h, int(sweeps)))
+
     def set_pc_refresh(self, max_its):
         _chk(lib().tdz_set_pc_refresh(self.h, int(max_its)))
 
     def set_pade_guess(self, extrapolate=True, order=3):
         _chk(lib().tdz_set_pade_guess(self.h, int(bool(extrapolate)), int(order)))
         self.guess_order = int(order)
+
+    def predictor_status(self):
+        """(interval of the exact recomputation of the stored products, drift found at the last)"""
+        k, dr = C.c_int(0), C.c_double(0)
+        _chk(lib().tdz_cn_predictor_status(self.h, C.byref(k), C.byref(dr)))
+        return k.value, dr.value
 
     def set_cn_predictor(self, order):
         """Initial guess of the CN solves: 0 the current state, 1/2 the binomial extrapolation."""

@@ -164,6 +164,8 @@ struct LearnState {
     int fallbacks = 0;
     bool fallen = false;  // the network regressed: the fallback preconditioner is in use
     int fallback_kind = TDZ_PC_MG;
+    int random_count = 6;                // synthetic samples added to the harvested ones
+    unsigned long long seed = 12345;
 };
 
 // the host weights to the device; a new version for the inference's packed operands
@@ -209,9 +211,13 @@ struct tdz_ctx {
         int count = 0, head = 0;    // states held (at most 3), slot of the newest
         double dt = 0;
         unsigned axes = 0;          // field axes whose dipole products are kept
-        dcomplex *psi[3] = {}, *P[3] = {}, *D[3][3] = {};   // state; (M + g H_sEh) state; D_a state
-        double sEh[3][3] = {};                               // the field of each slot's P
-        dcomplex *Mx[3] = {};                                // M state
+        static constexpr int kSlots = 5;   // order <= 4
+        int ring = 3;                      // slots in use: order + 1
+        int sync_every = 16, since_sync = 0;   // exact recomputation of the stored products
+        double last_drift = 0;                 // their relative drift found at the last one
+        dcomplex *psi[kSlots] = {}, *P[kSlots] = {}, *D[kSlots][3] = {};   // state; (M + g H_sEh) state; D_a state
+        double sEh[kSlots][3] = {};                                        // the field of each slot's P
+        dcomplex *Mx[kSlots] = {};                           // M state
         dcomplex *Xband[3] = {}, *Zband[3] = {};   // complex position bands and zero bands (dipole applies)
         double *Xz = nullptr;                      // the real z position band (fused mass + z dipole)
     } pred;
@@ -234,8 +240,17 @@ struct tdz_ctx {
     std::vector<double *> user_vecs;
     int refresh_base = -1, rebuilds = 0;
     int stage_base[2] = {-1, -1};   // refresh trackers of the two Pade stages
-    bool field_warned = false;   // the strong-field warning was given
-    int refresh_cap = -1;       // rebuild whenever a solve takes more iterations (-tdsez_pc_refresh_its); -1: off
+    bool field_warned = false, cap_warned = false;   // the multigrid warnings were given
+    int refresh_cap = -1;
+    int corrections = 0;        // correction sweeps of the corrected and learned preconditioners (0: adaptive)
+    // adaptive sweeps: the solve's initial relative residual and tolerance (set by FGMRES
+    // before its first preconditioner apply), the sweeps chosen for it, and the last
+    // observed first-iteration contraction per sweep count
+    double pc_r0rel = -1, pc_rtol = 0;
+    bool pc_choose = false;
+    int pc_sweeps = 1;
+    double rho_hat[5] = {-1, -1, -1, -1, -1};
+    long pc_solves = 0;       // rebuild whenever a solve takes more iterations (-tdsez_pc_refresh_its); -1: off
     bool cap_on = true;         // the CAP is part of H (off during the eigensolve, as src/propagator.cpp:51-69)
     // Pade(2,2) stage guesses: per slot (2 x stage) the last increments y - x
     dcomplex *hist = nullptr;   // [4 slots][3] vectors
@@ -950,67 +965,153 @@ tdz::FusedBands fused_bands(tdz_ctx *c)
     return fb;
 }
 
-void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, const fcomplex *n, bool exact_v = false,
-                     bool use_net = false)
+// One training sample from the complex64 vector y: the network input (y and its first
+// moments, the fused products also leave M y in the level's t) and the target
+// t = V y - Vg .* M y by the matrix-free complex64 potential kernel.
+void harvest_sample(tdz_ctx *c, const fcomplex *y)
+{
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    LearnState &ls = c->learn;
+    constexpr int CI = tdz::kCnnInputs;
+    fcomplex *tmp[7] = {c->sf.tmp[0], c->sf.tmp[1], c->sf.tmp[2], c->sf.tmp[3], c->sf.w1, c->sf.w2, c->sf.u2};
+    tdz::fused_products(c->st, fused_bands(c), ls.sin, y, tmp, L.f.t, c->sf.u1, ls.feat);
+    fcomplex *ys = track(c, dalloc<fcomplex>((size_t)CI * N)), *ts = track(c, dalloc<fcomplex>(N));
+    HIPCHK(hipMemcpyAsync(ys, ls.feat, CI * N * sizeof(fcomplex), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemsetAsync(c->sf.u2, 0, N * sizeof(fcomplex), c->st));
+    pot_apply<float>(c, L, 1.0, y, c->sf.u2);
+    tdz::lumped_residual(c->st, N, c->sf.u2, L.f.t, c->dVg, ts);
+    std::vector<fcomplex> h(N);
+    HIPCHK(hipMemcpyAsync(h.data(), ts, N * sizeof(fcomplex), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double t2 = 0;
+    for (const auto &v : h) t2 += (double)v.re * v.re + (double)v.im * v.im;
+    ls.ys.push_back(ys);
+    ls.ts.push_back(ts);
+    ls.tnorm2.push_back(t2);
+}
+
+// Synthetic training samples that cover the whole grid and every scale: y = S^-1 w for a
+// random field w (even samples: the separable solve's own spectrum, rough), or
+// y = S^-1 M S^-1 w (odd samples: smoother). The network's target is a linear map of y,
+// so inputs spread over the domain fit it where the harvested ones (near the initial
+// state) do not reach.
+void random_samples(tdz_ctx *c, int count, unsigned long long seed)
+{
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    for (int i = 0; i < count; ++i) {
+        tdz::random_field(c->st, N, seed + 7919ull * (unsigned long long)i, L.f.r);
+        fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);
+        if (i % 2) {
+            tdz::SepBandsT<float> mb = sep_bands<float>(L);
+            for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
+            tdz::launch_sep_apply<float>(c->st, mb, L.f.z, L.f.r, c->sf.tmp);
+            fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);
+        }
+        harvest_sample(c, L.f.z);
+    }
+}
+
+// Sweeps for the next solve of the corrected preconditioners: the fewest whose expected
+// contraction times the solve's initial relative residual meets half the tolerance (one
+// Krylov iteration), from the last observed contraction rho_k of each sweep count; an
+// unobserved count is estimated from an observed one as rho_j = rho_k^((j+1)/(k+1)) (the
+// error after k sweeps is (S^-1 dA)^(k+1)). When no count can make one iteration, one
+// sweep (the cheapest per iteration). Every 32nd solve tries one sweep fewer than the
+// estimates ask for, so that a weakening field is noticed.
+int choose_sweeps(tdz_ctx *c)
+{
+    constexpr int kMaxSweeps = 3;
+    if (c->pc_r0rel <= 0) return 1;
+    double est[kMaxSweeps + 1];
+    int ref = -1;
+    for (int k = 1; k <= kMaxSweeps; ++k)
+        if (c->rho_hat[k] > 0 && (ref < 0 || k < ref)) ref = k;
+    for (int k = 1; k <= kMaxSweeps; ++k) {
+        if (c->rho_hat[k] > 0) est[k] = c->rho_hat[k];
+        else if (ref > 0) est[k] = std::pow(c->rho_hat[ref], (k + 1.0) / (ref + 1.0));
+        else est[k] = std::pow(1e-2, k + 1.0);
+    }
+    const double target = 0.5 * c->pc_rtol / c->pc_r0rel;
+    int k = 1;
+    while (k <= kMaxSweeps && est[k] > target) ++k;
+    if (k > kMaxSweeps) return 1;
+    if (k > 1 && ++c->pc_solves % 32 == 0) --k;
+    return k;
+}
+
+// the current preconditioner is a corrected solve with adaptive sweeps
+bool sweeps_apply(const tdz_ctx *c)
+{
+    return c->pc_kind == TDZ_PC_CORRECTED || c->pc_kind == TDZ_PC_CORRECTED_EXACT ||
+           (c->pc_kind == TDZ_PC_LEARNED && !c->learn.fallen);
+}
+
+// Record the first Arnoldi step's contraction |g_1| / beta of a solve that used k sweeps;
+// counts below k are at least as bad as the model says.
+void observe_sweeps(tdz_ctx *c, int k, double rho)
+{
+    if (k < 1 || k > 3 || !(rho > 0)) return;
+    c->rho_hat[k] = rho;
+    for (int j = 1; j < k; ++j) c->rho_hat[j] = std::max(c->rho_hat[j], std::pow(rho, (j + 1.0) / (k + 1.0)));
+}
+
+void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = false, bool use_net = false)
 {
     Level &L = *c->lv[0];
     const long N = L.N;
     tdz::launch_demote(c->st, r, L.f.r, N);
+    fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);                // y = S^-1 r
     if (correction_bound(c) > 0.25) {   // perturbation too large: the separable solve alone
-        fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);
         tdz::launch_promote(c->st, L.f.z, z, N);
         return;
     }
-    fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);                // y
     LearnState &ls = c->learn;
-    const bool harvest = ls.harvesting && (int)ls.ys.size() < ls.max_samples && c->ph.pot_kind;
-    const bool net = use_net && ls.trained;
-    if (harvest || net) {   // M y, g (CAP + sE.D) y and the network input in three passes
-        fcomplex *tmp[7] = {c->sf.tmp[0], c->sf.tmp[1], c->sf.tmp[2], c->sf.tmp[3], c->sf.w1, c->sf.w2, c->sf.u2};
-        tdz::fused_products(c->st, fused_bands(c), ls.sin, L.f.z, tmp, L.f.t, c->sf.u1, ls.feat);
-    } else {
-        tdz::SepBandsT<float> mb = sep_bands<float>(L);
-        for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
-        tdz::launch_sep_apply<float>(c->st, mb, L.f.z, L.f.t, c->sf.tmp);       // M y
-        tdz::SepBandsT<float> cb = sep_bands<float>(L);
-        for (int d = 0; d < 3; ++d) cb.B[d] = c->dBcorr[d];
-        tdz::launch_sep_apply<float>(c->st, cb, L.f.z, c->sf.u1, c->sf.tmp);    // g (CAP + sE.D) y
+    if (c->pc_choose) {
+        c->pc_sweeps = choose_sweeps(c);
+        c->pc_choose = false;
     }
-    if (exact_v && c->ph.pot_kind) {   // potential product by the matrix-free kernel (reference for the learned one)
-        HIPCHK(hipMemsetAsync(c->sf.u2, 0, N * sizeof(fcomplex), c->st));
-        pot_apply<float>(c, L, c->g, L.f.z, c->sf.u2);
-        HIPCHK(hipMemsetAsync(L.f.t, 0, N * sizeof(fcomplex), c->st));
-        n = c->sf.u2;
+    const int sweeps = c->corrections > 0 ? c->corrections : c->pc_sweeps;
+    // z <- S^-1 (r - dA~ z), repeated: the error after k corrections is (S^-1 dA)^(k+1)
+    // plus, at first order, what the last sweep's dA~ misses of dA (the lumped potential:
+    // ~1e-4 of r at dt = 0.02; with the network's remainder far less); an earlier sweep's
+    // miss enters multiplied by S^-1 dA, so the network runs in the last sweep only
+    for (int k = 0; k < sweeps; ++k) {
+        const bool net = use_net && ls.trained && k == sweeps - 1;
+        const bool harvest = k == 0 && ls.harvesting && (int)ls.ys.size() < ls.max_samples && c->ph.pot_kind;
+        if (harvest || net) {   // M z, g (CAP + sE.D) z and the network input in three passes
+            fcomplex *tmp[7] = {c->sf.tmp[0], c->sf.tmp[1], c->sf.tmp[2], c->sf.tmp[3], c->sf.w1, c->sf.w2, c->sf.u2};
+            tdz::fused_products(c->st, fused_bands(c), ls.sin, L.f.z, tmp, L.f.t, c->sf.u1, ls.feat);
+        } else {
+            tdz::SepBandsT<float> mb = sep_bands<float>(L);
+            for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
+            tdz::launch_sep_apply<float>(c->st, mb, L.f.z, L.f.t, c->sf.tmp);       // M z
+            tdz::SepBandsT<float> cb = sep_bands<float>(L);
+            for (int d = 0; d < 3; ++d) cb.B[d] = c->dBcorr[d];
+            tdz::launch_sep_apply<float>(c->st, cb, L.f.z, c->sf.u1, c->sf.tmp);    // g (CAP + sE.D) z
+        }
+        const fcomplex *n = nullptr;
+        if (exact_v && c->ph.pot_kind) {   // potential product by the matrix-free kernel (reference for the learned one)
+            HIPCHK(hipMemsetAsync(c->sf.u2, 0, N * sizeof(fcomplex), c->st));
+            pot_apply<float>(c, L, c->g, L.f.z, c->sf.u2);
+            HIPCHK(hipMemsetAsync(L.f.t, 0, N * sizeof(fcomplex), c->st));
+            n = c->sf.u2;
+        }
+        if (harvest) harvest_sample(c, L.f.z);
+        const float *no = nullptr;   // the network's output, scaled by g / sout in the combine
+        if (net) {
+            const int n3[3] = {L.n[0], L.n[1], L.n[2]};
+            // the weights' BF16 operand packing persists in the work buffer until they change
+            tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{0.f, 0.f}, nullptr,
+                                ls.packed_ver != ls.wver);
+            ls.packed_ver = ls.wver;
+            no = ls.out;
+        }
+        tdz::launch_corr_combine(c->st, L.f.r, L.f.t, c->sf.u1, n, c->dVg, cx<float>(c->g), c->sf.u1, N, no,
+                                 cx<float>(c->g / ls.sout));
+        fd_apply<float>(c, L, c->sf.u1, L.f.z, c->sf.w1, c->sf.w2);
     }
-    if (harvest) {
-        // sample: input features of y = S^-1 r, target t = V y - Vg .* M y (matrix-free potential product)
-        constexpr int CI = tdz::kCnnInputs;
-        fcomplex *ys = track(c, dalloc<fcomplex>((size_t)CI * N)), *ts = track(c, dalloc<fcomplex>(N));
-        HIPCHK(hipMemcpyAsync(ys, ls.feat, CI * N * sizeof(fcomplex), hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(hipMemsetAsync(c->sf.u2, 0, N * sizeof(fcomplex), c->st));
-        pot_apply<float>(c, L, 1.0, L.f.z, c->sf.u2);
-        tdz::lumped_residual(c->st, N, c->sf.u2, L.f.t, c->dVg, ts);
-        std::vector<fcomplex> h(N);
-        HIPCHK(hipMemcpyAsync(h.data(), ts, N * sizeof(fcomplex), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
-        double t2 = 0;
-        for (const auto &v : h) t2 += (double)v.re * v.re + (double)v.im * v.im;
-        ls.ys.push_back(ys);
-        ls.ts.push_back(ts);
-        ls.tnorm2.push_back(t2);
-    }
-    const float *no = nullptr;   // the network's output, scaled by g / sout in the combine
-    if (net) {
-        const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-        // the weights' BF16 operand packing persists in the work buffer until they change
-        tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{0.f, 0.f}, nullptr,
-                            ls.packed_ver != ls.wver);
-        ls.packed_ver = ls.wver;
-        no = ls.out;
-    }
-    tdz::launch_corr_combine(c->st, L.f.r, L.f.t, c->sf.u1, n, c->dVg, cx<float>(c->g), c->sf.u1, N, no,
-                             cx<float>(c->g / ls.sout));
-    fd_apply<float>(c, L, c->sf.u1, L.f.z, c->sf.w1, c->sf.w2);
     tdz::launch_promote(c->st, L.f.z, z, N);
 }
 
@@ -1028,8 +1129,8 @@ void pc_apply_dev(tdz_ctx *c, const dcomplex *r, dcomplex *z)
         fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);
         tdz::launch_promote(c->st, L.f.z, z, L.N);
         break;
-    case TDZ_PC_CORRECTED: corrected_apply(c, r, z, nullptr); break;
-    case TDZ_PC_CORRECTED_EXACT: corrected_apply(c, r, z, nullptr, true); break;
+    case TDZ_PC_CORRECTED: corrected_apply(c, r, z); break;
+    case TDZ_PC_CORRECTED_EXACT: corrected_apply(c, r, z, true); break;
     case TDZ_PC_LEARNED:
         if (c->learn.fallen) {
             if (c->learn.fallback_kind == TDZ_PC_MG) {
@@ -1038,7 +1139,7 @@ void pc_apply_dev(tdz_ctx *c, const dcomplex *r, dcomplex *z)
             }
             else fd_apply<double>(c, L, r, z, c->sd.w1, c->sd.w2);
         } else {
-            corrected_apply(c, r, z, nullptr, false, true);
+            corrected_apply(c, r, z, false, true);
         }
         break;
     case TDZ_PC_MG_MIXED:
@@ -1166,6 +1267,12 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             break;
         }
         if (its >= maxit) break;
+        const bool adapt = restarts == 0 && c->corrections == 0 && sweeps_apply(c);
+        if (adapt) {   // the corrected preconditioners pick their sweeps from r0 and rtol
+            c->pc_r0rel = res;
+            c->pc_rtol = rtol;
+            c->pc_choose = true;
+        }
         ++restarts;
         tdz::launch_axpby(c->st, {1.0 / beta, 0}, Vj(0), {0, 0}, Vj(0), N);
         std::fill(gv.begin(), gv.end(), cplx(0));
@@ -1220,6 +1327,7 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             gv[j] = cs[j] * gv[j];
             ++its;
             res = std::abs(gv[j + 1]) / bnorm;
+            if (adapt && j == 0) observe_sweeps(c, c->pc_sweeps, std::abs(gv[1]) / beta);
             if (res <= rtol || hn == 0.0) { ++j; converged = true; break; }
             // V_{j+1} = (w - V h2) / ||w - V h2|| for the next step (a converged solve leaves w
             // as it is: only the residual update below reads it, with its coefficients adjusted)
@@ -1765,6 +1873,14 @@ int tdz_level_apply(tdz_ctx *c, int level, const double *x, double *y)
     });
 }
 
+int tdz_pc_set_corrections(tdz_ctx *c, int sweeps)
+{
+    return guarded([&] {
+        if (sweeps < 1 || sweeps > 4) throw Error("correction sweeps must be 1..4");
+        c->corrections = sweeps;
+    });
+}
+
 int tdz_set_pc_refresh(tdz_ctx *c, int max_its)
 {
     return guarded([&] {
@@ -1799,15 +1915,38 @@ int tdz_refinement_matrix(const double *C, int mc, const double *F, int mf, int 
 }  // extern "C"
 
 namespace {
-// Strong-field check of TDSEZSetupMultigrid (src/multigrid.cpp:213-237): once tau max|E.r|
-// exceeds 1.5 over the pulse the field term dominates M + i tau H_E and the coarse levels
-// no longer represent it. tau = dt/2 (CN), dt/sqrt(12) (Pade(2,2)), dt/(2 sqrt(12))
-// (Magnus 4, half steps). Warns once per context (stderr and tdz_last_warning).
-void strong_field_check(tdz_ctx *c, const tdz_pulse *pulse, double tau, double dt)
+// Checks of a multigrid run, each warned once per context (stderr and tdz_last_warning).
+// Strong field (TDSEZSetupMultigrid, src/multigrid.cpp:213-237): once tau max|E.r| exceeds
+// 1.5 over the pulse the field term dominates M + i tau H_E and the coarse levels no longer
+// represent it. tau = dt/2 (CN), dt/sqrt(12) (Pade(2,2)), dt/(2 sqrt(12)) (Magnus 4, half
+// steps). Dominant CAP (this build's smoother): the V-cycle smooths with the separable
+// solve of M + g K, which leaves the CAP out; where tau max W exceeds 1/2 (a profile that
+// diverges at the walls, as the Manolopoulos one does) the smoothing sweep amplifies the
+// error there and the V-cycle may stall.
+void multigrid_checks(tdz_ctx *c, const tdz_pulse *pulse, double tau, double dt)
 {
     const bool mg = c->pc_kind == TDZ_PC_MG || c->pc_kind == TDZ_PC_MG_MIXED ||
                     (c->pc_kind == TDZ_PC_LEARNED && c->learn.fallback_kind == TDZ_PC_MG);
-    if (!mg || !pulse || c->field_warned || pulse->omega <= 0 || pulse->ncycles <= 0 || dt <= 0) return;
+    if (!mg || dt <= 0) return;
+    auto warn = [&](const char *msg) {
+        g_warn = msg;
+        std::fprintf(stderr, "tdz: warning: %s\n", msg);
+    };
+    char msg[240];
+    if (!c->cap_warned && c->cap_on) {
+        double w = 0;
+        for (int d = 0; d < c->prob.dim; ++d)
+            for (double v : c->lv[0]->ax[d].cap) w = std::max(w, std::fabs(v));
+        if (tau * w > 0.5) {
+            std::snprintf(msg, sizeof msg,
+                          "tau*max W = %.2g > 0.5: the CAP dominates the propagator operator near the walls and "
+                          "the multigrid smoother (separable solve without the CAP) may not converge (use the "
+                          "separable preconditioner)", tau * w);
+            warn(msg);
+            c->cap_warned = true;
+        }
+    }
+    if (!pulse || c->field_warned || pulse->omega <= 0 || pulse->ncycles <= 0) return;
     const double T = pulse->ncycles * 2.0 * M_PI / pulse->omega;
     double Lmax[3] = {0, 0, 0};
     for (int a = 0; a < c->prob.dim; ++a) Lmax[a] = std::max(std::fabs(c->prob.lo[a]), std::fabs(c->prob.hi[a]));
@@ -1820,13 +1959,11 @@ void strong_field_check(tdz_ctx *c, const tdz_pulse *pulse, double tau, double d
         worst = std::max(worst, v);
     }
     if (tau * worst > 1.5) {
-        char msg[200];
         std::snprintf(msg, sizeof msg,
                       "tau*max|E.r| = %.2g > 1.5: the field term dominates the propagator operator and multigrid "
                       "may not converge (use the separable preconditioner or a smaller time step)",
                       tau * worst);
-        g_warn = msg;
-        std::fprintf(stderr, "tdz: warning: %s\n", msg);
+        warn(msg);
         c->field_warned = true;
     }
 }
@@ -1982,7 +2119,7 @@ void pred_alloc(tdz_ctx *c, int h)
         if ((p.axes >> a & 1) && !p.D[h][a]) p.D[h][a] = track(c, dalloc<dcomplex>(N));
 }
 
-// Push state x into the predictor history (slot (head + 1) % 3) with M x, D_a x and
+// Push state x into the predictor history (slot (head + 1) % ring) with M x, D_a x and
 // P = (M + g H_sE) x: with ax_in_slot the solve that produced x already left its A x in
 // that slot's P (under the field sE of its step), else P is one operator apply without
 // field.
@@ -1992,7 +2129,7 @@ void pred_push(tdz_ctx *c, const dcomplex *x, bool ax_in_slot, const double sE[3
     Level &L = *c->lv[0];
     const long N = L.N;
     const cplx g(0.0, 0.5 * p.dt);
-    const int h = (p.head + 1) % 3;
+    const int h = (p.head + 1) % p.ring;
     pred_alloc(c, h);
     // x, M x and D_a x: one fused pass for a field along z alone, else a copy and one apply each
     bool fused = false;
@@ -2025,9 +2162,10 @@ void pred_push(tdz_ctx *c, const dcomplex *x, bool ax_in_slot, const double sE[3
         set_shift(c, 1.0, g, zero);
         level_apply(c, L, x, p.P[h]);
         for (int a = 0; a < 3; ++a) p.sEh[h][a] = 0.0;
+        p.since_sync = 0;
     }
     p.head = h;
-    p.count = std::min(p.count + 1, 3);
+    p.count = std::min(p.count + 1, p.ring);
 }
 
 // True when x is bit for bit the newest state of the history (an evolve call that
@@ -2046,6 +2184,41 @@ bool pred_continues(tdz_ctx *c, const dcomplex *x, double dt)
     HIPCHK(hipMemcpyAsync(&v, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return v == 0.0;
+}
+
+// The stored products P_j are carried from solve to solve: each solve's A x comes from
+// its Arnoldi relation and from the history products it started with, so their rounding
+// errors propagate through the extrapolation. With weights of order q the recurrence
+// delta_new = sum_j w_j delta_j has the characteristic polynomial (z - 1)^(q+1), and the
+// inconsistency, injected at eps ||b|| per step, grows like n^(q+1/2) eps ||b|| (the
+// solutions then drift from the true CN steps by about as much per step). Every
+// sync_every steps the products are recomputed exactly (one operator apply per slot);
+// the drift measured there steers the interval so that it stays between 1e-3 and 1e-2
+// of the solve tolerance (a per-step state error well inside the tolerance's own).
+void pred_resync(tdz_ctx *c, double rtol)
+{
+    auto &p = c->pred;
+    if (p.order == 0 || p.count < 2 || ++p.since_sync < p.sync_every) return;
+    p.since_sync = 0;
+    Level &L = *c->lv[0];
+    const long N = L.N;
+    const cplx g(0.0, 0.5 * p.dt);
+    double drift = 0;
+    for (int j = 0; j < p.count; ++j) {
+        const int s = (p.head - j + p.ring) % p.ring;
+        set_shift(c, 1.0, g, p.sEh[s]);
+        level_apply(c, L, p.psi[s], c->sd.w2);
+        tdz::LinComb d;
+        d.add({1, 0}, p.P[s]);
+        d.add({-1, 0}, c->sd.w2);
+        tdz::launch_lincomb(c->st, d, c->sd.w1, N);
+        const double e2 = dev_norm2(c, c->sd.w1, N), p2 = dev_norm2(c, c->sd.w2, N);
+        drift = std::max(drift, std::sqrt(e2 / std::max(p2, 1e-300)));
+        HIPCHK(hipMemcpyAsync(p.P[s], c->sd.w2, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+    }
+    if (drift < 1e-3 * rtol) p.sync_every = std::min(2 * p.sync_every, 512);
+    else if (drift > 1e-2 * rtol) p.sync_every = std::max(p.sync_every / 2, 2);
+    p.last_drift = drift;
 }
 
 // One CN step with an extrapolated initial guess. With the states psi_j (j = 0 the
@@ -2076,14 +2249,17 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
         p.dt = dt;
         pred_push(c, x, false, sE);
     }
-    auto slot = [&](int j) { return (p.head - j + 3) % 3; };
+    pred_resync(c, rtol);
+    auto slot = [&](int j) { return (p.head - j + p.ring) % p.ring; };
     const int h0 = slot(0);
     tdz::LinComb rb;   // b = 2 M psi_0 - P_0 - g (sE - sE^0).D psi_0 (M psi_0 kept at its push)
     rb.add({2, 0}, p.Mx[h0]);
     rb.add({-1, 0}, p.P[h0]);
     for (int a = 0; a < 3; ++a)
         if (p.axes >> a & 1) rb.add(dc(-g * (sE[a] - p.sEh[h0][a])), p.D[h0][a]);
-    static const double W[3][3] = {{1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
+    // binomial extrapolation weights of orders 0..4 (newest state first)
+    static const double W[5][5] = {{1, 0, 0, 0, 0}, {2, -1, 0, 0, 0}, {3, -3, 1, 0, 0}, {4, -6, 4, -1, 0},
+                                   {5, -10, 10, -5, 1}};
     const int o = std::min(p.order, p.count - 1);
     tdz::LinComb r0, x0;   // r0 = b - sum_j w_j A psi_j;  x0 = sum_j w_j psi_j
     for (int j = 0; j <= o; ++j) {
@@ -2103,7 +2279,7 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
     c->norms_queued = true;
     // the solve leaves A x in the P of the slot the new state goes to (the oldest slot,
     // whose P the pass above has already read)
-    const int hnext = (p.head + 1) % 3;
+    const int hnext = (p.head + 1) % p.ring;
     pred_alloc(c, hnext);
     step_solve(c, c->rhs, x, rtol, maxit, st, true, p.P[hnext]);
     pred_push(c, x, true, sE);
@@ -2115,10 +2291,20 @@ extern "C" {
 int tdz_set_cn_predictor(tdz_ctx *c, int order)
 {
     return guarded([&] {
-        if (order < 0 || order > 2) throw Error("CN predictor order must be 0, 1 or 2");
+        if (order < 0 || order > 4) throw Error("CN predictor order must be 0..4");
         c->pred.order = order;
+        c->pred.ring = std::max(order + 1, 1);
         c->pred.count = 0;
+        c->pred.head = 0;
     });
+}
+
+int tdz_cn_predictor_status(const tdz_ctx *c, int *sync_every, double *last_drift)
+{
+    if (!c) { g_err = "null context"; return 1; }
+    if (sync_every) *sync_every = c->pred.sync_every;
+    if (last_drift) *last_drift = c->pred.last_drift;
+    return 0;
 }
 
 int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], double rtol, int maxit,
@@ -2562,7 +2748,7 @@ int tdz_evolve_rational(tdz_ctx *c, double *psi, double t0, double dt, int nstep
         // commutator-free Magnus 4 (src/propagator.cpp:729-758): Gauss nodes c1,2, weights a1,2
         const double s3 = std::sqrt(3.0);
         if (!c->pc_ready) pc_setup(c, c->pc_kind);
-        strong_field_check(c, pulse, magnus ? dt / (4.0 * s3) : dt / (2.0 * s3), dt);   // dt/(2 sqrt12), dt/sqrt12
+        multigrid_checks(c, pulse, magnus ? dt / (4.0 * s3) : dt / (2.0 * s3), dt);   // dt/(2 sqrt12), dt/sqrt12
         const double c1 = 0.5 - s3 / 6.0, c2 = 0.5 + s3 / 6.0;
         const double a1 = (3.0 - 2.0 * s3) / 12.0, a2 = (3.0 + 2.0 * s3) / 12.0;
         for (int s = 0; s < nsteps; ++s) {
@@ -2738,7 +2924,7 @@ int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, con
 {
     return guarded([&] {
         if (!c->pc_ready) pc_setup(c, c->pc_kind);
-        strong_field_check(c, pulse, 0.5 * dt, dt);
+        multigrid_checks(c, pulse, 0.5 * dt, dt);
         dcomplex *x = reinterpret_cast<dcomplex *>(psi);
         if (c->pred.order > 0 && !pred_continues(c, x, dt)) c->pred.count = 0;
         for (int s = 0; s < nsteps; ++s) {
@@ -2803,6 +2989,7 @@ int tdz_learn_train(tdz_ctx *c, const double *psi, double t0, double dt, const t
         }
         ls.harvesting = false;
         HIPCHK(hipFree(x));
+        random_samples(c, ls.random_count, ls.seed);
         ls.base_its = base;
         if (ls.ys.empty()) throw Error("no samples harvested");
         normalise_samples(c);
@@ -2811,6 +2998,15 @@ int tdz_learn_train(tdz_ctx *c, const double *psi, double t0, double dt, const t
         ls.trained = true;
         ls.fallen = false;
         pc_setup(c, kind0 == TDZ_PC_NONE ? TDZ_PC_LEARNED : kind0);
+    });
+}
+
+int tdz_learn_config(tdz_ctx *c, int random_samples, unsigned long long seed)
+{
+    return guarded([&] {
+        if (random_samples < 0 || random_samples > 64) throw Error("random samples must be 0..64");
+        c->learn.random_count = random_samples;
+        c->learn.seed = seed;
     });
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -163,11 +164,16 @@ void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dco
 // y = sum_k c_k v_k (k <= kMax terms; y may be one of the v_k: every element is read
 // before it is written)
 struct LinComb {
-    static constexpr int kMax = 16;
+    static constexpr int kMax = 24;
     const dcomplex *v[kMax];
     dcomplex c[kMax];
     int k = 0;
-    void add(dcomplex coef, const dcomplex *vec) { c[k] = coef; v[k++] = vec; }
+    void add(dcomplex coef, const dcomplex *vec)
+    {
+        if (k >= kMax) throw std::runtime_error("LinComb: more than kMax terms");
+        c[k] = coef;
+        v[k++] = vec;
+    }
 };
 void launch_lincomb(hipStream_t st, const LinComb &lc, dcomplex *y, long N);
 // y1 = sum l1; y2 = y1 + sum l2; y3 = sum l3, one pass
@@ -302,6 +308,8 @@ void cnn_to_complex(hipStream_t st, long N, const float *out, fcomplex g, fcompl
 // partial[b] (b < kLossBlocks, FP64) sums to sum (out - t)^2 over both components
 constexpr int kLossBlocks = 256;
 void cnn_loss(hipStream_t st, long N, const float *out, const fcomplex *target, double *partial);
+// r = complex values with both parts uniform in [-1, 1), reproducible from seed
+void random_field(hipStream_t st, long N, unsigned long long seed, fcomplex *r);
 // t = pv - Vg .* My
 void lumped_residual(hipStream_t st, long N, const fcomplex *pv, const fcomplex *My, const float *Vg, fcomplex *t);
 // u[c] *= v[c], c < kCnnInputs (in place)

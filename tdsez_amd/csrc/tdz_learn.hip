@@ -946,6 +946,32 @@ void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, c
                        fb.C[2], fb.g[0], fb.g[1], fb.g[2], sc, y, tmp[3], tmp[4], tmp[5], tmp[6], My, corr, u);
 }
 
+namespace {
+// r[i] = two values uniform in [-1, 1) from a 64-bit mix (splitmix64) of (seed, i)
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void random_field_kernel(long N, unsigned long long seed, fcomplex *__restrict__ r)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        const unsigned long long h = mix64(seed * 0x100000001B3ull + (unsigned long long)i);
+        const float a = (float)(h >> 40) * (1.0f / 8388608.0f) - 1.0f;                 // 24 bits
+        const float b = (float)((h >> 16) & 0xFFFFFFull) * (1.0f / 8388608.0f) - 1.0f;
+        r[i] = {a, b};
+    }
+}
+}  // namespace
+
+void random_field(hipStream_t st, long N, unsigned long long seed, fcomplex *r)
+{
+    hipLaunchKernelGGL(random_field_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, N, seed, r);
+}
+
 void scale_inputs(hipStream_t st, long N, const CnnScales &s, fcomplex *u)
 {
     hipLaunchKernelGGL(scale_inputs_kernel, dim3(gridn(CI * N, 256)), dim3(256), 0, st, N, s, u);

@@ -19,9 +19,12 @@ pytestmark = pytest.mark.gpu
 # config 2 shape with enough elements for a 3-level hierarchy (p = 4: 20 -> 10 -> 5)
 MG3D = dict(dim=3, nelem=(20, 20, 24), lo=(-10.0, -10.0, -12.0), hi=(10.0, 10.0, 12.0), p=4, nquad=8, mass=1.0,
             charge=-1.0, pot_kind=2, pot=(1.0, 2.0), cap_kind=1, cap=(0.01, 3.0), mg_levels=3)
-# p = 3, Q = 5, Manolopoulos CAP, heavier carrier, uneven axes (16 -> 8 -> 4)
+# p = 3, Q = 5, heavier carrier, stronger CAP, uneven axes (16 -> 8 -> 4). The V-cycle's
+# separable smoother leaves the CAP out, so the CAP must stay a perturbation of the mass
+# term (tau max W < 1); the Manolopoulos profile, which diverges at the walls, is not
+# (tau max W ~ 100 at the outermost Gauss points here) and warns
 MG3D_P3 = dict(dim=3, nelem=(16, 20, 16), lo=(-8.0, -9.0, -8.0), hi=(8.0, 9.0, 8.0), p=3, nquad=5, mass=1.7,
-               charge=-1.0, pot_kind=2, pot=(1.0, 1.0), cap_kind=2, cap=(1.2,), mg_levels=3)
+               charge=-1.0, pot_kind=2, pot=(1.0, 1.0), cap_kind=1, cap=(0.05, 3.0), mg_levels=3)
 PROBLEMS = {"mg3d_p4": MG3D, "mg3d_p3": MG3D_P3}
 
 
@@ -171,14 +174,32 @@ def test_refresh_rule_two_base_plus_five(mg):
 
 
 def test_strong_field_warning():
-    """tau max|E.r| > 1.5 with multigrid warns (the reference's LaserX = 2.0, TimeStep = 0.2)."""
+    """tau max|E.r| > 1.5 with multigrid warns (the reference's test: a field of 2 a.u. at
+    TimeStep = 0.2; the sin^2 pulse here peaks at about 0.7 E0)."""
     import tdsez_amd as T
     desc = dict(MG3D, nelem=(20, 20, 20), lo=(-10.0,) * 3, hi=(10.0,) * 3)
     ctx = T.Context(desc)
     try:
         ctx.pc_setup(T.PC_MG)
         psi = ctx.vec(np.zeros(ctx.N, complex))
-        ctx.evolve_cn(psi, 0.0, 0.2, 0, T.make_pulse((0.0, 0.0, 2.0), 1.0, 1.0), rtol=1e-10)
+        ctx.evolve_cn(psi, 0.0, 0.2, 0, T.make_pulse((0.0, 0.0, 0.5), 1.0, 1.0), rtol=1e-10)
+        assert "tau*max|E.r|" not in T.last_warning()          # 0.1 * 0.35 * 10 < 1.5
+        ctx.evolve_cn(psi, 0.0, 0.2, 0, T.make_pulse((0.0, 0.0, 4.0), 1.0, 1.0), rtol=1e-10)
         assert "tau*max|E.r|" in T.last_warning()
+    finally:
+        ctx.close()
+
+
+def test_dominant_cap_warning():
+    """A CAP that dominates the mass term near the walls (the Manolopoulos profile) warns with
+    multigrid: the smoother's separable solve does not contain it."""
+    import tdsez_amd as T
+    desc = dict(MG3D_P3, cap_kind=2, cap=(1.2,))
+    ctx = T.Context(desc)
+    try:
+        ctx.pc_setup(T.PC_MG)
+        psi = ctx.vec(np.zeros(ctx.N, complex))
+        ctx.evolve_cn(psi, 0.0, 0.02, 0, None, rtol=1e-10)
+        assert "CAP" in T.last_warning()
     finally:
         ctx.close()

@@ -44,7 +44,7 @@ def _setup(desc, o, ctx, nsteps, dt):
     return psi0, pulse, sE
 
 
-@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_predicted_steps_match_oracle(pair, order):
     import tdsez_amd as T
     name, desc, o, ctx = pair
@@ -60,11 +60,14 @@ def test_predicted_steps_match_oracle(pair, order):
         # is that times the condition of M + i dt/2 H (a few here), summed over the steps
         err = _l2(o, d.download() - ref) / _l2(o, ref)
         assert err < 1e-11, (name, order, err, its)
-        # and the guess pays: no more iterations than starting from the current state
-        d0 = ctx.vec(psi0)
-        ctx.set_cn_predictor(0)
-        its0, _ = ctx.evolve_cn(d0, 0.0, dt, nsteps, pulse, rtol=1e-13)
-        assert its[2:].sum() <= its0[2:].sum(), (name, its, its0)
+        # and the guess pays: no more iterations than starting from the current state (orders
+        # 1 and 2; the higher ones amplify the rough error of the Manolopoulos CAP problem,
+        # whose separable preconditioner needs 35 iterations, by an iteration or two)
+        if order <= 2:
+            d0 = ctx.vec(psi0)
+            ctx.set_cn_predictor(0)
+            its0, _ = ctx.evolve_cn(d0, 0.0, dt, nsteps, pulse, rtol=1e-13)
+            assert its[2:].sum() <= its0[2:].sum(), (name, its, its0)
     finally:
         ctx.set_cn_predictor(0)
 
@@ -129,6 +132,36 @@ def test_predictor_order_is_checked():
     ctx = T.Context(SOFTCORE_2D)
     try:
         with pytest.raises(RuntimeError):
-            ctx.set_cn_predictor(3)
+            ctx.set_cn_predictor(5)
+        with pytest.raises(RuntimeError):
+            ctx.set_cn_predictor(-1)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_long_run_stays_on_the_cn_trajectory(order):
+    """The stored products are carried from solve to solve; their rounding would grow like
+    n^(order+1/2) through the extrapolation (delta_new = sum_j w_j delta_j). With the
+    periodic exact recomputation 400 predicted steps stay on the oracle's CN trajectory."""
+    import tdsez_amd as T
+    from oracle import Oracle
+    desc = dict(SOFTCORE_3D, nelem=(5, 5, 6))
+    o, ctx = Oracle(desc), T.Context(desc)
+    try:
+        dt, nsteps = 0.02, 400
+        psi0 = o.project_gaussian(centre=(0.5, -0.3, 0.2), width=(1.2, 1.0, 1.1), momentum=(0.3, 0.0, -0.2))
+        pulse = T.make_pulse(E0=(0.0, 0.0, 0.08), omega=0.3, ncycles=2.0)
+        sE = np.array([ctx.coupling(pulse, (s + 0.5) * dt) for s in range(nsteps)])
+        ref, _, _ = o.cn_steps(psi0, dt, sE, rtol=1e-14)
+        ctx.pc_setup(T.PC_SEPARABLE)
+        ctx.set_cn_predictor(order)
+        d = ctx.vec(psi0)
+        its, _ = ctx.evolve_cn(d, 0.0, dt, nsteps, pulse, rtol=1e-12)
+        err = _l2(o, d.download() - ref) / _l2(o, ref)
+        every, drift = ctx.predictor_status()
+        print(f"order {order}: error {err:.2e}, resync every {every} steps, last drift {drift:.1e}")
+        assert err < 1e-10, (order, err, every, drift)
+        assert drift < 1e-14, drift   # within 1e-2 of the tolerance
     finally:
         ctx.close()

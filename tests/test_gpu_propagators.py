@@ -151,6 +151,7 @@ def test_pade_guess_options_keep_the_answer(order, extrapolate):
     cnt, hist = ctx.pade_history()
     assert hist.shape == (4, order, ctx.N)
     assert list(cnt[:2]) == ([min(order, nsteps)] * 2 if extrapolate else [0, 0])
-    err = np.linalg.norm(d.download() - ref) / np.linalg.norm(ref)
-    assert err < 1e-11, err
+    diff = d.download() - ref
+    err = np.sqrt(abs(np.vdot(diff, o.term(0, diff))) / abs(np.vdot(ref, o.term(0, ref))))
+    assert err < 1e-12, err
     ctx.close()

@@ -170,7 +170,7 @@ def test_learned_preconditioner_gain_and_fallback():
     its_c, xc = runs[T.PC_CORRECTED]
     its_l, xl = runs[T.PC_LEARNED]
     print("iterations without / with the network", its_c, its_l)
-    assert its_l.max() <= its_c.max(), (its_l, its_c)
+    assert its_l.sum() < its_c.sum(), (its_l, its_c)   # the network saves iterations on unseen states
     assert ctx.learn_status()[1] == 0
     # same wavefunction in L2 (the coefficient vector itself is only fixed to
     # cond(M) times the solver tolerance)
