@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed steps before each timed run (predictor history)")
     ap.add_argument("--repeats", type=int, default=3, help="timed runs per leg and window (median reported)")
     ap.add_argument("--windows", default="early,peak", help="comma list of early, peak")
+    ap.add_argument("--legs", default="fp64_mg,learned,network_free",
+                    help="comma list of fp64_mg, learned, network_free, mixed_mg (profiling runs)")
     ap.add_argument("--train-iters", type=int, default=150)
     ap.add_argument("--nel", type=int, default=128)
     ap.add_argument("--rtol", type=float, default=1e-10)
@@ -82,7 +84,7 @@ def main():
                     help="initial guess of the CN solves of the value's runs, every leg: 0 the current state "
                          "(TSTHETA's, the reference's configuration), k > 0 the binomial extrapolation of the "
                          "last k+1 states")
-    ap.add_argument("--alt-predictor", type=int, default=2, choices=[0, 1, 2, 3, 4],
+    ap.add_argument("--alt-predictor", type=int, default=2, choices=[-1, 0, 1, 2, 3, 4],
                     help="guess of the second set of runs reported beside the value (-1 to skip)")
     ap.add_argument("--workload", default="atom128", choices=["atom128", "fullpulse", "scan", "hetero"],
                     help="atom128: the 3D soft-core CN benchmark (default); fullpulse: the whole pulse with "
@@ -106,7 +108,7 @@ def main():
 class Atom:
     """The 128^3 soft-core context with its eigenstate and the trained network."""
 
-    def __init__(self, args):
+    def __init__(self, args, train=True):
         import tdsez_amd as T
         self.T, self.args = T, args
         self.desc = problem(args.nel)
@@ -119,6 +121,14 @@ class Atom:
         log(f"ground state E0 = {self.E0:.10f} (eigen-residual {res0:.1e}, {it0} inverse iterations)")
         self.pulse = T.make_pulse(E0=(0.0, 0.0, E0Z), omega=OMEGA, ncycles=NCYC)
         self.x0 = psi.download()
+        self.trained = train
+        self.learn = None
+        if not train:
+            psi.free()
+            self.setup_s = time.perf_counter() - t0
+            self.kinds = {"learned": T.PC_LEARNED, "fp64_mg": T.PC_MG, "network_free": T.PC_CORRECTED,
+                          "mixed_mg": T.PC_MG_MIXED}
+            return
         tl = time.perf_counter()
         l0, l1 = ctx.learn_train(psi, 0.0, DT, self.pulse, harvest_steps=2, max_samples=6, iters=args.train_iters,
                                  lr=3e-3, rtol=args.rtol)
@@ -185,10 +195,11 @@ class Atom:
 def atom(args):
     if args.workload == "fullpulse":
         return fullpulse(args)
-    A = Atom(args)
+    A = Atom(args, train="learned" in args.legs.split(","))
     T, ctx = A.T, A.ctx
     K, W, R = args.steps, args.warmup, args.repeats
     windows = [w for w in args.windows.split(",") if w]
+    legs = [g for g in args.legs.split(",") if g]
     roof = None
     states = {}
     for win in windows:
@@ -199,7 +210,7 @@ def atom(args):
             # (omega t = 3.5 pi: |E| = 0.048)
             npre = int(round(T_CREST / DT)) - W - K // 2
             t1 = time.perf_counter()
-            xw, _ = A.propagate("learned", A.x0, 0.0, npre, chunk=2000, order=2)
+            xw, _ = A.propagate("learned" if A.trained else "fp64_mg", A.x0, 0.0, npre, chunk=2000, order=2)
             states[win] = (xw, npre * DT)
             log(f"reached t = {npre * DT:.2f} in {time.perf_counter() - t1:.1f} s")
     guesses = [args.predictor] + ([args.alt_predictor] if args.alt_predictor >= 0 and
@@ -212,7 +223,7 @@ def atom(args):
             xw, tw = states[win]
             r = {"t_start": tw + W * DT, "field_at_start": float(ctx.coupling(A.pulse, tw + W * DT)[2])}
             finals = {}
-            for leg in ("fp64_mg", "learned", "network_free"):
+            for leg in legs:
                 sps, itsm = [], []
                 for rep in range(R):
                     timing = leg == "learned" and rep == 0 and win == windows[-1] and order == args.predictor
@@ -227,15 +238,17 @@ def atom(args):
                           "krylov_its_per_step": float(np.mean(itsm))}
                 log(f"guess {order}, {win}: {leg} {np.median(sps):.2f} steps/s (runs {['%.1f' % v for v in sps]}), "
                     f"its/step {np.mean(itsm):.2f}")
-            r["speedup_vs_fp64_mg"] = r["learned"]["steps_per_s"] / r["fp64_mg"]["steps_per_s"]
-            r["psi_rel_l2_diff_learned_vs_fp64"] = A.mnorm(finals["learned"] - finals["fp64_mg"]) / \
-                A.mnorm(finals["fp64_mg"])
+            if "learned" in r and "fp64_mg" in r:
+                r["speedup_vs_fp64_mg"] = r["learned"]["steps_per_s"] / r["fp64_mg"]["steps_per_s"]
+                r["psi_rel_l2_diff_learned_vs_fp64"] = A.mnorm(finals["learned"] - finals["fp64_mg"]) / \
+                    A.mnorm(finals["fp64_mg"])
             res[key][win] = r
     main_key = "current_state" if args.predictor == 0 else f"extrapolation_order_{args.predictor}"
     main_win = windows[-1]
-    sps = res[main_key][main_win]["learned"]["steps_per_s"]
-    _, fallbacks, active = ctx.learn_status()
-    A.learn.update({"fallbacks": fallbacks, "network_active": bool(active)})
+    sps = res[main_key][main_win][legs[0] if "learned" not in legs else "learned"]["steps_per_s"]
+    if A.learn is not None:
+        _, fallbacks, active = ctx.learn_status()
+        A.learn.update({"fallbacks": fallbacks, "network_active": bool(active)})
 
     p, Q, nel = A.desc["p"], A.desc["nquad"], args.nel
     flops = pot_flops_per_element(p, Q) * nel ** 3
@@ -270,7 +283,7 @@ def atom(args):
                    "cn_guess": ("current state (as TSTHETA)" if args.predictor == 0 else
                                 f"binomial extrapolation of the last {args.predictor + 1} states (every leg)"),
                    "dofs": ctx.N, "mg_levels": ctx.levels, "setup_s": A.setup_s},
-        "speedup_vs_fp64_mg": res[main_key][main_win]["speedup_vs_fp64_mg"],
+        "speedup_vs_fp64_mg": res[main_key][main_win].get("speedup_vs_fp64_mg"),
         "runs": res,
         "learned": A.learn,
         "roofline": roofline,

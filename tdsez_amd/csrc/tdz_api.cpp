@@ -263,6 +263,8 @@ struct tdz_ctx {
     // Greville points (lumped product) and the bands of g (CAP + sE.D), fine level, complex64
     float *dVg = nullptr;
     fcomplex *dBcorr[3] = {};
+    dcomplex *pot_scr = nullptr;   // scratch of the tiled 3D potential kernel (finest level)
+    fcomplex *pot_scrf = nullptr;
     LearnState learn;
     // kernel timing (measurement hook)
     bool timing = false;
@@ -540,6 +542,19 @@ tdz::SepBandsT<R> sep_bands(Level &L)
 template <typename R>
 inline cpx<R> cx(cplx z) { return {(R)z.real(), (R)z.imag()}; }
 
+// Scratch of the tiled 3D potential kernel (the finest level only; allocated at first use).
+template <typename R> cpx<R> *&pot_scratch_slot(tdz_ctx *c);
+template <> dcomplex *&pot_scratch_slot<double>(tdz_ctx *c) { return c->pot_scr; }
+template <> fcomplex *&pot_scratch_slot<float>(tdz_ctx *c) { return c->pot_scrf; }
+template <typename R>
+cpx<R> *pot_scratch(tdz_ctx *c, const Level &L)
+{
+    if (&L != c->lv[0].get() || !L.simple) return nullptr;
+    cpx<R> *&s = pot_scratch_slot<R>(c);
+    if (!s) s = track(c, dalloc<cpx<R>>(tdz::pot3d_scratch_len(L.N)));
+    return s;
+}
+
 // y += g (s V) x on level L: the tiled 3D kernel where it applies, else the generic one.
 template <typename R>
 void pot_apply(tdz_ctx *c, Level &L, cplx g, const cpx<R> *x, cpx<R> *y)
@@ -558,7 +573,8 @@ void pot_apply(tdz_ctx *c, Level &L, cplx g, const cpx<R> *x, cpx<R> *y)
     }
     const bool tiled =
         c->use_tiled &&
-        ((L.simple && tdz::launch_pot_apply_3d<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y, L.pot_variant)) ||
+        ((L.simple && tdz::launch_pot_apply_3d<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y, L.pot_variant,
+                                                  pot_scratch<R>(c, L))) ||
          (L.simple2 && tdz::launch_pot_apply_2d<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y, L.pot_variant)));
     if (!tiled) tdz::launch_pot_apply_generic<R>(c->st, ax, pp, (R)g.real(), (R)g.imag(), x, y);
     if (timed) HIPCHK(hipEventRecord(c->ev[c->ev_used++].second, c->st));
@@ -617,7 +633,7 @@ void autotune_pot(tdz_ctx *c, Level &L)
     tdz::PotParams pp{c->ph.pot_kind, c->ph.s, c->ph.pot[0], c->ph.pot[1]};
     float best = 1e30f;
     auto launch = [&](int v) {
-        if (L.simple) tdz::launch_pot_apply_3d<double>(c->st, ax, pp, 1.0, 0.0, x, y, v);
+        if (L.simple) tdz::launch_pot_apply_3d<double>(c->st, ax, pp, 1.0, 0.0, x, y, v, pot_scratch<double>(c, L));
         else tdz::launch_pot_apply_2d<double>(c->st, ax, pp, 1.0, 0.0, x, y, v);
     };
     for (int v = 0; v < nv; ++v) {
@@ -1013,32 +1029,40 @@ void random_samples(tdz_ctx *c, int count, unsigned long long seed)
     }
 }
 
-// Sweeps for the next solve of the corrected preconditioners: the fewest whose expected
-// contraction times the solve's initial relative residual meets half the tolerance (one
-// Krylov iteration), from the last observed contraction rho_k of each sweep count; an
-// unobserved count is estimated from an observed one as rho_j = rho_k^((j+1)/(k+1)) (the
-// error after k sweeps is (S^-1 dA)^(k+1)). When no count can make one iteration, one
-// sweep (the cheapest per iteration). Every 32nd solve tries one sweep fewer than the
-// estimates ask for, so that a weakening field is noticed.
+// Sweeps for the next solve of the corrected preconditioners, by expected cost: k sweeps
+// contract the residual by rho_k per Krylov iteration, so the solve needs about
+// its_k = ceil(log(tol / r0) / log(rho_k)) iterations (tol = half the relative tolerance,
+// r0 the solve's initial relative residual), each costing an operator apply and the
+// Krylov work (about 2.5 units of 1 ms at 128^3) plus the preconditioner (0.17 units per
+// separable solve, 0.1 per correction, 0.19 for the network). rho_k is the last
+// observed first-iteration contraction with k sweeps; an unobserved count is estimated
+// from an observed one as rho_j = rho_k^((j+1)/(k+1)) (the error after k sweeps is
+// (S^-1 dA)^(k+1)). Every 32nd solve that picks k > 1 tries k - 1 instead, so that a
+// weakening field is noticed.
 int choose_sweeps(tdz_ctx *c)
 {
     constexpr int kMaxSweeps = 3;
     if (c->pc_r0rel <= 0) return 1;
-    double est[kMaxSweeps + 1];
     int ref = -1;
     for (int k = 1; k <= kMaxSweeps; ++k)
         if (c->rho_hat[k] > 0 && (ref < 0 || k < ref)) ref = k;
-    for (int k = 1; k <= kMaxSweeps; ++k) {
-        if (c->rho_hat[k] > 0) est[k] = c->rho_hat[k];
-        else if (ref > 0) est[k] = std::pow(c->rho_hat[ref], (k + 1.0) / (ref + 1.0));
-        else est[k] = std::pow(1e-2, k + 1.0);
-    }
     const double target = 0.5 * c->pc_rtol / c->pc_r0rel;
-    int k = 1;
-    while (k <= kMaxSweeps && est[k] > target) ++k;
-    if (k > kMaxSweeps) return 1;
-    if (k > 1 && ++c->pc_solves % 32 == 0) --k;
-    return k;
+    const bool net = c->pc_kind == TDZ_PC_LEARNED;
+    int best = 1;
+    double best_cost = 1e300;
+    for (int k = 1; k <= kMaxSweeps; ++k) {
+        double rho = c->rho_hat[k];
+        if (!(rho > 0)) rho = ref > 0 ? std::pow(c->rho_hat[ref], (k + 1.0) / (ref + 1.0)) : std::pow(1e-2, k + 1.0);
+        rho = std::min(rho, 0.5);
+        const double its = target >= 1.0 ? 1.0 : std::max(1.0, std::ceil(std::log(target) / std::log(rho) - 1e-9));
+        const double cost = its * (2.5 + 0.17 * (k + 1) + 0.1 * k + (net ? 0.19 : 0.0));
+        if (cost < best_cost * (1 - 1e-9)) {
+            best_cost = cost;
+            best = k;
+        }
+    }
+    if (best > 1 && ++c->pc_solves % 32 == 0) --best;
+    return best;
 }
 
 // the current preconditioner is a corrected solve with adaptive sweeps

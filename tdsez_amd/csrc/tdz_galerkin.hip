@@ -121,47 +121,55 @@ __global__ void __launch_bounds__(256) stencil_apply_kernel(int nx, int ny, int 
     }
 }
 
-// The same for a 3D stencil of half-width P on every axis, fully unrolled over x and y
-// offsets: the neighbour index is clamped into the grid (a clamped neighbour carries a zero
-// stencil entry: no pair of functions at that offset), so the loops have no branches and
-// each thread has 2 (2P+1)^2 independent loads in flight per z offset.
+// The same for a 3D stencil of half-width P on every axis. A block of 64 x D threads takes
+// 64 consecutive rows; thread (r, dz) sums the (2P+1)^2 terms of z offset dz of row r
+// (fully unrolled: every load of the slab is in flight at once), and the D partial sums
+// of a row are added in LDS in a fixed order. Neighbour indices are clamped into the
+// grid: a clamped neighbour carries a zero stencil entry (no pair of functions at that
+// offset), so the loops have no branches.
 template <typename R, int P>
-__global__ void __launch_bounds__(256) stencil3_kernel(int nx, int ny, int nz, const R *__restrict__ S, R gr, R gi,
-                                                       const cpx<R> *__restrict__ x, cpx<R> *__restrict__ y)
+__global__ void __launch_bounds__(64 * (2 * P + 1)) stencil3_kernel(int nx, int ny, int nz, const R *__restrict__ S,
+                                                                    R gr, R gi, const cpx<R> *__restrict__ x,
+                                                                    cpx<R> *__restrict__ y)
 {
     constexpr int D = 2 * P + 1;
+    __shared__ R part[2][D][64];
     const long N = (long)nx * ny * nz;
     const long sX = nx, sY = (long)ny * D * nx, sZ = (long)nz * D * sY;   // strides of dx', dy', dz'
-    for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < N; I += (long)gridDim.x * blockDim.x) {
+    const int r = threadIdx.x, dz = threadIdx.y;
+    const long I = blockIdx.x * 64L + r;
+    R ar = 0, ai = 0;
+    if (I < N) {
         const int ix = (int)(I % nx);
         const int iy = (int)((I / nx) % ny);
         const int iz = (int)(I / ((long)nx * ny));
-        const R *Srow = S + (long)iz * D * sY + (long)iy * D * nx + ix;
-        int cx[D], cy[D];
+        const R *Sz = S + dz * sZ + (long)iz * D * sY + (long)iy * D * nx + ix;
+        const cpx<R> *xz = x + (long)min(max(iz + dz - P, 0), nz - 1) * ny * nx;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            cx[d] = min(max(ix + d - P, 0), nx - 1);
-            cy[d] = min(max(iy + d - P, 0), ny - 1) * nx;
-        }
-        R ar = 0, ai = 0;
-        for (int dz = 0; dz < D; ++dz) {
-            const long zoff = (long)min(max(iz + dz - P, 0), nz - 1) * ny * nx;
-            const R *Sz = Srow + dz * sZ;
+        for (int dy = 0; dy < D; ++dy) {
+            const cpx<R> *xr = xz + (long)min(max(iy + dy - P, 0), ny - 1) * nx;
 #pragma unroll
-            for (int dy = 0; dy < D; ++dy) {
-                const cpx<R> *xr = x + zoff + cy[dy];
-#pragma unroll
-                for (int dx = 0; dx < D; ++dx) {
-                    const R sv = Sz[dy * sY + dx * sX];
-                    const cpx<R> v = xr[cx[dx]];
-                    ar += sv * v.re;
-                    ai += sv * v.im;
-                }
+            for (int dx = 0; dx < D; ++dx) {
+                const R sv = Sz[dy * sY + dx * sX];
+                const cpx<R> v = xr[min(max(ix + dx - P, 0), nx - 1)];
+                ar += sv * v.re;
+                ai += sv * v.im;
             }
         }
+    }
+    part[0][dz][r] = ar;
+    part[1][dz][r] = ai;
+    __syncthreads();
+    if (dz == 0 && I < N) {
+        R sr = 0, si = 0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            sr += part[0][k][r];
+            si += part[1][k][r];
+        }
         cpx<R> o = y[I];
-        o.re += gr * ar - gi * ai;
-        o.im += gr * ai + gi * ar;
+        o.re += gr * sr - gi * si;
+        o.im += gr * si + gi * sr;
         y[I] = o;
     }
 }
@@ -208,12 +216,13 @@ void launch_stencil_apply(hipStream_t st, const int n[3], const int p[3], const 
 {
     const long N = (long)n[0] * n[1] * n[2];
     const bool cube = p[0] == p[1] && p[1] == p[2] && n[0] > 1 && n[1] > 1 && n[2] > 1;
+    const unsigned nb = (unsigned)((N + 63) / 64);
     if (cube) {
         switch (p[0]) {
-        case 2: stencil3_kernel<R, 2><<<grid_for(N), 256, 0, st>>>(n[0], n[1], n[2], S, gr, gi, x, y); return;
-        case 3: stencil3_kernel<R, 3><<<grid_for(N), 256, 0, st>>>(n[0], n[1], n[2], S, gr, gi, x, y); return;
-        case 4: stencil3_kernel<R, 4><<<grid_for(N), 256, 0, st>>>(n[0], n[1], n[2], S, gr, gi, x, y); return;
-        case 5: stencil3_kernel<R, 5><<<grid_for(N), 256, 0, st>>>(n[0], n[1], n[2], S, gr, gi, x, y); return;
+        case 2: stencil3_kernel<R, 2><<<nb, dim3(64, 5), 0, st>>>(n[0], n[1], n[2], S, gr, gi, x, y); return;
+        case 3: stencil3_kernel<R, 3><<<nb, dim3(64, 7), 0, st>>>(n[0], n[1], n[2], S, gr, gi, x, y); return;
+        case 4: stencil3_kernel<R, 4><<<nb, dim3(64, 9), 0, st>>>(n[0], n[1], n[2], S, gr, gi, x, y); return;
+        case 5: stencil3_kernel<R, 5><<<nb, dim3(64, 11), 0, st>>>(n[0], n[1], n[2], S, gr, gi, x, y); return;
         default: break;
         }
     }

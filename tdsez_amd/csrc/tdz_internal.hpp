@@ -93,10 +93,13 @@ template <typename R>
 void launch_pot_apply_generic(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
                               cpx<R> *y);
 // Same for dim 3 with simple interior knots (tiled z-marching kernel); returns
-// false when no instantiation covers (p, Q) or variant.
+// false when no instantiation covers (p, Q) or variant. scr: pot3d_scratch_len(N)
+// values of scratch (the tiles' contributions by offset class, summed into y in a fixed
+// order: the result does not depend on the schedule).
 template <typename R>
 bool launch_pot_apply_3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
-                         cpx<R> *y, int variant);
+                         cpx<R> *y, int variant, cpx<R> *scr);
+long pot3d_scratch_len(long N);
 // number of tile-shape variants of the tiled kernel for (p, Q) (0: none)
 int pot3d_variants(int p, int Q);
 // 2D grids with simple knots, p in 3..5, Q = 8: tiled kernel, variant = tile shape

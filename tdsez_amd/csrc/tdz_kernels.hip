@@ -296,13 +296,14 @@ __global__ void __launch_bounds__(TXE * Q * TYE * Q) pot2d_kernel(AxisDevT<R> a0
 //   B  z stage at the Q points of element ez, times s V(r) w, transposed z
 //      stage into the output window
 //   C  output plane ez is complete: transposed y then x stages through LDS,
-//      scaled by g, atomically added to y (neighbouring tiles share p planes
-//      of functions in x and y).
+//      scaled by g, stored to a scratch plane of the tile's offset class
+//      (neighbouring tiles share p planes of functions in x and y; pot3d_sum_kernel
+//      adds the classes into y in a fixed order, so the result is deterministic).
 // Requires simple interior knots (element e starts at function e) and dim 3.
 template <typename R, int P, int Q, int TE, int CPT, int KIND>
 __global__ void __launch_bounds__((TE * Q) * (TE * Q) / CPT)
 pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr, R gi,
-             const cpx<R> *__restrict__ x, cpx<R> *__restrict__ y)
+             const cpx<R> *__restrict__ x, cpx<R> *__restrict__ scr)
 {
     constexpr int NP = P + 1, TQ = TE * Q, NCOL = TQ * TQ, NT = NCOL / CPT, TW = TE + P;
     __shared__ cpx<R> cin[TW * TW];   // input plane  [jy][jx]
@@ -438,9 +439,10 @@ pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr,
                     si += b * v.im;
                 }
             }
-            R *dst = reinterpret_cast<R *>(y + gx + (long)n0 * gy + plane * jz);
-            unsafeAtomicAdd(dst, sr);
-            unsafeAtomicAdd(dst + 1, si);
+            // the tile's contribution to function (gx, gy, jz) goes to the scratch plane of its
+            // offset class (jxl / TE, jyl / TE): every (function, class) has one writer
+            constexpr int MW = (TW + TE - 1) / TE;
+            scr[((long)((jyl / TE) * MW + jxl / TE) * az.n + jz) * plane + gx + (long)n0 * gy] = {sr, si};
         }
         // no barrier here: the next writes of H1 and G2p follow the next emit's first
         // barrier, those of cin and the z tables (stage_in) only replace data whose
@@ -483,6 +485,35 @@ pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr,
         emit(ez);
     }
     for (int k = 1; k <= P; ++k) emit(Ez - 1 + k);
+}
+
+// y[I] += the contributions of the up to MW^2 tiles that cover function I, summed in a
+// fixed order (class y outer, class x inner). Tile bx covers functions bx TE .. bx TE + TW - 1.
+template <typename R>
+__global__ void pot3d_sum_kernel(int n0, int n1, int n2, int TE, int TW, int nbx, int nby,
+                                 const cpx<R> *__restrict__ scr, cpx<R> *__restrict__ y)
+{
+    const long plane = (long)n0 * n1, N = plane * n2;
+    const int MW = (TW + TE - 1) / TE;
+    for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < N; I += (long)gridDim.x * blockDim.x) {
+        const int gx = (int)(I % n0), gy = (int)((I / n0) % n1);
+        R sr = 0, si = 0;
+        for (int my = 0; my < MW; ++my) {
+            const int jyl = my * TE + gy % TE, by = gy / TE - my;
+            if (by < 0 || by >= nby || jyl >= TW) continue;
+            for (int mx = 0; mx < MW; ++mx) {
+                const int jxl = mx * TE + gx % TE, bx = gx / TE - mx;
+                if (bx < 0 || bx >= nbx || jxl >= TW) continue;
+                const cpx<R> v = scr[(long)(my * MW + mx) * N + I];
+                sr += v.re;
+                si += v.im;
+            }
+        }
+        cpx<R> o = y[I];
+        o.re += sr;
+        o.im += si;
+        y[I] = o;
+    }
 }
 
 // ------------------------------------------------------------------ separable
@@ -1553,17 +1584,23 @@ void launch_pot_apply_generic(hipStream_t st, const AxisDevT<R> ax[3], PotParams
 
 template <typename R, int P, int Q, int TE, int CPT>
 static void launch_pot3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
-                         cpx<R> *y)
+                         cpx<R> *y, cpx<R> *scr)
 {
-    constexpr int NT = (TE * Q) * (TE * Q) / CPT;
-    const dim3 grid((unsigned)((ax[0].E + TE - 1) / TE), (unsigned)((ax[1].E + TE - 1) / TE));
+    constexpr int NT = (TE * Q) * (TE * Q) / CPT, TW = TE + P;
+    const int nbx = (ax[0].E + TE - 1) / TE, nby = (ax[1].E + TE - 1) / TE;
+    const dim3 grid((unsigned)nbx, (unsigned)nby);
     if (pot.kind == 2)
         hipLaunchKernelGGL((pot3d_kernel<R, P, Q, TE, CPT, 2>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr,
-                           gi, x, y);
+                           gi, x, scr);
     else
         hipLaunchKernelGGL((pot3d_kernel<R, P, Q, TE, CPT, 1>), grid, dim3(NT), 0, st, ax[0], ax[1], ax[2], pot, gr,
-                           gi, x, y);
+                           gi, x, scr);
+    const long N = (long)ax[0].n * ax[1].n * ax[2].n;
+    hipLaunchKernelGGL(pot3d_sum_kernel<R>, dim3((unsigned)std::min<long>((N + 255) / 256, 1L << 20)), dim3(256), 0,
+                       st, ax[0].n, ax[1].n, ax[2].n, TE, TW, nbx, nby, scr, y);
 }
+
+long pot3d_scratch_len(long N) { return 9 * N; }   // MW^2 <= 9 classes for every variant (TW <= 3 TE)
 
 template <typename R, int P, int Q, int TXE, int TYE>
 static void launch_pot2d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
@@ -1620,8 +1657,9 @@ int pot3d_variants(int p, int Q)
 
 template <typename R>
 bool launch_pot_apply_3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot, R gr, R gi, const cpx<R> *x,
-                         cpx<R> *y, int variant)
+                         cpx<R> *y, int variant, cpx<R> *scr)
 {
+    if (!scr) return false;
     const int p = ax[0].p, Q = ax[0].Q;
     for (int d = 1; d < 3; ++d)
         if (ax[d].p != p || ax[d].Q != Q) return false;
@@ -1631,23 +1669,23 @@ bool launch_pot_apply_3d(hipStream_t st, const AxisDevT<R> ax[3], PotParams pot,
     // tiles, 128 threads of 2 point columns; 3: 4 x 4 tiles, 256 threads of 4 columns
     if (p == 4 && Q == 8) {
         switch (variant) {
-        case 0: launch_pot3d<R, 4, 8, 4, 1>(st, ax, pot, gr, gi, x, y); break;
-        case 1: launch_pot3d<R, 4, 8, 2, 1>(st, ax, pot, gr, gi, x, y); break;
-        case 2: launch_pot3d<R, 4, 8, 2, 2>(st, ax, pot, gr, gi, x, y); break;
-        default: launch_pot3d<R, 4, 8, 4, 4>(st, ax, pot, gr, gi, x, y); break;
+        case 0: launch_pot3d<R, 4, 8, 4, 1>(st, ax, pot, gr, gi, x, y, scr); break;
+        case 1: launch_pot3d<R, 4, 8, 2, 1>(st, ax, pot, gr, gi, x, y, scr); break;
+        case 2: launch_pot3d<R, 4, 8, 2, 2>(st, ax, pot, gr, gi, x, y, scr); break;
+        default: launch_pot3d<R, 4, 8, 4, 4>(st, ax, pot, gr, gi, x, y, scr); break;
         }
         return true;
     }
     if (p == 3 && Q == 8) {
         switch (variant) {
-        case 0: launch_pot3d<R, 3, 8, 4, 1>(st, ax, pot, gr, gi, x, y); break;
-        case 1: launch_pot3d<R, 3, 8, 2, 1>(st, ax, pot, gr, gi, x, y); break;
-        case 2: launch_pot3d<R, 3, 8, 2, 2>(st, ax, pot, gr, gi, x, y); break;
-        default: launch_pot3d<R, 3, 8, 4, 4>(st, ax, pot, gr, gi, x, y); break;
+        case 0: launch_pot3d<R, 3, 8, 4, 1>(st, ax, pot, gr, gi, x, y, scr); break;
+        case 1: launch_pot3d<R, 3, 8, 2, 1>(st, ax, pot, gr, gi, x, y, scr); break;
+        case 2: launch_pot3d<R, 3, 8, 2, 2>(st, ax, pot, gr, gi, x, y, scr); break;
+        default: launch_pot3d<R, 3, 8, 4, 4>(st, ax, pot, gr, gi, x, y, scr); break;
         }
         return true;
     }
-    if (p == 3 && Q == 5) { launch_pot3d<R, 3, 5, 4, 1>(st, ax, pot, gr, gi, x, y); return true; }
+    if (p == 3 && Q == 5) { launch_pot3d<R, 3, 5, 4, 1>(st, ax, pot, gr, gi, x, y, scr); return true; }
     return false;
 }
 
@@ -2274,7 +2312,7 @@ void launch_zline_solve(hipStream_t st, const int n[3], int p, const R *lx, cons
     template void launch_pot_apply_generic<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,  \
                                               cpx<R> *);                                                           \
     template bool launch_pot_apply_3d<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,       \
-                                         cpx<R> *, int);                                                           \
+                                         cpx<R> *, int, cpx<R> *);                                                 \
     template bool launch_pot_apply_2d<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,       \
                                          cpx<R> *, int);                                                           \
     template void launch_sep_apply<R>(hipStream_t, const SepBandsT<R> &, const cpx<R> *, cpx<R> *, cpx<R> *[4]);   \

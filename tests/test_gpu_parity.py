@@ -133,8 +133,7 @@ def test_cn_steps_match_oracle(pair):
     # wavefunction error in L2 (sqrt(c^H M c)), as for the chained steps below: the
     # coefficient vector is only fixed to cond(M) times the solver tolerance (the oracle's
     # own rtol 1e-13 and 1e-15 steps differ by 2e-12 in coefficients, 7e-14 in L2, on the
-    # 4^3 hole problem), and the FP64 atomic adds of the potential kernel make the last
-    # Krylov iterate vary from run to run at that level
+    # 4^3 hole problem)
     err = np.sqrt(abs(np.vdot(diff, o.term(0, diff)))) / np.sqrt(abs(np.vdot(ref, o.term(0, ref))))
     assert err < 1e-12, (name, err, np.linalg.norm(diff) / np.linalg.norm(ref))
 
