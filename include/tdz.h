@@ -227,6 +227,14 @@ int tdz_pade_history(tdz_ctx *ctx, int set, int counts[4], double *hist);
 int tdz_set_pade_guess(tdz_ctx *ctx, int extrapolate, int order);
 int tdz_trace_row(tdz_ctx *ctx, const double *psi, const double *psi0, double t, const double E[3], double *dip,
                   double *energy, double *ac);
+/* Allocation-free variant of tdz_observables (no counterpart in the reference; for callers
+ * that cannot allocate: code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). The same six values, with the products on the device and the reductions'
+ * partial sums in a static device array of the library (65.7 KB per GPU, any N: a fixed
+ * number of partials per reduction); no device or host allocation during the call. Calls
+ * that overlap on one GPU share that array and race: serialise them. Not for layered
+ * structures. */
+int tdz_observables_noalloc(tdz_ctx *ctx, const double *psi, double out[6]);
 /* Field sE_a(t) = s E_a(t) of a pulse (TDSEZParser::Coupling, include/tdsez_parser.hpp:190-194). */
 int tdz_pulse_coupling(const tdz_ctx *ctx, const tdz_pulse *pulse, double t, double sE[3]);
 

@@ -35,7 +35,7 @@ EXPORTS = [
     "tdz_trace_row", "tdz_pade_history", "tdz_batch_create", "tdz_batch_context", "tdz_batch_evolve",
     "tdz_batch_fixups", "tdz_batch_destroy", "tdz_ifunction", "tdz_level_dims", "tdz_mg_transfer",
     "tdz_level_apply", "tdz_set_pc_refresh", "tdz_coarsen_knots", "tdz_refinement_matrix", "tdz_last_warning",
-    "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
+    "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_observables_noalloc", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
 ]
 
 
@@ -176,6 +176,7 @@ def lib():
         L.tdz_pc_set_corrections.argtypes = [vp, i]
         L.tdz_learn_config.argtypes = [vp, i, C.c_ulonglong]
         L.tdz_cn_predictor_status.argtypes = [vp, ip, dp]
+        L.tdz_observables_noalloc.argtypes = [vp, vp, dp]
         _lib = L
     return _lib
 
@@ -446,6 +447,12 @@ class Context:
     def observables(self, psi):
         out = np.zeros(6)
         _chk(lib().tdz_observables(self.h, psi.ptr, _dp(out)))
+        return out
+
+    def observables_noalloc(self, psi):
+        """The same as observables(), from the allocation-free entry point."""
+        out = np.zeros(6)
+        _chk(lib().tdz_observables_noalloc(self.h, psi.ptr, _dp(out)))
         return out
 
     def coupling(self, pulse, t):

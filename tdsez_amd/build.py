@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libtdz.so")
 OBJ_DIR = os.path.join(HERE, "_build")
-SOURCES = ["tdz_kernels.hip", "tdz_learn.hip", "tdz_galerkin.hip", "tdz_api.cpp", "tdz_basis.cpp"]
+SOURCES = ["tdz_kernels.hip", "tdz_learn.hip", "tdz_galerkin.hip", "tdz_noalloc.hip", "tdz_api.cpp", "tdz_basis.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -178,6 +178,7 @@ void upload_weights(LearnState &ls)
 }  // namespace
 
 struct tdz_ctx {
+    int device = -1;            // the HIP device current at creation (its stream's device)
     tdz_problem prob{};
     tdz::PhysParams ph;
     hipStream_t st = nullptr;
@@ -1516,6 +1517,7 @@ void build(tdz_ctx *c, const tdz_problem &pr)
         if (!(pr.hi[d] > pr.lo[d])) throw Error("empty domain");
     }
     c->prob = pr;
+    HIPCHK(hipGetDevice(&c->device));
     c->ph.mass = pr.mass;
     c->ph.s = -pr.charge;
     c->ph.pot_kind = pr.pot_kind;
@@ -1652,6 +1654,20 @@ void build(tdz_ctx *c, const tdz_problem &pr)
     c->prob.layer_bounds = c->prob.layer_mass = c->prob.layer_v = nullptr;
     const double zero[3] = {0, 0, 0};
     set_shift(c, 1.0, 0.0, zero);
+    // device data that later calls would otherwise allocate at first use: the potential
+    // kernel's scratch, the complex position and zero bands of the dipole products, the zero
+    // vector of the wall rows (tdz_observables_noalloc allocates nothing)
+    if (c->ph.pot_kind) pot_scratch<double>(c, *c->lv[0]);
+    auto &pd = c->pred;
+    Level &L0 = *c->lv[0];
+    for (int d = 0; d < 3; ++d) {
+        if (!pd.Zband[d]) pd.Zband[d] = upload(c, std::vector<dcomplex>(L0.M[d].size(), dcomplex{0, 0}));
+        std::vector<dcomplex> B(L0.Xb[d].size());
+        for (size_t k = 0; k < B.size(); ++k) B[k] = {L0.Xb[d][k], 0.0};
+        if (!pd.Xband[d]) pd.Xband[d] = upload(c, B);
+    }
+    if (pr.dirichlet && !c->zerov) c->zerov = upload(c, std::vector<dcomplex>(L0.N, dcomplex{0, 0}));
+    tdz::noalloc_init();
 }
 
 template <typename F>
@@ -2861,6 +2877,64 @@ void expectations(tdz_ctx *c, const double *psi, const double *psi0, cplx *v)
     v[7] = psi0 ? dot(TDZ_TERM_M, h0) : cplx(0);
 }
 }  // namespace
+
+extern "C" {
+
+}  // extern "C"
+
+namespace tdz {
+int fail(const std::string &msg)
+{
+    g_err = msg;
+    return 1;
+}
+
+int ctx_device(const tdz_ctx *c) { return c ? c->device : -1; }
+
+// The observables of tdz_observables on the device, with the reductions' partial sums and
+// results in the caller's scratch scr (noalloc_scratch_len values, device memory): no
+// allocation, no host staging. The products are those of tdz_term_apply (wall rows: the
+// identity) and of the shifted operator with alpha = 0, g = 1 (H without field).
+int observables_device(tdz_ctx *c, const double *psi, double out[6], dcomplex *scr)
+{
+    return guarded([&] {
+        if (!c || !psi || !out || !scr) throw Error("null argument");
+        if (c->layered) throw Error("tdz_observables_noalloc: layered structures are not supported");
+        Level &L = *c->lv[0];
+        const long N = L.N;
+        if (c->ph.pot_kind && L.simple && !c->pot_scr) throw Error("potential scratch missing");
+        const dcomplex *x = reinterpret_cast<const dcomplex *>(psi);
+        dcomplex *w = c->sd.w1, *res = scr + (long)kObsDots * kRedBlocks;
+        const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+        auto dot = [&](int slot) {   // res[slot] = <psi, w>; the partials reuse the front of scr
+            tdz::launch_multi_dot(c->st, x, N, 1, w, N, scr, res + slot);
+        };
+        mass_apply(c, x, w);
+        dot(0);
+        const cplx a0 = c->alpha, g0 = c->g;
+        const double s0[3] = {c->sE[0], c->sE[1], c->sE[2]}, zero[3] = {0, 0, 0};
+        set_shift(c, 0.0, 1.0, zero);
+        level_apply(c, L, x, w);
+        set_shift(c, a0, g0, s0);
+        dot(1);
+        for (int a = 0; a < c->prob.dim; ++a) {
+            tdz::SepBandsT<double> db = sep_bands<double>(L);
+            for (int d = 0; d < 3; ++d) db.B[d] = d == a ? c->pred.Xband[a] : c->pred.Zband[d];
+            tdz::launch_sep_apply<double>(c->st, db, x, w, c->sd.tmp);
+            if (c->prob.dirichlet) tdz::launch_wall_fix<double>(c->st, L.n, wall, x, w);
+            dot(2 + a);
+        }
+        dcomplex h[kObsDots] = {};
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h, res, (2 + c->prob.dim) * sizeof(dcomplex), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        out[0] = h[0].re;
+        out[1] = h[1].re;
+        out[2] = h[1].im;
+        for (int a = 0; a < 3; ++a) out[3 + a] = a < c->prob.dim ? h[2 + a].re : 0.0;
+    });
+}
+}  // namespace tdz
 
 extern "C" {
 

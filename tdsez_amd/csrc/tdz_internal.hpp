@@ -242,6 +242,14 @@ template <typename R>
 void launch_zline_solve(hipStream_t st, const int n[3], int p, const R *lx, const R *ly, const cpx<R> *Bz,
                         const R *Mw, cpx<R> g, const cpx<R> *rhs, cpx<R> *x, cpx<R> *work);
 
+// ---------------------------------------------------------------- allocation-free entry
+// Reductions of tdz_observables_noalloc: kObsDots dot products of kRedBlocks partials
+// each (one set at a time) and their results.
+constexpr int kObsDots = 8;
+constexpr long noalloc_scratch_len() { return (long)kObsDots * kRedBlocks + kObsDots; }
+// resolve the device address of the static scratch on the current device (once per device)
+void noalloc_init();
+
 // ---------------------------------------------------------------- exact Galerkin potential
 // Per axis: for every (function i, offset d) pair of a coarse level (index (d+p) n + i),
 // the products N_i N_{i+d} at the run of the finest level's Gauss points [q0, q0+len)
