@@ -1330,13 +1330,26 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             const double wn2 = reinterpret_cast<const double *>(hback + 2 * (m + 1))[0];
             for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = cplx(h1[i].re + h2[i].re, h1[i].im + h2[i].im);
             double hn2 = wn2;
-            if (fused_cgs)
-                for (int i = 0; i <= j; ++i) hn2 -= h2[i].re * h2[i].re + h2[i].im * h2[i].im;
+            bool deferred = fused_cgs;
+            if (fused_cgs) {
+                double h2sq = 0;
+                for (int i = 0; i <= j; ++i) h2sq += h2[i].re * h2[i].re + h2[i].im * h2[i].im;
+                if (h2sq > 0.5 * wn2) {
+                    // w1 lies largely in span(V): ||w1||^2 - ||h2||^2 would cancel, so the
+                    // second pass is applied now and the norm taken of its result
+                    tdz::launch_multi_axpy_norm(c->st, c->Vb, N, nv, c->dh2, w, N, c->dpartr, c->dnorm);
+                    HIPCHK(hipMemcpyAsync(&hn2, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->st));
+                    HIPCHK(hipStreamSynchronize(c->st));
+                    deferred = false;
+                } else {
+                    hn2 = wn2 - h2sq;
+                }
+            }
             const double hn = hn2 > 0 ? std::sqrt(hn2) : 0.0;
             H[(size_t)(j + 1) * m + j] = hn;
             last_hn = hn;
-            last_fused = fused_cgs;
-            if (fused_cgs) std::copy(h2, h2 + nv, last_h2.begin());
+            last_fused = deferred;
+            if (deferred) std::copy(h2, h2 + nv, last_h2.begin());
             for (int i = 0; i < j; ++i) {
                 const cplx a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
                 H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
@@ -1356,7 +1369,7 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             if (res <= rtol || hn == 0.0) { ++j; converged = true; break; }
             // V_{j+1} = (w - V h2) / ||w - V h2|| for the next step (a converged solve leaves w
             // as it is: only the residual update below reads it, with its coefficients adjusted)
-            if (fused_cgs) tdz::launch_correct_scale(c->st, c->Vb, N, nv, c->dh2, w, N, 1.0 / hn);
+            if (deferred) tdz::launch_correct_scale(c->st, c->Vb, N, nv, c->dh2, w, N, 1.0 / hn);
             else tdz::launch_scale_invsqrt(c->st, w, c->dnorm, w, N);
         }
         std::vector<cplx> yv(j);

@@ -71,6 +71,12 @@ def test_restart_continues_the_run(tmp_path):
         return math.sqrt(abs(np.vdot(v, mv.download())))
 
     assert l2(got - ref) / l2(ref) < 1e-12
+    # and both are the oracle's CN steps under the same pulse (midpoint fields)
+    from oracle import Oracle
+    o = Oracle(desc)
+    sE = np.array([ctx.coupling(pulse, (s + 0.5) * dt) for s in range(nsteps)])
+    oref, _, _ = o.cn_steps(psi0.download(), dt, sE, rtol=1e-14)
+    assert l2(got - oref) / l2(oref) < 1e-11
     a, b = io.read_trace(full, "energies"), io.read_trace(part, "energies")
     assert a.shape == (5, 7) and b.shape == (2, 7)          # rows at steps 0, 5, 10, 15, 20 / 15, 20
     assert np.allclose(b, a[3:], rtol=1e-11, atol=1e-14)

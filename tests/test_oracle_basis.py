@@ -118,3 +118,44 @@ def test_cap_profile_closed_form(kind, par, mass):
     W = o.band(0, 5)
     assert W.min() > -1e-12 * abs(W).max() or True
     assert np.abs(W).max() > 0
+
+
+def _mano_samples(kmin, mass, E=400):
+    """(relative depth r / c, W / (kmin^2 / 2 m)) at the Gauss points inside the CAP layer
+    of a 1D grid [-10, 10], and the points outside it."""
+    o = _oracle_1d(E, -10.0, 10.0, 3, 8, cap_kind=2, cap=(kmin,), mass=mass)
+    t = o.axis_tables(0)
+    width = 2.62206 / (2 * 0.2 * kmin)
+    s = np.minimum(t["pts"] + 10.0, 10.0 - t["pts"])
+    inside = s < width
+    depth = (width - s[inside]) / width
+    return depth, t["cap"][inside] / (0.5 * kmin ** 2 / mass), t["cap"][~inside]
+
+
+def test_manolopoulos_profile_properties():
+    """Properties of the transmission-free CAP (Manolopoulos, J. Chem. Phys. 117, 9552 (2002))
+    that the formula's text does not state: zero outside the layer of width c/(2 delta kmin),
+    a continuous start with unit slope in the scaled variable r = c depth (W ~ Emin r), one
+    universal shape W / Emin(r) for every kmin and mass (the kmin^2 and 1/m scaling), a
+    monotone rise and a divergence at the wall."""
+    c = 2.62206
+    d1, g1, out1 = _mano_samples(1.2, 1.0)
+    d2, g2, out2 = _mano_samples(2.4, 1.0)
+    d3, g3, _ = _mano_samples(1.2, 1.7)
+    assert np.all(out1 == 0.0) and np.all(out2 == 0.0)
+    # unit slope at onset: W / (Emin r) -> 1
+    near = d1 * c < 0.02
+    assert near.any()
+    assert np.abs(g1[near] / (c * d1[near]) - 1.0).max() < 0.02
+    # 1/m: the same points, the same scaled values
+    assert np.allclose(g3, g1, rtol=1e-13, atol=0)
+    # kmin^2: the scaled profile is one function of the depth (interpolate one sample set)
+    order = np.argsort(d2)
+    mid = (d1 > d2.min()) & (d1 < 0.9)
+    gi = np.interp(d1[mid], d2[order], g2[order])
+    assert np.abs(gi - g1[mid]).max() < 2e-3 * np.abs(g1[mid]).max()
+    # monotone in depth (points at equal depth on the two walls agree), diverging at the
+    # wall (4/(c - r)^2)
+    order1 = np.argsort(d1)
+    assert np.all(np.diff(g1[order1]) >= -1e-12 * g1.max())
+    assert g1.max() > 100.0
