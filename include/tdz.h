@@ -146,6 +146,13 @@ int tdz_level_apply(tdz_ctx *ctx, int level, const double *x, double *y);
  * max_its iterations (max_its < 0: off, the default). Rebuilds are counted in
  * tdz_ksp_stats.pc_rebuilds. */
 int tdz_set_pc_refresh(tdz_ctx *ctx, int max_its);
+/* Multigrid smoother on every level but the coarsest (which keeps its separable sweeps):
+ * kind 0 (default) Richardson with the level's fast-diagonalisation separable solve,
+ * kind 1 Jacobi-Chebyshev of the given degree (degree - 1 level applies per sweep) over
+ * [lmax/30, lmax] of D^-1 M, D = diag of the level operator (the mixed-precision
+ * configuration's "FP32 Chebyshev smoother" under TDZ_PC_MG_MIXED; the reference smooths
+ * with Richardson + restricted ASM/ILU(0), src/multigrid.cpp:303-313). */
+int tdz_mg_set_smoother(tdz_ctx *ctx, int kind, int degree);
 /* Correction sweeps of TDZ_PC_CORRECTED* and TDZ_PC_LEARNED (no reference counterpart):
  * z <- S^-1 (r - dA~ z) applied sweeps times after z = S^-1 r (1..4, default 1), dA~ the
  * terms the separable solve S leaves out (CAP and field exactly, the potential lumped,

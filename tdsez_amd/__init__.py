@@ -35,7 +35,7 @@ EXPORTS = [
     "tdz_trace_row", "tdz_pade_history", "tdz_batch_create", "tdz_batch_context", "tdz_batch_evolve",
     "tdz_batch_fixups", "tdz_batch_destroy", "tdz_ifunction", "tdz_level_dims", "tdz_mg_transfer",
     "tdz_level_apply", "tdz_set_pc_refresh", "tdz_coarsen_knots", "tdz_refinement_matrix", "tdz_last_warning",
-    "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_observables_noalloc", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
+    "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_observables_noalloc", "tdz_mg_set_smoother", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
 ]
 
 
@@ -177,6 +177,7 @@ def lib():
         L.tdz_learn_config.argtypes = [vp, i, C.c_ulonglong]
         L.tdz_cn_predictor_status.argtypes = [vp, ip, dp]
         L.tdz_observables_noalloc.argtypes = [vp, vp, dp]
+        L.tdz_mg_set_smoother.argtypes = [vp, i, i]
         _lib = L
     return _lib
 
@@ -321,6 +322,10 @@ class Context:
 
     def set_corrections(self, sweeps):
         _chk(lib().tdz_pc_set_corrections(self.h, int(sweeps)))
+
+    def set_smoother(self, kind, degree=2):
+        """Multigrid smoother: 0 separable-solve Richardson (default), 1 Jacobi-Chebyshev."""
+        _chk(lib().tdz_mg_set_smoother(self.h, int(kind), int(degree)))
 
     def set_pc_refresh(self, max_its):
         _chk(lib().tdz_set_pc_refresh(self.h, int(max_its)))

@@ -79,6 +79,12 @@ struct Prec {
     R *Mw = nullptr;                         // layered z axis: 1/m-weighted mass band
     cpx<R> *lwork = nullptr;                 // layered z axis: work of the z-line solves
     cpx<R> *r = nullptr, *z = nullptr, *t = nullptr;   // multigrid work vectors
+    // Jacobi-Chebyshev smoother: 1 / diag(A) for the shift of version dinv_ver, the
+    // recurrence's direction, residual, operator product and sum, the lumped finest-level
+    // potential s V(Greville)
+    cpx<R> *dinv = nullptr, *cd = nullptr, *cres = nullptr, *cad = nullptr, *ce = nullptr;
+    R *Vl = nullptr;
+    unsigned dinv_ver = ~0u;
     bool ready = false;
 };
 
@@ -110,6 +116,7 @@ struct Level {
     bool simple = false;    // 3D, simple knots on every axis: the tiled 3D potential kernel
     bool simple2 = false;   // 2D, simple knots: the tiled 2D potential kernel
     int pot_variant = -1;   // tile shape of the tiled kernel picked by the autotuner (-1: generic)
+    double cheb_max = 0;    // Chebyshev smoother: upper eigenvalue bound of D^-1 M (0: not set up)
     // coarse levels: the exact Galerkin potential P^T (s V) P as an offset-major stencil
     double *Sv = nullptr;
     float *Svf = nullptr;
@@ -243,6 +250,9 @@ struct tdz_ctx {
     int stage_base[2] = {-1, -1};   // refresh trackers of the two Pade stages
     bool field_warned = false, cap_warned = false;   // the multigrid warnings were given
     int refresh_cap = -1;
+    int smoother = 0;           // multigrid smoother: 0 separable-solve Richardson, 1 Jacobi-Chebyshev
+    int cheb_degree = 2;
+    unsigned shift_ver = 0;     // incremented by every set_shift
     int corrections = 0;        // correction sweeps of the corrected and learned preconditioners (0: adaptive)
     // adaptive sweeps: the solve's initial relative residual and tolerance (set by FGMRES
     // before its first preconditioner apply), the sweeps chosen for it, and the last
@@ -753,14 +763,115 @@ void transfer(tdz_ctx *c, const CSR ops[3], const int nin[3], const int nout[3],
     tdz::launch_csr_axis<R>(c->st, 2, a, nout[2], ops[2].rp, ops[2].col, csr_val<R>(ops[2]), s2, out, accumulate);
 }
 
+double dev_norm2(tdz_ctx *c, const dcomplex *x, long N);
+
+// ---- Jacobi-Chebyshev smoother (the multigrid smoother of the mixed-precision
+// configuration): a Chebyshev polynomial in D^-1 A, D = diag(A), over the interval
+// [lambda_max / 30, lambda_max] of D^-1 M (M dominates A = M + i dt/2 H at CN steps; the
+// bound comes from a power iteration on the level's mass operator at set-up).
+template <typename R>
+void jacobi_setup(tdz_ctx *c, Level &L)
+{
+    Prec<R> &P = prec<R>(L);
+    const long N = L.N;
+    if (!P.dinv) {
+        P.dinv = track(c, dalloc<cpx<R>>(N));
+        P.cd = track(c, dalloc<cpx<R>>(N));
+        P.cres = track(c, dalloc<cpx<R>>(N));
+        P.cad = track(c, dalloc<cpx<R>>(N));
+        P.ce = track(c, dalloc<cpx<R>>(N));
+    }
+    const bool fine = &L == c->lv[0].get();
+    if (fine && c->ph.pot_kind && !P.Vl) {   // s V at the Greville points (the lumped diagonal)
+        std::vector<double> gr[3];
+        for (int d = 0; d < 3; ++d)
+            gr[d] = d < c->prob.dim ? tdz::greville(L.ax[d].t, L.ax[d].p) : std::vector<double>{0.0};
+        std::vector<R> v(N);
+        for (int k = 0; k < L.n[2]; ++k)
+            for (int j = 0; j < L.n[1]; ++j)
+                for (int i = 0; i < L.n[0]; ++i) {
+                    const double r2 = gr[0][i] * gr[0][i] + gr[1][j] * gr[1][j] + gr[2][k] * gr[2][k];
+                    const double V = c->ph.pot_kind == 1 ? 0.5 * c->ph.pot[0] * r2
+                                                         : -c->ph.pot[0] / std::sqrt(r2 + c->ph.pot[1]);
+                    v[i + (size_t)L.n[0] * (j + (size_t)L.n[1] * k)] = (R)(c->ph.s * V);
+                }
+        P.Vl = upload(c, v);
+    }
+    if (P.dinv_ver == c->shift_ver) return;
+    const R *M[3] = {P.M[0], P.M[1], P.M[2]};
+    const cpx<R> *B[3] = {P.B[0], P.B[1], P.B[2]};
+    const int p[3] = {L.ax[0].p, L.ax[1].p, L.ax[2].p};
+    const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+    const R *Vc = (!fine && c->ph.pot_kind) ? stencil_of<R>(L) : nullptr;
+    tdz::launch_jacobi_diag<R>(c->st, L.n, p, M, B, Vc, fine ? P.Vl : nullptr, cx<R>(c->g), wall, P.dinv);
+    P.dinv_ver = c->shift_ver;
+}
+
+// Upper eigenvalue bound of D_M^-1 M on level L (12 power iterations, FP64, 10% margin).
+double cheb_bound(tdz_ctx *c, Level &L)
+{
+    const long N = L.N;
+    jacobi_setup<double>(c, L);
+    Prec<double> &P = L.d;
+    const double *M[3] = {P.M[0], P.M[1], P.M[2]};
+    const dcomplex *B[3] = {P.Bmass[0], P.Bmass[1], P.Bmass[2]};
+    const int p[3] = {L.ax[0].p, L.ax[1].p, L.ax[2].p};
+    const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+    tdz::launch_jacobi_diag<double>(c->st, L.n, p, M, B, nullptr, nullptr, dcomplex{0, 0}, wall, P.cres);   // 1/diag M
+    std::vector<dcomplex> h(N);
+    for (long i = 0; i < N; ++i) h[i] = {std::sin(0.37 * (double)i + 0.1), std::cos(0.53 * (double)i)};
+    HIPCHK(hipMemcpy(P.cd, h.data(), N * sizeof(dcomplex), hipMemcpyHostToDevice));
+    tdz::SepBandsT<double> mb = sep_bands<double>(L);
+    for (int d = 0; d < 3; ++d) mb.B[d] = P.Bmass[d];
+    double lam = 0;
+    for (int it = 0; it < 12; ++it) {
+        tdz::launch_sep_apply<double>(c->st, mb, P.cd, P.cad, c->sd.tmp);
+        tdz::launch_cheb_update<double>(c->st, N, 0.0, 1.0, P.cres, P.cad, P.ce, nullptr);   // D^-1 M v
+        const double nv = dev_norm2(c, P.cd, N), nw = dev_norm2(c, P.ce, N);
+        lam = std::sqrt(nw / nv);
+        tdz::launch_axpby_t<double>(c->st, {1.0 / std::sqrt(nw), 0}, P.ce, {0, 0}, P.cd, N);
+    }
+    P.dinv_ver = ~0u;   // dinv's storage held 1/diag M above
+    return 1.1 * lam;
+}
+
+// e = p(D^-1 A) D^-1 r, the degree-k Chebyshev approximation of A^-1 r (Saad, Iterative
+// Methods for Sparse Linear Systems, Alg. 12.1, started from zero), k - 1 operator applies.
+template <typename R>
+void chebyshev(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *e)
+{
+    jacobi_setup<R>(c, L);
+    Prec<R> &P = prec<R>(L);
+    const long N = L.N;
+    const double hi = L.cheb_max, lo = hi / 30.0;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    tdz::launch_cheb_update<R>(c->st, N, R(0), R(1.0 / theta), P.dinv, r, P.cd, nullptr);
+    HIPCHK(hipMemcpyAsync(e, P.cd, N * sizeof(cpx<R>), hipMemcpyDeviceToDevice, c->st));
+    const cpx<R> one{1, 0}, mone{-1, 0}, zero{0, 0};
+    for (int k = 1; k < c->cheb_degree; ++k) {
+        level_apply<R>(c, L, P.cd, P.cad);
+        if (k == 1) tdz::launch_axpby_t<R>(c->st, one, r, zero, P.cres, N);   // res = r - A d
+        tdz::launch_axpby_t<R>(c->st, mone, P.cad, one, P.cres, N);
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        tdz::launch_cheb_update<R>(c->st, N, R(rn * rho), R(2.0 * rn / delta), P.dinv, P.cres, P.cd, e);
+        rho = rn;
+    }
+}
+
 // Richardson sweep z += S^-1 (r - A z) on level L (residual in the level's t)
 template <typename R>
-void smooth(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<R> *s2, cpx<R> *s3)
+void smooth(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<R> *s2, cpx<R> *s3, bool cheb)
 {
     cpx<R> *t = prec<R>(L).t;
     const cpx<R> one{1, 0}, mone{-1, 0};
     level_apply<R>(c, L, z, t);
     tdz::launch_axpby_t<R>(c->st, one, r, mone, t, L.N);  // t = r - A z
+    if (cheb) {
+        chebyshev<R>(c, L, t, prec<R>(L).ce);
+        tdz::launch_axpby_t<R>(c->st, one, prec<R>(L).ce, one, z, L.N);
+        return;
+    }
     fd_apply<R>(c, L, t, s3, s1, s2);
     tdz::launch_axpby_t<R>(c->st, one, s3, one, z, L.N);   // z += S^-1 t
 }
@@ -776,10 +887,12 @@ void vcycle(tdz_ctx *c, int l, const cpx<R> *r, cpx<R> *z)
     Scratch<R> &S = scratch<R>(c);
     cpx<R> *s1 = S.w1, *s2 = S.w2;
     const cpx<R> one{1, 0}, mone{-1, 0};
-    fd_apply<R>(c, L, r, z, s1, s2);
     const bool coarsest = l + 1 == (int)c->lv.size();
+    const bool cheb = c->smoother == 1 && !coarsest;   // the coarsest level keeps its separable sweeps
+    if (cheb) chebyshev<R>(c, L, r, z);
+    else fd_apply<R>(c, L, r, z, s1, s2);
     const int pre = coarsest ? nu + 1 : nu;
-    for (int s = 1; s < pre; ++s) smooth<R>(c, L, r, z, s1, s2, S.tmp[3]);
+    for (int s = 1; s < pre; ++s) smooth<R>(c, L, r, z, s1, s2, S.tmp[3], cheb);
     if (coarsest) return;
     Level &Cl = *c->lv[l + 1];
     cpx<R> *t = prec<R>(L).t;
@@ -788,7 +901,7 @@ void vcycle(tdz_ctx *c, int l, const cpx<R> *r, cpx<R> *z)
     transfer<R>(c, L.R, L.n, Cl.n, t, prec<R>(Cl).r, s1, s2, false);       // r_c = P^T t
     vcycle<R>(c, l + 1, prec<R>(Cl).r, prec<R>(Cl).z);
     transfer<R>(c, L.P, Cl.n, L.n, prec<R>(Cl).z, z, s1, s2, true);        // z += P z_c
-    for (int s = 0; s < nu; ++s) smooth<R>(c, L, r, z, s1, s2, S.tmp[3]);
+    for (int s = 0; s < nu; ++s) smooth<R>(c, L, r, z, s1, s2, S.tmp[3], cheb);
 }
 
 // ---- exact Galerkin potential of the coarse levels
@@ -1422,7 +1535,15 @@ void set_shift(tdz_ctx *c, cplx alpha, cplx g, const double sE[3])
     c->alpha = alpha;
     c->g = g;
     for (int a = 0; a < 3; ++a) c->sE[a] = sE ? sE[a] : 0.0;
+    ++c->shift_ver;
     set_shift_bands(c);
+}
+
+void cheb_setup(tdz_ctx *c)
+{
+    if (c->layered) throw Error("the Jacobi-Chebyshev smoother does not take layered structures");
+    for (size_t l = 0; l + 1 < c->lv.size(); ++l)
+        if (c->lv[l]->cheb_max <= 0) c->lv[l]->cheb_max = cheb_bound(c, *c->lv[l]);
 }
 
 void pc_setup(tdz_ctx *c, int kind)
@@ -1436,6 +1557,7 @@ void pc_setup(tdz_ctx *c, int kind)
     if (mg || (kind == TDZ_PC_LEARNED && c->lv.size() >= 2)) ensure_galerkin(c);
     if (mg)
         for (auto &L : c->lv) setup_fd(c, *L);
+    if ((mg || kind == TDZ_PC_LEARNED) && c->smoother == 1) cheb_setup(c);
     else if (kind != TDZ_PC_NONE)
         setup_fd(c, *c->lv[0]);
     const bool corr = kind == TDZ_PC_CORRECTED || kind == TDZ_PC_CORRECTED_EXACT || kind == TDZ_PC_LEARNED;
@@ -1931,6 +2053,19 @@ int tdz_pc_set_corrections(tdz_ctx *c, int sweeps)
     return guarded([&] {
         if (sweeps < 1 || sweeps > 4) throw Error("correction sweeps must be 1..4");
         c->corrections = sweeps;
+    });
+}
+
+int tdz_mg_set_smoother(tdz_ctx *c, int kind, int degree)
+{
+    return guarded([&] {
+        if (kind != 0 && kind != 1) throw Error("smoother kind must be 0 (separable solve) or 1 (Jacobi-Chebyshev)");
+        if (degree < 1 || degree > 8) throw Error("Chebyshev degree must be 1..8");
+        c->smoother = kind;
+        c->cheb_degree = degree;
+        if (kind == 1 && c->pc_ready &&
+            (c->pc_kind == TDZ_PC_MG || c->pc_kind == TDZ_PC_MG_MIXED || c->pc_kind == TDZ_PC_LEARNED))
+            cheb_setup(c);
     });
 }
 

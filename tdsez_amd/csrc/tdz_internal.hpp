@@ -228,6 +228,16 @@ template <typename R>
 void launch_diag_scale(hipStream_t st, const int n[3], const R *lam[3], const unsigned char *wall[3], cpx<R> alpha,
                        cpx<R> g, const cpx<R> *x, cpx<R> *y);
 
+// Jacobi-Chebyshev smoother: dinv = 1 / diag(A) of a level (Kronecker bands, plus g times
+// the potential diagonal from the Galerkin stencil Vc or the lumped values Vl times diag M;
+// walls: 1), and the recurrence update d = a d + b dinv .* res, e += d (e may be null).
+template <typename R>
+void launch_jacobi_diag(hipStream_t st, const int n[3], const int p[3], const R *M[3], const cpx<R> *B[3],
+                        const R *Vc, const R *Vl, cpx<R> g, const unsigned char *wall[3], cpx<R> *dinv);
+template <typename R>
+void launch_cheb_update(hipStream_t st, long N, R a, R b, const cpx<R> *dinv, const cpx<R> *res, cpx<R> *d,
+                        cpx<R> *e);
+
 // Sparse 1D transfer along axis d: out (axis d of size m) = A in (axis d of size nin),
 // A in CSR (rowptr[m+1], col, val); other axes unchanged. accumulate: out += A in.
 template <typename R>

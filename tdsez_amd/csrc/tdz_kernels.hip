@@ -516,6 +516,70 @@ __global__ void pot3d_sum_kernel(int n0, int n1, int n2, int TE, int TW, int nbx
     }
 }
 
+// ------------------------------------------------------------------ Jacobi-Chebyshev smoother
+// dinv[I] = 1 / diag(A)[I], A = the level operator with its current shift: the Kronecker
+// terms' diagonal from the band diagonals (B_d complex, M_d real), plus g times the
+// potential's diagonal (Vc: the Galerkin stencil's centre entries, offset-major with
+// half-widths p; or Vl: s V at the Greville points times diag(M), the finest level's
+// lumped value; or none). Wall rows are the identity.
+template <typename R>
+__global__ void jacobi_diag_kernel(int n0, int n1, int n2, int p0, int p1, int p2, const R *__restrict__ M0,
+                                   const R *__restrict__ M1, const R *__restrict__ M2, const cpx<R> *__restrict__ B0,
+                                   const cpx<R> *__restrict__ B1, const cpx<R> *__restrict__ B2,
+                                   const R *__restrict__ Vc, const R *__restrict__ Vl, R gr, R gi,
+                                   const unsigned char *__restrict__ w0, const unsigned char *__restrict__ w1,
+                                   const unsigned char *__restrict__ w2, cpx<R> *__restrict__ dinv)
+{
+    const long N = (long)n0 * n1 * n2;
+    for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < N; I += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(I % n0), j = (int)((I / n0) % n1), k = (int)(I / ((long)n0 * n1));
+        if (w0[i] || w1[j] || w2[k]) {
+            dinv[I] = {1, 0};
+            continue;
+        }
+        const R mx = M0[(long)i * (2 * p0 + 1) + p0], my = M1[(long)j * (2 * p1 + 1) + p1],
+                mz = M2[(long)k * (2 * p2 + 1) + p2];
+        const cpx<R> bx = B0[(long)i * (2 * p0 + 1) + p0], by = B1[(long)j * (2 * p1 + 1) + p1],
+                     bz = B2[(long)k * (2 * p2 + 1) + p2];
+        R dr = bx.re * my * mz + mx * by.re * mz + mx * my * bz.re;
+        R di = bx.im * my * mz + mx * by.im * mz + mx * my * bz.im;
+        R v = 0;
+        if (Vc) {   // centre entry (offsets p0, p1, p2) of the offset-major stencil
+            const long D0 = 2 * p0 + 1, D1 = 2 * p1 + 1;
+            v = Vc[((((long)p2 * n2 + k) * D1 + p1) * n1 + j) * D0 * n0 + (long)p0 * n0 + i];
+        } else if (Vl) {
+            v = Vl[I] * mx * my * mz;
+        }
+        dr += gr * v;
+        di += gi * v;
+        const R den = dr * dr + di * di;
+        dinv[I] = {dr / den, -di / den};
+    }
+}
+
+// d = a d + b (dinv .* res) (a = 0: d is written without being read); e += d when e
+template <typename R>
+__global__ void cheb_update_kernel(long N, R a, R b, const cpx<R> *__restrict__ dinv, const cpx<R> *__restrict__ res,
+                                   cpx<R> *__restrict__ d, cpx<R> *__restrict__ e)
+{
+    for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < N; I += (long)gridDim.x * blockDim.x) {
+        const cpx<R> q = dinv[I], r = res[I];
+        cpx<R> v = {b * (q.re * r.re - q.im * r.im), b * (q.re * r.im + q.im * r.re)};
+        if (a != R(0)) {
+            const cpx<R> o = d[I];
+            v.re += a * o.re;
+            v.im += a * o.im;
+        }
+        d[I] = v;
+        if (e) {
+            cpx<R> t = e[I];
+            t.re += v.re;
+            t.im += v.im;
+            e[I] = t;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ separable
 // x pass: t = Mx x, u = Bx x (one thread per output)
 template <typename R>
@@ -2308,7 +2372,29 @@ void launch_zline_solve(hipStream_t st, const int n[3], int p, const R *lx, cons
                        work);
 }
 
+template <typename R>
+void launch_jacobi_diag(hipStream_t st, const int n[3], const int p[3], const R *M[3], const cpx<R> *B[3],
+                        const R *Vc, const R *Vl, cpx<R> g, const unsigned char *wall[3], cpx<R> *dinv)
+{
+    const long N = (long)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(jacobi_diag_kernel<R>, dim3((unsigned)std::min<long>((N + 255) / 256, 1L << 20)), dim3(256), 0,
+                       st, n[0], n[1], n[2], p[0], p[1], p[2], M[0], M[1], M[2], B[0], B[1], B[2], Vc, Vl, g.re, g.im,
+                       wall[0], wall[1], wall[2], dinv);
+}
+
+template <typename R>
+void launch_cheb_update(hipStream_t st, long N, R a, R b, const cpx<R> *dinv, const cpx<R> *res, cpx<R> *d,
+                        cpx<R> *e)
+{
+    hipLaunchKernelGGL(cheb_update_kernel<R>, dim3((unsigned)std::min<long>((N + 255) / 256, 1L << 20)), dim3(256), 0,
+                       st, N, a, b, dinv, res, d, e);
+}
+
 #define TDZ_INSTANTIATE(R)                                                                                          \
+    template void launch_jacobi_diag<R>(hipStream_t, const int[3], const int[3], const R *[3], const cpx<R> *[3],   \
+                                        const R *, const R *, cpx<R>, const unsigned char *[3], cpx<R> *);         \
+    template void launch_cheb_update<R>(hipStream_t, long, R, R, const cpx<R> *, const cpx<R> *, cpx<R> *,         \
+                                        cpx<R> *);                                                                 \
     template void launch_pot_apply_generic<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,  \
                                               cpx<R> *);                                                           \
     template bool launch_pot_apply_3d<R>(hipStream_t, const AxisDevT<R>[3], PotParams, R, R, const cpx<R> *,       \
