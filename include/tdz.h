@@ -153,6 +153,19 @@ int tdz_set_pc_refresh(tdz_ctx *ctx, int max_its);
  * configuration's "FP32 Chebyshev smoother" under TDZ_PC_MG_MIXED; the reference smooths
  * with Richardson + restricted ASM/ILU(0), src/multigrid.cpp:303-313). */
 int tdz_mg_set_smoother(tdz_ctx *ctx, int kind, int degree);
+/* Coarsest-level solve of the multigrid preconditioners (the reference: KSPPREONLY with an
+ * LU factorisation, PCLU/MUMPS, src/multigrid.cpp:314-325): kind 0 (default) the separable
+ * solve followed by MGSmooth Richardson sweeps with it; kind 1 a Krylov solve to the relative residual rtol
+ * (GMRES with the separable solve as right preconditioner, restarted every min(8, maxit)
+ * iterations, at most maxit iterations rounded up to whole cycles, in FP64 also under
+ * TDZ_PC_MG_MIXED). tdz_mg_coarse_status: the iterations
+ * and residual estimate of the last Krylov coarse solve. */
+int tdz_mg_set_coarse(tdz_ctx *ctx, int kind, double rtol, int maxit);
+int tdz_mg_coarse_status(const tdz_ctx *ctx, int *its, double *res);
+/* x = the coarse solve of the multigrid preconditioners applied to b on the coarsest level
+ * (level = tdz_levels - 1 >= 1; PCMGGetCoarseSolve + KSPSolve on it), with the current
+ * shift and the kind tdz_mg_set_coarse chose. Device vectors of the level's size. */
+int tdz_level_solve(tdz_ctx *ctx, int level, const double *b, double *x);
 /* Correction sweeps of TDZ_PC_CORRECTED* and TDZ_PC_LEARNED (no reference counterpart):
  * z <- S^-1 (r - dA~ z) applied sweeps times after z = S^-1 r (1..4, default 1), dA~ the
  * terms the separable solve S leaves out (CAP and field exactly, the potential lumped,

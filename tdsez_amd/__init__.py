@@ -35,7 +35,7 @@ EXPORTS = [
     "tdz_trace_row", "tdz_pade_history", "tdz_batch_create", "tdz_batch_context", "tdz_batch_evolve",
     "tdz_batch_fixups", "tdz_batch_destroy", "tdz_ifunction", "tdz_level_dims", "tdz_mg_transfer",
     "tdz_level_apply", "tdz_set_pc_refresh", "tdz_coarsen_knots", "tdz_refinement_matrix", "tdz_last_warning",
-    "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_observables_noalloc", "tdz_mg_set_smoother", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
+    "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_observables_noalloc", "tdz_mg_set_smoother", "tdz_mg_set_coarse", "tdz_mg_coarse_status", "tdz_level_solve", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
 ]
 
 
@@ -178,6 +178,9 @@ def lib():
         L.tdz_cn_predictor_status.argtypes = [vp, ip, dp]
         L.tdz_observables_noalloc.argtypes = [vp, vp, dp]
         L.tdz_mg_set_smoother.argtypes = [vp, i, i]
+        L.tdz_mg_set_coarse.argtypes = [vp, i, C.c_double, i]
+        L.tdz_mg_coarse_status.argtypes = [vp, ip, dp]
+        L.tdz_level_solve.argtypes = [vp, i, vp, vp]
         _lib = L
     return _lib
 
@@ -326,6 +329,20 @@ class Context:
     def set_smoother(self, kind, degree=2):
         """Multigrid smoother: 0 separable-solve Richardson (default), 1 Jacobi-Chebyshev."""
         _chk(lib().tdz_mg_set_smoother(self.h, int(kind), int(degree)))
+
+    def set_coarse_solve(self, kind, rtol=1e-12, maxit=64):
+        """Coarsest multigrid level: 0 separable sweeps (default), 1 Krylov solve to rtol."""
+        _chk(lib().tdz_mg_set_coarse(self.h, int(kind), float(rtol), int(maxit)))
+
+    def level_solve(self, level, b, x):
+        """x = the coarse solve applied to b on the coarsest level (device vectors)."""
+        _chk(lib().tdz_level_solve(self.h, int(level), b.ptr, x.ptr))
+
+    def coarse_status(self):
+        """(iterations, residual estimate) of the last Krylov coarse solve."""
+        its, res = C.c_int(0), C.c_double(0.0)
+        _chk(lib().tdz_mg_coarse_status(self.h, C.byref(its), C.byref(res)))
+        return its.value, res.value
 
     def set_pc_refresh(self, max_its):
         _chk(lib().tdz_set_pc_refresh(self.h, int(max_its)))

@@ -252,6 +252,17 @@ struct tdz_ctx {
     int refresh_cap = -1;
     int smoother = 0;           // multigrid smoother: 0 separable-solve Richardson, 1 Jacobi-Chebyshev
     int cheb_degree = 2;
+    // coarsest multigrid level: 0 separable-solve sweeps, 1 Krylov solve to coarse_rtol
+    int coarse_kind = 0, coarse_maxit = 64;
+    double coarse_rtol = 1e-12;
+    struct CoarseKrylov {       // workspace of the coarse Krylov solve (allocated on first use)
+        dcomplex *V = nullptr, *Z = nullptr, *dpart = nullptr, *dh = nullptr, *y = nullptr, *b = nullptr, *x = nullptr;
+        double *dpartr = nullptr, *bnorm2 = nullptr, *res = nullptr;
+        int *step = nullptr;
+        tdz::GmState *gm = nullptr;
+        int last_its = 0;
+        double last_res = 0;
+    } ck;
     unsigned shift_ver = 0;     // incremented by every set_shift
     int corrections = 0;        // correction sweeps of the corrected and learned preconditioners (0: adaptive)
     // adaptive sweeps: the solve's initial relative residual and tolerance (set by FGMRES
@@ -876,6 +887,76 @@ void smooth(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<R>
     tdz::launch_axpby_t<R>(c->st, one, s3, one, z, L.N);   // z += S^-1 t
 }
 
+// Coarsest-level solve z = A_L^-1 r to the relative residual c->coarse_rtol: GMRES with the
+// level's separable solve as right preconditioner, restarted every min(kGmMax, maxit)
+// iterations, at most maxit iterations (rounded up to whole cycles), from z = 0. The Hessenberg least-squares problem stays on the device (gm_* kernels); the host
+// reads the residual estimate once per cycle. This takes the place of the reference's exact
+// coarse factorisation (KSPPREONLY + PCLU/MUMPS, src/multigrid.cpp:314-325): a dense or
+// banded factor of the 36^3-function coarsest level of a 128^3 grid would hold 35 GB (dense)
+// or 12 GB (band) and cost a pass over them per V-cycle, where a Krylov solve costs a few
+// applies of the level's 729-point operator.
+void coarse_krylov(tdz_ctx *c, Level &L, const dcomplex *r, dcomplex *z)
+{
+    constexpr int km = tdz::kGmMax;
+    const int k = std::min(km, std::max(c->coarse_maxit, 1));   // iterations per cycle
+    const long N = L.N;
+    tdz_ctx::CoarseKrylov &K = c->ck;
+    if (!K.V) {
+        K.V = track(c, dalloc<dcomplex>((size_t)(km + 1) * N));
+        K.Z = track(c, dalloc<dcomplex>((size_t)km * N));
+        K.b = track(c, dalloc<dcomplex>(N));
+        K.dpart = track(c, dalloc<dcomplex>((size_t)(km + 1) * tdz::kRedBlocks));
+        K.dh = track(c, dalloc<dcomplex>(2 * (km + 1) + 1));
+        K.y = track(c, dalloc<dcomplex>(km));
+        K.dpartr = track(c, dalloc<double>(tdz::kRedBlocks));
+        K.bnorm2 = track(c, dalloc<double>(1));
+        K.res = track(c, dalloc<double>(1));
+        K.step = track(c, dalloc<int>(1));
+        K.gm = track(c, dalloc<tdz::GmState>(1));
+        HIPCHK(hipMemsetAsync(K.step, 0, sizeof(int), c->st));
+    }
+    dcomplex *dh2 = K.dh + (km + 1);
+    double *dnorm = reinterpret_cast<double *>(K.dh + 2 * (km + 1));
+    Scratch<double> &S = scratch<double>(c);
+    const dcomplex one{1, 0}, mone{-1, 0};
+    HIPCHK(hipMemcpyAsync(K.b, r, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemsetAsync(z, 0, N * sizeof(dcomplex), c->st));
+    tdz::launch_norm2(c->st, K.b, N, K.dpartr, K.bnorm2);
+    int its = 0;
+    double res = 1.0;
+    for (int cycle = 0; its < std::max(c->coarse_maxit, 1); ++cycle) {
+        dcomplex *V0 = K.V;
+        if (cycle == 0) {
+            HIPCHK(hipMemcpyAsync(V0, K.b, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
+        } else {
+            level_apply<double>(c, L, z, V0);
+            tdz::launch_axpby_t<double>(c->st, one, K.b, mone, V0, N);   // V0 = b - A z
+        }
+        tdz::launch_norm2(c->st, V0, N, K.dpartr, dnorm);
+        tdz::launch_gm_start(c->st, dnorm, K.bnorm2, K.gm);
+        tdz::launch_scale_invsqrt(c->st, V0, dnorm, V0, N);
+        for (int j = 0; j < k; ++j) {
+            dcomplex *Vj = K.V + (long)j * N, *Zj = K.Z + (long)j * N, *w = K.V + (long)(j + 1) * N;
+            fd_apply<double>(c, L, Vj, Zj, S.w1, S.w2);
+            level_apply<double>(c, L, Zj, w);
+            tdz::launch_multi_dot(c->st, K.V, N, j + 1, w, N, K.dpart, K.dh);
+            tdz::launch_multi_axpy_norm(c->st, K.V, N, j + 1, K.dh, w, N, K.dpartr, dnorm);
+            tdz::launch_multi_dot(c->st, K.V, N, j + 1, w, N, K.dpart, dh2);
+            tdz::launch_multi_axpy_norm(c->st, K.V, N, j + 1, dh2, w, N, K.dpartr, dnorm);
+            tdz::launch_gm_arnoldi(c->st, j, k, K.dh, dh2, dnorm, K.gm);
+            tdz::launch_scale_invsqrt(c->st, w, dnorm, w, N);
+        }
+        tdz::launch_gm_solve(c->st, k, k, K.gm, K.y, K.res, K.step);
+        tdz::launch_multi_axpy(c->st, K.Z, N, k, K.y, z, N);
+        its += k;
+        HIPCHK(hipMemcpyAsync(&res, K.res, sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (!(res > c->coarse_rtol)) break;
+    }
+    K.last_its = its;
+    K.last_res = res;
+}
+
 // V-cycle z = B_l r on level l (z is overwritten), in precision R.
 template <typename R>
 void vcycle(tdz_ctx *c, int l, const cpx<R> *r, cpx<R> *z)
@@ -889,6 +970,16 @@ void vcycle(tdz_ctx *c, int l, const cpx<R> *r, cpx<R> *z)
     const cpx<R> one{1, 0}, mone{-1, 0};
     const bool coarsest = l + 1 == (int)c->lv.size();
     const bool cheb = c->smoother == 1 && !coarsest;   // the coarsest level keeps its separable sweeps
+    if (coarsest && l > 0 && c->coarse_kind == 1) {    // Krylov solve, in FP64 for either precision
+        if constexpr (std::is_same<R, double>::value) {
+            coarse_krylov(c, L, r, z);
+        } else {
+            tdz::launch_promote(c->st, r, L.d.z, L.N);
+            coarse_krylov(c, L, L.d.z, L.d.r);
+            tdz::launch_demote(c->st, L.d.r, z, L.N);
+        }
+        return;
+    }
     if (cheb) chebyshev<R>(c, L, r, z);
     else fd_apply<R>(c, L, r, z, s1, s2);
     const int pre = coarsest ? nu + 1 : nu;
@@ -2048,6 +2139,21 @@ int tdz_level_apply(tdz_ctx *c, int level, const double *x, double *y)
     });
 }
 
+int tdz_level_solve(tdz_ctx *c, int level, const double *b, double *x)
+{
+    return guarded([&] {
+        const int last = (int)c->lv.size() - 1;
+        if (level != last || level < 1) throw Error("the level solve takes the coarsest level of a hierarchy");
+        Level &L = *c->lv[level];
+        ensure_galerkin(c);
+        setup_fd(c, L);
+        refresh_coarse(c);
+        vcycle<double>(c, level, reinterpret_cast<const dcomplex *>(b), reinterpret_cast<dcomplex *>(x));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
 int tdz_pc_set_corrections(tdz_ctx *c, int sweeps)
 {
     return guarded([&] {
@@ -2066,6 +2172,28 @@ int tdz_mg_set_smoother(tdz_ctx *c, int kind, int degree)
         if (kind == 1 && c->pc_ready &&
             (c->pc_kind == TDZ_PC_MG || c->pc_kind == TDZ_PC_MG_MIXED || c->pc_kind == TDZ_PC_LEARNED))
             cheb_setup(c);
+    });
+}
+
+int tdz_mg_set_coarse(tdz_ctx *c, int kind, double rtol, int maxit)
+{
+    return guarded([&] {
+        if (kind != 0 && kind != 1) throw Error("coarse solve kind must be 0 (separable sweeps) or 1 (Krylov)");
+        if (kind == 1 && !(rtol > 0 && rtol < 1)) throw Error("coarse solve rtol must be in (0, 1)");
+        if (kind == 1 && (maxit < 1 || maxit > 4096)) throw Error("coarse solve maxit must be 1..4096");
+        c->coarse_kind = kind;
+        if (kind == 1) {
+            c->coarse_rtol = rtol;
+            c->coarse_maxit = maxit;
+        }
+    });
+}
+
+int tdz_mg_coarse_status(const tdz_ctx *c, int *its, double *res)
+{
+    return guarded([&] {
+        if (its) *its = c->ck.last_its;
+        if (res) *res = c->ck.last_res;
     });
 }
 
