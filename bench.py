@@ -163,6 +163,9 @@ class Atom:
         el = time.perf_counter() - t1
         kms, nl = ctx.timing_stats() if timing else (0.0, 0)
         ctx.timing(False)
+        if leg == "learned":
+            _, fallbacks, active = ctx.learn_status()
+            self.learned_status = {"fallbacks": fallbacks, "network_active": bool(active)}
         out = d.download()
         d.free()
         return nsteps / el, its, out, kms, nl
@@ -251,8 +254,7 @@ def atom(args):
     main_win = windows[-1]
     sps = res[main_key][main_win][legs[0] if "learned" not in legs else "learned"]["steps_per_s"]
     if A.learn is not None:
-        _, fallbacks, active = ctx.learn_status()
-        A.learn.update({"fallbacks": fallbacks, "network_active": bool(active)})
+        A.learn.update(getattr(A, "learned_status", {}))
 
     p, Q, nel = A.desc["p"], A.desc["nquad"], args.nel
     flops = pot_flops_per_element(p, Q) * nel ** 3

@@ -267,11 +267,12 @@ struct tdz_ctx {
     int corrections = 0;        // correction sweeps of the corrected and learned preconditioners (0: adaptive)
     // adaptive sweeps: the solve's initial relative residual and tolerance (set by FGMRES
     // before its first preconditioner apply), the sweeps chosen for it, and the last
-    // observed first-iteration contraction per sweep count
+    // observed mean contraction per iteration of each choice
     double pc_r0rel = -1, pc_rtol = 0;
     bool pc_choose = false;
     int pc_sweeps = 1;
-    double rho_hat[5] = {-1, -1, -1, -1, -1};
+    bool pc_net = true;        // the learned preconditioner runs its network (in the last sweep)
+    double rho_hat[2][5] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};   // [network][sweeps]
     long pc_solves = 0;       // rebuild whenever a solve takes more iterations (-tdsez_pc_refresh_its); -1: off
     bool cap_on = true;         // the CAP is part of H (off during the eigensolve, as src/propagator.cpp:51-69)
     // Pade(2,2) stage guesses: per slot (2 x stage) the last increments y - x
@@ -1234,40 +1235,68 @@ void random_samples(tdz_ctx *c, int count, unsigned long long seed)
     }
 }
 
-// Sweeps for the next solve of the corrected preconditioners, by expected cost: k sweeps
-// contract the residual by rho_k per Krylov iteration, so the solve needs about
-// its_k = ceil(log(tol / r0) / log(rho_k)) iterations (tol = half the relative tolerance,
-// r0 the solve's initial relative residual), each costing an operator apply and the
-// Krylov work (about 2.5 units of 1 ms at 128^3) plus the preconditioner (0.17 units per
-// separable solve, 0.1 per correction, 0.19 for the network). rho_k is the last
-// observed first-iteration contraction with k sweeps; an unobserved count is estimated
-// from an observed one as rho_j = rho_k^((j+1)/(k+1)) (the error after k sweeps is
-// (S^-1 dA)^(k+1)). Every 32nd solve that picks k > 1 tries k - 1 instead, so that a
-// weakening field is noticed.
-int choose_sweeps(tdz_ctx *c)
+// Sweeps (and, for the learned preconditioner, whether the network runs in the last one)
+// for the next solve of the corrected preconditioners, by expected cost: k sweeps contract
+// the residual by rho per Krylov iteration, so the solve needs about
+// its = ceil(log(tol / r0) / log(rho)) iterations (tol the relative tolerance, r0 the
+// solve's initial relative residual), each costing an operator apply and the Krylov
+// work (about 2.5 units of 1 ms at 128^3) plus the preconditioner (0.17 units per separable
+// solve, 0.07 per correction, 0.19 for the network). rho is the mean contraction per
+// iteration, (final / initial residual)^(1/its), of the last solve that made that choice;
+// an unobserved sweep count is estimated from
+// an observed one of the same choice as rho_j = rho_k^((j+1)/(k+1)) (the error after k
+// sweeps is (S^-1 dA)^(k+1)), and an unobserved network choice from the other one (the
+// network assumed to gain half an order per iteration, so that it is tried first). Every
+// 32nd solve that picks k > 1 tries k - 1 instead, and every 32nd (offset 16) the other
+// network choice, so that estimates follow the field.
+void choose_sweeps(tdz_ctx *c)
 {
     constexpr int kMaxSweeps = 3;
-    if (c->pc_r0rel <= 0) return 1;
-    int ref = -1;
-    for (int k = 1; k <= kMaxSweeps; ++k)
-        if (c->rho_hat[k] > 0 && (ref < 0 || k < ref)) ref = k;
-    const double target = 0.5 * c->pc_rtol / c->pc_r0rel;
-    const bool net = c->pc_kind == TDZ_PC_LEARNED;
-    int best = 1;
-    double best_cost = 1e300;
-    for (int k = 1; k <= kMaxSweeps; ++k) {
-        double rho = c->rho_hat[k];
-        if (!(rho > 0)) rho = ref > 0 ? std::pow(c->rho_hat[ref], (k + 1.0) / (ref + 1.0)) : std::pow(1e-2, k + 1.0);
-        rho = std::min(rho, 0.5);
-        const double its = target >= 1.0 ? 1.0 : std::max(1.0, std::ceil(std::log(target) / std::log(rho) - 1e-9));
-        const double cost = its * (2.5 + 0.17 * (k + 1) + 0.1 * k + (net ? 0.19 : 0.0));
-        if (cost < best_cost * (1 - 1e-9)) {
-            best_cost = cost;
-            best = k;
+    const bool can_net = c->pc_kind == TDZ_PC_LEARNED && c->learn.trained;
+    c->pc_sweeps = 1;
+    c->pc_net = can_net;
+    if (c->pc_r0rel <= 0) return;
+    const double target = c->pc_rtol / c->pc_r0rel;
+    auto observed_ref = [&](int net) {
+        int ref = -1;
+        for (int k = 1; k <= kMaxSweeps; ++k)
+            if (c->rho_hat[net][k] > 0 && (ref < 0 || k < ref)) ref = k;
+        return ref;
+    };
+    auto estimate = [&](int net, int k) {
+        if (c->rho_hat[net][k] > 0) return c->rho_hat[net][k];
+        const int ref = observed_ref(net);
+        if (ref > 0) return std::pow(c->rho_hat[net][ref], (k + 1.0) / (ref + 1.0));
+        const int other = 1 - net, oref = observed_ref(other);
+        const double po = net ? 1.5 : 1.0 / 1.5;
+        if (oref > 0) {
+            const double ro = c->rho_hat[other][k] > 0 ? c->rho_hat[other][k]
+                                                       : std::pow(c->rho_hat[other][oref], (k + 1.0) / (oref + 1.0));
+            return std::pow(ro, po);
         }
-    }
-    if (best > 1 && ++c->pc_solves % 32 == 0) --best;
-    return best;
+        return std::pow(1e-2, (k + 1.0) * (net ? 1.5 : 1.0));
+    };
+    int best = 1, best_net = can_net ? 1 : 0;
+    double best_cost = 1e300;
+    for (int net = can_net ? 1 : 0; net >= 0; --net)
+        for (int k = 1; k <= kMaxSweeps; ++k) {
+            const double rho = std::min(estimate(net, k), 0.5);
+            // a solve that met the tolerance at its contraction is counted as such (-0.05: the
+            // observed mean contraction of an its-iteration solve reproduces its iterations)
+            const double its =
+                target >= 1.0 ? 1.0 : std::max(1.0, std::ceil(std::log(target) / std::log(rho) - 0.05));
+            const double cost = its * (2.5 + 0.17 * (k + 1) + 0.07 * k + (net ? 0.19 : 0.0));
+            if (cost < best_cost * (1 - 1e-9)) {
+                best_cost = cost;
+                best = k;
+                best_net = net;
+            }
+        }
+    ++c->pc_solves;
+    if (best > 1 && c->pc_solves % 32 == 0) --best;
+    if (can_net && c->pc_solves % 32 == 16) best_net = 1 - best_net;
+    c->pc_sweeps = best;
+    c->pc_net = best_net == 1;
 }
 
 // the current preconditioner is a corrected solve with adaptive sweeps
@@ -1277,13 +1306,14 @@ bool sweeps_apply(const tdz_ctx *c)
            (c->pc_kind == TDZ_PC_LEARNED && !c->learn.fallen);
 }
 
-// Record the first Arnoldi step's contraction |g_1| / beta of a solve that used k sweeps;
-// counts below k are at least as bad as the model says.
-void observe_sweeps(tdz_ctx *c, int k, double rho)
+// Record the mean contraction per iteration of a solve that used k sweeps (with or without
+// the network); counts below k are at least as bad as the model says.
+void observe_sweeps(tdz_ctx *c, int k, bool net, double rho)
 {
     if (k < 1 || k > 3 || !(rho > 0)) return;
-    c->rho_hat[k] = rho;
-    for (int j = 1; j < k; ++j) c->rho_hat[j] = std::max(c->rho_hat[j], std::pow(rho, (j + 1.0) / (k + 1.0)));
+    double *r = c->rho_hat[net ? 1 : 0];
+    r[k] = rho;
+    for (int j = 1; j < k; ++j) r[j] = std::max(r[j], std::pow(rho, (j + 1.0) / (k + 1.0)));
 }
 
 void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = false, bool use_net = false)
@@ -1298,7 +1328,7 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = 
     }
     LearnState &ls = c->learn;
     if (c->pc_choose) {
-        c->pc_sweeps = choose_sweeps(c);
+        choose_sweeps(c);
         c->pc_choose = false;
     }
     const int sweeps = c->corrections > 0 ? c->corrections : c->pc_sweeps;
@@ -1307,11 +1337,14 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = 
     // ~1e-4 of r at dt = 0.02; with the network's remainder far less); an earlier sweep's
     // miss enters multiplied by S^-1 dA, so the network runs in the last sweep only
     for (int k = 0; k < sweeps; ++k) {
-        const bool net = use_net && ls.trained && k == sweeps - 1;
+        const bool net = use_net && ls.trained && k == sweeps - 1 && (c->corrections > 0 || c->pc_net);
         const bool harvest = k == 0 && ls.harvesting && (int)ls.ys.size() < ls.max_samples && c->ph.pot_kind;
-        if (harvest || net) {   // M z, g (CAP + sE.D) z and the network input in three passes
+        if (harvest || net || ls.ready) {
+            // M z, g (CAP + sE.D) z and, for the network, its input, in one z-marching pass
+            // (equal orders in 3D; three passes otherwise)
             fcomplex *tmp[7] = {c->sf.tmp[0], c->sf.tmp[1], c->sf.tmp[2], c->sf.tmp[3], c->sf.w1, c->sf.w2, c->sf.u2};
-            tdz::fused_products(c->st, fused_bands(c), ls.sin, L.f.z, tmp, L.f.t, c->sf.u1, ls.feat);
+            tdz::fused_products(c->st, fused_bands(c), ls.sin, L.f.z, tmp, L.f.t, c->sf.u1,
+                                harvest || net ? ls.feat : nullptr);
         } else {
             tdz::SepBandsT<float> mb = sep_bands<float>(L);
             for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
@@ -1469,6 +1502,7 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
     std::vector<dcomplex> last_h2(tdz::kCgsFusedMax);
     int its = 0, restarts = 0;
     double res = 0;
+    bool chose = false;   // this solve's sweeps were chosen by the cost model
     if (bnorm == 0.0) {
         HIPCHK(hipMemsetAsync(x, 0, N * sizeof(dcomplex), c->st));
         if (ax_out) HIPCHK(hipMemsetAsync(ax_out, 0, N * sizeof(dcomplex), c->st));
@@ -1501,6 +1535,7 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             c->pc_r0rel = res;
             c->pc_rtol = rtol;
             c->pc_choose = true;
+            chose = true;
         }
         ++restarts;
         tdz::launch_axpby(c->st, {1.0 / beta, 0}, Vj(0), {0, 0}, Vj(0), N);
@@ -1569,7 +1604,6 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             gv[j] = cs[j] * gv[j];
             ++its;
             res = std::abs(gv[j + 1]) / bnorm;
-            if (adapt && j == 0) observe_sweeps(c, c->pc_sweeps, std::abs(gv[1]) / beta);
             if (res <= rtol || hn == 0.0) { ++j; converged = true; break; }
             // V_{j+1} = (w - V h2) / ||w - V h2|| for the next step (a converged solve leaves w
             // as it is: only the residual update below reads it, with its coefficients adjusted)
@@ -1609,6 +1643,8 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
         if (with_ax) tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N, b);   // ax_out = b + sum
         if (converged || its >= maxit) break;
     }
+    if (chose && converged && its > 0 && c->pc_r0rel > 0 && res > 0)   // mean contraction per iteration
+        observe_sweeps(c, c->pc_sweeps, c->pc_net, std::pow(std::min(res / c->pc_r0rel, 1.0), 1.0 / its));
     if (st) *st = {its, restarts, res, bnorm, c->rebuilds};
     if (!converged) {
         if (allow_fail) return -1;
@@ -1645,6 +1681,9 @@ void pc_setup(tdz_ctx *c, int kind)
     if (c->layered && (kind == TDZ_PC_CORRECTED || kind == TDZ_PC_CORRECTED_EXACT || kind == TDZ_PC_LEARNED))
         throw Error("a layered structure is separable: its separable solve is exact (no correction to learn)");
     c->pc_kind = kind;
+    for (auto &row : c->rho_hat) std::fill(std::begin(row), std::end(row), -1.0);   // new contraction estimates
+    c->pc_sweeps = 1;
+    c->pc_net = true;
     if (mg || (kind == TDZ_PC_LEARNED && c->lv.size() >= 2)) ensure_galerkin(c);
     if (mg)
         for (auto &L : c->lv) setup_fd(c, *L);
