@@ -166,6 +166,11 @@ int tdz_mg_coarse_status(const tdz_ctx *ctx, int *its, double *res);
  * (level = tdz_levels - 1 >= 1; PCMGGetCoarseSolve + KSPSolve on it), with the current
  * shift and the kind tdz_mg_set_coarse chose. Device vectors of the level's size. */
 int tdz_level_solve(tdz_ctx *ctx, int level, const double *b, double *x);
+/* Transforms of the complex64 separable solves (every mixed-precision preconditioner; no
+ * reference counterpart): kind 0 FP32 MFMA, kind 1 (default) split FP16 MFMA (each operand
+ * the sum of two FP16 halves, three FP16 products in FP32 accumulation, about 2^-22 relative
+ * per product; axes up to 160 functions, longer ones keep FP32). */
+int tdz_set_fd_transform(tdz_ctx *ctx, int kind);
 /* Correction sweeps of TDZ_PC_CORRECTED* and TDZ_PC_LEARNED (no reference counterpart):
  * z <- S^-1 (r - dA~ z) applied sweeps times after z = S^-1 r (1..4, default 1), dA~ the
  * terms the separable solve S leaves out (CAP and field exactly, the potential lumped,

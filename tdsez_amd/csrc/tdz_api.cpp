@@ -76,6 +76,8 @@ struct Prec {
     cpx<R> *Bmass[3] = {};                   // bands of M alone (M on x, zero on y and z)
     R *V[3] = {}, *lam[3] = {};              // separable preconditioner
     R *Vf[3][2] = {};                        // complex64: V and V^T in MFMA fragment order (or null)
+    unsigned short *Vh[3][2] = {};           // complex64: V and V^T split into FP16 halves (or null)
+    float Vh_inv[3][2] = {};                 // the power of two their scaling undoes
     R *Mw = nullptr;                         // layered z axis: 1/m-weighted mass band
     cpx<R> *lwork = nullptr;                 // layered z axis: work of the z-line solves
     cpx<R> *r = nullptr, *z = nullptr, *t = nullptr;   // multigrid work vectors
@@ -251,6 +253,7 @@ struct tdz_ctx {
     bool field_warned = false, cap_warned = false;   // the multigrid warnings were given
     int refresh_cap = -1;
     int smoother = 0;           // multigrid smoother: 0 separable-solve Richardson, 1 Jacobi-Chebyshev
+    bool fd_split16 = true;     // complex64 separable solves with split-FP16 transforms
     int cheb_degree = 2;
     // coarsest multigrid level: 0 separable-solve sweeps, 1 Krylov solve to coarse_rtol
     int coarse_kind = 0, coarse_maxit = 64;
@@ -686,6 +689,9 @@ void upload_fd_float(tdz_ctx *c, Level &L, int d)
         std::vector<float> fr;
         if (L.hV[d].size() == (size_t)n * n) fr = tdz::axis_fragments(n, L.hV[d], tr == 1);
         L.f.Vf[d][tr] = fr.empty() ? nullptr : upload(c, fr);
+        std::vector<unsigned short> fh;
+        if (L.hV[d].size() == (size_t)n * n) fh = tdz::axis_fragments_split16(n, L.hV[d], tr == 1, L.f.Vh_inv[d][tr]);
+        L.f.Vh[d][tr] = fh.empty() ? nullptr : upload(c, fh);
     }
 }
 
@@ -742,10 +748,24 @@ void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<
         tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, s2, z, P.Vf[1][0]);
         return;
     }
-    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1, P.Vf[0][1]);
-    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2, P.Vf[1][1]);
     const R *lam[3] = {P.lam[0], P.lam[1], P.lam[2]};
     const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
+    if constexpr (std::is_same<R, float>::value) {
+        // split-FP16 transforms on every axis that takes them (the z pass with the diagonal solve)
+        if (c->fd_split16 && P.Vh[0][0] && P.Vh[1][0] && P.Vh[2][0]) {
+            tdz::launch_dense_axis_split16(c->st, 0, L.n, P.Vh[0][1], P.Vh_inv[0][1], r, s1);
+            tdz::launch_dense_axis_split16(c->st, 1, L.n, P.Vh[1][1], P.Vh_inv[1][1], s1, s2);
+            tdz::launch_dense_axis_diag_split16(c->st, L.n, P.Vh[2][1], P.Vh_inv[2][1], lam, w, cx<R>(c->alpha),
+                                                cx<R>(c->g), s2, s1);
+            tdz::launch_dense_axis_split16(c->st, 0, L.n, P.Vh[0][0], P.Vh_inv[0][0], s1, s2);
+            tdz::launch_dense_axis_split16(c->st, 1, L.n, P.Vh[1][0], P.Vh_inv[1][0], s2, s1);
+            tdz::launch_dense_axis_split16(c->st, 2, L.n, P.Vh[2][0], P.Vh_inv[2][0], s1, z);
+            if (c->prob.dirichlet) tdz::launch_wall_fix<R>(c->st, L.n, w, r, z);
+            return;
+        }
+    }
+    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1, P.Vf[0][1]);
+    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2, P.Vf[1][1]);
     cpx<R> *d = s2, *e = s1;   // the diagonal solve's output and the other scratch vector
     bool fused = false;
     if constexpr (std::is_same<R, float>::value) {   // z pass and diagonal solve in one kernel
@@ -2211,6 +2231,14 @@ int tdz_mg_set_smoother(tdz_ctx *c, int kind, int degree)
         if (kind == 1 && c->pc_ready &&
             (c->pc_kind == TDZ_PC_MG || c->pc_kind == TDZ_PC_MG_MIXED || c->pc_kind == TDZ_PC_LEARNED))
             cheb_setup(c);
+    });
+}
+
+int tdz_set_fd_transform(tdz_ctx *c, int kind)
+{
+    return guarded([&] {
+        if (kind != 0 && kind != 1) throw Error("transform kind must be 0 (FP32 MFMA) or 1 (split FP16 MFMA)");
+        c->fd_split16 = kind == 1;
     });
 }
 

@@ -215,6 +215,17 @@ void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool t
 int axis_frag_steps(int n);
 // n x n row-major T (or its transpose) as the transform's A fragments (empty: too long)
 std::vector<float> axis_fragments(int n, const std::vector<double> &T, bool trans);
+// Split-FP16 MFMA form of the complex64 transform (each product as hi*hi + hi*lo + lo*hi of
+// FP16 halves, FP32 accumulation; about 2^-22 relative): the matrix pre-split and scaled by
+// 2^-e (|A| < 1; ainv = 2^e undoes it) in the fragment order of axis_split16_kernel (empty:
+// axis too long); false if not applicable.
+int axis_split16_blocks(int n);
+std::vector<unsigned short> axis_fragments_split16(int n, const std::vector<double> &T, bool trans, float &ainv);
+bool launch_dense_axis_split16(hipStream_t st, int d, const int n[3], const unsigned short *Th, float ainv,
+                               const fcomplex *x, fcomplex *y);
+bool launch_dense_axis_diag_split16(hipStream_t st, const int n[3], const unsigned short *Th, float ainv,
+                                    const float *lam[3], const unsigned char *wall[3], fcomplex alpha, fcomplex g,
+                                    const fcomplex *x, fcomplex *y);
 // One thread per output, T read through the caches (reference kernel for tests).
 // complex64 forward transform along z fused with the diagonal solve (Tfrag: fragments
 // of T^T; then divide by alpha + g (l0 + l1 + l2) off the walls); false if not applicable

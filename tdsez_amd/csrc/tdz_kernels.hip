@@ -1376,6 +1376,113 @@ __global__ void __launch_bounds__(256) axis_wave_mfma_kernel(int n, long ncol, l
     }
 }
 
+// The same transform in split FP16 on v_mfma_f32_16x16x32_f16 (lane l holds A[l%16][8(l/16)+j],
+// B[8(l/16)+j][l%16], j < 8, and D[4(l/16)+q][l%16]). Each operand is a sum of two FP16
+// numbers, hi = fp16(v) and lo = fp16(v - hi), and a product is hi*hi + hi*lo + lo*hi in FP32
+// accumulation (the dropped lo*lo term and the splits leave about 2^-22 relative). The matrix
+// comes pre-split and scaled by a power of two (|A| <= 1) in fragment order: uint4 (8 halves)
+// [it][kb][hi, lo][lane]. The input column is scaled by a power of two from its largest
+// component (a 4-lane maximum: the lanes l, l^16, l^32, l^48 hold one column), so that its
+// values use FP16's range; both scales are undone exactly in the epilogue. Work mapping,
+// column numbering and the diagonal epilogue are those of axis_wave_mfma_kernel.
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+template <int KB, int G, bool DIAG>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) axis_split16_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
+                                                           int nit, const uint4 *__restrict__ Th, float ainv,
+                                                           const fcomplex *__restrict__ in,
+                                                           fcomplex *__restrict__ out, DiagF dg)
+{
+    const int ngr = (nit + G - 1) / G;
+    const int nb = gridDim.x, b = blockIdx.x, per = nb / 8;
+    const long t = b < 8 * per ? (long)(b % 8) * per + b / 8 : b;
+    const long item = 4 * t + (threadIdx.x >> 6);
+    const long nct = (ncol + 15) / 16;
+    if (item >= nct * ngr) return;   // whole wave: no barriers below
+    const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+    const int ig = (int)(item % ngr);
+    const long col = (item / ngr) * 16 + r16;
+    const bool cv = col < ncol;
+    const long base = cv ? (col / Lin) * sB + (col % Lin) * sL : 0;
+    fcomplex bv[KB][8];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 32 * kb + 8 * kq + j;
+            bv[kb][j] = in[base + (long)min(k, n - 1) * sK];
+            if (!cv || k >= n) bv[kb][j] = {0.f, 0.f};
+        }
+    float m = 0.f;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) m = fmaxf(m, fmaxf(fabsf(bv[kb][j].re), fabsf(bv[kb][j].im)));
+    m = fmaxf(m, __shfl_xor(m, 16, 64));
+    m = fmaxf(m, __shfl_xor(m, 32, 64));
+    int e = 0;
+    if (m > 0.f) frexpf(m, &e);
+    const float sc = ldexpf(1.f, -e);   // |v sc| < 1
+    half8_t rh[KB], rl[KB], ih[KB], il[KB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float xr = bv[kb][j].re * sc, xi = bv[kb][j].im * sc;
+            const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+            rh[kb][j] = hr;
+            rl[kb][j] = (_Float16)(xr - (float)hr);
+            ih[kb][j] = hi;
+            il[kb][j] = (_Float16)(xi - (float)hi);
+        }
+    f32x4 ar[G], ai[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) ar[g] = ai[g] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int it = min(ig * G + g, nit - 1);   // past the last row tile: a copy, not stored
+            const half8_t ah = __builtin_bit_cast(half8_t, Th[((long)(it * KB + kb) * 2) * 64 + lane]);
+            const half8_t al = __builtin_bit_cast(half8_t, Th[((long)(it * KB + kb) * 2 + 1) * 64 + lane]);
+            ar[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, rh[kb], ar[g], 0, 0, 0);
+            ar[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rl[kb], ar[g], 0, 0, 0);
+            ar[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rh[kb], ar[g], 0, 0, 0);
+            ai[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, ih[kb], ai[g], 0, 0, 0);
+            ai[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, il[kb], ai[g], 0, 0, 0);
+            ai[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, ih[kb], ai[g], 0, 0, 0);
+        }
+    if (!cv) return;
+    const float back = ainv / sc;   // powers of two: exact
+    float la = 0.f;
+    bool wl = false;
+    if constexpr (DIAG) {   // z pass: column col = i0 + n0 i1, row i = i2
+        const int i0 = (int)(col % dg.n0), i1 = (int)(col / dg.n0);
+        la = dg.lam[0][i0] + dg.lam[1][i1];
+        wl = (dg.wall[0][i0] | dg.wall[1][i1]) != 0;
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int it = ig * G + g;
+        if (it >= nit) break;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * it + 4 * kq + q;
+            if (i >= n) continue;
+            fcomplex v = {ar[g][q] * back, ai[g][q] * back};
+            if constexpr (DIAG) {
+                if (!wl && !dg.wall[2][i]) {
+                    const float lam = la + dg.lam[2][i];
+                    const float dr = dg.alpha.re + dg.g.re * lam, di = dg.alpha.im + dg.g.im * lam;
+                    const float inv = 1.f / (dr * dr + di * di);
+                    v = {(v.re * dr + v.im * di) * inv, (v.im * dr - v.re * di) * inv};
+                }
+            }
+            out[base + (long)i * sK] = v;
+        }
+    }
+}
+
 // largest axis length the register-resident transform takes; longer axes use the
 // LDS-tiled kernel
 constexpr int kWaveAxisMax = 160;
@@ -2000,6 +2107,89 @@ std::vector<float> axis_fragments(int n, const std::vector<double> &T, bool tran
                             (float)(trans ? T[(size_t)k * n + row] : T[(size_t)row * n + k]);
                 }
     return f;
+}
+
+int axis_split16_blocks(int n) { return n > kWaveAxisMax ? 0 : (n + 31) / 32; }
+
+std::vector<unsigned short> axis_fragments_split16(int n, const std::vector<double> &T, bool trans, float &ainv)
+{
+    const int KB = axis_split16_blocks(n);
+    if (!KB) return {};
+    double amax = 0;
+    for (double v : T) amax = std::max(amax, std::fabs(v));
+    int e = 0;
+    if (amax > 0) std::frexp(amax, &e);
+    const double as = std::ldexp(1.0, -e);   // |A as| < 1
+    ainv = (float)std::ldexp(1.0, e);
+    const int nit = (n + 15) / 16;
+    std::vector<unsigned short> f((size_t)nit * KB * 2 * 64 * 8, 0);
+    for (int it = 0; it < nit; ++it)
+        for (int kb = 0; kb < KB; ++kb)
+            for (int l = 0; l < 64; ++l)
+                for (int j = 0; j < 8; ++j) {
+                    const int row = 16 * it + l % 16, k = 32 * kb + 8 * (l / 16) + j;
+                    if (row >= n || k >= n) continue;
+                    const double v = as * (trans ? T[(size_t)k * n + row] : T[(size_t)row * n + k]);
+                    const _Float16 hi = (_Float16)v, lo = (_Float16)(v - (double)hi);
+                    const size_t o = ((((size_t)it * KB + kb) * 2) * 64 + l) * 8 + j;
+                    f[o] = __builtin_bit_cast(unsigned short, hi);
+                    f[o + 64 * 8] = __builtin_bit_cast(unsigned short, lo);
+                }
+    return f;
+}
+
+namespace {
+template <bool DIAG>
+void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, long sL, long sK, const unsigned short *Th,
+                         float ainv, const fcomplex *x, fcomplex *y, const DiagF &dg)
+{
+    const int nit = (nd + 15) / 16, G = nit < 3 ? nit : 3, ngr = (nit + G - 1) / G;
+    const unsigned nblk = (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
+    const uint4 *T4 = reinterpret_cast<const uint4 *>(Th);
+#define TDZ_S16(KB, GG)                                                                                             \
+    hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, sB, sL, sK, \
+                       nit, T4, ainv, x, y, dg)
+#define TDZ_S16_G(KB)                 \
+    do {                              \
+        if (G == 1) TDZ_S16(KB, 1);   \
+        else if (G == 2) TDZ_S16(KB, 2); \
+        else TDZ_S16(KB, 3);          \
+    } while (0)
+    switch (axis_split16_blocks(nd)) {
+    case 1: TDZ_S16_G(1); break;
+    case 2: TDZ_S16_G(2); break;
+    case 3: TDZ_S16_G(3); break;
+    case 4: TDZ_S16_G(4); break;
+    default: TDZ_S16_G(5); break;
+    }
+#undef TDZ_S16_G
+#undef TDZ_S16
+}
+}  // namespace
+
+bool launch_dense_axis_split16(hipStream_t st, int d, const int n[3], const unsigned short *Th, float ainv,
+                               const fcomplex *x, fcomplex *y)
+{
+    if (!Th || !axis_split16_blocks(n[d])) return false;
+    const long n01 = (long)n[0] * n[1];
+    long L, sB, sK, sL;
+    unsigned B;
+    if (d == 0) { L = (long)n[1] * n[2]; sB = 0; sK = 1; sL = n[0]; B = 1; }
+    else if (d == 1) { L = n[0]; sB = n01; sK = n[0]; sL = 1; B = (unsigned)n[2]; }
+    else { L = n01; sB = 0; sK = n01; sL = 1; B = 1; }
+    launch_split16_axis<false>(st, n[d], L * B, L, sB, sL, sK, Th, ainv, x, y, DiagF{});
+    return true;
+}
+
+bool launch_dense_axis_diag_split16(hipStream_t st, const int n[3], const unsigned short *Th, float ainv,
+                                    const float *lam[3], const unsigned char *wall[3], fcomplex alpha, fcomplex g,
+                                    const fcomplex *x, fcomplex *y)
+{
+    if (!Th || !axis_split16_blocks(n[2])) return false;
+    const long ncol = (long)n[0] * n[1];
+    const DiagF dg{{lam[0], lam[1], lam[2]}, {wall[0], wall[1], wall[2]}, alpha, g, n[0]};
+    launch_split16_axis<true>(st, n[2], ncol, ncol, 0, 1, ncol, Th, ainv, x, y, dg);
+    return true;
 }
 
 namespace {

@@ -1,4 +1,5 @@
-"""Complex64 fast-diagonalisation solve on long axes (through the C ABI).
+"""Complex64 fast-diagonalisation solve on long axes (through the C ABI), with the default
+split-FP16 transforms and with the FP32 ones.
 
 Axis lengths n = E + p of 65 .. 154 take every register-resident MFMA transform size
 (up to 8, 16, 24, 33, 40 k-steps of 4); n = 205 takes the LDS-tiled transform. The
@@ -44,4 +45,37 @@ def test_complex64_fd_matches_fp64(name):
             ref = o.fd_apply(1.0, 0.02j, r)
             assert np.abs(a - ref).max() / np.abs(ref).max() < 1e-10, name
     finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_split16_transforms_match_fp64(name):
+    """The split-FP16 transforms (three FP16 MFMA products per FP32 product) give the
+    complex64 solve at least the accuracy of the FP32 MFMA transforms' order (1e-5 relative
+    to the FP64 solve; FP32 rounding alone leaves about 1e-6), on residual-like inputs whose
+    magnitudes span six decades across the grid. Axes above 160 functions keep the FP32
+    transform."""
+    import tdsez_amd as T
+    desc = SHAPES[name]
+    ctx = T.Context(desc)
+    try:
+        ctx.set_shift(1.0, 0.02j, (0, 0, 0))
+        rng = np.random.default_rng(5)
+        scale = 10.0 ** rng.uniform(-6, 0, ctx.N)
+        r = (rng.standard_normal(ctx.N) + 1j * rng.standard_normal(ctx.N)) * scale
+        dr, z64, z32, z16 = ctx.vec(r), ctx.vec(), ctx.vec(), ctx.vec()
+        ctx.pc_setup(T.PC_SEPARABLE)
+        ctx.pc_apply(dr, z64)
+        ctx.pc_setup(T.PC_SEPARABLE_MIXED)
+        ctx.set_fd_transform(0)
+        ctx.pc_apply(dr, z32)
+        ctx.set_fd_transform(1)
+        ctx.pc_apply(dr, z16)
+        a = z64.download()
+        e32 = np.linalg.norm(z32.download() - a) / np.linalg.norm(a)
+        e16 = np.linalg.norm(z16.download() - a) / np.linalg.norm(a)
+        print(name, "relative error of the complex64 solve: FP32 transforms", e32, "split FP16", e16)
+        assert e16 < 1e-5, (name, e16, e32)
+    finally:
+        ctx.set_fd_transform(1)
         ctx.close()
