@@ -222,7 +222,7 @@ struct tdz_ctx {
         double dt = 0;
         unsigned axes = 0;          // field axes whose dipole products are kept
         static constexpr int kSlots = 5;   // order <= 4
-        int ring = 3;                      // slots in use: order + 1
+        int ring = 1;                      // slots in use: order + 1
         int sync_every = 16, since_sync = 0;   // exact recomputation of the stored products
         double last_drift = 0;                 // their relative drift found at the last one
         dcomplex *psi[kSlots] = {}, *P[kSlots] = {}, *D[kSlots][3] = {};   // state; (M + g H_sEh) state; D_a state
@@ -231,10 +231,7 @@ struct tdz_ctx {
         dcomplex *Xband[3] = {}, *Zband[3] = {};   // complex position bands and zero bands (dipole applies)
         double *Xz = nullptr;                      // the real z position band (fused mass + z dipole)
     } pred;
-    // chained CN right-hand side: A psi of the last solve of an evolve run
-    dcomplex *ax = nullptr, *zerov = nullptr, *dDip[3] = {};
-    bool ax_valid = false;
-    double ax_dt = 0, ax_sE[3] = {0, 0, 0};
+    dcomplex *zerov = nullptr;   // a zero vector (wall rows of the field terms)
     int restart = 30;
     dcomplex *Vb = nullptr, *Zb = nullptr;
     dcomplex *dpart = nullptr, *dh = nullptr, *dh2 = nullptr;
@@ -251,9 +248,10 @@ struct tdz_ctx {
     int refresh_base = -1, rebuilds = 0;
     int stage_base[2] = {-1, -1};   // refresh trackers of the two Pade stages
     bool field_warned = false, cap_warned = false;   // the multigrid warnings were given
+    double field_checked[8] = {};                      // the (pulse, tau, dt) the field check last passed
     int refresh_cap = -1;
     int smoother = 0;           // multigrid smoother: 0 separable-solve Richardson, 1 Jacobi-Chebyshev
-    bool fd_split16 = true;     // complex64 separable solves with split-FP16 transforms
+    int fd_transform = 1;       // complex64 separable solves: 0 FP32 MFMA, 1 split FP16 MFMA
     int cheb_degree = 2;
     // coarsest multigrid level: 0 separable-solve sweeps, 1 Krylov solve to coarse_rtol
     int coarse_kind = 0, coarse_maxit = 64;
@@ -752,7 +750,7 @@ void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<
     const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
     if constexpr (std::is_same<R, float>::value) {
         // split-FP16 transforms on every axis that takes them (the z pass with the diagonal solve)
-        if (c->fd_split16 && P.Vh[0][0] && P.Vh[1][0] && P.Vh[2][0]) {
+        if (c->fd_transform > 0 && P.Vh[0][0] && P.Vh[1][0] && P.Vh[2][0]) {
             tdz::launch_dense_axis_split16(c->st, 0, L.n, P.Vh[0][1], P.Vh_inv[0][1], r, s1);
             tdz::launch_dense_axis_split16(c->st, 1, L.n, P.Vh[1][1], P.Vh_inv[1][1], s1, s2);
             tdz::launch_dense_axis_diag_split16(c->st, L.n, P.Vh[2][1], P.Vh_inv[2][1], lam, w, cx<R>(c->alpha),
@@ -2238,7 +2236,7 @@ int tdz_set_fd_transform(tdz_ctx *c, int kind)
 {
     return guarded([&] {
         if (kind != 0 && kind != 1) throw Error("transform kind must be 0 (FP32 MFMA) or 1 (split FP16 MFMA)");
-        c->fd_split16 = kind == 1;
+        c->fd_transform = kind;
     });
 }
 
@@ -2330,6 +2328,10 @@ void multigrid_checks(tdz_ctx *c, const tdz_pulse *pulse, double tau, double dt)
         }
     }
     if (!pulse || c->field_warned || pulse->omega <= 0 || pulse->ncycles <= 0) return;
+    // the scan below samples the whole pulse: once per (pulse, tau, dt)
+    const double key[8] = {pulse->E0[0], pulse->E0[1], pulse->E0[2], pulse->omega, pulse->ncycles, pulse->cep, tau, dt};
+    if (std::equal(key, key + 8, c->field_checked)) return;
+    std::copy(key, key + 8, c->field_checked);
     const double T = pulse->ncycles * 2.0 * M_PI / pulse->omega;
     double Lmax[3] = {0, 0, 0};
     for (int a = 0; a < c->prob.dim; ++a) Lmax[a] = std::max(std::fabs(c->prob.lo[a]), std::fabs(c->prob.hi[a]));
@@ -2394,73 +2396,30 @@ void step_solve(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int max
     st->restarts += s1.restarts;
 }
 
-// One CN step psi <- (M + g H_E)^-1 (M - g H_E) psi, g = i dt/2 (src/propagator.cpp:470-540).
-// Unchained, the right-hand side is an operator apply. Chained (the previous step of
-// the same run solved (M + g H_E') psi = b' with this dt and left A psi = b' - r' in
-// c->ax), the identity
-//   (M - g H_E) psi = 2 M psi - (M + g H_E') psi - g sum_a (sE_a - sE'_a) D_a psi
-// gives it from one mass apply and one dipole apply; both the identity and A psi are
-// exact up to rounding, so the step is the same.
-void cn_step(tdz_ctx *c, dcomplex *x, double dt, const double sE_mid[3], double rtol, int maxit, tdz_ksp_stats *st,
-             bool chained)
+// One CN step psi <- (M + g H_E)^-1 (M - g H_E) psi, g = i dt/2 (src/propagator.cpp:470-540),
+// on its own (tdz_step_cn): the right-hand side is an operator apply. The steps of an evolve
+// run go through cn_step_pred, which forms it from the products its last solve left.
+void cn_step(tdz_ctx *c, dcomplex *x, double dt, const double sE_mid[3], double rtol, int maxit, tdz_ksp_stats *st)
 {
     if (!c->pc_ready) pc_setup(c, c->pc_kind);
     Level &L = *c->lv[0];
     const long N = L.N;
     const cplx g(0.0, 0.5 * dt);
     c->pred.count = 0;   // the state moves on without the predictor's bookkeeping
-    if (!c->ax) c->ax = track(c, dalloc<dcomplex>(N));
-    chained = chained && c->ax_valid && c->ax_dt == dt;
     tdz::SepBandsT<double> mb = sep_bands<double>(L);
     for (int d = 0; d < 3; ++d) mb.B[d] = L.d.Bmass[d];
     const unsigned char *wall[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
-    if (chained) {
-        tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);   // M psi
-        if (c->prob.dirichlet) tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
-        // rhs = 2 M psi - A' psi
-        HIPCHK(hipMemcpyAsync(c->rhs, c->sd.w1, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
-        tdz::launch_axpby(c->st, {-1, 0}, c->ax, {2, 0}, c->rhs, N);
-        bool field = false;
-        for (int a = 0; a < c->prob.dim; ++a) field = field || sE_mid[a] != c->ax_sE[a];
-        if (field) {   // rhs -= g sum_a dsE_a D_a psi: one Kronecker apply with scaled position bands
-            tdz::SepBandsT<double> db = sep_bands<double>(L);
-            for (int d = 0; d < 3; ++d) {
-                const size_t sz = L.M[d].size();
-                if (!c->dDip[d]) c->dDip[d] = track(c, dalloc<dcomplex>(sz));
-                std::vector<dcomplex> B(sz, dcomplex{0, 0});
-                if (d < c->prob.dim)
-                    for (size_t k = 0; k < sz; ++k) B[k] = dc(-g * (sE_mid[d] - c->ax_sE[d]) * L.Xb[d][k]);
-                HIPCHK(hipMemcpyAsync(c->dDip[d], B.data(), sz * sizeof(dcomplex), hipMemcpyHostToDevice, c->st));
-                HIPCHK(hipStreamSynchronize(c->st));
-                db.B[d] = c->dDip[d];
-            }
-            tdz::launch_sep_apply<double>(c->st, db, x, c->sd.w2, c->sd.tmp);
-            if (c->prob.dirichlet) {   // the walls carry the identity rows of M - g H: no field term
-                if (!c->zerov) {
-                    c->zerov = track(c, dalloc<dcomplex>(N));
-                    HIPCHK(hipMemsetAsync(c->zerov, 0, N * sizeof(dcomplex), c->st));
-                }
-                tdz::launch_wall_fix<double>(c->st, L.n, wall, c->zerov, c->sd.w2);
-            }
-            tdz::launch_axpby(c->st, {1, 0}, c->sd.w2, {1, 0}, c->rhs, N);
-        }
-    } else {
-        set_shift(c, 1.0, -g, sE_mid);        // rhs = (M - i dt/2 H_E) psi
-        level_apply(c, L, x, c->rhs);
-        tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
-        if (c->prob.dirichlet) tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
-    }
+    set_shift(c, 1.0, -g, sE_mid);        // rhs = (M - i dt/2 H_E) psi
+    level_apply(c, L, x, c->rhs);
+    tdz::launch_sep_apply<double>(c->st, mb, x, c->sd.w1, c->sd.tmp);
+    if (c->prob.dirichlet) tdz::launch_wall_fix<double>(c->st, L.n, wall, x, c->sd.w1);
     // (M + i dt/2 H_E) psi = 2 M psi - rhs, so the initial residual of the guess psi
     // is r0 = 2 (rhs - M psi): one mass apply instead of a full operator apply
     // (the same identity as the stage right-hand sides of src/propagator.cpp:589-624)
     HIPCHK(hipMemcpyAsync(c->Vb, c->rhs, N * sizeof(dcomplex), hipMemcpyDeviceToDevice, c->st));
     tdz::launch_axpby(c->st, {-2.0, 0}, c->sd.w1, {2.0, 0}, c->Vb, N);
     set_shift(c, 1.0, g, sE_mid);         // solve (M + i dt/2 H_E) psi' = rhs, guess psi
-    c->ax_valid = false;
-    step_solve(c, c->rhs, x, rtol, maxit, st, true, c->ax);
-    c->ax_valid = true;
-    c->ax_dt = dt;
-    for (int a = 0; a < 3; ++a) c->ax_sE[a] = sE_mid[a];
+    step_solve(c, c->rhs, x, rtol, maxit, st, true);
 }
 
 // ------------------------------------------------------------------ CN predictor
@@ -2621,7 +2580,6 @@ void cn_step_pred(tdz_ctx *c, dcomplex *x, double dt, const double sE[3], double
     Level &L = *c->lv[0];
     const long N = L.N;
     const cplx g(0.0, 0.5 * dt);
-    c->ax_valid = false;   // the chained right-hand side of cn_step does not follow this run
     for (int a = 0; a < c->prob.dim; ++a)
         if (sE[a] != 0 && !(p.axes >> a & 1)) {   // a new field axis: restart the history
             p.axes |= 1u << a;
@@ -2695,7 +2653,7 @@ int tdz_step_cn(tdz_ctx *c, double *psi, double dt, const double sE_mid[3], doub
 {
     return guarded([&] {
         tdz_ksp_stats st{};
-        cn_step(c, reinterpret_cast<dcomplex *>(psi), dt, sE_mid, rtol, maxit, &st, false);
+        cn_step(c, reinterpret_cast<dcomplex *>(psi), dt, sE_mid, rtol, maxit, &st);
         if (stats) *stats = st;
         HIPCHK(hipStreamSynchronize(c->st));   // the results are final when the call returns
     });
@@ -3367,14 +3325,15 @@ int tdz_evolve_cn(tdz_ctx *c, double *psi, double t0, double dt, int nsteps, con
         if (!c->pc_ready) pc_setup(c, c->pc_kind);
         multigrid_checks(c, pulse, 0.5 * dt, dt);
         dcomplex *x = reinterpret_cast<dcomplex *>(psi);
-        if (c->pred.order > 0 && !pred_continues(c, x, dt)) c->pred.count = 0;
+        if (!pred_continues(c, x, dt)) c->pred.count = 0;
         for (int s = 0; s < nsteps; ++s) {
             const double tm = t0 + (s + 0.5) * dt;
             double sE[3] = {0, 0, 0};
             if (pulse) tdz_pulse_coupling(c, pulse, tm, sE);
             tdz_ksp_stats st{};
-            if (c->pred.order > 0) cn_step_pred(c, x, dt, sE, rtol, maxit, &st);
-            else cn_step(c, x, dt, sE, rtol, maxit, &st, s > 0);
+            // order 0 is the current state as the guess (TSTHETA's): the same fused
+            // right-hand side and history products, and no unchained first step per call
+            cn_step_pred(c, x, dt, sE, rtol, maxit, &st);
             // the separable solves depend on (alpha, g) only and every level operator follows
             // the field through set_shift, so a rebuild has nothing to redo but is counted
             if (refresh_due(c, c->refresh_base, st.its)) pc_setup(c, c->pc_kind);

@@ -1387,8 +1387,8 @@ __global__ void __launch_bounds__(256) axis_wave_mfma_kernel(int n, long ncol, l
 // column numbering and the diagonal epilogue are those of axis_wave_mfma_kernel.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 
-template <int KB, int G, bool DIAG>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) axis_split16_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
+template <int KB, int G, bool DIAG, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) axis_split16_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
                                                            int nit, const uint4 *__restrict__ Th, float ainv,
                                                            const fcomplex *__restrict__ in,
                                                            fcomplex *__restrict__ out, DiagF dg)
@@ -2147,8 +2147,8 @@ void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, l
     const unsigned nblk = (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
     const uint4 *T4 = reinterpret_cast<const uint4 *>(Th);
 #define TDZ_S16(KB, GG)                                                                                             \
-    hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, sB, sL, sK, \
-                       nit, T4, ainv, x, y, dg)
+    hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG, DIAG ? 4 : 3>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, \
+                       sB, sL, sK, nit, T4, ainv, x, y, dg)
 #define TDZ_S16_G(KB)                 \
     do {                              \
         if (G == 1) TDZ_S16(KB, 1);   \
@@ -2165,6 +2165,7 @@ void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, l
 #undef TDZ_S16_G
 #undef TDZ_S16
 }
+
 }  // namespace
 
 bool launch_dense_axis_split16(hipStream_t st, int d, const int n[3], const unsigned short *Th, float ainv,

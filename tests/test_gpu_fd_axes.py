@@ -19,6 +19,8 @@ SHAPES = {
     "z103": dict(BASE, dim=3, nelem=(4, 3, 100), lo=(-4.0, -4.0, -30.0), hi=(4.0, 4.0, 30.0), p=3),
     "2d65x105": dict(BASE, dim=2, nelem=(60, 100), lo=(-20.0, -30.0), hi=(20.0, 30.0), p=5),
     "1d205": dict(BASE, dim=1, nelem=(200,), lo=(-40.0,), hi=(40.0,), p=5),
+    "walls3d": dict(BASE, dim=3, nelem=(20, 14, 30), lo=(-10.0, -8.0, -12.0), hi=(10.0, 8.0, 12.0), p=4,
+                    dirichlet=1),
 }
 
 
@@ -54,7 +56,7 @@ def test_split16_transforms_match_fp64(name):
     complex64 solve at least the accuracy of the FP32 MFMA transforms' order (1e-5 relative
     to the FP64 solve; FP32 rounding alone leaves about 1e-6), on residual-like inputs whose
     magnitudes span six decades across the grid. Axes above 160 functions keep the FP32
-    transform."""
+    transform. walls3d runs the diagonal solve's wall rows (identity) in the fused z pass."""
     import tdsez_amd as T
     desc = SHAPES[name]
     ctx = T.Context(desc)
