@@ -750,7 +750,7 @@ void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<
     const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
     if constexpr (std::is_same<R, float>::value) {
         // split-FP16 transforms on every axis that takes them (the z pass with the diagonal solve)
-        if (c->fd_transform > 0 && P.Vh[0][0] && P.Vh[1][0] && P.Vh[2][0]) {
+        if (c->fd_transform > 0 && P.Vh[0][0] && P.Vh[1][0] && P.Vh[2][0] && tdz::split16_fits(L.n)) {
             tdz::launch_dense_axis_split16(c->st, 0, L.n, P.Vh[0][1], P.Vh_inv[0][1], r, s1);
             tdz::launch_dense_axis_split16(c->st, 1, L.n, P.Vh[1][1], P.Vh_inv[1][1], s1, s2);
             tdz::launch_dense_axis_diag_split16(c->st, L.n, P.Vh[2][1], P.Vh_inv[2][1], lam, w, cx<R>(c->alpha),

@@ -220,6 +220,7 @@ std::vector<float> axis_fragments(int n, const std::vector<double> &T, bool tran
 // 2^-e (|A| < 1; ainv = 2^e undoes it) in the fragment order of axis_split16_kernel (empty:
 // axis too long); false if not applicable.
 int axis_split16_blocks(int n);
+bool split16_fits(const int n[3]);   // the vectors fit the 32-bit byte offsets of its buffer accesses
 std::vector<unsigned short> axis_fragments_split16(int n, const std::vector<double> &T, bool trans, float &ainv);
 bool launch_dense_axis_split16(hipStream_t st, int d, const int n[3], const unsigned short *Th, float ainv,
                                const fcomplex *x, fcomplex *y);

@@ -1386,12 +1386,19 @@ __global__ void __launch_bounds__(256) axis_wave_mfma_kernel(int n, long ncol, l
 // values use FP16's range; both scales are undone exactly in the epilogue. Work mapping,
 // column numbering and the diagonal epilogue are those of axis_wave_mfma_kernel.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
 
+// Loads and stores are raw buffer accesses with 32-bit byte offsets (the vectors hold at most
+// 2^32 bytes; nbytes is their size): a lane's offset is its column start plus its k group,
+// the (kb, j) step a wave-uniform stride, one integer add per access. Every offset is in
+// bounds (k past the axis, only in the last k block, reads row 0 and is zeroed; padding
+// columns past ncol read the last column, and their output columns are not stored); the
+// buffer's own range check is a second guard.
 template <int KB, int G, bool DIAG, int OCC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) axis_split16_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
                                                            int nit, const uint4 *__restrict__ Th, float ainv,
                                                            const fcomplex *__restrict__ in,
-                                                           fcomplex *__restrict__ out, DiagF dg)
+                                                           fcomplex *__restrict__ out, DiagF dg, unsigned nbytes)
 {
     const int ngr = (nit + G - 1) / G;
     const int nb = gridDim.x, b = blockIdx.x, per = nb / 8;
@@ -1403,15 +1410,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     const int ig = (int)(item % ngr);
     const long col = (item / ngr) * 16 + r16;
     const bool cv = col < ncol;
-    const long base = cv ? (col / Lin) * sB + (col % Lin) * sL : 0;
+    const unsigned colu = (unsigned)(cv ? col : ncol - 1), linu = (unsigned)Lin, cq = colu / linu;
+    const unsigned base8 = 8u * (cq * (unsigned)sB + (colu - cq * linu) * (unsigned)sL);   // column start, bytes
+    const unsigned skb = 8u * (unsigned)sK;                                                  // bytes per k
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<fcomplex *>(in), (short)0, (int)nbytes, 0x00020000);
+    const unsigned vin = base8 + (unsigned)(8 * kq) * skb;
     fcomplex bv[KB][8];
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int k = 32 * kb + 8 * kq + j;
-            bv[kb][j] = in[base + (long)min(k, n - 1) * sK];
-            if (!cv || k >= n) bv[kb][j] = {0.f, 0.f};
+            // k < 32 (KB - 1) < n in every block but the last
+            const bool in_axis = kb < KB - 1 || k < n;
+            const unsigned off = in_axis ? vin + (unsigned)(32 * kb + j) * skb : base8;
+            bv[kb][j] = __builtin_bit_cast(fcomplex, __builtin_amdgcn_raw_buffer_load_b64(rin, off, 0, 0));
+            if (!in_axis) bv[kb][j] = {0.f, 0.f};
         }
     float m = 0.f;
 #pragma unroll
@@ -1454,6 +1469,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
         }
     if (!cv) return;
     const float back = ainv / sc;   // powers of two: exact
+    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(out, (short)0, (int)nbytes, 0x00020000);
+    const unsigned vout = base8 + (unsigned)(4 * kq) * skb;
     float la = 0.f;
     bool wl = false;
     if constexpr (DIAG) {   // z pass: column col = i0 + n0 i1, row i = i2
@@ -1470,15 +1487,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
             const int i = 16 * it + 4 * kq + q;
             if (i >= n) continue;
             fcomplex v = {ar[g][q] * back, ai[g][q] * back};
-            if constexpr (DIAG) {
-                if (!wl && !dg.wall[2][i]) {
-                    const float lam = la + dg.lam[2][i];
-                    const float dr = dg.alpha.re + dg.g.re * lam, di = dg.alpha.im + dg.g.im * lam;
-                    const float inv = 1.f / (dr * dr + di * di);
-                    v = {(v.re * dr + v.im * di) * inv, (v.im * dr - v.re * di) * inv};
-                }
+            if constexpr (DIAG) {   // branch-free; the hardware reciprocal (about 1 ulp)
+                const float lam = la + dg.lam[2][i];
+                const float dr = dg.alpha.re + dg.g.re * lam, di = dg.alpha.im + dg.g.im * lam;
+                const float inv = __builtin_amdgcn_rcpf(dr * dr + di * di);
+                const fcomplex d = {(v.re * dr + v.im * di) * inv, (v.im * dr - v.re * di) * inv};
+                if (!(wl || dg.wall[2][i])) v = d;
             }
-            out[base + (long)i * sK] = v;
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), rout,
+                                                  vout + (unsigned)(16 * it + q) * skb, 0, 0);
         }
     }
 }
@@ -2111,6 +2128,8 @@ std::vector<float> axis_fragments(int n, const std::vector<double> &T, bool tran
 
 int axis_split16_blocks(int n) { return n > kWaveAxisMax ? 0 : (n + 31) / 32; }
 
+bool split16_fits(const int n[3]) { return 8L * n[0] * n[1] * n[2] <= 0xFFFFFFFFL; }
+
 std::vector<unsigned short> axis_fragments_split16(int n, const std::vector<double> &T, bool trans, float &ainv)
 {
     const int KB = axis_split16_blocks(n);
@@ -2141,14 +2160,14 @@ std::vector<unsigned short> axis_fragments_split16(int n, const std::vector<doub
 namespace {
 template <bool DIAG>
 void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, long sL, long sK, const unsigned short *Th,
-                         float ainv, const fcomplex *x, fcomplex *y, const DiagF &dg)
+                         float ainv, const fcomplex *x, fcomplex *y, const DiagF &dg, unsigned nbytes)
 {
     const int nit = (nd + 15) / 16, G = nit < 3 ? nit : 3, ngr = (nit + G - 1) / G;
     const unsigned nblk = (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
     const uint4 *T4 = reinterpret_cast<const uint4 *>(Th);
 #define TDZ_S16(KB, GG)                                                                                             \
     hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG, DIAG ? 4 : 3>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, \
-                       sB, sL, sK, nit, T4, ainv, x, y, dg)
+                       sB, sL, sK, nit, T4, ainv, x, y, dg, nbytes)
 #define TDZ_S16_G(KB)                 \
     do {                              \
         if (G == 1) TDZ_S16(KB, 1);   \
@@ -2171,14 +2190,14 @@ void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, l
 bool launch_dense_axis_split16(hipStream_t st, int d, const int n[3], const unsigned short *Th, float ainv,
                                const fcomplex *x, fcomplex *y)
 {
-    if (!Th || !axis_split16_blocks(n[d])) return false;
+    if (!Th || !split16_fits(n) || !axis_split16_blocks(n[d])) return false;
     const long n01 = (long)n[0] * n[1];
     long L, sB, sK, sL;
     unsigned B;
     if (d == 0) { L = (long)n[1] * n[2]; sB = 0; sK = 1; sL = n[0]; B = 1; }
     else if (d == 1) { L = n[0]; sB = n01; sK = n[0]; sL = 1; B = (unsigned)n[2]; }
     else { L = n01; sB = 0; sK = n01; sL = 1; B = 1; }
-    launch_split16_axis<false>(st, n[d], L * B, L, sB, sL, sK, Th, ainv, x, y, DiagF{});
+    launch_split16_axis<false>(st, n[d], L * B, L, sB, sL, sK, Th, ainv, x, y, DiagF{}, (unsigned)(8 * n01 * n[2]));
     return true;
 }
 
@@ -2186,10 +2205,10 @@ bool launch_dense_axis_diag_split16(hipStream_t st, const int n[3], const unsign
                                     const float *lam[3], const unsigned char *wall[3], fcomplex alpha, fcomplex g,
                                     const fcomplex *x, fcomplex *y)
 {
-    if (!Th || !axis_split16_blocks(n[2])) return false;
+    if (!Th || !split16_fits(n) || !axis_split16_blocks(n[2])) return false;
     const long ncol = (long)n[0] * n[1];
     const DiagF dg{{lam[0], lam[1], lam[2]}, {wall[0], wall[1], wall[2]}, alpha, g, n[0]};
-    launch_split16_axis<true>(st, n[2], ncol, ncol, 0, 1, ncol, Th, ainv, x, y, dg);
+    launch_split16_axis<true>(st, n[2], ncol, ncol, 0, 1, ncol, Th, ainv, x, y, dg, (unsigned)(8 * ncol * n[2]));
     return true;
 }
 
