@@ -88,10 +88,11 @@ def main():
                          "last k+1 states")
     ap.add_argument("--alt-predictor", type=int, default=2, choices=[-1, 0, 1, 2, 3, 4],
                     help="guess of the second set of runs reported beside the value (-1 to skip)")
-    ap.add_argument("--workload", default="atom128", choices=["atom128", "fullpulse", "scan", "hetero"],
+    ap.add_argument("--workload", default="atom128", choices=["atom128", "fullpulse", "scan", "hetero", "fdsolve"],
                     help="atom128: the 3D soft-core CN benchmark (default); fullpulse: the whole pulse with "
                          "per-cycle reports; scan: the batched 2D intensity/CEP scan; hetero: the layered "
-                         "heterostructure with Magnus-4")
+                         "heterostructure with Magnus-4; fdsolve: the separable (fast-diagonalisation) solve "
+                         "alone at 128^3, complex64 with each transform kind and FP64 (profiling runs)")
     ap.add_argument("--runs", type=int, default=64, help="scan: number of runs")
     ap.add_argument("--seed", type=int, default=7, help="scan: seed of the drawn grids and pulses")
     ap.add_argument("--kiters", type=int, default=None,
@@ -104,6 +105,8 @@ def main():
         return scan(args)
     if args.workload == "hetero":
         return hetero(args)
+    if args.workload == "fdsolve":
+        return fdsolve(args)
     return atom(args)
 
 
@@ -425,6 +428,46 @@ def scan(args):
 
 NM = 18.897261   # bohr per nanometre
 EV = 1.0 / 27.211386   # hartree per electronvolt
+
+
+def fdsolve(args):
+    """The separable solve of the preconditioners alone on the 128^3 atom's fine level
+    (shift 1 + 0.01i): complex64 with the split-FP16 transforms (kind 1) and the FP32 MFMA
+    transforms (kind 0), and FP64 with the FP64 MFMA transforms and with the VALU ones. A
+    solve is 6 axis transform passes (complex64: the z pass carries the diagonal solve, and
+    r is demoted and z promoted; FP64: a separate diagonal pass).
+    Prints microseconds per solve (wall clock over --steps solves after --warmup)."""
+    import tdsez_amd as T
+    desc = problem(args.nel)
+    desc.pop("mg_levels")
+    ctx = T.Context(desc)
+    ctx.set_shift(1.0, 0.01j, (0, 0, 0))
+    rng = np.random.default_rng(1)
+    r = rng.standard_normal(ctx.N) + 1j * rng.standard_normal(ctx.N)
+    dr, z = ctx.vec(r), ctx.vec()
+    out = {}
+    for name, kind, fdk in (("complex64_split_fp16", T.PC_SEPARABLE_MIXED, 1),
+                            ("complex64_fp32_mfma", T.PC_SEPARABLE_MIXED, 0), ("fp64_mfma", T.PC_SEPARABLE, 1),
+                            ("fp64_valu", T.PC_SEPARABLE, 0)):
+        ctx.pc_setup(kind)
+        if kind == T.PC_SEPARABLE:
+            ctx.set_fd64_transform(fdk)
+        else:
+            ctx.set_fd_transform(fdk)
+        for _ in range(max(args.warmup, 1)):
+            ctx.pc_apply(dr, z)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.pc_apply(dr, z)
+        ctx.sync()
+        out[name] = 1e6 * (time.perf_counter() - t0) / args.steps
+        log(f"{name}: {out[name]:.1f} us per separable solve")
+    ctx.set_fd_transform(1)
+    ctx.set_fd64_transform(1)
+    ctx.close()
+    print(json.dumps({"metric": "separable solve time, 128^3 p=4", "unit": "us", "solves": args.steps,
+                      "us_per_solve": out, "dofs": int(np.prod([n + desc["p"] for n in desc["nelem"]]))}))
 
 
 def hetero_problem(seed, nxy=64, nz=256, p=4):

@@ -171,6 +171,10 @@ int tdz_level_solve(tdz_ctx *ctx, int level, const double *b, double *x);
  * the sum of two FP16 halves, three FP16 products in FP32 accumulation, about 2^-22 relative
  * per product; axes up to 160 functions, longer ones keep FP32). */
 int tdz_set_fd_transform(tdz_ctx *ctx, int kind);
+/* Transforms of the FP64 separable solves (TDZ_PC_SEPARABLE and the FP64 multigrid levels;
+ * no reference counterpart): kind 1 (default) FP64 MFMA for axes up to 160 functions, kind 0
+ * the VALU form (LDS-tiled), which longer axes keep. */
+int tdz_set_fd64_transform(tdz_ctx *ctx, int kind);
 /* Correction sweeps of TDZ_PC_CORRECTED* and TDZ_PC_LEARNED (no reference counterpart):
  * z <- S^-1 (r - dA~ z) applied sweeps times after z = S^-1 r (1..4, default 1), dA~ the
  * terms the separable solve S leaves out (CAP and field exactly, the potential lumped,

@@ -35,7 +35,7 @@ EXPORTS = [
     "tdz_trace_row", "tdz_pade_history", "tdz_batch_create", "tdz_batch_context", "tdz_batch_evolve",
     "tdz_batch_fixups", "tdz_batch_destroy", "tdz_ifunction", "tdz_level_dims", "tdz_mg_transfer",
     "tdz_level_apply", "tdz_set_pc_refresh", "tdz_coarsen_knots", "tdz_refinement_matrix", "tdz_last_warning",
-    "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_observables_noalloc", "tdz_mg_set_smoother", "tdz_mg_set_coarse", "tdz_set_fd_transform", "tdz_mg_coarse_status", "tdz_level_solve", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
+    "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_observables_noalloc", "tdz_mg_set_smoother", "tdz_mg_set_coarse", "tdz_set_fd_transform", "tdz_set_fd64_transform", "tdz_mg_coarse_status", "tdz_level_solve", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
 ]
 
 
@@ -180,6 +180,7 @@ def lib():
         L.tdz_mg_set_smoother.argtypes = [vp, i, i]
         L.tdz_mg_set_coarse.argtypes = [vp, i, C.c_double, i]
         L.tdz_set_fd_transform.argtypes = [vp, i]
+        L.tdz_set_fd64_transform.argtypes = [vp, i]
         L.tdz_mg_coarse_status.argtypes = [vp, ip, dp]
         L.tdz_level_solve.argtypes = [vp, i, vp, vp]
         _lib = L
@@ -330,6 +331,10 @@ class Context:
     def set_smoother(self, kind, degree=2):
         """Multigrid smoother: 0 separable-solve Richardson (default), 1 Jacobi-Chebyshev."""
         _chk(lib().tdz_mg_set_smoother(self.h, int(kind), int(degree)))
+
+    def set_fd64_transform(self, kind):
+        """FP64 separable solves: 1 FP64 MFMA transforms (default), 0 VALU."""
+        _chk(lib().tdz_set_fd64_transform(self.h, int(kind)))
 
     def set_fd_transform(self, kind):
         """Complex64 separable-solve transforms: 0 FP32 MFMA, 1 split FP16 MFMA (default)."""

@@ -252,6 +252,7 @@ struct tdz_ctx {
     int refresh_cap = -1;
     int smoother = 0;           // multigrid smoother: 0 separable-solve Richardson, 1 Jacobi-Chebyshev
     int fd_transform = 1;       // complex64 separable solves: 0 FP32 MFMA, 1 split FP16 MFMA
+    bool fd64_mfma = true;      // FP64 separable solves: FP64 MFMA transforms (else VALU, LDS-tiled)
     int cheb_degree = 2;
     // coarsest multigrid level: 0 separable-solve sweeps, 1 Krylov solve to coarse_rtol
     int coarse_kind = 0, coarse_maxit = 64;
@@ -718,6 +719,11 @@ void setup_fd(tdz_ctx *c, Level &L)
         L.hLam[d] = lam;
         L.d.V[d] = upload(c, V);
         L.d.lam[d] = upload(c, lam);
+        for (int tr = 0; tr < 2; ++tr) {   // the FP64 MFMA transform's fragments (axes it takes)
+            std::vector<double> f;
+            if (V.size() == (size_t)L.n[d] * L.n[d]) f = tdz::axis_fragments_f64(L.n[d], V, tr == 1);
+            L.d.Vf[d][tr] = f.empty() ? nullptr : upload(c, f);
+        }
         if (L.f.ready) upload_fd_float(c, L, d);
     }
     if (L.layered) {
@@ -737,13 +743,18 @@ template <typename R>
 void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<R> *s2)
 {
     Prec<R> &P = prec<R>(L);
+    // the register-resident MFMA transforms' fragments (FP64: unless the VALU form is chosen)
+    const bool frag = !std::is_same<R, double>::value || c->fd64_mfma;
+    R *Vf[3][2];
+    for (int d = 0; d < 3; ++d)
+        for (int tr = 0; tr < 2; ++tr) Vf[d][tr] = frag ? P.Vf[d][tr] : nullptr;
     if (L.layered) {
-        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1, P.Vf[0][1]);
-        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2, P.Vf[1][1]);
+        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1, Vf[0][1]);
+        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2, Vf[1][1]);
         tdz::launch_zline_solve<R>(c->st, L.n, L.ax[2].p, P.lam[0], P.lam[1], P.B[2], P.Mw, cx<R>(c->g), s2, s1,
                                    P.lwork);
-        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, s1, s2, P.Vf[0][0]);
-        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, s2, z, P.Vf[1][0]);
+        tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, s1, s2, Vf[0][0]);
+        tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, s2, z, Vf[1][0]);
         return;
     }
     const R *lam[3] = {P.lam[0], P.lam[1], P.lam[2]};
@@ -762,21 +773,21 @@ void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<
             return;
         }
     }
-    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1, P.Vf[0][1]);
-    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2, P.Vf[1][1]);
+    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], true, r, s1, Vf[0][1]);
+    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], true, s1, s2, Vf[1][1]);
     cpx<R> *d = s2, *e = s1;   // the diagonal solve's output and the other scratch vector
     bool fused = false;
     if constexpr (std::is_same<R, float>::value) {   // z pass and diagonal solve in one kernel
-        fused = tdz::launch_dense_axis_diag(c->st, L.n, P.Vf[2][1], lam, w, cx<R>(c->alpha), cx<R>(c->g), s2, s1);
+        fused = tdz::launch_dense_axis_diag(c->st, L.n, Vf[2][1], lam, w, cx<R>(c->alpha), cx<R>(c->g), s2, s1);
         if (fused) d = s1, e = s2;
     }
     if (!fused) {
-        tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], true, s2, s1, P.Vf[2][1]);
+        tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], true, s2, s1, Vf[2][1]);
         tdz::launch_diag_scale<R>(c->st, L.n, lam, w, cx<R>(c->alpha), cx<R>(c->g), s1, s2);
     }
-    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, d, e, P.Vf[0][0]);
-    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, e, d, P.Vf[1][0]);
-    tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], false, d, z, P.Vf[2][0]);
+    tdz::launch_dense_axis<R>(c->st, 0, L.n, P.V[0], false, d, e, Vf[0][0]);
+    tdz::launch_dense_axis<R>(c->st, 1, L.n, P.V[1], false, e, d, Vf[1][0]);
+    tdz::launch_dense_axis<R>(c->st, 2, L.n, P.V[2], false, d, z, Vf[2][0]);
     if (c->prob.dirichlet) tdz::launch_wall_fix<R>(c->st, L.n, w, r, z);
 }
 
@@ -2229,6 +2240,14 @@ int tdz_mg_set_smoother(tdz_ctx *c, int kind, int degree)
         if (kind == 1 && c->pc_ready &&
             (c->pc_kind == TDZ_PC_MG || c->pc_kind == TDZ_PC_MG_MIXED || c->pc_kind == TDZ_PC_LEARNED))
             cheb_setup(c);
+    });
+}
+
+int tdz_set_fd64_transform(tdz_ctx *c, int kind)
+{
+    return guarded([&] {
+        if (kind != 0 && kind != 1) throw Error("FP64 transform kind must be 0 (VALU) or 1 (FP64 MFMA)");
+        c->fd64_mfma = kind == 1;
     });
 }
 

@@ -219,6 +219,10 @@ std::vector<float> axis_fragments(int n, const std::vector<double> &T, bool tran
 // FP16 halves, FP32 accumulation; about 2^-22 relative): the matrix pre-split and scaled by
 // 2^-e (|A| < 1; ainv = 2^e undoes it) in the fragment order of axis_split16_kernel (empty:
 // axis too long); false if not applicable.
+// FP64 MFMA form (axis_wave_f64_kernel): k-steps of 4 (even, 0 for axes above 160) and the
+// matrix (trans: its transpose) in that kernel's fragment order
+int axis_f64_steps(int n);
+std::vector<double> axis_fragments_f64(int n, const std::vector<double> &T, bool trans);
 int axis_split16_blocks(int n);
 bool split16_fits(const int n[3]);   // the vectors fit the 32-bit byte offsets of its buffer accesses
 std::vector<unsigned short> axis_fragments_split16(int n, const std::vector<double> &T, bool trans, float &ainv);

@@ -1297,6 +1297,62 @@ struct DiagF {   // epilogue of the last forward pass: divide by alpha + g (l0 +
     int n0;
 };
 
+// FP64 transform along one axis on FP64 MFMA (v_mfma_f64_16x16x4_f64; lane l holds
+// A[l%16][l/16], B[l/16][l%16] and D[4q+l/16][l%16], q < 4: rows interleaved across the lane
+// groups, unlike the FP32 and FP16 forms). One wave per (16-column tile,
+// 16-row tile) of the output: its KS k-steps of the matrix row tile (fragment order,
+// double2 per lane: A(16 it + l % 16, 4 (2 s2 + j) + l / 16), zero past n) and its 16 input
+// columns (branch-free: clamped index, zeroed value) in registers, the real and imaginary
+// parts as two products sharing A, each in two accumulator chains. Column numbering and the
+// XCD-aware work mapping are those of axis_wave_mfma_kernel (here the row tiles of a column
+// tile are its "row groups": they run on one CU and share the input lines through L1).
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <int KS>
+__global__ void __launch_bounds__(256) axis_wave_f64_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
+                                                            int nit, const double2 *__restrict__ Tf,
+                                                            const dcomplex *__restrict__ in,
+                                                            dcomplex *__restrict__ out)
+{
+    constexpr int KS2 = KS / 2;
+    const int nb = gridDim.x, b = blockIdx.x, per = nb / 8;
+    const long t = b < 8 * per ? (long)(b % 8) * per + b / 8 : b;
+    const long item = 4 * t + (threadIdx.x >> 6);
+    const long nct = (ncol + 15) / 16;
+    if (item >= nct * nit) return;   // whole wave: no barriers below
+    const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const int it = (int)(item % nit);
+    const long col = (item / nit) * 16 + r16;
+    const bool cv = col < ncol;
+    const long base = cv ? (col / Lin) * sB + (col % Lin) * sL : 0;
+    double a[KS];
+#pragma unroll
+    for (int s2 = 0; s2 < KS2; ++s2) {
+        const double2 v = Tf[((long)it * KS2 + s2) * 64 + lane];
+        a[2 * s2] = v.x;
+        a[2 * s2 + 1] = v.y;
+    }
+    dcomplex bv[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + k4;
+        bv[s] = in[base + (long)min(k, n - 1) * sK];
+        if (!cv || k >= n) bv[s] = {0.0, 0.0};
+    }
+    f64x4 ar[2], ai[2];
+    ar[0] = ar[1] = ai[0] = ai[1] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        ar[s & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bv[s].re, ar[s & 1], 0, 0, 0);
+        ai[s & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bv[s].im, ai[s & 1], 0, 0, 0);
+    }
+    if (!cv) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = 16 * it + 4 * q + k4;   // the FP64 form interleaves the rows
+        if (i < n) out[base + (long)i * sK] = {ar[0][q] + ar[1][q], ai[0][q] + ai[1][q]};
+    }
+}
+
 template <int KS, int G, bool DIAG>
 __global__ void __launch_bounds__(256) axis_wave_mfma_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
                                                              int nit, const float4 *__restrict__ Tf,
@@ -2126,6 +2182,25 @@ std::vector<float> axis_fragments(int n, const std::vector<double> &T, bool tran
     return f;
 }
 
+int axis_f64_steps(int n) { return n > kWaveAxisMax ? 0 : 2 * ((n + 7) / 8); }
+
+std::vector<double> axis_fragments_f64(int n, const std::vector<double> &T, bool trans)
+{
+    const int KS = axis_f64_steps(n);
+    if (!KS) return {};
+    const int KS2 = KS / 2, nit = (n + 15) / 16;
+    std::vector<double> f((size_t)nit * KS2 * 128, 0.0);
+    for (int it = 0; it < nit; ++it)
+        for (int s2 = 0; s2 < KS2; ++s2)
+            for (int l = 0; l < 64; ++l)
+                for (int j = 0; j < 2; ++j) {
+                    const int row = 16 * it + l % 16, k = 4 * (2 * s2 + j) + l / 16;
+                    if (row < n && k < n)
+                        f[(((size_t)it * KS2 + s2) * 64 + l) * 2 + j] = trans ? T[(size_t)k * n + row] : T[(size_t)row * n + k];
+                }
+    return f;
+}
+
 int axis_split16_blocks(int n) { return n > kWaveAxisMax ? 0 : (n + 31) / 32; }
 
 bool split16_fits(const int n[3]) { return 8L * n[0] * n[1] * n[2] <= 0xFFFFFFFFL; }
@@ -2263,9 +2338,29 @@ void launch_dense_axis(hipStream_t st, int d, const int n[3], const R *T, bool t
         }
         hipLaunchKernelGGL(axis_gemm_mfma_kernel, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T,
                            trans ? 1 : 0, x, y);
-    } else
+    } else {
+        const int KS = axis_f64_steps(n[d]);
+        if (Tfrag && KS) {   // FP64 MFMA, a wave per (column tile, row tile)
+            const int nit = (n[d] + 15) / 16;
+            const long ncol = L * B;
+            const unsigned nblk = (unsigned)(((ncol + 15) / 16 * nit + 3) / 4);
+            const double2 *T2 = reinterpret_cast<const double2 *>(Tfrag);
+#define TDZ_F64(K)                                                                                                 \
+    case K:                                                                                                        \
+        hipLaunchKernelGGL(axis_wave_f64_kernel<K>, dim3(nblk), dim3(256), 0, st, n[d], ncol, L, sB, sL, sK, nit, T2, \
+                           x, y);                                                                                  \
+        return;
+            switch (KS) {
+                TDZ_F64(2) TDZ_F64(4) TDZ_F64(6) TDZ_F64(8) TDZ_F64(10) TDZ_F64(12) TDZ_F64(14) TDZ_F64(16)
+                TDZ_F64(18) TDZ_F64(20) TDZ_F64(22) TDZ_F64(24) TDZ_F64(26) TDZ_F64(28) TDZ_F64(30) TDZ_F64(32)
+                TDZ_F64(34) TDZ_F64(36) TDZ_F64(38) TDZ_F64(40)
+            default: break;
+            }
+#undef TDZ_F64
+        }
         hipLaunchKernelGGL(axis_gemm_kernel<R>, grid, dim3(256), 0, st, n[d], L, sB, sK, sL, kfast, T,
                            trans ? 1 : 0, x, y);
+    }
 }
 
 bool launch_dense_axis_diag(hipStream_t st, const int n[3], const float *Tfrag, const float *lam[3],
