@@ -1443,6 +1443,10 @@ __global__ void __launch_bounds__(256) axis_wave_mfma_kernel(int n, long ncol, l
 // column numbering and the diagonal epilogue are those of axis_wave_mfma_kernel.
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+struct fcomplex2 {
+    fcomplex a, b;
+};
 
 // Loads and stores are raw buffer accesses with 32-bit byte offsets (the vectors hold at most
 // 2^32 bytes; nbytes is their size): a lane's offset is its column start plus its k group,
@@ -1450,7 +1454,9 @@ typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
 // bounds (k past the axis, only in the last k block, reads row 0 and is zeroed; padding
 // columns past ncol read the last column, and their output columns are not stored); the
 // buffer's own range check is a second guard.
-template <int KB, int G, bool DIAG, int OCC>
+// KF: the axis is the contiguous one (sK = 1, even line length): a lane's 8 consecutive k of a
+// k block and its 4 consecutive output rows are 16-byte loads and stores (pairs of values).
+template <int KB, int G, bool DIAG, int OCC, bool KF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) axis_split16_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
                                                            int nit, const uint4 *__restrict__ Th, float ainv,
                                                            const fcomplex *__restrict__ in,
@@ -1474,7 +1480,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     const unsigned vin = base8 + (unsigned)(8 * kq) * skb;
     fcomplex bv[KB][8];
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb)
+    for (int kb = 0; kb < KB; ++kb) {
+        if (KF && kb < KB - 1) {   // k < 32 (KB - 1) < n: four 16-byte loads of consecutive k
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const fcomplex2 v = __builtin_bit_cast(
+                    fcomplex2, __builtin_amdgcn_raw_buffer_load_b128(rin, vin + (unsigned)(32 * kb + 2 * jj) * skb, 0, 0));
+                bv[kb][2 * jj] = v.a;
+                bv[kb][2 * jj + 1] = v.b;
+            }
+            continue;
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int k = 32 * kb + 8 * kq + j;
@@ -1484,6 +1500,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
             bv[kb][j] = __builtin_bit_cast(fcomplex, __builtin_amdgcn_raw_buffer_load_b64(rin, off, 0, 0));
             if (!in_axis) bv[kb][j] = {0.f, 0.f};
         }
+    }
     float m = 0.f;
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb)
@@ -1538,6 +1555,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     for (int g = 0; g < G; ++g) {
         const int it = ig * G + g;
         if (it >= nit) break;
+        if (KF && 16 * it + 4 * kq + 3 < n) {   // four consecutive rows: two 16-byte stores
+#pragma unroll
+            for (int qq = 0; qq < 2; ++qq) {
+                const fcomplex2 v = {{ar[g][2 * qq] * back, ai[g][2 * qq] * back},
+                                     {ar[g][2 * qq + 1] * back, ai[g][2 * qq + 1] * back}};
+                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), rout,
+                                                       vout + (unsigned)(16 * it + 2 * qq) * skb, 0, 0);
+            }
+            continue;
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = 16 * it + 4 * kq + q;
@@ -2241,8 +2268,14 @@ void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, l
     const unsigned nblk = (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
     const uint4 *T4 = reinterpret_cast<const uint4 *>(Th);
 #define TDZ_S16(KB, GG)                                                                                             \
-    hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG, DIAG ? 4 : 3>), dim3(nblk), dim3(256), 0, st, nd, ncol, Lin, \
-                       sB, sL, sK, nit, T4, ainv, x, y, dg, nbytes)
+    do {                                                                                                        \
+        if (!DIAG && sK == 1 && nd % 2 == 0)                                                                    \
+            hipLaunchKernelGGL((axis_split16_kernel<KB, GG, false, 3, true>), dim3(nblk), dim3(256), 0, st, nd,  \
+                               ncol, Lin, sB, sL, sK, nit, T4, ainv, x, y, dg, nbytes);                          \
+        else                                                                                                    \
+            hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG, DIAG ? 4 : 3, false>), dim3(nblk), dim3(256), 0, \
+                               st, nd, ncol, Lin, sB, sL, sK, nit, T4, ainv, x, y, dg, nbytes);                  \
+    } while (0)
 #define TDZ_S16_G(KB)                 \
     do {                              \
         if (G == 1) TDZ_S16(KB, 1);   \
