@@ -1368,6 +1368,12 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = 
     for (int k = 0; k < sweeps; ++k) {
         const bool net = use_net && ls.trained && k == sweeps - 1 && (c->corrections > 0 || c->pc_net);
         const bool harvest = k == 0 && ls.harvesting && (int)ls.ys.size() < ls.max_samples && c->ph.pot_kind;
+        // a sweep without the network: its right-hand side r - dA~ z in one z-marching pass
+        if (!harvest && !net && !exact_v &&
+            tdz::fused_correction(c->st, fused_bands(c), L.f.z, L.f.r, c->dVg, cx<float>(c->g), c->sf.u1)) {
+            fd_apply<float>(c, L, c->sf.u1, L.f.z, c->sf.w1, c->sf.w2);
+            continue;
+        }
         if (harvest || net || ls.ready) {
             // M z, g (CAP + sE.D) z and, for the network, its input, in one z-marching pass
             // (equal orders in 3D; three passes otherwise)

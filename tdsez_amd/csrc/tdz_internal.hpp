@@ -162,7 +162,7 @@ struct BandUpdate {
     int cap = 0;
 };
 void launch_band_update(hipStream_t st, const BandUpdate &bu);
-// y = a x + b y (scalars on host)
+// y = a x + b y (scalars on host; b = 0: y is not read)
 void launch_axpby(hipStream_t st, dcomplex a, const dcomplex *x, dcomplex b, dcomplex *y, long N);
 // y = sum_k c_k v_k (k <= kMax terms; y may be one of the v_k: every element is read
 // before it is written)
@@ -363,5 +363,10 @@ struct FusedBands {
 };
 void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, const fcomplex *y, fcomplex *tmp[7],
                     fcomplex *My, fcomplex *corr, fcomplex *u);
+// A correction sweep without the network in one z-marching pass: q = r - g Vg (M y) - corr(y)
+// (corr: the CAP and field bands); false where the fused pass does not apply (unequal
+// orders, not 3D), the caller then takes fused_products and the combine.
+bool fused_correction(hipStream_t st, const FusedBands &fb, const fcomplex *y, const fcomplex *r, const float *Vg,
+                      fcomplex g, fcomplex *q);
 
 }  // namespace tdz

@@ -697,20 +697,25 @@ __global__ void __launch_bounds__(256) sep3_kernel(int n0, int n1, int n2, int k
     }
     const long plane = (long)n0 * n1;
     const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
-    // the plane tile of input plane K, each thread's share in registers: the next
-    // plane's loads are in flight while the current plane is computed
-    constexpr int NX = (H * H + 255) / 256;
-    dcomplex xv[NX];
-    auto load_plane = [&](int K) {
+    // the plane tiles of the next D input planes, each thread's share in registers (plane K in
+    // slot K mod D, compile-time in the loop unrolled by R): D - 1 planes' loads stay in flight
+    // while a plane is computed, so the march does not wait a memory round trip per plane
+    constexpr int NX = (H * H + 255) / 256, D = R % 3 == 0 ? 3 : 1;
+    dcomplex xv[D][NX];
+    auto load_plane = [&](int K, dcomplex(&dst)[NX]) {
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
             const int e = tid + 256 * q;
             const int ii = i0 - P + e % H, jj = j0 - P + e / H;
-            xv[q] = {0.0, 0.0};
-            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) xv[q] = x[K * plane + (long)jj * n0 + ii];
+            dst[q] = {0.0, 0.0};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) dst[q] = x[K * plane + (long)jj * n0 + ii];
         }
     };
-    load_plane(Kbeg);
+#pragma unroll
+    for (int sl = 0; sl < D; ++sl) {   // the first plane of each residue class
+        const int K = Kbeg + ((sl - Kbeg % D) % D + D) % D;
+        if (K < Kend) load_plane(K, xv[sl]);
+    }
     for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -719,10 +724,10 @@ __global__ void __launch_bounds__(256) sep3_kernel(int n0, int n1, int n2, int k
 #pragma unroll
                 for (int q = 0; q < NX; ++q) {
                     const int e = tid + 256 * q;
-                    if (e < H * H) Xs[e] = xv[q];
+                    if (e < H * H) Xs[e] = xv[r % D][q];
                 }
                 __syncthreads();
-                if (K + 1 < Kend) load_plane(K + 1);
+                if (K + D < Kend) load_plane(K + D, xv[r % D]);
                 for (int e = tid; e < H * T; e += 256) {   // e % T == ti: the same x band row
                     const int jj = e / T;
                     double tr = 0, tim = 0, ur = 0, ui = 0;
@@ -988,11 +993,13 @@ __global__ void multi_axpy_kernel(const dcomplex *__restrict__ Z, long ld, int n
     }
 }
 
+// y = a x + b y; with b = 0 y is not read (BLAS semantics: y may hold uninitialised memory)
 __global__ void axpby_kernel(dcomplex a, const dcomplex *__restrict__ x, dcomplex b, dcomplex *__restrict__ y, long N)
 {
     const long stride = (long)gridDim.x * blockDim.x;
+    const bool ry = b.re != 0.0 || b.im != 0.0;
     for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
-        const dcomplex xv = x[I], yv = y[I];
+        const dcomplex xv = x[I], yv = ry ? y[I] : dcomplex{0.0, 0.0};
         y[I] = {a.re * xv.re - a.im * xv.im + b.re * yv.re - b.im * yv.im,
                 a.re * xv.im + a.im * xv.re + b.re * yv.im + b.im * yv.re};
     }
@@ -1027,8 +1034,9 @@ template <typename R>
 __global__ void axpby_t_kernel(cpx<R> a, const cpx<R> *__restrict__ x, cpx<R> b, cpx<R> *__restrict__ y, long N)
 {
     const long stride = (long)gridDim.x * blockDim.x;
+    const bool ry = b.re != R(0) || b.im != R(0);   // b = 0: y is not read
     for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
-        const cpx<R> xv = x[I], yv = y[I];
+        const cpx<R> xv = x[I], yv = ry ? y[I] : cpx<R>{R(0), R(0)};
         y[I] = {a.re * xv.re - a.im * xv.im + b.re * yv.re - b.im * yv.im,
                 a.re * xv.im + a.im * xv.re + b.re * yv.im + b.im * yv.re};
     }
@@ -1719,18 +1727,23 @@ __global__ void __launch_bounds__(256) mass_dip_kernel(int n0, int n1, int n2, i
     }
     const long plane = (long)n0 * n1;
     const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
-    constexpr int NX = (H * H + 255) / 256;
-    dcomplex xv[NX];
-    auto load_plane = [&](int K) {
+    // prefetch ring of D planes (slot K mod D), as in sep3_kernel
+    constexpr int NX = (H * H + 255) / 256, D = R % 3 == 0 ? 3 : 1;
+    dcomplex xv[D][NX];
+    auto load_plane = [&](int K, dcomplex(&dst)[NX]) {
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
             const int e = tid + 256 * q;
             const int ii = i0 - P + e % H, jj = j0 - P + e / H;
-            xv[q] = {0.0, 0.0};
-            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) xv[q] = x[K * plane + (long)jj * n0 + ii];
+            dst[q] = {0.0, 0.0};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) dst[q] = x[K * plane + (long)jj * n0 + ii];
         }
     };
-    load_plane(Kbeg);
+#pragma unroll
+    for (int sl = 0; sl < D; ++sl) {
+        const int K = Kbeg + ((sl - Kbeg % D) % D + D) % D;
+        if (K < Kend) load_plane(K, xv[sl]);
+    }
     for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -1739,10 +1752,10 @@ __global__ void __launch_bounds__(256) mass_dip_kernel(int n0, int n1, int n2, i
 #pragma unroll
                 for (int q = 0; q < NX; ++q) {
                     const int e = tid + 256 * q;
-                    if (e < H * H) Xs[e] = xv[q];
+                    if (e < H * H) Xs[e] = xv[r % D][q];
                 }
                 __syncthreads();
-                if (K + 1 < Kend) load_plane(K + 1);
+                if (K + D < Kend) load_plane(K + D, xv[r % D]);
                 if (xc && own && K >= k0 && K < k1) xc[K * plane + (long)j * n0 + i] = Xs[(tj + P) * H + ti + P];
                 for (int e = tid; e < H * T; e += 256) {   // e % T == ti
                     const int jj = e / T;

@@ -749,11 +749,15 @@ __global__ void feat_z_kernel(int n0, int n1, int n2, int p, const float *__rest
 //   corr = Mz My Cx y + Mz Cy Mx y + Cz My Mx y.
 // A finished plane writes My, corr and (if u) the scaled inputs
 // u = [y, xx - x My, xy - y My, xz - z My]: y is read once instead of the passes' 22
-// vector transfers.
-template <int P>
+// vector transfers. COMB (a correction sweep without the network): only My and corr are
+// formed, and the plane writes the next solve's right-hand side q = r - g Vg My - corr
+// itself (the combine of corr_combine_kernel without its network and exact-product terms).
+template <int P, bool COMB>
 __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int kchunk, FusedBands fb, CnnScales sc,
                                                     const fcomplex *__restrict__ y, fcomplex *__restrict__ My,
-                                                    fcomplex *__restrict__ corr, fcomplex *__restrict__ u)
+                                                    fcomplex *__restrict__ corr, fcomplex *__restrict__ u,
+                                                    const fcomplex *__restrict__ rhs, const float *__restrict__ Vg,
+                                                    fcomplex gc, fcomplex *__restrict__ q)
 {
     constexpr int R = 2 * P + 1, T = 16, H = T + 2 * P;
     __shared__ fcomplex Ys[H * H];
@@ -771,33 +775,40 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
     float mx[R], xx[R], my[R], xy[R];
     fcomplex cx[R], cy[R];
 #pragma unroll
-    for (int l = 0; l < R; ++l) {
+    for (int l = 0; l < R; ++l) {   // COMB: no position bands (they need not exist)
         mx[l] = fb.M[0][ic * R + l];
-        xx[l] = fb.X[0][ic * R + l];
+        xx[l] = COMB ? 0.f : fb.X[0][ic * R + l];
         cx[l] = fb.C[0][ic * R + l];
         my[l] = fb.M[1][jc * R + l];
-        xy[l] = fb.X[1][jc * R + l];
+        xy[l] = COMB ? 0.f : fb.X[1][jc * R + l];
         cy[l] = fb.C[1][jc * R + l];
     }
     fcomplex aM[R], aXX[R], aXY[R], aXZ[R], aC[R];
     fcomplex yr[R];   // y at this thread's (i, j) of the last R staged planes (plane K in slot K mod R)
 #pragma unroll
     for (int l = 0; l < R; ++l) aM[l] = aXX[l] = aXY[l] = aXZ[l] = aC[l] = yr[l] = {0.f, 0.f};
-    const float g0 = fb.g[0][ic], g1 = fb.g[1][jc];
+    const float g0 = COMB ? 0.f : fb.g[0][ic], g1 = COMB ? 0.f : fb.g[1][jc];
     const long plane = (long)n0 * n1;
     const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
-    constexpr int NX = (H * H + 255) / 256;
-    fcomplex yv[NX];
-    auto load_plane = [&](int K) {
+    // the plane tiles of the next D input planes in registers (plane K in slot K mod D,
+    // compile-time in the loop unrolled by R): D - 1 planes' loads stay in flight while a
+    // plane is computed
+    constexpr int NX = (H * H + 255) / 256, D = R % 3 == 0 ? 3 : 1;
+    fcomplex yv[D][NX];
+    auto load_plane = [&](int K, fcomplex(&dst)[NX]) {
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
             const int e = tid + 256 * q;
             const int ii = i0 - P + e % H, jj = j0 - P + e / H;
-            yv[q] = {0.f, 0.f};
-            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) yv[q] = y[K * plane + (long)jj * n0 + ii];
+            dst[q] = {0.f, 0.f};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) dst[q] = y[K * plane + (long)jj * n0 + ii];
         }
     };
-    load_plane(Kbeg);
+#pragma unroll
+    for (int sl = 0; sl < D; ++sl) {   // the first plane of each residue class
+        const int K = Kbeg + ((sl - Kbeg % D) % D + D) % D;
+        if (K < Kend) load_plane(K, yv[sl]);
+    }
     for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -806,11 +817,11 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
 #pragma unroll
                 for (int q = 0; q < NX; ++q) {
                     const int e = tid + 256 * q;
-                    if (e < H * H) Ys[e] = yv[q];
+                    if (e < H * H) Ys[e] = yv[r % D][q];
                 }
                 __syncthreads();
-                if (K + 1 < Kend) load_plane(K + 1);
-                if (u) yr[r] = Ys[(tj + P) * H + ti + P];
+                if (K + D < Kend) load_plane(K + D, yv[r % D]);
+                if (!COMB && u) yr[r] = Ys[(tj + P) * H + ti + P];
                 for (int e = tid; e < H * T; e += 256) {   // e % T == ti
                     const int jj = e / T;
                     float tr = 0, tim = 0, qr = 0, qi = 0, rr = 0, ri = 0;
@@ -819,13 +830,15 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
                         const fcomplex v = Ys[jj * H + ti + l];
                         tr += mx[l] * v.re;
                         tim += mx[l] * v.im;
-                        qr += xx[l] * v.re;
-                        qi += xx[l] * v.im;
+                        if (!COMB) {
+                            qr += xx[l] * v.re;
+                            qi += xx[l] * v.im;
+                        }
                         rr += cx[l].re * v.re - cx[l].im * v.im;
                         ri += cx[l].re * v.im + cx[l].im * v.re;
                     }
                     Ts[e] = {tr, tim};
-                    Qs[e] = {qr, qi};
+                    if (!COMB) Qs[e] = {qr, qi};
                     Rs[e] = {rr, ri};
                 }
                 __syncthreads();
@@ -833,13 +846,16 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
 #pragma unroll
                 for (int l = 0; l < R; ++l) {
                     const int o = (tj + l) * T + ti;
-                    const fcomplex tv = Ts[o], qv = Qs[o], rv = Rs[o];
+                    const fcomplex tv = Ts[o], rv = Rs[o];
                     pp0 += my[l] * tv.re;
                     pp1 += my[l] * tv.im;
-                    px0 += xy[l] * tv.re;
-                    px1 += xy[l] * tv.im;
-                    qq0 += my[l] * qv.re;
-                    qq1 += my[l] * qv.im;
+                    if (!COMB) {
+                        const fcomplex qv = Qs[o];
+                        px0 += xy[l] * tv.re;
+                        px1 += xy[l] * tv.im;
+                        qq0 += my[l] * qv.re;
+                        qq1 += my[l] * qv.im;
+                    }
                     s0 += my[l] * rv.re + cy[l].re * tv.re - cy[l].im * tv.im;
                     s1 += my[l] * rv.im + cy[l].re * tv.im + cy[l].im * tv.re;
                 }
@@ -848,17 +864,19 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
                     const int k = K + d;
                     if (k >= k0 && k < k1) {
                         const int e = k * R + P - d;
-                        const float mz = fb.M[2][e], xz = fb.X[2][e];
+                        const float mz = fb.M[2][e], xz = COMB ? 0.f : fb.X[2][e];
                         const fcomplex cz = fb.C[2][e];
                         const int sl = (r + d + P) % R;
                         aM[sl].re += mz * pp0;
                         aM[sl].im += mz * pp1;
-                        aXX[sl].re += mz * qq0;
-                        aXX[sl].im += mz * qq1;
-                        aXY[sl].re += mz * px0;
-                        aXY[sl].im += mz * px1;
-                        aXZ[sl].re += xz * pp0;
-                        aXZ[sl].im += xz * pp1;
+                        if (!COMB) {
+                            aXX[sl].re += mz * qq0;
+                            aXX[sl].im += mz * qq1;
+                            aXY[sl].re += mz * px0;
+                            aXY[sl].im += mz * px1;
+                            aXZ[sl].re += xz * pp0;
+                            aXZ[sl].im += xz * pp1;
+                        }
                         aC[sl].re += mz * s0 + cz.re * pp0 - cz.im * pp1;
                         aC[sl].im += mz * s1 + cz.re * pp1 + cz.im * pp0;
                     }
@@ -868,9 +886,16 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
             if (ko >= k0 && ko < k1 && own) {
                 const long I = ko * plane + (long)j * n0 + i;
                 const fcomplex m0 = aM[r];
-                My[I] = m0;
-                corr[I] = aC[r];
-                if (u) {
+                if (COMB) {
+                    const fcomplex rv = rhs[I], cv = aC[r];
+                    const float v = Vg[I];
+                    q[I] = {rv.re - v * (gc.re * m0.re - gc.im * m0.im) - cv.re,
+                            rv.im - v * (gc.re * m0.im + gc.im * m0.re) - cv.im};
+                } else {
+                    My[I] = m0;
+                    corr[I] = aC[r];
+                }
+                if (!COMB && u) {
                     const long N = plane * n2;
                     const float g2 = fb.g[2][ko];
                     const fcomplex yc = yr[(r + P + 1) % R];   // plane ko = K - P
@@ -927,8 +952,9 @@ void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, c
         const int chunks = std::max(1, std::min(2 * ncu / tiles, n2 / (2 * p + 2)));
         const int kchunk = (n2 + chunks - 1) / chunks;
         const unsigned nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
-#define TDZ_FEAT3(PP) \
-    hipLaunchKernelGGL(feat3_kernel<PP>, dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, My, corr, u)
+#define TDZ_FEAT3(PP)                                                                                               \
+    hipLaunchKernelGGL((feat3_kernel<PP, false>), dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, My, corr, \
+                       u, nullptr, nullptr, fcomplex{0.f, 0.f}, nullptr)
         switch (p) {
         case 2: TDZ_FEAT3(2); break;
         case 3: TDZ_FEAT3(3); break;
@@ -944,6 +970,37 @@ void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, c
                        fb.C[1], tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], tmp[6]);
     hipLaunchKernelGGL(feat_z_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, n0, n1, n2, fb.p[2], fb.M[2], fb.X[2],
                        fb.C[2], fb.g[0], fb.g[1], fb.g[2], sc, y, tmp[3], tmp[4], tmp[5], tmp[6], My, corr, u);
+}
+
+bool fused_correction(hipStream_t st, const FusedBands &fb, const fcomplex *y, const fcomplex *r, const float *Vg,
+                      fcomplex g, fcomplex *q)
+{
+    const int n0 = fb.n[0], n1 = fb.n[1], n2 = fb.n[2];
+    const int p = fb.p[0];
+    if (!(p == fb.p[1] && p == fb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p)) return false;
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+    }
+    const int tiles = ((n0 + 15) / 16) * ((n1 + 15) / 16);
+    const int chunks = std::max(1, std::min(2 * ncu / tiles, n2 / (2 * p + 2)));
+    const int kchunk = (n2 + chunks - 1) / chunks;
+    const unsigned nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
+    const CnnScales sc{{1.f, 1.f, 1.f, 1.f}};
+#define TDZ_FC(PP)                                                                                                \
+    hipLaunchKernelGGL((feat3_kernel<PP, true>), dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, nullptr, \
+                       nullptr, nullptr, r, Vg, g, q)
+    switch (p) {
+    case 2: TDZ_FC(2); break;
+    case 3: TDZ_FC(3); break;
+    case 4: TDZ_FC(4); break;
+    default: TDZ_FC(5); break;
+    }
+#undef TDZ_FC
+    return true;
 }
 
 namespace {
