@@ -697,25 +697,20 @@ __global__ void __launch_bounds__(256) sep3_kernel(int n0, int n1, int n2, int k
     }
     const long plane = (long)n0 * n1;
     const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
-    // the plane tiles of the next D input planes, each thread's share in registers (plane K in
-    // slot K mod D, compile-time in the loop unrolled by R): D - 1 planes' loads stay in flight
-    // while a plane is computed, so the march does not wait a memory round trip per plane
-    constexpr int NX = (H * H + 255) / 256, D = R % 3 == 0 ? 3 : 1;
-    dcomplex xv[D][NX];
-    auto load_plane = [&](int K, dcomplex(&dst)[NX]) {
+    // the plane tile of input plane K, each thread's share in registers: the next
+    // plane's loads are in flight while the current plane is computed
+    constexpr int NX = (H * H + 255) / 256;
+    dcomplex xv[NX];
+    auto load_plane = [&](int K) {
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
             const int e = tid + 256 * q;
             const int ii = i0 - P + e % H, jj = j0 - P + e / H;
-            dst[q] = {0.0, 0.0};
-            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) dst[q] = x[K * plane + (long)jj * n0 + ii];
+            xv[q] = {0.0, 0.0};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) xv[q] = x[K * plane + (long)jj * n0 + ii];
         }
     };
-#pragma unroll
-    for (int sl = 0; sl < D; ++sl) {   // the first plane of each residue class
-        const int K = Kbeg + ((sl - Kbeg % D) % D + D) % D;
-        if (K < Kend) load_plane(K, xv[sl]);
-    }
+    load_plane(Kbeg);
     for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -724,10 +719,10 @@ __global__ void __launch_bounds__(256) sep3_kernel(int n0, int n1, int n2, int k
 #pragma unroll
                 for (int q = 0; q < NX; ++q) {
                     const int e = tid + 256 * q;
-                    if (e < H * H) Xs[e] = xv[r % D][q];
+                    if (e < H * H) Xs[e] = xv[q];
                 }
                 __syncthreads();
-                if (K + D < Kend) load_plane(K + D, xv[r % D]);
+                if (K + 1 < Kend) load_plane(K + 1);
                 for (int e = tid; e < H * T; e += 256) {   // e % T == ti: the same x band row
                     const int jj = e / T;
                     double tr = 0, tim = 0, ur = 0, ui = 0;
@@ -1727,23 +1722,18 @@ __global__ void __launch_bounds__(256) mass_dip_kernel(int n0, int n1, int n2, i
     }
     const long plane = (long)n0 * n1;
     const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
-    // prefetch ring of D planes (slot K mod D), as in sep3_kernel
-    constexpr int NX = (H * H + 255) / 256, D = R % 3 == 0 ? 3 : 1;
-    dcomplex xv[D][NX];
-    auto load_plane = [&](int K, dcomplex(&dst)[NX]) {
+    constexpr int NX = (H * H + 255) / 256;
+    dcomplex xv[NX];
+    auto load_plane = [&](int K) {
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
             const int e = tid + 256 * q;
             const int ii = i0 - P + e % H, jj = j0 - P + e / H;
-            dst[q] = {0.0, 0.0};
-            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) dst[q] = x[K * plane + (long)jj * n0 + ii];
+            xv[q] = {0.0, 0.0};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) xv[q] = x[K * plane + (long)jj * n0 + ii];
         }
     };
-#pragma unroll
-    for (int sl = 0; sl < D; ++sl) {
-        const int K = Kbeg + ((sl - Kbeg % D) % D + D) % D;
-        if (K < Kend) load_plane(K, xv[sl]);
-    }
+    load_plane(Kbeg);
     for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -1752,10 +1742,10 @@ __global__ void __launch_bounds__(256) mass_dip_kernel(int n0, int n1, int n2, i
 #pragma unroll
                 for (int q = 0; q < NX; ++q) {
                     const int e = tid + 256 * q;
-                    if (e < H * H) Xs[e] = xv[r % D][q];
+                    if (e < H * H) Xs[e] = xv[q];
                 }
                 __syncthreads();
-                if (K + D < Kend) load_plane(K + D, xv[r % D]);
+                if (K + 1 < Kend) load_plane(K + 1);
                 if (xc && own && K >= k0 && K < k1) xc[K * plane + (long)j * n0 + i] = Xs[(tj + P) * H + ti + P];
                 for (int e = tid; e < H * T; e += 256) {   // e % T == ti
                     const int jj = e / T;

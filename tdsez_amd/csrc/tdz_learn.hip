@@ -790,25 +790,18 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
     const float g0 = COMB ? 0.f : fb.g[0][ic], g1 = COMB ? 0.f : fb.g[1][jc];
     const long plane = (long)n0 * n1;
     const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
-    // the plane tiles of the next D input planes in registers (plane K in slot K mod D,
-    // compile-time in the loop unrolled by R): D - 1 planes' loads stay in flight while a
-    // plane is computed
-    constexpr int NX = (H * H + 255) / 256, D = R % 3 == 0 ? 3 : 1;
-    fcomplex yv[D][NX];
-    auto load_plane = [&](int K, fcomplex(&dst)[NX]) {
+    constexpr int NX = (H * H + 255) / 256;
+    fcomplex yv[NX];
+    auto load_plane = [&](int K) {
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
             const int e = tid + 256 * q;
             const int ii = i0 - P + e % H, jj = j0 - P + e / H;
-            dst[q] = {0.f, 0.f};
-            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) dst[q] = y[K * plane + (long)jj * n0 + ii];
+            yv[q] = {0.f, 0.f};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) yv[q] = y[K * plane + (long)jj * n0 + ii];
         }
     };
-#pragma unroll
-    for (int sl = 0; sl < D; ++sl) {   // the first plane of each residue class
-        const int K = Kbeg + ((sl - Kbeg % D) % D + D) % D;
-        if (K < Kend) load_plane(K, yv[sl]);
-    }
+    load_plane(Kbeg);
     for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -817,10 +810,10 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
 #pragma unroll
                 for (int q = 0; q < NX; ++q) {
                     const int e = tid + 256 * q;
-                    if (e < H * H) Ys[e] = yv[r % D][q];
+                    if (e < H * H) Ys[e] = yv[q];
                 }
                 __syncthreads();
-                if (K + D < Kend) load_plane(K + D, yv[r % D]);
+                if (K + 1 < Kend) load_plane(K + 1);
                 if (!COMB && u) yr[r] = Ys[(tj + P) * H + ti + P];
                 for (int e = tid; e < H * T; e += 256) {   // e % T == ti
                     const int jj = e / T;
