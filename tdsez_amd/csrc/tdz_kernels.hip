@@ -310,7 +310,7 @@ pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr,
     __shared__ cpx<R> T1[TW * TQ];    // x stage      [jy][qx]
     __shared__ cpx<R> G2p[NCOL];      // output plane at the points [qy][qx]
     __shared__ cpx<R> H1[TW * TQ];    // transposed y [jy][qx]
-    __shared__ R Bxs[TE * NP * Q], Bys[TE * NP * Q], Bzs[NP * Q], z2s[Q], wzs[Q];
+    __shared__ R Bxs[TE * NP * Q], Bys[TE * NP * Q];
 
     const int tid = threadIdx.x;
     const int ex0 = blockIdx.x * TE, ey0 = blockIdx.y * TE;
@@ -351,19 +351,11 @@ pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr,
 #pragma unroll
         for (int a = 0; a < NP; ++a) tw[c][a][0] = tw[c][a][1] = gw[c][a][0] = gw[c][a][1] = 0.0;
 
-    // A: plane jz -> window slot `slot` (also stages the z tables of element ez when ez >= 0)
-    auto stage_in = [&](int jz, int slot, int ez) {
+    // A: plane jz -> window slot `slot`
+    auto stage_in = [&](int jz, int slot) {
         for (int i = tid; i < TW * TW; i += NT) {
             const int jx = ex0 + i % TW, jy = ey0 + i / TW;
             cin[i] = (jx < n0 && jy < n1) ? x[jx + (long)n0 * jy + plane * jz] : cpx<R>{0.0, 0.0};
-        }
-        if (ez >= 0) {
-            for (int i = tid; i < NP * Q; i += NT) Bzs[i] = az.B0[(long)ez * NP * Q + i];
-            for (int i = tid; i < Q; i += NT) {
-                const R zv = az.X[(long)ez * Q + i];
-                z2s[i] = zv * zv;
-                wzs[i] = az.W[(long)ez * Q + i];
-            }
         }
         __syncthreads();
         for (int i = tid; i < TW * TQ; i += NT) {
@@ -449,10 +441,16 @@ pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr,
         // last readers ran before this emit's barriers
     };
 
-    for (int s = 0; s < P; ++s) stage_in(s, s, -1);
+    for (int s = 0; s < P; ++s) stage_in(s, s);
     for (int ez = 0; ez < Ez; ++ez) {
-        stage_in(ez + P, P, ez);
-        // B: z stage, coefficient, transposed z stage
+        stage_in(ez + P, P);
+        // B: z stage, coefficient, transposed z stage. The z tables of element ez are the same
+        // for every lane: read from global memory at a wave-uniform address they become scalar
+        // loads and SGPR operands of the FMAs (no LDS traffic for them)
+        typedef const __attribute__((address_space(4))) R *cptr;   // constant address space: invariant
+        const cptr bz = (cptr)(az.B0 + (long)ez * NP * Q);
+        const cptr zq = (cptr)(az.X + (long)ez * Q);
+        const cptr wq = (cptr)(az.W + (long)ez * Q);
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
             if (!valid[c]) continue;
@@ -461,17 +459,17 @@ pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr,
                 R fr = 0.0, fi = 0.0;
 #pragma unroll
                 for (int a = 0; a < NP; ++a) {
-                    const R b = Bzs[a * Q + qz];
+                    const R b = bz[a * Q + qz];
                     fr += b * tw[c][a][0];
                     fi += b * tw[c][a][1];
                 }
-                const R r2 = xy2[c] + z2s[qz];
-                const R v = (KIND == 2 ? rsqrt_pos(r2) : r2) * (wxy[c] * wzs[qz]);
+                const R r2 = xy2[c] + zq[qz] * zq[qz];
+                const R v = (KIND == 2 ? rsqrt_pos(r2) : r2) * (wxy[c] * wq[qz]);
                 fr *= v;
                 fi *= v;
 #pragma unroll
                 for (int a = 0; a < NP; ++a) {
-                    const R b = Bzs[a * Q + qz];
+                    const R b = bz[a * Q + qz];
                     gw[c][a][0] += b * fr;
                     gw[c][a][1] += b * fi;
                 }
