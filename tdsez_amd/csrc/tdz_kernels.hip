@@ -385,15 +385,13 @@ pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr,
         }
     };
 
-    // C: emit output plane jz from gw[.][0], then shift the output window
-    auto emit = [&](int jz) {
+    // C: emit output plane jz from ring slot `slot` (then cleared: it becomes plane jz + P + 1)
+    auto emit = [&](int jz, int slot) {
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-            const R r = gw[c][0][0], i = gw[c][0][1];
+            const R r = gw[c][slot][0], i = gw[c][slot][1];
             G2p[tid + c * NT] = {gr * r - gi * i, gr * i + gi * r};
-#pragma unroll
-            for (int a = 0; a < P; ++a) { gw[c][a][0] = gw[c][a + 1][0]; gw[c][a][1] = gw[c][a + 1][1]; }
-            gw[c][P][0] = gw[c][P][1] = 0.0;
+            gw[c][slot][0] = gw[c][slot][1] = 0.0;
         }
         __syncthreads();
         for (int i = tid; i < TW * TQ; i += NT) {
@@ -441,48 +439,52 @@ pot3d_kernel(AxisDevT<R> ax, AxisDevT<R> ay, AxisDevT<R> az, PotParams pp, R gr,
         // last readers ran before this emit's barriers
     };
 
+    // the z windows are rings of NP planes (plane j in slot j mod NP): with the layer loop
+    // unrolled by NP every slot index is a compile-time constant and no window is shifted
+#pragma unroll
     for (int s = 0; s < P; ++s) stage_in(s, s);
-    for (int ez = 0; ez < Ez; ++ez) {
-        stage_in(ez + P, P);
-        // B: z stage, coefficient, transposed z stage. The z tables of element ez are the same
-        // for every lane: read from global memory at a wave-uniform address they become scalar
-        // loads and SGPR operands of the FMAs (no LDS traffic for them)
-        typedef const __attribute__((address_space(4))) R *cptr;   // constant address space: invariant
-        const cptr bz = (cptr)(az.B0 + (long)ez * NP * Q);
-        const cptr zq = (cptr)(az.X + (long)ez * Q);
-        const cptr wq = (cptr)(az.W + (long)ez * Q);
+    for (int e0 = 0; e0 < Ez + P; e0 += NP) {
 #pragma unroll
-        for (int c = 0; c < CPT; ++c) {
-            if (!valid[c]) continue;
+        for (int u = 0; u < NP; ++u) {
+            const int ez = e0 + u;   // block-uniform
+            if (ez >= Ez + P) break;
+            if (ez < Ez) {
+                stage_in(ez + P, (u + P) % NP);
+                // B: z stage, coefficient, transposed z stage. The z tables of element ez are the same
+                // for every lane: read from global memory at a wave-uniform address they become scalar
+                // loads and SGPR operands of the FMAs (no LDS traffic for them)
+                typedef const __attribute__((address_space(4))) R *cptr;   // constant address space: invariant
+                const cptr bz = (cptr)(az.B0 + (long)ez * NP * Q);
+                const cptr zq = (cptr)(az.X + (long)ez * Q);
+                const cptr wq = (cptr)(az.W + (long)ez * Q);
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    if (!valid[c]) continue;
 #pragma unroll 2
-            for (int qz = 0; qz < Q; ++qz) {
-                R fr = 0.0, fi = 0.0;
+                    for (int qz = 0; qz < Q; ++qz) {
+                        R fr = 0.0, fi = 0.0;
 #pragma unroll
-                for (int a = 0; a < NP; ++a) {
-                    const R b = bz[a * Q + qz];
-                    fr += b * tw[c][a][0];
-                    fi += b * tw[c][a][1];
-                }
-                const R r2 = xy2[c] + zq[qz] * zq[qz];
-                const R v = (KIND == 2 ? rsqrt_pos(r2) : r2) * (wxy[c] * wq[qz]);
-                fr *= v;
-                fi *= v;
+                        for (int a = 0; a < NP; ++a) {
+                            const R b = bz[a * Q + qz];
+                            fr += b * tw[c][(u + a) % NP][0];
+                            fi += b * tw[c][(u + a) % NP][1];
+                        }
+                        const R r2 = xy2[c] + zq[qz] * zq[qz];
+                        const R v = (KIND == 2 ? rsqrt_pos(r2) : r2) * (wxy[c] * wq[qz]);
+                        fr *= v;
+                        fi *= v;
 #pragma unroll
-                for (int a = 0; a < NP; ++a) {
-                    const R b = bz[a * Q + qz];
-                    gw[c][a][0] += b * fr;
-                    gw[c][a][1] += b * fi;
+                        for (int a = 0; a < NP; ++a) {
+                            const R b = bz[a * Q + qz];
+                            gw[c][(u + a) % NP][0] += b * fr;
+                            gw[c][(u + a) % NP][1] += b * fi;
+                        }
+                    }
                 }
             }
+            emit(ez, u);
         }
-        // shift the input window
-#pragma unroll
-        for (int c = 0; c < CPT; ++c)
-#pragma unroll
-            for (int a = 0; a < P; ++a) { tw[c][a][0] = tw[c][a + 1][0]; tw[c][a][1] = tw[c][a + 1][1]; }
-        emit(ez);
     }
-    for (int k = 1; k <= P; ++k) emit(Ez - 1 + k);
 }
 
 // y[I] += the contributions of the up to MW^2 tiles that cover function I, summed in a
