@@ -446,7 +446,8 @@ def fdsolve(args):
     r = rng.standard_normal(ctx.N) + 1j * rng.standard_normal(ctx.N)
     dr, z = ctx.vec(r), ctx.vec()
     out = {}
-    for name, kind, fdk in (("complex64_split_fp16", T.PC_SEPARABLE_MIXED, 1),
+    for name, kind, fdk in (("complex64_split_fp16_lds", T.PC_SEPARABLE_MIXED, 2),
+                            ("complex64_split_fp16", T.PC_SEPARABLE_MIXED, 1),
                             ("complex64_fp32_mfma", T.PC_SEPARABLE_MIXED, 0), ("fp64_mfma", T.PC_SEPARABLE, 1),
                             ("fp64_valu", T.PC_SEPARABLE, 0)):
         ctx.pc_setup(kind)
@@ -463,6 +464,21 @@ def fdsolve(args):
         ctx.sync()
         out[name] = 1e6 * (time.perf_counter() - t0) / args.steps
         log(f"{name}: {out[name]:.1f} us per separable solve")
+    # the network-free corrected solve with k correction sweeps: k + 1 separable solves and
+    # k fused correction passes
+    ctx.pc_setup(T.PC_CORRECTED)
+    ctx.set_fd_transform(1)
+    for k in (1, 2):
+        ctx.set_corrections(k)
+        for _ in range(max(args.warmup, 1)):
+            ctx.pc_apply(dr, z)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.pc_apply(dr, z)
+        ctx.sync()
+        out[f"corrected_{k}_sweeps"] = 1e6 * (time.perf_counter() - t0) / args.steps
+        log(f"corrected, {k} sweeps: {out[f'corrected_{k}_sweeps']:.1f} us per preconditioner apply")
     ctx.set_fd_transform(1)
     ctx.set_fd64_transform(1)
     ctx.close()

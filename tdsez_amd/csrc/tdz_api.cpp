@@ -762,13 +762,14 @@ void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<
     if constexpr (std::is_same<R, float>::value) {
         // split-FP16 transforms on every axis that takes them (the z pass with the diagonal solve)
         if (c->fd_transform > 0 && P.Vh[0][0] && P.Vh[1][0] && P.Vh[2][0] && tdz::split16_fits(L.n)) {
-            tdz::launch_dense_axis_split16(c->st, 0, L.n, P.Vh[0][1], P.Vh_inv[0][1], r, s1);
-            tdz::launch_dense_axis_split16(c->st, 1, L.n, P.Vh[1][1], P.Vh_inv[1][1], s1, s2);
+            const int k16 = c->fd_transform;
+            tdz::launch_dense_axis_split16(c->st, 0, L.n, P.Vh[0][1], P.Vh_inv[0][1], r, s1, k16);
+            tdz::launch_dense_axis_split16(c->st, 1, L.n, P.Vh[1][1], P.Vh_inv[1][1], s1, s2, k16);
             tdz::launch_dense_axis_diag_split16(c->st, L.n, P.Vh[2][1], P.Vh_inv[2][1], lam, w, cx<R>(c->alpha),
-                                                cx<R>(c->g), s2, s1);
-            tdz::launch_dense_axis_split16(c->st, 0, L.n, P.Vh[0][0], P.Vh_inv[0][0], s1, s2);
-            tdz::launch_dense_axis_split16(c->st, 1, L.n, P.Vh[1][0], P.Vh_inv[1][0], s2, s1);
-            tdz::launch_dense_axis_split16(c->st, 2, L.n, P.Vh[2][0], P.Vh_inv[2][0], s1, z);
+                                                cx<R>(c->g), s2, s1, k16);
+            tdz::launch_dense_axis_split16(c->st, 0, L.n, P.Vh[0][0], P.Vh_inv[0][0], s1, s2, k16);
+            tdz::launch_dense_axis_split16(c->st, 1, L.n, P.Vh[1][0], P.Vh_inv[1][0], s2, s1, k16);
+            tdz::launch_dense_axis_split16(c->st, 2, L.n, P.Vh[2][0], P.Vh_inv[2][0], s1, z, k16);
             if (c->prob.dirichlet) tdz::launch_wall_fix<R>(c->st, L.n, w, r, z);
             return;
         }
@@ -1269,8 +1270,8 @@ void random_samples(tdz_ctx *c, int count, unsigned long long seed)
 // the residual by rho per Krylov iteration, so the solve needs about
 // its = ceil(log(tol / r0) / log(rho)) iterations (tol the relative tolerance, r0 the
 // solve's initial relative residual), each costing an operator apply and the Krylov
-// work (about 2.5 units of 1 ms at 128^3) plus the preconditioner (0.17 units per separable
-// solve, 0.07 per correction, 0.19 for the network). rho is the mean contraction per
+// work (about 2.1 units of 1 ms at 128^3) plus the preconditioner (0.13 units per separable
+// solve, 0.062 per correction, 0.24 for the network). rho is the mean contraction per
 // iteration, (final / initial residual)^(1/its), of the last solve that made that choice;
 // an unobserved sweep count is estimated from
 // an observed one of the same choice as rho_j = rho_k^((j+1)/(k+1)) (the error after k
@@ -1314,7 +1315,9 @@ void choose_sweeps(tdz_ctx *c)
             // observed mean contraction of an its-iteration solve reproduces its iterations)
             const double its =
                 target >= 1.0 ? 1.0 : std::max(1.0, std::ceil(std::log(target) / std::log(rho) - 0.05));
-            const double cost = its * (2.5 + 0.17 * (k + 1) + 0.07 * k + (net ? 0.19 : 0.0));
+            // ms at 128^3: operator apply and Krylov work per iteration, a complex64 separable
+            // solve, a fused correction pass, the network (its feature pass, two layers, combine)
+            const double cost = its * (2.1 + 0.13 * (k + 1) + 0.062 * k + (net ? 0.24 : 0.0));
             if (cost < best_cost * (1 - 1e-9)) {
                 best_cost = cost;
                 best = k;
@@ -2260,7 +2263,8 @@ int tdz_set_fd64_transform(tdz_ctx *c, int kind)
 int tdz_set_fd_transform(tdz_ctx *c, int kind)
 {
     return guarded([&] {
-        if (kind != 0 && kind != 1) throw Error("transform kind must be 0 (FP32 MFMA) or 1 (split FP16 MFMA)");
+        if (kind < 0 || kind > 2)
+            throw Error("transform kind must be 0 (FP32 MFMA), 1 (split FP16 MFMA) or 2 (split FP16 MFMA, matrix in LDS)");
         c->fd_transform = kind;
     });
 }

@@ -226,11 +226,13 @@ std::vector<double> axis_fragments_f64(int n, const std::vector<double> &T, bool
 int axis_split16_blocks(int n);
 bool split16_fits(const int n[3]);   // the vectors fit the 32-bit byte offsets of its buffer accesses
 std::vector<unsigned short> axis_fragments_split16(int n, const std::vector<double> &T, bool trans, float &ainv);
+// split16: 1 a wave per (column tile, three row tiles); 2 a wave per column tile and every
+// row tile, the matrix staged in LDS per block
 bool launch_dense_axis_split16(hipStream_t st, int d, const int n[3], const unsigned short *Th, float ainv,
-                               const fcomplex *x, fcomplex *y);
+                               const fcomplex *x, fcomplex *y, int split16);
 bool launch_dense_axis_diag_split16(hipStream_t st, const int n[3], const unsigned short *Th, float ainv,
                                     const float *lam[3], const unsigned char *wall[3], fcomplex alpha, fcomplex g,
-                                    const fcomplex *x, fcomplex *y);
+                                    const fcomplex *x, fcomplex *y, int split16);
 // One thread per output, T read through the caches (reference kernel for tests).
 // complex64 forward transform along z fused with the diagonal solve (Tfrag: fragments
 // of T^T; then divide by alpha + g (l0 + l1 + l2) off the walls); false if not applicable

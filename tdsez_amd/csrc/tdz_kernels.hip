@@ -1586,6 +1586,161 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     }
 }
 
+// The split-FP16 transform with a wave per 16-column tile and ALL NIT row tiles: the input
+// column tile is loaded, scaled and split once (axis_split16_kernel splits it once per
+// group of three row tiles), and the matrix's fragments are staged once per block in LDS,
+// where every wave of the block reads them (ds_read_b128, one 16-byte entry per lane).
+// LDS layout per row tile: the KB - 1 full k blocks as [kb][hi, lo][64 lanes], then the last
+// k block with only its first vg lane groups (the groups holding some k < n), [hi, lo][16 vg];
+// a lane of a later group reads lane r16 of the first group instead, a finite value whose
+// product with the zeroed input (k >= n) vanishes. At n = 132 that is 78 KB per block, so
+// two blocks share a CU. Same operand layout, scaling, buffer accesses and epilogues as
+// axis_split16_kernel.
+template <int NIT, bool DIAG, bool KF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+axis_split16_lds_kernel(int n, long ncol, long Lin, long sB, long sL, long sK, const uint4 *__restrict__ Th, int vg,
+                        float ainv, const fcomplex *__restrict__ in, fcomplex *__restrict__ out, DiagF dg,
+                        unsigned nbytes)
+{
+    constexpr int KB = (NIT + 1) / 2;
+    extern __shared__ uint4 As[];
+    const int itS = (KB - 1) * 128 + 32 * vg;   // entries per row tile
+    const int nb = gridDim.x, b = blockIdx.x, per = nb / 8;
+    const long t = b < 8 * per ? (long)(b % 8) * per + b / 8 : b;
+    const long ct = 4 * t + (threadIdx.x >> 6);
+    const long nct = (ncol + 15) / 16;
+    const bool live = ct < nct;   // a wave past the last tile only helps stage the matrix
+    const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+    const long col = ct * 16 + r16;
+    const bool cv = live && col < ncol;
+    const unsigned colu = (unsigned)(cv ? col : ncol - 1), linu = (unsigned)Lin, cq = colu / linu;
+    const unsigned base8 = 8u * (cq * (unsigned)sB + (colu - cq * linu) * (unsigned)sL);
+    const unsigned skb = 8u * (unsigned)sK;
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<fcomplex *>(in), (short)0, (int)nbytes, 0x00020000);
+    const unsigned vin = base8 + (unsigned)(8 * kq) * skb;
+    fcomplex bv[KB][8];
+    if (live) {   // the column loads go out before the matrix is staged
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            if (KF && kb < KB - 1) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const fcomplex2 v = __builtin_bit_cast(
+                        fcomplex2,
+                        __builtin_amdgcn_raw_buffer_load_b128(rin, vin + (unsigned)(32 * kb + 2 * jj) * skb, 0, 0));
+                    bv[kb][2 * jj] = v.a;
+                    bv[kb][2 * jj + 1] = v.b;
+                }
+                continue;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = 32 * kb + 8 * kq + j;
+                const bool in_axis = kb < KB - 1 || k < n;
+                const unsigned off = in_axis ? vin + (unsigned)(32 * kb + j) * skb : base8;
+                bv[kb][j] = __builtin_bit_cast(fcomplex, __builtin_amdgcn_raw_buffer_load_b64(rin, off, 0, 0));
+                if (!in_axis) bv[kb][j] = {0.f, 0.f};
+            }
+        }
+    }
+    // stage the matrix (source: the padded fragment order [it][kb][hi, lo][64 lanes])
+    const int full = (KB - 1) * 128, last = 16 * vg;
+    for (int e = threadIdx.x; e < NIT * itS; e += 256) {
+        const int it = e / itS, rem = e - it * itS;
+        int src;
+        if (rem < full) {
+            src = it * KB * 128 + rem;
+        } else {
+            const int r2 = rem - full, h = r2 >= last ? 1 : 0, l = r2 - h * last;
+            src = ((it * KB + KB - 1) * 2 + h) * 64 + l;
+        }
+        As[e] = Th[src];
+    }
+    __syncthreads();
+    if (!live) return;
+    float m = 0.f;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) m = fmaxf(m, fmaxf(fabsf(bv[kb][j].re), fabsf(bv[kb][j].im)));
+    m = fmaxf(m, __shfl_xor(m, 16, 64));
+    m = fmaxf(m, __shfl_xor(m, 32, 64));
+    int e = 0;
+    if (m > 0.f) frexpf(m, &e);
+    const float sc = ldexpf(1.f, -e);
+    half8_t rh[KB], rl[KB], ih[KB], il[KB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float xr = bv[kb][j].re * sc, xi = bv[kb][j].im * sc;
+            const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+            rh[kb][j] = hr;
+            rl[kb][j] = (_Float16)(xr - (float)hr);
+            ih[kb][j] = hi;
+            il[kb][j] = (_Float16)(xi - (float)hi);
+        }
+    f32x4 ar[NIT], ai[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) ar[it] = ai[it] = {0.f, 0.f, 0.f, 0.f};
+    const int ll = lane < last ? lane : r16;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int o = it * itS + kb * 128;
+            const bool lb = kb == KB - 1;
+            const half8_t ah = __builtin_bit_cast(half8_t, As[o + (lb ? ll : lane)]);
+            const half8_t al = __builtin_bit_cast(half8_t, As[o + (lb ? last + ll : 64 + lane)]);
+            ar[it] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, rh[kb], ar[it], 0, 0, 0);
+            ai[it] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, ih[kb], ai[it], 0, 0, 0);
+            ar[it] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rl[kb], ar[it], 0, 0, 0);
+            ai[it] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, il[kb], ai[it], 0, 0, 0);
+            ar[it] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rh[kb], ar[it], 0, 0, 0);
+            ai[it] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, ih[kb], ai[it], 0, 0, 0);
+        }
+    if (!cv) return;
+    const float back = ainv / sc;
+    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(out, (short)0, (int)nbytes, 0x00020000);
+    const unsigned vout = base8 + (unsigned)(4 * kq) * skb;
+    float la = 0.f;
+    bool wl = false;
+    if constexpr (DIAG) {
+        const int i0 = (int)(col % dg.n0), i1 = (int)(col / dg.n0);
+        la = dg.lam[0][i0] + dg.lam[1][i1];
+        wl = (dg.wall[0][i0] | dg.wall[1][i1]) != 0;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        if (KF && 16 * it + 4 * kq + 3 < n) {
+#pragma unroll
+            for (int qq = 0; qq < 2; ++qq) {
+                const fcomplex2 v = {{ar[it][2 * qq] * back, ai[it][2 * qq] * back},
+                                     {ar[it][2 * qq + 1] * back, ai[it][2 * qq + 1] * back}};
+                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), rout,
+                                                       vout + (unsigned)(16 * it + 2 * qq) * skb, 0, 0);
+            }
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 16 * it + 4 * kq + q;
+            if (i >= n) continue;
+            fcomplex v = {ar[it][q] * back, ai[it][q] * back};
+            if constexpr (DIAG) {
+                const float lam = la + dg.lam[2][i];
+                const float dr = dg.alpha.re + dg.g.re * lam, di = dg.alpha.im + dg.g.im * lam;
+                const float inv = __builtin_amdgcn_rcpf(dr * dr + di * di);
+                const fcomplex d = {(v.re * dr + v.im * di) * inv, (v.im * dr - v.re * di) * inv};
+                if (!(wl || dg.wall[2][i])) v = d;
+            }
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), rout,
+                                                  vout + (unsigned)(16 * it + q) * skb, 0, 0);
+        }
+    }
+}
+
 // largest axis length the register-resident transform takes; longer axes use the
 // LDS-tiled kernel
 constexpr int kWaveAxisMax = 160;
@@ -2263,10 +2418,62 @@ std::vector<unsigned short> axis_fragments_split16(int n, const std::vector<doub
 }
 
 namespace {
+// one launch of the LDS-staged form (a wave per column tile, all row tiles); false if the
+// kernel cannot be given its LDS
+template <int NIT, bool DIAG, bool KF>
+bool launch_split16_lds_nit(hipStream_t st, int nd, long ncol, long Lin, long sB, long sL, long sK, const uint4 *T4,
+                            float ainv, const fcomplex *x, fcomplex *y, const DiagF &dg, unsigned nbytes)
+{
+    constexpr int KB = (NIT + 1) / 2;
+    const int vg = (nd - 32 * (KB - 1) + 7) / 8;
+    const unsigned lds = (unsigned)(NIT * ((KB - 1) * 128 + 32 * vg) * 16);
+    static int ok = -1;   // per instantiation: the dynamic LDS limit raised once
+    if (ok < 0) {
+        const void *fn = reinterpret_cast<const void *>(&axis_split16_lds_kernel<NIT, DIAG, KF>);
+        ok = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess ? 1 : 0;
+        (void)hipGetLastError();
+    }
+    if (!ok) return false;
+    const unsigned nblk = (unsigned)(((ncol + 15) / 16 + 3) / 4);
+    hipLaunchKernelGGL((axis_split16_lds_kernel<NIT, DIAG, KF>), dim3(nblk), dim3(256), lds, st, nd, ncol, Lin, sB,
+                       sL, sK, T4, vg, ainv, x, y, dg, nbytes);
+    return true;
+}
+
+template <bool DIAG, bool KF>
+bool launch_split16_lds(hipStream_t st, int nd, long ncol, long Lin, long sB, long sL, long sK, const uint4 *T4,
+                        float ainv, const fcomplex *x, fcomplex *y, const DiagF &dg, unsigned nbytes)
+{
+#define TDZ_SL(NN) return launch_split16_lds_nit<NN, DIAG, KF>(st, nd, ncol, Lin, sB, sL, sK, T4, ainv, x, y, dg, nbytes)
+    switch ((nd + 15) / 16) {
+    case 1: TDZ_SL(1);
+    case 2: TDZ_SL(2);
+    case 3: TDZ_SL(3);
+    case 4: TDZ_SL(4);
+    case 5: TDZ_SL(5);
+    case 6: TDZ_SL(6);
+    case 7: TDZ_SL(7);
+    case 8: TDZ_SL(8);
+    case 9: TDZ_SL(9);
+    case 10: TDZ_SL(10);
+    default: return false;
+    }
+#undef TDZ_SL
+}
+
+// split16 selects the form: 1 a wave per (column tile, group of three row tiles), matrix
+// fragments through the caches; 2 the LDS-staged form
 template <bool DIAG>
 void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, long sL, long sK, const unsigned short *Th,
-                         float ainv, const fcomplex *x, fcomplex *y, const DiagF &dg, unsigned nbytes)
+                         float ainv, const fcomplex *x, fcomplex *y, const DiagF &dg, unsigned nbytes, int split16)
 {
+    if (split16 == 2) {
+        const uint4 *T4 = reinterpret_cast<const uint4 *>(Th);
+        const bool kf = !DIAG && sK == 1 && nd % 2 == 0;
+        if (kf ? launch_split16_lds<DIAG, true>(st, nd, ncol, Lin, sB, sL, sK, T4, ainv, x, y, dg, nbytes)
+               : launch_split16_lds<DIAG, false>(st, nd, ncol, Lin, sB, sL, sK, T4, ainv, x, y, dg, nbytes))
+            return;
+    }
     const int nit = (nd + 15) / 16, G = nit < 3 ? nit : 3, ngr = (nit + G - 1) / G;
     const unsigned nblk = (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
     const uint4 *T4 = reinterpret_cast<const uint4 *>(Th);
@@ -2276,7 +2483,7 @@ void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, l
             hipLaunchKernelGGL((axis_split16_kernel<KB, GG, false, 3, true>), dim3(nblk), dim3(256), 0, st, nd,  \
                                ncol, Lin, sB, sL, sK, nit, T4, ainv, x, y, dg, nbytes);                          \
         else                                                                                                    \
-            hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG, DIAG ? 4 : 3, false>), dim3(nblk), dim3(256), 0, \
+            hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG, 4, false>), dim3(nblk), dim3(256), 0, \
                                st, nd, ncol, Lin, sB, sL, sK, nit, T4, ainv, x, y, dg, nbytes);                  \
     } while (0)
 #define TDZ_S16_G(KB)                 \
@@ -2299,7 +2506,7 @@ void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, l
 }  // namespace
 
 bool launch_dense_axis_split16(hipStream_t st, int d, const int n[3], const unsigned short *Th, float ainv,
-                               const fcomplex *x, fcomplex *y)
+                               const fcomplex *x, fcomplex *y, int split16)
 {
     if (!Th || !split16_fits(n) || !axis_split16_blocks(n[d])) return false;
     const long n01 = (long)n[0] * n[1];
@@ -2308,18 +2515,20 @@ bool launch_dense_axis_split16(hipStream_t st, int d, const int n[3], const unsi
     if (d == 0) { L = (long)n[1] * n[2]; sB = 0; sK = 1; sL = n[0]; B = 1; }
     else if (d == 1) { L = n[0]; sB = n01; sK = n[0]; sL = 1; B = (unsigned)n[2]; }
     else { L = n01; sB = 0; sK = n01; sL = 1; B = 1; }
-    launch_split16_axis<false>(st, n[d], L * B, L, sB, sL, sK, Th, ainv, x, y, DiagF{}, (unsigned)(8 * n01 * n[2]));
+    launch_split16_axis<false>(st, n[d], L * B, L, sB, sL, sK, Th, ainv, x, y, DiagF{}, (unsigned)(8 * n01 * n[2]),
+                               split16);
     return true;
 }
 
 bool launch_dense_axis_diag_split16(hipStream_t st, const int n[3], const unsigned short *Th, float ainv,
                                     const float *lam[3], const unsigned char *wall[3], fcomplex alpha, fcomplex g,
-                                    const fcomplex *x, fcomplex *y)
+                                    const fcomplex *x, fcomplex *y, int split16)
 {
     if (!Th || !split16_fits(n) || !axis_split16_blocks(n[2])) return false;
     const long ncol = (long)n[0] * n[1];
     const DiagF dg{{lam[0], lam[1], lam[2]}, {wall[0], wall[1], wall[2]}, alpha, g, n[0]};
-    launch_split16_axis<true>(st, n[2], ncol, ncol, 0, 1, ncol, Th, ainv, x, y, dg, (unsigned)(8 * ncol * n[2]));
+    launch_split16_axis<true>(st, n[2], ncol, ncol, 0, 1, ncol, Th, ainv, x, y, dg, (unsigned)(8 * ncol * n[2]),
+                              split16);
     return true;
 }
 

@@ -924,6 +924,29 @@ __global__ void lumped_residual_kernel(long N, const fcomplex *__restrict__ pv, 
 
 inline unsigned gridn(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Grid of a z-marching feature pass: one round of blocks at the kernel's own occupancy
+// (blocks of 256 per CU, from the runtime), each z chunk at least 2P + 2 planes long.
+template <int P, bool COMB>
+void feat3_grid(int n0, int n1, int n2, int &kchunk, unsigned &nb)
+{
+    static int slots = 0;   // per instantiation: CUs x resident blocks
+    if (!slots) {
+        int dev = 0, ncu = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&feat3_kernel<P, COMB>),
+                                                         256, 0) != hipSuccess || per < 1)
+            per = 2;
+        (void)hipGetLastError();
+        slots = ncu * per;
+    }
+    const int tiles = ((n0 + 15) / 16) * ((n1 + 15) / 16);
+    const int chunks = std::max(1, std::min(slots / tiles, n2 / (2 * P + 2)));
+    kchunk = (n2 + chunks - 1) / chunks;
+    nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
+}
+
 }  // namespace
 
 void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, const fcomplex *y, fcomplex *tmp[7],
@@ -933,19 +956,10 @@ void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, c
     const long N = (long)n0 * n1 * n2;
     const int p = fb.p[0];
     if (p == fb.p[1] && p == fb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p) {
-        // z chunks: one round of blocks at two per CU, each chunk at least 2P + 2 planes
-        static int ncu = 0;
-        if (!ncu) {
-            int dev = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
-                ncu = 256;
-        }
-        const int tiles = ((n0 + 15) / 16) * ((n1 + 15) / 16);
-        const int chunks = std::max(1, std::min(2 * ncu / tiles, n2 / (2 * p + 2)));
-        const int kchunk = (n2 + chunks - 1) / chunks;
-        const unsigned nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
+        int kchunk = 0;
+        unsigned nb = 0;
 #define TDZ_FEAT3(PP)                                                                                               \
+    feat3_grid<PP, false>(n0, n1, n2, kchunk, nb);                                                                  \
     hipLaunchKernelGGL((feat3_kernel<PP, false>), dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, My, corr, \
                        u, nullptr, nullptr, fcomplex{0.f, 0.f}, nullptr)
         switch (p) {
@@ -971,19 +985,11 @@ bool fused_correction(hipStream_t st, const FusedBands &fb, const fcomplex *y, c
     const int n0 = fb.n[0], n1 = fb.n[1], n2 = fb.n[2];
     const int p = fb.p[0];
     if (!(p == fb.p[1] && p == fb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p)) return false;
-    static int ncu = 0;
-    if (!ncu) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
-            ncu = 256;
-    }
-    const int tiles = ((n0 + 15) / 16) * ((n1 + 15) / 16);
-    const int chunks = std::max(1, std::min(2 * ncu / tiles, n2 / (2 * p + 2)));
-    const int kchunk = (n2 + chunks - 1) / chunks;
-    const unsigned nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
     const CnnScales sc{{1.f, 1.f, 1.f, 1.f}};
+    int kchunk = 0;
+    unsigned nb = 0;
 #define TDZ_FC(PP)                                                                                                \
+    feat3_grid<PP, true>(n0, n1, n2, kchunk, nb);                                                                 \
     hipLaunchKernelGGL((feat3_kernel<PP, true>), dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, nullptr, \
                        nullptr, nullptr, r, Vg, g, q)
     switch (p) {

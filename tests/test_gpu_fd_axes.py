@@ -65,7 +65,7 @@ def test_split16_transforms_match_fp64(name):
         rng = np.random.default_rng(5)
         scale = 10.0 ** rng.uniform(-6, 0, ctx.N)
         r = (rng.standard_normal(ctx.N) + 1j * rng.standard_normal(ctx.N)) * scale
-        dr, z64, z32, z16 = ctx.vec(r), ctx.vec(), ctx.vec(), ctx.vec()
+        dr, z64, z32, z16, zl = ctx.vec(r), ctx.vec(), ctx.vec(), ctx.vec(), ctx.vec()
         ctx.pc_setup(T.PC_SEPARABLE)
         ctx.pc_apply(dr, z64)
         ctx.pc_setup(T.PC_SEPARABLE_MIXED)
@@ -73,11 +73,19 @@ def test_split16_transforms_match_fp64(name):
         ctx.pc_apply(dr, z32)
         ctx.set_fd_transform(1)
         ctx.pc_apply(dr, z16)
+        ctx.set_fd_transform(2)
+        ctx.pc_apply(dr, zl)
         a = z64.download()
         e32 = np.linalg.norm(z32.download() - a) / np.linalg.norm(a)
         e16 = np.linalg.norm(z16.download() - a) / np.linalg.norm(a)
-        print(name, "relative error of the complex64 solve: FP32 transforms", e32, "split FP16", e16)
+        el = np.linalg.norm(zl.download() - a) / np.linalg.norm(a)
+        # the LDS-staged form runs the same products in the same order per accumulator
+        dl = np.linalg.norm(zl.download() - z16.download()) / np.linalg.norm(a)
+        print(name, "relative error of the complex64 solve: FP32 transforms", e32, "split FP16", e16,
+              "split FP16 (matrix in LDS)", el, "difference of the two split forms", dl)
         assert e16 < 1e-5, (name, e16, e32)
+        assert el < 1e-5, (name, el, e32)
+        assert dl < 1e-7, (name, dl)
     finally:
-        ctx.set_fd_transform(1)
+        ctx.set_fd_transform(2)
         ctx.close()
