@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--nel", type=int, default=128)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--correction-form", type=int, default=1, choices=[0, 1],
+                    help="correction sweeps of the corrected/learned legs: 1 (default) the correction alone on "
+                         "single FP16 transforms where a solve needs two or more iterations, 0 the whole iterate")
     ap.add_argument("--predictor", type=int, default=0, choices=[0, 1, 2, 3, 4],
                     help="initial guess of the CN solves of the value's runs, every leg: 0 the current state "
                          "(TSTHETA's, the reference's configuration), k > 0 the binomial extrapolation of the "
@@ -119,6 +122,7 @@ class Atom:
         self.desc = problem(args.nel)
         t0 = time.perf_counter()
         self.ctx = ctx = T.Context(self.desc)
+        ctx.set_correction_form(args.correction_form)
         ctx.pc_setup(T.PC_MG)
         psi = ctx.vec()
         ctx.project_gaussian(psi, (0.0, 0.0, 0.0), (1.5, 1.5, 1.5))
@@ -130,6 +134,7 @@ class Atom:
         self.x0 = psi.download()
         self.trained = train
         self.learn = None
+        self.sweeps = {}
         if not train:
             psi.free()
             self.setup_s = time.perf_counter() - t0
@@ -166,6 +171,7 @@ class Atom:
         el = time.perf_counter() - t1
         kms, nl = ctx.timing_stats() if timing else (0.0, 0)
         ctx.timing(False)
+        self.sweeps = ctx.sweep_stats()
         if leg == "learned":
             _, fallbacks, active = ctx.learn_status()
             self.learned_status = {"fallbacks": fallbacks, "network_active": bool(active)}
@@ -246,6 +252,8 @@ def atom(args):
                         roof = (kms, nl)
                 r[leg] = {"steps_per_s": float(np.median(sps)), "runs_steps_per_s": sps,
                           "krylov_its_per_step": float(np.mean(itsm))}
+                if A.sweeps:   # the corrected legs' sweep/network choices (last run, warmup included)
+                    r[leg]["sweep_choices"] = A.sweeps
                 log(f"guess {order}, {win}: {leg} {np.median(sps):.2f} steps/s (runs {['%.1f' % v for v in sps]}), "
                     f"its/step {np.mean(itsm):.2f}")
             if "learned" in r and "fp64_mg" in r:
@@ -448,6 +456,7 @@ def fdsolve(args):
     out = {}
     for name, kind, fdk in (("complex64_split_fp16_lds", T.PC_SEPARABLE_MIXED, 2),
                             ("complex64_split_fp16", T.PC_SEPARABLE_MIXED, 1),
+                            ("complex64_single_fp16", T.PC_SEPARABLE_MIXED, 3),
                             ("complex64_fp32_mfma", T.PC_SEPARABLE_MIXED, 0), ("fp64_mfma", T.PC_SEPARABLE, 1),
                             ("fp64_valu", T.PC_SEPARABLE, 0)):
         ctx.pc_setup(kind)

@@ -169,8 +169,16 @@ int tdz_level_solve(tdz_ctx *ctx, int level, const double *b, double *x);
 /* Transforms of the complex64 separable solves (every mixed-precision preconditioner; no
  * reference counterpart): kind 0 FP32 MFMA, kind 1 (default) split FP16 MFMA (each operand
  * the sum of two FP16 halves, three FP16 products in FP32 accumulation, about 2^-22 relative
- * per product; axes up to 160 functions, longer ones keep FP32). */
+ * per product; axes up to 160 functions, longer ones keep FP32), kind 2 the same products
+ * with the matrix staged in LDS per block (a wave per column tile and every row tile),
+ * kind 3 single FP16 products (about 2^-11 relative; what the correction form below uses
+ * for its corrections). */
 int tdz_set_fd_transform(tdz_ctx *ctx, int kind);
+/* Correction sweeps of TDZ_PC_CORRECTED* and TDZ_PC_LEARNED with the split-FP16 transforms (no
+ * reference counterpart): form 1 (default) keeps y = S^-1 r and solves each sweep for its
+ * correction alone, z = y + S^-1 (-dA~ z), on single FP16 products (the correction is about
+ * 1e-2 of z, so about 1e-5 of z in rounding); form 0 solves z = S^-1 (r - dA~ z) whole. */
+int tdz_set_correction_form(tdz_ctx *ctx, int form);
 /* Transforms of the FP64 separable solves (TDZ_PC_SEPARABLE and the FP64 multigrid levels;
  * no reference counterpart): kind 1 (default) FP64 MFMA for axes up to 160 functions, kind 0
  * the VALU form (LDS-tiled), which longer axes keep. */
@@ -297,6 +305,10 @@ int tdz_learn_features(tdz_ctx *ctx, const double *y, double *u);
 int tdz_learn_eval(tdz_ctx *ctx, const double *u, const double *target, double *loss, float *dW1, float *dW2,
                    double *out_f32, double *out_mfma);
 int tdz_learn_status(tdz_ctx *ctx, int *base_its, int *fallbacks, int *active);
+/* Choices of the adaptive corrected preconditioners since tdz_pc_setup (no reference
+ * counterpart): counts[4 net + k] = solves that ran k correction sweeps (1..3), with
+ * (net = 1) or without (net = 0) the network in the last one. */
+int tdz_pc_sweep_stats(tdz_ctx *ctx, long *counts);
 
 /* Batched scan (no reference counterpart: the reference runs a scan as separate
  * jobs). nruns independent problems (any grids, degrees and pulses) advance together:

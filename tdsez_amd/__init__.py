@@ -36,6 +36,7 @@ EXPORTS = [
     "tdz_batch_fixups", "tdz_batch_destroy", "tdz_ifunction", "tdz_level_dims", "tdz_mg_transfer",
     "tdz_level_apply", "tdz_set_pc_refresh", "tdz_coarsen_knots", "tdz_refinement_matrix", "tdz_last_warning",
     "tdz_set_pade_guess", "tdz_pc_set_corrections", "tdz_cn_predictor_status", "tdz_observables_noalloc", "tdz_mg_set_smoother", "tdz_mg_set_coarse", "tdz_set_fd_transform", "tdz_set_fd64_transform", "tdz_mg_coarse_status", "tdz_level_solve", "tdz_learn_config", "tdz_learn_train", "tdz_learn_weights", "tdz_learn_fields", "tdz_learn_features", "tdz_learn_eval", "tdz_learn_status",
+    "tdz_pc_sweep_stats", "tdz_set_correction_form",
 ]
 
 
@@ -162,6 +163,8 @@ def lib():
         L.tdz_learn_features.argtypes = [vp, vp, dp]
         L.tdz_learn_eval.argtypes = [vp, dp, dp, dp, fp, fp, dp, dp]
         L.tdz_learn_status.argtypes = [vp, ip, ip, ip]
+        L.tdz_pc_sweep_stats.argtypes = [vp, C.POINTER(C.c_long)]
+        L.tdz_set_correction_form.argtypes = [vp, i]
         L.tdz_timing.argtypes = [vp, i]
         L.tdz_timing_stats.argtypes = [vp, dp, C.POINTER(C.c_long)]
         L.tdz_ifunction.argtypes = [vp, dp, vp, vp, vp]
@@ -454,6 +457,15 @@ class Context:
         b, f, a = C.c_int(0), C.c_int(0), C.c_int(0)
         _chk(lib().tdz_learn_status(self.h, C.byref(b), C.byref(f), C.byref(a)))
         return b.value, f.value, a.value
+
+    def set_correction_form(self, form):
+        _chk(lib().tdz_set_correction_form(self.h, int(form)))
+
+    def sweep_stats(self):
+        """Solves per correction-sweep choice since pc_setup: {"k1": n, ..., "net_k3": n}."""
+        c = (C.c_long * 8)()
+        _chk(lib().tdz_pc_sweep_stats(self.h, c))
+        return {("net_" if i >= 4 else "") + f"k{i % 4}": int(c[i]) for i in range(8) if i % 4 and c[i]}
 
     def trace_row(self, psi, t, E=(0.0, 0.0, 0.0), psi0=None):
         """(dipoles row, energies row, autocorrelation row) in the observables-file layout."""

@@ -139,6 +139,7 @@ struct Scratch {
     cpx<R> *tmp[4] = {};
     cpx<R> *w1 = nullptr, *w2 = nullptr;
     cpx<R> *u1 = nullptr, *u2 = nullptr;
+    cpx<R> *y0 = nullptr;   // corrected preconditioners: y = S^-1 r across the sweeps
 };
 
 }  // namespace
@@ -252,6 +253,8 @@ struct tdz_ctx {
     int refresh_cap = -1;
     int smoother = 0;           // multigrid smoother: 0 separable-solve Richardson, 1 Jacobi-Chebyshev
     int fd_transform = 1;       // complex64 separable solves: 0 FP32 MFMA, 1 split FP16 MFMA
+    int corr_form = 1;          // correction sweeps: 0 solve for the whole iterate, 1 for the correction alone
+    bool pc_cform = false;      // this adaptive solve takes the correction form (tol / r0 below 1e-6)
     bool fd64_mfma = true;      // FP64 separable solves: FP64 MFMA transforms (else VALU, LDS-tiled)
     int cheb_degree = 2;
     // coarsest multigrid level: 0 separable-solve sweeps, 1 Krylov solve to coarse_rtol
@@ -275,7 +278,10 @@ struct tdz_ctx {
     int pc_sweeps = 1;
     bool pc_net = true;        // the learned preconditioner runs its network (in the last sweep)
     double rho_hat[2][5] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};   // [network][sweeps]
-    long pc_solves = 0;       // rebuild whenever a solve takes more iterations (-tdsez_pc_refresh_its); -1: off
+    long sweep_count[2][4] = {};   // corrected solves per [network][sweeps] choice since pc_setup
+    double its_hat[2][4] = {};     // moving average of the Krylov iterations of each choice
+    double its_lt[2][4] = {};      // and of log10(tol / r0) of the solves it averages
+    long pc_solves = 0;         // adaptive solves since pc_setup (the exploration schedule)
     bool cap_on = true;         // the CAP is part of H (off during the eigensolve, as src/propagator.cpp:51-69)
     // Pade(2,2) stage guesses: per slot (2 x stage) the last increments y - x
     dcomplex *hist = nullptr;   // [4 slots][3] vectors
@@ -740,7 +746,7 @@ void setup_fd(tdz_ctx *c, Level &L)
 // plus g (lambda_x + lambda_y) times the 1/m band): the exact inverse without a
 // non-separable potential.
 template <typename R>
-void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<R> *s2)
+void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<R> *s2, int kind = -1)
 {
     Prec<R> &P = prec<R>(L);
     // the register-resident MFMA transforms' fragments (FP64: unless the VALU form is chosen)
@@ -761,8 +767,8 @@ void fd_apply(tdz_ctx *c, Level &L, const cpx<R> *r, cpx<R> *z, cpx<R> *s1, cpx<
     const unsigned char *w[3] = {L.dWall[0], L.dWall[1], L.dWall[2]};
     if constexpr (std::is_same<R, float>::value) {
         // split-FP16 transforms on every axis that takes them (the z pass with the diagonal solve)
-        if (c->fd_transform > 0 && P.Vh[0][0] && P.Vh[1][0] && P.Vh[2][0] && tdz::split16_fits(L.n)) {
-            const int k16 = c->fd_transform;
+        const int k16 = kind >= 0 ? kind : c->fd_transform;   // complex64 transform kind
+        if (k16 > 0 && P.Vh[0][0] && P.Vh[1][0] && P.Vh[2][0] && tdz::split16_fits(L.n)) {
             tdz::launch_dense_axis_split16(c->st, 0, L.n, P.Vh[0][1], P.Vh_inv[0][1], r, s1, k16);
             tdz::launch_dense_axis_split16(c->st, 1, L.n, P.Vh[1][1], P.Vh_inv[1][1], s1, s2, k16);
             tdz::launch_dense_axis_diag_split16(c->st, L.n, P.Vh[2][1], P.Vh_inv[2][1], lam, w, cx<R>(c->alpha),
@@ -1175,6 +1181,7 @@ void setup_corrected(tdz_ctx *c)
     const long N = L.N;
     c->sf.u1 = track(c, dalloc<fcomplex>(N));
     c->sf.u2 = track(c, dalloc<fcomplex>(N));
+    c->sf.y0 = track(c, dalloc<fcomplex>(N));
     set_shift_bands(c);
 }
 
@@ -1276,17 +1283,24 @@ void random_samples(tdz_ctx *c, int count, unsigned long long seed)
 // an unobserved sweep count is estimated from
 // an observed one of the same choice as rho_j = rho_k^((j+1)/(k+1)) (the error after k
 // sweeps is (S^-1 dA)^(k+1)), and an unobserved network choice from the other one (the
-// network assumed to gain half an order per iteration, so that it is tried first). Every
-// 32nd solve that picks k > 1 tries k - 1 instead, and every 32nd (offset 16) the other
-// network choice, so that estimates follow the field.
+// network assumed to gain half an order per iteration, so that it is tried first). The
+// first solve after pc_setup takes two sweeps (with the network); from then on every 32nd
+// solve (offset 4) tries the other network choice and every 32nd (offset 12) that picks
+// k > 1 tries k - 1, so that estimates follow the field. Where a choice has been observed at
+// a similar target (within a decade of tol / r0), the moving average of its iteration counts
+// (weight 0.05 per solve) replaces the prediction from rho.
 void choose_sweeps(tdz_ctx *c)
 {
     constexpr int kMaxSweeps = 3;
     const bool can_net = c->pc_kind == TDZ_PC_LEARNED && c->learn.trained;
     c->pc_sweeps = 1;
     c->pc_net = can_net;
+    c->pc_cform = false;
     if (c->pc_r0rel <= 0) return;
     const double target = c->pc_rtol / c->pc_r0rel;
+    // the correction form's single-FP16 corrections leave about 1e-4 of the residual in a
+    // strong field: right for solves of two or more iterations, not for one
+    c->pc_cform = target < 1e-6;
     auto observed_ref = [&](int net) {
         int ref = -1;
         for (int k = 1; k <= kMaxSweeps; ++k)
@@ -1306,6 +1320,15 @@ void choose_sweeps(tdz_ctx *c)
         }
         return std::pow(1e-2, (k + 1.0) * (net ? 1.5 : 1.0));
     };
+    bool seen = false;
+    for (const auto &row : c->rho_hat)
+        for (double v : row) seen = seen || v > 0;
+    if (!seen) {   // the first solve after pc_setup: two sweeps, with the network if there is one
+        ++c->pc_solves;
+        c->pc_sweeps = 2;
+        ++c->sweep_count[can_net ? 1 : 0][2];
+        return;
+    }
     int best = 1, best_net = can_net ? 1 : 0;
     double best_cost = 1e300;
     for (int net = can_net ? 1 : 0; net >= 0; --net)
@@ -1313,8 +1336,12 @@ void choose_sweeps(tdz_ctx *c)
             const double rho = std::min(estimate(net, k), 0.5);
             // a solve that met the tolerance at its contraction is counted as such (-0.05: the
             // observed mean contraction of an its-iteration solve reproduces its iterations)
-            const double its =
+            double its =
                 target >= 1.0 ? 1.0 : std::max(1.0, std::ceil(std::log(target) / std::log(rho) - 0.05));
+            // near the targets it has seen, a choice's own iteration average decides: one
+            // harder solve then moves the estimate by a twentieth of an iteration, not a whole one
+            if (c->its_hat[net][k] > 0 && std::fabs(std::log10(target) - c->its_lt[net][k]) < 1.0)
+                its = c->its_hat[net][k];
             // ms at 128^3: operator apply and Krylov work per iteration, a complex64 separable
             // solve, a fused correction pass, the network (its feature pass, two layers, combine)
             const double cost = its * (2.1 + 0.13 * (k + 1) + 0.062 * k + (net ? 0.24 : 0.0));
@@ -1325,10 +1352,11 @@ void choose_sweeps(tdz_ctx *c)
             }
         }
     ++c->pc_solves;
-    if (best > 1 && c->pc_solves % 32 == 0) --best;
-    if (can_net && c->pc_solves % 32 == 16) best_net = 1 - best_net;
+    if (best > 1 && c->pc_solves % 32 == 12) --best;
+    if (can_net && c->pc_solves % 32 == 4) best_net = 1 - best_net;
     c->pc_sweeps = best;
     c->pc_net = best_net == 1;
+    ++c->sweep_count[c->pc_net ? 1 : 0][best];
 }
 
 // the current preconditioner is a corrected solve with adaptive sweeps
@@ -1352,53 +1380,78 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = 
 {
     Level &L = *c->lv[0];
     const long N = L.N;
-    tdz::launch_demote(c->st, r, L.f.r, N);
-    fd_apply<float>(c, L, L.f.r, L.f.z, c->sf.w1, c->sf.w2);                // y = S^-1 r
-    if (correction_bound(c) > 0.25) {   // perturbation too large: the separable solve alone
-        tdz::launch_promote(c->st, L.f.z, z, N);
-        return;
-    }
-    LearnState &ls = c->learn;
-    if (c->pc_choose) {
+    if (c->pc_choose) {   // sweeps, network and form of this solve
         choose_sweeps(c);
         c->pc_choose = false;
     }
+    tdz::launch_demote(c->st, r, L.f.r, N);
+    // Correction form (corr_form 1, with the split-FP16 transforms): y = S^-1 r is kept, and
+    // each sweep solves for the correction alone, z = y + S^-1 (-dA~ z), whose right-hand side
+    // is about |S^-1 dA| ~ 1e-2 of r, so its transforms run on single FP16 products (about
+    // 1e-3 relative, 1e-5 of z) at 80% of the split form's cost. Adaptive solves take it
+    // when they need two or more Krylov iterations (tol / r0 < 1e-6, choose_sweeps).
+    const bool cform = c->corr_form == 1 && c->fd_transform > 0 && (c->corrections > 0 || c->pc_cform);
+    fcomplex *y0 = cform ? c->sf.y0 : L.f.z;
+    const int ckind = cform ? 3 : -1;
+    fd_apply<float>(c, L, L.f.r, y0, c->sf.w1, c->sf.w2);                // y = S^-1 r
+    if (correction_bound(c) > 0.25) {   // perturbation too large: the separable solve alone
+        tdz::launch_promote(c->st, y0, z, N);
+        return;
+    }
+    LearnState &ls = c->learn;
     const int sweeps = c->corrections > 0 ? c->corrections : c->pc_sweeps;
     // z <- S^-1 (r - dA~ z), repeated: the error after k corrections is (S^-1 dA)^(k+1)
     // plus, at first order, what the last sweep's dA~ misses of dA (the lumped potential:
     // ~1e-4 of r at dt = 0.02; with the network's remainder far less); an earlier sweep's
     // miss enters multiplied by S^-1 dA, so the network runs in the last sweep only
+    const fcomplex *zc = y0;                        // the current iterate
+    const fcomplex *rr = cform ? nullptr : L.f.r;   // the sweeps' right-hand side: r, or zero
+    bool promoted = false;
+    // the sweep's solve into L.f.z; in the correction form y is added (into z on the last sweep)
+    auto solve = [&](bool last) {
+        fd_apply<float>(c, L, c->sf.u1, L.f.z, c->sf.w1, c->sf.w2, ckind);
+        if (cform) {
+            if (last) {
+                tdz::launch_promote_add(c->st, y0, L.f.z, z, N);
+                promoted = true;
+            } else {
+                tdz::launch_axpby_t<float>(c->st, {1.f, 0.f}, y0, {1.f, 0.f}, L.f.z, N);
+            }
+        }
+        zc = L.f.z;
+    };
     for (int k = 0; k < sweeps; ++k) {
         const bool net = use_net && ls.trained && k == sweeps - 1 && (c->corrections > 0 || c->pc_net);
         const bool harvest = k == 0 && ls.harvesting && (int)ls.ys.size() < ls.max_samples && c->ph.pot_kind;
-        // a sweep without the network: its right-hand side r - dA~ z in one z-marching pass
+        // a sweep without the network: its right-hand side r - dA~ z (or - dA~ z) in one
+        // z-marching pass
         if (!harvest && !net && !exact_v &&
-            tdz::fused_correction(c->st, fused_bands(c), L.f.z, L.f.r, c->dVg, cx<float>(c->g), c->sf.u1)) {
-            fd_apply<float>(c, L, c->sf.u1, L.f.z, c->sf.w1, c->sf.w2);
+            tdz::fused_correction(c->st, fused_bands(c), zc, rr, c->dVg, cx<float>(c->g), c->sf.u1)) {
+            solve(k == sweeps - 1);
             continue;
         }
         if (harvest || net || ls.ready) {
             // M z, g (CAP + sE.D) z and, for the network, its input, in one z-marching pass
             // (equal orders in 3D; three passes otherwise)
             fcomplex *tmp[7] = {c->sf.tmp[0], c->sf.tmp[1], c->sf.tmp[2], c->sf.tmp[3], c->sf.w1, c->sf.w2, c->sf.u2};
-            tdz::fused_products(c->st, fused_bands(c), ls.sin, L.f.z, tmp, L.f.t, c->sf.u1,
+            tdz::fused_products(c->st, fused_bands(c), ls.sin, zc, tmp, L.f.t, c->sf.u1,
                                 harvest || net ? ls.feat : nullptr);
         } else {
             tdz::SepBandsT<float> mb = sep_bands<float>(L);
             for (int d = 0; d < 3; ++d) mb.B[d] = L.f.Bmass[d];
-            tdz::launch_sep_apply<float>(c->st, mb, L.f.z, L.f.t, c->sf.tmp);       // M z
+            tdz::launch_sep_apply<float>(c->st, mb, zc, L.f.t, c->sf.tmp);       // M z
             tdz::SepBandsT<float> cb = sep_bands<float>(L);
             for (int d = 0; d < 3; ++d) cb.B[d] = c->dBcorr[d];
-            tdz::launch_sep_apply<float>(c->st, cb, L.f.z, c->sf.u1, c->sf.tmp);    // g (CAP + sE.D) z
+            tdz::launch_sep_apply<float>(c->st, cb, zc, c->sf.u1, c->sf.tmp);    // g (CAP + sE.D) z
         }
         const fcomplex *n = nullptr;
         if (exact_v && c->ph.pot_kind) {   // potential product by the matrix-free kernel (reference for the learned one)
             HIPCHK(hipMemsetAsync(c->sf.u2, 0, N * sizeof(fcomplex), c->st));
-            pot_apply<float>(c, L, c->g, L.f.z, c->sf.u2);
+            pot_apply<float>(c, L, c->g, zc, c->sf.u2);
             HIPCHK(hipMemsetAsync(L.f.t, 0, N * sizeof(fcomplex), c->st));
             n = c->sf.u2;
         }
-        if (harvest) harvest_sample(c, L.f.z);
+        if (harvest) harvest_sample(c, zc);
         const float *no = nullptr;   // the network's output, scaled by g / sout in the combine
         if (net) {
             const int n3[3] = {L.n[0], L.n[1], L.n[2]};
@@ -1408,11 +1461,11 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = 
             ls.packed_ver = ls.wver;
             no = ls.out;
         }
-        tdz::launch_corr_combine(c->st, L.f.r, L.f.t, c->sf.u1, n, c->dVg, cx<float>(c->g), c->sf.u1, N, no,
+        tdz::launch_corr_combine(c->st, rr, L.f.t, c->sf.u1, n, c->dVg, cx<float>(c->g), c->sf.u1, N, no,
                                  cx<float>(c->g / ls.sout));
-        fd_apply<float>(c, L, c->sf.u1, L.f.z, c->sf.w1, c->sf.w2);
+        solve(k == sweeps - 1);
     }
-    tdz::launch_promote(c->st, L.f.z, z, N);
+    if (!promoted) tdz::launch_promote(c->st, zc, z, N);
 }
 
 void pc_apply_dev(tdz_ctx *c, const dcomplex *r, dcomplex *z)
@@ -1681,8 +1734,21 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
         if (with_ax) tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N, b);   // ax_out = b + sum
         if (converged || its >= maxit) break;
     }
-    if (chose && converged && its > 0 && c->pc_r0rel > 0 && res > 0)   // mean contraction per iteration
+    if (chose && converged && its > 0 && c->pc_r0rel > 0 && res > 0) {   // mean contraction per iteration
         observe_sweeps(c, c->pc_sweeps, c->pc_net, std::pow(std::min(res / c->pc_r0rel, 1.0), 1.0 / its));
+        const int k = c->pc_sweeps, n = c->pc_net ? 1 : 0;
+        if (k >= 1 && k <= 3) {
+            const double lt = std::log10(c->pc_rtol / c->pc_r0rel);
+            double &ih = c->its_hat[n][k], &il = c->its_lt[n][k];
+            if (ih > 0 && std::fabs(lt - il) < 1.0) {
+                ih = 0.95 * ih + 0.05 * its;
+                il = 0.95 * il + 0.05 * lt;
+            } else {
+                ih = its;
+                il = lt;
+            }
+        }
+    }
     if (st) *st = {its, restarts, res, bnorm, c->rebuilds};
     if (!converged) {
         if (allow_fail) return -1;
@@ -1720,6 +1786,9 @@ void pc_setup(tdz_ctx *c, int kind)
         throw Error("a layered structure is separable: its separable solve is exact (no correction to learn)");
     c->pc_kind = kind;
     for (auto &row : c->rho_hat) std::fill(std::begin(row), std::end(row), -1.0);   // new contraction estimates
+    for (auto &row : c->sweep_count) std::fill(std::begin(row), std::end(row), 0L);
+    c->pc_solves = 0;
+    for (auto &row : c->its_hat) std::fill(std::begin(row), std::end(row), 0.0);
     c->pc_sweeps = 1;
     c->pc_net = true;
     if (mg || (kind == TDZ_PC_LEARNED && c->lv.size() >= 2)) ensure_galerkin(c);
@@ -2263,9 +2332,18 @@ int tdz_set_fd64_transform(tdz_ctx *c, int kind)
 int tdz_set_fd_transform(tdz_ctx *c, int kind)
 {
     return guarded([&] {
-        if (kind < 0 || kind > 2)
-            throw Error("transform kind must be 0 (FP32 MFMA), 1 (split FP16 MFMA) or 2 (split FP16 MFMA, matrix in LDS)");
+        if (kind < 0 || kind > 3)
+            throw Error("transform kind must be 0 (FP32 MFMA), 1 (split FP16 MFMA), 2 (split FP16 MFMA, matrix in "
+                        "LDS) or 3 (single FP16 products, for corrections)");
         c->fd_transform = kind;
+    });
+}
+
+int tdz_set_correction_form(tdz_ctx *c, int form)
+{
+    return guarded([&] {
+        if (form != 0 && form != 1) throw Error("correction form must be 0 (whole iterate) or 1 (correction alone)");
+        c->corr_form = form;
     });
 }
 
@@ -3540,6 +3618,14 @@ int tdz_learn_status(tdz_ctx *c, int *base_its, int *fallbacks, int *active)
     if (base_its) *base_its = c->learn.base_its;
     if (fallbacks) *fallbacks = c->learn.fallbacks;
     if (active) *active = c->pc_kind == TDZ_PC_LEARNED && c->learn.trained && !c->learn.fallen;
+    return 0;
+}
+
+int tdz_pc_sweep_stats(tdz_ctx *c, long *counts)
+{
+    if (!c || !counts) { g_err = "null argument"; return 1; }
+    for (int net = 0; net < 2; ++net)
+        for (int k = 0; k < 4; ++k) counts[4 * net + k] = c->sweep_count[net][k];
     return 0;
 }
 

@@ -195,6 +195,7 @@ void launch_axpby_t(hipStream_t st, cpx<R> a, const cpx<R> *x, cpx<R> b, cpx<R> 
 // precision conversions
 void launch_demote(hipStream_t st, const dcomplex *x, fcomplex *y, long N);
 void launch_promote(hipStream_t st, const fcomplex *x, dcomplex *y, long N);
+void launch_promote_add(hipStream_t st, const fcomplex *x, const fcomplex *y, dcomplex *z, long N);   // z = x + y
 // q = r - g Vg .* My - c - n (n may be null), complex64; with no (the network's [2N]
 // output as two real halves) also - ng (no_re + i no_im)
 void launch_corr_combine(hipStream_t st, const fcomplex *r, const fcomplex *My, const fcomplex *c,
@@ -366,8 +367,8 @@ struct FusedBands {
 void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, const fcomplex *y, fcomplex *tmp[7],
                     fcomplex *My, fcomplex *corr, fcomplex *u);
 // A correction sweep without the network in one z-marching pass: q = r - g Vg (M y) - corr(y)
-// (corr: the CAP and field bands); false where the fused pass does not apply (unequal
-// orders, not 3D), the caller then takes fused_products and the combine.
+// (corr: the CAP and field bands; r null: zero); false where the fused pass does not apply
+// (unequal orders, not 3D), the caller then takes fused_products and the combine.
 bool fused_correction(hipStream_t st, const FusedBands &fb, const fcomplex *y, const fcomplex *r, const float *Vg,
                       fcomplex g, fcomplex *q);
 

@@ -1066,7 +1066,7 @@ __global__ void corr_combine_kernel(const fcomplex *__restrict__ r, const fcompl
 {
     const long stride = (long)gridDim.x * blockDim.x;
     for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
-        const fcomplex m = My[I], cv = c[I], rv = r[I];
+        const fcomplex m = My[I], cv = c[I], rv = r ? r[I] : fcomplex{0.f, 0.f};
         const float v = Vg[I];
         float qr = rv.re - v * (g.re * m.re - g.im * m.im) - cv.re;
         float qi = rv.im - v * (g.re * m.im + g.im * m.re) - cv.im;
@@ -1459,7 +1459,9 @@ struct fcomplex2 {
 // buffer's own range check is a second guard.
 // KF: the axis is the contiguous one (sK = 1, even line length): a lane's 8 consecutive k of a
 // k block and its 4 consecutive output rows are 16-byte loads and stores (pairs of values).
-template <int KB, int G, bool DIAG, int OCC, bool KF>
+// SPLIT false: one FP16 product per MFMA (hi halves only, about 2^-11 relative), for solves
+// whose result enters as a small correction.
+template <int KB, int G, bool DIAG, int OCC, bool KF, bool SPLIT = true>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) axis_split16_kernel(int n, long ncol, long Lin, long sB, long sL, long sK,
                                                            int nit, const uint4 *__restrict__ Th, float ainv,
                                                            const fcomplex *__restrict__ in,
@@ -1522,9 +1524,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
             const float xr = bv[kb][j].re * sc, xi = bv[kb][j].im * sc;
             const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
             rh[kb][j] = hr;
-            rl[kb][j] = (_Float16)(xr - (float)hr);
             ih[kb][j] = hi;
-            il[kb][j] = (_Float16)(xi - (float)hi);
+            if (SPLIT) {
+                rl[kb][j] = (_Float16)(xr - (float)hr);
+                il[kb][j] = (_Float16)(xi - (float)hi);
+            }
         }
     f32x4 ar[G], ai[G];
 #pragma unroll
@@ -1535,12 +1539,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
         for (int g = 0; g < G; ++g) {
             const int it = min(ig * G + g, nit - 1);   // past the last row tile: a copy, not stored
             const half8_t ah = __builtin_bit_cast(half8_t, Th[((long)(it * KB + kb) * 2) * 64 + lane]);
-            const half8_t al = __builtin_bit_cast(half8_t, Th[((long)(it * KB + kb) * 2 + 1) * 64 + lane]);
-            ar[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, rh[kb], ar[g], 0, 0, 0);
-            ar[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rl[kb], ar[g], 0, 0, 0);
+            if (SPLIT) {
+                const half8_t al = __builtin_bit_cast(half8_t, Th[((long)(it * KB + kb) * 2 + 1) * 64 + lane]);
+                ar[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, rh[kb], ar[g], 0, 0, 0);
+                ar[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rl[kb], ar[g], 0, 0, 0);
+                ai[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, ih[kb], ai[g], 0, 0, 0);
+                ai[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, il[kb], ai[g], 0, 0, 0);
+            }
             ar[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rh[kb], ar[g], 0, 0, 0);
-            ai[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, ih[kb], ai[g], 0, 0, 0);
-            ai[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, il[kb], ai[g], 0, 0, 0);
             ai[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, ih[kb], ai[g], 0, 0, 0);
         }
     if (!cv) return;
@@ -2317,6 +2323,22 @@ void launch_demote(hipStream_t st, const dcomplex *x, fcomplex *y, long N)
     hipLaunchKernelGGL(demote_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, x, y, N);
 }
 
+// z (complex128) = x + y (complex64 sum, then widened)
+__global__ void promote_add_kernel(const fcomplex *__restrict__ x, const fcomplex *__restrict__ y,
+                                   dcomplex *__restrict__ z, long N)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
+        const fcomplex a = x[I], b = y[I];
+        z[I] = {(double)(a.re + b.re), (double)(a.im + b.im)};
+    }
+}
+
+void launch_promote_add(hipStream_t st, const fcomplex *x, const fcomplex *y, dcomplex *z, long N)
+{
+    hipLaunchKernelGGL(promote_add_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, x, y, z, N);
+}
+
 void launch_promote(hipStream_t st, const fcomplex *x, dcomplex *y, long N)
 {
     hipLaunchKernelGGL(promote_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, x, y, N);
@@ -2467,6 +2489,37 @@ template <bool DIAG>
 void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, long sL, long sK, const unsigned short *Th,
                          float ainv, const fcomplex *x, fcomplex *y, const DiagF &dg, unsigned nbytes, int split16)
 {
+    if (split16 == 3) {   // single FP16 products
+        const int nit = (nd + 15) / 16, G = nit < 3 ? nit : 3, ngr = (nit + G - 1) / G;
+        const unsigned nblk = (unsigned)(((ncol + 15) / 16 * ngr + 3) / 4);
+        const uint4 *T4 = reinterpret_cast<const uint4 *>(Th);
+        const bool kf = !DIAG && sK == 1 && nd % 2 == 0;
+#define TDZ_S16L(KB, GG)                                                                                            \
+    do {                                                                                                        \
+        if (kf)                                                                                                 \
+            hipLaunchKernelGGL((axis_split16_kernel<KB, GG, false, 4, true, false>), dim3(nblk), dim3(256), 0, st, \
+                               nd, ncol, Lin, sB, sL, sK, nit, T4, ainv, x, y, dg, nbytes);                     \
+        else                                                                                                    \
+            hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG, 4, false, false>), dim3(nblk), dim3(256), 0, st, \
+                               nd, ncol, Lin, sB, sL, sK, nit, T4, ainv, x, y, dg, nbytes);                     \
+    } while (0)
+#define TDZ_S16L_G(KB)                 \
+    do {                               \
+        if (G == 1) TDZ_S16L(KB, 1);   \
+        else if (G == 2) TDZ_S16L(KB, 2); \
+        else TDZ_S16L(KB, 3);          \
+    } while (0)
+        switch (axis_split16_blocks(nd)) {
+        case 1: TDZ_S16L_G(1); break;
+        case 2: TDZ_S16L_G(2); break;
+        case 3: TDZ_S16L_G(3); break;
+        case 4: TDZ_S16L_G(4); break;
+        default: TDZ_S16L_G(5); break;
+        }
+#undef TDZ_S16L_G
+#undef TDZ_S16L
+        return;
+    }
     if (split16 == 2) {
         const uint4 *T4 = reinterpret_cast<const uint4 *>(Th);
         const bool kf = !DIAG && sK == 1 && nd % 2 == 0;

@@ -880,7 +880,7 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
                 const long I = ko * plane + (long)j * n0 + i;
                 const fcomplex m0 = aM[r];
                 if (COMB) {
-                    const fcomplex rv = rhs[I], cv = aC[r];
+                    const fcomplex rv = rhs ? rhs[I] : fcomplex{0.f, 0.f}, cv = aC[r];
                     const float v = Vg[I];
                     q[I] = {rv.re - v * (gc.re * m0.re - gc.im * m0.im) - cv.re,
                             rv.im - v * (gc.re * m0.im + gc.im * m0.re) - cv.im};

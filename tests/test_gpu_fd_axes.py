@@ -87,5 +87,5 @@ def test_split16_transforms_match_fp64(name):
         assert el < 1e-5, (name, el, e32)
         assert dl < 1e-7, (name, dl)
     finally:
-        ctx.set_fd_transform(2)
+        ctx.set_fd_transform(1)
         ctx.close()

@@ -185,3 +185,51 @@ def test_learned_preconditioner_gain_and_fallback():
     assert fallbacks >= 1 and active == 0
     assert _mnorm(ctx, d.download() - xc) / _mnorm(ctx, xc) < 1e-10
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_correction_forms_agree():
+    """The corrected preconditioner in its two sweep forms: form 0 solves each sweep's whole
+    iterate z = S^-1 (r - dA~ z) on split-FP16 transforms, form 1 keeps y = S^-1 r and
+    solves for the correction alone on single FP16 products. The correction is about 1e-2
+    of z, so the two applies contract a CN residual alike and agree in the M-norm (their
+    coefficient vectors differ in directions the operator hardly sees, as every solve's
+    coefficients do to cond(M) times its accuracy); CN steps with either form reach the
+    same wavefunction (the FP64 residual judges)."""
+    import tdsez_amd as T
+    ctx = T.Context(MID)
+    psi = ctx.vec()
+    ctx.project_gaussian(psi, (0.3, -0.2, 0.1), (1.0, 1.2, 0.9), (0.2, 0.0, -0.3))
+    g, sE = 0.01j, (0.0, 0.0, 0.04)
+    hphi = ctx.vec()
+    ctx.set_shift(0.0, 1.0, sE)
+    ctx.apply(psi, hphi)
+    r = -2.0 * g * hphi.download()
+    ctx.set_shift(1.0, g, sE)
+    ctx.pc_setup(T.PC_CORRECTED)
+    out, contraction = {}, {}
+    for k in (1, 2):
+        ctx.set_corrections(k)
+        for form in (0, 1):
+            ctx.set_correction_form(form)
+            dr, dz, dw = ctx.vec(r), ctx.vec(), ctx.vec()
+            ctx.pc_apply(dr, dz)
+            ctx.apply(dz, dw)
+            out[k, form] = dz.download()
+            contraction[k, form] = np.linalg.norm(r - dw.download()) / np.linalg.norm(r)
+        diff = _mnorm(ctx, out[k, 1] - out[k, 0]) / _mnorm(ctx, out[k, 0])
+        print("sweeps", k, "contraction form 0 / 1", contraction[k, 0], contraction[k, 1], "M-norm difference", diff)
+        assert diff < 1e-3, (k, diff)
+        assert contraction[k, 1] < 1.2 * contraction[k, 0] + 1e-6, (k, contraction)
+    pulse = T.make_pulse(E0=(0.0, 0.0, 0.05), omega=0.057, ncycles=4.0)
+    x0 = psi.download()
+    finals = {}
+    for form in (0, 1):
+        ctx.set_correction_form(form)
+        ctx.pc_setup(T.PC_CORRECTED)
+        d = ctx.vec(x0)
+        its, _ = ctx.evolve_cn(d, 0.0, 0.02, 4, pulse, rtol=1e-12)
+        finals[form] = d.download()
+        print("form", form, "iterations", its)
+    assert _mnorm(ctx, finals[1] - finals[0]) / _mnorm(ctx, finals[0]) < 1e-10
+    ctx.close()
