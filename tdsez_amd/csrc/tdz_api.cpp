@@ -1426,7 +1426,8 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = 
         // a sweep without the network: its right-hand side r - dA~ z (or - dA~ z) in one
         // z-marching pass
         if (!harvest && !net && !exact_v &&
-            tdz::fused_correction(c->st, fused_bands(c), zc, rr, c->dVg, cx<float>(c->g), c->sf.u1)) {
+            tdz::fused_correction(c->st, fused_bands(c), zc, rr, c->dVg, cx<float>(c->g), c->sf.u1, c->sf.tmp[0],
+                                  c->sf.tmp[1])) {
             solve(k == sweeps - 1);
             continue;
         }

@@ -366,10 +366,12 @@ struct FusedBands {
 };
 void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, const fcomplex *y, fcomplex *tmp[7],
                     fcomplex *My, fcomplex *corr, fcomplex *u);
-// A correction sweep without the network in one z-marching pass: q = r - g Vg (M y) - corr(y)
-// (corr: the CAP and field bands; r null: zero); false where the fused pass does not apply
-// (unequal orders, not 3D), the caller then takes fused_products and the combine.
+// The right-hand side of a correction sweep without the network, q = r - g Vg (M y) - corr(y)
+// (corr: the CAP and field bands; r null: zero), in two passes: the x and y band passes of
+// every plane into s1, s2 (complex64 scratch vectors), then the z band passes with the
+// combine. False where it does not apply (unequal orders, not 3D): the caller then takes
+// fused_products and the combine.
 bool fused_correction(hipStream_t st, const FusedBands &fb, const fcomplex *y, const fcomplex *r, const float *Vg,
-                      fcomplex g, fcomplex *q);
+                      fcomplex g, fcomplex *q, fcomplex *s1, fcomplex *s2);
 
 }  // namespace tdz

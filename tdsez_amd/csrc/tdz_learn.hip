@@ -749,15 +749,11 @@ __global__ void feat_z_kernel(int n0, int n1, int n2, int p, const float *__rest
 //   corr = Mz My Cx y + Mz Cy Mx y + Cz My Mx y.
 // A finished plane writes My, corr and (if u) the scaled inputs
 // u = [y, xx - x My, xy - y My, xz - z My]: y is read once instead of the passes' 22
-// vector transfers. COMB (a correction sweep without the network): only My and corr are
-// formed, and the plane writes the next solve's right-hand side q = r - g Vg My - corr
-// itself (the combine of corr_combine_kernel without its network and exact-product terms).
-template <int P, bool COMB>
+// vector transfers.
+template <int P>
 __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int kchunk, FusedBands fb, CnnScales sc,
                                                     const fcomplex *__restrict__ y, fcomplex *__restrict__ My,
-                                                    fcomplex *__restrict__ corr, fcomplex *__restrict__ u,
-                                                    const fcomplex *__restrict__ rhs, const float *__restrict__ Vg,
-                                                    fcomplex gc, fcomplex *__restrict__ q)
+                                                    fcomplex *__restrict__ corr, fcomplex *__restrict__ u)
 {
     constexpr int R = 2 * P + 1, T = 16, H = T + 2 * P;
     __shared__ fcomplex Ys[H * H];
@@ -775,19 +771,19 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
     float mx[R], xx[R], my[R], xy[R];
     fcomplex cx[R], cy[R];
 #pragma unroll
-    for (int l = 0; l < R; ++l) {   // COMB: no position bands (they need not exist)
+    for (int l = 0; l < R; ++l) {
         mx[l] = fb.M[0][ic * R + l];
-        xx[l] = COMB ? 0.f : fb.X[0][ic * R + l];
+        xx[l] = fb.X[0][ic * R + l];
         cx[l] = fb.C[0][ic * R + l];
         my[l] = fb.M[1][jc * R + l];
-        xy[l] = COMB ? 0.f : fb.X[1][jc * R + l];
+        xy[l] = fb.X[1][jc * R + l];
         cy[l] = fb.C[1][jc * R + l];
     }
     fcomplex aM[R], aXX[R], aXY[R], aXZ[R], aC[R];
     fcomplex yr[R];   // y at this thread's (i, j) of the last R staged planes (plane K in slot K mod R)
 #pragma unroll
     for (int l = 0; l < R; ++l) aM[l] = aXX[l] = aXY[l] = aXZ[l] = aC[l] = yr[l] = {0.f, 0.f};
-    const float g0 = COMB ? 0.f : fb.g[0][ic], g1 = COMB ? 0.f : fb.g[1][jc];
+    const float g0 = fb.g[0][ic], g1 = fb.g[1][jc];
     const long plane = (long)n0 * n1;
     const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
     constexpr int NX = (H * H + 255) / 256;
@@ -814,7 +810,7 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
                 }
                 __syncthreads();
                 if (K + 1 < Kend) load_plane(K + 1);
-                if (!COMB && u) yr[r] = Ys[(tj + P) * H + ti + P];
+                if (u) yr[r] = Ys[(tj + P) * H + ti + P];
                 for (int e = tid; e < H * T; e += 256) {   // e % T == ti
                     const int jj = e / T;
                     float tr = 0, tim = 0, qr = 0, qi = 0, rr = 0, ri = 0;
@@ -823,15 +819,13 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
                         const fcomplex v = Ys[jj * H + ti + l];
                         tr += mx[l] * v.re;
                         tim += mx[l] * v.im;
-                        if (!COMB) {
-                            qr += xx[l] * v.re;
-                            qi += xx[l] * v.im;
-                        }
+                        qr += xx[l] * v.re;
+                        qi += xx[l] * v.im;
                         rr += cx[l].re * v.re - cx[l].im * v.im;
                         ri += cx[l].re * v.im + cx[l].im * v.re;
                     }
                     Ts[e] = {tr, tim};
-                    if (!COMB) Qs[e] = {qr, qi};
+                    Qs[e] = {qr, qi};
                     Rs[e] = {rr, ri};
                 }
                 __syncthreads();
@@ -842,13 +836,11 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
                     const fcomplex tv = Ts[o], rv = Rs[o];
                     pp0 += my[l] * tv.re;
                     pp1 += my[l] * tv.im;
-                    if (!COMB) {
-                        const fcomplex qv = Qs[o];
-                        px0 += xy[l] * tv.re;
-                        px1 += xy[l] * tv.im;
-                        qq0 += my[l] * qv.re;
-                        qq1 += my[l] * qv.im;
-                    }
+                    const fcomplex qv = Qs[o];
+                    px0 += xy[l] * tv.re;
+                    px1 += xy[l] * tv.im;
+                    qq0 += my[l] * qv.re;
+                    qq1 += my[l] * qv.im;
                     s0 += my[l] * rv.re + cy[l].re * tv.re - cy[l].im * tv.im;
                     s1 += my[l] * rv.im + cy[l].re * tv.im + cy[l].im * tv.re;
                 }
@@ -857,19 +849,17 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
                     const int k = K + d;
                     if (k >= k0 && k < k1) {
                         const int e = k * R + P - d;
-                        const float mz = fb.M[2][e], xz = COMB ? 0.f : fb.X[2][e];
+                        const float mz = fb.M[2][e], xz = fb.X[2][e];
                         const fcomplex cz = fb.C[2][e];
                         const int sl = (r + d + P) % R;
                         aM[sl].re += mz * pp0;
                         aM[sl].im += mz * pp1;
-                        if (!COMB) {
-                            aXX[sl].re += mz * qq0;
-                            aXX[sl].im += mz * qq1;
-                            aXY[sl].re += mz * px0;
-                            aXY[sl].im += mz * px1;
-                            aXZ[sl].re += xz * pp0;
-                            aXZ[sl].im += xz * pp1;
-                        }
+                        aXX[sl].re += mz * qq0;
+                        aXX[sl].im += mz * qq1;
+                        aXY[sl].re += mz * px0;
+                        aXY[sl].im += mz * px1;
+                        aXZ[sl].re += xz * pp0;
+                        aXZ[sl].im += xz * pp1;
                         aC[sl].re += mz * s0 + cz.re * pp0 - cz.im * pp1;
                         aC[sl].im += mz * s1 + cz.re * pp1 + cz.im * pp0;
                     }
@@ -879,16 +869,9 @@ __global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int 
             if (ko >= k0 && ko < k1 && own) {
                 const long I = ko * plane + (long)j * n0 + i;
                 const fcomplex m0 = aM[r];
-                if (COMB) {
-                    const fcomplex rv = rhs ? rhs[I] : fcomplex{0.f, 0.f}, cv = aC[r];
-                    const float v = Vg[I];
-                    q[I] = {rv.re - v * (gc.re * m0.re - gc.im * m0.im) - cv.re,
-                            rv.im - v * (gc.re * m0.im + gc.im * m0.re) - cv.im};
-                } else {
-                    My[I] = m0;
-                    corr[I] = aC[r];
-                }
-                if (!COMB && u) {
+                My[I] = m0;
+                corr[I] = aC[r];
+                if (u) {
                     const long N = plane * n2;
                     const float g2 = fb.g[2][ko];
                     const fcomplex yc = yr[(r + P + 1) % R];   // plane ko = K - P
@@ -922,11 +905,153 @@ __global__ void lumped_residual_kernel(long N, const fcomplex *__restrict__ pv, 
     t[v] = {pv[v].re - w * My[v].re, pv[v].im - w * My[v].im};
 }
 
+// A correction sweep's right-hand side in two passes (3D, equal orders P):
+//   corr_xy_kernel: per z plane and 16 x 16 tile of (x, y) columns, U1 = (My (x) Mx) y and
+//     U2 = (My (x) Cx + Cy (x) Mx) y (the plane with its halo staged in LDS, the x band
+//     passes on the 16 + 2P rows the y passes need, then the y passes); a block takes PZ
+//     consecutive planes (the next plane's loads in flight during this one's passes);
+//   corr_z_kernel: per (x, y) column and chunk of z, the z band passes over a ring of the
+//     last 2P + 1 input planes, q = r - g Vg (Mz U1) - (Mz U2 + Cz U1) (r null: zero).
+// Every plane's x and y passes run once (the one-pass z-marching form repeats them on 2P
+// halo planes per z chunk).
+template <int P, int PZ>
+__global__ void __launch_bounds__(256) corr_xy_kernel(int n0, int n1, int n2, FusedBands fb,
+                                                      const fcomplex *__restrict__ y, fcomplex *__restrict__ U1,
+                                                      fcomplex *__restrict__ U2)
+{
+    constexpr int R = 2 * P + 1, T = 16, H = T + 2 * P, NX = (H * H + 255) / 256;
+    __shared__ fcomplex Ys[H * H];
+    __shared__ fcomplex Ts[H * T], Rs[H * T];
+    const int tid = threadIdx.x, ti = tid % T, tj = tid / T;
+    const int tilesx = (n0 + T - 1) / T, tilesy = (n1 + T - 1) / T;
+    int b = blockIdx.x;
+    const int bx = b % tilesx;
+    b /= tilesx;
+    const int by = b % tilesy, bz = b / tilesy;
+    const int i0 = bx * T, j0 = by * T, k0 = bz * PZ, k1 = min(k0 + PZ, n2);
+    const int i = i0 + ti, j = j0 + tj;
+    const bool own = i < n0 && j < n1;
+    const int ic = min(i, n0 - 1), jc = min(j, n1 - 1);
+    float mx[R], my[R];
+    fcomplex cx[R], cy[R];
+#pragma unroll
+    for (int l = 0; l < R; ++l) {
+        mx[l] = fb.M[0][ic * R + l];
+        cx[l] = fb.C[0][ic * R + l];
+        my[l] = fb.M[1][jc * R + l];
+        cy[l] = fb.C[1][jc * R + l];
+    }
+    const long plane = (long)n0 * n1;
+    fcomplex yv[NX];
+    auto load_plane = [&](int K) {
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + 256 * q;
+            const int ii = i0 - P + e % H, jj = j0 - P + e / H;
+            yv[q] = {0.f, 0.f};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) yv[q] = y[K * plane + (long)jj * n0 + ii];
+        }
+    };
+    load_plane(k0);
+    for (int K = k0; K < k1; ++K) {
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + 256 * q;
+            if (e < H * H) Ys[e] = yv[q];
+        }
+        __syncthreads();
+        if (K + 1 < k1) load_plane(K + 1);
+        for (int e = tid; e < H * T; e += 256) {   // e % T == ti
+            const int jj = e / T;
+            float tr = 0, tim = 0, rr = 0, ri = 0;
+#pragma unroll
+            for (int l = 0; l < R; ++l) {
+                const fcomplex v = Ys[jj * H + ti + l];
+                tr += mx[l] * v.re;
+                tim += mx[l] * v.im;
+                rr += cx[l].re * v.re - cx[l].im * v.im;
+                ri += cx[l].re * v.im + cx[l].im * v.re;
+            }
+            Ts[e] = {tr, tim};
+            Rs[e] = {rr, ri};
+        }
+        __syncthreads();
+        float p0 = 0, p1 = 0, s0 = 0, s1 = 0;
+#pragma unroll
+        for (int l = 0; l < R; ++l) {
+            const int o = (tj + l) * T + ti;
+            const fcomplex tv = Ts[o], rv = Rs[o];
+            p0 += my[l] * tv.re;
+            p1 += my[l] * tv.im;
+            s0 += my[l] * rv.re + cy[l].re * tv.re - cy[l].im * tv.im;
+            s1 += my[l] * rv.im + cy[l].re * tv.im + cy[l].im * tv.re;
+        }
+        if (own) {
+            const long I = K * plane + (long)j * n0 + i;
+            U1[I] = {p0, p1};
+            U2[I] = {s0, s1};
+        }
+    }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) corr_z_kernel(int n0, int n1, int n2, int kchunk, FusedBands fb,
+                                                     const fcomplex *__restrict__ U1, const fcomplex *__restrict__ U2,
+                                                     const fcomplex *__restrict__ rhs, const float *__restrict__ Vg,
+                                                     fcomplex gc, fcomplex *__restrict__ q)
+{
+    constexpr int R = 2 * P + 1;
+    const long plane = (long)n0 * n1;
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= plane) return;   // no barriers below
+    const int k0 = blockIdx.y * kchunk, k1 = min(k0 + kchunk, n2);
+    if (k0 >= k1) return;
+    // ring of the input planes K - 2P .. K (slot K mod R); planes outside the axis are zero
+    fcomplex u1[R], u2[R];
+#pragma unroll
+    for (int l = 0; l < R; ++l) u1[l] = u2[l] = {0.f, 0.f};
+    const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
+    for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int K = Kb + r;
+            if (K < Kbeg) continue;
+            if (K < Kend) {
+                u1[r] = U1[K * plane + col];
+                u2[r] = U2[K * plane + col];
+            } else {
+                u1[r] = u2[r] = {0.f, 0.f};
+            }
+            const int ko = K - P;   // its inputs ko - P .. ko + P are in the ring
+            if (ko < k0 || ko >= k1) continue;
+            float m0 = 0, m1 = 0, c0 = 0, c1 = 0;
+#pragma unroll
+            for (int d = -P; d <= P; ++d) {
+                const int kk = ko + d;   // band entry (ko, kk) = ko * R + P + d
+                if (kk < 0 || kk >= n2) continue;
+                const int e = ko * R + P + d;
+                const float mz = fb.M[2][e];
+                const fcomplex cz = fb.C[2][e];
+                const int sl = (r + R + d - P) % R;   // slot of plane kk = K + d - P
+                const fcomplex a = u1[sl], c = u2[sl];
+                m0 += mz * a.re;
+                m1 += mz * a.im;
+                c0 += mz * c.re + cz.re * a.re - cz.im * a.im;
+                c1 += mz * c.im + cz.re * a.im + cz.im * a.re;
+            }
+            const long I = ko * plane + col;
+            const fcomplex rv = rhs ? rhs[I] : fcomplex{0.f, 0.f};
+            const float v = Vg[I];
+            q[I] = {rv.re - v * (gc.re * m0 - gc.im * m1) - c0, rv.im - v * (gc.re * m1 + gc.im * m0) - c1};
+        }
+    }
+}
+
 inline unsigned gridn(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Grid of a z-marching feature pass: one round of blocks at the kernel's own occupancy
 // (blocks of 256 per CU, from the runtime), each z chunk at least 2P + 2 planes long.
-template <int P, bool COMB>
+template <int P>
 void feat3_grid(int n0, int n1, int n2, int &kchunk, unsigned &nb)
 {
     static int slots = 0;   // per instantiation: CUs x resident blocks
@@ -935,7 +1060,7 @@ void feat3_grid(int n0, int n1, int n2, int &kchunk, unsigned &nb)
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
             ncu = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&feat3_kernel<P, COMB>),
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&feat3_kernel<P>),
                                                          256, 0) != hipSuccess || per < 1)
             per = 2;
         (void)hipGetLastError();
@@ -959,9 +1084,8 @@ void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, c
         int kchunk = 0;
         unsigned nb = 0;
 #define TDZ_FEAT3(PP)                                                                                               \
-    feat3_grid<PP, false>(n0, n1, n2, kchunk, nb);                                                                  \
-    hipLaunchKernelGGL((feat3_kernel<PP, false>), dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, My, corr, \
-                       u, nullptr, nullptr, fcomplex{0.f, 0.f}, nullptr)
+    feat3_grid<PP>(n0, n1, n2, kchunk, nb);                                                                         \
+    hipLaunchKernelGGL((feat3_kernel<PP>), dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, My, corr, u)
         switch (p) {
         case 2: TDZ_FEAT3(2); break;
         case 3: TDZ_FEAT3(3); break;
@@ -980,26 +1104,33 @@ void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, c
 }
 
 bool fused_correction(hipStream_t st, const FusedBands &fb, const fcomplex *y, const fcomplex *r, const float *Vg,
-                      fcomplex g, fcomplex *q)
+                      fcomplex g, fcomplex *q, fcomplex *s1, fcomplex *s2)
 {
-    const int n0 = fb.n[0], n1 = fb.n[1], n2 = fb.n[2];
-    const int p = fb.p[0];
-    if (!(p == fb.p[1] && p == fb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p)) return false;
-    const CnnScales sc{{1.f, 1.f, 1.f, 1.f}};
-    int kchunk = 0;
-    unsigned nb = 0;
-#define TDZ_FC(PP)                                                                                                \
-    feat3_grid<PP, true>(n0, n1, n2, kchunk, nb);                                                                 \
-    hipLaunchKernelGGL((feat3_kernel<PP, true>), dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, nullptr, \
-                       nullptr, nullptr, r, Vg, g, q)
-    switch (p) {
-    case 2: TDZ_FC(2); break;
-    case 3: TDZ_FC(3); break;
-    case 4: TDZ_FC(4); break;
-    default: TDZ_FC(5); break;
+    if (s1 && s2 && fb.p[0] == fb.p[1] && fb.p[0] == fb.p[2] && fb.p[0] >= 2 && fb.p[0] <= 5 &&
+        fb.n[2] > 2 * fb.p[0]) {   // the two-pass form
+        const int n0 = fb.n[0], n1 = fb.n[1], n2 = fb.n[2];
+        constexpr int PZ = 4;
+        const unsigned nxy = (unsigned)(((n0 + 15) / 16) * ((n1 + 15) / 16) * ((n2 + PZ - 1) / PZ));
+        // z chunks of at least 2P + 2 planes, about a thousand blocks in all
+        const long plane = (long)n0 * n1;
+        const unsigned cb = (unsigned)((plane + 255) / 256);
+        const int p = fb.p[0];
+        const int chunks = std::max(1, std::min((int)(1024 / std::max(1u, cb)), n2 / (2 * p + 2)));
+        const int kchunk = (n2 + chunks - 1) / chunks;
+        const dim3 gz(cb, (unsigned)((n2 + kchunk - 1) / kchunk));
+#define TDZ_C2(PP)                                                                                                 \
+    hipLaunchKernelGGL((corr_xy_kernel<PP, PZ>), dim3(nxy), dim3(256), 0, st, n0, n1, n2, fb, y, s1, s2);        \
+    hipLaunchKernelGGL((corr_z_kernel<PP>), gz, dim3(256), 0, st, n0, n1, n2, kchunk, fb, s1, s2, r, Vg, g, q)
+        switch (p) {
+        case 2: TDZ_C2(2); break;
+        case 3: TDZ_C2(3); break;
+        case 4: TDZ_C2(4); break;
+        default: TDZ_C2(5); break;
+        }
+#undef TDZ_C2
+        return true;
     }
-#undef TDZ_FC
-    return true;
+    return false;
 }
 
 namespace {
