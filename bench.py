@@ -274,7 +274,7 @@ def atom(args):
     achieved = flops / avg_s if nlaunch else 0.0
     roofline = {"bound": "mfma", "compute_unit": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6)",
                 "kernel": "pot3d_kernel + pot3d_sum_kernel (fine-level potential apply)",
-                "limiter": "FP64 VALU issue 59% and LDS 51% busy (profiles/pot3d_softcore128_sq_counters.md)",
+                "limiter": "FP64 VALU issue 60% busy, LDS 33% (profiles/pot3d_softcore128_sq_counters.md)",
                 "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64, "traffic": None, "launches": nlaunch, "avg_ms": avg_s * 1e3,
                 "flops_per_launch": flops}

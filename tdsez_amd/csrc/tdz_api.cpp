@@ -1392,7 +1392,7 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = 
     // when they need two or more Krylov iterations (tol / r0 < 1e-6, choose_sweeps).
     const bool cform = c->corr_form == 1 && c->fd_transform > 0 && (c->corrections > 0 || c->pc_cform);
     fcomplex *y0 = cform ? c->sf.y0 : L.f.z;
-    const int ckind = cform ? 3 : -1;
+    const int ckind = cform ? 3 : -1;   // single FP16 products
     fd_apply<float>(c, L, L.f.r, y0, c->sf.w1, c->sf.w2);                // y = S^-1 r
     if (correction_bound(c) > 0.25) {   // perturbation too large: the separable solve alone
         tdz::launch_promote(c->st, y0, z, N);
