@@ -2290,11 +2290,24 @@ bool launch_mass_dipole_z(hipStream_t st, const int n[3], int p, const double *M
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
             ncu = 256;
     }
+    // z chunks: one round of blocks at the kernel's occupancy (from the runtime, per order)
+    static int per[6] = {};
     const int tiles = ((n0 + 15) / 16) * ((n1 + 15) / 16);
-    const int chunks = std::max(1, std::min(2 * ncu / tiles, n2 / (2 * p + 2)));
-    const int kchunk = (n2 + chunks - 1) / chunks;
-    const unsigned nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
+    int kchunk = 0;
+    unsigned nb = 0;
+    auto grid = [&](const void *fn) {
+        if (!per[p]) {
+            int b = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, 256, 0) != hipSuccess || b < 1) b = 2;
+            (void)hipGetLastError();
+            per[p] = b;
+        }
+        const int chunks = std::max(1, std::min(per[p] * ncu / tiles, n2 / (2 * p + 2)));
+        kchunk = (n2 + chunks - 1) / chunks;
+        nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
+    };
 #define TDZ_MD(PP)                                                                                              \
+    grid(reinterpret_cast<const void *>(&mass_dip_kernel<PP>));                                                   \
     hipLaunchKernelGGL(mass_dip_kernel<PP>, dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, M[0], M[1], M[2], Xz, x, \
                        m, dz, xc)
     switch (p) {
@@ -2533,7 +2546,7 @@ void launch_split16_axis(hipStream_t st, int nd, long ncol, long Lin, long sB, l
 #define TDZ_S16(KB, GG)                                                                                             \
     do {                                                                                                        \
         if (!DIAG && sK == 1 && nd % 2 == 0)                                                                    \
-            hipLaunchKernelGGL((axis_split16_kernel<KB, GG, false, 3, true>), dim3(nblk), dim3(256), 0, st, nd,  \
+            hipLaunchKernelGGL((axis_split16_kernel<KB, GG, false, 4, true>), dim3(nblk), dim3(256), 0, st, nd,  \
                                ncol, Lin, sB, sL, sK, nit, T4, ainv, x, y, dg, nbytes);                          \
         else                                                                                                    \
             hipLaunchKernelGGL((axis_split16_kernel<KB, GG, DIAG, 4, false>), dim3(nblk), dim3(256), 0, \
