@@ -1550,8 +1550,9 @@ double dev_norm2(tdz_ctx *c, const dcomplex *x, long N)
 // (j+1)-term vector update instead of an operator apply.
 // The host waits for the device once per Arnoldi step (the projections) and once at the
 // start (||b|| and, with r0_in_v0, ||r0|| together); the solution update at the end is
-// queued without a wait (its coefficients are uploaded from pinned staging, which the
-// next solve rewrites only after its own first wait).
+// queued without a wait (its coefficients travel in the kernel arguments; above
+// kCoefArgMax they are uploaded from pinned staging, which the next solve rewrites only
+// after its own first wait).
 int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, tdz_ksp_stats *st,
            bool r0_in_v0 = false, bool allow_fail = false, dcomplex *ax_out = nullptr)
 {
@@ -1562,6 +1563,17 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
     auto Zj = [&](int j) { return c->pc_kind == TDZ_PC_NONE ? Vj(j) : c->Zb + (long)j * N; };
     std::vector<cplx> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hv(m + 1), hv2(m + 1);
     double bnorm, beta0 = -1;   // beta0 >= 0: ||V_0|| of the first cycle, fetched with ||b||
+    // pinned staging: [0, 2(m+1)] the projections and the norm read back (the device layout of
+    // dh, dh2, dnorm); then the solution and residual coefficients of the update (passed in
+    // the kernel arguments, or uploaded to dh, dh2 from here for restarts above kCoefArgMax)
+    const int slen = 4 * (m + 1) + 1;
+    if (c->hstage_len < slen) {
+        HIPCHK(hipStreamSynchronize(c->st));   // no queued upload still reads the old staging
+        if (c->hstage) HIPCHK(hipHostFree(c->hstage));
+        c->hstage = nullptr;
+        HIPCHK(hipHostMalloc((void **)&c->hstage, slen * sizeof(dcomplex), hipHostMallocDefault));
+        c->hstage_len = slen;
+    }
     const bool queued = c->norms_queued;
     c->norms_queued = false;
     if (r0_in_v0) {
@@ -1569,23 +1581,15 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             tdz::launch_norm2(c->st, b, N, c->dpartr, c->dnorm + 1);
             tdz::launch_norm2(c->st, Vj(0), N, c->dpartr, c->dnorm);
         }
-        double h2[2] = {0, 0};
+        // into the readback part of the staging (an earlier solve's queued upload, if any,
+        // reads the coefficient part, and precedes this copy on the stream)
+        double *h2 = reinterpret_cast<double *>(c->hstage);
         HIPCHK(hipMemcpyAsync(h2, c->dnorm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         bnorm = std::sqrt(h2[1]);
         beta0 = std::sqrt(h2[0]);
     } else {
         bnorm = std::sqrt(dev_norm2(c, b, N));
-    }
-    // staging: now past a wait, so no earlier copy still uses it. [0, 2(m+1)]: the
-    // projections and the norm read back (the device layout of dh, dh2, dnorm); then the
-    // solution and residual coefficients uploaded to dh, dh2
-    const int slen = 4 * (m + 1) + 1;
-    if (c->hstage_len < slen) {
-        if (c->hstage) HIPCHK(hipHostFree(c->hstage));
-        c->hstage = nullptr;
-        HIPCHK(hipHostMalloc((void **)&c->hstage, slen * sizeof(dcomplex), hipHostMallocDefault));
-        c->hstage_len = slen;
     }
     const dcomplex *hback = c->hstage;
     dcomplex *ystage = c->hstage + 2 * (m + 1) + 1, *rstage = ystage + (m + 1);
@@ -1728,11 +1732,16 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
                 rstage[j] = dc(cw);
             }
         }
-        // y to dh and, with A x, the residual coefficients to dh2 = dh + m + 1: one copy
-        HIPCHK(hipMemcpyAsync(c->dh, ystage, (with_ax ? (m + 1) + (j + 1) : j) * sizeof(dcomplex),
-                              hipMemcpyHostToDevice, c->st));
-        tdz::launch_multi_axpy(c->st, c->pc_kind == TDZ_PC_NONE ? c->Vb : c->Zb, N, j, c->dh, x, N);
-        if (with_ax) tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N, b);   // ax_out = b + sum
+        const dcomplex *Zb = c->pc_kind == TDZ_PC_NONE ? c->Vb : c->Zb;
+        if (j + 1 <= tdz::kCoefArgMax) {   // the coefficients in the kernel arguments: no upload
+            tdz::launch_multi_axpy_args(c->st, Zb, N, j, ystage, x, N);
+            if (with_ax) tdz::launch_multi_axpy_args(c->st, c->Vb, N, j + 1, rstage, ax_out, N, b);   // b + sum
+        } else {   // y to dh and, with A x, the residual coefficients to dh2 = dh + m + 1: one copy
+            HIPCHK(hipMemcpyAsync(c->dh, ystage, (with_ax ? (m + 1) + (j + 1) : j) * sizeof(dcomplex),
+                                  hipMemcpyHostToDevice, c->st));
+            tdz::launch_multi_axpy(c->st, Zb, N, j, c->dh, x, N);
+            if (with_ax) tdz::launch_multi_axpy(c->st, c->Vb, N, j + 1, c->dh2, ax_out, N, b);   // ax_out = b + sum
+        }
         if (converged || its >= maxit) break;
     }
     if (chose && converged && its > 0 && c->pc_r0rel > 0 && res > 0) {   // mean contraction per iteration
@@ -1890,6 +1899,10 @@ void build(tdz_ctx *c, const tdz_problem &pr)
         if (!(pr.hi[d] > pr.lo[d])) throw Error("empty domain");
     }
     c->prob = pr;
+    // The Krylov solves wait for a few scalars once per Arnoldi step: the host thread spins on
+    // completion instead of yielding (the default sleep-and-wake adds tens of microseconds per
+    // wait). Refused once the device is in use with other flags: then the default stays.
+    if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
     HIPCHK(hipGetDevice(&c->device));
     c->ph.mass = pr.mass;
     c->ph.s = -pr.charge;

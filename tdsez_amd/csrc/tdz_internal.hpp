@@ -143,8 +143,16 @@ void launch_multi_axpy_norm_dot(hipStream_t st, const dcomplex *V, long ld, int 
 void launch_correct_scale(hipStream_t st, const dcomplex *V, long ld, int nv, const dcomplex *h, dcomplex *w, long N,
                           double s);
 // x = base + sum_i c_i Z_i (c on device; base null: x itself)
+// up to kCoefArgMax coefficients passed by value in the kernel arguments (no upload)
+constexpr int kCoefArgMax = 64;
+struct CoefArg {
+    dcomplex v[kCoefArgMax];
+};
 void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *c, dcomplex *x, long N,
                        const dcomplex *base = nullptr);
+// the same with the coefficients (host memory, nv <= kCoefArgMax) in the kernel's arguments
+void launch_multi_axpy_args(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *hc, dcomplex *x,
+                            long N, const dcomplex *base = nullptr);
 // The bands of the shifted operator alpha M + g H (non-layered), from the device copies
 // of the axis bands, per axis d (active: d < dim):
 //   B_d = g (S_d / 2m - i W_d + sE_d X_d) (+ alpha M_d for d = 0; g (0) on inactive axes),

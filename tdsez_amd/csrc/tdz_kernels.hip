@@ -971,10 +971,10 @@ __global__ void correct_scale_kernel(const dcomplex *__restrict__ V, long ld, in
 }
 
 __global__ void multi_axpy_kernel(const dcomplex *__restrict__ Z, long ld, int nv, const dcomplex *__restrict__ c,
-                                  const dcomplex *base, dcomplex *x, long N)
+                                  const dcomplex *base, dcomplex *x, long N, CoefArg cv)
 {
-    __shared__ dcomplex cs[64];
-    for (int i = threadIdx.x; i < nv; i += blockDim.x) cs[i] = c[i];
+    __shared__ dcomplex cs[kCoefArgMax];
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) cs[i] = c ? c[i] : cv.v[i];
     __syncthreads();
     const long stride = (long)gridDim.x * blockDim.x;
     for (long I = (long)blockIdx.x * blockDim.x + threadIdx.x; I < N; I += stride) {
@@ -2213,7 +2213,16 @@ void launch_multi_axpy(hipStream_t st, const dcomplex *Z, long ld, int nv, const
                        const dcomplex *base)
 {
     hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, Z, ld, nv, c,
-                       base ? base : x, x, N);
+                       base ? base : x, x, N, CoefArg{});
+}
+
+void launch_multi_axpy_args(hipStream_t st, const dcomplex *Z, long ld, int nv, const dcomplex *hc, dcomplex *x,
+                            long N, const dcomplex *base)
+{
+    CoefArg cv{};
+    for (int i = 0; i < nv && i < kCoefArgMax; ++i) cv.v[i] = hc[i];
+    hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_capped(N, 256, 4096)), dim3(256), 0, st, Z, ld, nv,
+                       (const dcomplex *)nullptr, base ? base : x, x, N, cv);
 }
 
 // one thread per band entry, blockIdx.y the axis
