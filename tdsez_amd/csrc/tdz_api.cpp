@@ -282,6 +282,9 @@ struct tdz_ctx {
     double its_hat[2][4] = {};     // moving average of the Krylov iterations of each choice
     double its_lt[2][4] = {};      // and of log10(tol / r0) of the solves it averages
     long pc_solves = 0;         // adaptive solves since pc_setup (the exploration schedule)
+    long pc_next_drop = 12;     // the next solve that tries one sweep fewer
+    int pc_drop_iv = 32;        // solves between such tries (doubled after each that costs iterations)
+    bool pc_exploring = false;  // this solve tries one sweep fewer than the model's choice
     bool cap_on = true;         // the CAP is part of H (off during the eigensolve, as src/propagator.cpp:51-69)
     // Pade(2,2) stage guesses: per slot (2 x stage) the last increments y - x
     dcomplex *hist = nullptr;   // [4 slots][3] vectors
@@ -1283,10 +1286,12 @@ void random_samples(tdz_ctx *c, int count, unsigned long long seed)
 // an unobserved sweep count is estimated from
 // an observed one of the same choice as rho_j = rho_k^((j+1)/(k+1)) (the error after k
 // sweeps is (S^-1 dA)^(k+1)), and an unobserved network choice from the other one (the
-// network assumed to gain half an order per iteration, so that it is tried first). The
-// first solve after pc_setup takes two sweeps (with the network); from then on every 32nd
-// solve (offset 4) tries the other network choice and every 32nd (offset 12) that picks
-// k > 1 tries k - 1, so that estimates follow the field. Where a choice has been observed at
+// network assumed to gain half an order per iteration where only the network-free choice
+// has been observed; the reverse is taken as equal). The first solve after pc_setup takes
+// two sweeps (with the network); from then on every 32nd solve (offset 4) tries the other
+// network choice, and a solve that picks k > 1 tries k - 1 at solve 12 and then every 32
+// solves, the interval doubled (up to 512) after each try that took more iterations than k
+// does, so that estimates follow the field. Where a choice has been observed at
 // a similar target (within a decade of tol / r0), the moving average of its iteration counts
 // (weight 0.05 per solve) replaces the prediction from rho.
 void choose_sweeps(tdz_ctx *c)
@@ -1312,7 +1317,7 @@ void choose_sweeps(tdz_ctx *c)
         const int ref = observed_ref(net);
         if (ref > 0) return std::pow(c->rho_hat[net][ref], (k + 1.0) / (ref + 1.0));
         const int other = 1 - net, oref = observed_ref(other);
-        const double po = net ? 1.5 : 1.0 / 1.5;
+        const double po = net ? 1.5 : 1.0;   // the unobserved network gains half an order
         if (oref > 0) {
             const double ro = c->rho_hat[other][k] > 0 ? c->rho_hat[other][k]
                                                        : std::pow(c->rho_hat[other][oref], (k + 1.0) / (oref + 1.0));
@@ -1352,7 +1357,12 @@ void choose_sweeps(tdz_ctx *c)
             }
         }
     ++c->pc_solves;
-    if (best > 1 && c->pc_solves % 32 == 12) --best;
+    c->pc_exploring = false;
+    if (best > 1 && c->pc_solves >= c->pc_next_drop) {
+        --best;
+        c->pc_exploring = true;
+        c->pc_next_drop = c->pc_solves + c->pc_drop_iv;
+    }
     if (can_net && c->pc_solves % 32 == 4) best_net = 1 - best_net;
     c->pc_sweeps = best;
     c->pc_net = best_net == 1;
@@ -1747,6 +1757,10 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
     if (chose && converged && its > 0 && c->pc_r0rel > 0 && res > 0) {   // mean contraction per iteration
         observe_sweeps(c, c->pc_sweeps, c->pc_net, std::pow(std::min(res / c->pc_r0rel, 1.0), 1.0 / its));
         const int k = c->pc_sweeps, n = c->pc_net ? 1 : 0;
+        if (c->pc_exploring && k + 1 <= 3) {   // one sweep fewer: try again soon only if it paid
+            const double ih = c->its_hat[n][k + 1];
+            c->pc_drop_iv = (ih > 0 && its > ih + 0.5) ? std::min(2 * c->pc_drop_iv, 512) : 32;
+        }
         if (k >= 1 && k <= 3) {
             const double lt = std::log10(c->pc_rtol / c->pc_r0rel);
             double &ih = c->its_hat[n][k], &il = c->its_lt[n][k];
@@ -1798,6 +1812,9 @@ void pc_setup(tdz_ctx *c, int kind)
     for (auto &row : c->rho_hat) std::fill(std::begin(row), std::end(row), -1.0);   // new contraction estimates
     for (auto &row : c->sweep_count) std::fill(std::begin(row), std::end(row), 0L);
     c->pc_solves = 0;
+    c->pc_next_drop = 12;
+    c->pc_drop_iv = 32;
+    c->pc_exploring = false;
     for (auto &row : c->its_hat) std::fill(std::begin(row), std::end(row), 0.0);
     c->pc_sweeps = 1;
     c->pc_net = true;
