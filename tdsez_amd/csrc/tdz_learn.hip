@@ -1047,6 +1047,177 @@ __global__ void __launch_bounds__(256) corr_z_kernel(int n0, int n1, int n2, int
     }
 }
 
+// The network sweep's products in two passes (3D, equal orders P), the same outputs as
+// feat3_kernel:
+//   feat_xy_kernel: per z plane and 16 x 16 tile of (x, y) columns, the four x/y products
+//     PP = (My (x) Mx) y, QQ = (My (x) Xx) y, PX = (Xy (x) Mx) y and
+//     S = (My (x) Cx + Cy (x) Mx) y (the plane with its halo in LDS; PZ planes per block);
+//   feat_zr_kernel: per (x, y) column and chunk of z, the z band passes over a ring of the
+//     last 2P + 1 planes of the four products: My = Mz PP, corr = Mz S + Cz PP and the
+//     scaled network input u = [y, Mz QQ - x My, Mz PX - y My, Xz PP - z My].
+template <int P, int PZ>
+__global__ void __launch_bounds__(256) feat_xy_kernel(int n0, int n1, int n2, FusedBands fb,
+                                                      const fcomplex *__restrict__ y, fcomplex *__restrict__ PPo,
+                                                      fcomplex *__restrict__ QQo, fcomplex *__restrict__ PXo,
+                                                      fcomplex *__restrict__ So)
+{
+    constexpr int R = 2 * P + 1, T = 16, H = T + 2 * P, NX = (H * H + 255) / 256;
+    __shared__ fcomplex Ys[H * H];
+    __shared__ fcomplex Ts[H * T], Qs[H * T], Rs[H * T];   // x passes: M, X, C
+    const int tid = threadIdx.x, ti = tid % T, tj = tid / T;
+    const int tilesx = (n0 + T - 1) / T, tilesy = (n1 + T - 1) / T;
+    int b = blockIdx.x;
+    const int bx = b % tilesx;
+    b /= tilesx;
+    const int by = b % tilesy, bz = b / tilesy;
+    const int i0 = bx * T, j0 = by * T, k0 = bz * PZ, k1 = min(k0 + PZ, n2);
+    const int i = i0 + ti, j = j0 + tj;
+    const bool own = i < n0 && j < n1;
+    const int ic = min(i, n0 - 1), jc = min(j, n1 - 1);
+    float mx[R], xx[R], my[R], xy[R];
+    fcomplex cx[R], cy[R];
+#pragma unroll
+    for (int l = 0; l < R; ++l) {
+        mx[l] = fb.M[0][ic * R + l];
+        xx[l] = fb.X[0][ic * R + l];
+        cx[l] = fb.C[0][ic * R + l];
+        my[l] = fb.M[1][jc * R + l];
+        xy[l] = fb.X[1][jc * R + l];
+        cy[l] = fb.C[1][jc * R + l];
+    }
+    const long plane = (long)n0 * n1;
+    fcomplex yv[NX];
+    auto load_plane = [&](int K) {
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + 256 * q;
+            const int ii = i0 - P + e % H, jj = j0 - P + e / H;
+            yv[q] = {0.f, 0.f};
+            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) yv[q] = y[K * plane + (long)jj * n0 + ii];
+        }
+    };
+    load_plane(k0);
+    for (int K = k0; K < k1; ++K) {
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            const int e = tid + 256 * q;
+            if (e < H * H) Ys[e] = yv[q];
+        }
+        __syncthreads();
+        if (K + 1 < k1) load_plane(K + 1);
+        for (int e = tid; e < H * T; e += 256) {   // e % T == ti
+            const int jj = e / T;
+            float tr = 0, tim = 0, qr = 0, qi = 0, rr = 0, ri = 0;
+#pragma unroll
+            for (int l = 0; l < R; ++l) {
+                const fcomplex v = Ys[jj * H + ti + l];
+                tr += mx[l] * v.re;
+                tim += mx[l] * v.im;
+                qr += xx[l] * v.re;
+                qi += xx[l] * v.im;
+                rr += cx[l].re * v.re - cx[l].im * v.im;
+                ri += cx[l].re * v.im + cx[l].im * v.re;
+            }
+            Ts[e] = {tr, tim};
+            Qs[e] = {qr, qi};
+            Rs[e] = {rr, ri};
+        }
+        __syncthreads();
+        float pp0 = 0, pp1 = 0, px0 = 0, px1 = 0, qq0 = 0, qq1 = 0, s0 = 0, s1 = 0;
+#pragma unroll
+        for (int l = 0; l < R; ++l) {
+            const int o = (tj + l) * T + ti;
+            const fcomplex tv = Ts[o], qv = Qs[o], rv = Rs[o];
+            pp0 += my[l] * tv.re;
+            pp1 += my[l] * tv.im;
+            px0 += xy[l] * tv.re;
+            px1 += xy[l] * tv.im;
+            qq0 += my[l] * qv.re;
+            qq1 += my[l] * qv.im;
+            s0 += my[l] * rv.re + cy[l].re * tv.re - cy[l].im * tv.im;
+            s1 += my[l] * rv.im + cy[l].re * tv.im + cy[l].im * tv.re;
+        }
+        if (own) {
+            const long I = K * plane + (long)j * n0 + i;
+            PPo[I] = {pp0, pp1};
+            QQo[I] = {qq0, qq1};
+            PXo[I] = {px0, px1};
+            So[I] = {s0, s1};
+        }
+    }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) feat_zr_kernel(int n0, int n1, int n2, int kchunk, FusedBands fb, CnnScales sc,
+                                                      const fcomplex *__restrict__ y, const fcomplex *__restrict__ PP,
+                                                      const fcomplex *__restrict__ QQ, const fcomplex *__restrict__ PX,
+                                                      const fcomplex *__restrict__ S, fcomplex *__restrict__ My,
+                                                      fcomplex *__restrict__ corr, fcomplex *__restrict__ u)
+{
+    constexpr int R = 2 * P + 1;
+    const long plane = (long)n0 * n1, N = plane * n2;
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= plane) return;   // no barriers below
+    const int k0 = blockIdx.y * kchunk, k1 = min(k0 + kchunk, n2);
+    if (k0 >= k1) return;
+    const float x0 = fb.g[0][col % n0], x1 = fb.g[1][col / n0];
+    // ring of the input planes K - 2P .. K of the four products (slot K mod R)
+    fcomplex pr[R], qr[R], xr[R], sr[R];
+#pragma unroll
+    for (int l = 0; l < R; ++l) pr[l] = qr[l] = xr[l] = sr[l] = {0.f, 0.f};
+    const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
+    for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int K = Kb + r;
+            if (K < Kbeg) continue;
+            if (K < Kend) {
+                const long src = K * plane + col;
+                pr[r] = PP[src];
+                qr[r] = QQ[src];
+                xr[r] = PX[src];
+                sr[r] = S[src];
+            } else {
+                pr[r] = qr[r] = xr[r] = sr[r] = {0.f, 0.f};
+            }
+            const int ko = K - P;   // its inputs ko - P .. ko + P are in the ring
+            if (ko < k0 || ko >= k1) continue;
+            float m0 = 0, m1 = 0, a0 = 0, a1 = 0, b0 = 0, b1 = 0, z0 = 0, z1 = 0, c0 = 0, c1 = 0;
+#pragma unroll
+            for (int d = -P; d <= P; ++d) {
+                const int kk = ko + d;   // band entry (ko, kk) = ko * R + P + d
+                if (kk < 0 || kk >= n2) continue;
+                const int e = ko * R + P + d;
+                const float mz = fb.M[2][e], xz = fb.X[2][e];
+                const fcomplex cz = fb.C[2][e];
+                const int sl = (r + R + d - P) % R;
+                const fcomplex pv = pr[sl], qv = qr[sl], xv = xr[sl], sv = sr[sl];
+                m0 += mz * pv.re;
+                m1 += mz * pv.im;
+                a0 += mz * qv.re;
+                a1 += mz * qv.im;
+                b0 += mz * xv.re;
+                b1 += mz * xv.im;
+                z0 += xz * pv.re;
+                z1 += xz * pv.im;
+                c0 += mz * sv.re + cz.re * pv.re - cz.im * pv.im;
+                c1 += mz * sv.im + cz.re * pv.im + cz.im * pv.re;
+            }
+            const long I = ko * plane + col;
+            My[I] = {m0, m1};
+            corr[I] = {c0, c1};
+            if (u) {
+                const float x2 = fb.g[2][ko];
+                const fcomplex yv = y[I];
+                u[I] = {sc.v[0] * yv.re, sc.v[0] * yv.im};
+                u[N + I] = {sc.v[1] * (a0 - x0 * m0), sc.v[1] * (a1 - x0 * m1)};
+                u[2 * N + I] = {sc.v[2] * (b0 - x1 * m0), sc.v[2] * (b1 - x1 * m1)};
+                u[3 * N + I] = {sc.v[3] * (z0 - x2 * m0), sc.v[3] * (z1 - x2 * m1)};
+            }
+        }
+    }
+}
+
 inline unsigned gridn(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Grid of a z-marching feature pass: one round of blocks at the kernel's own occupancy
@@ -1075,11 +1246,33 @@ void feat3_grid(int n0, int n1, int n2, int &kchunk, unsigned &nb)
 }  // namespace
 
 void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, const fcomplex *y, fcomplex *tmp[7],
-                    fcomplex *My, fcomplex *corr, fcomplex *u)
+                    fcomplex *My, fcomplex *corr, fcomplex *u, bool two_pass)
 {
     const int n0 = fb.n[0], n1 = fb.n[1], n2 = fb.n[2];
     const long N = (long)n0 * n1 * n2;
     const int p = fb.p[0];
+    if (two_pass && p == fb.p[1] && p == fb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p) {
+        constexpr int PZ = 4;
+        const unsigned nxy = (unsigned)(((n0 + 15) / 16) * ((n1 + 15) / 16) * ((n2 + PZ - 1) / PZ));
+        const long plane = (long)n0 * n1;
+        const unsigned cb = (unsigned)((plane + 255) / 256);
+        const int chunks = std::max(1, std::min((int)(1024 / std::max(1u, cb)), n2 / (2 * p + 2)));
+        const int kchunk = (n2 + chunks - 1) / chunks;
+        const dim3 gz(cb, (unsigned)((n2 + kchunk - 1) / kchunk));
+#define TDZ_F2(PP)                                                                                                    \
+    hipLaunchKernelGGL((feat_xy_kernel<PP, PZ>), dim3(nxy), dim3(256), 0, st, n0, n1, n2, fb, y, tmp[0], tmp[1],        \
+                       tmp[2], tmp[3]);                                                                             \
+    hipLaunchKernelGGL((feat_zr_kernel<PP>), gz, dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, tmp[0], tmp[1],       \
+                       tmp[2], tmp[3], My, corr, u)
+        switch (p) {
+        case 2: TDZ_F2(2); break;
+        case 3: TDZ_F2(3); break;
+        case 4: TDZ_F2(4); break;
+        default: TDZ_F2(5); break;
+        }
+#undef TDZ_F2
+        return;
+    }
     if (p == fb.p[1] && p == fb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p) {
         int kchunk = 0;
         unsigned nb = 0;
