@@ -372,11 +372,10 @@ struct FusedBands {
     const fcomplex *C[3];        // complex bands of the C term
     const float *g[3];           // Greville coordinates
 };
-// two_pass (3D, equal orders): the x/y products of every plane into tmp[0..3], then the z
-// passes; otherwise one z-marching pass (feat3_kernel) or, where that does not apply, three
-// separate passes.
+// 3D with equal orders: the x/y products of every plane into tmp[0..3], then the z passes;
+// otherwise three separate passes.
 void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, const fcomplex *y, fcomplex *tmp[7],
-                    fcomplex *My, fcomplex *corr, fcomplex *u, bool two_pass = true);
+                    fcomplex *My, fcomplex *corr, fcomplex *u);
 // The right-hand side of a correction sweep without the network, q = r - g Vg (M y) - corr(y)
 // (corr: the CAP and field bands; r null: zero), in two passes: the x and y band passes of
 // every plane into s1, s2 (complex64 scratch vectors), then the z band passes with the

@@ -739,153 +739,6 @@ __global__ void feat_z_kernel(int n0, int n1, int n2, int p, const float *__rest
     u[3 * N + I] = {sc.v[3] * (xz0 - x2 * m0), sc.v[3] * (xz1 - x2 * m1)};
 }
 
-// The three feature passes fused (complex64, 3D, equal orders P), z-marching like the
-// FP64 separable apply: a block owns a 16 x 16 tile of (x, y) columns and a chunk of
-// output planes; per input plane it stages the tile with halo (next plane in flight),
-// runs the x band passes (M, X, C) on the 16 + 2P rows the y passes need, the y passes
-// (My M, Xy M, My X, My C + Cy M) per column, and scatters the z band passes into five
-// rings of 2P + 1 output planes in registers:
-//   My = Mz My Mx y, xx = Mz My Xx y, xy = Mz Xy Mx y, xz = Xz My Mx y,
-//   corr = Mz My Cx y + Mz Cy Mx y + Cz My Mx y.
-// A finished plane writes My, corr and (if u) the scaled inputs
-// u = [y, xx - x My, xy - y My, xz - z My]: y is read once instead of the passes' 22
-// vector transfers.
-template <int P>
-__global__ void __launch_bounds__(256) feat3_kernel(int n0, int n1, int n2, int kchunk, FusedBands fb, CnnScales sc,
-                                                    const fcomplex *__restrict__ y, fcomplex *__restrict__ My,
-                                                    fcomplex *__restrict__ corr, fcomplex *__restrict__ u)
-{
-    constexpr int R = 2 * P + 1, T = 16, H = T + 2 * P;
-    __shared__ fcomplex Ys[H * H];
-    __shared__ fcomplex Ts[H * T], Qs[H * T], Rs[H * T];   // x passes: M, X, C
-    const int tid = threadIdx.x, ti = tid % T, tj = tid / T;
-    const int tilesx = (n0 + T - 1) / T, tilesy = (n1 + T - 1) / T;
-    int b = blockIdx.x;
-    const int bx = b % tilesx;
-    b /= tilesx;
-    const int by = b % tilesy, bz = b / tilesy;
-    const int i0 = bx * T, j0 = by * T, k0 = bz * kchunk, k1 = min(k0 + kchunk, n2);
-    const int i = i0 + ti, j = j0 + tj;
-    const bool own = i < n0 && j < n1;
-    const int ic = min(i, n0 - 1), jc = min(j, n1 - 1);
-    float mx[R], xx[R], my[R], xy[R];
-    fcomplex cx[R], cy[R];
-#pragma unroll
-    for (int l = 0; l < R; ++l) {
-        mx[l] = fb.M[0][ic * R + l];
-        xx[l] = fb.X[0][ic * R + l];
-        cx[l] = fb.C[0][ic * R + l];
-        my[l] = fb.M[1][jc * R + l];
-        xy[l] = fb.X[1][jc * R + l];
-        cy[l] = fb.C[1][jc * R + l];
-    }
-    fcomplex aM[R], aXX[R], aXY[R], aXZ[R], aC[R];
-    fcomplex yr[R];   // y at this thread's (i, j) of the last R staged planes (plane K in slot K mod R)
-#pragma unroll
-    for (int l = 0; l < R; ++l) aM[l] = aXX[l] = aXY[l] = aXZ[l] = aC[l] = yr[l] = {0.f, 0.f};
-    const float g0 = fb.g[0][ic], g1 = fb.g[1][jc];
-    const long plane = (long)n0 * n1;
-    const int Kbeg = max(k0 - P, 0), Kend = min(k1 + P, n2);
-    constexpr int NX = (H * H + 255) / 256;
-    fcomplex yv[NX];
-    auto load_plane = [&](int K) {
-#pragma unroll
-        for (int q = 0; q < NX; ++q) {
-            const int e = tid + 256 * q;
-            const int ii = i0 - P + e % H, jj = j0 - P + e / H;
-            yv[q] = {0.f, 0.f};
-            if (e < H * H && ii >= 0 && ii < n0 && jj >= 0 && jj < n1) yv[q] = y[K * plane + (long)jj * n0 + ii];
-        }
-    };
-    load_plane(Kbeg);
-    for (int Kb = (Kbeg / R) * R; Kb < k1 + P; Kb += R) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int K = Kb + r;
-            if (K >= Kbeg && K < Kend) {   // block-uniform
-#pragma unroll
-                for (int q = 0; q < NX; ++q) {
-                    const int e = tid + 256 * q;
-                    if (e < H * H) Ys[e] = yv[q];
-                }
-                __syncthreads();
-                if (K + 1 < Kend) load_plane(K + 1);
-                if (u) yr[r] = Ys[(tj + P) * H + ti + P];
-                for (int e = tid; e < H * T; e += 256) {   // e % T == ti
-                    const int jj = e / T;
-                    float tr = 0, tim = 0, qr = 0, qi = 0, rr = 0, ri = 0;
-#pragma unroll
-                    for (int l = 0; l < R; ++l) {
-                        const fcomplex v = Ys[jj * H + ti + l];
-                        tr += mx[l] * v.re;
-                        tim += mx[l] * v.im;
-                        qr += xx[l] * v.re;
-                        qi += xx[l] * v.im;
-                        rr += cx[l].re * v.re - cx[l].im * v.im;
-                        ri += cx[l].re * v.im + cx[l].im * v.re;
-                    }
-                    Ts[e] = {tr, tim};
-                    Qs[e] = {qr, qi};
-                    Rs[e] = {rr, ri};
-                }
-                __syncthreads();
-                float pp0 = 0, pp1 = 0, px0 = 0, px1 = 0, qq0 = 0, qq1 = 0, s0 = 0, s1 = 0;
-#pragma unroll
-                for (int l = 0; l < R; ++l) {
-                    const int o = (tj + l) * T + ti;
-                    const fcomplex tv = Ts[o], rv = Rs[o];
-                    pp0 += my[l] * tv.re;
-                    pp1 += my[l] * tv.im;
-                    const fcomplex qv = Qs[o];
-                    px0 += xy[l] * tv.re;
-                    px1 += xy[l] * tv.im;
-                    qq0 += my[l] * qv.re;
-                    qq1 += my[l] * qv.im;
-                    s0 += my[l] * rv.re + cy[l].re * tv.re - cy[l].im * tv.im;
-                    s1 += my[l] * rv.im + cy[l].re * tv.im + cy[l].im * tv.re;
-                }
-#pragma unroll
-                for (int d = -P; d <= P; ++d) {   // output k = K + d, band entry (k, K) = P - d
-                    const int k = K + d;
-                    if (k >= k0 && k < k1) {
-                        const int e = k * R + P - d;
-                        const float mz = fb.M[2][e], xz = fb.X[2][e];
-                        const fcomplex cz = fb.C[2][e];
-                        const int sl = (r + d + P) % R;
-                        aM[sl].re += mz * pp0;
-                        aM[sl].im += mz * pp1;
-                        aXX[sl].re += mz * qq0;
-                        aXX[sl].im += mz * qq1;
-                        aXY[sl].re += mz * px0;
-                        aXY[sl].im += mz * px1;
-                        aXZ[sl].re += xz * pp0;
-                        aXZ[sl].im += xz * pp1;
-                        aC[sl].re += mz * s0 + cz.re * pp0 - cz.im * pp1;
-                        aC[sl].im += mz * s1 + cz.re * pp1 + cz.im * pp0;
-                    }
-                }
-            }
-            const int ko = K - P;   // complete: every input plane within P of it is done
-            if (ko >= k0 && ko < k1 && own) {
-                const long I = ko * plane + (long)j * n0 + i;
-                const fcomplex m0 = aM[r];
-                My[I] = m0;
-                corr[I] = aC[r];
-                if (u) {
-                    const long N = plane * n2;
-                    const float g2 = fb.g[2][ko];
-                    const fcomplex yc = yr[(r + P + 1) % R];   // plane ko = K - P
-                    u[I] = {sc.v[0] * yc.re, sc.v[0] * yc.im};
-                    u[N + I] = {sc.v[1] * (aXX[r].re - g0 * m0.re), sc.v[1] * (aXX[r].im - g0 * m0.im)};
-                    u[2 * N + I] = {sc.v[2] * (aXY[r].re - g1 * m0.re), sc.v[2] * (aXY[r].im - g1 * m0.im)};
-                    u[3 * N + I] = {sc.v[3] * (aXZ[r].re - g2 * m0.re), sc.v[3] * (aXZ[r].im - g2 * m0.im)};
-                }
-            }
-            aM[r] = aXX[r] = aXY[r] = aXZ[r] = aC[r] = {0.f, 0.f};
-        }
-    }
-}
-
 // u[c N + v] *= s.v[c] for the CI input fields
 __global__ void scale_inputs_kernel(long N, CnnScales s, fcomplex *__restrict__ u)
 {
@@ -1047,8 +900,8 @@ __global__ void __launch_bounds__(256) corr_z_kernel(int n0, int n1, int n2, int
     }
 }
 
-// The network sweep's products in two passes (3D, equal orders P), the same outputs as
-// feat3_kernel:
+// The network sweep's products in two passes (3D, equal orders P): My, corr and the
+// network's input from four x/y products:
 //   feat_xy_kernel: per z plane and 16 x 16 tile of (x, y) columns, the four x/y products
 //     PP = (My (x) Mx) y, QQ = (My (x) Xx) y, PX = (Xy (x) Mx) y and
 //     S = (My (x) Cx + Cy (x) Mx) y (the plane with its halo in LDS; PZ planes per block);
@@ -1220,38 +1073,15 @@ __global__ void __launch_bounds__(256) feat_zr_kernel(int n0, int n1, int n2, in
 
 inline unsigned gridn(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// Grid of a z-marching feature pass: one round of blocks at the kernel's own occupancy
-// (blocks of 256 per CU, from the runtime), each z chunk at least 2P + 2 planes long.
-template <int P>
-void feat3_grid(int n0, int n1, int n2, int &kchunk, unsigned &nb)
-{
-    static int slots = 0;   // per instantiation: CUs x resident blocks
-    if (!slots) {
-        int dev = 0, ncu = 0, per = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
-            ncu = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&feat3_kernel<P>),
-                                                         256, 0) != hipSuccess || per < 1)
-            per = 2;
-        (void)hipGetLastError();
-        slots = ncu * per;
-    }
-    const int tiles = ((n0 + 15) / 16) * ((n1 + 15) / 16);
-    const int chunks = std::max(1, std::min(slots / tiles, n2 / (2 * P + 2)));
-    kchunk = (n2 + chunks - 1) / chunks;
-    nb = (unsigned)(tiles * ((n2 + kchunk - 1) / kchunk));
-}
-
 }  // namespace
 
 void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, const fcomplex *y, fcomplex *tmp[7],
-                    fcomplex *My, fcomplex *corr, fcomplex *u, bool two_pass)
+                    fcomplex *My, fcomplex *corr, fcomplex *u)
 {
     const int n0 = fb.n[0], n1 = fb.n[1], n2 = fb.n[2];
     const long N = (long)n0 * n1 * n2;
     const int p = fb.p[0];
-    if (two_pass && p == fb.p[1] && p == fb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p) {
+    if (p == fb.p[1] && p == fb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p) {
         constexpr int PZ = 4;
         const unsigned nxy = (unsigned)(((n0 + 15) / 16) * ((n1 + 15) / 16) * ((n2 + PZ - 1) / PZ));
         const long plane = (long)n0 * n1;
@@ -1271,21 +1101,6 @@ void fused_products(hipStream_t st, const FusedBands &fb, const CnnScales &sc, c
         default: TDZ_F2(5); break;
         }
 #undef TDZ_F2
-        return;
-    }
-    if (p == fb.p[1] && p == fb.p[2] && p >= 2 && p <= 5 && n2 > 2 * p) {
-        int kchunk = 0;
-        unsigned nb = 0;
-#define TDZ_FEAT3(PP)                                                                                               \
-    feat3_grid<PP>(n0, n1, n2, kchunk, nb);                                                                         \
-    hipLaunchKernelGGL((feat3_kernel<PP>), dim3(nb), dim3(256), 0, st, n0, n1, n2, kchunk, fb, sc, y, My, corr, u)
-        switch (p) {
-        case 2: TDZ_FEAT3(2); break;
-        case 3: TDZ_FEAT3(3); break;
-        case 4: TDZ_FEAT3(4); break;
-        default: TDZ_FEAT3(5); break;
-        }
-#undef TDZ_FEAT3
         return;
     }
     hipLaunchKernelGGL(feat_x_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, n0, N / n0, fb.p[0], fb.M[0], fb.X[0],
