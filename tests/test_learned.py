@@ -167,6 +167,11 @@ def test_learned_preconditioner_gain_and_fallback():
         d = ctx.vec(x0)
         its, _ = ctx.evolve_cn(d, 0.0, 0.02, 5, pulse, rtol=rtol)
         runs[kind] = (its, d.download())
+        # one adaptive choice of sweeps (and, learned, of the network) per CN solve
+        choices = ctx.sweep_stats()
+        print(T.PC_LEARNED == kind and "learned" or "network-free", "sweep choices", choices)
+        assert sum(choices.values()) == 5, choices
+        assert kind == T.PC_LEARNED or not any(k.startswith("net_") for k in choices), choices
     its_c, xc = runs[T.PC_CORRECTED]
     its_l, xl = runs[T.PC_LEARNED]
     print("iterations without / with the network", its_c, its_l)
