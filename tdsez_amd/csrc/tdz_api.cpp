@@ -285,6 +285,11 @@ struct tdz_ctx {
     long pc_next_drop = 12;     // the next solve that tries one sweep fewer
     int pc_drop_iv = 32;        // solves between such tries (doubled after each that costs iterations)
     bool pc_exploring = false;  // this solve tries one sweep fewer than the model's choice
+    long pc_next_flip = 4;      // the next solve that tries the other network choice
+    int pc_flip_iv = 32;        // solves between such tries (doubled after each that gained nothing)
+    bool pc_flipping = false;   // this solve tries the other network choice
+    int pc_flip_from_k = 0;     // the model's choice that solve replaced: sweeps and network
+    bool pc_flip_from_net = false;
     bool cap_on = true;         // the CAP is part of H (off during the eigensolve, as src/propagator.cpp:51-69)
     // Pade(2,2) stage guesses: per slot (2 x stage) the last increments y - x
     dcomplex *hist = nullptr;   // [4 slots][3] vectors
@@ -1294,10 +1299,10 @@ void random_samples(tdz_ctx *c, int count, unsigned long long seed)
 // sweeps is (S^-1 dA)^(k+1)), and an unobserved network choice from the other one (the
 // network assumed to gain half an order per iteration where only the network-free choice
 // has been observed; the reverse is taken as equal). The first solve after pc_setup takes
-// two sweeps (with the network); from then on every 32nd solve (offset 4) tries the other
-// network choice, and a solve that picks k > 1 tries k - 1 at solve 12 and then every 32
-// solves, the interval doubled (up to 512) after each try that took more iterations than k
-// does, so that estimates follow the field. Where a choice has been observed at
+// two sweeps (with the network); from then on the other network choice is tried at solve 4
+// and then every 32 solves, and a solve that picks k > 1 tries k - 1 at solve 12 and then
+// every 32 solves; either interval is doubled (up to 512) after each try that gained no
+// iterations, so that estimates follow the field at little cost. Where a choice has been observed at
 // a similar target (within a decade of tol / r0), the moving average of its iteration counts
 // (weight 0.05 per solve) replaces the prediction from rho.
 void choose_sweeps(tdz_ctx *c)
@@ -1336,6 +1341,7 @@ void choose_sweeps(tdz_ctx *c)
         for (double v : row) seen = seen || v > 0;
     if (!seen) {   // the first solve after pc_setup: two sweeps, with the network if there is one
         ++c->pc_solves;
+        c->pc_exploring = c->pc_flipping = false;
         c->pc_sweeps = 2;
         ++c->sweep_count[can_net ? 1 : 0][2];
         return;
@@ -1369,7 +1375,14 @@ void choose_sweeps(tdz_ctx *c)
         c->pc_exploring = true;
         c->pc_next_drop = c->pc_solves + c->pc_drop_iv;
     }
-    if (can_net && c->pc_solves % 32 == 4) best_net = 1 - best_net;
+    c->pc_flipping = false;
+    if (can_net && c->pc_solves >= c->pc_next_flip) {
+        c->pc_flipping = true;
+        c->pc_flip_from_k = best;
+        c->pc_flip_from_net = best_net == 1;
+        best_net = 1 - best_net;
+        c->pc_next_flip = c->pc_solves + c->pc_flip_iv;
+    }
     c->pc_sweeps = best;
     c->pc_net = best_net == 1;
     ++c->sweep_count[c->pc_net ? 1 : 0][best];
@@ -1767,6 +1780,10 @@ int fgmres(tdz_ctx *c, const dcomplex *b, dcomplex *x, double rtol, int maxit, t
             const double ih = c->its_hat[n][k + 1];
             c->pc_drop_iv = (ih > 0 && its > ih + 0.5) ? std::min(2 * c->pc_drop_iv, 512) : 32;
         }
+        if (c->pc_flipping) {   // the other network choice: try again soon only if it saved iterations
+            const double ih = c->its_hat[c->pc_flip_from_net ? 1 : 0][c->pc_flip_from_k];
+            c->pc_flip_iv = (ih > 0 && its > ih - 0.5) ? std::min(2 * c->pc_flip_iv, 512) : 32;
+        }
         if (k >= 1 && k <= 3) {
             const double lt = std::log10(c->pc_rtol / c->pc_r0rel);
             double &ih = c->its_hat[n][k], &il = c->its_lt[n][k];
@@ -1821,6 +1838,9 @@ void pc_setup(tdz_ctx *c, int kind)
     c->pc_next_drop = 12;
     c->pc_drop_iv = 32;
     c->pc_exploring = false;
+    c->pc_next_flip = 4;
+    c->pc_flip_iv = 32;
+    c->pc_flipping = false;
     for (auto &row : c->its_hat) std::fill(std::begin(row), std::end(row), 0.0);
     c->pc_sweeps = 1;
     c->pc_net = true;
