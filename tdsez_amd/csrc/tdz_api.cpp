@@ -1942,10 +1942,6 @@ void build(tdz_ctx *c, const tdz_problem &pr)
         if (!(pr.hi[d] > pr.lo[d])) throw Error("empty domain");
     }
     c->prob = pr;
-    // The Krylov solves wait for a few scalars once per Arnoldi step: the host thread spins on
-    // completion instead of yielding (the default sleep-and-wake adds tens of microseconds per
-    // wait). Refused once the device is in use with other flags: then the default stays.
-    if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
     HIPCHK(hipGetDevice(&c->device));
     c->ph.mass = pr.mass;
     c->ph.s = -pr.charge;
