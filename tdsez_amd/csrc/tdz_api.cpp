@@ -150,7 +150,7 @@ namespace {
 struct LearnState {
     bool ready = false;
     float *P = nullptr;                  // [N][C] positional fields
-    unsigned short *Pb = nullptr;        // the same as bf16 (inference)
+    tdz::CnnFieldGen fgen;               // the same fields' closed forms (inference)
     std::vector<double> fscale;          // [C] maximum of each field before scaling to 1
     float *W1 = nullptr, *W2 = nullptr;  // device weights [27][CI][C], [27][C][C]
     float *gr[3] = {};                   // Greville coordinates per axis (float)
@@ -1486,7 +1486,7 @@ void corrected_apply(tdz_ctx *c, const dcomplex *r, dcomplex *z, bool exact_v = 
         if (net) {
             const int n3[3] = {L.n[0], L.n[1], L.n[2]};
             // the weights' BF16 operand packing persists in the work buffer until they change
-            tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{0.f, 0.f}, nullptr,
+            tdz::cnn_infer_mfma(c->st, n3, ls.feat, ls.W1, ls.W2, ls.fgen, ls.mb, ls.out, fcomplex{0.f, 0.f}, nullptr,
                                 ls.packed_ver != ls.wver);
             ls.packed_ver = ls.wver;
             no = ls.out;
@@ -2987,7 +2987,8 @@ void ground_state(tdz_ctx *c, dcomplex *x, double sigma, double tol, int maxit, 
 // Positional fields of the network at the Greville points of the fine level: 1, V,
 // h dV, h^2 d2V (6), V^2, x/L (3), r^2/L^2 for the soft-core potential
 // V = -s Z / sqrt(r^2 + a); each channel scaled to unit maximum over the grid.
-void learned_fields(tdz_ctx *c, std::vector<float> &P, std::vector<double> &scale)
+// fg: the same fields' parameters for their evaluation on the device (without coordinates)
+void learned_fields(tdz_ctx *c, std::vector<float> &P, std::vector<double> &scale, tdz::CnnFieldGen &fg)
 {
     Level &L = *c->lv[0];
     constexpr int C = tdz::kCnnChannels;
@@ -3030,6 +3031,14 @@ void learned_fields(tdz_ctx *c, std::vector<float> &P, std::vector<double> &scal
     for (size_t v = 0; v < (size_t)L.N; ++v)
         for (int q = 0; q < C; ++q) P[v * C + q] = (float)(mx[q] > 0 ? f[v * C + q] / mx[q] : 0.0);
     scale = mx;
+    fg.sZ = (float)sZ;
+    fg.a = (float)a;
+    for (int d = 0; d < 3; ++d) {
+        fg.h[d] = (float)h[d];
+        fg.inv_l[d] = (float)(1.0 / Lx[d]);
+    }
+    fg.inv_r2 = (float)(1.0 / (Lx[0] * Lx[0] + Lx[1] * Lx[1] + Lx[2] * Lx[2]));
+    for (int q = 0; q < C; ++q) fg.scale[q] = (float)(mx[q] > 0 ? 1.0 / mx[q] : 0.0);
 }
 
 void init_weights(tdz_ctx *c);
@@ -3044,12 +3053,8 @@ void setup_learned(tdz_ctx *c)
     constexpr int C = tdz::kCnnChannels, CI = tdz::kCnnInputs;
     const long N = L.N;
     std::vector<float> P;
-    learned_fields(c, P, ls.fscale);
+    learned_fields(c, P, ls.fscale, ls.fgen);
     ls.P = upload(c, P);
-    {
-        const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-        ls.Pb = upload(c, tdz::cnn_fields_bf16(P));
-    }
     ls.W1 = track(c, dalloc<float>(27 * CI * C));
     ls.W2 = track(c, dalloc<float>(27 * C * C));
     ls.mb = track(c, dalloc<unsigned short>(tdz::cnn_infer_work_bytes(N) / 2));
@@ -3064,6 +3069,7 @@ void setup_learned(tdz_ctx *c)
             for (int i = 0; i < L.n[d]; ++i) g[i] = (float)gd[i];
         }
         ls.gr[d] = upload(c, g);
+        ls.fgen.gr[d] = ls.gr[d];
     }
     ls.ready = true;
     init_weights(c);
@@ -3654,7 +3660,7 @@ int tdz_learn_eval(tdz_ctx *c, const double *u, const double *target, double *lo
         }
         if (out_mfma) {
             const int n3[3] = {L.n[0], L.n[1], L.n[2]};
-            tdz::cnn_infer_mfma(c->st, n3, y32, ls.W1, ls.W2, ls.Pb, ls.mb, ls.out, fcomplex{1.f, 0.f}, nullptr, true);
+            tdz::cnn_infer_mfma(c->st, n3, y32, ls.W1, ls.W2, ls.fgen, ls.mb, ls.out, fcomplex{1.f, 0.f}, nullptr, true);
             ls.packed_ver = ls.wver;
             HIPCHK(hipStreamSynchronize(c->st));
             HIPCHK(hipMemcpy(o.data(), ls.out, o.size() * 4, hipMemcpyDeviceToHost));

@@ -343,14 +343,26 @@ void cnn_forward_f32(hipStream_t st, const int n[3], const fcomplex *u, const fl
 void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const fcomplex *target, float inv_t2,
                       const float *W2, const float *P, const float *m, const float *out, float *dov, float *dh,
                       float *partial, int nchunk, float *dW1, float *dW2);
+// The positional fields as closed forms of the Greville coordinates (x, y, z) =
+// (gr[0][i], gr[1][j], gr[2][k]): field q of voxel (i, j, k) is scale[q] f_q with
+// rho = (r^2 + a)^-1/2, V = -sZ rho,
+//   f_0..3   = 1, V, sZ x rho^3 h_x, sZ y rho^3 h_y
+//   f_4..7   = sZ z rho^3 h_z, sZ (rho^3 - 3 x_d^2 rho^5) h_d^2 (d = x, y, z)
+//   f_8..11  = -3 sZ x_d x_e rho^5 h_d h_e (xy, xz, yz), V^2
+//   f_12..15 = x / L_x, y / L_y, z / L_z, r^2 inv_r2
+// (the host's FP32 table [N][C] of the same fields feeds training).
+struct CnnFieldGen {
+    const float *gr[3] = {nullptr, nullptr, nullptr};
+    float sZ = 0.f, a = 1.f, h[3] = {0.f, 0.f, 0.f}, inv_l[3] = {0.f, 0.f, 0.f}, inv_r2 = 0.f;
+    float scale[kCnnChannels] = {};
+};
 // BF16 MFMA forward; mb work of cnn_infer_work_bytes(N) bytes ([2N][C] bf16 plus the
-// packed weights), out [2N]; if nout: nout = g (out_re + i out_im). Pb: the positional
-// fields as bf16, [N][C] (cnn_fields_bf16 of the [N][C] float fields). pack: pack W1, W2
-// into their BF16 operand order in mb first (else mb holds them from an earlier call).
+// packed weights), out [2N]; if nout: nout = g (out_re + i out_im). The positional fields
+// are evaluated in FP32 from fg where they are used. pack: pack W1, W2 into their BF16
+// operand order in mb first (else mb holds them from an earlier call).
 inline size_t cnn_infer_work_bytes(long N) { return (size_t)2 * N * kCnnChannels * 2 + 64 * 64 * 8; }
-std::vector<unsigned short> cnn_fields_bf16(const std::vector<float> &P);
 void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
-                    const unsigned short *Pb, unsigned short *mb, float *out, fcomplex g, fcomplex *nout,
+                    const CnnFieldGen &fg, unsigned short *mb, float *out, fcomplex g, fcomplex *nout,
                     bool pack = true);
 void cnn_to_complex(hipStream_t st, long N, const float *out, fcomplex g, fcomplex *n);
 // partial[b] (b < kLossBlocks, FP64) sums to sum (out - t)^2 over both components

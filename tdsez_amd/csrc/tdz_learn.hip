@@ -13,7 +13,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstring>
 #include <vector>
 
 #include "tdz_internal.hpp"
@@ -369,24 +368,50 @@ __global__ void cnn_pack_kernel(const float *__restrict__ W1, const float *__res
     wp2[s2 * 64 + lane] = {lo.x, lo.y, hi.x, hi.y};
 }
 
-// The inference reads the positional fields as bf16, voxel-major: Pb[v C + a]. The
-// lane that holds channels kb .. kb + 3 of voxel (gx, gy, gz) reads their 4 fields as
-// one aligned 8-byte load; the 64 lanes of a wave (16 voxels along x, 4 channel groups)
-// read 512 contiguous bytes.
-// Issued well ahead of their use (their latency is not covered by one plane's MFMAs).
-__device__ __forceinline__ uint2 load_fields(const unsigned short *__restrict__ Pb, int n0, int n1, int n2, int kb,
-                                             int gx, int gy, int gz)
+// The inference evaluates the positional fields where it uses them, in FP32, from the
+// Greville coordinates (the closed forms of CnnFieldGen) instead of reading a [N][C] table:
+// a lane needs the fields of its channel group kb .. kb + 3 at its output voxels, about 40
+// VALU operations and one reciprocal square root per voxel against 8 bytes of HBM traffic
+// per voxel and lane group (222 MB per inference at 132^3). Coordinates past the grid's
+// end are clamped (those voxels are not stored).
+// The lane's weights of the 16 fields: the scale of field 4 k + q where k is the lane's
+// channel group, zero for the other groups, so that a field value is a weighted sum with
+// no lane-dependent branch (a select per field lets the compiler branch per field).
+__device__ __forceinline__ void field_weights(const CnnFieldGen &f, int kb, float cw[16])
 {
-    if (gx < n0 && gy < n1 && gz < n2)
-        return *reinterpret_cast<const uint2 *>(Pb + (gx + (long)n0 * (gy + (long)n1 * gz)) * C + kb);
-    return {0u, 0u};
+    const int g = kb >> 2;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) cw[e] = g == e / 4 ? f.scale[e] : 0.f;
 }
-__device__ __forceinline__ void unpack_fields(uint2 raw, float pv[4])
+__device__ __forceinline__ float field_coord(const CnnFieldGen &f, int d, int i, int n)
 {
-    pv[0] = __uint_as_float(raw.x << 16);
-    pv[1] = __uint_as_float(raw.x & 0xFFFF0000u);
-    pv[2] = __uint_as_float(raw.y << 16);
-    pv[3] = __uint_as_float(raw.y & 0xFFFF0000u);
+    return f.gr[d][i < n ? i : n - 1];
+}
+__device__ __forceinline__ void gen_fields(const CnnFieldGen &f, const float cw[16], float x, float y, float z,
+                                           float pv[4])
+{
+    const float r2 = x * x + y * y + z * z;
+    const float rho = rsqrtf(r2 + f.a), rho2 = rho * rho, rho3 = rho2 * rho, rho5 = rho3 * rho2;
+    const float V = -f.sZ * rho, s3 = f.sZ * rho3, s5 = -3.f * f.sZ * rho5;
+    const float hx = x * f.h[0], hy = y * f.h[1], hz = z * f.h[2];   // x_d h_d
+    const float v[16] = {1.f,
+                         V,
+                         s3 * hx,
+                         s3 * hy,
+                         s3 * hz,
+                         s3 * (f.h[0] * f.h[0]) + s5 * hx * hx,
+                         s3 * (f.h[1] * f.h[1]) + s5 * hy * hy,
+                         s3 * (f.h[2] * f.h[2]) + s5 * hz * hz,
+                         s5 * hx * hy,
+                         s5 * hx * hz,
+                         s5 * hy * hz,
+                         V * V,
+                         x * f.inv_l[0],
+                         y * f.inv_l[1],
+                         z * f.inv_l[2],
+                         r2 * f.inv_r2};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pv[q] = cw[q] * v[q] + cw[4 + q] * v[4 + q] + cw[8 + q] * v[8 + q] + cw[12 + q] * v[12 + q];
 }
 
 // Layer 1 on MFMA, v_mfma_f32_16x16x16_bf16, both components per block (the inputs are
@@ -398,9 +423,8 @@ __device__ __forceinline__ void unpack_fields(uint2 raw, float pv[4])
 // m = P[v_i][a] D[a][i] is one 8-byte bf16 store at mb[(comp N + v) C + a].
 // The halo tile is staged in LDS as bf16 (the CI channels of a voxel are one 8-byte
 // operand, no conversion per MFMA).
-__global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2, const fcomplex *__restrict__ u,
-                                                          const uint2 *__restrict__ wp,
-                                                          const unsigned short *__restrict__ Pb,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) cnn_l1_mfma_kernel(int n0, int n1, int n2, const fcomplex *__restrict__ u,
+                                                          const uint2 *__restrict__ wp, CnnFieldGen fg,
                                                           unsigned short *__restrict__ mb)
 {
     static_assert(CI == 4, "a lane's 4 consecutive k must be the CI channels of one tap");
@@ -436,19 +460,19 @@ __global__ void __launch_bounds__(256) cnn_l1_mfma_kernel(int n0, int n1, int n2
     __syncthreads();
     const uint2 *us0 = usv[0], *us1 = usv[1];
     const int gx = T.x0 + row;
-    // positional values of this lane's outputs, one plane ahead
-    uint2 pnext[TY];
+    // the coordinates of this lane's outputs and its channel group's field scales
+    float cw[16], fy[TY];
+    field_weights(fg, kb, cw);
+    const float fx = field_coord(fg, 0, gx, n0);
 #pragma unroll
-    for (int ry = 0; ry < TY; ++ry) pnext[ry] = load_fields(Pb, n0, n1, n2, kb, gx, T.y0 + ry, T.z0 + w * WZ);
+    for (int ry = 0; ry < TY; ++ry) fy[ry] = field_coord(fg, 1, T.y0 + ry, n1);
 #pragma unroll 1   // unrolled, the plane loop runs slower (code size and register pressure)
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
+        const float fz = field_coord(fg, 2, gz, n2);
         float pv[TY][4];
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry) unpack_fields(pnext[ry], pv[ry]);
-        if (pz + 1 < WZ)
-#pragma unroll
-            for (int ry = 0; ry < TY; ++ry) pnext[ry] = load_fields(Pb, n0, n1, n2, kb, gx, T.y0 + ry, gz + 1);
+        for (int ry = 0; ry < TY; ++ry) gen_fields(fg, cw, fx, fy[ry], fz, pv[ry]);
         floatx4 acc[2][TY];
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) acc[0][ry] = acc[1][ry] = {0.f, 0.f, 0.f, 0.f};
@@ -495,8 +519,7 @@ constexpr int MS = 24;   // LDS stride of one voxel of m, in bf16 elements
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2,
                                                           const unsigned short *__restrict__ mb,
-                                                          const uint4 *__restrict__ wp2,
-                                                          const unsigned short *__restrict__ Pb,
+                                                          const uint4 *__restrict__ wp2, CnnFieldGen fg,
                                                           float *__restrict__ out)
 {
     __shared__ uint4 ms[HV * MS / 8];
@@ -529,13 +552,13 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
     bf16x8 bw[NPACK2];
 #pragma unroll
     for (int s = 0; s < NPACK2; ++s) bw[s] = __builtin_bit_cast(bf16x8, wp2[s * 64 + lane]);
-    // positional values of this lane's outputs, in flight during the products
-    uint2 praw[WZ][TY];
+    // the coordinates of this lane's outputs, in flight during the products
+    float fy[TY], fz[WZ];
+    const float fx = field_coord(fg, 0, T.x0 + row, n0);
 #pragma unroll
-    for (int pz = 0; pz < WZ; ++pz)
+    for (int ry = 0; ry < TY; ++ry) fy[ry] = field_coord(fg, 1, T.y0 + ry, n1);
 #pragma unroll
-        for (int ry = 0; ry < TY; ++ry)
-            praw[pz][ry] = load_fields(Pb, n0, n1, n2, kb, T.x0 + row, T.y0 + ry, T.z0 + w * WZ + pz);
+    for (int pz = 0; pz < WZ; ++pz) fz[pz] = field_coord(fg, 2, T.z0 + w * WZ + pz, n2);
     __syncthreads();
     // Halo-row-major order: the two B fragments of a halo row (16 voxels shifted by dx =
     // -1 | 0 and by +1, 8 channels per lane) are read from LDS once and feed the up to 9
@@ -569,6 +592,8 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
         }
     static_assert(TY == 4, "the 4 channel groups of a voxel end with one row each");
     const int gx = T.x0 + row;
+    float cw[16];
+    field_weights(fg, kb, cw);
 #pragma unroll
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
@@ -577,7 +602,7 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) {
             float pv[4];
-            unpack_fields(praw[pz][ry], pv);
+            gen_fields(fg, cw, fx, fy[ry], fz[pz], pv);
             v4[ry] = pv[0] * acc[pz][ry][0] + pv[1] * acc[pz][ry][1] + pv[2] * acc[pz][ry][2] +
                      pv[3] * acc[pz][ry][3];
         }
@@ -1200,20 +1225,8 @@ void cnn_backward_f32(hipStream_t st, const int n[3], const fcomplex *u, const f
                        NTAP * CI * C, dW1);
 }
 
-std::vector<unsigned short> cnn_fields_bf16(const std::vector<float> &P)
-{
-    std::vector<unsigned short> out(P.size());
-    for (size_t e = 0; e < P.size(); ++e) {
-        unsigned bits;
-        std::memcpy(&bits, &P[e], 4);
-        bits += 0x7FFFu + ((bits >> 16) & 1u);   // round to nearest even
-        out[e] = (unsigned short)(bits >> 16);
-    }
-    return out;
-}
-
 void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const float *W1, const float *W2,
-                    const unsigned short *Pb, unsigned short *mb, float *out, fcomplex g, fcomplex *nout, bool pack)
+                    const CnnFieldGen &fg, unsigned short *mb, float *out, fcomplex g, fcomplex *nout, bool pack)
 {
     const long N = (long)n[0] * n[1] * n[2];
     const long tiles = (long)((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ);
@@ -1222,9 +1235,9 @@ void cnn_infer_mfma(hipStream_t st, const int n[3], const fcomplex *u, const flo
     uint4 *wp2 = reinterpret_cast<uint4 *>(reinterpret_cast<char *>(wp) + kPack2Offset);
     if (pack) hipLaunchKernelGGL(cnn_pack_kernel, dim3(gridn(NPACK * 64, 256)), dim3(256), 0, st, W1, W2, wp, wp2);
     // layer 1: one block per tile (both components); layer 2: one per tile and component
-    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)tiles), dim3(256), 0, st, n[0], n[1], n[2], u, wp, Pb, mb);
+    hipLaunchKernelGGL(cnn_l1_mfma_kernel, dim3((unsigned)tiles), dim3(256), 0, st, n[0], n[1], n[2], u, wp, fg, mb);
     hipLaunchKernelGGL(cnn_l2_mfma_kernel, dim3((unsigned)(2 * tiles)), dim3(256), 0, st, n[0], n[1], n[2], mb, wp2,
-                       Pb, out);
+                       fg, out);
     if (nout) hipLaunchKernelGGL(cnn_to_complex_kernel, dim3(gridn(N, 256)), dim3(256), 0, st, N, out, g, nout);
 }
 

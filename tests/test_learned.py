@@ -109,6 +109,33 @@ def test_gpu_forward_and_gradients_match_oracle():
     ctx.close()
 
 
+@pytest.mark.gpu
+def test_inference_fields_match_table():
+    """The MFMA inference evaluates the positional fields from their closed forms in FP32.
+    With the constant input and field 0 (identically 1) routed through hidden channel 0 by
+    the centre taps and read out through field b, every product has one nonzero term, so the
+    output is the evaluated field b itself: it must equal the FP32 table that training uses
+    to a few FP32 ulps of the field's maximum (BF16 fields would differ by about 4e-3)."""
+    import tdsez_amd as T
+    ctx = T.Context(SMALL)
+    P = ctx.learn_fields()                                   # [N][C], each field scaled to max 1
+    assert np.all(P[:, 0] == 1.0)
+    u = np.zeros((CI, ctx.N), complex)
+    u[0] = 1.0
+    t = np.zeros(ctx.N, complex)
+    W1 = np.zeros((27, CI, C), np.float32)
+    W1[13, 0, 0] = 1.0                                       # centre tap, input 0 -> hidden 0
+    for b in range(C):
+        W2 = np.zeros((27, C, C), np.float32)
+        W2[13, 0, b] = 1.0                                   # centre tap, hidden 0 -> output group b
+        ctx.learn_weights(W1, W2)
+        om = ctx.learn_eval(u, t)[4]
+        err = np.max(np.abs(om[:ctx.N] - P[:, b]))
+        assert err < 1e-5, (b, err)
+        assert np.all(om[ctx.N:] == 0.0)
+    ctx.close()
+
+
 def _mnorm(ctx, x):
     """L2 norm of the wavefunction with coefficients x: sqrt(x^H M x)."""
     import tdsez_amd as T
