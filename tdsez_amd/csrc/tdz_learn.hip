@@ -517,7 +517,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) c
 // 16-byte read, and the 16 lanes of a lane group start on distinct 4-bank groups.
 constexpr int MS = 24;   // LDS stride of one voxel of m, in bf16 elements
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-__global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) cnn_l2_mfma_kernel(int n0, int n1, int n2,
                                                           const unsigned short *__restrict__ mb,
                                                           const uint4 *__restrict__ wp2, CnnFieldGen fg,
                                                           float *__restrict__ out)
@@ -525,6 +525,16 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
     __shared__ uint4 ms[HV * MS / 8];
     const TileIdx T = tile_of(n0, n1, n2);
     const long N = (long)n0 * n1 * n2;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row = lane & 15, kb = (lane >> 4) * 4, g = lane >> 4;
+    // the coordinates of this lane's outputs (their loads first: the fields are evaluated
+    // while the halo loads below are in flight)
+    float fy[TY], fz[WZ];
+    const float fx = field_coord(fg, 0, T.x0 + row, n0);
+#pragma unroll
+    for (int ry = 0; ry < TY; ++ry) fy[ry] = field_coord(fg, 1, T.y0 + ry, n1);
+#pragma unroll
+    for (int pz = 0; pz < WZ; ++pz) fz[pz] = field_coord(fg, 2, T.z0 + w * WZ + pz, n2);
     // stage one halo voxel (its 16 channels, 2 x 16 bytes) per thread and round, in two
     // phases (all loads in flight, then all LDS stores)
     constexpr int NV = (HV + 255) / 256;
@@ -539,6 +549,18 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
         val[r][0] = in ? src[0] : uint4{0u, 0u, 0u, 0u};
         val[r][1] = in ? src[1] : uint4{0u, 0u, 0u, 0u};
     }
+    bf16x8 bw[NPACK2];
+#pragma unroll
+    for (int s = 0; s < NPACK2; ++s) bw[s] = __builtin_bit_cast(bf16x8, wp2[s * 64 + lane]);
+    float pf[WZ][TY][4];   // the fields of this lane's channel group at its outputs
+    {
+        float cw[16];
+        field_weights(fg, kb, cw);
+#pragma unroll
+        for (int pz = 0; pz < WZ; ++pz)
+#pragma unroll
+            for (int ry = 0; ry < TY; ++ry) gen_fields(fg, cw, fx, fy[ry], fz[pz], pf[pz][ry]);
+    }
 #pragma unroll
     for (int r = 0; r < NV; ++r) {
         const int h = threadIdx.x + 256 * r;
@@ -547,18 +569,6 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
             ms[h * (MS / 8) + 1] = val[r][1];
         }
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int row = lane & 15, kb = (lane >> 4) * 4, g = lane >> 4;
-    bf16x8 bw[NPACK2];
-#pragma unroll
-    for (int s = 0; s < NPACK2; ++s) bw[s] = __builtin_bit_cast(bf16x8, wp2[s * 64 + lane]);
-    // the coordinates of this lane's outputs, in flight during the products
-    float fy[TY], fz[WZ];
-    const float fx = field_coord(fg, 0, T.x0 + row, n0);
-#pragma unroll
-    for (int ry = 0; ry < TY; ++ry) fy[ry] = field_coord(fg, 1, T.y0 + ry, n1);
-#pragma unroll
-    for (int pz = 0; pz < WZ; ++pz) fz[pz] = field_coord(fg, 2, T.z0 + w * WZ + pz, n2);
     __syncthreads();
     // Halo-row-major order: the two B fragments of a halo row (16 voxels shifted by dx =
     // -1 | 0 and by +1, 8 channels per lane) are read from LDS once and feed the up to 9
@@ -592,8 +602,6 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
         }
     static_assert(TY == 4, "the 4 channel groups of a voxel end with one row each");
     const int gx = T.x0 + row;
-    float cw[16];
-    field_weights(fg, kb, cw);
 #pragma unroll
     for (int pz = 0; pz < WZ; ++pz) {
         const int lz = w * WZ + pz, gz = T.z0 + lz;
@@ -601,8 +609,7 @@ __global__ void __launch_bounds__(256) cnn_l2_mfma_kernel(int n0, int n1, int n2
         float v4[TY];
 #pragma unroll
         for (int ry = 0; ry < TY; ++ry) {
-            float pv[4];
-            gen_fields(fg, cw, fx, fy[ry], fz[pz], pv);
+            const float *pv = pf[pz][ry];
             v4[ry] = pv[0] * acc[pz][ry][0] + pv[1] * acc[pz][ry][1] + pv[2] * acc[pz][ry][2] +
                      pv[3] * acc[pz][ry][3];
         }
