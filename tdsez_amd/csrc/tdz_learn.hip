@@ -422,7 +422,8 @@ __device__ __forceinline__ void gen_fields(const CnnFieldGen &f, const float cw[
 // holds the 4 consecutive hidden channels a = 4(l/16) .. +3 of voxel v_{l%16}:
 // m = P[v_i][a] D[a][i] is one 8-byte bf16 store at mb[(comp N + v) C + a].
 // The halo tile is staged in LDS as bf16 (the CI channels of a voxel are one 8-byte
-// operand, no conversion per MFMA).
+// operand, no conversion per MFMA). At least three waves per SIMD (the field evaluation
+// otherwise takes the kernel to 176 registers and two).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) cnn_l1_mfma_kernel(int n0, int n1, int n2, const fcomplex *__restrict__ u,
                                                           const uint2 *__restrict__ wp, CnnFieldGen fg,
                                                           unsigned short *__restrict__ mb)
@@ -515,6 +516,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) c
 // the 4 lanes that hold voxel i's channel groups). The halo tile of m lives in LDS with a
 // 48-byte voxel stride (16 bf16 channels + 8 padding): a lane's 8 channels are one aligned
 // 16-byte read, and the 16 lanes of a lane group start on distinct 4-bank groups.
+// Two waves per SIMD is what the 72.6 KB tile allows (two blocks per CU); the attribute
+// keeps the registers within that (unconstrained, the compiler takes 260 and one wave).
 constexpr int MS = 24;   // LDS stride of one voxel of m, in bf16 elements
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) cnn_l2_mfma_kernel(int n0, int n1, int n2,
